@@ -1,0 +1,19 @@
+"""casa-imaging gridding / degridding on MI355X (gfx950).
+
+The compute path is libcasa_imaging_hip.so (hand-written HIP kernels behind a
+C ABI, header ``include/casa_imaging_hip.h``); this package is the host-side
+mirror of the reference interface over that ABI plus the autograd surface.
+It has no CPU fallback: without the built library or a HIP device, every
+entry point raises.
+"""
+from . import abi
+from .abi import (F32, F64, MODE_DIRTY, MODE_PSF, MODE_RESIDUAL, ImagingError, InvalidRequest,
+                  NoUsableSamples, Normalization, Unsupported)
+from .imaging import (Context, RowChannelBatch, StandardGridder, VisibilityBatch, VisibilitySet,
+                      default_context)
+
+__all__ = [
+    "abi", "F32", "F64", "MODE_DIRTY", "MODE_PSF", "MODE_RESIDUAL", "ImagingError",
+    "InvalidRequest", "NoUsableSamples", "Normalization", "Unsupported", "Context",
+    "RowChannelBatch", "StandardGridder", "VisibilityBatch", "VisibilitySet", "default_context",
+]

@@ -1,0 +1,179 @@
+"""ctypes binding of libcasa_imaging_hip.so (declarations in include/casa_imaging_hip.h).
+
+The library is loaded from ``casa-rs_amd/lib/`` (built in place by
+``casa-rs_amd/csrc/Makefile``).  There is no fallback: if the library is
+missing or no HIP device is present, loading or context creation raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libcasa_imaging_hip.so")
+
+CHIP_OK = 0
+CHIP_ERR_INVALID = 1
+CHIP_ERR_NO_SAMPLES = 2
+CHIP_ERR_NORMALIZATION = 3
+CHIP_ERR_DEVICE = 4
+
+MODE_PSF = 0
+MODE_DIRTY = 1
+MODE_RESIDUAL = 2
+
+F32 = 0
+F64 = 1
+
+PAD_COMPOSITE_1_2 = 0
+PAD_COMPOSITE_1_3 = 1
+PAD_NONE = 2
+
+# Every symbol the header declares; tests check that the library exports them.
+EXPORTS = (
+    "chip_ctx_create", "chip_ctx_destroy", "chip_error_free", "chip_version",
+    "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
+    "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
+    "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_grid",
+    "chip_vset_degrid", "chip_vset_grid_values", "chip_grid_plain", "chip_degrid_plain",
+    "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
+)
+
+
+class ImagingError(RuntimeError):
+    """Base error; subclasses mirror ImagingError variants (error.rs:8-24)."""
+
+    code = CHIP_ERR_INVALID
+
+
+class InvalidRequest(ImagingError):
+    code = CHIP_ERR_INVALID
+
+
+class NoUsableSamples(ImagingError):
+    code = CHIP_ERR_NO_SAMPLES
+
+
+class Normalization(ImagingError):
+    code = CHIP_ERR_NORMALIZATION
+
+
+class Unsupported(ImagingError):
+    code = CHIP_ERR_DEVICE
+
+
+_ERRORS = {
+    CHIP_ERR_INVALID: InvalidRequest,
+    CHIP_ERR_NO_SAMPLES: NoUsableSamples,
+    CHIP_ERR_NORMALIZATION: Normalization,
+    CHIP_ERR_DEVICE: Unsupported,
+}
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_uint64), ("ny", ctypes.c_uint64),
+        ("cell_x", ctypes.c_double), ("cell_y", ctypes.c_double),
+        ("padding_mode", ctypes.c_uint32), ("oversampling", ctypes.c_uint32),
+    ]
+
+
+class GridderInfo(ctypes.Structure):
+    _fields_ = [
+        ("gx", ctypes.c_uint64), ("gy", ctypes.c_uint64),
+        ("blc_x", ctypes.c_uint64), ("blc_y", ctypes.c_uint64),
+        ("du", ctypes.c_double), ("dv", ctypes.c_double),
+        ("oversampling", ctypes.c_uint32), ("wproject", ctypes.c_uint32),
+        ("planes", ctypes.c_uint32), ("sampling", ctypes.c_uint32),
+        ("width", ctypes.c_uint32), ("max_support", ctypes.c_uint32),
+        ("norm_radius", ctypes.c_uint32), ("_pad", ctypes.c_uint32),
+        ("w_scale", ctypes.c_double),
+    ]
+
+
+class Sums(ctypes.Structure):
+    _fields_ = [
+        ("normalization_sumwt", ctypes.c_double),
+        ("reported_sumwt", ctypes.c_double),
+        ("gridded", ctypes.c_uint64),
+        ("skipped", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Samples(ctypes.Structure):
+    _fields_ = [
+        ("n_rows", ctypes.c_uint64), ("n_chan", ctypes.c_uint64),
+        ("u_dev", ctypes.c_void_p), ("v_dev", ctypes.c_void_p), ("w_dev", ctypes.c_void_p),
+        ("uvw_m_dev", ctypes.c_void_p), ("lambda_scale_dev", ctypes.c_void_p),
+        ("vis_dev", ctypes.c_void_p), ("weight_dev", ctypes.c_void_p),
+        ("sumwt_factor_dev", ctypes.c_void_p), ("gridable_dev", ctypes.c_void_p),
+        ("flag_dev", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_PP = ctypes.POINTER(ctypes.c_void_p)
+_ERR = ctypes.POINTER(ctypes.c_char_p)
+_U32 = ctypes.c_uint32
+_U64 = ctypes.c_uint64
+
+
+def load():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Unsupported(
+            f"{LIB_PATH} is missing: build it with `make -C casa-rs_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "chip_version": (ctypes.c_char_p, []),
+        "chip_error_free": (None, [_P]),
+        "chip_ctx_create": (ctypes.c_int, [ctypes.c_int, _PP, _PP]),
+        "chip_ctx_destroy": (None, [_P]),
+        "chip_gridder_create": (ctypes.c_int, [_P, ctypes.POINTER(Geometry), _PP, _PP]),
+        "chip_gridder_destroy": (None, [_P]),
+        "chip_gridder_info_get": (ctypes.c_int, [_P, ctypes.POINTER(GridderInfo), _PP]),
+        "chip_gridder_tables": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
+        "chip_max_abs_w": (ctypes.c_int, [_P, ctypes.POINTER(Samples), ctypes.POINTER(ctypes.c_double), _PP]),
+        "chip_gridder_attach_wproject": (ctypes.c_int, [_P, ctypes.c_double, _U32, _U32, _PP]),
+        "chip_wproject_tables": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
+        "chip_plan": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _P, _PP]),
+        "chip_vset_create": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _PP, _PP]),
+        "chip_vset_destroy": (None, [_P]),
+        "chip_vset_sums": (ctypes.c_int, [_P, _P, _PP]),
+        "chip_vset_grid": (ctypes.c_int, [_P, _U32, _P, _P, _P, _PP]),
+        "chip_vset_degrid": (ctypes.c_int, [_P, _P, _P, _PP]),
+        "chip_vset_grid_values": (ctypes.c_int, [_P, _P, _P, _PP]),
+        "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
+        "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),
+        "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),
+        "chip_model_to_grid": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _PP]),
+        "chip_grid_to_model_adjoint": (ctypes.c_int, [_P, _U64, _P, _P, _PP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    """Call an entry point that ends in ``char **err`` and raise on failure."""
+    L = load()
+    err = ctypes.c_void_p()
+    rc = getattr(L, name)(*args, ctypes.byref(err))
+    if rc != CHIP_OK:
+        msg = ctypes.cast(err, ctypes.c_char_p).value.decode() if err.value else name
+        if err.value:
+            L.chip_error_free(err)
+        raise _ERRORS.get(rc, ImagingError)(f"{name}: {msg}")
+    return rc

@@ -1,0 +1,306 @@
+// extern "C" entry points of libcasa_imaging_hip.so.  Each wrapper converts
+// exceptions into the status / message convention of the header.
+#include <cstdlib>
+#include <cstring>
+
+#include "internal.hpp"
+
+namespace chip {
+void vset_build(chip_vset &vs, const chip_samples &cs);
+void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, float *dirty);
+void vset_degrid(chip_vset &vs, const float *model, float *vis_out);
+void vset_grid_values(chip_vset &vs, const float *values, float *grid);
+}  // namespace chip
+
+using namespace chip;
+
+namespace {
+
+char *dup_message(const char *m) {
+    const size_t n = std::strlen(m);
+    char *r = static_cast<char *>(std::malloc(n + 1));
+    if (r) std::memcpy(r, m, n + 1);
+    return r;
+}
+
+template <class F>
+int guarded(char **err, F &&f) {
+    if (err) *err = nullptr;
+    try {
+        f();
+        return CHIP_OK;
+    } catch (const Error &e) {
+        if (err) *err = dup_message(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        if (err) *err = dup_message("host allocation failed");
+        return CHIP_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        if (err) *err = dup_message(e.what());
+        return CHIP_ERR_INVALID;
+    }
+}
+
+void set_device(const chip_ctx *ctx) { CHIP_HIP(hipSetDevice(ctx->device)); }
+
+}  // namespace
+
+extern "C" {
+
+const char *chip_version(void) { return "casa-imaging-hip 0.1.0 (gfx950)"; }
+
+void chip_error_free(char *err) { std::free(err); }
+
+int chip_ctx_create(int device, chip_ctx **out, char **err) {
+    return guarded(err, [&] {
+        if (!out) fail(CHIP_ERR_INVALID, "out is NULL");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+            fail(CHIP_ERR_DEVICE, "no HIP device is available");
+        if (device < 0 || device >= count) fail(CHIP_ERR_INVALID, "device index out of range");
+        std::unique_ptr<chip_ctx> c(new chip_ctx);
+        c->device = device;
+        CHIP_HIP(hipSetDevice(device));
+        CHIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c.release();
+    });
+}
+
+void chip_ctx_destroy(chip_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    ctx->fft.clear();
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int chip_gridder_create(chip_ctx *ctx, const chip_geometry *geo, chip_gridder **out, char **err) {
+    return guarded(err, [&] {
+        if (!ctx || !geo || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        // ImageGeometry::validate (types.rs:910-925)
+        if (geo->nx < 8 || geo->ny < 8) fail(CHIP_ERR_INVALID, "image shape must be at least 8x8");
+        if (!(std::isfinite(geo->cell_x) && geo->cell_x > 0.0 && std::isfinite(geo->cell_y) &&
+              geo->cell_y > 0.0))
+            fail(CHIP_ERR_INVALID, "cell sizes must be finite positive radians");
+        if (geo->oversampling < 2 || geo->oversampling > 254 || geo->oversampling % 2)
+            fail(CHIP_ERR_INVALID, "oversampling must be even and in 2..254");
+        set_device(ctx);
+        std::unique_ptr<chip_gridder> g(new chip_gridder);
+        g->ctx = ctx;
+        g->geom = *geo;
+        switch (geo->padding_mode) {
+            case CHIP_PAD_COMPOSITE_1_2:
+                g->gx = composite_padded_len(geo->nx, 1.2);
+                g->gy = composite_padded_len(geo->ny, 1.2);
+                break;
+            case CHIP_PAD_COMPOSITE_1_3:
+                g->gx = composite_padded_len(geo->nx, 1.3);
+                g->gy = composite_padded_len(geo->ny, 1.3);
+                break;
+            case CHIP_PAD_NONE:
+                g->gx = geo->nx;
+                g->gy = geo->ny;
+                break;
+            default:
+                fail(CHIP_ERR_INVALID, "unknown padding mode");
+        }
+        if (g->gx > 65535 || g->gy > 65535) fail(CHIP_ERR_INVALID, "grid edge above 65535");
+        // gridder.rs:403-406, :426-427
+        g->blc_x = (g->gx - geo->nx + (g->gx % 2 == 0 ? 1 : 0)) / 2;
+        g->blc_y = (g->gy - geo->ny + (g->gy % 2 == 0 ? 1 : 0)) / 2;
+        g->du = 1.0 / (static_cast<double>(g->gx) * geo->cell_x);
+        g->dv = 1.0 / (static_cast<double>(g->gy) * geo->cell_y);
+        g->os = geo->oversampling;
+        g->taps = tap_weights(g->os);
+        g->corr_x = correction_axis(g->gx);
+        g->corr_y = correction_axis(g->gy);
+        hipStream_t st = ctx->stream;
+        g->d_taps.alloc(g->taps.size() * sizeof(float));
+        g->d_corr_x.alloc(g->corr_x.size() * sizeof(float));
+        g->d_corr_y.alloc(g->corr_y.size() * sizeof(float));
+        CHIP_HIP(hipMemcpyAsync(g->d_taps.p, g->taps.data(), g->d_taps.bytes, hipMemcpyHostToDevice, st));
+        CHIP_HIP(hipMemcpyAsync(g->d_corr_x.p, g->corr_x.data(), g->d_corr_x.bytes, hipMemcpyHostToDevice, st));
+        CHIP_HIP(hipMemcpyAsync(g->d_corr_y.p, g->corr_y.data(), g->d_corr_y.bytes, hipMemcpyHostToDevice, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+        *out = g.release();
+    });
+}
+
+void chip_gridder_destroy(chip_gridder *g) {
+    if (!g) return;
+    (void)hipSetDevice(g->ctx->device);
+    delete g;
+}
+
+int chip_gridder_info_get(const chip_gridder *g, chip_gridder_info *out, char **err) {
+    return guarded(err, [&] {
+        if (!g || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        chip_gridder_info r{};
+        r.gx = g->gx; r.gy = g->gy; r.blc_x = g->blc_x; r.blc_y = g->blc_y;
+        r.du = g->du; r.dv = g->dv; r.oversampling = g->os;
+        r.wproject = g->wproject ? 1u : 0u;
+        r.planes = g->planes; r.sampling = g->sampling; r.width = g->width;
+        r.max_support = g->max_support;
+        r.norm_radius = static_cast<uint32_t>(g->norm_radius);
+        r.w_scale = g->w_scale;
+        *out = r;
+    });
+}
+
+int chip_gridder_tables(const chip_gridder *g, float *taps, float *corr_x, float *corr_y, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        if (taps) std::memcpy(taps, g->taps.data(), g->taps.size() * sizeof(float));
+        if (corr_x) std::memcpy(corr_x, g->corr_x.data(), g->corr_x.size() * sizeof(float));
+        if (corr_y) std::memcpy(corr_y, g->corr_y.data(), g->corr_y.size() * sizeof(float));
+    });
+}
+
+int chip_max_abs_w(chip_gridder *g, const chip_samples *s, double *out, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        *out = launch_max_abs_w(*g, make_source(*s), g->ctx->stream);
+    });
+}
+
+int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t explicit_planes,
+                                 uint32_t sampling, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        if (g->wproject) fail(CHIP_ERR_INVALID, "W projection already attached");
+        set_device(g->ctx);
+        build_wproject(*g, max_abs_w, explicit_planes, sampling);
+    });
+}
+
+int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels, float *norm_table,
+                         char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        if (!g->wproject) fail(CHIP_ERR_INVALID, "no W projection attached");
+        set_device(g->ctx);
+        if (supports) std::memcpy(supports, g->supports.data(), g->supports.size() * sizeof(int32_t));
+        if (norm_table) std::memcpy(norm_table, g->norm_table.data(), g->norm_table.size() * sizeof(float));
+        if (kernels)
+            CHIP_HIP(hipMemcpy(kernels, g->d_kernels.p, g->d_kernels.bytes, hipMemcpyDeviceToHost));
+    });
+}
+
+int chip_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, float *norm_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s || !plan_dev) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        launch_plan_raw(*g, make_source(*s), plan_dev, norm_dev, g->ctx->stream);
+        CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int chip_vset_create(chip_gridder *g, const chip_samples *s, chip_vset **out, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        std::unique_ptr<chip_vset> vs(new chip_vset);
+        vs->g = g;
+        vset_build(*vs, *s);
+        CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+        *out = vs.release();
+    });
+}
+
+void chip_vset_destroy(chip_vset *vs) {
+    if (!vs) return;
+    (void)hipSetDevice(vs->g->ctx->device);
+    delete vs;
+}
+
+int chip_vset_sums(const chip_vset *vs, chip_sums *sums, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        if (sums) std::memcpy(sums, vs->sums.data(), vs->sums.size() * sizeof(chip_sums));
+    });
+}
+
+int chip_vset_grid(chip_vset *vs, uint32_t mode, const float *model_dev, float *psf_dev,
+                   float *dirty_dev, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        set_device(vs->g->ctx);
+        vset_grid(*vs, mode, model_dev, psf_dev, dirty_dev);
+        CHIP_HIP(hipStreamSynchronize(vs->g->ctx->stream));
+    });
+}
+
+int chip_vset_degrid(chip_vset *vs, const float *model_dev, float *vis_out_dev, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        set_device(vs->g->ctx);
+        vset_degrid(*vs, model_dev, vis_out_dev);
+        CHIP_HIP(hipStreamSynchronize(vs->g->ctx->stream));
+    });
+}
+
+int chip_vset_grid_values(chip_vset *vs, const float *values_dev, float *grid_dev, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        set_device(vs->g->ctx);
+        vset_grid_values(*vs, values_dev, grid_dev);
+        CHIP_HIP(hipStreamSynchronize(vs->g->ctx->stream));
+    });
+}
+
+int chip_grid_plain(chip_gridder *g, const chip_samples *s, uint32_t mode, uint32_t precision,
+                    const void *model_dev, void *psf_dev, void *dirty_dev, chip_sums *sums_host,
+                    char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        grid_plain(*g, make_source(*s), mode, precision, model_dev, psf_dev, dirty_dev, sums_host);
+    });
+}
+
+int chip_degrid_plain(chip_gridder *g, const chip_samples *s, uint32_t precision,
+                      const void *model_dev, float *vis_out_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        degrid_plain(*g, make_source(*s), precision, model_dev, vis_out_dev);
+        CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int chip_finish_products(chip_gridder *g, uint32_t precision, uint64_t batch, void *psf_grids_dev,
+                         void *dirty_grids_dev, const double *nsum, float *psf_out_dev,
+                         float *residual_out_dev, float *psf_peak_host, char **err) {
+    return guarded(err, [&] {
+        if (!g || !nsum || !psf_peak_host) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        finish_products(*g, precision, batch, psf_grids_dev, dirty_grids_dev, nsum, psf_out_dev,
+                        residual_out_dev, psf_peak_host);
+    });
+}
+
+int chip_model_to_grid(chip_gridder *g, uint32_t precision, uint64_t batch, const float *model_dev,
+                       void *grid_out_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        set_device(g->ctx);
+        model_to_grid(*g, precision, batch, model_dev, grid_out_dev);
+    });
+}
+
+int chip_grid_to_model_adjoint(chip_gridder *g, uint64_t batch, float *grid_dev, float *image_out_dev,
+                               char **err) {
+    return guarded(err, [&] {
+        if (!g || !grid_dev || !image_out_dev) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        grid_to_model_adjoint(*g, batch, grid_dev, image_out_dev);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,202 @@
+// Device-side helpers shared by the HIP translation units: the FP64 planner,
+// sample loading and the packed stream format.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace chip {
+
+// Packed plan of one accepted sample (8 bytes).
+//   x: bits 0-15  x anchor (standard: first tap column x_start; W: loc_x)
+//      bits 16-31 y anchor (standard: y_start; W: loc_y)
+//   y: bits 0-7   x selector (standard: tap row wix; W: off_x + 128)
+//      bits 8-15  y selector
+//      bits 16-26 W plane
+//      bit 27     conjugate kernel (w > 0)
+//      bit 28     visibility finite (contributes to dirty grids)
+//      bit 29     usable for residual refresh
+constexpr uint32_t kConjBit = 1u << 27;
+constexpr uint32_t kVisOkBit = 1u << 28;
+constexpr uint32_t kResidOkBit = 1u << 29;
+
+struct Plan {
+    int32_t x, y, sx, sy, plane, conj;
+    float norm;
+};
+
+// One work item: a run of at most kChunk sorted samples of one (channel, tile).
+struct WorkItem {
+    uint32_t chan;
+    uint32_t tile;   // tile_x * tiles_y + tile_y
+    uint32_t start;  // first stream position
+    uint32_t count;
+};
+
+struct SampleIn {
+    double u, v, w;
+    float weight, sf;
+    float2 vis;
+    bool usable;  // gridable and not flagged
+    uint32_t chan;
+};
+
+__device__ __forceinline__ SampleIn load_sample(const Source &s, uint64_t i) {
+    SampleIn r;
+    const uint64_t row = s.n_chan == 1 ? i : i / s.n_chan;
+    r.chan = s.n_chan == 1 ? 0u : static_cast<uint32_t>(i - row * s.n_chan);
+    if (s.uvw_m) {
+        const double scale = s.lambda_scale[r.chan];
+        r.u = s.uvw_m[3 * row] * scale;
+        r.v = s.uvw_m[3 * row + 1] * scale;
+        r.w = s.uvw_m[3 * row + 2] * scale;
+    } else {
+        r.u = s.u[i];
+        r.v = s.v[i];
+        r.w = s.w[i];
+    }
+    r.weight = s.weight[i];
+    r.sf = s.sumwt_factor ? s.sumwt_factor[i] : 1.0f;
+    r.vis = s.vis[i];
+    r.usable = (!s.gridable || s.gridable[i]) && (!s.flag || !s.flag[i]);
+    return r;
+}
+
+__device__ __forceinline__ bool finite_f(float x) { return isfinite(x); }
+
+// sample_tap_span (gridder.rs:1958-1974) for one axis.  Every step is a
+// separately rounded IEEE operation (no contraction), with round() = half
+// away from zero as f64::round.
+__device__ __forceinline__ bool std_axis_span(double coord, uint64_t size, uint32_t os,
+                                              int32_t &start, int32_t &widx) {
+#pragma clang fp contract(off)
+    if (!isfinite(coord)) return false;
+    if (coord > 4.0e18 || coord < -4.0e18) return false;
+    const double a = round(coord);
+    const int64_t anchor = static_cast<int64_t>(a);
+    const int64_t off = static_cast<int64_t>(round((static_cast<double>(anchor) - coord) *
+                                                   static_cast<double>(os)));
+    const int64_t s = anchor - kSupport, e = anchor + kSupport;
+    if (s < 0 || e >= static_cast<int64_t>(size)) return false;
+    const int64_t half = os / 2;
+    const int64_t idx = off + half;
+    if (idx < 0 || idx > 2 * half) return false;
+    start = static_cast<int32_t>(s);
+    widx = static_cast<int32_t>(idx);
+    return true;
+}
+
+// Plan one sample (StandardGridder::plan_positive_taps, gridder.rs:557-566 or
+// WProjector::plan_sample, gridder.rs:3588-3624).  Validity filters are
+// applied by the caller.
+__device__ __forceinline__ bool plan_sample(const PlanParams &p, double u, double v, double w,
+                                            Plan &out) {
+#pragma clang fp contract(off)
+    if (!p.wproject) {
+        // grid_coordinate_x / _y (gridder.rs:1877-1886)
+        const double cx = u / p.du + p.half_gx;
+        const double cy = -v / p.dv + p.half_gy;
+        int32_t xs, xw, ys, yw;
+        if (!std_axis_span(cx, p.gx, p.os, xs, xw)) return false;
+        if (!std_axis_span(cy, p.gy, p.os, ys, yw)) return false;
+        out.x = xs; out.y = ys; out.sx = xw; out.sy = yw;
+        out.plane = 0; out.conj = 0; out.norm = 1.0f;
+        return true;
+    }
+    const double pos_x = u / p.du + p.half_gx;
+    const double pos_y = -v / p.dv + p.half_gy;
+    if (!(isfinite(pos_x) && isfinite(pos_y) && isfinite(w))) return false;
+    if (pos_x > 4.0e18 || pos_x < -4.0e18 || pos_y > 4.0e18 || pos_y < -4.0e18) return false;
+    const int64_t lx = static_cast<int64_t>(round(pos_x));
+    const int64_t ly = static_cast<int64_t>(round(pos_y));
+    const double s = static_cast<double>(p.sampling);
+    const int64_t ox = static_cast<int64_t>(round((static_cast<double>(lx) - pos_x) * s));
+    const int64_t oy = static_cast<int64_t>(round((static_cast<double>(ly) - pos_y) * s));
+    // w_plane_index (gridder.rs:3749-3757)
+    uint32_t plane = 0;
+    if (p.planes > 1) {
+        double c = p.w_scale * fabs(w);
+        c = sqrt(c < 0.0 ? 0.0 : c);
+        double r = round(c);
+        const double top = static_cast<double>(p.planes - 1);
+        r = r < 0.0 ? 0.0 : (r > top ? top : r);
+        plane = static_cast<uint32_t>(r);
+    }
+    const int64_t S = p.supports[plane];
+    if (S <= 0 || lx - S < 0 || ly - S < 0 || lx + S >= static_cast<int64_t>(p.gx) ||
+        ly + S >= static_cast<int64_t>(p.gy))
+        return false;
+    const int64_t R = p.norm_radius;
+    // |off| <= sampling/2 <= R always holds because |loc - pos| <= 1/2.
+    if (ox > R || ox < -R || oy > R || oy < -R) return false;
+    const int64_t axis = 2 * R + 1;
+    out.norm = p.norm_table[(plane * axis + (ox + R)) * axis + (oy + R)];
+    out.x = static_cast<int32_t>(lx);
+    out.y = static_cast<int32_t>(ly);
+    out.sx = static_cast<int32_t>(ox);
+    out.sy = static_cast<int32_t>(oy);
+    out.plane = static_cast<int32_t>(plane);
+    out.conj = w > 0.0 ? 1 : 0;
+    return true;
+}
+
+// Validity filters ahead of planning.
+//  standard: execution.rs:18270-18300 (gridable, weight and sumwt_factor
+//            finite and positive);
+//  W:        raw_w_project_sample_is_valid, lib.rs:73666-73676.
+__device__ __forceinline__ bool sample_accepted(const PlanParams &p, const SampleIn &s,
+                                                Plan &plan) {
+    if (!s.usable) return false;
+    if (!(finite_f(s.weight) && s.weight > 0.0f && finite_f(s.sf) && s.sf > 0.0f)) return false;
+    if (p.wproject) {
+        if (!(finite_f(s.vis.x) && finite_f(s.vis.y) && isfinite(s.u) && isfinite(s.v) &&
+              isfinite(s.w)))
+            return false;
+    }
+    return plan_sample(p, s.u, s.v, s.w, plan);
+}
+
+__device__ __forceinline__ uint2 pack_plan(const PlanParams &p, const Plan &pl, uint32_t flags) {
+    uint2 r;
+    r.x = static_cast<uint32_t>(pl.x) | (static_cast<uint32_t>(pl.y) << 16);
+    uint32_t sx, sy;
+    if (p.wproject) {
+        sx = static_cast<uint32_t>(pl.sx + 128);
+        sy = static_cast<uint32_t>(pl.sy + 128);
+    } else {
+        sx = static_cast<uint32_t>(pl.sx);
+        sy = static_cast<uint32_t>(pl.sy);
+    }
+    r.y = (sx & 0xffu) | ((sy & 0xffu) << 8) | ((static_cast<uint32_t>(pl.plane) & 0x7ffu) << 16) |
+          (pl.conj ? kConjBit : 0u) | flags;
+    return r;
+}
+
+// Weight a sample carries into the grids: weight * sumwt_factor (f32) for the
+// standard gridder (execution.rs:18306), weight alone for W-projection
+// (lib.rs:71499-71505).
+__device__ __forceinline__ float grid_weight(const PlanParams &p, const SampleIn &s) {
+    return p.wproject ? s.weight : s.weight * s.sf;
+}
+
+__device__ __forceinline__ uint32_t sample_flags(const PlanParams &p, const SampleIn &s) {
+    const bool vis_ok = finite_f(s.vis.x) && finite_f(s.vis.y);
+    uint32_t f = vis_ok ? kVisOkBit : 0u;
+    const float gw = grid_weight(p, s);
+    if (vis_ok && finite_f(gw) && gw > 0.0f) f |= kResidOkBit;
+    return f;
+}
+
+// Wavefront-wide sum of a double (64 lanes).
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace chip

@@ -1,0 +1,198 @@
+// Internal declarations shared by the HIP translation units of
+// libcasa_imaging_hip.so.  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/casa_imaging_hip.h"
+
+namespace chip {
+
+constexpr int kSupport = 3;  // STANDARD_GRIDDER_SUPPORT, gridder.rs:15
+constexpr int kTaps = 7;     // STANDARD_GRIDDER_TAP_COUNT, gridder.rs:16
+
+// Error carrying a C-ABI status code.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+
+#define CHIP_HIP(expr)                                                                     \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess)                                                              \
+            ::chip::fail(CHIP_ERR_DEVICE, std::string(#expr " failed: ") +                 \
+                                              hipGetErrorString(_e));                      \
+    } while (0)
+
+#define CHIP_FFT(expr)                                                                     \
+    do {                                                                                   \
+        hipfftResult _r = (expr);                                                          \
+        if (_r != HIPFFT_SUCCESS)                                                          \
+            ::chip::fail(CHIP_ERR_DEVICE,                                                  \
+                         std::string(#expr " failed with hipfft status ") +                \
+                             std::to_string(static_cast<int>(_r)));                        \
+    } while (0)
+
+// Owning device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t n) { alloc(n); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t n) {
+        release();
+        if (n) CHIP_HIP(hipMalloc(&p, n));
+        bytes = n;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+// hipFFT plans cached per (gx, gy, batch, type).
+struct FftCache {
+    std::map<std::tuple<uint64_t, uint64_t, uint64_t, int>, hipfftHandle> plans;
+    hipfftHandle get(uint64_t gx, uint64_t gy, uint64_t batch, hipfftType type, hipStream_t s);
+    void clear();
+    ~FftCache() { clear(); }
+};
+
+}  // namespace chip
+
+struct chip_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    chip::FftCache fft;
+};
+
+// Host-side tables of a StandardGridder and, optionally, its WProjector.
+struct chip_gridder {
+    chip_ctx *ctx = nullptr;
+    chip_geometry geom{};
+    uint64_t gx = 0, gy = 0, blc_x = 0, blc_y = 0;
+    double du = 0, dv = 0;
+    uint32_t os = 0;
+    std::vector<float> taps;          // (os+1) x 7
+    std::vector<float> corr_x, corr_y;
+    chip::DevBuf d_taps, d_corr_x, d_corr_y;
+
+    // W projection
+    bool wproject = false;
+    uint32_t planes = 1, sampling = 1, width = 0, max_support = 0;
+    int32_t norm_radius = 0;
+    double w_scale = 1.0;
+    std::vector<int32_t> supports;
+    std::vector<float> norm_table;  // planes x (2R+1)^2
+    chip::DevBuf d_kernels;         // planes x width x width float2
+    chip::DevBuf d_supports, d_norm_table;
+    std::vector<float> sinc;        // build_sinc_axis(max(gx, gy), sampling)
+    std::vector<float> sinc_x, sinc_y;  // per-axis sinc used by model apodisation
+    chip::DevBuf d_sinc, d_sinc_x, d_sinc_y;
+};
+
+namespace chip {
+
+// ---- host math (tables.cpp) ----
+double grdsf(double nu);
+float spheroidal_kernel(double distance, double support);
+uint64_t padded_len(uint64_t n, double padding);
+bool is_composite(uint64_t n);
+uint64_t composite_padded_len(uint64_t n, double padding);
+std::vector<float> correction_axis(uint64_t n);
+std::vector<float> gridder_correction_axis(uint64_t n);
+std::vector<float> sinc_axis(uint64_t n, uint64_t sampling);
+std::vector<float> tap_weights(uint32_t os);
+uint32_t suggested_w_plane_count(const chip_geometry &g, double max_abs_w);
+double w_max_uvw(const chip_geometry &g, double max_abs_w, bool explicit_planes);
+uint32_t find_w_support(const std::vector<float> &axis_mag_over_peak, uint32_t conv,
+                        uint32_t sampling);
+
+// ---- device launch helpers ----
+// Parameters of the FP64 planner, passed by value to kernels.
+struct PlanParams {
+    uint64_t gx, gy;
+    double du, dv;
+    double half_gx, half_gy;  // gx/2.0, gy/2.0
+    uint32_t os;              // standard tap oversampling
+    uint32_t wproject;
+    uint32_t sampling;
+    uint32_t planes;
+    double w_scale;
+    int32_t norm_radius;
+    const int32_t *supports;  // device
+    const float *norm_table;  // device
+};
+
+PlanParams plan_params(const chip_gridder &g);
+
+// Source descriptor for device kernels (mirror of chip_samples).
+struct Source {
+    uint64_t n_rows, n_chan;
+    const double *u, *v, *w, *uvw_m, *lambda_scale;
+    const float2 *vis;
+    const float *weight, *sumwt_factor;
+    const uint8_t *gridable, *flag;
+};
+Source make_source(const chip_samples &s);
+void validate_source(const chip_samples &s);
+
+void launch_plan_raw(const chip_gridder &g, const Source &src, int32_t *plan, float *norm,
+                     hipStream_t st);
+double launch_max_abs_w(const chip_gridder &g, const Source &src, hipStream_t st);
+void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
+                    uint32_t sampling);
+
+void grid_plain(chip_gridder &g, const Source &src, uint32_t mode, uint32_t precision,
+                const void *model, void *psf, void *dirty, chip_sums *sums);
+void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,
+                  float *out);
+
+void finish_products(chip_gridder &g, uint32_t precision, uint64_t batch, void *psf, void *dirty,
+                     const double *nsumwt, float *psf_out, float *res_out, float *peaks);
+void model_to_grid(chip_gridder &g, uint32_t precision, uint64_t batch, const float *model,
+                   void *grid_out);
+void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *image_out);
+
+}  // namespace chip
+
+// Resident, binned visibility stream.
+struct chip_vset {
+    chip_gridder *g = nullptr;
+    uint64_t n_samples = 0;   // input samples (rows x channels)
+    uint64_t n_chan = 0;
+    uint64_t n_valid = 0;     // accepted samples in the stream
+    uint32_t tile = 32;       // uv tile edge (cells)
+    uint32_t halo = 3;        // max half-support
+    uint32_t tiles_x = 0, tiles_y = 0;
+    uint64_t n_items = 0;
+    // stream (sorted by channel, tile)
+    chip::DevBuf plan;    // uint2 per sample
+    chip::DevBuf vis;     // float2
+    chip::DevBuf wt;      // float
+    chip::DevBuf index;   // uint32 original sample index
+    chip::DevBuf items;   // WorkItem
+    std::vector<chip_sums> sums;
+};

@@ -1,0 +1,221 @@
+/*
+ * casa_imaging_hip.h -- C ABI of libcasa_imaging_hip.so, the MI355X (gfx950)
+ * gridding / degridding path of casa-imaging.
+ *
+ * The entry points replace the device seams of bglenden/casa-rs
+ * `crates/casa-imaging` that return ImagingError::Unsupported on non-macOS
+ * hosts (lib.rs:71813-71838, :72454-72472, :68057-68070).  Conventions follow
+ * the reference's own foreign-oracle shim (crates/casa-test-support/src/cpp/
+ * casacore_cpp_gridder_shim.cpp:23-60, casacore_cpp_common.h:47-55):
+ *   - every call returns an int status (CHIP_OK = 0) and, on failure, sets
+ *     *err to a malloc'd message that the caller frees with chip_error_free;
+ *   - the status maps onto ImagingError (crates/casa-imaging/src/error.rs:8-24):
+ *       CHIP_ERR_INVALID       -> InvalidRequest
+ *       CHIP_ERR_NO_SAMPLES    -> NoUsableSamples
+ *       CHIP_ERR_NORMALIZATION -> Normalization
+ *       CHIP_ERR_DEVICE        -> Unsupported (no usable HIP device / runtime error)
+ *   - calls are synchronous on return (the Metal path commits and waits,
+ *     lib.rs:72289-72290); one context per host thread.
+ *
+ * Memory: every array argument whose name ends in `_dev` is device memory on
+ * the context's device (hipMalloc'd or a torch CUDA tensor).  Grids are the
+ * reference's Array2 layout: logical (x, y), y fastest, flat index x*gy + y
+ * (gridder.rs:61-66), complex values interleaved (re, im), one grid per
+ * channel, channels outermost.  Grid precision CHIP_F32 means complex64
+ * (float2) grids, CHIP_F64 means complex128 (double2) grids.
+ */
+#ifndef CASA_IMAGING_HIP_H
+#define CASA_IMAGING_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CHIP_OK 0
+#define CHIP_ERR_INVALID 1
+#define CHIP_ERR_NO_SAMPLES 2
+#define CHIP_ERR_NORMALIZATION 3
+#define CHIP_ERR_DEVICE 4
+
+/* Grid modes of accumulate_w_project_grid_metal_samples (lib.rs:71688-71690). */
+#define CHIP_MODE_PSF 0      /* PSF grid only */
+#define CHIP_MODE_DIRTY 1    /* PSF + dirty (V*w) grids */
+#define CHIP_MODE_RESIDUAL 2 /* degrid model grid, grid (V - model)*w */
+
+#define CHIP_F32 0 /* binned, LDS-tiled, FP32 complex MACs, complex64 grids */
+#define CHIP_F64 1 /* plain per-channel launch, FP64 MACs and atomics, complex128 grids */
+
+/* Padding rule of StandardGridder (gridder.rs:371-390). */
+#define CHIP_PAD_COMPOSITE_1_2 0 /* StandardGridder::new */
+#define CHIP_PAD_COMPOSITE_1_3 1 /* new_with_casa_composite_padding (model predictor) */
+#define CHIP_PAD_NONE 2          /* new_unpadded */
+
+typedef struct chip_ctx chip_ctx;
+typedef struct chip_gridder chip_gridder;
+typedef struct chip_vset chip_vset;
+
+/* ImageGeometry (types.rs:885-925) plus the gridder's padding rule and tap-table
+ * oversampling (the reference fixes 100, gridder.rs:407). */
+typedef struct {
+    uint64_t nx, ny;
+    double cell_x, cell_y; /* radians, finite and positive */
+    uint32_t padding_mode;
+    uint32_t oversampling; /* even, 2..254 */
+} chip_geometry;
+
+typedef struct {
+    uint64_t gx, gy;       /* padded grid shape */
+    uint64_t blc_x, blc_y; /* image origin in the grid, gridder.rs:403-406 */
+    double du, dv;         /* grid spacing in wavelengths, gridder.rs:426-427 */
+    uint32_t oversampling;
+    uint32_t wproject;     /* 1 once chip_gridder_attach_wproject succeeded */
+    uint32_t planes;       /* W planes */
+    uint32_t sampling;     /* W kernel oversampling */
+    uint32_t width;        /* W kernel quarter-plane width */
+    uint32_t max_support;  /* largest W half-support */
+    uint32_t norm_radius;  /* (sampling + 1) / 2 */
+    uint32_t _pad;
+    double w_scale;
+} chip_gridder_info;
+
+/* Per-channel accumulation summary (WProjectGridAccumulation, lib.rs:70978-70993;
+ * StandardMfsDirtyAccumulation).  Sums are accumulated in FP64 on the device. */
+typedef struct {
+    double normalization_sumwt;
+    double reported_sumwt;
+    uint64_t gridded;
+    uint64_t skipped;
+} chip_sums;
+
+/* Visibility input, device resident.  Two shapes share one descriptor:
+ *  - per-sample wavelengths (VisibilityBatch, types.rs:1376-1405): set u,v,w
+ *    (length n_rows), n_chan = 1, uvw_m = lambda_scale = NULL;
+ *  - rows x channels (casars-imager's MS rows): set uvw_m (n_rows x 3 metres)
+ *    and lambda_scale (n_chan values of freq/c, casars-imager/src/lib.rs:557),
+ *    u = v = w = NULL.  Sample (row, chan) is index row*n_chan + chan and its
+ *    wavelength coordinates are uvw_m[row][k] * lambda_scale[chan] in FP64.
+ * weight is required; sumwt_factor (NULL = 1), gridable (NULL = all true) and
+ * flag (NULL = none flagged, nonzero = flagged) are optional, one per sample. */
+typedef struct {
+    uint64_t n_rows;
+    uint64_t n_chan;
+    const double *u_dev, *v_dev, *w_dev;
+    const double *uvw_m_dev;
+    const double *lambda_scale_dev;
+    const float *vis_dev; /* complex64 interleaved, one per sample */
+    const float *weight_dev;
+    const float *sumwt_factor_dev;
+    const uint8_t *gridable_dev;
+    const uint8_t *flag_dev;
+} chip_samples;
+
+/* ---- context ------------------------------------------------------------ */
+int chip_ctx_create(int device, chip_ctx **out, char **err);
+void chip_ctx_destroy(chip_ctx *ctx);
+void chip_error_free(char *err);
+const char *chip_version(void);
+
+/* ---- gridder (StandardGridder::new_with_padding_factor, gridder.rs:392-443) */
+int chip_gridder_create(chip_ctx *ctx, const chip_geometry *geometry, chip_gridder **out,
+                        char **err);
+void chip_gridder_destroy(chip_gridder *g);
+int chip_gridder_info_get(const chip_gridder *g, chip_gridder_info *out, char **err);
+/* Host copies of the tap table ((os+1) x 7, gridder.rs:2026-2051) and grid
+ * correction axes (gridder.rs:4175-4188). Any pointer may be NULL. */
+int chip_gridder_tables(const chip_gridder *g, float *taps, float *corr_x, float *corr_y,
+                        char **err);
+
+/* Largest |w| over usable samples (w_project_raw_sample_summary, lib.rs:73642-73664). */
+int chip_max_abs_w(chip_gridder *g, const chip_samples *s, double *out, char **err);
+
+/* WProjector::new (gridder.rs:3403-3586) built on the GPU: screen in FP64
+ * phase, FP32 FFT per plane, 1e-3 support threshold, crop, plane-zero peak and
+ * integral normalisation, per-(plane, offset) normalisation table.
+ * explicit_planes = 0 selects the automatic plane count (gridder.rs:3801-3830);
+ * sampling = 0 selects the reference rule (4 if planes > 1 else 1, :3413). */
+int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t explicit_planes,
+                                 uint32_t sampling, char **err);
+/* Host copies: supports[planes], kernels[planes][width][width] complex64,
+ * norm_table[planes][2R+1][2R+1].  Any pointer may be NULL. */
+int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels,
+                         float *norm_table, char **err);
+
+/* ---- FP64 planning ------------------------------------------------------ */
+/* Per-sample plan, bit-exact with the reference CPU planner:
+ *  standard (no W attached): plan_positive_taps (gridder.rs:557-566, :1958-1974)
+ *     -> {x_start, y_start, wix, wiy, 0, 0}
+ *  W-projection: WProjector::plan_sample (gridder.rs:3588-3624)
+ *     -> {loc_x, loc_y, off_x, off_y, plane, conj}
+ * Rejected samples (validity filters of execution.rs:18270-18300 /
+ * lib.rs:73666-73676, out of grid) get all -1.  plan_dev: int32[n][6];
+ * norm_dev (may be NULL): W normalisation per sample, NaN when rejected. */
+int chip_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, float *norm_dev,
+              char **err);
+
+/* ---- resident visibility set: plan once, bin, grid/degrid many times ------ */
+/* Plans every sample in FP64, sorts the accepted ones by (channel, uv tile)
+ * with an integer radix sort, and stores a packed 20-byte stream plus work
+ * items of bounded size.  The input arrays may be freed afterwards. */
+int chip_vset_create(chip_gridder *g, const chip_samples *s, chip_vset **out, char **err);
+void chip_vset_destroy(chip_vset *vs);
+/* sums_host: n_chan entries (may be NULL). */
+int chip_vset_sums(const chip_vset *vs, chip_sums *sums_host, char **err);
+/* One launch over every (channel, tile) work item.  Grids are CHIP_F32
+ * complex64, n_chan grids each; they are accumulated into, not cleared.
+ * mode PSF/DIRTY use psf_dev (and dirty_dev); RESIDUAL reads model_dev
+ * (may be NULL = zero model) and writes dirty_dev. */
+int chip_vset_grid(chip_vset *vs, uint32_t mode, const float *model_dev, float *psf_dev,
+                   float *dirty_dev, char **err);
+/* Degrid model_dev into vis_out_dev (complex64, one per input sample, in input
+ * order); rejected samples are written as 0.  This is the forward operator
+ * (StandardGridder::degrid_sample_taps_planned_normalized, gridder.rs:1278-1312;
+ * WProjector::degrid_sample_planned, :3726-3747). */
+int chip_vset_degrid(chip_vset *vs, const float *model_dev, float *vis_out_dev, char **err);
+/* Exact adjoint of chip_vset_degrid: grids values_dev (complex64, one per input
+ * sample, input order) with unit weight into grid_dev (accumulated), i.e.
+ * grid += value * tap (gridder.rs:3680-3699 with value in place of V*w). */
+int chip_vset_grid_values(chip_vset *vs, const float *values_dev, float *grid_dev, char **err);
+
+/* ---- plain per-channel path (CHIP_F64 or CHIP_F32 grids, no binning) ------ */
+/* One launch per channel, one thread per sample, global atomics.  CHIP_F64
+ * accumulates complex128 grids with FP64 MACs; CHIP_F32 complex64 grids with
+ * FP32 MACs.  sums_host: n_chan entries (may be NULL). */
+int chip_grid_plain(chip_gridder *g, const chip_samples *s, uint32_t mode, uint32_t precision,
+                    const void *model_dev, void *psf_dev, void *dirty_dev, chip_sums *sums_host,
+                    char **err);
+int chip_degrid_plain(chip_gridder *g, const chip_samples *s, uint32_t precision,
+                      const void *model_dev, float *vis_out_dev, char **err);
+
+/* ---- grid <-> image ------------------------------------------------------ */
+/* Dirty products (dirty_grids_to_psf_and_residual_owned, lib.rs:48987-49131;
+ * W-projection: raw_psf_grid_to_w_project_state, lib.rs:71009-71039 and
+ * :70883-70885): centered inverse FFT scaled 1/(gx*gy), crop, grid correction
+ * (x 1/sinc for W), PSF / sumwt -> peak -> / peak, residual / sumwt / peak.
+ * psf_grids_dev / dirty_grids_dev hold `batch` grids and are overwritten by
+ * the transform.  psf_grids_dev may be NULL (residual only): then
+ * psf_peak_host supplies the peaks.  Otherwise psf_peak_host receives them.
+ * normalization_sumwt_host: batch values. */
+int chip_finish_products(chip_gridder *g, uint32_t precision, uint64_t batch, void *psf_grids_dev,
+                         void *dirty_grids_dev, const double *normalization_sumwt_host,
+                         float *psf_out_dev, float *residual_out_dev, float *psf_peak_host,
+                         char **err);
+/* Model image(s) -> model grid(s): apodize (x grid correction, x sinc for W,
+ * gridder.rs:1554-1592) into a zero grid, unnormalised centered forward FFT
+ * (fft.rs:34). model_dev: batch x nx x ny float; grid_out_dev: batch grids. */
+int chip_model_to_grid(chip_gridder *g, uint32_t precision, uint64_t batch, const float *model_dev,
+                       void *grid_out_dev, char **err);
+/* Adjoint of chip_model_to_grid for real images: Re(factor * crop(F^H grid))
+ * with F^H the unnormalised centered inverse transform (the conjugate
+ * transpose of the forward model transform).  grid_dev (complex64) is
+ * overwritten.  image_out_dev: batch x nx x ny float. */
+int chip_grid_to_model_adjoint(chip_gridder *g, uint64_t batch, float *grid_dev,
+                               float *image_out_dev, char **err);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CASA_IMAGING_HIP_H */

@@ -1,0 +1,784 @@
+/*
+ * casa_oracle.c -- CPU oracle for the casa-imaging gridding/degridding path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file restates, in plain C, the reference CPU
+ * algorithm of bglenden/casa-rs `crates/casa-imaging` so that the HIP library can
+ * be checked against it.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * CPU-baseline leg may load it; the product library never links or calls it.
+ *
+ * Every function names the reference file:line it follows.  Arithmetic is kept
+ * in the same precision and the same operation order as the reference
+ * (f32 tap products widened to f64, f32 complex products for W-projection,
+ * sequential f32 sums for kernel normalisation) and the file is compiled with
+ * -ffp-contract=off so that no multiply-add is fused where Rust fuses none.
+ *
+ * Grid layout follows ndarray's Array2 default: logical (x, y), y fastest,
+ * flat index x*ny + y  (gridder.rs:61-66, :711-715).  Complex grids are stored
+ * interleaved (re, im).
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define CO_SUPPORT 3
+#define CO_TAPS 7
+
+/* ------------------------------------------------------------------------- */
+/* Closed-form pieces                                                         */
+/* ------------------------------------------------------------------------- */
+
+/* Schwab rational approximation of the prolate spheroidal function.
+ * gridder.rs:4230-4268.  `powi` is evaluated by repeated squaring, which is
+ * how both LLVM's constant-exponent expansion and __powidf2 evaluate it. */
+double co_grdsf(double nu) {
+    static const double P0[5] = {8.203343e-2, -3.644705e-1, 6.278660e-1, -5.335581e-1, 2.312756e-1};
+    static const double P1[5] = {4.028559e-3, -3.697768e-2, 1.021332e-1, -1.201436e-1, 6.412774e-2};
+    static const double Q0[3] = {1.0, 8.212018e-1, 2.078043e-1};
+    static const double Q1[3] = {1.0, 9.599102e-1, 2.918724e-1};
+    if (!(nu >= 0.0 && nu <= 1.0)) return 0.0;
+    const double *p, *q;
+    double nu_end;
+    if (nu < 0.75) { p = P0; q = Q0; nu_end = 0.75; }
+    else { p = P1; q = Q1; nu_end = 1.0; }
+    double d = nu * nu - nu_end * nu_end;
+    double pw[5];
+    pw[0] = 1.0;
+    pw[1] = d;
+    pw[2] = d * d;
+    pw[3] = d * pw[2];
+    pw[4] = pw[2] * pw[2];
+    double num = 0.0, den = 0.0;
+    for (int k = 0; k < 5; ++k) num = num + p[k] * pw[k];
+    for (int k = 0; k < 3; ++k) den = den + q[k] * pw[k];
+    if (den == 0.0) return 0.0;
+    return num / den;
+}
+
+/* gridder.rs:4200-4209 */
+float co_spheroidal_kernel(double distance, double support) {
+    if (!(isfinite(distance) && distance <= support)) return 0.0f;
+    double nu = distance / support;
+    if (nu > 1.0) return 0.0f;
+    return (float)((1.0 - nu * nu) * co_grdsf(nu));
+}
+
+/* gridder.rs:4152-4156 */
+uint64_t co_padded_len(uint64_t n, double padding) {
+    double f = floor(padding * (double)n - 0.5);
+    uint64_t padded = f <= 0.0 ? 0 : (uint64_t)f;
+    if (padded < n) padded = n;
+    return (padded % 2 == 0) ? padded : padded + 1;
+}
+
+/* gridder.rs:4166-4173 */
+int co_is_composite(uint64_t value) {
+    static const uint64_t factors[3] = {2, 3, 5};
+    for (int i = 0; i < 3; ++i)
+        while (value > 1 && value % factors[i] == 0) value /= factors[i];
+    return value == 1;
+}
+
+/* gridder.rs:4158-4164 */
+uint64_t co_composite_padded_len(uint64_t n, double padding) {
+    uint64_t padded = co_padded_len(n, padding);
+    while (!co_is_composite(padded)) padded += 2;
+    return padded;
+}
+
+/* Grid correction 1/psi, zero where psi <= 1e-6.  gridder.rs:4175-4188 */
+void co_correction_axis(uint64_t size, float *out) {
+    double center = (double)size / 2.0;
+    for (uint64_t i = 0; i < size; ++i) {
+        double nu = fabs((double)i - center) / center;
+        if (nu < 0.0) nu = 0.0;
+        if (nu > 1.0) nu = 1.0;
+        double v = co_grdsf(nu);
+        out[i] = v > 1.0e-6 ? (float)(1.0 / v) : 0.0f;
+    }
+}
+
+/* psi itself, used to taper the W screen.  gridder.rs:4190-4198 */
+void co_gridder_correction_axis(uint64_t size, float *out) {
+    double center = (double)size / 2.0;
+    for (uint64_t i = 0; i < size; ++i) {
+        double nu = fabs((double)i - center) / center;
+        if (nu < 0.0) nu = 0.0;
+        if (nu > 1.0) nu = 1.0;
+        out[i] = (float)co_grdsf(nu);
+    }
+}
+
+/* Float sinc of the sampling-cell offset.  gridder.rs:4211-4228 */
+void co_sinc_axis(uint64_t size, uint64_t conv_sampling, float *out) {
+    int64_t half = (int64_t)(size / 2);
+    for (uint64_t i = 0; i < size; ++i) {
+        float offset = (float)((int64_t)i - half);
+        float denominator = (float)size * (float)conv_sampling;
+        float argument = (float)(3.14159265358979323846 * (double)offset / (double)denominator);
+        out[i] = (i == size / 2) ? 1.0f : sinf(argument) / argument;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* StandardGridder (prolate spheroidal, 7x7)                                  */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint64_t nx, ny;       /* image shape */
+    double cell_x, cell_y; /* radians */
+    uint64_t gx, gy;       /* grid shape */
+    uint64_t blc_x, blc_y; /* image origin inside the grid */
+    uint64_t os;           /* oversampling of the tap table */
+    double du, dv;         /* grid spacing in wavelengths */
+} co_std;
+
+/* StandardGridder::new_with_padding_factor, gridder.rs:392-443.
+ * padding_mode 0: composite padding 1.2 (`new`, :371-373, :385-390)
+ * padding_mode 1: composite padding 1.3 (`new_with_casa_composite_padding`, :379-383)
+ * padding_mode 2: unpadded (`new_unpadded`, :375-377).
+ * The reference fixes os = 100 (:407); `os` is a parameter here so that the
+ * oversampling-128 configuration can be restated with the same formulas. */
+int co_std_init(co_std *g, uint64_t nx, uint64_t ny, double cell_x, double cell_y,
+                int padding_mode, uint64_t os) {
+    if (nx < 8 || ny < 8) return 1; /* ImageGeometry::validate, types.rs:910-925 */
+    if (!(isfinite(cell_x) && cell_x > 0.0 && isfinite(cell_y) && cell_y > 0.0)) return 1;
+    if (os < 2) return 1;
+    g->nx = nx; g->ny = ny; g->cell_x = cell_x; g->cell_y = cell_y; g->os = os;
+    if (padding_mode == 0) { g->gx = co_composite_padded_len(nx, 1.2); g->gy = co_composite_padded_len(ny, 1.2); }
+    else if (padding_mode == 1) { g->gx = co_composite_padded_len(nx, 1.3); g->gy = co_composite_padded_len(ny, 1.3); }
+    else { g->gx = nx; g->gy = ny; }
+    g->blc_x = (g->gx - nx + (g->gx % 2 == 0 ? 1 : 0)) / 2;
+    g->blc_y = (g->gy - ny + (g->gy % 2 == 0 ? 1 : 0)) / 2;
+    g->du = 1.0 / ((double)g->gx * cell_x);
+    g->dv = 1.0 / ((double)g->gy * cell_y);
+    return 0;
+}
+
+/* kernel_table[os*(S+1)], first os*S entries filled.  gridder.rs:407-416 */
+void co_std_kernel_table(uint64_t os, float *table) {
+    uint64_t len = os * (CO_SUPPORT + 1);
+    for (uint64_t i = 0; i < len; ++i) table[i] = 0.0f;
+    for (uint64_t i = 0; i < os * CO_SUPPORT; ++i) {
+        double distance = (double)i / ((double)CO_SUPPORT * (double)os);
+        table[i] = co_spheroidal_kernel(distance * (double)CO_SUPPORT, (double)CO_SUPPORT);
+    }
+}
+
+/* Per-offset 7-tap weights normalised to sum 1.  gridder.rs:2026-2051.
+ * Output rows: 2*(os/2)+1, row r <-> offset r - os/2. */
+void co_std_tap_weights(uint64_t os, float *out) {
+    uint64_t len = os * (CO_SUPPORT + 1);
+    float *table = (float *)malloc(len * sizeof(float));
+    co_std_kernel_table(os, table);
+    int64_t half = (int64_t)os / 2;
+    uint64_t row = 0;
+    for (int64_t offset = -half; offset <= half; ++offset, ++row) {
+        float w[CO_TAPS];
+        float norm = 0.0f;
+        for (int tap = 0; tap < CO_TAPS; ++tap) {
+            int64_t delta = tap - CO_SUPPORT;
+            int64_t lookup = delta * (int64_t)os + offset;
+            if (lookup < 0) lookup = -lookup;
+            float weight = (uint64_t)lookup < len ? table[lookup] : 0.0f;
+            w[tap] = weight;
+            norm += weight;
+        }
+        if (norm > 0.0f)
+            for (int tap = 0; tap < CO_TAPS; ++tap) w[tap] /= norm;
+        memcpy(out + row * CO_TAPS, w, sizeof(w));
+    }
+    free(table);
+}
+
+/* sample_tap_span, gridder.rs:1958-1974 (with normalized_tap_weight_index :1991-1995).
+ * Returns 1 and writes (start, weight_index) when the 7 taps fit the axis. */
+static int co_tap_span(double coordinate, uint64_t size, uint64_t os, int64_t *start,
+                       int64_t *weight_index) {
+    if (!isfinite(coordinate)) return 0;
+    /* `as isize` saturates; anything this far out is rejected anyway. */
+    if (coordinate > 4.0e18 || coordinate < -4.0e18) return 0;
+    double anchor_f = round(coordinate);
+    int64_t anchor = (int64_t)anchor_f;
+    int64_t offset = (int64_t)round(((double)anchor - coordinate) * (double)os);
+    int64_t s = anchor - CO_SUPPORT, e = anchor + CO_SUPPORT;
+    if (s < 0 || e >= (int64_t)size) return 0;
+    int64_t half = (int64_t)os / 2;
+    int64_t index = offset + half;
+    if (index < 0 || index >= 2 * half + 1) return 0;
+    *start = s;
+    *weight_index = index;
+    return 1;
+}
+
+/* grid_coordinate_x/y, gridder.rs:1877-1886 */
+static double co_std_coord_x(const co_std *g, double u) { return u / g->du + (double)g->gx / 2.0; }
+static double co_std_coord_y(const co_std *g, double v) { return -v / g->dv + (double)g->gy / 2.0; }
+
+/* plan_positive_taps, gridder.rs:557-566.  out = {x_start, y_start, wix, wiy}. */
+int co_std_plan(const co_std *g, double u, double v, int64_t *out) {
+    int64_t xs, xw, ys, yw;
+    if (!co_tap_span(co_std_coord_x(g, u), g->gx, g->os, &xs, &xw)) return 0;
+    if (!co_tap_span(co_std_coord_y(g, v), g->gy, g->os, &ys, &yw)) return 0;
+    out[0] = xs; out[1] = ys; out[2] = xw; out[3] = yw;
+    return 1;
+}
+
+void co_std_plan_many(const co_std *g, uint64_t n, const double *u, const double *v, int64_t *out) {
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!co_std_plan(g, u[i], v[i], out + 4 * i)) {
+            out[4 * i] = out[4 * i + 1] = out[4 * i + 2] = out[4 * i + 3] = -1;
+        }
+    }
+}
+
+typedef struct {
+    double normalization_sumwt;
+    double reported_sumwt;
+    uint64_t gridded;
+    uint64_t skipped;
+} co_sums;
+
+/* ---- threaded private-grid accumulation with ordered merge ----
+ * The reference splits the sample range into `threads` contiguous chunks, each
+ * worker accumulates into private grids, and the partial grids are added to
+ * the output in worker order (lib.rs:71204-71286, merge :71133-71139;
+ * standard streaming lib.rs:68306-68428). */
+
+typedef struct {
+    const co_std *g;
+    const float *taps;
+    uint64_t begin, end;
+    const double *u, *v;
+    const float *weight, *sumwt_factor, *vis;
+    const uint8_t *gridable;
+    const float *model; /* complex f32 model grid (residual mode) */
+    int mode;           /* 0 psf, 1 dirty (psf + residual), 2 residual (degrid+grid) */
+    double *psf, *dirty;
+    co_sums sums;
+} co_std_job;
+
+static inline int co_finite_vis(const float *vis) { return isfinite(vis[0]) && isfinite(vis[1]); }
+
+/* accumulate_batches, execution.rs:18255-18338 (mode 0/1), and the residual
+ * refresh per-sample path, execution.rs:8890-8940 with
+ * degrid_model_and_grid_residual_taps_planned_f64, gridder.rs:1315-1396 (mode 2). */
+static void *co_std_worker(void *arg) {
+    co_std_job *j = (co_std_job *)arg;
+    const co_std *g = j->g;
+    const uint64_t stride = g->gy;
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        if (j->gridable && !j->gridable[i]) { j->sums.skipped++; continue; }
+        float weight = j->weight[i];
+        float sf = j->sumwt_factor[i];
+        const float *vis = j->vis + 2 * i;
+        if (j->mode == 2) {
+            if (!(isfinite(weight) && weight > 0.0f)) { j->sums.skipped++; continue; }
+            if (!co_finite_vis(vis)) { j->sums.skipped++; continue; }
+        } else if (!(isfinite(weight) && weight > 0.0f && isfinite(sf) && sf > 0.0f)) {
+            j->sums.skipped++;
+            continue;
+        }
+        int64_t plan[4];
+        if (!co_std_plan(g, j->u[i], j->v[i], plan)) { j->sums.skipped++; continue; }
+        float grid_weight = weight * sf;
+        if (j->mode == 2 && !(isfinite(grid_weight) && grid_weight > 0.0f)) { j->sums.skipped++; continue; }
+        double sumwt = (double)grid_weight;
+        const float *wx = j->taps + plan[2] * CO_TAPS;
+        const float *wy = j->taps + plan[3] * CO_TAPS;
+        j->sums.gridded++;
+        if (j->mode != 2) {
+            j->sums.normalization_sumwt += sumwt;
+            j->sums.reported_sumwt += sumwt;
+            int vis_ok = co_finite_vis(vis);
+            double rr = (double)vis[0] * sumwt, ri = (double)vis[1] * sumwt;
+            for (int tx = 0; tx < CO_TAPS; ++tx) {
+                uint64_t xi = (uint64_t)(plan[0] + tx) * stride;
+                float xw = wx[tx];
+                for (int ty = 0; ty < CO_TAPS; ++ty) {
+                    double w = (double)(xw * wy[ty]);
+                    uint64_t idx = xi + (uint64_t)plan[1] + ty;
+                    j->psf[2 * idx] += sumwt * w;
+                    if (j->mode == 1 && vis_ok) {
+                        j->dirty[2 * idx] += rr * w;
+                        j->dirty[2 * idx + 1] += ri * w;
+                    }
+                }
+            }
+        } else {
+            float pr = 0.0f, pi = 0.0f;
+            if (j->model) {
+                for (int tx = 0; tx < CO_TAPS; ++tx) {
+                    uint64_t xi = (uint64_t)(plan[0] + tx) * stride;
+                    float xw = wx[tx];
+                    for (int ty = 0; ty < CO_TAPS; ++ty) {
+                        uint64_t idx = xi + (uint64_t)plan[1] + ty;
+                        float w = xw * wy[ty];
+                        pr += j->model[2 * idx] * w;
+                        pi += j->model[2 * idx + 1] * w;
+                    }
+                }
+            }
+            float rvr = vis[0] - pr, rvi = vis[1] - pi;
+            double rr = (double)rvr * sumwt, ri = (double)rvi * sumwt;
+            for (int tx = 0; tx < CO_TAPS; ++tx) {
+                uint64_t xi = (uint64_t)(plan[0] + tx) * stride;
+                float xw = wx[tx];
+                for (int ty = 0; ty < CO_TAPS; ++ty) {
+                    double w = (double)(xw * wy[ty]);
+                    uint64_t idx = xi + (uint64_t)plan[1] + ty;
+                    j->dirty[2 * idx] += rr * w;
+                    j->dirty[2 * idx + 1] += ri * w;
+                }
+            }
+        }
+    }
+    return NULL;
+}
+
+static void co_add_f64(double *dst, const double *src, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) dst[i] += src[i];
+}
+
+/* Accumulates into psf/dirty (f64 complex interleaved, gx*gy) in place.
+ * mode 0: PSF only, 1: PSF + dirty, 2: residual (model may be NULL). */
+void co_std_grid(const co_std *g, const float *taps, uint64_t n, const double *u, const double *v,
+                 const float *weight, const float *sumwt_factor, const uint8_t *gridable,
+                 const float *vis, const float *model, int mode, int threads, double *psf,
+                 double *dirty, co_sums *sums) {
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > n) threads = n > 0 ? (int)n : 1;
+    uint64_t cells = g->gx * g->gy * 2;
+    uint64_t chunk = (n + threads - 1) / threads;
+    co_std_job *jobs = (co_std_job *)calloc(threads, sizeof(co_std_job));
+    pthread_t *tids = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    for (int t = 0; t < threads; ++t) {
+        co_std_job *j = &jobs[t];
+        j->g = g; j->taps = taps; j->u = u; j->v = v; j->weight = weight;
+        j->sumwt_factor = sumwt_factor; j->vis = vis; j->gridable = gridable; j->model = model;
+        j->mode = mode;
+        j->begin = (uint64_t)t * chunk;
+        j->end = j->begin + chunk < n ? j->begin + chunk : n;
+        if (j->begin > n) j->begin = n;
+        if (threads == 1) { j->psf = psf; j->dirty = dirty; }
+        else {
+            j->psf = mode != 2 ? (double *)calloc(cells, sizeof(double)) : NULL;
+            j->dirty = mode != 0 ? (double *)calloc(cells, sizeof(double)) : NULL;
+        }
+        pthread_create(&tids[t], NULL, co_std_worker, j);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
+    for (int t = 0; t < threads; ++t) {
+        co_std_job *j = &jobs[t];
+        sums->normalization_sumwt += j->sums.normalization_sumwt;
+        sums->reported_sumwt += j->sums.reported_sumwt;
+        sums->gridded += j->sums.gridded;
+        sums->skipped += j->sums.skipped;
+        if (threads > 1) {
+            if (j->psf) { co_add_f64(psf, j->psf, cells); free(j->psf); }
+            if (j->dirty) { co_add_f64(dirty, j->dirty, cells); free(j->dirty); }
+        }
+    }
+    free(jobs);
+    free(tids);
+}
+
+/* degrid_sample_taps_planned_normalized, gridder.rs:1278-1312.
+ * Rejected samples produce NaN.  model: complex f32 grid. */
+void co_std_degrid(const co_std *g, const float *taps, uint64_t n, const double *u, const double *v,
+                   const float *model, float *out) {
+    for (uint64_t i = 0; i < n; ++i) {
+        int64_t plan[4];
+        if (!co_std_plan(g, u[i], v[i], plan)) { out[2 * i] = out[2 * i + 1] = NAN; continue; }
+        const float *wx = taps + plan[2] * CO_TAPS;
+        const float *wy = taps + plan[3] * CO_TAPS;
+        float re = 0.0f, im = 0.0f;
+        for (int tx = 0; tx < CO_TAPS; ++tx) {
+            uint64_t xi = (uint64_t)(plan[0] + tx) * g->gy;
+            float xw = wx[tx];
+            for (int ty = 0; ty < CO_TAPS; ++ty) {
+                float w = xw * wy[ty];
+                uint64_t idx = xi + (uint64_t)plan[1] + ty;
+                /* Complex32 * f32 then += : (re*w, im*w) */
+                float tr = model[2 * idx] * w, ti = model[2 * idx + 1] * w;
+                re += tr;
+                im += ti;
+            }
+        }
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* WProjector                                                                 */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint64_t gx, gy;
+    double du, dv;
+    uint64_t sampling;
+    double w_scale;
+    uint64_t planes;
+    uint64_t width;          /* common kernel width (quarter plane) */
+    const int32_t *support;  /* per plane */
+    const float *kernels;    /* planes * width * width complex f32, flat x*width + y */
+    int64_t norm_radius;     /* (sampling + 1) / 2 */
+    const float *norm_table; /* planes * (2R+1)^2 */
+} co_wproj;
+
+/* Only the pieces of WProjector::new that are sums; the screen FFT is done by
+ * the Python side of the oracle in f64 (gridder.rs:3438-3502).
+ * w_project_plane_sum, gridder.rs:3991-4002 (sequential f32 sum of Re at
+ * integer taps, iy outer, ix inner). */
+float co_w_plane_sum(const float *kernel, uint64_t width, int64_t support, uint64_t sampling) {
+    float sum = 0.0f;
+    for (int64_t iy = -support; iy <= support; ++iy) {
+        int64_t ky = iy * (int64_t)sampling; if (ky < 0) ky = -ky;
+        for (int64_t ix = -support; ix <= support; ++ix) {
+            int64_t kx = ix * (int64_t)sampling; if (kx < 0) kx = -kx;
+            sum += kernel[2 * ((uint64_t)kx * width + (uint64_t)ky)];
+        }
+    }
+    return sum;
+}
+
+/* compute_w_project_sample_normalization, gridder.rs:3782-3799.
+ * Returns 0 and sets *ok = 0 when a tap falls outside the stored quarter plane. */
+float co_w_sample_normalization(const float *kernel, uint64_t width, int64_t support,
+                                uint64_t sampling, int64_t off_x, int64_t off_y, int *ok) {
+    float sum = 0.0f;
+    *ok = 1;
+    for (int64_t iy = -support; iy <= support; ++iy) {
+        int64_t ky = iy * (int64_t)sampling + off_y; if (ky < 0) ky = -ky;
+        for (int64_t ix = -support; ix <= support; ++ix) {
+            int64_t kx = ix * (int64_t)sampling + off_x; if (kx < 0) kx = -kx;
+            if ((uint64_t)kx >= width || (uint64_t)ky >= width) { *ok = 0; return 0.0f; }
+            sum += kernel[2 * ((uint64_t)kx * width + (uint64_t)ky)];
+        }
+    }
+    return sum;
+}
+
+/* w_plane_index, gridder.rs:3749-3757 */
+static uint64_t co_w_plane_index(const co_wproj *p, double w) {
+    if (p->planes <= 1) return 0;
+    double c = p->w_scale * fabs(w);
+    if (c < 0.0) c = 0.0;
+    c = sqrt(c);
+    double r = round(c);
+    if (r < 0.0) r = 0.0;
+    if (r > (double)(p->planes - 1)) r = (double)(p->planes - 1);
+    return (uint64_t)r;
+}
+
+/* WProjector::plan_sample, gridder.rs:3588-3624 (sample_normalization :3759-3779).
+ * out = {loc_x, loc_y, off_x, off_y, plane, conj}; *norm = normalization. */
+int co_w_plan(const co_wproj *p, double u, double v, double w, int64_t *out, float *norm) {
+    double pos_x = u / p->du + (double)p->gx / 2.0;
+    double pos_y = -v / p->dv + (double)p->gy / 2.0;
+    if (!(isfinite(pos_x) && isfinite(pos_y) && isfinite(w))) return 0;
+    if (pos_x > 4.0e18 || pos_x < -4.0e18 || pos_y > 4.0e18 || pos_y < -4.0e18) return 0;
+    int64_t loc_x = (int64_t)round(pos_x), loc_y = (int64_t)round(pos_y);
+    int64_t off_x = (int64_t)round(((double)loc_x - pos_x) * (double)p->sampling);
+    int64_t off_y = (int64_t)round(((double)loc_y - pos_y) * (double)p->sampling);
+    uint64_t plane = co_w_plane_index(p, w);
+    int64_t support = p->support[plane];
+    if (support <= 0 || loc_x - support < 0 || loc_y - support < 0 ||
+        loc_x + support >= (int64_t)p->gx || loc_y + support >= (int64_t)p->gy)
+        return 0;
+    int64_t R = p->norm_radius;
+    float nrm;
+    if (off_x <= R && off_x >= -R && off_y <= R && off_y >= -R) {
+        uint64_t axis = (uint64_t)(2 * R + 1);
+        nrm = p->norm_table[plane * axis * axis + (uint64_t)(off_x + R) * axis + (uint64_t)(off_y + R)];
+    } else {
+        int ok;
+        nrm = co_w_sample_normalization(p->kernels + 2 * plane * p->width * p->width, p->width,
+                                        support, p->sampling, off_x, off_y, &ok);
+        if (!ok) return 0;
+    }
+    out[0] = loc_x; out[1] = loc_y; out[2] = off_x; out[3] = off_y;
+    out[4] = (int64_t)plane; out[5] = w > 0.0 ? 1 : 0;
+    *norm = nrm;
+    return 1;
+}
+
+void co_w_plan_many(const co_wproj *p, uint64_t n, const double *u, const double *v,
+                    const double *w, int64_t *out, float *norm) {
+    for (uint64_t i = 0; i < n; ++i) {
+        if (!co_w_plan(p, u[i], v[i], w[i], out + 6 * i, norm + i)) {
+            for (int k = 0; k < 6; ++k) out[6 * i + k] = -1;
+            norm[i] = NAN;
+        }
+    }
+}
+
+/* raw_w_project_sample_is_valid, lib.rs:73666-73676 */
+static int co_w_sample_valid(float weight, float sf, const float *vis, double u, double v, double w) {
+    return isfinite(weight) && weight > 0.0f && isfinite(sf) && sf > 0.0f && isfinite(vis[0]) &&
+           isfinite(vis[1]) && isfinite(u) && isfinite(v) && isfinite(w);
+}
+
+typedef struct {
+    const co_wproj *p;
+    uint64_t begin, end;
+    const double *u, *v, *w;
+    const float *weight, *sumwt_factor, *vis;
+    const uint8_t *gridable;
+    const float *model;
+    int mode; /* 0 psf, 1 dirty (psf+residual), 2 residual */
+    float *psf, *dirty;
+    co_sums sums;
+} co_w_job;
+
+/* WProjector::grid_sample_planned / _pair / degrid_sample_planned,
+ * gridder.rs:3680-3747, driven per sample as in
+ * accumulate_w_project_{psf,dirty,residual}_streaming_sample, lib.rs:71472-71540. */
+static void *co_w_worker(void *arg) {
+    co_w_job *j = (co_w_job *)arg;
+    const co_wproj *p = j->p;
+    const uint64_t W = p->width, stride = p->gy;
+    const int64_t s = (int64_t)p->sampling;
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        const float *vis = j->vis + 2 * i;
+        if ((j->gridable && !j->gridable[i]) ||
+            !co_w_sample_valid(j->weight[i], j->sumwt_factor[i], vis, j->u[i], j->v[i], j->w[i])) {
+            j->sums.skipped++;
+            continue;
+        }
+        int64_t plan[6];
+        float nrm;
+        if (!co_w_plan(p, j->u[i], j->v[i], j->w[i], plan, &nrm)) { j->sums.skipped++; continue; }
+        float weight = j->weight[i];
+        /* w_project_accumulation_for_sample, lib.rs:71460-71470 */
+        j->sums.normalization_sumwt += 2.0 * (double)weight * (double)nrm;
+        j->sums.reported_sumwt += (double)weight * (double)j->sumwt_factor[i];
+        j->sums.gridded++;
+        const int64_t S = p->support[plan[4]];
+        const float *K = p->kernels + 2 * (uint64_t)plan[4] * W * W;
+        const int conj = (int)plan[5];
+        float val_r, val_i;
+        if (j->mode == 2) {
+            float pr = 0.0f, pi = 0.0f;
+            if (j->model) {
+                for (int64_t iy = -S; iy <= S; ++iy) {
+                    int64_t ky = iy * s + plan[3]; if (ky < 0) ky = -ky;
+                    for (int64_t ix = -S; ix <= S; ++ix) {
+                        int64_t kx = ix * s + plan[2]; if (kx < 0) kx = -kx;
+                        float cr = K[2 * (kx * W + ky)], ci = K[2 * (kx * W + ky) + 1];
+                        if (conj) ci = -ci;
+                        /* conj(cwt) * grid */
+                        ci = -ci;
+                        uint64_t idx = (uint64_t)(plan[0] + ix) * stride + (uint64_t)(plan[1] + iy);
+                        float gr = j->model[2 * idx], gi = j->model[2 * idx + 1];
+                        float tr = cr * gr - ci * gi;
+                        float ti = cr * gi + ci * gr;
+                        pr += tr;
+                        pi += ti;
+                    }
+                }
+            }
+            /* (visibility - predicted) * weight, lib.rs:71536 */
+            float dr = vis[0] - pr, di = vis[1] - pi;
+            val_r = dr * weight;
+            val_i = di * weight;
+        } else {
+            val_r = vis[0] * weight;
+            val_i = vis[1] * weight;
+        }
+        for (int64_t iy = -S; iy <= S; ++iy) {
+            int64_t ky = iy * s + plan[3]; if (ky < 0) ky = -ky;
+            for (int64_t ix = -S; ix <= S; ++ix) {
+                int64_t kx = ix * s + plan[2]; if (kx < 0) kx = -kx;
+                float cr = K[2 * (kx * W + ky)], ci = K[2 * (kx * W + ky) + 1];
+                if (conj) ci = -ci;
+                uint64_t idx = (uint64_t)(plan[0] + ix) * stride + (uint64_t)(plan[1] + iy);
+                if (j->mode != 2) {
+                    /* PSF value = weight + 0i: (w*cr - 0*ci, w*ci + 0*cr) */
+                    float pr_ = weight * cr - 0.0f * ci;
+                    float pi_ = weight * ci + 0.0f * cr;
+                    j->psf[2 * idx] += pr_;
+                    j->psf[2 * idx + 1] += pi_;
+                }
+                if (j->mode != 0) {
+                    float tr = val_r * cr - val_i * ci;
+                    float ti = val_r * ci + val_i * cr;
+                    j->dirty[2 * idx] += tr;
+                    j->dirty[2 * idx + 1] += ti;
+                }
+            }
+        }
+    }
+    return NULL;
+}
+
+static void co_add_f32(float *dst, const float *src, uint64_t n) {
+    for (uint64_t i = 0; i < n; ++i) dst[i] += src[i];
+}
+
+/* Threaded as accumulate_w_project_streaming_parallel, lib.rs:71204-71286,
+ * with Complex32 private grids merged in worker order (add_complex32_grid,
+ * lib.rs:69280). */
+void co_w_grid(const co_wproj *p, uint64_t n, const double *u, const double *v, const double *w,
+               const float *weight, const float *sumwt_factor, const uint8_t *gridable,
+               const float *vis, const float *model, int mode, int threads, float *psf,
+               float *dirty, co_sums *sums) {
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > n) threads = n > 0 ? (int)n : 1;
+    uint64_t cells = p->gx * p->gy * 2;
+    uint64_t chunk = (n + threads - 1) / threads;
+    co_w_job *jobs = (co_w_job *)calloc(threads, sizeof(co_w_job));
+    pthread_t *tids = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    for (int t = 0; t < threads; ++t) {
+        co_w_job *j = &jobs[t];
+        j->p = p; j->u = u; j->v = v; j->w = w; j->weight = weight; j->sumwt_factor = sumwt_factor;
+        j->vis = vis; j->gridable = gridable; j->model = model; j->mode = mode;
+        j->begin = (uint64_t)t * chunk;
+        j->end = j->begin + chunk < n ? j->begin + chunk : n;
+        if (j->begin > n) j->begin = n;
+        if (threads == 1) { j->psf = psf; j->dirty = dirty; }
+        else {
+            j->psf = mode != 2 ? (float *)calloc(cells, sizeof(float)) : NULL;
+            j->dirty = mode != 0 ? (float *)calloc(cells, sizeof(float)) : NULL;
+        }
+        pthread_create(&tids[t], NULL, co_w_worker, j);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
+    for (int t = 0; t < threads; ++t) {
+        co_w_job *j = &jobs[t];
+        sums->normalization_sumwt += j->sums.normalization_sumwt;
+        sums->reported_sumwt += j->sums.reported_sumwt;
+        sums->gridded += j->sums.gridded;
+        sums->skipped += j->sums.skipped;
+        if (threads > 1) {
+            if (j->psf) { co_add_f32(psf, j->psf, cells); free(j->psf); }
+            if (j->dirty) { co_add_f32(dirty, j->dirty, cells); free(j->dirty); }
+        }
+    }
+    free(jobs);
+    free(tids);
+}
+
+/* WProjector::degrid_sample_planned, gridder.rs:3726-3747.  NaN when rejected. */
+void co_w_degrid(const co_wproj *p, uint64_t n, const double *u, const double *v, const double *w,
+                 const float *model, float *out) {
+    const uint64_t W = p->width, stride = p->gy;
+    const int64_t s = (int64_t)p->sampling;
+    for (uint64_t i = 0; i < n; ++i) {
+        int64_t plan[6];
+        float nrm;
+        if (!co_w_plan(p, u[i], v[i], w[i], plan, &nrm)) { out[2 * i] = out[2 * i + 1] = NAN; continue; }
+        const int64_t S = p->support[plan[4]];
+        const float *K = p->kernels + 2 * (uint64_t)plan[4] * W * W;
+        float pr = 0.0f, pi = 0.0f;
+        for (int64_t iy = -S; iy <= S; ++iy) {
+            int64_t ky = iy * s + plan[3]; if (ky < 0) ky = -ky;
+            for (int64_t ix = -S; ix <= S; ++ix) {
+                int64_t kx = ix * s + plan[2]; if (kx < 0) kx = -kx;
+                float cr = K[2 * (kx * W + ky)], ci = K[2 * (kx * W + ky) + 1];
+                if (plan[5]) ci = -ci;
+                ci = -ci;
+                uint64_t idx = (uint64_t)(plan[0] + ix) * stride + (uint64_t)(plan[1] + iy);
+                float gr = model[2 * idx], gi = model[2 * idx + 1];
+                float tr = cr * gr - ci * gi;
+                float ti = cr * gi + ci * gr;
+                pr += tr;
+                pi += ti;
+            }
+        }
+        out[2 * i] = pr;
+        out[2 * i + 1] = pi;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Direct DFT (FP64 phase), lib.rs:73709-73910                                */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+    uint64_t nx, ny;
+    double cell_x, cell_y;
+    uint64_t begin, end; /* pixel range */
+    uint64_t n;
+    const double *u, *v, *w;
+    const float *value; /* complex per sample */
+    float *image;
+} co_dft_job;
+
+/* build_direct_pixel_coordinates, lib.rs:73823-73845 */
+static void co_pixel_lmn(uint64_t nx, uint64_t ny, double cx, double cy, uint64_t x, uint64_t y,
+                         double *l, double *m, double *nm1) {
+    double center_x = (double)nx / 2.0, center_y = (double)ny / 2.0;
+    *l = ((double)x - center_x) * cx;
+    *m = (center_y - (double)y) * cy;
+    double r2 = (*l) * (*l) + (*m) * (*m);
+    *nm1 = r2 < 1.0 ? sqrt(1.0 - r2) - 1.0 : -1.0;
+}
+
+/* accumulate_direct_adjoint, lib.rs:73890-73910; pixels are split over
+ * threads (each pixel still sums samples in order). */
+static void *co_dft_adjoint_worker(void *arg) {
+    co_dft_job *j = (co_dft_job *)arg;
+    const double TAU = 6.28318530717958647692;
+    for (uint64_t p = j->begin; p < j->end; ++p) {
+        uint64_t x = p / j->ny, y = p % j->ny;
+        double l, m, nm1;
+        co_pixel_lmn(j->nx, j->ny, j->cell_x, j->cell_y, x, y, &l, &m, &nm1);
+        float acc = j->image[p];
+        for (uint64_t i = 0; i < j->n; ++i) {
+            double phase = TAU * (j->u[i] * l + j->v[i] * m + j->w[i] * nm1);
+            float c = (float)cos(phase), s = (float)sin(phase);
+            float contribution = 2.0f * (j->value[2 * i] * c - j->value[2 * i + 1] * s);
+            acc += contribution;
+        }
+        j->image[p] = acc;
+    }
+    return NULL;
+}
+
+void co_direct_adjoint(uint64_t nx, uint64_t ny, double cell_x, double cell_y, uint64_t n,
+                       const double *u, const double *v, const double *w, const float *value,
+                       int threads, float *image) {
+    if (threads < 1) threads = 1;
+    uint64_t pixels = nx * ny;
+    uint64_t chunk = (pixels + threads - 1) / threads;
+    co_dft_job *jobs = (co_dft_job *)calloc(threads, sizeof(co_dft_job));
+    pthread_t *tids = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    for (int t = 0; t < threads; ++t) {
+        co_dft_job *j = &jobs[t];
+        j->nx = nx; j->ny = ny; j->cell_x = cell_x; j->cell_y = cell_y;
+        j->begin = (uint64_t)t * chunk;
+        j->end = j->begin + chunk < pixels ? j->begin + chunk : pixels;
+        if (j->begin > pixels) j->begin = pixels;
+        j->n = n; j->u = u; j->v = v; j->w = w; j->value = value; j->image = image;
+        pthread_create(&tids[t], NULL, co_dft_adjoint_worker, j);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
+    free(jobs);
+    free(tids);
+}
+
+/* direct_predict_visibility, lib.rs:73870-73887: components are the non-zero
+ * model pixels in (x, y) order. */
+void co_direct_predict(uint64_t nx, uint64_t ny, double cell_x, double cell_y, const float *model,
+                       uint64_t n, const double *u, const double *v, const double *w, float *out) {
+    const double TAU = 6.28318530717958647692;
+    for (uint64_t i = 0; i < n; ++i) {
+        float re = 0.0f, im = 0.0f;
+        for (uint64_t x = 0; x < nx; ++x) {
+            for (uint64_t y = 0; y < ny; ++y) {
+                float value = model[x * ny + y];
+                if (fabsf(value) <= 0.0f) continue;
+                double l, m, nm1;
+                co_pixel_lmn(nx, ny, cell_x, cell_y, x, y, &l, &m, &nm1);
+                double phase = TAU * (u[i] * l + v[i] * m + w[i] * nm1);
+                re += value * (float)cos(phase);
+                im -= value * (float)sin(phase);
+            }
+        }
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+}

@@ -1,0 +1,48 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the header
+declares, and fails loudly (no CPU fallback) when there is no device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "casa_imaging_hip.h")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(chip_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_and_binding_agree(pkg):
+    assert sorted(pkg.abi.EXPORTS) == header_functions()
+
+
+def test_library_exports_every_header_symbol(pkg):
+    L = pkg.abi.load()
+    for name in header_functions():
+        assert hasattr(L, name), name
+    assert L.chip_version().decode().startswith("casa-imaging-hip")
+
+
+def test_error_path_without_device(pkg):
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    with pytest.raises(pkg.Unsupported):
+        pkg.Context(0)
+
+
+def test_invalid_geometry_is_rejected_before_device_work(pkg):
+    """A NULL context is an InvalidRequest, not a crash."""
+    L = pkg.abi.load()
+    geo = pkg.abi.Geometry(4, 4, 1e-5, 1e-5, 0, 100)
+    out = ctypes.c_void_p()
+    err = ctypes.c_void_p()
+    rc = L.chip_gridder_create(None, ctypes.byref(geo), ctypes.byref(out), ctypes.byref(err))
+    assert rc == pkg.abi.CHIP_ERR_INVALID
+    assert b"NULL" in ctypes.cast(err, ctypes.c_char_p).value
+    L.chip_error_free(err)
