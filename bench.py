@@ -1,0 +1,375 @@
+#!/usr/bin/env python3
+"""Throughput of the W-projection grid+degrid major cycle on one MI355X.
+
+Workload (default ``cube256``, the batched spectral cube): 27 antennas in a Y
+with 21 km arms (seeded jitter), latitude 34 deg, phase centre dec +30 deg,
+1 h at 2 s (1800 x 351 = 631,800 rows) x 256 channels 1.0-2.0 GHz, per-channel
+flag fraction U(0, 0.6), Briggs robust 0 weights from a density grid, 200
+Gaussian sources (FWHM 1-10 px, flux log-uniform 10 mJy-1 Jy), 1024^2 image
+per channel (1250^2 grid), 32 W planes (explicit, W sampling 4), cell =
+1/(3 max|uv|).
+
+One step = one residual refresh of every channel: model cube -> apodise +
+forward FFT -> degrid and grid (V - model)*w -> inverse FFT, crop, correct,
+normalise -> residual cube.  The batched FP32 path runs every channel in one
+binned launch over a resident, pre-sorted visibility stream; the baseline is
+the plain per-channel FP64 path (one launch per channel, FP64 MACs and
+atomics, FP64 FFTs) on the same GPU.  ``value`` is the batched FP32 Mvis/s;
+the FP64 rate, the ratio and the dirty/residual image errors between the two
+(relative to the FP64 image peak) are reported beside it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+C = 299_792_458.0
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+WORKLOADS = {
+    "cube256": dict(hours=1.0, dt=2.0, n_chan=256, f0=1.0e9, f1=2.0e9, nx=1024, planes=32,
+                    sampling=None, n_src=200, flag_max=0.6, arm_m=21_000.0),
+    # reduced cube for quick checks (same geometry, fewer rows and channels)
+    "cube16": dict(hours=0.25, dt=2.0, n_chan=16, f0=1.0e9, f1=2.0e9, nx=1024, planes=32,
+                   sampling=None, n_src=200, flag_max=0.6, arm_m=21_000.0),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def y_array(rng, n_ant=27, arm_m=21_000.0, jitter_m=50.0):
+    per_arm = n_ant // 3
+    frac = (np.arange(1, per_arm + 1) / per_arm) ** 1.716
+    pos = []
+    for arm in range(3):
+        az = np.deg2rad(5.0 + 120.0 * arm)
+        for f in frac:
+            r = arm_m * f
+            pos.append([r * np.sin(az) + rng.normal(0, jitter_m), r * np.cos(az) + rng.normal(0, jitter_m),
+                        rng.normal(0, 2.0)])
+    return np.asarray(pos)
+
+
+def uvw_tracks(enu, lat_deg, dec_deg, hours, dt_s):
+    lat, dec = np.deg2rad(lat_deg), np.deg2rad(dec_deg)
+    e, nn, up = enu[:, 0], enu[:, 1], enu[:, 2]
+    ant = np.stack([-np.sin(lat) * nn + np.cos(lat) * up, e, np.cos(lat) * nn + np.sin(lat) * up], 1)
+    i, j = np.triu_indices(len(enu), 1)
+    b = ant[j] - ant[i]
+    n_t = int(round(hours * 3600.0 / dt_s))
+    ha = np.deg2rad(15.0 * (np.arange(n_t) * dt_s / 3600.0 - hours / 2.0))
+    sh, ch = np.sin(ha)[:, None], np.cos(ha)[:, None]
+    sd, cd = np.sin(dec), np.cos(dec)
+    bx, by, bz = b[None, :, 0], b[None, :, 1], b[None, :, 2]
+    u = sh * bx + ch * by
+    v = -sd * ch * bx + sd * sh * by + cd * bz
+    w = cd * ch * bx - cd * sh * by + sd * bz
+    return np.stack([u, v, w], -1).reshape(-1, 3)
+
+
+def make_cube(torch, wl, seed, dev):
+    """Synthetic cube on the GPU (data generation is not timed)."""
+    rng = np.random.default_rng(seed)
+    enu = y_array(rng, arm_m=wl["arm_m"])
+    uvw = uvw_tracks(enu, 34.0, 30.0, wl["hours"], wl["dt"])
+    n_rows, n_chan = uvw.shape[0], wl["n_chan"]
+    freq = np.linspace(wl["f0"], wl["f1"], n_chan)
+    scale = freq / C
+    max_uv = float(np.max(np.hypot(uvw[:, 0], uvw[:, 1])) * scale[-1])
+    cell = 1.0 / (3.0 * max_uv)
+    nx = wl["nx"]
+    # sources: Gaussians inside the inner 80% of the field
+    ns = wl["n_src"]
+    flux = np.exp(rng.uniform(np.log(0.01), np.log(1.0), ns))
+    lpix = rng.uniform(-0.4 * nx, 0.4 * nx, ns)
+    mpix = rng.uniform(-0.4 * nx, 0.4 * nx, ns)
+    fwhm = rng.uniform(1.0, 10.0, ns)
+    sig = fwhm / (2.0 * np.sqrt(2.0 * np.log(2.0))) * cell
+    l, m = lpix * cell, mpix * cell
+    n1 = np.sqrt(1.0 - l * l - m * m) - 1.0
+    U = torch.tensor(uvw, dtype=torch.float64, device=dev)
+    vis = torch.empty((n_rows, n_chan), dtype=torch.complex64, device=dev)
+    L_ = torch.tensor(l, dtype=torch.float64, device=dev)
+    M_ = torch.tensor(m, dtype=torch.float64, device=dev)
+    N_ = torch.tensor(n1, dtype=torch.float64, device=dev)
+    S_ = torch.tensor(flux, dtype=torch.float64, device=dev)
+    G_ = torch.tensor(2.0 * np.pi ** 2 * sig ** 2, dtype=torch.float64, device=dev)
+    for c in range(n_chan):
+        u = U[:, 0:1] * scale[c]
+        v = U[:, 1:2] * scale[c]
+        w = U[:, 2:3] * scale[c]
+        ph = -2.0 * np.pi * (u * L_ + v * M_ + w * N_)
+        amp = S_ * torch.exp(-G_ * (u * u + v * v))
+        vis[:, c] = torch.complex((amp * torch.cos(ph)).sum(1), (amp * torch.sin(ph)).sum(1)).to(torch.complex64)
+    frac = rng.uniform(0.0, wl["flag_max"], n_chan)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    flag = torch.rand((n_rows, n_chan), generator=gen, device=dev) < torch.tensor(frac, device=dev)[None, :]
+    weight = briggs_weights(torch, U, scale, flag, nx, cell, robust=0.0)
+    model = torch.zeros((n_chan, nx, nx), dtype=torch.float32, device=dev)
+    # CLEAN-like point components at the source positions (x = l/cell + nx/2, y = ny/2 - m/cell)
+    xs = np.clip(np.round(lpix + nx / 2).astype(int), 0, nx - 1)
+    ys = np.clip(np.round(nx / 2 - mpix).astype(int), 0, nx - 1)
+    model[:, xs, ys] = torch.tensor(0.5 * flux, dtype=torch.float32, device=dev)[None, :]
+    return dict(uvw=U, scale=torch.tensor(scale, dtype=torch.float64, device=dev), vis=vis, flag=flag,
+                weight=weight, model=model, cell=cell, nx=nx, n_rows=n_rows, n_chan=n_chan,
+                freq=freq, max_uv=max_uv, max_w=float(np.max(np.abs(uvw[:, 2])) * scale[-1]))
+
+
+def briggs_weights(torch, U, scale, flag, nx, cell, robust):
+    """Briggs weights per channel from a density grid of natural weights on the
+    image-sized uv grid (cube convention: nearest cell, Hermitian mirror),
+    f2 = (5 10^-R)^2 / (sum d^2 / sum d), w = 1 / (d f2 + 1)
+    (weighting.rs:2440-2445, :2758-2777).  Data preparation, not timed."""
+    n_rows, n_chan = flag.shape
+    out = torch.empty((n_rows, n_chan), dtype=torch.float32, device=U.device)
+    uscale = nx * cell
+    for c in range(n_chan):
+        u = U[:, 0] * scale[c] * uscale
+        v = U[:, 1] * scale[c] * uscale
+        ok = ~flag[:, c]
+        dens = torch.zeros(nx * nx, dtype=torch.float32, device=U.device)
+        for sgn in (1.0, -1.0):
+            x = torch.round(sgn * u + nx / 2).long()
+            y = torch.round(-sgn * v + nx / 2).long()
+            inb = ok & (x > 0) & (y > 0) & (x < nx) & (y < nx)
+            dens.index_add_(0, (x * nx + y)[inb], torch.ones(int(inb.sum()), dtype=torch.float32, device=U.device))
+        sw = dens.double().sum()
+        s2 = (dens.double() ** 2).sum()
+        f2 = float((5.0 * 10.0 ** (-robust)) ** 2 / (s2 / sw)) if s2 > 0 else 0.0
+        x = torch.round(u + nx / 2).long().clamp(0, nx - 1)
+        y = torch.round(-v + nx / 2).long().clamp(0, nx - 1)
+        d = dens[x * nx + y]
+        out[:, c] = 1.0 / (d * np.float32(f2) + 1.0)
+    return out
+
+
+def cpu_baseline(pkg, g, data, cpu_seconds, log):
+    """Oracle residual refresh (degrid + grid, FP32 like the reference W path)
+    on host cores for a bounded sample of channels."""
+    import torch
+    import oracle
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    nx, cell = data["nx"], data["cell"]
+    og = oracle.StandardGridder(nx, nx, cell, cell)
+    sup, ker, nt = g.wproject_tables()
+    wp = oracle.WProjector.from_tables(og, g.info.planes, g.info.sampling, g.info.w_scale, ker, sup, nt)
+    uvw = data["uvw"].cpu().numpy()
+    scale = data["scale"].cpu().numpy()
+    order = np.random.default_rng(1).permutation(data["n_chan"])
+    nvis, spent, chans = 0, 0.0, 0
+    for c in order:
+        u, v, w = (uvw[:, k] * scale[c] for k in range(3))
+        vis = data["vis"][:, c].cpu().numpy()
+        wt = data["weight"][:, c].cpu().numpy()
+        gr = ~data["flag"][:, c].cpu().numpy()
+        model = g.model_to_grid(data["model"][c])[0].cpu().numpy()
+        t0 = time.perf_counter()
+        _, _, s = wp.grid(u, v, w, wt, np.ones_like(wt), vis, gridable=gr, mode=2, model=model,
+                          threads=threads)
+        spent += time.perf_counter() - t0
+        nvis += int(gr.sum())
+        chans += 1
+        if spent >= cpu_seconds:
+            break
+    log(f"cpu baseline: {chans} channels, {nvis} vis in {spent:.2f} s on {threads} threads")
+    return {"value": nvis / spent / 1e6, "unit": "Mvis/s", "cores": threads, "kind": "port",
+            "sample": f"oracle W-projection residual refresh (degrid+grid, FP32 grids as the "
+                      f"reference), {chans} of {data['n_chan']} channels, {nvis} unflagged visibilities, "
+                      f"FFTs excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="cube256", choices=sorted(WORKLOADS))
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--seed", type=int, default=2026)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every GPU box here has one, and the sharded path "
+              f"is described, not built (asked for --gpus {args.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__
+
+    pkg = __graft_entry__.load_package()
+    dev = torch.device("cuda", 0)
+    wl = WORKLOADS[args.workload]
+    t0 = time.perf_counter()
+    data = make_cube(torch, wl, args.seed, dev)
+    torch.cuda.synchronize()
+    log(f"data: {data['n_rows']} rows x {data['n_chan']} chan, cell {np.rad2deg(data['cell']) * 3600:.4f} arcsec, "
+        f"max|uv| {data['max_uv']:.0f}, max|w| {data['max_w']:.0f} ({time.perf_counter() - t0:.1f} s)")
+
+    nx, cell, n_chan = data["nx"], data["cell"], data["n_chan"]
+    g = pkg.StandardGridder(nx, nx, cell, cell)
+    t0 = time.perf_counter()
+    g.attach_wproject(data["max_w"], planes=wl["planes"], sampling=wl["sampling"])
+    kernel_build_s = time.perf_counter() - t0
+    sup, _, _ = g.wproject_tables()
+    log(f"W kernels: {g.info.planes} planes, sampling {g.info.sampling}, width {g.info.width}, "
+        f"supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
+    batch = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
+    n_unflagged = int((~data["flag"]).sum())
+    t0 = time.perf_counter()
+    vs = g.visibility_set(batch)
+    torch.cuda.synchronize()
+    plan_s = time.perf_counter() - t0
+    st = vs.stats()
+    log(f"plan+bin: {st['n_valid']} accepted of {n_unflagged} unflagged, {st['n_items']} work items, "
+        f"{st['taps']} taps ({plan_s * 1e3:.1f} ms)")
+    nsum = np.array([s["normalization_sumwt"] for s in vs.sums])
+
+    # initial dirty images: batched FP32 vs plain FP64
+    psf32, dirty32 = g.new_grids(n_chan), g.new_grids(n_chan)
+    vs.grid(pkg.MODE_DIRTY, psf=psf32, dirty=dirty32)
+    p32, d32, peak32 = g.finish_products(psf32, dirty32, nsum)
+    del psf32, dirty32
+    result_extra = {}
+    if not args.no_fp64:
+        psf64, dirty64 = g.new_grids(n_chan, pkg.F64), g.new_grids(n_chan, pkg.F64)
+        sums64 = g.grid_plain(batch, pkg.MODE_DIRTY, pkg.F64, psf=psf64, dirty=dirty64)
+        nsum64 = np.array([s["normalization_sumwt"] for s in sums64])
+        p64, d64, peak64 = g.finish_products(psf64, dirty64, nsum64, precision=pkg.F64)
+        del psf64, dirty64
+        ref_peak = torch.amax(torch.abs(d64), dim=(1, 2))
+        dirty_err = float(torch.max(torch.amax(torch.abs(d32 - d64), dim=(1, 2)) / ref_peak))
+        psf_err = float(torch.max(torch.abs(p32 - p64)))
+        result_extra.update(dirty_err_vs_fp64=dirty_err, psf_err_vs_fp64=psf_err)
+        log(f"dirty image error FP32 batched vs FP64 plain: {dirty_err:.2e} of peak; PSF {psf_err:.2e}")
+    torch.cuda.empty_cache()
+
+    model = data["model"]
+    mgrid = g.new_grids(n_chan)
+    res = g.new_grids(n_chan)
+
+    kernel_ms = []
+
+    def step32():
+        g.model_to_grid(model, pkg.F32, out=mgrid)
+        res.zero_()
+        vs.grid(pkg.MODE_RESIDUAL, model=mgrid, dirty=res)
+        kernel_ms.append(vs.stats()["last_kernel_ms"])
+        return g.finish_products(None, res, nsum, psf_peak=peak32)[1]
+
+    for _ in range(args.warmup):
+        step32()
+    kernel_ms.clear()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        r32 = step32()
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / args.steps
+    rate32 = n_unflagged / t32 / 1e6
+    log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
+        f"binned kernel {np.mean(kernel_ms):.3f} ms")
+
+    if not args.no_fp64:
+        del mgrid, res
+        torch.cuda.empty_cache()
+        mgrid64 = g.new_grids(n_chan, pkg.F64)
+        res64 = g.new_grids(n_chan, pkg.F64)
+
+        def step64():
+            g.model_to_grid(model, pkg.F64, out=mgrid64)
+            res64.zero_()
+            g.grid_plain(batch, pkg.MODE_RESIDUAL, pkg.F64, model=mgrid64, dirty=res64)
+            return g.finish_products(None, res64, nsum64, psf_peak=peak64, precision=pkg.F64)[1]
+
+        step64()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            r64 = step64()
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - t0) / args.fp64_steps
+        rate64 = n_unflagged / t64 / 1e6
+        rpk = torch.amax(torch.abs(r64), dim=(1, 2))
+        res_err = float(torch.max(torch.amax(torch.abs(r32 - r64), dim=(1, 2)) / torch.clamp(rpk, min=1e-30)))
+        result_extra.update(fp64_plain_mvis_s=rate64, fp64_plain_ms_per_step=t64 * 1e3,
+                            speedup_vs_fp64_plain=rate32 / rate64, residual_err_vs_fp64=res_err)
+        log(f"plain FP64 per-channel major cycle: {t64 * 1e3:.2f} ms/step, {rate64:.1f} Mvis/s "
+            f"(batched FP32 is {rate32 / rate64:.2f}x); residual error {res_err:.2e} of peak")
+        del mgrid64, res64
+        torch.cuda.empty_cache()
+
+    # roofline of the dominant kernel (binned RESIDUAL: degrid + grid)
+    kms = float(np.mean(kernel_ms))
+    n_valid = st["n_valid"]
+    cells = n_chan * g.info.gx * g.info.gy
+    bytes_alg = n_unflagged * (8 + 4 + 1) + data["n_rows"] * 24 + cells * 8 * 2
+    flops_alg = st["taps"] * 16.0
+    gbs = bytes_alg / (kms * 1e-3) / 1e9
+    tfl = flops_alg / (kms * 1e-3) / 1e12
+    hbm_time = bytes_alg / (HBM_PEAK_GBS * 1e9)
+    flop_time = flops_alg / (FP32_PEAK_TFLOPS * 1e12)
+    if hbm_time >= flop_time:
+        roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+    else:
+        roof = {"bound": "valu_fp32", "achieved": tfl, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tfl / FP32_PEAK_TFLOPS}
+    roof["traffic"] = None
+    roof.update(kernel="k_binned<RESIDUAL, W>", kernel_ms=kms, algorithmic_bytes=bytes_alg,
+                algorithmic_flops=flops_alg, fp32_tflops=tfl, hbm_gbs=gbs)
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(pkg, g, data, args.cpu_seconds, log)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e!r}")
+
+    out = {
+        "metric": "Mvis/s grid+degrid, batched FP32 vs plain FP64, same GPU, err <=1e-5 peak",
+        "value": rate32,
+        "unit": "Mvis/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: W-projection residual refresh (degrid+grid+FFTs), "
+                        f"{data['n_rows']} rows x {n_chan} channels, {n_unflagged} unflagged vis, "
+                        f"{nx}^2 image / {g.info.gx}^2 grid, {g.info.planes} W planes, sampling {g.info.sampling}",
+            "n_rows": data["n_rows"], "n_chan": n_chan, "n_vis_unflagged": n_unflagged,
+            "n_vis_gridded": n_valid, "grid": g.info.gx, "image": nx, "w_planes": g.info.planes,
+            "w_sampling": g.info.sampling, "w_support_range": [int(sup.min()), int(sup.max())],
+            "grid_point_updates_per_pass": st["taps"], "plan_bin_ms": plan_s * 1e3,
+            "w_kernel_build_ms": kernel_build_s * 1e3, "parallelism": "single GPU",
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    out.update(result_extra)
+    out["grid_point_updates_per_s"] = st["taps"] / t32
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

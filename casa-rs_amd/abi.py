@@ -34,7 +34,7 @@ EXPORTS = (
     "chip_ctx_create", "chip_ctx_destroy", "chip_error_free", "chip_version",
     "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
     "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
-    "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_grid",
+    "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_grid_plain", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
 )
@@ -149,6 +149,7 @@ def load():
         "chip_vset_create": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _PP, _PP]),
         "chip_vset_destroy": (None, [_P]),
         "chip_vset_sums": (ctypes.c_int, [_P, _P, _PP]),
+        "chip_vset_stats": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
         "chip_vset_grid": (ctypes.c_int, [_P, _U32, _P, _P, _P, _PP]),
         "chip_vset_degrid": (ctypes.c_int, [_P, _P, _P, _PP]),
         "chip_vset_grid_values": (ctypes.c_int, [_P, _P, _P, _PP]),

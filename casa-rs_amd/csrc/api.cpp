@@ -224,6 +224,26 @@ int chip_vset_sums(const chip_vset *vs, chip_sums *sums, char **err) {
     });
 }
 
+int chip_vset_stats(chip_vset *vs, uint64_t *n_valid, uint64_t *n_items, uint64_t *taps,
+                    double *last_kernel_ms, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        if (n_valid) *n_valid = vs->n_valid;
+        if (n_items) *n_items = vs->n_items;
+        if (taps) *taps = vs->taps;
+        if (last_kernel_ms) {
+            *last_kernel_ms = -1.0;
+            if (vs->timed) {
+                set_device(vs->g->ctx);
+                float ms = 0.0f;
+                CHIP_HIP(hipEventSynchronize(vs->ev_stop));
+                CHIP_HIP(hipEventElapsedTime(&ms, vs->ev_start, vs->ev_stop));
+                *last_kernel_ms = ms;
+            }
+        }
+    });
+}
+
 int chip_vset_grid(chip_vset *vs, uint32_t mode, const float *model_dev, float *psf_dev,
                    float *dirty_dev, char **err) {
     return guarded(err, [&] {

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *i
 }
 
 template <int MODE, bool WP>
-static void launch_binned_t(const chip_vset &vs, const GridParams &gp, const float2 *values,
+static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
                             const float2 *model, float *psf, float *acc, float2 *vis_out,
                             hipStream_t st) {
     const int tap_floats = WP ? 0 : ((static_cast<int>(gp.tap_rows) * kTaps + 3) & ~3);
@@ -265,15 +265,20 @@ static void launch_binned_t(const chip_vset &vs, const GridParams &gp, const flo
         fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
                                    std::to_string(gp.halo) + ")");
     if (vs.n_items == 0) return;
+    // The kernel is bracketed by events on its own stream so callers can read
+    // its device duration (chip_vset_last_kernel_ms).
+    CHIP_HIP(hipEventRecord(vs.ev_start, st));
     hipLaunchKernelGGL((k_binned<MODE, WP>), dim3(static_cast<unsigned>(vs.n_items)), dim3(256),
                        lds, st, gp, vs.items.as<WorkItem>(), vs.plan.as<uint2>(),
                        vs.vis.as<float2>(), vs.wt.as<float>(), vs.index.as<uint32_t>(), values,
                        model, psf, acc, vis_out);
     CHIP_HIP(hipGetLastError());
+    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
+    vs.timed = true;
 }
 
 template <int MODE>
-static void launch_binned(const chip_vset &vs, const float2 *values, const float2 *model,
+static void launch_binned(chip_vset &vs, const float2 *values, const float2 *model,
                           float *psf, float *acc, float2 *vis_out) {
     const chip_gridder &g = *vs.g;
     const GridParams gp = grid_params(g, vs.tile, vs.halo, vs.tiles_y);

@@ -195,4 +195,12 @@ struct chip_vset {
     chip::DevBuf index;   // uint32 original sample index
     chip::DevBuf items;   // WorkItem
     std::vector<chip_sums> sums;
+    uint64_t taps = 0;    // sum over accepted samples of (2S+1)^2
+    // events around the last binned kernel launch
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    bool timed = false;
+    ~chip_vset() {
+        if (ev_start) (void)hipEventDestroy(ev_start);
+        if (ev_stop) (void)hipEventDestroy(ev_stop);
+    }
 };

@@ -155,23 +155,21 @@ __global__ __launch_bounds__(256) void k_keys(PlanParams p, Source s, BinParams 
     vals[i] = static_cast<uint32_t>(i);
 }
 
-// Writes the packed stream in sorted order and accumulates per-channel sums.
-// Sorted samples of one block span few channels, so sums are first reduced in
-// LDS slots relative to the block's first channel.
+// Writes the packed stream in sorted order and accumulates per-channel sums
+// (normalisation sumwt, reported sumwt, accepted count, tap count).  Sorted
+// samples of one block span few channels, so sums are first reduced in LDS
+// slots relative to the block's first channel.
 constexpr int kSumSlots = 8;
+constexpr int kSumFields = 4;
 
 __global__ __launch_bounds__(256) void k_build_stream(PlanParams p, Source s, uint64_t n_valid,
                                                       const uint32_t *sorted_idx, uint2 *plan,
                                                       float2 *vis, float *wt, uint32_t *index,
-                                                      double *sums /* n_chan x 3 */) {
-    __shared__ double slot_nsum[kSumSlots], slot_rsum[kSumSlots], slot_cnt[kSumSlots];
+                                                      double *sums /* n_chan x 4 */) {
+    __shared__ double slot[kSumSlots][kSumFields];
     __shared__ uint32_t first_chan;
     const uint64_t pidx = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (threadIdx.x < kSumSlots) {
-        slot_nsum[threadIdx.x] = 0.0;
-        slot_rsum[threadIdx.x] = 0.0;
-        slot_cnt[threadIdx.x] = 0.0;
-    }
+    if (threadIdx.x < kSumSlots * kSumFields) slot[threadIdx.x / kSumFields][threadIdx.x % kSumFields] = 0.0;
     if (threadIdx.x == 0) {
         const uint64_t p0 = static_cast<uint64_t>(blockIdx.x) * blockDim.x;
         const uint64_t i0 = sorted_idx[p0];
@@ -188,33 +186,30 @@ __global__ __launch_bounds__(256) void k_build_stream(PlanParams p, Source s, ui
         vis[pidx] = smp.vis;
         wt[pidx] = gw;
         index[pidx] = i;
-        double nsum, rsum;
+        double f[kSumFields];
         if (p.wproject) {
             // w_project_accumulation_for_sample (lib.rs:71460-71470)
-            nsum = 2.0 * static_cast<double>(smp.weight) * static_cast<double>(pl.norm);
-            rsum = static_cast<double>(smp.weight) * static_cast<double>(smp.sf);
+            f[0] = 2.0 * static_cast<double>(smp.weight) * static_cast<double>(pl.norm);
+            f[1] = static_cast<double>(smp.weight) * static_cast<double>(smp.sf);
+            const double side = 2.0 * p.supports[pl.plane] + 1.0;
+            f[3] = side * side;
         } else {
             // execution.rs:18306-18309
-            nsum = static_cast<double>(gw);
-            rsum = nsum;
+            f[0] = static_cast<double>(gw);
+            f[1] = f[0];
+            f[3] = static_cast<double>(kTaps * kTaps);
         }
+        f[2] = 1.0;
         const uint32_t rel = smp.chan - first_chan;
-        if (rel < kSumSlots) {
-            atomicAdd(&slot_nsum[rel], nsum);
-            atomicAdd(&slot_rsum[rel], rsum);
-            atomicAdd(&slot_cnt[rel], 1.0);
-        } else {
-            atomicAdd(&sums[3 * smp.chan], nsum);
-            atomicAdd(&sums[3 * smp.chan + 1], rsum);
-            atomicAdd(&sums[3 * smp.chan + 2], 1.0);
+        for (int k = 0; k < kSumFields; ++k) {
+            if (rel < kSumSlots) atomicAdd(&slot[rel][k], f[k]);
+            else atomicAdd(&sums[kSumFields * smp.chan + k], f[k]);
         }
     }
     __syncthreads();
-    if (threadIdx.x < kSumSlots && slot_cnt[threadIdx.x] > 0.0) {
-        const uint32_t c = first_chan + threadIdx.x;
-        atomicAdd(&sums[3 * c], slot_nsum[threadIdx.x]);
-        atomicAdd(&sums[3 * c + 1], slot_rsum[threadIdx.x]);
-        atomicAdd(&sums[3 * c + 2], slot_cnt[threadIdx.x]);
+    if (threadIdx.x < kSumSlots * kSumFields) {
+        const int r = threadIdx.x / kSumFields, k = threadIdx.x % kSumFields;
+        if (slot[r][2] > 0.0) atomicAdd(&sums[kSumFields * (first_chan + r) + k], slot[r][k]);
     }
 }
 
@@ -329,7 +324,7 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     vs.vis.alloc(nv * sizeof(float2));
     vs.wt.alloc(nv * sizeof(float));
     vs.index.alloc(nv * sizeof(uint32_t));
-    DevBuf sums(src.n_chan * 3 * sizeof(double));
+    DevBuf sums(src.n_chan * kSumFields * sizeof(double));
     CHIP_HIP(hipMemsetAsync(sums.p, 0, sums.bytes, st));
     if (n_valid > 0) {
         hipLaunchKernelGGL(k_build_stream, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st, pp,
@@ -338,15 +333,20 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
                            vs.index.as<uint32_t>(), sums.as<double>());
         CHIP_HIP(hipGetLastError());
     }
-    std::vector<double> hs(src.n_chan * 3);
+    std::vector<double> hs(src.n_chan * kSumFields);
     CHIP_HIP(hipMemcpyAsync(hs.data(), sums.p, sums.bytes, hipMemcpyDeviceToHost, st));
     CHIP_HIP(hipStreamSynchronize(st));
+    vs.taps = 0;
     for (uint64_t c = 0; c < src.n_chan; ++c) {
-        vs.sums[c].normalization_sumwt = hs[3 * c];
-        vs.sums[c].reported_sumwt = hs[3 * c + 1];
-        vs.sums[c].gridded = static_cast<uint64_t>(hs[3 * c + 2]);
+        const double *h = hs.data() + kSumFields * c;
+        vs.sums[c].normalization_sumwt = h[0];
+        vs.sums[c].reported_sumwt = h[1];
+        vs.sums[c].gridded = static_cast<uint64_t>(h[2]);
         vs.sums[c].skipped = src.n_rows - vs.sums[c].gridded;
+        vs.taps += static_cast<uint64_t>(h[3]);
     }
+    CHIP_HIP(hipEventCreate(&vs.ev_start));
+    CHIP_HIP(hipEventCreate(&vs.ev_stop));
 }
 
 }  // namespace chip

@@ -310,6 +310,15 @@ class VisibilitySet:
         abi.call("chip_vset_sums", h, sums)
         self.sums = [x.as_dict() for x in sums]
 
+    def stats(self):
+        """n_valid, n_items, taps (sum of (2S+1)^2) and the last kernel's device ms."""
+        nv, ni, tp = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        ms = ctypes.c_double()
+        abi.call("chip_vset_stats", self._h, ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(tp),
+                 ctypes.byref(ms))
+        return {"n_valid": nv.value, "n_items": ni.value, "taps": tp.value,
+                "last_kernel_ms": ms.value}
+
     def grid(self, mode, model=None, psf=None, dirty=None):
         abi.call("chip_vset_grid", self._h, mode, _ptr(model), _ptr(psf), _ptr(dirty))
 

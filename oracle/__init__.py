@@ -543,6 +543,16 @@ class WProjector:
                          self.width, self._sup32.ctypes.data, self._k32.ctypes.data,
                          self.norm_radius, self._n32.ctypes.data)
 
+    @classmethod
+    def from_tables(cls, gridder, planes, sampling, w_scale, kernels, supports, norm_table):
+        """A projector over given tables (no kernel construction)."""
+        self = cls.__new__(cls)
+        self.gridder, self.max_abs_w, self.explicit_planes = gridder, float("nan"), planes
+        self.planes, self.sampling, self.w_scale = int(planes), int(sampling), float(w_scale)
+        self.norm_radius = (int(sampling) + 1) // 2
+        self.support_margin = []
+        return self.with_kernels(kernels, supports, norm_table)
+
     def with_kernels(self, kernels, supports, norm_table):
         """Return a copy that uses externally built kernels (e.g. the GPU's), so
         that gridding can be compared independently of FFT rounding."""

@@ -79,7 +79,20 @@ def test_standard_plan_bit_exact(gpu, oracle_mod, nx, arcsec, os_, pad):
     assert 0 < np.sum(ref[:, 0] >= 0) < n  # both accepted and rejected samples present
 
 
-def _std_setup(gpu, oracle_mod, seed=5, n=30000, nx=100, arcsec=7.0, os_=128):
+def with_sources(seed, u, v, w, vis, nx, cell, n_src=8, noise=0.05):
+    """Replace the finite visibilities by point sources plus noise, so that an
+    image error 'of peak' is measured against a real dirty-image peak."""
+    rng = np.random.default_rng(seed)
+    src = [(rng.uniform(0.2, 1.0), rng.integers(-nx // 3, nx // 3) * cell,
+            rng.integers(-nx // 3, nx // 3) * cell) for _ in range(n_src)]
+    ok = np.isfinite(u) & np.isfinite(v) & np.isfinite(w)
+    sky = np.zeros_like(vis)
+    sky[ok] = synth.point_source_vis((u[ok], v[ok], w[ok]), src)
+    noisy = (sky + noise * (rng.normal(size=len(u)) + 1j * rng.normal(size=len(u)))).astype(np.complex64)
+    return np.where(np.isfinite(vis), noisy, vis).astype(np.complex64)
+
+
+def _std_setup(gpu, oracle_mod, seed=5, n=30000, nx=100, arcsec=7.0, os_=128, sources=False):
     cell = np.deg2rad(arcsec / 3600.0)
     g = gpu.StandardGridder(nx, nx, cell, cell, oversampling=os_)
     o = oracle_mod.StandardGridder(nx, nx, cell, cell, oversampling=os_)
@@ -88,6 +101,8 @@ def _std_setup(gpu, oracle_mod, seed=5, n=30000, nx=100, arcsec=7.0, os_=128):
     # cluster half the samples near the uv centre (dense tiles, atomics contention)
     u[: n // 2] *= 0.1
     v[: n // 2] *= 0.1
+    if sources:
+        vis = with_sources(seed, u, v, w, vis, nx, cell)
     return g, o, (u, v, w, wt, sf, vis, gr)
 
 
@@ -151,7 +166,7 @@ def test_standard_residual_and_degrid(gpu, oracle_mod):
 
 
 def test_standard_dirty_products(gpu, oracle_mod):
-    g, o, (u, v, w, wt, sf, vis, gr) = _std_setup(gpu, oracle_mod, seed=13)
+    g, o, (u, v, w, wt, sf, vis, gr) = _std_setup(gpu, oracle_mod, seed=13, sources=True)
     b = batch_of(gpu, u, v, w, wt, sf, vis, gr)
     opsf, odirty, osums = o.grid(u, v, wt, sf, vis, gridable=gr, mode=1)
     ref_psf, ref_res, ref_peak = o.dirty_products(opsf, odirty, osums)
@@ -239,6 +254,7 @@ def test_w_dirty_residual_degrid(gpu, oracle_mod, nx, arcsec, planes, sampling, 
     u, v, w, wt, sf, vis, gr = synth.random_samples(31, 30000, umax, wmax=maxw)
     u[:10000] *= 0.1
     v[:10000] *= 0.1
+    vis = with_sources(31, u, v, w, vis, nx, np.deg2rad(arcsec / 3600.0))
     b = batch_of(gpu, u, v, w, wt, sf, vis, gr)
     opsf, odirty, osums = wpg.grid(u, v, w, wt, sf, vis, gridable=gr, mode=1)
 
