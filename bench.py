@@ -253,7 +253,14 @@ def main():
         p64, d64, peak64 = g.finish_products(psf64, dirty64, nsum64, precision=pkg.F64)
         del psf64, dirty64
         ref_peak = torch.amax(torch.abs(d64), dim=(1, 2))
-        dirty_err = float(torch.max(torch.amax(torch.abs(d32 - d64), dim=(1, 2)) / ref_peak))
+        per_chan = torch.amax(torch.abs(d32 - d64), dim=(1, 2)) / ref_peak
+        bad = (~torch.isfinite(per_chan)).nonzero().flatten().tolist()
+        if bad:
+            c = bad[0]
+            log(f"non-finite dirty error in {len(bad)} channels, e.g. {c}: ref peak {ref_peak[c].item()}, "
+                f"FP32 finite {bool(torch.isfinite(d32[c]).all())}, FP64 finite {bool(torch.isfinite(d64[c]).all())}, "
+                f"nsum32 {nsum[c]}, nsum64 {nsum64[c]}, peak32 {peak32[c]}, peak64 {peak64[c]}")
+        dirty_err = float(torch.max(per_chan))
         psf_err = float(torch.max(torch.abs(p32 - p64)))
         result_extra.update(dirty_err_vs_fp64=dirty_err, psf_err_vs_fp64=psf_err)
         log(f"dirty image error FP32 batched vs FP64 plain: {dirty_err:.2e} of peak; PSF {psf_err:.2e}")

@@ -31,7 +31,7 @@ PAD_NONE = 2
 
 # Every symbol the header declares; tests check that the library exports them.
 EXPORTS = (
-    "chip_ctx_create", "chip_ctx_destroy", "chip_error_free", "chip_version",
+    "chip_ctx_create", "chip_ctx_destroy", "chip_ctx_set_stream", "chip_error_free", "chip_version",
     "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
     "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
@@ -138,6 +138,7 @@ def load():
         "chip_error_free": (None, [_P]),
         "chip_ctx_create": (ctypes.c_int, [ctypes.c_int, _PP, _PP]),
         "chip_ctx_destroy": (None, [_P]),
+        "chip_ctx_set_stream": (ctypes.c_int, [_P, ctypes.c_int, _P, _PP]),
         "chip_gridder_create": (ctypes.c_int, [_P, ctypes.POINTER(Geometry), _PP, _PP]),
         "chip_gridder_destroy": (None, [_P]),
         "chip_gridder_info_get": (ctypes.c_int, [_P, ctypes.POINTER(GridderInfo), _PP]),

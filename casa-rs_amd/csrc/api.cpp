@@ -61,8 +61,16 @@ int chip_ctx_create(int device, chip_ctx **out, char **err) {
         std::unique_ptr<chip_ctx> c(new chip_ctx);
         c->device = device;
         CHIP_HIP(hipSetDevice(device));
-        CHIP_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        CHIP_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+        c->stream = c->own_stream;
         *out = c.release();
+    });
+}
+
+int chip_ctx_set_stream(chip_ctx *ctx, int external, void *stream, char **err) {
+    return guarded(err, [&] {
+        if (!ctx) fail(CHIP_ERR_INVALID, "NULL context");
+        ctx->stream = external ? static_cast<hipStream_t>(stream) : ctx->own_stream;
     });
 }
 
@@ -70,7 +78,7 @@ void chip_ctx_destroy(chip_ctx *ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     ctx->fft.clear();
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
 }
 
@@ -87,6 +95,7 @@ int chip_gridder_create(chip_ctx *ctx, const chip_geometry *geo, chip_gridder **
         set_device(ctx);
         std::unique_ptr<chip_gridder> g(new chip_gridder);
         g->ctx = ctx;
+        g->device = ctx->device;
         g->geom = *geo;
         switch (geo->padding_mode) {
             case CHIP_PAD_COMPOSITE_1_2:
@@ -126,9 +135,11 @@ int chip_gridder_create(chip_ctx *ctx, const chip_geometry *geo, chip_gridder **
     });
 }
 
+// Destroy functions only touch the object itself (its device index is copied
+// at creation), so children and parents may be destroyed in any order.
 void chip_gridder_destroy(chip_gridder *g) {
     if (!g) return;
-    (void)hipSetDevice(g->ctx->device);
+    (void)hipSetDevice(g->device);
     delete g;
 }
 
@@ -205,6 +216,7 @@ int chip_vset_create(chip_gridder *g, const chip_samples *s, chip_vset **out, ch
         set_device(g->ctx);
         std::unique_ptr<chip_vset> vs(new chip_vset);
         vs->g = g;
+        vs->device = g->device;
         vset_build(*vs, *s);
         CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
         *out = vs.release();
@@ -213,7 +225,7 @@ int chip_vset_create(chip_gridder *g, const chip_samples *s, chip_vset **out, ch
 
 void chip_vset_destroy(chip_vset *vs) {
     if (!vs) return;
-    (void)hipSetDevice(vs->g->ctx->device);
+    (void)hipSetDevice(vs->device);
     delete vs;
 }
 

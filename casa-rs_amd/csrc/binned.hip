@@ -1,0 +1,346 @@
+// Binned FP32 gridding and degridding over the resident visibility stream.
+//
+// One workgroup per work item: a run of <= 2048 samples of one channel whose
+// anchor cells fall in one 32x32 uv tile.  The workgroup owns an LDS tile of
+// (32 + 2*halo)^2 cells (halo = largest half-support), accumulates into it
+// with LDS float atomics, and finally adds the non-zero LDS cells to the
+// global grid with global float atomics (few per tile).  Degridding reads a
+// model tile staged in LDS.
+//
+// Work is tap-parallel: one wavefront handles one sample at a time and its 64
+// lanes own distinct taps.  Lanes of one atomic instruction therefore always
+// hit distinct cells (consecutive samples of a bin are the same baseline a few
+// seconds apart and share cells, which made a lane-per-sample mapping
+// serialise on LDS atomics), taps of consecutive lanes are consecutive cells
+// in a row (bank friendly), and a sample's W taps are one contiguous
+// pre-gathered run (coalesced loads).  The degridded value is reduced across
+// the wavefront with shuffles.  MFMA is not used: gridding is a scatter, not a
+// dense contraction.
+//
+// Arithmetic per tap follows the reference (gridder.rs:1036-1050 standard,
+// :3680-3747 W): tap = wx*wy (f32) or cwt = K[|ix*s+ox|][|iy*s+oy|] conjugated
+// when w > 0; grid += value*tap; degrid += conj(cwt)*grid.
+#include "device.hpp"
+
+namespace chip {
+
+GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint32_t tiles_y) {
+    GridParams gp{};
+    gp.gx = static_cast<uint32_t>(g.gx);
+    gp.gy = static_cast<uint32_t>(g.gy);
+    gp.tile = tile;
+    gp.halo = halo;
+    gp.L = tile + 2 * halo;
+    gp.Ls = gp.L | 1u;
+    gp.tiles_y = tiles_y;
+    gp.sampling = g.sampling;
+    gp.width = g.width;
+    gp.tap_rows = g.os + 1;
+    gp.n_off = static_cast<uint32_t>(2 * g.norm_radius + 1);
+    gp.norm_radius = g.norm_radius;
+    gp.tap_stride = g.tap_stride;
+    gp.cells = g.gx * g.gy;
+    gp.taps = g.d_taps.as<float>();
+    gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
+    gp.compact = g.wproject ? g.d_compact.as<float2>() : nullptr;
+    gp.supports = g.wproject ? g.d_supports.as<int32_t>() : nullptr;
+    return gp;
+}
+
+// Number of L*Ls float arrays a mode keeps in LDS.
+template <int MODE>
+struct LdsArrays {
+    static constexpr int n = (MODE == kModeDirty || MODE == kModeResidual) ? 4 : 2;
+};
+
+template <int MODE, bool WP>
+static size_t binned_lds_bytes(const GridParams &gp) {
+    const size_t tap_floats = WP ? 0 : ((static_cast<size_t>(gp.tap_rows) * kTaps + 3) & ~size_t(3));
+    return (tap_floats + static_cast<size_t>(LdsArrays<MODE>::n) * gp.L * gp.Ls) * sizeof(float);
+}
+
+template <int MODE, bool WP>
+__global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
+                                                const uint32_t *__restrict__ gstart,
+                                                const uint2 *__restrict__ plan,
+                                                const float2 *__restrict__ vis,
+                                                const float *__restrict__ wt,
+                                                const uint32_t *__restrict__ index,
+                                                const float2 *__restrict__ values,
+                                                const float2 *__restrict__ model, float *psf,
+                                                float *acc_out, float2 *vis_out) {
+    extern __shared__ float lds[];
+    const WorkItem it = items[blockIdx.x];
+    const int L = static_cast<int>(gp.L);
+    const int Ls = static_cast<int>(gp.Ls);
+    const int cells = L * Ls;
+    const int tap_floats = WP ? 0 : ((static_cast<int>(gp.tap_rows) * kTaps + 3) & ~3);
+    float *taps = lds;
+    float *a0 = lds + tap_floats;
+    float *a1 = a0 + cells;
+    float *a2 = a1 + cells;
+    float *a3 = a2 + cells;
+    float *P_re = nullptr, *P_im = nullptr, *A_re = nullptr, *A_im = nullptr, *M_re = nullptr,
+          *M_im = nullptr;
+    if (MODE == kModePsf) { P_re = a0; P_im = a1; }
+    if (MODE == kModeDirty) { P_re = a0; P_im = a1; A_re = a2; A_im = a3; }
+    if (MODE == kModeResidual) { M_re = a0; M_im = a1; A_re = a2; A_im = a3; }
+    if (MODE == kModeAdjoint) { A_re = a0; A_im = a1; }
+    if (MODE == kModeDegrid) { M_re = a0; M_im = a1; }
+
+    const int tid = threadIdx.x;
+    const int nth = blockDim.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nwaves = nth >> 6;
+    const uint32_t tile_x = it.tile / gp.tiles_y;
+    const uint32_t tile_y = it.tile - tile_x * gp.tiles_y;
+    const int x0 = static_cast<int>(tile_x * gp.tile) - static_cast<int>(gp.halo);
+    const int y0 = static_cast<int>(tile_y * gp.tile) - static_cast<int>(gp.halo);
+    const uint64_t chan_off = static_cast<uint64_t>(it.chan) * gp.cells;
+
+    if (!WP)
+        for (int i = tid; i < static_cast<int>(gp.tap_rows) * kTaps; i += nth) taps[i] = gp.taps[i];
+    constexpr bool kModel = MODE == kModeResidual || MODE == kModeDegrid;
+    constexpr bool kAcc = MODE == kModeResidual || MODE == kModeDirty || MODE == kModeAdjoint;
+    constexpr bool kPsf = MODE == kModePsf || MODE == kModeDirty;
+    for (int c = tid; c < cells; c += nth) {
+        if (kModel) {
+            const int lx = c / Ls, ly = c - lx * Ls;
+            const int gxx = x0 + lx, gyy = y0 + ly;
+            float2 m = make_float2(0.0f, 0.0f);
+            if (model && ly < L && gxx >= 0 && gyy >= 0 && gxx < static_cast<int>(gp.gx) &&
+                gyy < static_cast<int>(gp.gy))
+                m = model[chan_off + static_cast<uint64_t>(gxx) * gp.gy + gyy];
+            M_re[c] = m.x;
+            M_im[c] = m.y;
+        }
+        if (kAcc) { A_re[c] = 0.0f; A_im[c] = 0.0f; }
+        if (kPsf) { P_re[c] = 0.0f; P_im[c] = 0.0f; }
+    }
+    __syncthreads();
+
+    const int R = gp.norm_radius;
+    for (uint32_t k = wave; k < it.count; k += nwaves) {
+        // One group of samples with identical plans (wavefront-uniform).
+        const uint32_t grp = it.start + k;
+        const uint32_t s0 = gstart[grp], s1 = gstart[grp + 1];
+        const uint2 pk = plan[s0];
+        const int ax = static_cast<int>(pk.x & 0xffffu) - x0;
+        const int ay = static_cast<int>(pk.x >> 16) - y0;
+        const uint32_t flags = pk.y;
+        // Group sums, lane-parallel over the samples, then across the wave:
+        //   PSF/DIRTY: w = sum w_i (all samples), v = sum w_i V_i (finite V)
+        //   RESIDUAL:  w = sum w_i, v = sum w_i V_i over usable samples
+        //   ADJOINT:   v = sum values_i
+        float2 v = make_float2(0.0f, 0.0f);
+        float w = 0.0f;
+        if (MODE != kModeDegrid) {
+            float vr = 0.0f, vi = 0.0f;
+            for (uint32_t j = s0 + lane; j < s1; j += 64) {
+                if (MODE == kModeAdjoint) {
+                    const float2 a = values[index[j]];
+                    vr += a.x;
+                    vi += a.y;
+                    continue;
+                }
+                const uint32_t fj = plan[j].y;
+                const float wj = wt[j];
+                const float2 vj = vis[j];
+                if (MODE == kModeResidual) {
+                    if (fj & kResidOkBit) {
+                        w += wj;
+                        vr += vj.x * wj;
+                        vi += vj.y * wj;
+                    }
+                } else {
+                    w += wj;
+                    if (MODE == kModeDirty && (fj & kVisOkBit)) {
+                        vr += vj.x * wj;
+                        vi += vj.y * wj;
+                    }
+                }
+            }
+            if (s1 - s0 > 1) {
+                vr = wave_sum_f(vr);
+                vi = wave_sum_f(vi);
+                if (MODE != kModeAdjoint) w = wave_sum_f(w);
+            } else {
+                vr = __shfl(vr, 0, 64);
+                vi = __shfl(vi, 0, 64);
+                w = __shfl(w, 0, 64);
+            }
+            v = make_float2(vr, vi);
+            if (MODE == kModeResidual && w == 0.0f) continue;
+        }
+        // Per-lane tap geometry.
+        int side, T;
+        const float2 *ktap = nullptr;
+        const float *wxr = nullptr, *wyr = nullptr;
+        float csign = 1.0f;
+        if (!WP) {
+            side = kTaps;
+            T = kTaps * kTaps;
+            wxr = taps + (pk.y & 0xffu) * kTaps;
+            wyr = taps + ((pk.y >> 8) & 0xffu) * kTaps;
+        } else {
+            const int ox = static_cast<int>(pk.y & 0xffu) - 128;
+            const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
+            const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
+            const int S = gp.supports[plane];
+            side = 2 * S + 1;
+            T = side * side;
+            ktap = gp.compact +
+                   (static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
+                    static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R))) *
+                       gp.tap_stride;
+            csign = (flags & kConjBit) ? -1.0f : 1.0f;
+            // anchor -> first tap
+        }
+        const int bx = WP ? ax - (side >> 1) : ax;  // standard anchors are already x_start
+        const int by = WP ? ay - (side >> 1) : ay;
+        float pr = 0.0f, pi = 0.0f;
+        if (kModel) {
+            for (int f = lane; f < T; f += 64) {
+                const int tx = f / side, ty = f - tx * side;
+                const int cell = (bx + tx) * Ls + (by + ty);
+                const float gr = M_re[cell], gi = M_im[cell];
+                if (!WP) {
+                    const float t = wxr[tx] * wyr[ty];
+                    pr += gr * t;
+                    pi += gi * t;
+                } else {
+                    const float2 c = ktap[f];
+                    const float cr = c.x, ci = -(c.y * csign);  // conj(cwt)
+                    pr += cr * gr - ci * gi;
+                    pi += cr * gi + ci * gr;
+                }
+            }
+            pr = wave_sum_f(pr);
+            pi = wave_sum_f(pi);
+            if (MODE == kModeDegrid) {
+                // every sample of the group has the same prediction
+                for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
+                continue;
+            }
+        }
+        // sum_i (V_i - p) w_i = sum_i w_i V_i - p sum_i w_i
+        float vr, vi;
+        if (MODE == kModeResidual) { vr = v.x - pr * w; vi = v.y - pi * w; }
+        else { vr = v.x; vi = v.y; }
+        const bool acc_on = MODE == kModeResidual || MODE == kModeAdjoint ||
+                            (MODE == kModeDirty && (vr != 0.0f || vi != 0.0f));
+        for (int f = lane; f < T; f += 64) {
+            const int tx = f / side, ty = f - tx * side;
+            const int cell = (bx + tx) * Ls + (by + ty);
+            if (!WP) {
+                const float t = wxr[tx] * wyr[ty];
+                if (kPsf) atomicAdd(P_re + cell, w * t);
+                if (acc_on) {
+                    atomicAdd(A_re + cell, vr * t);
+                    atomicAdd(A_im + cell, vi * t);
+                }
+            } else {
+                const float2 c = ktap[f];
+                const float cr = c.x, ci = c.y * csign;
+                if (kPsf) {
+                    atomicAdd(P_re + cell, w * cr);
+                    atomicAdd(P_im + cell, w * ci);
+                }
+                if (acc_on) {
+                    atomicAdd(A_re + cell, vr * cr - vi * ci);
+                    atomicAdd(A_im + cell, vr * ci + vi * cr);
+                }
+            }
+        }
+    }
+    if (MODE == kModeDegrid) return;
+    __syncthreads();
+
+    // Flush the non-zero cells of the LDS tile to the global grid.
+    for (int c = tid; c < cells; c += nth) {
+        const int lx = c / Ls, ly = c - lx * Ls;
+        const int gxx = x0 + lx, gyy = y0 + ly;
+        if (ly >= L || gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy))
+            continue;
+        const uint64_t gi = 2 * (chan_off + static_cast<uint64_t>(gxx) * gp.gy + gyy);
+        if (kPsf) {
+            const float r = P_re[c], i = P_im[c];
+            if (r != 0.0f) unsafeAtomicAdd(psf + gi, r);
+            if (i != 0.0f) unsafeAtomicAdd(psf + gi + 1, i);
+        }
+        if (kAcc) {
+            const float r = A_re[c], i = A_im[c];
+            if (r != 0.0f) unsafeAtomicAdd(acc_out + gi, r);
+            if (i != 0.0f) unsafeAtomicAdd(acc_out + gi + 1, i);
+        }
+    }
+}
+
+template <int MODE, bool WP>
+static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
+                            const float2 *model, float *psf, float *acc, float2 *vis_out,
+                            hipStream_t st) {
+    const size_t lds = binned_lds_bytes<MODE, WP>(gp);
+    if (lds > 64 * 1024)
+        fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
+                                   std::to_string(gp.halo) + ")");
+    if (vs.n_items == 0) return;
+    // Events on the launch stream bracket the kernel (chip_vset_stats).
+    CHIP_HIP(hipEventRecord(vs.ev_start, st));
+    hipLaunchKernelGGL((k_binned<MODE, WP>), dim3(static_cast<unsigned>(vs.n_items)), dim3(256),
+                       lds, st, gp, vs.items.as<WorkItem>(), vs.gstart.as<uint32_t>(),
+                       vs.plan.as<uint2>(),
+                       vs.vis.as<float2>(), vs.wt.as<float>(), vs.index.as<uint32_t>(), values,
+                       model, psf, acc, vis_out);
+    CHIP_HIP(hipGetLastError());
+    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
+    vs.timed = true;
+}
+
+template <int MODE>
+static void launch_binned(chip_vset &vs, const float2 *values, const float2 *model, float *psf,
+                          float *acc, float2 *vis_out) {
+    const chip_gridder &g = *vs.g;
+    const GridParams gp = grid_params(g, vs.tile, vs.halo, vs.tiles_y);
+    if (vs.n_items > 0x7fffffffull) fail(CHIP_ERR_INVALID, "too many work items");
+    if (g.wproject)
+        launch_binned_t<MODE, true>(vs, gp, values, model, psf, acc, vis_out, g.ctx->stream);
+    else
+        launch_binned_t<MODE, false>(vs, gp, values, model, psf, acc, vis_out, g.ctx->stream);
+}
+
+void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, float *dirty) {
+    const float2 *m = reinterpret_cast<const float2 *>(model);
+    switch (mode) {
+        case CHIP_MODE_PSF:
+            if (!psf) fail(CHIP_ERR_INVALID, "PSF mode needs a PSF grid");
+            launch_binned<kModePsf>(vs, nullptr, nullptr, psf, nullptr, nullptr);
+            break;
+        case CHIP_MODE_DIRTY:
+            if (!psf || !dirty) fail(CHIP_ERR_INVALID, "dirty mode needs PSF and dirty grids");
+            launch_binned<kModeDirty>(vs, nullptr, nullptr, psf, dirty, nullptr);
+            break;
+        case CHIP_MODE_RESIDUAL:
+            if (!dirty) fail(CHIP_ERR_INVALID, "residual mode needs a residual grid");
+            launch_binned<kModeResidual>(vs, nullptr, m, nullptr, dirty, nullptr);
+            break;
+        default:
+            fail(CHIP_ERR_INVALID, "unknown grid mode");
+    }
+}
+
+void vset_degrid(chip_vset &vs, const float *model, float *vis_out) {
+    if (!model || !vis_out) fail(CHIP_ERR_INVALID, "degrid needs a model grid and an output");
+    CHIP_HIP(hipMemsetAsync(vis_out, 0, vs.n_samples * sizeof(float2), vs.g->ctx->stream));
+    launch_binned<kModeDegrid>(vs, nullptr, reinterpret_cast<const float2 *>(model), nullptr,
+                               nullptr, reinterpret_cast<float2 *>(vis_out));
+}
+
+void vset_grid_values(chip_vset &vs, const float *values, float *grid) {
+    if (!values || !grid) fail(CHIP_ERR_INVALID, "adjoint gridding needs values and a grid");
+    launch_binned<kModeAdjoint>(vs, reinterpret_cast<const float2 *>(values), nullptr, nullptr,
+                                grid, nullptr);
+}
+
+}  // namespace chip

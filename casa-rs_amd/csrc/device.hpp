@@ -26,12 +26,12 @@ struct Plan {
     float norm;
 };
 
-// One work item: a run of at most kChunk sorted samples of one (channel, tile).
+// One work item: a run of groups (identical plans) of one (channel, tile).
 struct WorkItem {
     uint32_t chan;
     uint32_t tile;   // tile_x * tiles_y + tile_y
-    uint32_t start;  // first stream position
-    uint32_t count;
+    uint32_t start;  // first group
+    uint32_t count;  // groups
 };
 
 struct SampleIn {
@@ -186,6 +186,35 @@ __device__ __forceinline__ uint32_t sample_flags(const PlanParams &p, const Samp
     const float gw = grid_weight(p, s);
     if (vis_ok && finite_f(gw) && gw > 0.0f) f |= kResidOkBit;
     return f;
+}
+
+enum : int { kModePsf = 0, kModeDirty = 1, kModeResidual = 2, kModeAdjoint = 3, kModeDegrid = 4 };
+
+// Kernel-side view of a gridder.
+struct GridParams {
+    uint32_t gx, gy;
+    uint32_t tile, halo;
+    uint32_t L;              // LDS tile edge: tile + 2*halo
+    uint32_t Ls;             // LDS row stride (L rounded up to odd, fewer bank conflicts)
+    uint32_t tiles_y;
+    uint32_t sampling, width;
+    uint32_t tap_rows;       // standard tap-table rows (os + 1)
+    uint32_t n_off;          // W offsets per axis, 2R + 1
+    int32_t norm_radius;     // R
+    uint32_t tap_stride;     // W compact taps per (plane, offset): (2*max_support+1)^2
+    uint64_t cells;          // gx * gy
+    const float *taps;       // (os+1) x 7
+    const float2 *kernels;   // planes x width x width
+    const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest
+    const int32_t *supports;
+};
+
+GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint32_t tiles_y);
+
+// Wavefront-wide sum of a float (64 lanes); every lane receives the sum.
+__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
 // Wavefront-wide sum of a double (64 lanes).

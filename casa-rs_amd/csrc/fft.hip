@@ -16,7 +16,10 @@ hipfftHandle FftCache::get(uint64_t gx, uint64_t gy, uint64_t batch, hipfftType 
                            hipStream_t s) {
     auto key = std::make_tuple(gx, gy, batch, static_cast<int>(type));
     auto it = plans.find(key);
-    if (it != plans.end()) return it->second;
+    if (it != plans.end()) {
+        CHIP_FFT(hipfftSetStream(it->second, s));  // the context's stream may have changed
+        return it->second;
+    }
     hipfftHandle h;
     CHIP_FFT(hipfftCreate(&h));
     int n[2] = {static_cast<int>(gx), static_cast<int>(gy)};

@@ -1,12 +1,4 @@
-// Gridding and degridding kernels.
-//
-// Binned FP32 path (chip_vset_*): one workgroup per work item (a run of
-// samples of one channel that fall in one 32x32 uv tile).  The workgroup
-// accumulates the tile plus a halo of the largest half-support in LDS with
-// LDS float atomics, then adds the non-zero LDS cells to the global grid with
-// few global atomics.  Degridding reads a model tile staged in LDS.  Every
-// lane owns whole samples, so no cross-lane reduction is needed and all 64
-// lanes of a wavefront stay busy.  MFMA is not used: the work is a scatter.
+// Plain per-channel gridding and degridding (the binned path is in binned.hip).
 //
 // Plain path (chip_grid_plain): one launch per channel, one thread per
 // sample, taps added straight to the global grid with FP64 (or FP32) atomics.
@@ -22,305 +14,6 @@
 #include "device.hpp"
 
 namespace chip {
-
-enum : int { kModePsf = 0, kModeDirty = 1, kModeResidual = 2, kModeAdjoint = 3, kModeDegrid = 4 };
-
-struct GridParams {
-    uint32_t gx, gy;
-    uint32_t tile, halo, L;  // L = tile + 2*halo
-    uint32_t tiles_y;
-    uint32_t sampling, width;
-    uint32_t tap_rows;       // standard tap-table rows (os + 1)
-    uint64_t cells;          // gx * gy
-    const float *taps;       // (os+1) x 7
-    const float2 *kernels;   // planes x width x width
-    const int32_t *supports;
-};
-
-static GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo,
-                              uint32_t tiles_y) {
-    GridParams gp{};
-    gp.gx = static_cast<uint32_t>(g.gx);
-    gp.gy = static_cast<uint32_t>(g.gy);
-    gp.tile = tile;
-    gp.halo = halo;
-    gp.L = tile + 2 * halo;
-    gp.tiles_y = tiles_y;
-    gp.sampling = g.sampling;
-    gp.width = g.width;
-    gp.tap_rows = g.os + 1;
-    gp.cells = g.gx * g.gy;
-    gp.taps = g.d_taps.as<float>();
-    gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
-    gp.supports = g.wproject ? g.d_supports.as<int32_t>() : nullptr;
-    return gp;
-}
-
-template <int MODE>
-struct LdsPlan {
-    // number of float arrays of L*L cells
-    static constexpr int arrays = MODE == kModePsf ? 2 : (MODE == kModeDirty ? 4 : (MODE == kModeResidual ? 4 : 2));
-};
-
-__device__ __forceinline__ void lds_add(float *a, int i, float v) { atomicAdd(a + i, v); }
-
-// ---------------------------------------------------------------------------
-// Binned kernel.
-// ---------------------------------------------------------------------------
-template <int MODE, bool WP>
-__global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *items,
-                                                const uint2 *plan, const float2 *vis,
-                                                const float *wt, const uint32_t *index,
-                                                const float2 *values, const float2 *model,
-                                                float *psf, float *acc_out, float2 *vis_out) {
-    extern __shared__ float lds[];
-    const WorkItem it = items[blockIdx.x];
-    const int L = static_cast<int>(gp.L);
-    const int cells = L * L;
-    const int tap_floats = WP ? 0 : ((static_cast<int>(gp.tap_rows) * kTaps + 3) & ~3);
-    float *taps = lds;
-    float *a0 = lds + tap_floats;  // first array
-    float *a1 = a0 + cells;
-    float *a2 = a1 + cells;
-    float *a3 = a2 + cells;
-    constexpr int narr = LdsPlan<MODE>::arrays;
-    // Array roles.
-    float *P_re = nullptr, *P_im = nullptr, *A_re = nullptr, *A_im = nullptr, *M_re = nullptr,
-          *M_im = nullptr;
-    if (MODE == kModePsf) { P_re = a0; P_im = a1; }
-    if (MODE == kModeDirty) { P_re = a0; P_im = a1; A_re = a2; A_im = a3; }
-    if (MODE == kModeResidual) { M_re = a0; M_im = a1; A_re = a2; A_im = a3; }
-    if (MODE == kModeAdjoint) { A_re = a0; A_im = a1; }
-    if (MODE == kModeDegrid) { M_re = a0; M_im = a1; }
-
-    const int tid = threadIdx.x;
-    const int nth = blockDim.x;
-    const uint32_t tile_x = it.tile / gp.tiles_y;
-    const uint32_t tile_y = it.tile - tile_x * gp.tiles_y;
-    const int x0 = static_cast<int>(tile_x * gp.tile) - static_cast<int>(gp.halo);
-    const int y0 = static_cast<int>(tile_y * gp.tile) - static_cast<int>(gp.halo);
-    const uint64_t chan_off = static_cast<uint64_t>(it.chan) * gp.cells;
-
-    if (!WP)
-        for (int i = tid; i < static_cast<int>(gp.tap_rows) * kTaps; i += nth) taps[i] = gp.taps[i];
-    const bool has_model = (MODE == kModeResidual || MODE == kModeDegrid) && model != nullptr;
-    for (int c = tid; c < cells; c += nth) {
-        float m_re = 0.0f, m_im = 0.0f;
-        if (has_model) {
-            const int gxx = x0 + c / L, gyy = y0 + c % L;
-            if (gxx >= 0 && gyy >= 0 && gxx < static_cast<int>(gp.gx) && gyy < static_cast<int>(gp.gy)) {
-                const float2 m = model[chan_off + static_cast<uint64_t>(gxx) * gp.gy + gyy];
-                m_re = m.x;
-                m_im = m.y;
-            }
-        }
-        if (MODE == kModeResidual || MODE == kModeDegrid) { M_re[c] = m_re; M_im[c] = m_im; }
-        if (MODE == kModeResidual || MODE == kModeDirty || MODE == kModeAdjoint) { A_re[c] = 0.0f; A_im[c] = 0.0f; }
-        if (MODE == kModePsf || MODE == kModeDirty) { P_re[c] = 0.0f; if (WP) P_im[c] = 0.0f; }
-    }
-    __syncthreads();
-
-    const int s = static_cast<int>(gp.sampling);
-    const int W = static_cast<int>(gp.width);
-    for (uint32_t k = tid; k < it.count; k += nth) {
-        const uint32_t p = it.start + k;
-        const uint2 pk = plan[p];
-        const int ax = static_cast<int>(pk.x & 0xffffu) - x0;
-        const int ay = static_cast<int>(pk.x >> 16) - y0;
-        const uint32_t flags = pk.y;
-        float2 v = make_float2(0.0f, 0.0f);
-        float w = 1.0f;
-        if (MODE == kModeAdjoint) {
-            v = values[index[p]];
-        } else if (MODE != kModeDegrid) {
-            v = vis[p];
-            w = wt[p];
-        }
-        if (MODE == kModeResidual && !(flags & kResidOkBit)) continue;
-        if (!WP) {
-            const float *wxr = taps + (pk.y & 0xffu) * kTaps;
-            const float *wyr = taps + ((pk.y >> 8) & 0xffu) * kTaps;
-            float wx[kTaps], wy[kTaps];
-#pragma unroll
-            for (int t = 0; t < kTaps; ++t) { wx[t] = wxr[t]; wy[t] = wyr[t]; }
-            float pr = 0.0f, pi = 0.0f;
-            if (MODE == kModeResidual || MODE == kModeDegrid) {
-#pragma unroll
-                for (int tx = 0; tx < kTaps; ++tx) {
-                    const int row = (ax + tx) * L + ay;
-#pragma unroll
-                    for (int ty = 0; ty < kTaps; ++ty) {
-                        const float t = wx[tx] * wy[ty];
-                        pr += M_re[row + ty] * t;
-                        pi += M_im[row + ty] * t;
-                    }
-                }
-            }
-            if (MODE == kModeDegrid) {
-                vis_out[index[p]] = make_float2(pr, pi);
-                continue;
-            }
-            float vr, vi;
-            if (MODE == kModeResidual) { vr = (v.x - pr) * w; vi = (v.y - pi) * w; }
-            else if (MODE == kModeAdjoint) { vr = v.x; vi = v.y; }
-            else { vr = v.x * w; vi = v.y * w; }
-            const bool dirty_on = MODE == kModeResidual || MODE == kModeAdjoint ||
-                                  (MODE == kModeDirty && (flags & kVisOkBit));
-#pragma unroll
-            for (int tx = 0; tx < kTaps; ++tx) {
-                const int row = (ax + tx) * L + ay;
-#pragma unroll
-                for (int ty = 0; ty < kTaps; ++ty) {
-                    const float t = wx[tx] * wy[ty];
-                    if (MODE == kModePsf || MODE == kModeDirty) lds_add(P_re, row + ty, w * t);
-                    if (dirty_on) {
-                        lds_add(A_re, row + ty, vr * t);
-                        lds_add(A_im, row + ty, vi * t);
-                    }
-                }
-            }
-        } else {
-            const int ox = static_cast<int>(pk.y & 0xffu) - 128;
-            const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
-            const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
-            const float csign = (flags & kConjBit) ? -1.0f : 1.0f;
-            const int S = gp.supports[plane];
-            const float2 *K = gp.kernels + static_cast<uint64_t>(plane) * W * W;
-            float pr = 0.0f, pi = 0.0f;
-            if (MODE == kModeResidual || MODE == kModeDegrid) {
-                for (int iy = -S; iy <= S; ++iy) {
-                    const int ky = abs(iy * s + oy);
-                    for (int ix = -S; ix <= S; ++ix) {
-                        const int kx = abs(ix * s + ox);
-                        const float2 c = K[kx * W + ky];
-                        const float cr = c.x, ci = -(c.y * csign);  // conj(cwt)
-                        const int cell = (ax + ix) * L + (ay + iy);
-                        const float gr = M_re[cell], gi = M_im[cell];
-                        pr += cr * gr - ci * gi;
-                        pi += cr * gi + ci * gr;
-                    }
-                }
-            }
-            if (MODE == kModeDegrid) {
-                vis_out[index[p]] = make_float2(pr, pi);
-                continue;
-            }
-            float vr, vi;
-            if (MODE == kModeResidual) { vr = (v.x - pr) * w; vi = (v.y - pi) * w; }
-            else if (MODE == kModeAdjoint) { vr = v.x; vi = v.y; }
-            else { vr = v.x * w; vi = v.y * w; }
-            for (int iy = -S; iy <= S; ++iy) {
-                const int ky = abs(iy * s + oy);
-                for (int ix = -S; ix <= S; ++ix) {
-                    const int kx = abs(ix * s + ox);
-                    const float2 c = K[kx * W + ky];
-                    const float cr = c.x, ci = c.y * csign;
-                    const int cell = (ax + ix) * L + (ay + iy);
-                    if (MODE == kModePsf || MODE == kModeDirty) {
-                        lds_add(P_re, cell, w * cr);
-                        lds_add(P_im, cell, w * ci);
-                    }
-                    if (MODE != kModePsf) {
-                        lds_add(A_re, cell, vr * cr - vi * ci);
-                        lds_add(A_im, cell, vr * ci + vi * cr);
-                    }
-                }
-            }
-        }
-    }
-    if (MODE == kModeDegrid) return;
-    __syncthreads();
-
-    // Flush: add the non-zero cells of the LDS tile to the global grid.
-    for (int c = tid; c < cells; c += nth) {
-        const int gxx = x0 + c / L, gyy = y0 + c % L;
-        if (gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy))
-            continue;
-        const uint64_t gi = 2 * (chan_off + static_cast<uint64_t>(gxx) * gp.gy + gyy);
-        if (MODE == kModePsf || MODE == kModeDirty) {
-            const float r = P_re[c];
-            if (r != 0.0f) unsafeAtomicAdd(psf + gi, r);
-            if (WP) {
-                const float i = P_im[c];
-                if (i != 0.0f) unsafeAtomicAdd(psf + gi + 1, i);
-            }
-        }
-        if (MODE != kModePsf) {
-            const float r = A_re[c], i = A_im[c];
-            if (r != 0.0f) unsafeAtomicAdd(acc_out + gi, r);
-            if (i != 0.0f) unsafeAtomicAdd(acc_out + gi + 1, i);
-        }
-    }
-    (void)narr;
-}
-
-template <int MODE, bool WP>
-static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
-                            const float2 *model, float *psf, float *acc, float2 *vis_out,
-                            hipStream_t st) {
-    const int tap_floats = WP ? 0 : ((static_cast<int>(gp.tap_rows) * kTaps + 3) & ~3);
-    const size_t lds = (static_cast<size_t>(tap_floats) +
-                        static_cast<size_t>(LdsPlan<MODE>::arrays) * gp.L * gp.L) * sizeof(float);
-    if (lds > 64 * 1024)
-        fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
-                                   std::to_string(gp.halo) + ")");
-    if (vs.n_items == 0) return;
-    // The kernel is bracketed by events on its own stream so callers can read
-    // its device duration (chip_vset_last_kernel_ms).
-    CHIP_HIP(hipEventRecord(vs.ev_start, st));
-    hipLaunchKernelGGL((k_binned<MODE, WP>), dim3(static_cast<unsigned>(vs.n_items)), dim3(256),
-                       lds, st, gp, vs.items.as<WorkItem>(), vs.plan.as<uint2>(),
-                       vs.vis.as<float2>(), vs.wt.as<float>(), vs.index.as<uint32_t>(), values,
-                       model, psf, acc, vis_out);
-    CHIP_HIP(hipGetLastError());
-    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
-    vs.timed = true;
-}
-
-template <int MODE>
-static void launch_binned(chip_vset &vs, const float2 *values, const float2 *model,
-                          float *psf, float *acc, float2 *vis_out) {
-    const chip_gridder &g = *vs.g;
-    const GridParams gp = grid_params(g, vs.tile, vs.halo, vs.tiles_y);
-    if (vs.n_items > 0x7fffffffull) fail(CHIP_ERR_INVALID, "too many work items");
-    if (g.wproject)
-        launch_binned_t<MODE, true>(vs, gp, values, model, psf, acc, vis_out, g.ctx->stream);
-    else
-        launch_binned_t<MODE, false>(vs, gp, values, model, psf, acc, vis_out, g.ctx->stream);
-}
-
-void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, float *dirty) {
-    const float2 *m = reinterpret_cast<const float2 *>(model);
-    switch (mode) {
-        case CHIP_MODE_PSF:
-            if (!psf) fail(CHIP_ERR_INVALID, "PSF mode needs a PSF grid");
-            launch_binned<kModePsf>(vs, nullptr, nullptr, psf, nullptr, nullptr);
-            break;
-        case CHIP_MODE_DIRTY:
-            if (!psf || !dirty) fail(CHIP_ERR_INVALID, "dirty mode needs PSF and dirty grids");
-            launch_binned<kModeDirty>(vs, nullptr, nullptr, psf, dirty, nullptr);
-            break;
-        case CHIP_MODE_RESIDUAL:
-            if (!dirty) fail(CHIP_ERR_INVALID, "residual mode needs a residual grid");
-            launch_binned<kModeResidual>(vs, nullptr, m, nullptr, dirty, nullptr);
-            break;
-        default:
-            fail(CHIP_ERR_INVALID, "unknown grid mode");
-    }
-}
-
-void vset_degrid(chip_vset &vs, const float *model, float *vis_out) {
-    if (!model || !vis_out) fail(CHIP_ERR_INVALID, "degrid needs a model grid and an output");
-    CHIP_HIP(hipMemsetAsync(vis_out, 0, vs.n_samples * sizeof(float2), vs.g->ctx->stream));
-    launch_binned<kModeDegrid>(vs, nullptr, reinterpret_cast<const float2 *>(model), nullptr,
-                               nullptr, reinterpret_cast<float2 *>(vis_out));
-}
-
-void vset_grid_values(chip_vset &vs, const float *values, float *grid) {
-    if (!values || !grid) fail(CHIP_ERR_INVALID, "adjoint gridding needs values and a grid");
-    launch_binned<kModeAdjoint>(vs, reinterpret_cast<const float2 *>(values), nullptr, nullptr,
-                                grid, nullptr);
-}
 
 // ---------------------------------------------------------------------------
 // Plain per-channel kernels.

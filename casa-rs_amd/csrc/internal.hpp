@@ -84,13 +84,15 @@ struct FftCache {
 
 struct chip_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;  // created with the context
+    hipStream_t stream = nullptr;      // where work is launched: own_stream or the caller's
     chip::FftCache fft;
 };
 
 // Host-side tables of a StandardGridder and, optionally, its WProjector.
 struct chip_gridder {
     chip_ctx *ctx = nullptr;
+    int device = 0;  // kept so that destruction never dereferences ctx
     chip_geometry geom{};
     uint64_t gx = 0, gy = 0, blc_x = 0, blc_y = 0;
     double du = 0, dv = 0;
@@ -108,6 +110,10 @@ struct chip_gridder {
     std::vector<float> norm_table;  // planes x (2R+1)^2
     chip::DevBuf d_kernels;         // planes x width x width float2
     chip::DevBuf d_supports, d_norm_table;
+    // Taps of every (plane, off_x, off_y) gathered contiguously so that the
+    // lanes of a wavefront read one sample's taps with coalesced loads.
+    chip::DevBuf d_compact;         // planes x (2R+1)^2 x tap_stride float2
+    uint32_t tap_stride = 0;
     std::vector<float> sinc;        // build_sinc_axis(max(gx, gy), sampling)
     std::vector<float> sinc_x, sinc_y;  // per-axis sinc used by model apodisation
     chip::DevBuf d_sinc, d_sinc_x, d_sinc_y;
@@ -181,6 +187,7 @@ void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *
 // Resident, binned visibility stream.
 struct chip_vset {
     chip_gridder *g = nullptr;
+    int device = 0;  // kept so that destruction never dereferences g
     uint64_t n_samples = 0;   // input samples (rows x channels)
     uint64_t n_chan = 0;
     uint64_t n_valid = 0;     // accepted samples in the stream
@@ -193,7 +200,9 @@ struct chip_vset {
     chip::DevBuf vis;     // float2
     chip::DevBuf wt;      // float
     chip::DevBuf index;   // uint32 original sample index
-    chip::DevBuf items;   // WorkItem
+    chip::DevBuf items;   // WorkItem (chan, tile, first group, group count)
+    chip::DevBuf gstart;  // uint32 first stream position of each group, plus n_valid
+    uint64_t n_groups = 0;
     std::vector<chip_sums> sums;
     uint64_t taps = 0;    // sum over accepted samples of (2S+1)^2
     // events around the last binned kernel launch

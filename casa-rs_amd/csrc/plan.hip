@@ -6,10 +6,14 @@
 // round-half-away-from-zero, each separately rounded.
 //
 // The resident stream replaces the reference's per-chunk sample packing
-// (lib.rs:71708-71767): accepted samples are keyed by (channel, uv tile),
-// sorted with an integer radix sort, and split into work items of at most
-// kChunk samples so that one launch covers every channel with balanced
-// workgroups.
+// (lib.rs:71708-71767): accepted samples are keyed by (channel, uv tile,
+// anchor cell, plan) and sorted with an integer radix sort, so samples with
+// identical plans form contiguous groups (the reference's Metal path merges
+// adjacent identical plans, lib.rs:71646-71660; here every identical plan of
+// a bin is grouped, and the group's samples are summed exactly as the
+// per-sample sum would be: sum (V_i - p) w_i = sum w_i V_i - p sum w_i).
+// Groups of one bin are split into work items of at most kGroupsPerItem so
+// that one launch covers every channel with balanced workgroups.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -18,7 +22,7 @@
 
 namespace chip {
 
-constexpr uint32_t kChunk = 2048;  // samples per work item (upper bound)
+constexpr uint32_t kGroupsPerItem = 1024;  // groups of identical plans per work item (upper bound)
 
 PlanParams plan_params(const chip_gridder &g) {
     PlanParams p{};
@@ -127,32 +131,74 @@ double launch_max_abs_w(const chip_gridder &g, const Source &src, hipStream_t st
 }
 
 // ---- binned stream ---------------------------------------------------------
+// Sort key (64 bits, most significant first):
+//   bin   = channel * n_tiles + uv tile of the anchor cell
+//   cell  = anchor cell inside the tile (10 bits for 32x32 tiles)
+//   plan  = the rest of the plan: tap-row selectors (standard) or
+//           plane, conjugation and offsets (W)
+// so that samples with identical plans become contiguous groups inside a
+// bin.  Rejected samples get the invalid bin and sort last.  When the plan
+// does not fit, plan_bits = 0 and every sample is its own group.
 struct BinParams {
     uint32_t tile;
     uint32_t tiles_x, tiles_y;
     uint32_t n_tiles;
-    uint32_t invalid_key;  // n_chan * n_tiles
+    uint32_t n_bins;        // n_chan * n_tiles (also the invalid bin)
+    uint32_t cell_bits;     // log2(tile * tile)
+    uint32_t plan_bits;
+    uint32_t n_sel;         // selector values per axis
 };
 
-// Key = channel * n_tiles + tile of the anchor cell; rejected samples get
-// invalid_key and sort to the end.  Also histograms the keys.
+__device__ __forceinline__ uint64_t plan_code(const PlanParams &p, const BinParams &b, const Plan &pl) {
+    if (p.wproject) {
+        const uint64_t pc = static_cast<uint64_t>(pl.plane) * 2 + static_cast<uint64_t>(pl.conj);
+        return (pc * b.n_sel + static_cast<uint64_t>(pl.sx + p.norm_radius)) * b.n_sel +
+               static_cast<uint64_t>(pl.sy + p.norm_radius);
+    }
+    return static_cast<uint64_t>(pl.sx) * b.n_sel + static_cast<uint64_t>(pl.sy);
+}
+
 __global__ __launch_bounds__(256) void k_keys(PlanParams p, Source s, BinParams b,
-                                              uint32_t *keys, uint32_t *vals, uint32_t *hist) {
+                                              uint64_t *keys, uint32_t *vals,
+                                              unsigned long long *n_valid) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     const uint64_t n = s.n_rows * s.n_chan;
-    if (i >= n) return;
-    const SampleIn smp = load_sample(s, i);
-    Plan pl;
-    uint32_t key = b.invalid_key;
-    if (sample_accepted(p, smp, pl)) {
-        const uint32_t ax = static_cast<uint32_t>(p.wproject ? pl.x : pl.x + kSupport);
-        const uint32_t ay = static_cast<uint32_t>(p.wproject ? pl.y : pl.y + kSupport);
-        const uint32_t tile = (ax / b.tile) * b.tiles_y + (ay / b.tile);
-        key = smp.chan * b.n_tiles + tile;
-        atomicAdd(&hist[key], 1u);
+    bool ok = false;
+    if (i < n) {
+        const SampleIn smp = load_sample(s, i);
+        Plan pl;
+        uint64_t key = static_cast<uint64_t>(b.n_bins) << (b.cell_bits + b.plan_bits);
+        if (sample_accepted(p, smp, pl)) {
+            ok = true;
+            const uint32_t ax = static_cast<uint32_t>(p.wproject ? pl.x : pl.x + kSupport);
+            const uint32_t ay = static_cast<uint32_t>(p.wproject ? pl.y : pl.y + kSupport);
+            const uint32_t tile = (ax / b.tile) * b.tiles_y + (ay / b.tile);
+            const uint64_t bin = static_cast<uint64_t>(smp.chan) * b.n_tiles + tile;
+            const uint64_t cell = static_cast<uint64_t>((ax % b.tile) * b.tile + (ay % b.tile));
+            key = (bin << (b.cell_bits + b.plan_bits)) | (cell << b.plan_bits);
+            if (b.plan_bits) key |= plan_code(p, b, pl);
+        }
+        keys[i] = key;
+        vals[i] = static_cast<uint32_t>(i);
     }
-    keys[i] = key;
-    vals[i] = static_cast<uint32_t>(i);
+    const unsigned long long c = wave_sum_u64(ok ? 1ull : 0ull);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(n_valid, c);
+}
+
+// Group heads: a sample starts a group when its key differs from the previous
+// one (every sample when plans are not merged).
+__global__ void k_group_flags(const uint64_t *keys, uint64_t n, bool merge, uint32_t *flags) {
+    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    flags[p] = (!merge || p == 0 || keys[p] != keys[p - 1]) ? 1u : 0u;
+}
+
+__global__ void k_group_starts(const uint64_t *keys, const uint32_t *flags, const uint32_t *gid,
+                               uint64_t n, uint32_t low_bits, uint32_t *gstart, uint32_t *bin_groups) {
+    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n || !flags[p]) return;
+    gstart[gid[p]] = static_cast<uint32_t>(p);
+    atomicAdd(&bin_groups[keys[p] >> low_bits], 1u);
 }
 
 // Writes the packed stream in sorted order and accumulates per-channel sums
@@ -213,26 +259,25 @@ __global__ __launch_bounds__(256) void k_build_stream(PlanParams p, Source s, ui
     }
 }
 
-// Number of work items per bin.
-__global__ void k_items_per_bin(const uint32_t *hist, uint32_t n_bins, uint32_t *items_per_bin) {
+// Work items per bin: runs of at most kGroupsPerItem groups of one bin.
+__global__ void k_items_per_bin(const uint32_t *bin_groups, uint32_t n_bins, uint32_t *items_per_bin) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bins) return;
-    items_per_bin[k] = (hist[k] + kChunk - 1) / kChunk;
+    items_per_bin[k] = (bin_groups[k] + kGroupsPerItem - 1) / kGroupsPerItem;
 }
 
-__global__ void k_write_items(const uint32_t *hist, const uint32_t *bin_start,
+__global__ void k_write_items(const uint32_t *bin_groups, const uint32_t *bin_first_group,
                               const uint32_t *item_start, uint32_t n_bins, uint32_t n_tiles,
                               WorkItem *items) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bins) return;
-    const uint32_t count = hist[k];
+    const uint32_t count = bin_groups[k];
     if (count == 0) return;
     const uint32_t chan = k / n_tiles;
     const uint32_t tile = k - chan * n_tiles;
-    const uint32_t nit = (count + kChunk - 1) / kChunk;
-    // Spread the bin evenly over its items.
-    const uint32_t base = count / nit, extra = count % nit;
-    uint32_t pos = bin_start[k];
+    const uint32_t nit = (count + kGroupsPerItem - 1) / kGroupsPerItem;
+    const uint32_t base = count / nit, extra = count % nit;  // spread evenly
+    uint32_t pos = bin_first_group[k];
     for (uint32_t j = 0; j < nit; ++j) {
         const uint32_t c = base + (j < extra ? 1u : 0u);
         items[item_start[k] + j] = WorkItem{chan, tile, pos, c};
@@ -240,14 +285,29 @@ __global__ void k_write_items(const uint32_t *hist, const uint32_t *bin_start,
     }
 }
 
-}  // namespace chip
+static int bits_for(uint64_t v) {  // smallest b with v < 2^b
+    int b = 0;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return b;
+}
 
-using namespace chip;
+static void exclusive_sum(const uint32_t *in, uint32_t *out, uint64_t n, hipStream_t st) {
+    size_t bytes = 0;
+    CHIP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, static_cast<int>(n), st));
+    DevBuf tmp(std::max<size_t>(bytes, 16));
+    CHIP_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, bytes, in, out, static_cast<int>(n), st));
+    CHIP_HIP(hipStreamSynchronize(st));
+}
 
-// Builds the resident stream for a vset.  Defined here because it owns the
-// sort; the C-ABI wrapper in api.cpp calls it.
-namespace chip {
+static uint32_t read_u32(const uint32_t *dev, hipStream_t st) {
+    uint32_t v = 0;
+    CHIP_HIP(hipMemcpyAsync(&v, dev, 4, hipMemcpyDeviceToHost, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+    return v;
+}
 
+// Builds the resident stream for a vset: FP64 plans, 64-bit radix sort by
+// (bin, anchor cell, plan), groups of identical plans, work items.
 void vset_build(chip_vset &vs, const chip_samples &cs) {
     chip_gridder &g = *vs.g;
     hipStream_t st = g.ctx->stream;
@@ -264,73 +324,69 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     if (n_bins64 >= (1ull << 31)) fail(CHIP_ERR_INVALID, "too many (channel, tile) bins");
     const uint32_t n_bins = static_cast<uint32_t>(n_bins64);
     vs.sums.assign(src.n_chan, chip_sums{});
+    CHIP_HIP(hipEventCreate(&vs.ev_start));
+    CHIP_HIP(hipEventCreate(&vs.ev_stop));
     if (n == 0) return;
 
-    BinParams b{vs.tile, vs.tiles_x, vs.tiles_y, static_cast<uint32_t>(n_tiles), n_bins};
-    int end_bit = 1;
-    while (end_bit < 32 && (1ull << end_bit) <= n_bins) ++end_bit;
+    BinParams b{};
+    b.tile = vs.tile;
+    b.tiles_x = vs.tiles_x;
+    b.tiles_y = vs.tiles_y;
+    b.n_tiles = static_cast<uint32_t>(n_tiles);
+    b.n_bins = n_bins;
+    b.cell_bits = static_cast<uint32_t>(bits_for(static_cast<uint64_t>(vs.tile) * vs.tile - 1));
+    b.n_sel = g.wproject ? static_cast<uint32_t>(2 * g.norm_radius + 1) : g.os + 1;
+    const uint64_t max_code = g.wproject
+        ? (static_cast<uint64_t>(g.planes) * 2 * b.n_sel) * b.n_sel - 1
+        : static_cast<uint64_t>(b.n_sel) * b.n_sel - 1;
+    const int bin_bits = bits_for(n_bins);  // includes the invalid bin
+    const int plan_bits = bits_for(max_code);
+    const bool merge = bin_bits + static_cast<int>(b.cell_bits) + plan_bits <= 64;
+    b.plan_bits = merge ? static_cast<uint32_t>(plan_bits) : 0u;
+    const int end_bit = bin_bits + static_cast<int>(b.cell_bits + b.plan_bits);
+    if (end_bit > 64) fail(CHIP_ERR_INVALID, "sort key exceeds 64 bits");
 
-    DevBuf keys(n * 4), vals(n * 4), keys_s(n * 4), vals_s(n * 4);
-    DevBuf hist((static_cast<uint64_t>(n_bins) + 1) * 4);
-    CHIP_HIP(hipMemsetAsync(hist.p, 0, hist.bytes, st));
+    DevBuf keys(n * 8), vals(n * 4), keys_s(n * 8), vals_s(n * 4);
+    DevBuf d_nvalid(8);
+    CHIP_HIP(hipMemsetAsync(d_nvalid.p, 0, 8, st));
     const PlanParams pp = plan_params(g);
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, pp, src, b,
-                       keys.as<uint32_t>(), vals.as<uint32_t>(), hist.as<uint32_t>());
+                       keys.as<uint64_t>(), vals.as<uint32_t>(), d_nvalid.as<unsigned long long>());
     CHIP_HIP(hipGetLastError());
-
     size_t temp_bytes = 0;
-    CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.as<uint32_t>(),
-                                                keys_s.as<uint32_t>(), vals.as<uint32_t>(),
+    CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.as<uint64_t>(),
+                                                keys_s.as<uint64_t>(), vals.as<uint32_t>(),
                                                 vals_s.as<uint32_t>(), static_cast<int>(n), 0,
                                                 end_bit, st));
-    DevBuf temp(std::max<size_t>(temp_bytes, 16));
-    CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, keys.as<uint32_t>(),
-                                                keys_s.as<uint32_t>(), vals.as<uint32_t>(),
-                                                vals_s.as<uint32_t>(), static_cast<int>(n), 0,
-                                                end_bit, st));
+    {
+        DevBuf temp(std::max<size_t>(temp_bytes, 16));
+        CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, keys.as<uint64_t>(),
+                                                    keys_s.as<uint64_t>(), vals.as<uint32_t>(),
+                                                    vals_s.as<uint32_t>(), static_cast<int>(n), 0,
+                                                    end_bit, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+    }
     keys.release();
     vals.release();
-
-    // Bin starts and item starts by exclusive scans of the histogram.
-    DevBuf bin_start((static_cast<uint64_t>(n_bins) + 1) * 4);
-    DevBuf ipb((static_cast<uint64_t>(n_bins) + 1) * 4), item_start((static_cast<uint64_t>(n_bins) + 1) * 4);
-    size_t scan_bytes = 0;
-    CHIP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, hist.as<uint32_t>(),
-                                              bin_start.as<uint32_t>(), n_bins + 1, st));
-    DevBuf scan_tmp(std::max<size_t>(scan_bytes, 16));
-    CHIP_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, hist.as<uint32_t>(),
-                                              bin_start.as<uint32_t>(), n_bins + 1, st));
-    CHIP_HIP(hipMemsetAsync(ipb.p, 0, ipb.bytes, st));
-    hipLaunchKernelGGL(k_items_per_bin, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
-                       hist.as<uint32_t>(), n_bins, ipb.as<uint32_t>());
-    CHIP_HIP(hipGetLastError());
-    CHIP_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp.p, scan_bytes, ipb.as<uint32_t>(),
-                                              item_start.as<uint32_t>(), n_bins + 1, st));
-    uint32_t n_valid = 0, n_items = 0;
-    CHIP_HIP(hipMemcpyAsync(&n_valid, bin_start.as<uint32_t>() + n_bins, 4, hipMemcpyDeviceToHost, st));
-    CHIP_HIP(hipMemcpyAsync(&n_items, item_start.as<uint32_t>() + n_bins, 4, hipMemcpyDeviceToHost, st));
+    unsigned long long nv_ll = 0;
+    CHIP_HIP(hipMemcpyAsync(&nv_ll, d_nvalid.p, 8, hipMemcpyDeviceToHost, st));
     CHIP_HIP(hipStreamSynchronize(st));
+    const uint64_t n_valid = nv_ll;
     vs.n_valid = n_valid;
-    vs.n_items = n_items;
 
-    vs.items.alloc(std::max<uint64_t>(n_items, 1) * sizeof(WorkItem));
-    hipLaunchKernelGGL(k_write_items, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
-                       hist.as<uint32_t>(), bin_start.as<uint32_t>(), item_start.as<uint32_t>(),
-                       n_bins, static_cast<uint32_t>(n_tiles), vs.items.as<WorkItem>());
-    CHIP_HIP(hipGetLastError());
-
-    const uint64_t nv = std::max<uint64_t>(n_valid, 1);
-    vs.plan.alloc(nv * sizeof(uint2));
-    vs.vis.alloc(nv * sizeof(float2));
-    vs.wt.alloc(nv * sizeof(float));
-    vs.index.alloc(nv * sizeof(uint32_t));
+    // packed per-sample stream + per-channel sums
+    const uint64_t nvb = std::max<uint64_t>(n_valid, 1);
+    vs.plan.alloc(nvb * sizeof(uint2));
+    vs.vis.alloc(nvb * sizeof(float2));
+    vs.wt.alloc(nvb * sizeof(float));
+    vs.index.alloc(nvb * sizeof(uint32_t));
     DevBuf sums(src.n_chan * kSumFields * sizeof(double));
     CHIP_HIP(hipMemsetAsync(sums.p, 0, sums.bytes, st));
     if (n_valid > 0) {
         hipLaunchKernelGGL(k_build_stream, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st, pp,
-                           src, static_cast<uint64_t>(n_valid), vals_s.as<uint32_t>(),
-                           vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
-                           vs.index.as<uint32_t>(), sums.as<double>());
+                           src, n_valid, vals_s.as<uint32_t>(), vs.plan.as<uint2>(),
+                           vs.vis.as<float2>(), vs.wt.as<float>(), vs.index.as<uint32_t>(),
+                           sums.as<double>());
         CHIP_HIP(hipGetLastError());
     }
     std::vector<double> hs(src.n_chan * kSumFields);
@@ -345,8 +401,50 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
         vs.sums[c].skipped = src.n_rows - vs.sums[c].gridded;
         vs.taps += static_cast<uint64_t>(h[3]);
     }
-    CHIP_HIP(hipEventCreate(&vs.ev_start));
-    CHIP_HIP(hipEventCreate(&vs.ev_stop));
+    vals_s.release();
+    if (n_valid == 0) return;
+
+    // groups of identical plans
+    DevBuf flags((n_valid + 1) * 4), gid((n_valid + 1) * 4);
+    CHIP_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, st));
+    hipLaunchKernelGGL(k_group_flags, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
+                       keys_s.as<uint64_t>(), n_valid, merge, flags.as<uint32_t>());
+    CHIP_HIP(hipGetLastError());
+    exclusive_sum(flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid + 1, st);
+    const uint32_t n_groups = read_u32(gid.as<uint32_t>() + n_valid, st);
+    vs.n_groups = n_groups;
+    vs.gstart.alloc((static_cast<uint64_t>(n_groups) + 1) * 4);
+    DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
+    CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
+    const uint32_t low_bits = b.cell_bits + b.plan_bits;
+    hipLaunchKernelGGL(k_group_starts, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
+                       keys_s.as<uint64_t>(), flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid,
+                       low_bits, vs.gstart.as<uint32_t>(), bin_groups.as<uint32_t>());
+    CHIP_HIP(hipGetLastError());
+    const uint32_t nv32 = static_cast<uint32_t>(n_valid);
+    CHIP_HIP(hipMemcpyAsync(vs.gstart.as<uint32_t>() + n_groups, &nv32, 4, hipMemcpyHostToDevice, st));
+    keys_s.release();
+    flags.release();
+    gid.release();
+
+    // work items
+    DevBuf first_group((static_cast<uint64_t>(n_bins) + 1) * 4);
+    DevBuf ipb((static_cast<uint64_t>(n_bins) + 1) * 4), item_start((static_cast<uint64_t>(n_bins) + 1) * 4);
+    exclusive_sum(bin_groups.as<uint32_t>(), first_group.as<uint32_t>(), n_bins + 1, st);
+    CHIP_HIP(hipMemsetAsync(ipb.p, 0, ipb.bytes, st));
+    hipLaunchKernelGGL(k_items_per_bin, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
+                       bin_groups.as<uint32_t>(), n_bins, ipb.as<uint32_t>());
+    CHIP_HIP(hipGetLastError());
+    exclusive_sum(ipb.as<uint32_t>(), item_start.as<uint32_t>(), n_bins + 1, st);
+    const uint32_t n_items = read_u32(item_start.as<uint32_t>() + n_bins, st);
+    vs.n_items = n_items;
+    vs.items.alloc(std::max<uint64_t>(n_items, 1) * sizeof(WorkItem));
+    hipLaunchKernelGGL(k_write_items, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
+                       bin_groups.as<uint32_t>(), first_group.as<uint32_t>(),
+                       item_start.as<uint32_t>(), n_bins, static_cast<uint32_t>(n_tiles),
+                       vs.items.as<WorkItem>());
+    CHIP_HIP(hipGetLastError());
+    CHIP_HIP(hipStreamSynchronize(st));
 }
 
 }  // namespace chip

@@ -117,6 +117,34 @@ __global__ void k_norm_table(const float2 *kernels, const int32_t *supports, uin
     table[i] = ok ? sum : 0.0f;
 }
 
+// Gather the taps of every (plane, off_x, off_y) into one contiguous run in
+// the order the tap-parallel kernels walk them: f = (ix+S)*(2S+1) + (iy+S),
+// value K[|ix*s + off_x|][|iy*s + off_y|]; zero beyond the plane's (2S+1)^2.
+__global__ void k_compact(const float2 *kernels, const int32_t *supports, uint32_t planes,
+                          uint32_t width, uint32_t sampling, int32_t R, uint32_t stride,
+                          float2 *out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint32_t n_off = static_cast<uint32_t>(2 * R + 1);
+    const uint64_t total = static_cast<uint64_t>(planes) * n_off * n_off * stride;
+    if (i >= total) return;
+    const uint32_t f = static_cast<uint32_t>(i % stride);
+    const uint64_t po = i / stride;
+    const uint32_t plane = static_cast<uint32_t>(po / (n_off * n_off));
+    const uint32_t off = static_cast<uint32_t>(po - static_cast<uint64_t>(plane) * n_off * n_off);
+    const int ox = static_cast<int>(off / n_off) - R, oy = static_cast<int>(off % n_off) - R;
+    const int S = supports[plane];
+    const int side = 2 * S + 1;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (static_cast<int>(f) < side * side) {
+        const int ix = static_cast<int>(f) / side - S, iy = static_cast<int>(f) % side - S;
+        const int s = static_cast<int>(sampling);
+        const int kx = abs(ix * s + ox), ky = abs(iy * s + oy);
+        if (kx < static_cast<int>(width) && ky < static_cast<int>(width))
+            v = kernels[static_cast<uint64_t>(plane) * width * width + static_cast<uint64_t>(kx) * width + ky];
+    }
+    out[i] = v;
+}
+
 static unsigned nb(uint64_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
 
 void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
@@ -250,6 +278,13 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
     CHIP_HIP(hipGetLastError());
     std::vector<float> norm(static_cast<uint64_t>(planes) * axis * axis);
     CHIP_HIP(hipMemcpyAsync(norm.data(), d_norm.p, d_norm.bytes, hipMemcpyDeviceToHost, st));
+
+    const uint32_t stride = (2 * max_support + 1) * (2 * max_support + 1);
+    const uint64_t n_compact = static_cast<uint64_t>(planes) * axis * axis * stride;
+    DevBuf compact(n_compact * sizeof(float2));
+    hipLaunchKernelGGL(k_compact, dim3(nb(n_compact, 256)), dim3(256), 0, st, kernels.as<float2>(),
+                       d_sup.as<int32_t>(), planes, width, s, R, stride, compact.as<float2>());
+    CHIP_HIP(hipGetLastError());
     CHIP_HIP(hipStreamSynchronize(st));
 
     g.wproject = true;
@@ -262,6 +297,8 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
     g.supports = sup32;
     g.norm_table = norm;
     g.d_kernels = std::move(kernels);
+    g.d_compact = std::move(compact);
+    g.tap_stride = stride;
     g.d_supports = std::move(d_sup);
     g.d_norm_table = std::move(d_norm);
     g.sinc = sinc_axis(std::max(g.gx, g.gy), s);

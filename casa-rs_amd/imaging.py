@@ -48,9 +48,16 @@ class Context:
         h = ctypes.c_void_p()
         abi.call("chip_ctx_create", device, ctypes.byref(h))
         self._h = h
+        self._bound = None
 
     @property
     def handle(self):
+        """The chip_ctx, bound to torch's current stream on this device so that
+        library kernels are ordered after torch's work on the same tensors."""
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        if s != self._bound:
+            abi.call("chip_ctx_set_stream", self._h, 1, ctypes.c_void_p(s))
+            self._bound = s
         return self._h
 
     def close(self):
@@ -169,11 +176,12 @@ class StandardGridder:
 
     def _refresh(self):
         info = abi.GridderInfo()
-        abi.call("chip_gridder_info_get", self._h, ctypes.byref(info))
+        abi.call("chip_gridder_info_get", self.handle,ctypes.byref(info))
         self.info = info
 
     @property
     def handle(self):
+        self.ctx.handle  # bind torch's current stream
         return self._h
 
     nx = property(lambda s: s.geometry.nx)
@@ -187,21 +195,21 @@ class StandardGridder:
         taps = np.empty((self.info.oversampling + 1, 7), np.float32)
         cx = np.empty(self.info.gx, np.float32)
         cy = np.empty(self.info.gy, np.float32)
-        abi.call("chip_gridder_tables", self._h, taps.ctypes.data_as(ctypes.c_void_p),
+        abi.call("chip_gridder_tables", self.handle,taps.ctypes.data_as(ctypes.c_void_p),
                  cx.ctypes.data_as(ctypes.c_void_p), cy.ctypes.data_as(ctypes.c_void_p))
         return taps, cx, cy
 
     def max_abs_w(self, batch) -> float:
         s, keep = batch.to_samples()
         out = ctypes.c_double()
-        abi.call("chip_max_abs_w", self._h, ctypes.byref(s), ctypes.byref(out))
+        abi.call("chip_max_abs_w", self.handle,ctypes.byref(s), ctypes.byref(out))
         return out.value
 
     def attach_wproject(self, max_abs_w: float, planes: int | None = None,
                         sampling: int | None = None):
         """WProjector::new (gridder.rs:3403-3586); planes=None selects the
         automatic count, sampling=None the reference rule."""
-        abi.call("chip_gridder_attach_wproject", self._h, float(max_abs_w), int(planes or 0),
+        abi.call("chip_gridder_attach_wproject", self.handle,float(max_abs_w), int(planes or 0),
                  int(sampling or 0))
         self._refresh()
         return self
@@ -213,7 +221,7 @@ class StandardGridder:
         ker = np.empty((i.planes, i.width, i.width), np.complex64)
         ax = 2 * i.norm_radius + 1
         nt = np.empty(i.planes * ax * ax, np.float32)
-        abi.call("chip_wproject_tables", self._h, sup.ctypes.data_as(ctypes.c_void_p),
+        abi.call("chip_wproject_tables", self.handle,sup.ctypes.data_as(ctypes.c_void_p),
                  ker.ctypes.data_as(ctypes.c_void_p), nt.ctypes.data_as(ctypes.c_void_p))
         return sup, ker, nt
 
@@ -223,7 +231,7 @@ class StandardGridder:
         n = s.n_rows * s.n_chan
         plan = torch.empty((n, 6), dtype=torch.int32, device=self.ctx.torch_device)
         norm = torch.empty(n, dtype=torch.float32, device=self.ctx.torch_device)
-        abi.call("chip_plan", self._h, ctypes.byref(s), _ptr(plan), _ptr(norm))
+        abi.call("chip_plan", self.handle,ctypes.byref(s), _ptr(plan), _ptr(norm))
         return plan, norm
 
     def new_grids(self, n_chan=1, precision=abi.F32):
@@ -238,14 +246,14 @@ class StandardGridder:
         """One launch per channel, global atomics (the plain path)."""
         s, keep = batch.to_samples()
         sums = (abi.Sums * s.n_chan)()
-        abi.call("chip_grid_plain", self._h, ctypes.byref(s), mode, precision, _ptr(model),
+        abi.call("chip_grid_plain", self.handle,ctypes.byref(s), mode, precision, _ptr(model),
                  _ptr(psf), _ptr(dirty), sums)
         return [x.as_dict() for x in sums]
 
     def degrid_plain(self, batch, model, precision=abi.F64):
         s, keep = batch.to_samples()
         out = torch.empty(s.n_rows * s.n_chan, dtype=torch.complex64, device=self.ctx.torch_device)
-        abi.call("chip_degrid_plain", self._h, ctypes.byref(s), precision, _ptr(model), _ptr(out))
+        abi.call("chip_degrid_plain", self.handle,ctypes.byref(s), precision, _ptr(model), _ptr(out))
         return out
 
     def finish_products(self, psf_grids, dirty_grids, normalization_sumwt, psf_peak=None,
@@ -260,7 +268,7 @@ class StandardGridder:
         dev = self.ctx.torch_device
         psf = torch.empty((batch, self.nx, self.ny), dtype=torch.float32, device=dev) if psf_grids is not None else None
         res = torch.empty((batch, self.nx, self.ny), dtype=torch.float32, device=dev) if dirty_grids is not None else None
-        abi.call("chip_finish_products", self._h, precision, batch, _ptr(psf_grids), _ptr(dirty_grids),
+        abi.call("chip_finish_products", self.handle,precision, batch, _ptr(psf_grids), _ptr(dirty_grids),
                  nsum.ctypes.data_as(ctypes.c_void_p), _ptr(psf), _ptr(res),
                  peaks.ctypes.data_as(ctypes.c_void_p))
         return psf, res, peaks
@@ -273,14 +281,14 @@ class StandardGridder:
         batch = model.shape[0]
         if out is None:
             out = self.new_grids(batch, precision)
-        abi.call("chip_model_to_grid", self._h, precision, batch, _ptr(model), _ptr(out))
+        abi.call("chip_model_to_grid", self.handle,precision, batch, _ptr(model), _ptr(out))
         return out
 
     def grid_to_model_adjoint(self, grid):
         """Adjoint of model_to_grid (complex64 grids are overwritten)."""
         batch = grid.shape[0]
         out = torch.empty((batch, self.nx, self.ny), dtype=torch.float32, device=self.ctx.torch_device)
-        abi.call("chip_grid_to_model_adjoint", self._h, batch, _ptr(grid), _ptr(out))
+        abi.call("chip_grid_to_model_adjoint", self.handle,batch, _ptr(grid), _ptr(out))
         return out
 
     def close(self):
@@ -310,27 +318,32 @@ class VisibilitySet:
         abi.call("chip_vset_sums", h, sums)
         self.sums = [x.as_dict() for x in sums]
 
+    @property
+    def handle(self):
+        self.gridder.handle  # bind torch's current stream
+        return self._h
+
     def stats(self):
         """n_valid, n_items, taps (sum of (2S+1)^2) and the last kernel's device ms."""
         nv, ni, tp = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         ms = ctypes.c_double()
-        abi.call("chip_vset_stats", self._h, ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(tp),
+        abi.call("chip_vset_stats", self.handle,ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(tp),
                  ctypes.byref(ms))
         return {"n_valid": nv.value, "n_items": ni.value, "taps": tp.value,
                 "last_kernel_ms": ms.value}
 
     def grid(self, mode, model=None, psf=None, dirty=None):
-        abi.call("chip_vset_grid", self._h, mode, _ptr(model), _ptr(psf), _ptr(dirty))
+        abi.call("chip_vset_grid", self.handle,mode, _ptr(model), _ptr(psf), _ptr(dirty))
 
     def degrid(self, model, out=None):
         if out is None:
             out = torch.empty(self.n_samples, dtype=torch.complex64, device=model.device)
-        abi.call("chip_vset_degrid", self._h, _ptr(model), _ptr(out))
+        abi.call("chip_vset_degrid", self.handle,_ptr(model), _ptr(out))
         return out
 
     def grid_values(self, values, grid):
         values = values.to(torch.complex64).contiguous()
-        abi.call("chip_vset_grid_values", self._h, _ptr(values), _ptr(grid))
+        abi.call("chip_vset_grid_values", self.handle,_ptr(values), _ptr(grid))
         return grid
 
     def close(self):

@@ -115,6 +115,14 @@ typedef struct {
 /* ---- context ------------------------------------------------------------ */
 int chip_ctx_create(int device, chip_ctx **out, char **err);
 void chip_ctx_destroy(chip_ctx *ctx);
+/* external != 0: launch subsequent work on `stream` (a hipStream_t of the
+ * context's device, NULL meaning the legacy default stream), e.g. the caller's
+ * current stream, so that the library's kernels are ordered after the
+ * caller's own work on the same buffers.  external == 0 restores the
+ * context's own stream.  Every call still returns only after its work on that
+ * stream has completed.  No reference counterpart (the Metal path owns its
+ * command queue, lib.rs:72270-72290). */
+int chip_ctx_set_stream(chip_ctx *ctx, int external, void *stream, char **err);
 void chip_error_free(char *err);
 const char *chip_version(void);
 

@@ -36,6 +36,29 @@ def test_error_path_without_device(pkg):
         pkg.Context(0)
 
 
+def test_context_binds_the_callers_stream_once(pkg, monkeypatch):
+    """Library work is launched on torch's current stream (bound lazily, once
+    per stream change), so kernels are ordered after torch's work on the same
+    tensors."""
+    import importlib
+
+    im = importlib.import_module("casa_rs_amd.imaging")
+    calls = []
+    streams = iter([11, 11, 22])
+
+    class S:
+        def __init__(self, v):
+            self.cuda_stream = v
+
+    monkeypatch.setattr(im.torch.cuda, "current_stream", lambda d=None: S(next(streams)))
+    monkeypatch.setattr(im.abi, "call", lambda name, *a: calls.append((name, a[1], a[2].value)))
+    c = im.Context.__new__(im.Context)
+    c.device, c._h, c._bound = 0, "H", None
+    assert [c.handle, c.handle, c.handle] == ["H", "H", "H"]
+    assert calls == [("chip_ctx_set_stream", 1, 11), ("chip_ctx_set_stream", 1, 22)]
+    c._h = None
+
+
 def test_invalid_geometry_is_rejected_before_device_work(pkg):
     """A NULL context is an InvalidRequest, not a crash."""
     L = pkg.abi.load()
