@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every GPU box here has one, and the sharded path "
@@ -215,6 +217,16 @@ def main():
     pkg = __graft_entry__.load_package()
     dev = torch.device("cuda", 0)
     wl = WORKLOADS[args.workload]
+    if args.calibrate_copy:
+        # 1 GiB read + 1 GiB written by torch's copy kernel: the scale for
+        # FETCH_SIZE / WRITE_SIZE readings of the gridding kernels.
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 3 x 1 GiB device copies done")
     t0 = time.perf_counter()
     data = make_cube(torch, wl, args.seed, dev)
     torch.cuda.synchronize()
@@ -236,8 +248,8 @@ def main():
     torch.cuda.synchronize()
     plan_s = time.perf_counter() - t0
     st = vs.stats()
-    log(f"plan+bin: {st['n_valid']} accepted of {n_unflagged} unflagged, {st['n_items']} work items, "
-        f"{st['taps']} taps ({plan_s * 1e3:.1f} ms)")
+    log(f"plan+bin: {st['n_valid']} accepted of {n_unflagged} unflagged, {st['n_groups']} groups of "
+        f"identical plans, {st['n_items']} work items, {st['taps']} taps ({plan_s * 1e3:.1f} ms)")
     nsum = np.array([s["normalization_sumwt"] for s in vs.sums])
 
     # initial dirty images: batched FP32 vs plain FP64

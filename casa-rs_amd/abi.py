@@ -150,7 +150,7 @@ def load():
         "chip_vset_create": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _PP, _PP]),
         "chip_vset_destroy": (None, [_P]),
         "chip_vset_sums": (ctypes.c_int, [_P, _P, _PP]),
-        "chip_vset_stats": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
+        "chip_vset_stats": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _PP]),
         "chip_vset_grid": (ctypes.c_int, [_P, _U32, _P, _P, _P, _PP]),
         "chip_vset_degrid": (ctypes.c_int, [_P, _P, _P, _PP]),
         "chip_vset_grid_values": (ctypes.c_int, [_P, _P, _P, _PP]),

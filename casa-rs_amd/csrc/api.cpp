@@ -236,11 +236,12 @@ int chip_vset_sums(const chip_vset *vs, chip_sums *sums, char **err) {
     });
 }
 
-int chip_vset_stats(chip_vset *vs, uint64_t *n_valid, uint64_t *n_items, uint64_t *taps,
-                    double *last_kernel_ms, char **err) {
+int chip_vset_stats(chip_vset *vs, uint64_t *n_valid, uint64_t *n_groups, uint64_t *n_items,
+                    uint64_t *taps, double *last_kernel_ms, char **err) {
     return guarded(err, [&] {
         if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
         if (n_valid) *n_valid = vs->n_valid;
+        if (n_groups) *n_groups = vs->n_groups;
         if (n_items) *n_items = vs->n_items;
         if (taps) *taps = vs->taps;
         if (last_kernel_ms) {

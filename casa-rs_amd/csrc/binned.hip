@@ -1,25 +1,29 @@
 // Binned FP32 gridding and degridding over the resident visibility stream.
 //
-// One workgroup per work item: a run of <= 2048 samples of one channel whose
-// anchor cells fall in one 32x32 uv tile.  The workgroup owns an LDS tile of
-// (32 + 2*halo)^2 cells (halo = largest half-support), accumulates into it
-// with LDS float atomics, and finally adds the non-zero LDS cells to the
-// global grid with global float atomics (few per tile).  Degridding reads a
-// model tile staged in LDS.
+// One workgroup per work item: a run of groups (samples with identical
+// plans) of one channel whose anchor cells fall in one 32x32 uv tile.  The
+// workgroup owns an LDS tile of (32 + 2*halo)^2 cells (halo = largest
+// half-support), accumulates into it, and finally adds the non-zero LDS cells
+// to the global grid with global float atomics (few per tile).  Degridding
+// reads a model tile staged in LDS.
 //
-// Work is tap-parallel: one wavefront handles one sample at a time and its 64
-// lanes own distinct taps.  Lanes of one atomic instruction therefore always
-// hit distinct cells (consecutive samples of a bin are the same baseline a few
-// seconds apart and share cells, which made a lane-per-sample mapping
-// serialise on LDS atomics), taps of consecutive lanes are consecutive cells
-// in a row (bank friendly), and a sample's W taps are one contiguous
-// pre-gathered run (coalesced loads).  The degridded value is reduced across
-// the wavefront with shuffles.  MFMA is not used: gridding is a scatter, not a
-// dense contraction.
+// Work is tap-parallel: one wavefront handles one group at a time and its 64
+// lanes own distinct taps, so the lanes of one LDS atomic always hit distinct
+// cells, consecutive lanes hit consecutive cells of a row, and a group's W
+// taps are one contiguous pre-gathered run (coalesced loads).  Each wavefront
+// walks a contiguous run of groups; groups are sorted by anchor cell, so
+// consecutive groups mostly share their footprint and the lanes keep the
+// footprint's accumulators (and model values) in registers, touching LDS only
+// when the anchor moves.  The degridded value is reduced across the wavefront
+// with DPP lane swaps.  MFMA is not used: gridding is a scatter, not a dense
+// contraction.
 //
 // Arithmetic per tap follows the reference (gridder.rs:1036-1050 standard,
 // :3680-3747 W): tap = wx*wy (f32) or cwt = K[|ix*s+ox|][|iy*s+oy|] conjugated
-// when w > 0; grid += value*tap; degrid += conj(cwt)*grid.
+// when w > 0; grid += value*tap; degrid += conj(cwt)*grid.  A group's value is
+// the sum over its samples (sum w_i V_i; residual: sum w_i V_i - p sum w_i).
+#include <climits>
+
 #include "device.hpp"
 
 namespace chip {
@@ -59,6 +63,8 @@ static size_t binned_lds_bytes(const GridParams &gp) {
     return (tap_floats + static_cast<size_t>(LdsArrays<MODE>::n) * gp.L * gp.Ls) * sizeof(float);
 }
 
+constexpr int kSlots = 2;  // taps per lane kept in registers (footprints up to 128 taps)
+
 template <int MODE, bool WP>
 __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
                                                 const uint32_t *__restrict__ gstart,
@@ -87,6 +93,9 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
     if (MODE == kModeResidual) { M_re = a0; M_im = a1; A_re = a2; A_im = a3; }
     if (MODE == kModeAdjoint) { A_re = a0; A_im = a1; }
     if (MODE == kModeDegrid) { M_re = a0; M_im = a1; }
+    constexpr bool kModel = MODE == kModeResidual || MODE == kModeDegrid;
+    constexpr bool kAcc = MODE == kModeResidual || MODE == kModeDirty || MODE == kModeAdjoint;
+    constexpr bool kPsf = MODE == kModePsf || MODE == kModeDirty;
 
     const int tid = threadIdx.x;
     const int nth = blockDim.x;
@@ -101,9 +110,6 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
 
     if (!WP)
         for (int i = tid; i < static_cast<int>(gp.tap_rows) * kTaps; i += nth) taps[i] = gp.taps[i];
-    constexpr bool kModel = MODE == kModeResidual || MODE == kModeDegrid;
-    constexpr bool kAcc = MODE == kModeResidual || MODE == kModeDirty || MODE == kModeAdjoint;
-    constexpr bool kPsf = MODE == kModePsf || MODE == kModeDirty;
     for (int c = tid; c < cells; c += nth) {
         if (kModel) {
             const int lx = c / Ls, ly = c - lx * Ls;
@@ -120,23 +126,52 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
     }
     __syncthreads();
 
+    // Footprint state of this wavefront (uniform): anchor of the first tap
+    // and footprint side; per lane, for slots f = lane + 64*s: tap row/column,
+    // LDS cell, register accumulators and staged model values.
+    int cur_bx = INT_MIN, cur_by = 0, cur_side = 0, cur_T = 0;
+    int tx_s[kSlots], ty_s[kSlots], cell_s[kSlots];
+    float accr[kSlots], acci[kSlots], pacr[kSlots], paci[kSlots], mre[kSlots], mim[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; ++s) {
+        tx_s[s] = ty_s[s] = 0;
+        cell_s[s] = 0;
+        accr[s] = acci[s] = pacr[s] = paci[s] = mre[s] = mim[s] = 0.0f;
+    }
+    auto flush_regs = [&]() {
+        if (cur_side == 0) return;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (lane + 64 * s < cur_T) {
+                if (kPsf) {
+                    if (pacr[s] != 0.0f) atomicAdd(P_re + cell_s[s], pacr[s]);
+                    if (WP && paci[s] != 0.0f) atomicAdd(P_im + cell_s[s], paci[s]);
+                }
+                if (kAcc) {
+                    if (accr[s] != 0.0f) atomicAdd(A_re + cell_s[s], accr[s]);
+                    if (acci[s] != 0.0f) atomicAdd(A_im + cell_s[s], acci[s]);
+                }
+            }
+            accr[s] = acci[s] = pacr[s] = paci[s] = 0.0f;
+        }
+    };
+
     const int R = gp.norm_radius;
-    for (uint32_t k = wave; k < it.count; k += nwaves) {
-        // One group of samples with identical plans (wavefront-uniform).
-        const uint32_t grp = it.start + k;
+    const uint32_t per = (it.count + nwaves - 1) / nwaves;
+    const uint32_t g_begin = it.start + static_cast<uint32_t>(wave) * per;
+    const uint32_t g_end = min(it.start + it.count, g_begin + per);
+    for (uint32_t grp = g_begin; grp < g_end; ++grp) {
         const uint32_t s0 = gstart[grp], s1 = gstart[grp + 1];
         const uint2 pk = plan[s0];
         const int ax = static_cast<int>(pk.x & 0xffffu) - x0;
         const int ay = static_cast<int>(pk.x >> 16) - y0;
         const uint32_t flags = pk.y;
-        // Group sums, lane-parallel over the samples, then across the wave:
+        // Group sums, lane-parallel over the samples:
         //   PSF/DIRTY: w = sum w_i (all samples), v = sum w_i V_i (finite V)
         //   RESIDUAL:  w = sum w_i, v = sum w_i V_i over usable samples
         //   ADJOINT:   v = sum values_i
-        float2 v = make_float2(0.0f, 0.0f);
-        float w = 0.0f;
+        float vr = 0.0f, vi = 0.0f, w = 0.0f;
         if (MODE != kModeDegrid) {
-            float vr = 0.0f, vi = 0.0f;
             for (uint32_t j = s0 + lane; j < s1; j += 64) {
                 if (MODE == kModeAdjoint) {
                     const float2 a = values[index[j]];
@@ -166,55 +201,121 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
                 vi = wave_sum_f(vi);
                 if (MODE != kModeAdjoint) w = wave_sum_f(w);
             } else {
-                vr = __shfl(vr, 0, 64);
-                vi = __shfl(vi, 0, 64);
-                w = __shfl(w, 0, 64);
+                vr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vr)));
+                vi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vi)));
+                w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(w)));
             }
-            v = make_float2(vr, vi);
             if (MODE == kModeResidual && w == 0.0f) continue;
         }
-        // Per-lane tap geometry.
-        int side, T;
+        // Footprint geometry (uniform).
+        int side;
         const float2 *ktap = nullptr;
         const float *wxr = nullptr, *wyr = nullptr;
         float csign = 1.0f;
         if (!WP) {
             side = kTaps;
-            T = kTaps * kTaps;
             wxr = taps + (pk.y & 0xffu) * kTaps;
             wyr = taps + ((pk.y >> 8) & 0xffu) * kTaps;
         } else {
             const int ox = static_cast<int>(pk.y & 0xffu) - 128;
             const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
             const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
-            const int S = gp.supports[plane];
-            side = 2 * S + 1;
-            T = side * side;
+            side = 2 * gp.supports[plane] + 1;
             ktap = gp.compact +
                    (static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
                     static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R))) *
                        gp.tap_stride;
             csign = (flags & kConjBit) ? -1.0f : 1.0f;
-            // anchor -> first tap
         }
+        const int T = side * side;
         const int bx = WP ? ax - (side >> 1) : ax;  // standard anchors are already x_start
         const int by = WP ? ay - (side >> 1) : ay;
-        float pr = 0.0f, pi = 0.0f;
-        if (kModel) {
+
+        if (T > 64 * kSlots) {
+            // Large footprint: no register caching, taps go straight to LDS.
+            flush_regs();
+            cur_side = 0;
+            float pr = 0.0f, pi = 0.0f;
+            if (kModel) {
+                for (int f = lane; f < T; f += 64) {
+                    const int tx = f / side, ty = f - tx * side;
+                    const int cell = (bx + tx) * Ls + (by + ty);
+                    const float2 c = WP ? ktap[f] : make_float2(wxr[tx] * wyr[ty], 0.0f);
+                    const float cr = c.x, ci = -(c.y * csign);
+                    pr += cr * M_re[cell] - ci * M_im[cell];
+                    pi += cr * M_im[cell] + ci * M_re[cell];
+                }
+                pr = wave_sum_f(pr);
+                pi = wave_sum_f(pi);
+                if (MODE == kModeDegrid) {
+                    for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
+                    continue;
+                }
+            }
+            if (MODE == kModeResidual) { vr = vr - pr * w; vi = vi - pi * w; }
             for (int f = lane; f < T; f += 64) {
                 const int tx = f / side, ty = f - tx * side;
                 const int cell = (bx + tx) * Ls + (by + ty);
-                const float gr = M_re[cell], gi = M_im[cell];
-                if (!WP) {
-                    const float t = wxr[tx] * wyr[ty];
-                    pr += gr * t;
-                    pi += gi * t;
-                } else {
-                    const float2 c = ktap[f];
-                    const float cr = c.x, ci = -(c.y * csign);  // conj(cwt)
-                    pr += cr * gr - ci * gi;
-                    pi += cr * gi + ci * gr;
+                const float2 c = WP ? ktap[f] : make_float2(wxr[tx] * wyr[ty], 0.0f);
+                const float cr = c.x, ci = c.y * csign;
+                if (kPsf) {
+                    atomicAdd(P_re + cell, w * cr);
+                    if (WP) atomicAdd(P_im + cell, w * ci);
                 }
+                if (kAcc) {
+                    atomicAdd(A_re + cell, vr * cr - vi * ci);
+                    atomicAdd(A_im + cell, vr * ci + vi * cr);
+                }
+            }
+            continue;
+        }
+
+        if (bx != cur_bx || by != cur_by || side != cur_side) {
+            // The footprint moved: spill the register accumulators to LDS and
+            // set up the new footprint (and its model values).
+            flush_regs();
+            cur_bx = bx;
+            cur_by = by;
+            cur_side = side;
+            cur_T = T;
+            const float inv = 1.0f / static_cast<float>(side);
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                const int f = lane + 64 * s;
+                const int tx = static_cast<int>((static_cast<float>(f) + 0.5f) * inv);
+                tx_s[s] = tx;
+                ty_s[s] = f - tx * side;
+                cell_s[s] = (bx + tx) * Ls + (by + ty_s[s]);
+                if (kModel) {
+                    const bool on = f < T;
+                    mre[s] = on ? M_re[cell_s[s]] : 0.0f;
+                    mim[s] = on ? M_im[cell_s[s]] : 0.0f;
+                }
+            }
+        }
+        // This group's taps.
+        float cr[kSlots], ci[kSlots];
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            const int f = lane + 64 * s;
+            cr[s] = ci[s] = 0.0f;
+            if (f < T) {
+                if (WP) {
+                    const float2 c = ktap[f];
+                    cr[s] = c.x;
+                    ci[s] = c.y * csign;
+                } else {
+                    cr[s] = wxr[tx_s[s]] * wyr[ty_s[s]];
+                }
+            }
+        }
+        if (kModel) {
+            float pr = 0.0f, pi = 0.0f;
+#pragma unroll
+            for (int s = 0; s < kSlots; ++s) {
+                // conj(cwt) * model
+                pr += cr[s] * mre[s] + ci[s] * mim[s];
+                pi += cr[s] * mim[s] - ci[s] * mre[s];
             }
             pr = wave_sum_f(pr);
             pi = wave_sum_f(pi);
@@ -223,37 +324,23 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
                 for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
                 continue;
             }
+            // sum_i (V_i - p) w_i = sum_i w_i V_i - p sum_i w_i
+            vr = vr - pr * w;
+            vi = vi - pi * w;
         }
-        // sum_i (V_i - p) w_i = sum_i w_i V_i - p sum_i w_i
-        float vr, vi;
-        if (MODE == kModeResidual) { vr = v.x - pr * w; vi = v.y - pi * w; }
-        else { vr = v.x; vi = v.y; }
-        const bool acc_on = MODE == kModeResidual || MODE == kModeAdjoint ||
-                            (MODE == kModeDirty && (vr != 0.0f || vi != 0.0f));
-        for (int f = lane; f < T; f += 64) {
-            const int tx = f / side, ty = f - tx * side;
-            const int cell = (bx + tx) * Ls + (by + ty);
-            if (!WP) {
-                const float t = wxr[tx] * wyr[ty];
-                if (kPsf) atomicAdd(P_re + cell, w * t);
-                if (acc_on) {
-                    atomicAdd(A_re + cell, vr * t);
-                    atomicAdd(A_im + cell, vi * t);
-                }
-            } else {
-                const float2 c = ktap[f];
-                const float cr = c.x, ci = c.y * csign;
-                if (kPsf) {
-                    atomicAdd(P_re + cell, w * cr);
-                    atomicAdd(P_im + cell, w * ci);
-                }
-                if (acc_on) {
-                    atomicAdd(A_re + cell, vr * cr - vi * ci);
-                    atomicAdd(A_im + cell, vr * ci + vi * cr);
-                }
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s) {
+            if (kPsf) {
+                pacr[s] += w * cr[s];
+                paci[s] += w * ci[s];
+            }
+            if (kAcc) {
+                accr[s] += vr * cr[s] - vi * ci[s];
+                acci[s] += vr * ci[s] + vi * cr[s];
             }
         }
     }
+    flush_regs();
     if (MODE == kModeDegrid) return;
     __syncthreads();
 
@@ -290,9 +377,8 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
     CHIP_HIP(hipEventRecord(vs.ev_start, st));
     hipLaunchKernelGGL((k_binned<MODE, WP>), dim3(static_cast<unsigned>(vs.n_items)), dim3(256),
                        lds, st, gp, vs.items.as<WorkItem>(), vs.gstart.as<uint32_t>(),
-                       vs.plan.as<uint2>(),
-                       vs.vis.as<float2>(), vs.wt.as<float>(), vs.index.as<uint32_t>(), values,
-                       model, psf, acc, vis_out);
+                       vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
+                       vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
     CHIP_HIP(hipGetLastError());
     CHIP_HIP(hipEventRecord(vs.ev_stop, st));
     vs.timed = true;

@@ -211,10 +211,22 @@ struct GridParams {
 
 GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint32_t tiles_y);
 
-// Wavefront-wide sum of a float (64 lanes); every lane receives the sum.
+// Wavefront-wide sum of a float (64 lanes, all active); every lane receives
+// the same value.  DPP lane swaps inside each row of 16 (xor 1, xor 2,
+// half-row mirror, row mirror) leave the row sum in every lane of the row;
+// the four row sums are then read into scalars and added.
+#define CHIP_DPP_ADD(v, ctrl) \
+    ((v) + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), (ctrl), 0xF, 0xF, false)))
 __device__ __forceinline__ float wave_sum_f(float v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v = CHIP_DPP_ADD(v, 0xB1);   // quad_perm [1,0,3,2]
+    v = CHIP_DPP_ADD(v, 0x4E);   // quad_perm [2,3,0,1]
+    v = CHIP_DPP_ADD(v, 0x141);  // row_half_mirror
+    v = CHIP_DPP_ADD(v, 0x140);  // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return (r0 + r1) + (r2 + r3);
 }
 
 // Wavefront-wide sum of a double (64 lanes).

@@ -325,11 +325,11 @@ class VisibilitySet:
 
     def stats(self):
         """n_valid, n_items, taps (sum of (2S+1)^2) and the last kernel's device ms."""
-        nv, ni, tp = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        nv, ng, ni, tp = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         ms = ctypes.c_double()
-        abi.call("chip_vset_stats", self.handle,ctypes.byref(nv), ctypes.byref(ni), ctypes.byref(tp),
-                 ctypes.byref(ms))
-        return {"n_valid": nv.value, "n_items": ni.value, "taps": tp.value,
+        abi.call("chip_vset_stats", self.handle, ctypes.byref(nv), ctypes.byref(ng), ctypes.byref(ni),
+                 ctypes.byref(tp), ctypes.byref(ms))
+        return {"n_valid": nv.value, "n_groups": ng.value, "n_items": ni.value, "taps": tp.value,
                 "last_kernel_ms": ms.value}
 
     def grid(self, mode, model=None, psf=None, dirty=None):

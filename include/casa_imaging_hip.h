@@ -171,14 +171,15 @@ int chip_vset_create(chip_gridder *g, const chip_samples *s, chip_vset **out, ch
 void chip_vset_destroy(chip_vset *vs);
 /* sums_host: n_chan entries (may be NULL). */
 int chip_vset_sums(const chip_vset *vs, chip_sums *sums_host, char **err);
-/* Stream statistics: accepted samples, work items, total taps sum (2S+1)^2
- * over accepted samples (the exact grid-point updates of one gridding pass),
- * and the device duration in ms of the last binned kernel launch (measured
- * with events on the library's stream; -1 before the first launch).  Any
- * pointer may be NULL.  No reference counterpart: the reference reports
- * stage timings through ImagingStageTimings (lib.rs:3496-3503). */
-int chip_vset_stats(chip_vset *vs, uint64_t *n_valid, uint64_t *n_items, uint64_t *taps,
-                    double *last_kernel_ms, char **err);
+/* Stream statistics: accepted samples, groups of identical plans, work items,
+ * total taps sum (2S+1)^2 over accepted samples (the exact grid-point updates
+ * of one gridding pass), and the device duration in ms of the last binned
+ * kernel launch (measured with events on the launch stream; -1 before the
+ * first launch).  Any pointer may be NULL.  No reference counterpart: the
+ * reference reports stage timings through ImagingStageTimings
+ * (lib.rs:3496-3503). */
+int chip_vset_stats(chip_vset *vs, uint64_t *n_valid, uint64_t *n_groups, uint64_t *n_items,
+                    uint64_t *taps, double *last_kernel_ms, char **err);
 /* One launch over every (channel, tile) work item.  Grids are CHIP_F32
  * complex64, n_chan grids each; they are accumulated into, not cleared.
  * mode PSF/DIRTY use psf_dev (and dirty_dev); RESIDUAL reads model_dev
