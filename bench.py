@@ -192,6 +192,20 @@ def cpu_baseline(pkg, g, data, cpu_seconds, log):
                       f"FFTs excluded"}
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of k_binned<RESIDUAL, W> from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes, each scaled by the same counter's reading
+    for a device copy of known size (profiles/k_binned_residual_<workload>_traffic.json),
+    or None when no such measurement is committed."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"k_binned_residual_{workload}_traffic.json")
+    try:
+        with open(path) as f:
+            return float(json.load(f)["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -279,33 +293,34 @@ def main():
     torch.cuda.empty_cache()
 
     model = data["model"]
-    mgrid = g.new_grids(n_chan)
-    res = g.new_grids(n_chan)
-
     kernel_ms = []
+    r32 = torch.empty((n_chan, nx, nx), dtype=torch.float32, device=dev)
 
     def step32():
-        g.model_to_grid(model, pkg.F32, out=mgrid)
-        res.zero_()
-        vs.grid(pkg.MODE_RESIDUAL, model=mgrid, dirty=res)
+        # model -> apodise + FFT -> degrid + grid (V - p) w -> inverse FFT -> residual image
+        vs.major_cycle(model, nsum, peak32, out=r32)
         kernel_ms.append(vs.stats()["last_kernel_ms"])
-        return g.finish_products(None, res, nsum, psf_peak=peak32)[1]
+        return r32
 
     for _ in range(args.warmup):
         step32()
+    info = vs.cycle_info()
+    log(f"major-cycle FFT layout: {'interleaved [x][chan][y] 1-D passes' if info['interleaved'] else '[chan][x][y] 2-D plans'} "
+        f"(timed fwd+inv {info['fft_ms_1d']:.2f} ms 1-D vs {info['fft_ms_2d']:.2f} ms 2-D)")
     kernel_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
-        r32 = step32()
+        step32()
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / args.steps
     rate32 = n_unflagged / t32 / 1e6
     log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
         f"binned kernel {np.mean(kernel_ms):.3f} ms")
+    result_extra.update(fft_layout="interleaved" if info["interleaved"] else "2d",
+                        fft_fwd_inv_ms_1d=info["fft_ms_1d"], fft_fwd_inv_ms_2d=info["fft_ms_2d"])
 
     if not args.no_fp64:
-        del mgrid, res
         torch.cuda.empty_cache()
         mgrid64 = g.new_grids(n_chan, pkg.F64)
         res64 = g.new_grids(n_chan, pkg.F64)
@@ -333,7 +348,11 @@ def main():
         del mgrid64, res64
         torch.cuda.empty_cache()
 
-    # roofline of the dominant kernel (binned RESIDUAL: degrid + grid)
+    # Roofline of the dominant kernel, k_binned<RESIDUAL, W> (degrid + grid of
+    # one major cycle).  Algorithmic bytes per launch: 13 B per unflagged
+    # visibility (vis 8 + weight 4 + flag 1) + 24 B per row (uvw f64) + the
+    # model grid read once and the residual grid written once (8 B per cell
+    # each).  Algorithmic flops: 16 per tap (complex degrid MAC 8 + grid MAC 8).
     kms = float(np.mean(kernel_ms))
     n_valid = st["n_valid"]
     cells = n_chan * g.info.gx * g.info.gy
@@ -346,11 +365,14 @@ def main():
     if hbm_time >= flop_time:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     else:
-        roof = {"bound": "valu_fp32", "achieved": tfl, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": tfl / FP32_PEAK_TFLOPS}
-    roof["traffic"] = None
+        # compute-bound: priced at the FP32 peak (157.3 TFLOP/s); the kernel
+        # runs on the vector ALUs, whose FP32 peak is the same figure
+        roof = {"bound": "mfma", "engine": "valu_fp32", "achieved": tfl, "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": tfl / FP32_PEAK_TFLOPS}
+    roof["traffic"] = measured_traffic(args.workload)
     roof.update(kernel="k_binned<RESIDUAL, W>", kernel_ms=kms, algorithmic_bytes=bytes_alg,
-                algorithmic_flops=flops_alg, fp32_tflops=tfl, hbm_gbs=gbs)
+                algorithmic_flops=flops_alg, fp32_tflops=tfl, hbm_gbs=gbs,
+                hbm_frac=gbs / HBM_PEAK_GBS, fp32_frac=tfl / FP32_PEAK_TFLOPS)
 
     cpu = None
     if not args.no_cpu:

@@ -35,7 +35,9 @@ EXPORTS = (
     "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
     "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
-    "chip_vset_degrid", "chip_vset_grid_values", "chip_grid_plain", "chip_degrid_plain",
+    "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
+    "chip_vset_cycle_layout",
+    "chip_grid_plain", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
 )
 
@@ -154,6 +156,9 @@ def load():
         "chip_vset_grid": (ctypes.c_int, [_P, _U32, _P, _P, _P, _PP]),
         "chip_vset_degrid": (ctypes.c_int, [_P, _P, _P, _PP]),
         "chip_vset_grid_values": (ctypes.c_int, [_P, _P, _P, _PP]),
+        "chip_vset_major_cycle": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
+        "chip_vset_cycle_info": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
+        "chip_vset_cycle_layout": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
         "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
         "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),

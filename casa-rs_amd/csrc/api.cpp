@@ -10,6 +10,10 @@ void vset_build(chip_vset &vs, const chip_samples &cs);
 void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, float *dirty);
 void vset_degrid(chip_vset &vs, const float *model, float *vis_out);
 void vset_grid_values(chip_vset &vs, const float *values, float *grid);
+void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, const float *peaks,
+                      float *out);
+void vset_cycle_info(const chip_vset &vs, int *interleaved, double *ms_2d, double *ms_1d);
+void vset_cycle_layout(chip_vset &vs, int layout);
 }  // namespace chip
 
 using namespace chip;
@@ -282,6 +286,31 @@ int chip_vset_grid_values(chip_vset *vs, const float *values_dev, float *grid_de
         set_device(vs->g->ctx);
         vset_grid_values(*vs, values_dev, grid_dev);
         CHIP_HIP(hipStreamSynchronize(vs->g->ctx->stream));
+    });
+}
+
+int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *nsum,
+                          const float *psf_peak, float *residual_out_dev, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        set_device(vs->g->ctx);
+        vset_major_cycle(*vs, model_dev, nsum, psf_peak, residual_out_dev);
+    });
+}
+
+int chip_vset_cycle_layout(chip_vset *vs, int layout, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
+        set_device(vs->g->ctx);
+        vset_cycle_layout(*vs, layout);
+    });
+}
+
+int chip_vset_cycle_info(const chip_vset *vs, int *interleaved, double *fft_ms_2d,
+                         double *fft_ms_1d, char **err) {
+    return guarded(err, [&] {
+        if (!vs || !interleaved || !fft_ms_2d || !fft_ms_1d) fail(CHIP_ERR_INVALID, "NULL argument");
+        vset_cycle_info(*vs, interleaved, fft_ms_2d, fft_ms_1d);
     });
 }
 

@@ -44,6 +44,10 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.norm_radius = g.norm_radius;
     gp.tap_stride = g.tap_stride;
     gp.cells = g.gx * g.gy;
+    gp.cstride = gp.cells;
+    gp.xstride = g.gy;
+    gp.model_signed = 0;
+    gp.sign_base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
     gp.taps = g.d_taps.as<float>();
     gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
     gp.compact = g.wproject ? g.d_compact.as<float2>() : nullptr;
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
     const uint32_t tile_y = it.tile - tile_x * gp.tiles_y;
     const int x0 = static_cast<int>(tile_x * gp.tile) - static_cast<int>(gp.halo);
     const int y0 = static_cast<int>(tile_y * gp.tile) - static_cast<int>(gp.halo);
-    const uint64_t chan_off = static_cast<uint64_t>(it.chan) * gp.cells;
+    const uint64_t chan_off = static_cast<uint64_t>(it.chan) * gp.cstride;
 
     if (!WP)
         for (int i = tid; i < static_cast<int>(gp.tap_rows) * kTaps; i += nth) taps[i] = gp.taps[i];
@@ -116,8 +120,13 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
             const int gxx = x0 + lx, gyy = y0 + ly;
             float2 m = make_float2(0.0f, 0.0f);
             if (model && ly < L && gxx >= 0 && gyy >= 0 && gxx < static_cast<int>(gp.gx) &&
-                gyy < static_cast<int>(gp.gy))
-                m = model[chan_off + static_cast<uint64_t>(gxx) * gp.gy + gyy];
+                gyy < static_cast<int>(gp.gy)) {
+                m = model[chan_off + static_cast<uint64_t>(gxx) * gp.xstride + gyy];
+                if (gp.model_signed && ((gxx + gyy + gp.sign_base) & 1)) {
+                    m.x = -m.x;
+                    m.y = -m.y;
+                }
+            }
             M_re[c] = m.x;
             M_im[c] = m.y;
         }
@@ -350,7 +359,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
         const int gxx = x0 + lx, gyy = y0 + ly;
         if (ly >= L || gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy))
             continue;
-        const uint64_t gi = 2 * (chan_off + static_cast<uint64_t>(gxx) * gp.gy + gyy);
+        const uint64_t gi = 2 * (chan_off + static_cast<uint64_t>(gxx) * gp.xstride + gyy);
         if (kPsf) {
             const float r = P_re[c], i = P_im[c];
             if (r != 0.0f) unsafeAtomicAdd(psf + gi, r);
@@ -414,6 +423,15 @@ void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, flo
         default:
             fail(CHIP_ERR_INVALID, "unknown grid mode");
     }
+}
+
+// Residual pass of the major cycle over caller-laid-out grids (see GridParams
+// cstride / xstride / model_signed).
+void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res) {
+    if (vs.g->wproject)
+        launch_binned_t<kModeResidual, true>(vs, gp, nullptr, model, nullptr, res, nullptr, vs.g->ctx->stream);
+    else
+        launch_binned_t<kModeResidual, false>(vs, gp, nullptr, model, nullptr, res, nullptr, vs.g->ctx->stream);
 }
 
 void vset_degrid(chip_vset &vs, const float *model, float *vis_out) {

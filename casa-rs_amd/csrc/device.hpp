@@ -203,6 +203,15 @@ struct GridParams {
     int32_t norm_radius;     // R
     uint32_t tap_stride;     // W compact taps per (plane, offset): (2*max_support+1)^2
     uint64_t cells;          // gx * gy
+    // Grid addressing: cell (chan, x, y) is at chan*cstride + x*xstride + y.
+    // The public layout is cstride = gx*gy, xstride = gy; the major cycle may
+    // interleave channels inside x (cstride = gy, xstride = n_chan*gy).
+    uint64_t cstride;
+    uint64_t xstride;
+    // Major cycle: the model grid is the raw forward transform of the
+    // checkerboarded model, so (-1)^(x + y + sign_base) is applied on load.
+    uint32_t model_signed;
+    uint32_t sign_base;      // gx/2 + gy/2
     const float *taps;       // (os+1) x 7
     const float2 *kernels;   // planes x width x width
     const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest

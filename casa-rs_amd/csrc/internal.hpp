@@ -182,6 +182,9 @@ void model_to_grid(chip_gridder &g, uint32_t precision, uint64_t batch, const fl
                    void *grid_out);
 void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *image_out);
 
+// Major-cycle workspace (cycle.hip): grids and FFT plans kept with the set.
+struct CycleWorkspace;
+
 }  // namespace chip
 
 // Resident, binned visibility stream.
@@ -208,6 +211,9 @@ struct chip_vset {
     // events around the last binned kernel launch
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timed = false;
+    std::shared_ptr<chip::CycleWorkspace> cycle;
+    int cycle_layout = -1;    // -1 timed choice, 0 [chan][x][y] 2-D plans, 1 interleaved 1-D passes
+    chip::DevBuf cycle_norm;  // per channel: sumwt (f32), then psf peak
     ~chip_vset() {
         if (ev_start) (void)hipEventDestroy(ev_start);
         if (ev_stop) (void)hipEventDestroy(ev_stop);

@@ -346,6 +346,36 @@ class VisibilitySet:
         abi.call("chip_vset_grid_values", self.handle,_ptr(values), _ptr(grid))
         return grid
 
+    def major_cycle(self, model, normalization_sumwt, psf_peak, out=None):
+        """Residual images (n_chan, nx, ny) of ``model`` (n_chan, nx, ny) in one call.
+
+        Same result as model_to_grid -> grid(RESIDUAL) -> finish_products, with
+        the grids and FFT plans kept by the set between calls
+        (compute_residual_w_project_replay, lib.rs:70895-70963)."""
+        g = self.gridder
+        model = model.to(device=g.ctx.torch_device, dtype=torch.float32).contiguous()
+        if model.dim() == 2:
+            model = model[None]
+        if model.shape != (self.n_chan, g.nx, g.ny):
+            raise abi.InvalidRequest(f"model shape {tuple(model.shape)} != {(self.n_chan, g.nx, g.ny)}")
+        nsum = np.ascontiguousarray(np.broadcast_to(np.asarray(normalization_sumwt, np.float64), (self.n_chan,)))
+        peaks = np.ascontiguousarray(np.broadcast_to(np.asarray(psf_peak, np.float32), (self.n_chan,)))
+        if out is None:
+            out = torch.empty_like(model)
+        abi.call("chip_vset_major_cycle", self.handle, _ptr(model), nsum.ctypes.data_as(ctypes.c_void_p),
+                 peaks.ctypes.data_as(ctypes.c_void_p), _ptr(out))
+        return out
+
+    def set_cycle_layout(self, layout):
+        """-1: timed choice, 0: 2-D plans over [chan][x][y], 1: interleaved 1-D passes."""
+        abi.call("chip_vset_cycle_layout", self.handle, int(layout))
+
+    def cycle_info(self):
+        """FFT layout chosen by the first major cycle and the timed ms of both layouts."""
+        il, a, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        abi.call("chip_vset_cycle_info", self._h, ctypes.byref(il), ctypes.byref(a), ctypes.byref(b))
+        return {"interleaved": il.value, "fft_ms_2d": a.value, "fft_ms_1d": b.value}
+
     def close(self):
         if self._h:
             abi.load().chip_vset_destroy(self._h)

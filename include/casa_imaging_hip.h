@@ -195,6 +195,31 @@ int chip_vset_degrid(chip_vset *vs, const float *model_dev, float *vis_out_dev, 
  * sample, input order) with unit weight into grid_dev (accumulated), i.e.
  * grid += value * tap (gridder.rs:3680-3699 with value in place of V*w). */
 int chip_vset_grid_values(chip_vset *vs, const float *values_dev, float *grid_dev, char **err);
+/* Major-cycle residual refresh of every channel in one call
+ * (compute_residual_w_project_replay, lib.rs:70895-70963; standard:
+ * compute_residual_standard_tiled, lib.rs:68862-68932):
+ *   apodise model (gridder.rs:1554-1592) -> centered FFT -> degrid + grid
+ *   (V - p) w -> centered inverse FFT -> crop/correct -> / nsum / psf_peak
+ *   (x 2 for W-projection).
+ * model_dev: n_chan images float32 [chan][nx][ny]; residual_out_dev: same
+ * shape; nsum, psf_peak: host arrays of n_chan (normalization sum of weights,
+ * PSF peak of each channel, as returned with the PSF).  Grids and FFT plans
+ * live in a workspace owned by the set, allocated on the first call (two
+ * complex64 grid cubes); the first call also times the two FFT layouts and
+ * keeps the faster.  Synchronous on the context's stream. */
+int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *nsum,
+                          const float *psf_peak, float *residual_out_dev, char **err);
+/* FFT layout chosen by the first major cycle: interleaved = 1 for the two
+ * 1-D passes over [x][chan][y], 0 for 2-D plans over [chan][x][y], -1 before
+ * the first call; the timed forward+inverse ms of each.  No reference
+ * counterpart (the reference's DirtyProductFftPolicy picks its host FFT
+ * backend, lib.rs:1280-1310). */
+int chip_vset_cycle_info(const chip_vset *vs, int *interleaved, double *fft_ms_2d,
+                         double *fft_ms_1d, char **err);
+/* Fix the FFT layout of later major cycles: -1 timed choice (default),
+ * 0 2-D plans over [chan][x][y], 1 two 1-D passes over [x][chan][y].  Frees
+ * the current workspace.  No reference counterpart. */
+int chip_vset_cycle_layout(chip_vset *vs, int layout, char **err);
 
 /* ---- plain per-channel path (CHIP_F64 or CHIP_F32 grids, no binning) ------ */
 /* One launch per channel, one thread per sample, global atomics.  CHIP_F64

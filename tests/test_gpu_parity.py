@@ -377,3 +377,107 @@ def test_cube_rows_by_channels(gpu, oracle_mod):
         assert rel(dirty[c].cpu().numpy(), odirty) < 5e-6
         assert rel(psf[c].cpu().numpy(), opsf) < 5e-6
         assert rel(dirty64[c].cpu().numpy(), odirty) < 5e-6
+
+
+# --------------------------------------------------------------------------
+# fused major cycle (model -> residual image in one call)
+# --------------------------------------------------------------------------
+
+def _point_models(seed, n_chan, nx, n=20):
+    rng = np.random.default_rng(seed)
+    m = np.zeros((n_chan, nx, nx), np.float32)
+    for c in range(n_chan):
+        m[c, rng.integers(15, nx - 15, n), rng.integers(15, nx - 15, n)] = rng.uniform(0.1, 1.0, n)
+    return m
+
+
+@pytest.mark.parametrize("layout", [0, 1])
+def test_major_cycle_standard(gpu, oracle_mod, layout):
+    """Standard path: compute_residual_standard_tiled (lib.rs:68862-68932)
+    against the oracle's FP64 residual grid and FP64 FFT; <= 1e-5 of peak."""
+    g, o, (u, v, w, wt, sf, vis, gr) = _std_setup(gpu, oracle_mod, seed=21, sources=True)
+    b = batch_of(gpu, u, v, w, wt, sf, vis, gr)
+    opsf, _, osums = o.grid(u, v, wt, sf, vis, gridable=gr, mode=0)
+    vs = g.visibility_set(b)
+    vs.set_cycle_layout(layout)
+    psf32 = g.new_grids(1)
+    vs.grid(gpu.MODE_PSF, psf=psf32)
+    _, _, peak = g.finish_products(psf32, None, vs.sums[0]["normalization_sumwt"])
+    nsum = vs.sums[0]["normalization_sumwt"]
+    for seed in (3, 4):  # the second call reuses the workspace
+        model = _point_models(seed, 1, g.nx)
+        _, ores, _ = o.grid(u, v, wt, sf, vis, gridable=gr, mode=2, model=o.model_grid(model[0]))
+        _, ref_res, ref_peak = o.dirty_products(opsf, ores, osums)
+        res = vs.major_cycle(torch.from_numpy(model), nsum, peak).cpu().numpy()
+        assert abs(peak[0] - ref_peak) <= 1e-5 * ref_peak
+        assert rel(res[0], ref_res) < 1e-5
+    assert vs.cycle_info()["interleaved"] == layout
+
+
+@pytest.mark.parametrize("layout", [-1, 0, 1])
+def test_major_cycle_w_cube(gpu, oracle_mod, layout):
+    """W-projection cube, rows x channels: compute_residual_w_project_replay
+    (lib.rs:70895-70963) per channel against the oracle (FP32 grids as the
+    reference, FP64 FFT); <= 1e-5 of the residual peak.  Also equal to the
+    unfused path model_to_grid -> grid(RESIDUAL) -> finish_products to 5e-6
+    of the peak: both are FP32 transforms, each a few 1e-6 from the FP64 one,
+    over different layouts and summation orders."""
+    enu = synth.y_array(3, arm_m=2000.0)
+    uvw = synth.uvw_tracks(enu, 34.0, 30.0, 0.5, 60.0)
+    n_rows, n_chan = uvw.shape[0], 5
+    freq = np.linspace(1.0e9, 1.6e9, n_chan)
+    scale = freq / synth.C
+    rng = np.random.default_rng(17)
+    vis = (rng.normal(size=(n_rows, n_chan)) + 1j * rng.normal(size=(n_rows, n_chan))).astype(np.complex64)
+    wt = rng.uniform(0.5, 2.0, (n_rows, n_chan)).astype(np.float32)
+    flag = rng.uniform(size=(n_rows, n_chan)) < 0.2
+    maxuv = np.max(np.abs(uvw[:, :2])) * scale[-1]
+    nx = 96
+    cell = 1.0 / (3.0 * maxuv)
+    maxw = np.max(np.abs(uvw[:, 2])) * scale[-1]
+    g = gpu.StandardGridder(nx, nx, cell, cell)
+    g.attach_wproject(maxw, planes=8)
+    batch = gpu.RowChannelBatch(t(uvw), t(scale), t(vis), t(wt), t(flag.astype(np.uint8)))
+    vs = g.visibility_set(batch)
+    vs.set_cycle_layout(layout)
+    psf = g.new_grids(n_chan)
+    vs.grid(gpu.MODE_PSF, psf=psf)
+    nsum = np.array([s["normalization_sumwt"] for s in vs.sums])
+    _, _, peaks = g.finish_products(psf.clone(), None, nsum)
+
+    o = oracle_mod.StandardGridder(nx, nx, cell, cell)
+    sup, ker, nt = g.wproject_tables()
+    wp = oracle_mod.WProjector(o, maxw, explicit_planes=8).with_kernels(ker, sup, nt)
+    for seed in (5, 6):
+        model = _point_models(seed, n_chan, nx)
+        res = vs.major_cycle(torch.from_numpy(model), nsum, peaks).cpu().numpy()
+        gm = g.model_to_grid(torch.from_numpy(model))
+        r32 = g.new_grids(n_chan)
+        vs.grid(gpu.MODE_RESIDUAL, model=gm, dirty=r32)
+        _, unfused, _ = g.finish_products(None, r32, nsum, psf_peak=peaks)
+        assert rel(res, unfused.cpu().numpy()) < 5e-6
+        for c in range(n_chan):
+            u, v, w = (uvw[:, k] * scale[c] for k in range(3))
+            opsf, ores, osums = wp.grid(u, v, w, wt[:, c], np.ones(n_rows, np.float32), vis[:, c],
+                                        gridable=~flag[:, c], mode=2, model=wp.model_grid(model[c]))
+            opsf, _, _ = wp.grid(u, v, w, wt[:, c], np.ones(n_rows, np.float32), vis[:, c],
+                                 gridable=~flag[:, c], mode=0)
+            _, ref_res, ref_peak = wp.dirty_products(opsf, ores, osums)
+            assert abs(peaks[c] - ref_peak) <= 1e-5 * ref_peak
+            assert rel(res[c], ref_res) < 1e-5
+    info = vs.cycle_info()
+    assert info["interleaved"] in (0, 1) and info["fft_ms_2d"] > 0 and info["fft_ms_1d"] > 0
+    if layout >= 0:
+        assert info["interleaved"] == layout
+
+
+def test_major_cycle_rejects_bad_normalization(gpu, oracle_mod):
+    g, o, (u, v, w, wt, sf, vis, gr) = _std_setup(gpu, oracle_mod, seed=22)
+    vs = g.visibility_set(batch_of(gpu, u, v, w, wt, sf, vis, gr))
+    model = torch.zeros((1, g.nx, g.ny))
+    with pytest.raises(gpu.NoUsableSamples):
+        vs.major_cycle(model, 0.0, 1.0)
+    with pytest.raises(gpu.Normalization):
+        vs.major_cycle(model, 1.0, float("nan"))
+    with pytest.raises(gpu.InvalidRequest):
+        vs.major_cycle(torch.zeros((2, g.nx, g.ny)), 1.0, 1.0)
