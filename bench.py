@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--compare-hipfft", type=int, default=1,
+                    help="also time the step with hipFFT 2-D plans (reported beside value)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
@@ -305,8 +307,8 @@ def main():
     for _ in range(args.warmup):
         step32()
     info = vs.cycle_info()
-    log(f"major-cycle FFT layout: {'interleaved [x][chan][y] 1-D passes' if info['interleaved'] else '[chan][x][y] 2-D plans'} "
-        f"(timed fwd+inv {info['fft_ms_1d']:.2f} ms 1-D vs {info['fft_ms_2d']:.2f} ms 2-D)")
+    log(f"major-cycle transforms: {'fused row/column passes' if info['strategy'] == pkg.CYCLE_FUSED else 'hipFFT 2-D plans'}"
+        f", column block {info['columns']}, timed forward+inverse {info['transform_ms']:.2f} ms")
     kernel_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -317,8 +319,25 @@ def main():
     rate32 = n_unflagged / t32 / 1e6
     log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
         f"binned kernel {np.mean(kernel_ms):.3f} ms")
-    result_extra.update(fft_layout="interleaved" if info["interleaved"] else "2d",
-                        fft_fwd_inv_ms_1d=info["fft_ms_1d"], fft_fwd_inv_ms_2d=info["fft_ms_2d"])
+    result_extra.update(transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
+                        transform_columns=info["columns"], transform_fwd_inv_ms=info["transform_ms"])
+    if info["strategy"] == pkg.CYCLE_FUSED and args.compare_hipfft:
+        # the same step with hipFFT 2-D plans and separate placement / crop kernels
+        r_h = torch.empty_like(r32)
+        vs.set_cycle_strategy(pkg.CYCLE_HIPFFT)
+        vs.major_cycle(model, nsum, peak32, out=r_h)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(3):
+            vs.major_cycle(model, nsum, peak32, out=r_h)
+        torch.cuda.synchronize()
+        th = (time.perf_counter() - t0) / 3
+        diff = float(torch.max(torch.abs(r_h - r32)) / torch.max(torch.abs(r_h)))
+        result_extra.update(hipfft_ms_per_step=th * 1e3, fused_vs_hipfft_max_diff=diff)
+        log(f"same step with hipFFT 2-D plans: {th * 1e3:.2f} ms/step; fused vs hipFFT residual diff {diff:.2e} of peak")
+        vs.set_cycle_strategy(pkg.CYCLE_AUTO)
+        del r_h
+        torch.cuda.empty_cache()
 
     if not args.no_fp64:
         torch.cuda.empty_cache()

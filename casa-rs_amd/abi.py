@@ -22,6 +22,10 @@ MODE_PSF = 0
 MODE_DIRTY = 1
 MODE_RESIDUAL = 2
 
+CYCLE_AUTO = -1
+CYCLE_HIPFFT = 0
+CYCLE_FUSED = 1
+
 F32 = 0
 F64 = 1
 
@@ -36,7 +40,7 @@ EXPORTS = (
     "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
-    "chip_vset_cycle_layout",
+    "chip_vset_cycle_strategy",
     "chip_grid_plain", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
 )
@@ -158,7 +162,7 @@ def load():
         "chip_vset_grid_values": (ctypes.c_int, [_P, _P, _P, _PP]),
         "chip_vset_major_cycle": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
         "chip_vset_cycle_info": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
-        "chip_vset_cycle_layout": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
+        "chip_vset_cycle_strategy": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
         "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
         "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),

@@ -12,8 +12,8 @@ void vset_degrid(chip_vset &vs, const float *model, float *vis_out);
 void vset_grid_values(chip_vset &vs, const float *values, float *grid);
 void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, const float *peaks,
                       float *out);
-void vset_cycle_info(const chip_vset &vs, int *interleaved, double *ms_2d, double *ms_1d);
-void vset_cycle_layout(chip_vset &vs, int layout);
+void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t *columns, double *transform_ms);
+void vset_cycle_strategy(chip_vset &vs, int strategy);
 }  // namespace chip
 
 using namespace chip;
@@ -298,19 +298,19 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
     });
 }
 
-int chip_vset_cycle_layout(chip_vset *vs, int layout, char **err) {
+int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err) {
     return guarded(err, [&] {
         if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
         set_device(vs->g->ctx);
-        vset_cycle_layout(*vs, layout);
+        vset_cycle_strategy(*vs, strategy);
     });
 }
 
-int chip_vset_cycle_info(const chip_vset *vs, int *interleaved, double *fft_ms_2d,
-                         double *fft_ms_1d, char **err) {
+int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *columns, double *transform_ms,
+                         char **err) {
     return guarded(err, [&] {
-        if (!vs || !interleaved || !fft_ms_2d || !fft_ms_1d) fail(CHIP_ERR_INVALID, "NULL argument");
-        vset_cycle_info(*vs, interleaved, fft_ms_2d, fft_ms_1d);
+        if (!vs || !strategy || !columns || !transform_ms) fail(CHIP_ERR_INVALID, "NULL argument");
+        vset_cycle_info(*vs, strategy, columns, transform_ms);
     });
 }
 

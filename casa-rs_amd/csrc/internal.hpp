@@ -212,7 +212,7 @@ struct chip_vset {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     bool timed = false;
     std::shared_ptr<chip::CycleWorkspace> cycle;
-    int cycle_layout = -1;    // -1 timed choice, 0 [chan][x][y] 2-D plans, 1 interleaved 1-D passes
+    int cycle_strategy = CHIP_CYCLE_AUTO;
     chip::DevBuf cycle_norm;  // per channel: sumwt (f32), then psf peak
     ~chip_vset() {
         if (ev_start) (void)hipEventDestroy(ev_start);

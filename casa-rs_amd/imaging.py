@@ -366,15 +366,15 @@ class VisibilitySet:
                  peaks.ctypes.data_as(ctypes.c_void_p), _ptr(out))
         return out
 
-    def set_cycle_layout(self, layout):
-        """-1: timed choice, 0: 2-D plans over [chan][x][y], 1: interleaved 1-D passes."""
-        abi.call("chip_vset_cycle_layout", self.handle, int(layout))
+    def set_cycle_strategy(self, strategy):
+        """abi.CYCLE_AUTO, abi.CYCLE_HIPFFT or abi.CYCLE_FUSED for later major cycles."""
+        abi.call("chip_vset_cycle_strategy", self.handle, int(strategy))
 
     def cycle_info(self):
-        """FFT layout chosen by the first major cycle and the timed ms of both layouts."""
-        il, a, b = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
-        abi.call("chip_vset_cycle_info", self._h, ctypes.byref(il), ctypes.byref(a), ctypes.byref(b))
-        return {"interleaved": il.value, "fft_ms_2d": a.value, "fft_ms_1d": b.value}
+        """Transform strategy in use, fused column block width, timed transform ms."""
+        s, c, ms = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_double()
+        abi.call("chip_vset_cycle_info", self._h, ctypes.byref(s), ctypes.byref(c), ctypes.byref(ms))
+        return {"strategy": s.value, "columns": c.value, "transform_ms": ms.value}
 
     def close(self):
         if self._h:

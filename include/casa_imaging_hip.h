@@ -45,6 +45,11 @@ extern "C" {
 #define CHIP_MODE_DIRTY 1    /* PSF + dirty (V*w) grids */
 #define CHIP_MODE_RESIDUAL 2 /* degrid model grid, grid (V - model)*w */
 
+/* Transform strategies of chip_vset_major_cycle. */
+#define CHIP_CYCLE_AUTO (-1)
+#define CHIP_CYCLE_HIPFFT 0
+#define CHIP_CYCLE_FUSED 1
+
 #define CHIP_F32 0 /* binned, LDS-tiled, FP32 complex MACs, complex64 grids */
 #define CHIP_F64 1 /* plain per-channel launch, FP64 MACs and atomics, complex128 grids */
 
@@ -205,21 +210,27 @@ int chip_vset_grid_values(chip_vset *vs, const float *values_dev, float *grid_de
  * shape; nsum, psf_peak: host arrays of n_chan (normalization sum of weights,
  * PSF peak of each channel, as returned with the PSF).  Grids and FFT plans
  * live in a workspace owned by the set, allocated on the first call (two
- * complex64 grid cubes); the first call also times the two FFT layouts and
- * keeps the faster.  Synchronous on the context's stream. */
+ * complex64 grid cubes); with the fused strategy the first call also times
+ * the candidate column block widths and keeps the fastest.  Synchronous on
+ * the context's stream. */
 int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *nsum,
                           const float *psf_peak, float *residual_out_dev, char **err);
-/* FFT layout chosen by the first major cycle: interleaved = 1 for the two
- * 1-D passes over [x][chan][y], 0 for 2-D plans over [chan][x][y], -1 before
- * the first call; the timed forward+inverse ms of each.  No reference
- * counterpart (the reference's DirtyProductFftPolicy picks its host FFT
- * backend, lib.rs:1280-1310). */
-int chip_vset_cycle_info(const chip_vset *vs, int *interleaved, double *fft_ms_2d,
-                         double *fft_ms_1d, char **err);
-/* Fix the FFT layout of later major cycles: -1 timed choice (default),
- * 0 2-D plans over [chan][x][y], 1 two 1-D passes over [x][chan][y].  Frees
- * the current workspace.  No reference counterpart. */
-int chip_vset_cycle_layout(chip_vset *vs, int layout, char **err);
+/* Transform strategy of later major cycles (frees the current workspace):
+ *   CHIP_CYCLE_AUTO   fused when the grid allows it, else hipFFT (default);
+ *   CHIP_CYCLE_HIPFFT hipFFT 2-D plans over [chan][x][y], separate placement
+ *                     and crop kernels;
+ *   CHIP_CYCLE_FUSED  row / column FFT passes with placement, apodisation,
+ *                     Hermitian half spectrum, crop and normalisation fused
+ *                     in (grid lengths factoring into 2, 3, 5; CHIP_ERR_INVALID
+ *                     at the next major cycle otherwise).
+ * No reference counterpart (the reference's DirtyProductFftPolicy picks its
+ * host FFT backend, lib.rs:1280-1310). */
+int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
+/* Strategy in use (-1 before the first major cycle), the column block width
+ * of the fused column passes (0 for hipFFT) and the timed forward + inverse
+ * transform ms measured when the workspace was built (-1 if not timed). */
+int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *columns, double *transform_ms,
+                         char **err);
 
 /* ---- plain per-channel path (CHIP_F64 or CHIP_F32 grids, no binning) ------ */
 /* One launch per channel, one thread per sample, global atomics.  CHIP_F64
