@@ -308,7 +308,8 @@ def main():
         step32()
     info = vs.cycle_info()
     log(f"major-cycle transforms: {'fused row/column passes' if info['strategy'] == pkg.CYCLE_FUSED else 'hipFFT 2-D plans'}"
-        f", column block {info['columns']}, timed forward+inverse {info['transform_ms']:.2f} ms")
+        f", column block {info['columns']} x {info['column_threads']} threads, {info['rows_per_group']} rows per "
+        f"rows group, timed forward+inverse {info['transform_ms']:.2f} ms")
     kernel_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -320,7 +321,8 @@ def main():
     log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
         f"binned kernel {np.mean(kernel_ms):.3f} ms")
     result_extra.update(transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
-                        transform_columns=info["columns"], transform_fwd_inv_ms=info["transform_ms"])
+                        transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
+                        transform_fwd_inv_ms=info["transform_ms"])
     if info["strategy"] == pkg.CYCLE_FUSED and args.compare_hipfft:
         # the same step with hipFFT 2-D plans and separate placement / crop kernels
         r_h = torch.empty_like(r32)

@@ -12,7 +12,7 @@ void vset_degrid(chip_vset &vs, const float *model, float *vis_out);
 void vset_grid_values(chip_vset &vs, const float *values, float *grid);
 void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, const float *peaks,
                       float *out);
-void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t *columns, double *transform_ms);
+void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], double *transform_ms);
 void vset_cycle_strategy(chip_vset &vs, int strategy);
 }  // namespace chip
 
@@ -306,11 +306,11 @@ int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err) {
     });
 }
 
-int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *columns, double *transform_ms,
+int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *shape, double *transform_ms,
                          char **err) {
     return guarded(err, [&] {
-        if (!vs || !strategy || !columns || !transform_ms) fail(CHIP_ERR_INVALID, "NULL argument");
-        vset_cycle_info(*vs, strategy, columns, transform_ms);
+        if (!vs || !strategy || !shape || !transform_ms) fail(CHIP_ERR_INVALID, "NULL argument");
+        vset_cycle_info(*vs, strategy, shape, transform_ms);
     });
 }
 

@@ -46,6 +46,8 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.cells = g.gx * g.gy;
     gp.cstride = gp.cells;
     gp.xstride = g.gy;
+    gp.bstride = 0;
+    gp.blocked = 0;
     gp.model_signed = 0;
     gp.sign_base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
     gp.taps = g.d_taps.as<float>();
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
             float2 m = make_float2(0.0f, 0.0f);
             if (model && ly < L && gxx >= 0 && gyy >= 0 && gxx < static_cast<int>(gp.gx) &&
                 gyy < static_cast<int>(gp.gy)) {
-                m = model[chan_off + static_cast<uint64_t>(gxx) * gp.xstride + gyy];
+                m = model[chan_off + cell_offset(gp, gxx, gyy)];
                 if (gp.model_signed && ((gxx + gyy + gp.sign_base) & 1)) {
                     m.x = -m.x;
                     m.y = -m.y;
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
         const int gxx = x0 + lx, gyy = y0 + ly;
         if (ly >= L || gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy))
             continue;
-        const uint64_t gi = 2 * (chan_off + static_cast<uint64_t>(gxx) * gp.xstride + gyy);
+        const uint64_t gi = 2 * (chan_off + cell_offset(gp, gxx, gyy));
         if (kPsf) {
             const float r = P_re[c], i = P_im[c];
             if (r != 0.0f) unsafeAtomicAdd(psf + gi, r);
@@ -426,7 +428,7 @@ void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, flo
 }
 
 // Residual pass of the major cycle over caller-laid-out grids (see GridParams
-// cstride / xstride / model_signed).
+// cstride / blocked / model_signed).
 void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res) {
     if (vs.g->wproject)
         launch_binned_t<kModeResidual, true>(vs, gp, nullptr, model, nullptr, res, nullptr, vs.g->ctx->stream);

@@ -106,8 +106,8 @@ size_t lds_limit(int device) {
     return static_cast<size_t>(v);
 }
 
-// Forward + inverse transforms of the current fused plan, in ms.  Writes
-// scratch values into the workspace and `out` (overwritten afterwards).
+// Forward + inverse transforms of the fused plan, in ms.  Writes scratch
+// values into the workspace and `out` (overwritten afterwards).
 double time_fused(chip_vset &vs, CycleWorkspace &ws, const float *model, float *out, hipStream_t st) {
     const xf::XPlan &p = ws.xp;
     float2 *M = ws.model_grid.as<float2>();
@@ -133,41 +133,21 @@ void cycle_setup(chip_vset &vs, CycleWorkspace &ws, const float *model, float *o
     chip_gridder &g = *vs.g;
     hipStream_t st = g.ctx->stream;
     const uint64_t nc = vs.n_chan;
-    const uint64_t cells = nc * g.gx * g.gy;
-    ws.model_grid.alloc(cells * sizeof(float2));
-    ws.res_grid.alloc(cells * sizeof(float2));
-    CHIP_HIP(hipMemsetAsync(ws.model_grid.p, 0, ws.model_grid.bytes, st));
-    CHIP_HIP(hipMemsetAsync(ws.res_grid.p, 0, ws.res_grid.bytes, st));
-
     bool fused = false;
     if (vs.cycle_strategy != CHIP_CYCLE_HIPFFT) {
-        const size_t lds = lds_limit(g.device);
-        // Candidate column-block widths 4, 8, 16 (2^logc): keep the fastest.
-        double best = -1.0;
-        uint32_t best_logc = 0;
-        for (uint32_t lc = 2; lc <= 4; ++lc) {
-            if (!xf::plan_transforms(g, nc, lds, lc, ws.xp) || ws.xp.logc != lc) continue;
-            const double ms = time_fused(vs, ws, model, out, st);
-            if (best < 0.0 || ms < best) {
-                best = ms;
-                best_logc = lc;
-            }
-        }
-        if (best >= 0.0) {
-            if (!xf::plan_transforms(g, nc, lds, best_logc, ws.xp))
-                fail(CHIP_ERR_DEVICE, "fused transform plan could not be rebuilt");
-            ws.transform_ms = best;
-            fused = true;
-        } else if (xf::plan_transforms(g, nc, lds, 1, ws.xp)) {
-            // very long columns: blocks of 2 or 1 columns
-            ws.transform_ms = time_fused(vs, ws, model, out, st);
-            fused = true;
-        }
+        fused = xf::plan_transforms(g, nc, lds_limit(g.device), ws.xp);
         if (!fused && vs.cycle_strategy == CHIP_CYCLE_FUSED)
-            fail(CHIP_ERR_INVALID, "grid does not factor into 2, 3, 5 or exceeds the LDS for fused transforms");
+            fail(CHIP_ERR_INVALID,
+                 "grid does not factor into 2, 3, 5 or exceeds one workgroup for fused transforms");
     }
+    const uint64_t elements = fused ? xf::grid_elements(ws.xp) : nc * g.gx * g.gy;
+    ws.model_grid.alloc(elements * sizeof(float2));
+    ws.res_grid.alloc(elements * sizeof(float2));
+    CHIP_HIP(hipMemsetAsync(ws.model_grid.p, 0, ws.model_grid.bytes, st));
+    CHIP_HIP(hipMemsetAsync(ws.res_grid.p, 0, ws.res_grid.bytes, st));
     if (fused) {
         ws.strategy = CHIP_CYCLE_FUSED;
+        ws.transform_ms = time_fused(vs, ws, model, out, st);
         return;
     }
     if (g.gx * g.gy > 0x7fffffffull || nc > 0x7fffffffull)
@@ -225,6 +205,9 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
     gp.model_signed = 1;
 
     if (ws.strategy == CHIP_CYCLE_FUSED) {
+        gp.blocked = 1;
+        gp.cstride = ws.xp.xg.gstride;
+        gp.bstride = g.gx * xf::kBlock;
         // the half spectrum borrows the residual grid, the inverse rows the model grid
         xf::forward_model(ws.xp, model, Rg, M, st);
         CHIP_HIP(hipMemsetAsync(Rg, 0, ws.res_grid.bytes, st));
@@ -269,16 +252,20 @@ void vset_cycle_strategy(chip_vset &vs, int strategy) {
     vs.cycle.reset();  // the next call builds the workspace for the new strategy
 }
 
-void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t *columns, double *transform_ms) {
+void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], double *transform_ms) {
+    shape[0] = shape[1] = shape[2] = 0;
     if (!vs.cycle) {
         *strategy = -1;
-        *columns = 0;
         *transform_ms = -1.0;
         return;
     }
     *strategy = vs.cycle->strategy;
-    *columns = vs.cycle->strategy == CHIP_CYCLE_FUSED ? (1u << vs.cycle->xp.logc) : 0u;
     *transform_ms = vs.cycle->transform_ms;
+    if (vs.cycle->strategy == CHIP_CYCLE_FUSED) {
+        shape[0] = xf::kBlock;
+        shape[1] = xf::kThreads;
+        shape[2] = xf::kBlock;
+    }
 }
 
 }  // namespace chip

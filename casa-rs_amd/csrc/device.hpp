@@ -203,11 +203,14 @@ struct GridParams {
     int32_t norm_radius;     // R
     uint32_t tap_stride;     // W compact taps per (plane, offset): (2*max_support+1)^2
     uint64_t cells;          // gx * gy
-    // Grid addressing: cell (chan, x, y) is at chan*cstride + x*xstride + y.
-    // The public layout is cstride = gx*gy, xstride = gy; the major cycle may
-    // interleave channels inside x (cstride = gy, xstride = n_chan*gy).
+    // Grid addressing: cell (chan, x, y) is at chan*cstride + cell_offset(x, y):
+    // public layout x*xstride + y (cstride = gx*gy, xstride = gy); the
+    // major-cycle workspace is column blocked, (y/8)*bstride + 8x + y%8
+    // (blocked = 1, bstride = 8*gx).
     uint64_t cstride;
     uint64_t xstride;
+    uint64_t bstride;
+    uint32_t blocked;
     // Major cycle: the model grid is the raw forward transform of the
     // checkerboarded model, so (-1)^(x + y + sign_base) is applied on load.
     uint32_t model_signed;
@@ -217,6 +220,11 @@ struct GridParams {
     const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest
     const int32_t *supports;
 };
+
+__device__ __forceinline__ uint64_t cell_offset(const GridParams &gp, uint32_t x, uint32_t y) {
+    return gp.blocked ? static_cast<uint64_t>(y >> 3) * gp.bstride + x * 8u + (y & 7u)
+                      : static_cast<uint64_t>(x) * gp.xstride + y;
+}
 
 GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint32_t tiles_y);
 

@@ -4,30 +4,31 @@
 // Forward (centered_fft2(apodize_model(model)), gridder.rs:1554-1592,
 // fft.rs:34): the model is real, so the 2-D transform is Hermitian,
 // B[-kx, -ky] = conj(B[kx, ky]).
-//   rows pass   : for each image row x, the apodised, checkerboard-signed
-//                 model row zero-padded to gy points -> FFT along y; only
-//                 ky in [0, gy/2] is kept (half spectrum, nx rows).
-//   column pass : for each block of C columns ky <= gy/2, rows outside the
-//                 image are zero -> FFT along x; writes column ky and its
-//                 Hermitian mirror -ky, so the full grid is produced from
-//                 half of the columns.
+//   rows pass   : 8 image rows per workgroup; each apodised,
+//                 checkerboard-signed model row zero-padded to gy points ->
+//                 FFT along y; only ky in [0, gy/2] is kept (half spectrum).
+//   column pass : one block of 8 columns ky <= gy/2 per workgroup; rows
+//                 outside the image are zero -> FFT along x; writes column
+//                 ky and its Hermitian mirror -ky, so the full grid comes
+//                 from half of the columns.
 // Inverse (centered inverse FFT + crop + correction, gridder.rs:1646-1715):
-//   column pass : for each block of C columns, IFFT along x; keeps only the
-//                 nx rows the crop needs (shift theorem: output row
+//   column pass : one block of 8 columns per workgroup, IFFT along x; keeps
+//                 only the nx rows the crop needs (shift theorem: output row
 //                 (blc_x + x + gx/2) mod gx).
-//   rows pass   : IFFT along y of those rows; the crop, 1/(gx gy) scale,
-//                 sign (-1)^(x + y + gx/2 + gy/2), grid correction and
-//                 normalisation are applied on the way out.
+//   rows pass   : 8 rows per workgroup, IFFT along y; the crop, 1/(gx gy)
+//                 scale, sign (-1)^(x + y + gx/2 + gy/2), grid correction
+//                 and normalisation are applied on the way out.
 //
-// Each 1-D transform is a mixed-radix (2, 3, 4, 5) Stockham autosort FFT
-// held in LDS (one row, or a tile of gx x C complex values for the column
-// passes so that global accesses are C*8-byte row segments).  Every stage
-// reads its butterfly inputs into registers, synchronises, and writes the
-// outputs back, so one LDS buffer suffices.  Stockham stage with radix r
-// after sub-transforms of length ns: butterfly j reads elements j + t*n/r,
-// multiplies input t by exp(-+2 pi i k t / (ns r)) with k = j mod ns, and
-// writes output t to (j - k) r + k + t ns.  Twiddles are read from a table
-// exp(-2 pi i m / n) rounded once from FP64.
+// Each workgroup (1024 threads) holds 8 transforms in LDS, 128 threads per
+// transform, and runs a mixed-radix (2, 3, 4, 5) Stockham autosort FFT.
+// Stage with radix r after sub-transforms of length ns: butterfly j reads
+// elements j + t n/r, multiplies input t by exp(-+2 pi i k t / (ns r)) with
+// k = j mod ns, and writes output t to (j - k) r + k + t ns.  Every stage
+// reads its inputs into registers, synchronises, and writes its outputs
+// back, so one LDS buffer suffices.  Twiddles come from a table exp(-2 pi i
+// m / n) rounded once from FP64 and staged in LDS.  Column tiles are
+// interleaved in LDS (element i of transform c at 8 i + c) to match the
+// blocked grid layout; row tiles are row-major (c n + i).
 #include <cmath>
 
 #include "device.hpp"
@@ -103,34 +104,33 @@ template <bool INV> __device__ __forceinline__ void bf5(float2 *a) {
 template <> __device__ __forceinline__ void butterfly<5, false>(float2 *a) { bf5<false>(a); }
 template <> __device__ __forceinline__ void butterfly<5, true>(float2 *a) { bf5<true>(a); }
 
-// One Stockham stage over a tile of C = 2^logc interleaved transforms:
-// element i of transform c is buf[(i << logc) + c].  At most E elements per
-// thread (n * C <= E * T).
-template <int R, bool INV, int T, int E>
-__device__ __forceinline__ void stage(float2 *buf, uint32_t n, uint32_t logc, uint32_t ns,
-                                      const float2 *__restrict__ tw) {
-    constexpr int MB = (E + R - 1) / R;
+// One Stockham stage.  Thread (c, jt) handles butterflies j = jt + b * 128
+// of transform c; MB bounds the butterflies per thread.
+template <int R, bool INV, int MB, bool ILV>
+__device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n, uint32_t pitch,
+                                      uint32_t ns, uint32_t step, float inv_ns, uint32_t c, uint32_t jt) {
     const uint32_t nb = n / R;
-    const uint32_t total = nb << logc;
-    const uint32_t cmask = (1u << logc) - 1u;
-    const uint32_t step = n / (ns * R);
     float2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
-        const uint32_t bi = threadIdx.x + b * T;
-        if (bi < total) {
-            const uint32_t c = bi & cmask, j = bi >> logc;
+        const uint32_t j = jt + b * kLanes;
+        if (j < nb) {
 #pragma unroll
-            for (int t = 0; t < R; ++t) v[b][t] = buf[((j + t * nb) << logc) + c];
+            for (int t = 0; t < R; ++t) {
+                const uint32_t i = j + t * nb;
+                v[b][t] = buf[ILV ? (i << kLogBlock) + c : c * pitch + i];
+            }
         }
     }
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
-        const uint32_t bi = threadIdx.x + b * T;
-        if (bi < total) {
-            const uint32_t c = bi & cmask, j = bi >> logc;
-            const uint32_t k = j % ns;
+        const uint32_t j = jt + b * kLanes;
+        if (j < nb) {
+            // j / ns, exact for n < 2^22: the fraction of (j + 1/2) / ns is at
+            // least 1/(2 ns) away from an integer, the float error far smaller
+            const uint32_t q = static_cast<uint32_t>((static_cast<float>(j) + 0.5f) * inv_ns);
+            const uint32_t k = j - q * ns;
             if (k != 0) {
 #pragma unroll
                 for (int t = 1; t < R; ++t) {
@@ -140,139 +140,170 @@ __device__ __forceinline__ void stage(float2 *buf, uint32_t n, uint32_t logc, ui
                 }
             }
             butterfly<R, INV>(v[b]);
-            const uint32_t base = (j - k) * R + k;
+            const uint32_t base = q * ns * R + k;
 #pragma unroll
-            for (int t = 0; t < R; ++t) buf[((base + t * ns) << logc) + c] = v[b][t];
+            for (int t = 0; t < R; ++t) {
+                const uint32_t i = base + t * ns;
+                buf[ILV ? (i << kLogBlock) + c : c * pitch + i] = v[b][t];
+            }
         }
     }
     __syncthreads();
 }
 
-template <bool INV, int T, int E>
-__device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t logc) {
-    uint32_t ns = 1;
+// E: elements per thread per transform (n <= 128 E).
+template <bool INV, int E, bool ILV>
+__device__ __forceinline__ void run_fft(float2 *buf, const float2 *tw, const Axis &ax, uint32_t pitch,
+                                        uint32_t c, uint32_t jt) {
     for (uint32_t s = 0; s < ax.n_stages; ++s) {
-        const uint32_t r = ax.radix[s];
-        if (r == 5) stage<5, INV, T, E>(buf, ax.n, logc, ns, ax.tw);
-        else if (r == 4) stage<4, INV, T, E>(buf, ax.n, logc, ns, ax.tw);
-        else if (r == 3) stage<3, INV, T, E>(buf, ax.n, logc, ns, ax.tw);
-        else stage<2, INV, T, E>(buf, ax.n, logc, ns, ax.tw);
-        ns *= r;
+        const uint32_t r = ax.radix[s], ns = ax.ns[s], step = ax.step[s];
+        const float inv = ax.inv_ns[s];
+        if (r == 5) stage<5, INV, (E + 4) / 5, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        else if (r == 4) stage<4, INV, (E + 3) / 4, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        else if (r == 3) stage<3, INV, (E + 2) / 3, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        else stage<2, INV, (E + 1) / 2, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
     }
 }
 
 // ---------------------------------------------------------------- kernels
-// Launch grids: rows passes (nx, n_chan); column passes (column blocks, n_chan).
+// Dynamic LDS: 8 transforms of the axis length, then its n twiddles.
+// Rows passes: grid (nxp / 8, n_chan); column passes: grid (y blocks, n_chan).
 
-template <int T, int E>
-__global__ __launch_bounds__(T) void k_fwd_rows(XGeom xg, Axis ay, const float *__restrict__ model,
-                                                float2 *__restrict__ half) {
+template <int E>
+__global__ __launch_bounds__(kThreads) void k_fwd_rows(XGeom xg, Axis ay, const float *__restrict__ model,
+                                                       float2 *__restrict__ half) {
     extern __shared__ float2 buf[];
-    const uint32_t x = blockIdx.x, c = blockIdx.y;
+    const uint32_t n = xg.gy;
+    float2 *tw = buf + kBlock * n;
+    const uint32_t tid = threadIdx.x, r = tid / kLanes, jt = tid % kLanes;
+    const uint32_t xb = blockIdx.x, c = blockIdx.y;
+    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ay.tw[m];
+    const uint32_t x = xb * kBlock + r;
+    const bool live = x < xg.nx;
     const uint32_t gxx = xg.blc_x + x;
     const float *row = model + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
-    const float cx = xg.corr_x[gxx];
-    const float sx = xg.wproject ? xg.sinc_x[gxx] : 1.0f;
-    for (uint32_t i = threadIdx.x; i < xg.gy; i += T) {
+    const float cx = live ? xg.corr_x[gxx] : 0.0f;
+    const float sx = live && xg.wproject ? xg.sinc_x[gxx] : 1.0f;
+    for (uint32_t i = jt; i < n; i += kLanes) {
         float v = 0.0f;
-        if (i >= xg.blc_y && i < xg.blc_y + xg.ny) {
+        if (live && i >= xg.blc_y && i < xg.blc_y + xg.ny) {
             // factor evaluated left to right as the reference: cx*cy [*sx*sy]
             float f = cx * xg.corr_y[i];
             if (xg.wproject) f = f * sx * xg.sinc_y[i];
             v = row[i - xg.blc_y] * f;
             if ((gxx + i) & 1) v = -v;
         }
-        buf[i] = make_float2(v, 0.0f);
+        buf[r * n + i] = make_float2(v, 0.0f);
     }
     __syncthreads();
-    run_fft<false, T, E>(buf, ay, 0);
-    float2 *out = half + (static_cast<uint64_t>(c) * xg.nx + x) * xg.half_pitch;
-    for (uint32_t k = threadIdx.x; k <= xg.gy / 2; k += T) out[k] = buf[k];
+    run_fft<false, E, false>(buf, tw, ay, n, r, jt);
+    // half spectrum, blocked [yb][x][8]: 8 rows x 8 columns contiguous per block
+    float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(xb) * kBlock * kBlock;
+    const uint32_t hy = n / 2;
+    for (uint32_t e = tid; e < xg.nyb_h * kBlock * kBlock; e += kThreads) {
+        const uint32_t yb = e >> 6, rr = (e >> 3) & 7, ky = yb * kBlock + (e & 7);
+        h[static_cast<uint64_t>(yb) * xg.nxp * kBlock + (e & 63)] =
+            ky <= hy ? buf[rr * n + ky] : make_float2(0.0f, 0.0f);
+    }
 }
 
-template <int T, int E>
-__global__ __launch_bounds__(T) void k_fwd_cols(XGeom xg, Axis ax, const float2 *__restrict__ half,
-                                                float2 *__restrict__ grid) {
+template <int E>
+__global__ __launch_bounds__(kThreads) void k_fwd_cols(XGeom xg, Axis ax, const float2 *__restrict__ half,
+                                                       float2 *__restrict__ grid) {
     extern __shared__ float2 buf[];
-    const uint32_t logc = xg.logc, cmask = (1u << logc) - 1u;
-    const uint32_t y0 = blockIdx.x << logc, c = blockIdx.y;
-    const uint32_t hy = xg.gy / 2;
-    const float2 *src = half + static_cast<uint64_t>(c) * xg.nx * xg.half_pitch;
-    const uint32_t tile = xg.gx << logc;
-    for (uint32_t e = threadIdx.x; e < tile; e += T) {
-        const uint32_t i = e >> logc, ky = y0 + (e & cmask);
+    const uint32_t n = xg.gx;
+    float2 *tw = buf + kBlock * n;
+    const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
+    const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ax.tw[m];
+    const float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
+    for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
+        const uint32_t i = e >> 3;
         float2 v = make_float2(0.0f, 0.0f);
-        if (ky <= hy && i >= xg.blc_x && i < xg.blc_x + xg.nx)
-            v = src[static_cast<uint64_t>(i - xg.blc_x) * xg.half_pitch + ky];
+        if (i >= xg.blc_x && i < xg.blc_x + xg.nx) v = h[(i - xg.blc_x) * kBlock + (e & 7)];
         buf[e] = v;
     }
     __syncthreads();
-    run_fft<false, T, E>(buf, ax, logc);
-    float2 *dst = grid + static_cast<uint64_t>(c) * xg.cstride;
-    for (uint32_t e = threadIdx.x; e < tile; e += T) {
-        const uint32_t kx = e >> logc, ky = y0 + (e & cmask);
+    run_fft<false, E, true>(buf, tw, ax, 0, cc, jt);
+    float2 *g = grid + static_cast<uint64_t>(c) * xg.gstride;
+    const uint32_t hy = xg.gy / 2;
+    const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
+    for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
+        const uint32_t kx = e >> 3, ky = yb * kBlock + (e & 7);
         if (ky > hy) continue;
-        const float2 g = buf[e];
-        dst[static_cast<uint64_t>(kx) * xg.gy + ky] = g;
+        const float2 v = buf[e];
+        g[yb * bstride + e] = v;
         if (ky > 0 && ky < hy) {
-            const uint32_t mx = kx == 0 ? 0u : xg.gx - kx;
-            dst[static_cast<uint64_t>(mx) * xg.gy + (xg.gy - ky)] = make_float2(g.x, -g.y);
+            const uint32_t my = xg.gy - ky, mx = kx == 0 ? 0u : n - kx;
+            g[(my >> 3) * bstride + mx * kBlock + (my & 7)] = make_float2(v.x, -v.y);
         }
     }
 }
 
-template <int T, int E>
-__global__ __launch_bounds__(T) void k_inv_cols(XGeom xg, Axis ax, const float2 *__restrict__ grid,
-                                                float2 *__restrict__ rows) {
+template <int E>
+__global__ __launch_bounds__(kThreads) void k_inv_cols(XGeom xg, Axis ax, const float2 *__restrict__ grid,
+                                                       float2 *__restrict__ rows) {
     extern __shared__ float2 buf[];
-    const uint32_t logc = xg.logc, cmask = (1u << logc) - 1u;
-    const uint32_t y0 = blockIdx.x << logc, c = blockIdx.y;
-    const float2 *src = grid + static_cast<uint64_t>(c) * xg.cstride;
-    const uint32_t tile = xg.gx << logc;
-    for (uint32_t e = threadIdx.x; e < tile; e += T) {
-        const uint32_t i = e >> logc, ky = y0 + (e & cmask);
-        buf[e] = ky < xg.gy ? src[static_cast<uint64_t>(i) * xg.gy + ky] : make_float2(0.0f, 0.0f);
-    }
+    const uint32_t n = xg.gx;
+    float2 *tw = buf + kBlock * n;
+    const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
+    const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ax.tw[m];
+    const float2 *g = grid + static_cast<uint64_t>(c) * xg.gstride + static_cast<uint64_t>(yb) * n * kBlock;
+    const bool full = (yb + 1) * kBlock <= xg.gy;
+    for (uint32_t e = tid; e < n * kBlock; e += kThreads)
+        buf[e] = full || yb * kBlock + (e & 7) < xg.gy ? g[e] : make_float2(0.0f, 0.0f);
     __syncthreads();
-    run_fft<true, T, E>(buf, ax, logc);
-    float2 *dst = rows + static_cast<uint64_t>(c) * xg.nx * xg.gy;
-    const uint32_t out_n = xg.nx << logc;
-    for (uint32_t e = threadIdx.x; e < out_n; e += T) {
-        const uint32_t x = e >> logc, cc = e & cmask, ky = y0 + cc;
-        if (ky >= xg.gy) continue;
-        uint32_t i = xg.blc_x + x + xg.gx / 2;
-        if (i >= xg.gx) i -= xg.gx;
-        dst[static_cast<uint64_t>(x) * xg.gy + ky] = buf[(i << logc) + cc];
+    run_fft<true, E, true>(buf, tw, ax, 0, cc, jt);
+    float2 *o = rows + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
+    for (uint32_t e = tid; e < xg.nx * kBlock; e += kThreads) {
+        uint32_t i = xg.blc_x + (e >> 3) + n / 2;
+        if (i >= n) i -= n;
+        o[e] = buf[(i << kLogBlock) + (e & 7)];
     }
 }
 
-template <int T, int E>
-__global__ __launch_bounds__(T) void k_inv_rows(XGeom xg, Axis ay, const float2 *__restrict__ rows,
-                                                const float *__restrict__ nsum,
-                                                const float *__restrict__ peak, float *__restrict__ out) {
+template <int E>
+__global__ __launch_bounds__(kThreads) void k_inv_rows(XGeom xg, Axis ay, const float2 *__restrict__ rows,
+                                                       const float *__restrict__ nsum,
+                                                       const float *__restrict__ peak, float *__restrict__ out) {
     extern __shared__ float2 buf[];
-    const uint32_t x = blockIdx.x, c = blockIdx.y;
-    const float2 *src = rows + (static_cast<uint64_t>(c) * xg.nx + x) * xg.gy;
-    for (uint32_t i = threadIdx.x; i < xg.gy; i += T) buf[i] = src[i];
+    const uint32_t n = xg.gy;
+    float2 *tw = buf + kBlock * n;
+    const uint32_t tid = threadIdx.x, r = tid / kLanes, jt = tid % kLanes;
+    const uint32_t xb = blockIdx.x, c = blockIdx.y;
+    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ay.tw[m];
+    const float2 *a = rows + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(xb) * kBlock * kBlock;
+    const uint32_t rows_live = xg.nx - xb * kBlock;  // rows of this block inside the image (>= 1)
+    for (uint32_t e = tid; e < xg.nyb * kBlock * kBlock; e += kThreads) {
+        const uint32_t yb = e >> 6, rr = (e >> 3) & 7, ky = yb * kBlock + (e & 7);
+        if (ky < n)
+            buf[rr * n + ky] = rr < rows_live ? a[static_cast<uint64_t>(yb) * xg.nxp * kBlock + (e & 63)]
+                                              : make_float2(0.0f, 0.0f);
+    }
     __syncthreads();
-    run_fft<true, T, E>(buf, ay, 0);
+    run_fft<true, E, false>(buf, tw, ay, n, r, jt);
+    const uint32_t x = xb * kBlock + r;
+    if (x >= xg.nx) return;
     const uint32_t gxx = xg.blc_x + x;
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(xg.gx) * xg.gy);
     const float ns = nsum[c], pk = peak[c];
+    const float cx = xg.corr_x[gxx];
+    const float sincx = xg.wproject ? xg.sinc[gxx] : 1.0f;
     float *o = out + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
-    for (uint32_t y = threadIdx.x; y < xg.ny; y += T) {
+    for (uint32_t y = jt; y < xg.ny; y += kLanes) {
         const uint32_t gyy = xg.blc_y + y;
-        uint32_t i = gyy + xg.gy / 2;
-        if (i >= xg.gy) i -= xg.gy;
-        float re = buf[i].x * scale;
+        uint32_t i = gyy + n / 2;
+        if (i >= n) i -= n;
+        float re = buf[r * n + i].x * scale;
         if ((gxx + gyy + xg.base) & 1) re = -re;
-        const float factor = xg.corr_x[gxx] * xg.corr_y[gyy];
+        const float factor = cx * xg.corr_y[gyy];
         float v;
         if (!xg.wproject) {
-            v = re * factor;          // gridder.rs:1646-1657
-            v = v / ns / pk;          // lib.rs:68928-68929
+            v = re * factor;  // gridder.rs:1646-1657
+            v = v / ns / pk;  // lib.rs:68928-68929
         } else {
-            const float sf = xg.sinc[gxx] * xg.sinc[gyy];
+            const float sf = sincx * xg.sinc[gyy];
             v = fabsf(sf) > 1.0e-6f ? re * (factor / sf) : 0.0f;  // gridder.rs:1685-1715
             v = 2.0f * v / ns / pk;                                // lib.rs:70960-70961
         }
@@ -286,13 +317,19 @@ bool factor_axis(uint32_t n, Axis &ax) {
     ax.n = n;
     ax.n_stages = 0;
     const uint32_t order[4] = {5, 3, 4, 2};
+    uint32_t rest = n, ns = 1;
     for (uint32_t p : order)
-        while (n % p == 0) {
+        while (rest % p == 0) {
             if (ax.n_stages >= kMaxStages) return false;
-            ax.radix[ax.n_stages++] = p;
-            n /= p;
+            const uint32_t s = ax.n_stages++;
+            ax.radix[s] = p;
+            ax.ns[s] = ns;
+            ax.step[s] = n / (ns * p);
+            ax.inv_ns[s] = 1.0f / static_cast<float>(ns);
+            ns *= p;
+            rest /= p;
         }
-    return n == 1;
+    return rest == 1 && n < (1u << 22);
 }
 
 std::vector<float2> twiddles(uint32_t n) {
@@ -306,26 +343,16 @@ std::vector<float2> twiddles(uint32_t n) {
 
 namespace {
 
-// Rows: 256 threads, up to 40 elements each (gy <= 10240).  Columns: 1024
-// threads, up to 16 elements each (gx * C <= 16384, 128 KB of LDS); more
-// per thread would spill registers.
-constexpr int kRowT = 256;
-constexpr int kColT = 1024;
-
-int row_bucket(uint64_t per_thread) {
-    if (per_thread <= 8) return 8;
-    if (per_thread <= 16) return 16;
-    if (per_thread <= 24) return 24;
-    if (per_thread <= 40) return 40;
+int e_bucket(uint32_t n) {
+    const uint32_t per = (n + kLanes - 1) / kLanes;
+    if (per <= 4) return 4;
+    if (per <= 8) return 8;
+    if (per <= 12) return 12;
+    if (per <= 16) return 16;
     return 0;
 }
 
-int col_bucket(uint64_t per_thread) {
-    if (per_thread <= 8) return 8;
-    if (per_thread <= 12) return 12;
-    if (per_thread <= 16) return 16;
-    return 0;
-}
+size_t tile_lds(uint32_t n) { return static_cast<size_t>(kBlock + 1) * n * sizeof(float2); }
 
 template <class K> void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024)
@@ -333,77 +360,56 @@ template <class K> void allow_lds(K kernel, size_t bytes) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
 }
 
-#define CHIP_XF_CASE(KERNEL, T, EV, GDIM, LDS, ST, ...)                                           \
+#define CHIP_XF_CASE(KERNEL, EV, GDIM, LDS, ST, ...)                                              \
     case EV: {                                                                                     \
-        allow_lds(KERNEL<T, EV>, LDS);                                                             \
-        hipLaunchKernelGGL((KERNEL<T, EV>), GDIM, dim3(T), LDS, ST, __VA_ARGS__);                   \
+        allow_lds(KERNEL<EV>, LDS);                                                                \
+        hipLaunchKernelGGL((KERNEL<EV>), GDIM, dim3(kThreads), LDS, ST, __VA_ARGS__);               \
         break;                                                                                     \
     }
 
-#define CHIP_XF_ROWS(E, KERNEL, GDIM, LDS, ST, ...)                                               \
+#define CHIP_XF_LAUNCH(E, KERNEL, GDIM, LDS, ST, ...)                                             \
     do {                                                                                           \
         switch (E) {                                                                               \
-            CHIP_XF_CASE(KERNEL, kRowT, 8, GDIM, LDS, ST, __VA_ARGS__)                             \
-            CHIP_XF_CASE(KERNEL, kRowT, 16, GDIM, LDS, ST, __VA_ARGS__)                            \
-            CHIP_XF_CASE(KERNEL, kRowT, 24, GDIM, LDS, ST, __VA_ARGS__)                            \
-            CHIP_XF_CASE(KERNEL, kRowT, 40, GDIM, LDS, ST, __VA_ARGS__)                            \
-            default: fail(CHIP_ERR_INVALID, "row transform too long");                            \
-        }                                                                                          \
-        CHIP_HIP(hipGetLastError());                                                               \
-    } while (0)
-
-#define CHIP_XF_COLS(E, KERNEL, GDIM, LDS, ST, ...)                                               \
-    do {                                                                                           \
-        switch (E) {                                                                               \
-            CHIP_XF_CASE(KERNEL, kColT, 8, GDIM, LDS, ST, __VA_ARGS__)                             \
-            CHIP_XF_CASE(KERNEL, kColT, 12, GDIM, LDS, ST, __VA_ARGS__)                            \
-            CHIP_XF_CASE(KERNEL, kColT, 16, GDIM, LDS, ST, __VA_ARGS__)                            \
-            default: fail(CHIP_ERR_INVALID, "column transform tile too large");                   \
+            CHIP_XF_CASE(KERNEL, 4, GDIM, LDS, ST, __VA_ARGS__)                                    \
+            CHIP_XF_CASE(KERNEL, 8, GDIM, LDS, ST, __VA_ARGS__)                                    \
+            CHIP_XF_CASE(KERNEL, 12, GDIM, LDS, ST, __VA_ARGS__)                                   \
+            CHIP_XF_CASE(KERNEL, 16, GDIM, LDS, ST, __VA_ARGS__)                                   \
+            default: fail(CHIP_ERR_INVALID, "transform too long for one workgroup");              \
         }                                                                                          \
         CHIP_HIP(hipGetLastError());                                                               \
     } while (0)
 
 }  // namespace
 
-bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, uint32_t logc_max,
-                     XPlan &p) {
+bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, XPlan &p) {
     if (g.gx % 2 || g.gy % 2) return false;
-    if (n_chan == 0 || n_chan > 65535 || g.geom.nx > 0x7fffffffu) return false;
+    if (n_chan == 0 || n_chan > 65535 || g.gx > 0xffffffu || g.gy > 0xffffffu) return false;
     if (!factor_axis(static_cast<uint32_t>(g.gx), p.ax) || !factor_axis(static_cast<uint32_t>(g.gy), p.ay))
         return false;
-    p.row_e = row_bucket((g.gy + kRowT - 1) / kRowT);
-    if (!p.row_e || g.gy * sizeof(float2) > lds_limit) return false;
-    // widest column block that fits the LDS and the register budget
-    p.logc = 0;
-    bool ok = false;
-    for (int lc = static_cast<int>(logc_max); lc >= 0; --lc) {
-        const uint64_t tile = g.gx << lc;
-        if (tile * sizeof(float2) <= lds_limit && tile <= 16ull * kColT) {
-            p.logc = static_cast<uint32_t>(lc);
-            ok = true;
-            break;
-        }
-    }
-    if (!ok) return false;
-    p.col_e = col_bucket(((g.gx << p.logc) + kColT - 1) / kColT);
-    p.xg.nx = static_cast<uint32_t>(g.geom.nx);
-    p.xg.ny = static_cast<uint32_t>(g.geom.ny);
-    p.xg.gx = static_cast<uint32_t>(g.gx);
-    p.xg.gy = static_cast<uint32_t>(g.gy);
-    p.xg.blc_x = static_cast<uint32_t>(g.blc_x);
-    p.xg.blc_y = static_cast<uint32_t>(g.blc_y);
-    p.xg.base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
-    p.xg.wproject = g.wproject ? 1u : 0u;
-    p.xg.half_pitch = static_cast<uint32_t>(((g.gy / 2 + 1) + 7) / 8 * 8);
+    p.e_x = e_bucket(p.ax.n);
+    p.e_y = e_bucket(p.ay.n);
+    if (!p.e_x || !p.e_y || tile_lds(p.ax.n) > lds_limit || tile_lds(p.ay.n) > lds_limit) return false;
+    XGeom &xg = p.xg;
+    xg.nx = static_cast<uint32_t>(g.geom.nx);
+    xg.ny = static_cast<uint32_t>(g.geom.ny);
+    xg.gx = static_cast<uint32_t>(g.gx);
+    xg.gy = static_cast<uint32_t>(g.gy);
+    xg.blc_x = static_cast<uint32_t>(g.blc_x);
+    xg.blc_y = static_cast<uint32_t>(g.blc_y);
+    xg.base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
+    xg.wproject = g.wproject ? 1u : 0u;
+    xg.nxp = (xg.nx + kBlock - 1) / kBlock * kBlock;
+    xg.nyb = (xg.gy + kBlock - 1) / kBlock;
+    xg.nyb_h = (xg.gy / 2 + 1 + kBlock - 1) / kBlock;
     // the half spectrum and the inverse rows borrow the grid buffers
-    if (g.geom.nx * p.xg.half_pitch > g.gx * g.gy) return false;
-    p.xg.logc = p.logc;
-    p.xg.cstride = g.gx * g.gy;
-    p.xg.corr_x = g.d_corr_x.as<float>();
-    p.xg.corr_y = g.d_corr_y.as<float>();
-    p.xg.sinc = g.wproject ? g.d_sinc.as<float>() : nullptr;
-    p.xg.sinc_x = g.wproject ? g.d_sinc_x.as<float>() : nullptr;
-    p.xg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
+    if (xg.nxp > xg.gx) return false;
+    xg.gstride = static_cast<uint64_t>(xg.nyb) * xg.gx * kBlock;
+    xg.hstride = static_cast<uint64_t>(xg.nyb) * xg.nxp * kBlock;
+    xg.corr_x = g.d_corr_x.as<float>();
+    xg.corr_y = g.d_corr_y.as<float>();
+    xg.sinc = g.wproject ? g.d_sinc.as<float>() : nullptr;
+    xg.sinc_x = g.wproject ? g.d_sinc_x.as<float>() : nullptr;
+    xg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
     p.n_chan = static_cast<uint32_t>(n_chan);
     std::vector<float2> tx = twiddles(p.ax.n), ty = twiddles(p.ay.n);
     p.tw_x.alloc(tx.size() * sizeof(float2));
@@ -416,23 +422,18 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, u
 }
 
 void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st) {
-    const dim3 rows(p.xg.nx, p.n_chan);
-    const size_t row_lds = p.xg.gy * sizeof(float2);
-    CHIP_XF_ROWS(p.row_e, k_fwd_rows, rows, row_lds, st, p.xg, p.ay, model, half);
-    const uint32_t cols = p.xg.gy / 2 + 1;
-    const dim3 blocks((cols + (1u << p.logc) - 1) >> p.logc, p.n_chan);
-    const size_t col_lds = (static_cast<size_t>(p.xg.gx) << p.logc) * sizeof(float2);
-    CHIP_XF_COLS(p.col_e, k_fwd_cols, blocks, col_lds, st, p.xg, p.ax, half, grid);
+    const dim3 rows(p.xg.nxp / kBlock, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
+    const size_t row_lds = tile_lds(p.ay.n), col_lds = tile_lds(p.ax.n);
+    CHIP_XF_LAUNCH(p.e_y, k_fwd_rows, rows, row_lds, st, p.xg, p.ay, model, half);
+    CHIP_XF_LAUNCH(p.e_x, k_fwd_cols, cols, col_lds, st, p.xg, p.ax, half, grid);
 }
 
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows_buf, const float *nsum,
                       const float *peak, float *out, hipStream_t st) {
-    const dim3 blocks((p.xg.gy + (1u << p.logc) - 1) >> p.logc, p.n_chan);
-    const size_t col_lds = (static_cast<size_t>(p.xg.gx) << p.logc) * sizeof(float2);
-    CHIP_XF_COLS(p.col_e, k_inv_cols, blocks, col_lds, st, p.xg, p.ax, grid, rows_buf);
-    const dim3 rows(p.xg.nx, p.n_chan);
-    const size_t row_lds = p.xg.gy * sizeof(float2);
-    CHIP_XF_ROWS(p.row_e, k_inv_rows, rows, row_lds, st, p.xg, p.ay, rows_buf, nsum, peak, out);
+    const dim3 rows(p.xg.nxp / kBlock, p.n_chan), cols(p.xg.nyb, p.n_chan);
+    const size_t row_lds = tile_lds(p.ay.n), col_lds = tile_lds(p.ax.n);
+    CHIP_XF_LAUNCH(p.e_x, k_inv_cols, cols, col_lds, st, p.xg, p.ax, grid, rows_buf);
+    CHIP_XF_LAUNCH(p.e_y, k_inv_rows, rows, row_lds, st, p.xg, p.ay, rows_buf, nsum, peak, out);
 }
 
 }  // namespace xf

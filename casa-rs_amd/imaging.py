@@ -371,10 +371,12 @@ class VisibilitySet:
         abi.call("chip_vset_cycle_strategy", self.handle, int(strategy))
 
     def cycle_info(self):
-        """Transform strategy in use, fused column block width, timed transform ms."""
-        s, c, ms = ctypes.c_int(), ctypes.c_uint32(), ctypes.c_double()
-        abi.call("chip_vset_cycle_info", self._h, ctypes.byref(s), ctypes.byref(c), ctypes.byref(ms))
-        return {"strategy": s.value, "columns": c.value, "transform_ms": ms.value}
+        """Transform strategy in use, fused pass shape and the timed transform ms."""
+        s, ms = ctypes.c_int(), ctypes.c_double()
+        shape = (ctypes.c_uint32 * 3)()
+        abi.call("chip_vset_cycle_info", self._h, ctypes.byref(s), shape, ctypes.byref(ms))
+        return {"strategy": s.value, "columns": shape[0], "column_threads": shape[1],
+                "rows_per_group": shape[2], "transform_ms": ms.value}
 
     def close(self):
         if self._h:

@@ -211,8 +211,8 @@ int chip_vset_grid_values(chip_vset *vs, const float *values_dev, float *grid_de
  * PSF peak of each channel, as returned with the PSF).  Grids and FFT plans
  * live in a workspace owned by the set, allocated on the first call (two
  * complex64 grid cubes); with the fused strategy the first call also times
- * the candidate column block widths and keeps the fastest.  Synchronous on
- * the context's stream. */
+ * the candidate launch shapes of the passes and keeps the fastest.
+ * Synchronous on the context's stream. */
 int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *nsum,
                           const float *psf_peak, float *residual_out_dev, char **err);
 /* Transform strategy of later major cycles (frees the current workspace):
@@ -226,10 +226,11 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
  * No reference counterpart (the reference's DirtyProductFftPolicy picks its
  * host FFT backend, lib.rs:1280-1310). */
 int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
-/* Strategy in use (-1 before the first major cycle), the column block width
- * of the fused column passes (0 for hipFFT) and the timed forward + inverse
- * transform ms measured when the workspace was built (-1 if not timed). */
-int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *columns, double *transform_ms,
+/* Strategy in use (-1 before the first major cycle); shape[3] of the fused
+ * passes (columns per column-pass workgroup, its threads, rows per rows-pass
+ * workgroup; zeros for hipFFT); the timed forward + inverse transform ms of
+ * the chosen shape (-1 if not timed). */
+int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *shape, double *transform_ms,
                          char **err);
 
 /* ---- plain per-channel path (CHIP_F64 or CHIP_F32 grids, no binning) ------ */
