@@ -47,7 +47,9 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.cstride = gp.cells;
     gp.xstride = g.gy;
     gp.bstride = 0;
-    gp.blocked = 0;
+    gp.lower = 0;
+    gp.paired = 0;
+    gp.hy = static_cast<uint32_t>(g.gy / 2);
     gp.model_signed = 0;
     gp.sign_base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
     gp.taps = g.d_taps.as<float>();
@@ -428,7 +430,7 @@ void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, flo
 }
 
 // Residual pass of the major cycle over caller-laid-out grids (see GridParams
-// cstride / blocked / model_signed).
+// cstride / paired / model_signed).
 void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res) {
     if (vs.g->wproject)
         launch_binned_t<kModeResidual, true>(vs, gp, nullptr, model, nullptr, res, nullptr, vs.g->ctx->stream);

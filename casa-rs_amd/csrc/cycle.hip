@@ -205,9 +205,10 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
     gp.model_signed = 1;
 
     if (ws.strategy == CHIP_CYCLE_FUSED) {
-        gp.blocked = 1;
+        gp.paired = 1;
         gp.cstride = ws.xp.xg.gstride;
         gp.bstride = g.gx * xf::kBlock;
+        gp.lower = static_cast<uint64_t>(ws.xp.xg.nyb_h) * gp.bstride;
         // the half spectrum borrows the residual grid, the inverse rows the model grid
         xf::forward_model(ws.xp, model, Rg, M, st);
         CHIP_HIP(hipMemsetAsync(Rg, 0, ws.res_grid.bytes, st));
@@ -264,7 +265,7 @@ void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], doub
     if (vs.cycle->strategy == CHIP_CYCLE_FUSED) {
         shape[0] = xf::kBlock;
         shape[1] = xf::kThreads;
-        shape[2] = xf::kBlock;
+        shape[2] = xf::kRowsPerGroup;
     }
 }
 

@@ -205,12 +205,15 @@ struct GridParams {
     uint64_t cells;          // gx * gy
     // Grid addressing: cell (chan, x, y) is at chan*cstride + cell_offset(x, y):
     // public layout x*xstride + y (cstride = gx*gy, xstride = gy); the
-    // major-cycle workspace is column blocked, (y/8)*bstride + 8x + y%8
-    // (blocked = 1, bstride = 8*gx).
+    // major-cycle workspace uses mirror-paired blocks (paired = 1): column
+    // y <= hy at (y/8)*bstride + 8x + y%8, column y > hy by m = gy - y at
+    // lower + (m/8)*bstride + 8x + m%8, with bstride = 8*gx.
     uint64_t cstride;
     uint64_t xstride;
     uint64_t bstride;
-    uint32_t blocked;
+    uint64_t lower;
+    uint32_t paired;
+    uint32_t hy;             // gy / 2
     // Major cycle: the model grid is the raw forward transform of the
     // checkerboarded model, so (-1)^(x + y + sign_base) is applied on load.
     uint32_t model_signed;
@@ -222,8 +225,10 @@ struct GridParams {
 };
 
 __device__ __forceinline__ uint64_t cell_offset(const GridParams &gp, uint32_t x, uint32_t y) {
-    return gp.blocked ? static_cast<uint64_t>(y >> 3) * gp.bstride + x * 8u + (y & 7u)
-                      : static_cast<uint64_t>(x) * gp.xstride + y;
+    if (!gp.paired) return static_cast<uint64_t>(x) * gp.xstride + y;
+    if (y <= gp.hy) return static_cast<uint64_t>(y >> 3) * gp.bstride + x * 8u + (y & 7u);
+    const uint32_t m = 2 * gp.hy - y;
+    return gp.lower + static_cast<uint64_t>(m >> 3) * gp.bstride + x * 8u + (m & 7u);
 }
 
 GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint32_t tiles_y);

@@ -4,31 +4,39 @@
 // Forward (centered_fft2(apodize_model(model)), gridder.rs:1554-1592,
 // fft.rs:34): the model is real, so the 2-D transform is Hermitian,
 // B[-kx, -ky] = conj(B[kx, ky]).
-//   rows pass   : 8 image rows per workgroup; each apodised,
-//                 checkerboard-signed model row zero-padded to gy points ->
-//                 FFT along y; only ky in [0, gy/2] is kept (half spectrum).
+//   rows pass   : 8 image rows per workgroup; the apodised,
+//                 checkerboard-signed model rows zero-padded to gy points,
+//                 two real rows per complex FFT along y, split into their
+//                 spectra; only ky in [0, gy/2] is kept (half spectrum).
 //   column pass : one block of 8 columns ky <= gy/2 per workgroup; rows
 //                 outside the image are zero -> FFT along x; writes column
 //                 ky and its Hermitian mirror -ky, so the full grid comes
 //                 from half of the columns.
 // Inverse (centered inverse FFT + crop + correction, gridder.rs:1646-1715):
-//   column pass : one block of 8 columns per workgroup, IFFT along x; keeps
-//                 only the nx rows the crop needs (shift theorem: output row
+// only Re of the inverse is used, which is the inverse of the Hermitian part
+// H = (G[k] + conj G[-k]) / 2 of the residual grid G.
+//   column pass : one block of 8 columns ky <= gy/2 per workgroup, forms H
+//                 from the column and its mirror, IFFT along x; keeps only
+//                 the nx rows the crop needs (shift theorem: output row
 //                 (blc_x + x + gx/2) mod gx).
-//   rows pass   : 8 rows per workgroup, IFFT along y; the crop, 1/(gx gy)
-//                 scale, sign (-1)^(x + y + gx/2 + gy/2), grid correction
-//                 and normalisation are applied on the way out.
+//   rows pass   : 8 rows per workgroup, each Hermitian in ky, two rows per
+//                 complex IFFT along y (real and imaginary outputs); the
+//                 crop, 1/(gx gy) scale, sign (-1)^(x + y + gx/2 + gy/2),
+//                 grid correction and normalisation are applied on the way
+//                 out.
 //
-// Each workgroup (1024 threads) holds 8 transforms in LDS, 128 threads per
-// transform, and runs a mixed-radix (2, 3, 4, 5) Stockham autosort FFT.
+// A workgroup holds 8 (columns) or 4 (row pairs) transforms in LDS, 128
+// threads per transform, and runs a mixed-radix (2, 3, 4, 5) Stockham
+// autosort FFT.
 // Stage with radix r after sub-transforms of length ns: butterfly j reads
 // elements j + t n/r, multiplies input t by exp(-+2 pi i k t / (ns r)) with
 // k = j mod ns, and writes output t to (j - k) r + k + t ns.  Every stage
 // reads its inputs into registers, synchronises, and writes its outputs
 // back, so one LDS buffer suffices.  Twiddles come from a table exp(-2 pi i
-// m / n) rounded once from FP64 and staged in LDS.  Column tiles are
+// m / n) rounded once from FP64 and staged in LDS (for columns as two 64-way
+// split factors, so that two workgroups fit one CU).  Column tiles are
 // interleaved in LDS (element i of transform c at 8 i + c) to match the
-// blocked grid layout; row tiles are row-major (c n + i).
+// paired grid layout; row tiles are row-major (c n + i).
 #include <cmath>
 
 #include "device.hpp"
@@ -104,16 +112,27 @@ template <bool INV> __device__ __forceinline__ void bf5(float2 *a) {
 template <> __device__ __forceinline__ void butterfly<5, false>(float2 *a) { bf5<false>(a); }
 template <> __device__ __forceinline__ void butterfly<5, true>(float2 *a) { bf5<true>(a); }
 
-// One Stockham stage.  Thread (c, jt) handles butterflies j = jt + b * 128
+// Twiddle exp(-2 pi i m / n) from the LDS table: the full table, or (SPLIT)
+// hi + hi * d with hi = exp(-2 pi i 64 (m / 64) / n) and the small
+// d = exp(-2 pi i (m % 64) / n) - 1, both rounded once from FP64, which keeps
+// the product within about 0.7 ulp.
+template <bool SPLIT>
+__device__ __forceinline__ float2 twiddle(const float2 *tw, uint32_t m) {
+    if (!SPLIT) return tw[m];
+    const float2 hi = tw[64 + (m >> 6)], d = tw[m & 63];
+    return make_float2(fmaf(hi.x, d.x, fmaf(-hi.y, d.y, hi.x)), fmaf(hi.x, d.y, fmaf(hi.y, d.x, hi.y)));
+}
+
+// One Stockham stage.  Thread (c, jt) handles butterflies j = jt + b * LANES
 // of transform c; MB bounds the butterflies per thread.
-template <int R, bool INV, int MB, bool ILV>
+template <int R, bool INV, int MB, bool ILV, int LANES, bool SPLIT>
 __device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n, uint32_t pitch,
                                       uint32_t ns, uint32_t step, float inv_ns, uint32_t c, uint32_t jt) {
     const uint32_t nb = n / R;
     float2 v[MB][R];
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
-        const uint32_t j = jt + b * kLanes;
+        const uint32_t j = jt + b * LANES;
         if (j < nb) {
 #pragma unroll
             for (int t = 0; t < R; ++t) {
@@ -125,7 +144,7 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n,
     __syncthreads();
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
-        const uint32_t j = jt + b * kLanes;
+        const uint32_t j = jt + b * LANES;
         if (j < nb) {
             // j / ns, exact for n < 2^22: the fraction of (j + 1/2) / ns is at
             // least 1/(2 ns) away from an integer, the float error far smaller
@@ -134,7 +153,7 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n,
             if (k != 0) {
 #pragma unroll
                 for (int t = 1; t < R; ++t) {
-                    float2 w = tw[k * t * step];
+                    float2 w = twiddle<SPLIT>(tw, k * t * step);
                     if (INV) w.y = -w.y;
                     v[b][t] = cmul(v[b][t], w);
                 }
@@ -151,71 +170,106 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n,
     __syncthreads();
 }
 
-// E: elements per thread per transform (n <= 128 E).
-template <bool INV, int E, bool ILV>
+// E: elements per thread per transform (n <= LANES E).
+template <bool INV, int E, bool ILV, int LANES, bool SPLIT>
 __device__ __forceinline__ void run_fft(float2 *buf, const float2 *tw, const Axis &ax, uint32_t pitch,
                                         uint32_t c, uint32_t jt) {
     for (uint32_t s = 0; s < ax.n_stages; ++s) {
         const uint32_t r = ax.radix[s], ns = ax.ns[s], step = ax.step[s];
         const float inv = ax.inv_ns[s];
-        if (r == 5) stage<5, INV, (E + 4) / 5, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-        else if (r == 4) stage<4, INV, (E + 3) / 4, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-        else if (r == 3) stage<3, INV, (E + 2) / 3, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-        else stage<2, INV, (E + 1) / 2, ILV>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        if (r == 5) stage<5, INV, (E + 4) / 5, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        else if (r == 4) stage<4, INV, (E + 3) / 4, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        else if (r == 3) stage<3, INV, (E + 2) / 3, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
+        else stage<2, INV, (E + 1) / 2, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
     }
+}
+
+// Split twiddle table: the 64 low differences, then ceil(n / 64) high entries.
+__device__ __forceinline__ void load_split_twiddles(float2 *tw, const Axis &ax, uint32_t tid, uint32_t nthreads) {
+    const uint32_t nhi = (ax.n + 63) / 64;
+    for (uint32_t m = tid; m < 64 + nhi; m += nthreads) tw[m] = m < 64 ? ax.twd[m] : ax.tw[64 * (m - 64)];
 }
 
 // ---------------------------------------------------------------- kernels
-// Dynamic LDS: 8 transforms of the axis length, then its n twiddles.
-// Rows passes: grid (nxp / 8, n_chan); column passes: grid (y blocks, n_chan).
+// Rows passes: 8 rows per workgroup of kRowThreads, packed as 4 complex
+// transforms of two real rows each (row 2p real part, row 2p + 1 imaginary
+// part), row-major in LDS, then the full twiddle table; grid (nxp / 8,
+// n_chan).  Column passes: one block of 8 columns per workgroup of
+// kThreads, interleaved in LDS, then the split twiddle table (small enough
+// for two workgroups per CU); grid (nyb_h, n_chan).
 
-template <int E>
-__global__ __launch_bounds__(kThreads) void k_fwd_rows(XGeom xg, Axis ay, const float *__restrict__ model,
-                                                       float2 *__restrict__ half) {
-    extern __shared__ float2 buf[];
-    const uint32_t n = xg.gy;
-    float2 *tw = buf + kBlock * n;
-    const uint32_t tid = threadIdx.x, r = tid / kLanes, jt = tid % kLanes;
-    const uint32_t xb = blockIdx.x, c = blockIdx.y;
-    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ay.tw[m];
-    const uint32_t x = xb * kBlock + r;
-    const bool live = x < xg.nx;
-    const uint32_t gxx = xg.blc_x + x;
-    const float *row = model + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
-    const float cx = live ? xg.corr_x[gxx] : 0.0f;
-    const float sx = live && xg.wproject ? xg.sinc_x[gxx] : 1.0f;
-    for (uint32_t i = jt; i < n; i += kLanes) {
-        float v = 0.0f;
-        if (live && i >= xg.blc_y && i < xg.blc_y + xg.ny) {
-            // factor evaluated left to right as the reference: cx*cy [*sx*sy]
-            float f = cx * xg.corr_y[i];
-            if (xg.wproject) f = f * sx * xg.sinc_y[i];
-            v = row[i - xg.blc_y] * f;
-            if ((gxx + i) & 1) v = -v;
-        }
-        buf[r * n + i] = make_float2(v, 0.0f);
-    }
-    __syncthreads();
-    run_fft<false, E, false>(buf, tw, ay, n, r, jt);
-    // half spectrum, blocked [yb][x][8]: 8 rows x 8 columns contiguous per block
-    float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(xb) * kBlock * kBlock;
-    const uint32_t hy = n / 2;
-    for (uint32_t e = tid; e < xg.nyb_h * kBlock * kBlock; e += kThreads) {
-        const uint32_t yb = e >> 6, rr = (e >> 3) & 7, ky = yb * kBlock + (e & 7);
-        h[static_cast<uint64_t>(yb) * xg.nxp * kBlock + (e & 63)] =
-            ky <= hy ? buf[rr * n + ky] : make_float2(0.0f, 0.0f);
-    }
+__device__ __forceinline__ uint64_t half_at(const XGeom &xg, uint32_t yb, uint32_t x, uint32_t lane) {
+    return static_cast<uint64_t>(yb) * xg.nxp * kBlock + x * kBlock + lane;
 }
 
 template <int E>
-__global__ __launch_bounds__(kThreads) void k_fwd_cols(XGeom xg, Axis ax, const float2 *__restrict__ half,
-                                                       float2 *__restrict__ grid) {
+__global__ __launch_bounds__(kRowThreads) void k_fwd_rows(XGeom xg, Axis ay, const float *__restrict__ model,
+                                                          float2 *__restrict__ half) {
+    extern __shared__ float2 buf[];
+    const uint32_t n = xg.gy;
+    float2 *tw = buf + kPairs * n;
+    const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
+    const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
+    for (uint32_t m = tid; m < n; m += kRowThreads) tw[m] = ay.tw[m];
+    // z = a + i b for rows a = x0 + 2p, b = a + 1 (apodised, signed, zero-padded)
+    float f[2], sxv[2];
+    const float *row[2];
+    bool live[2];
+    uint32_t gxx[2];
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t x = x0 + 2 * p + q;
+        live[q] = x < xg.nx;
+        gxx[q] = xg.blc_x + x;
+        row[q] = model + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
+        f[q] = live[q] ? xg.corr_x[gxx[q]] : 0.0f;
+        sxv[q] = live[q] && xg.wproject ? xg.sinc_x[gxx[q]] : 1.0f;
+    }
+    for (uint32_t i = jt; i < n; i += kLanes) {
+        float v[2] = {0.0f, 0.0f};
+        if (i >= xg.blc_y && i < xg.blc_y + xg.ny) {
+            for (int q = 0; q < 2; ++q) {
+                if (!live[q]) continue;
+                // factor evaluated left to right as the reference: cx*cy [*sx*sy]
+                float fac = f[q] * xg.corr_y[i];
+                if (xg.wproject) fac = fac * sxv[q] * xg.sinc_y[i];
+                float val = row[q][i - xg.blc_y] * fac;
+                if ((gxx[q] + i) & 1) val = -val;
+                v[q] = val;
+            }
+        }
+        buf[p * n + i] = make_float2(v[0], v[1]);
+    }
+    __syncthreads();
+    run_fft<false, E, false, kLanes, false>(buf, tw, ay, n, p, jt);
+    // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i, k <= gy/2,
+    // into the half spectrum [yb][x][8]: 8 rows x 8 lanes contiguous per block
+    float2 *h = half + static_cast<uint64_t>(c) * xg.hstride;
+    const uint32_t hy = n / 2;
+    for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += kRowThreads) {
+        const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
+        const uint32_t k = yb * kBlock + lane;
+        float2 A = make_float2(0.0f, 0.0f), B = A;
+        if (k <= hy) {
+            const float2 z = buf[pp * n + k], zm = buf[pp * n + (k == 0 ? 0 : n - k)];
+            A = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+            B = make_float2(0.5f * (z.y + zm.y), 0.5f * (zm.x - z.x));
+        }
+        const uint32_t xa = x0 + 2 * pp;
+        h[half_at(xg, yb, xa, lane)] = A;
+        h[half_at(xg, yb, xa + 1, lane)] = B;
+    }
+}
+
+// Two workgroups per CU whenever the LDS allows it (transforms up to 1280).
+template <int E>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 10 ? 8 : 1))) void k_fwd_cols(
+    XGeom xg, Axis ax, const float2 *__restrict__ half, float2 *__restrict__ grid) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gx;
     float2 *tw = buf + kBlock * n;
     const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
-    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ax.tw[m];
+    load_split_twiddles(tw, ax, tid, kThreads);
     const float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
     for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
         const uint32_t i = e >> 3;
@@ -224,37 +278,51 @@ __global__ __launch_bounds__(kThreads) void k_fwd_cols(XGeom xg, Axis ax, const 
         buf[e] = v;
     }
     __syncthreads();
-    run_fft<false, E, true>(buf, tw, ax, 0, cc, jt);
-    float2 *g = grid + static_cast<uint64_t>(c) * xg.gstride;
-    const uint32_t hy = xg.gy / 2;
+    run_fft<false, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
+    // column ky into upper block yb; its mirror (row -kx, column gy - ky,
+    // i.e. m = ky) into lower block yb at the same lane
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
+    float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
+    float2 *lo = up + static_cast<uint64_t>(xg.nyb_h) * bstride;
+    const uint32_t hy = xg.gy / 2;
     for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
-        const uint32_t kx = e >> 3, ky = yb * kBlock + (e & 7);
+        const uint32_t kx = e >> 3, lane = e & 7, ky = yb * kBlock + lane;
         if (ky > hy) continue;
         const float2 v = buf[e];
-        g[yb * bstride + e] = v;
-        if (ky > 0 && ky < hy) {
-            const uint32_t my = xg.gy - ky, mx = kx == 0 ? 0u : n - kx;
-            g[(my >> 3) * bstride + mx * kBlock + (my & 7)] = make_float2(v.x, -v.y);
-        }
+        up[e] = v;
+        if (ky > 0 && ky < hy) lo[(kx == 0 ? 0u : n - kx) * kBlock + lane] = make_float2(v.x, -v.y);
     }
 }
 
 template <int E>
-__global__ __launch_bounds__(kThreads) void k_inv_cols(XGeom xg, Axis ax, const float2 *__restrict__ grid,
-                                                       float2 *__restrict__ rows) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 10 ? 8 : 1))) void k_inv_cols(
+    XGeom xg, Axis ax, const float2 *__restrict__ grid, float2 *__restrict__ rows) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gx;
     float2 *tw = buf + kBlock * n;
     const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
-    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ax.tw[m];
-    const float2 *g = grid + static_cast<uint64_t>(c) * xg.gstride + static_cast<uint64_t>(yb) * n * kBlock;
-    const bool full = (yb + 1) * kBlock <= xg.gy;
-    for (uint32_t e = tid; e < n * kBlock; e += kThreads)
-        buf[e] = full || yb * kBlock + (e & 7) < xg.gy ? g[e] : make_float2(0.0f, 0.0f);
+    load_split_twiddles(tw, ax, tid, kThreads);
+    // Hermitian part H[kx, ky] = (G[kx, ky] + conj G[-kx, -ky]) / 2 of the
+    // columns ky <= gy/2; its inverse transform is real and equals
+    // Re(centered inverse of G) after the shifts.
+    const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
+    const float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
+    const float2 *lo = up + static_cast<uint64_t>(xg.nyb_h) * bstride;
+    const uint32_t hy = xg.gy / 2;
+    for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
+        const uint32_t kx = e >> 3, lane = e & 7, ky = yb * kBlock + lane;
+        float2 v = make_float2(0.0f, 0.0f);
+        if (ky <= hy) {
+            const uint32_t mx = kx == 0 ? 0u : n - kx;
+            const float2 g = up[e];
+            const float2 m = (ky == 0 || ky == hy) ? up[mx * kBlock + lane] : lo[mx * kBlock + lane];
+            v = make_float2(0.5f * (g.x + m.x), 0.5f * (g.y - m.y));
+        }
+        buf[e] = v;
+    }
     __syncthreads();
-    run_fft<true, E, true>(buf, tw, ax, 0, cc, jt);
+    run_fft<true, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
     float2 *o = rows + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
     for (uint32_t e = tid; e < xg.nx * kBlock; e += kThreads) {
         uint32_t i = xg.blc_x + (e >> 3) + n / 2;
@@ -264,50 +332,61 @@ __global__ __launch_bounds__(kThreads) void k_inv_cols(XGeom xg, Axis ax, const 
 }
 
 template <int E>
-__global__ __launch_bounds__(kThreads) void k_inv_rows(XGeom xg, Axis ay, const float2 *__restrict__ rows,
-                                                       const float *__restrict__ nsum,
-                                                       const float *__restrict__ peak, float *__restrict__ out) {
+__global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, const float2 *__restrict__ rows,
+                                                          const float *__restrict__ nsum,
+                                                          const float *__restrict__ peak, float *__restrict__ out) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gy;
-    float2 *tw = buf + kBlock * n;
-    const uint32_t tid = threadIdx.x, r = tid / kLanes, jt = tid % kLanes;
-    const uint32_t xb = blockIdx.x, c = blockIdx.y;
-    for (uint32_t m = tid; m < n; m += kThreads) tw[m] = ay.tw[m];
-    const float2 *a = rows + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(xb) * kBlock * kBlock;
-    const uint32_t rows_live = xg.nx - xb * kBlock;  // rows of this block inside the image (>= 1)
-    for (uint32_t e = tid; e < xg.nyb * kBlock * kBlock; e += kThreads) {
-        const uint32_t yb = e >> 6, rr = (e >> 3) & 7, ky = yb * kBlock + (e & 7);
-        if (ky < n)
-            buf[rr * n + ky] = rr < rows_live ? a[static_cast<uint64_t>(yb) * xg.nxp * kBlock + (e & 63)]
-                                              : make_float2(0.0f, 0.0f);
+    float2 *tw = buf + kPairs * n;
+    const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
+    const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
+    for (uint32_t m = tid; m < n; m += kRowThreads) tw[m] = ay.tw[m];
+    // Rows a = x0 + 2p and b = a + 1 are Hermitian in ky: Z[k] = A[k] + i B[k]
+    // over the full length (A[-k] = conj A[k]), whose inverse is a + i b.
+    const float2 *h = rows + static_cast<uint64_t>(c) * xg.hstride;
+    const uint32_t hy = n / 2;
+    const uint32_t rows_live = xg.nx - x0;  // rows of this group inside the image (>= 1)
+    for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += kRowThreads) {
+        const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
+        const uint32_t k = yb * kBlock + lane;
+        if (k > hy) continue;
+        const uint32_t ra = 2 * pp;
+        const float2 zero = make_float2(0.0f, 0.0f);
+        const float2 A = ra < rows_live ? h[half_at(xg, yb, x0 + ra, lane)] : zero;
+        const float2 B = ra + 1 < rows_live ? h[half_at(xg, yb, x0 + ra + 1, lane)] : zero;
+        buf[pp * n + k] = make_float2(A.x - B.y, A.y + B.x);
+        if (k > 0 && k < hy) buf[pp * n + n - k] = make_float2(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    run_fft<true, E, false>(buf, tw, ay, n, r, jt);
-    const uint32_t x = xb * kBlock + r;
-    if (x >= xg.nx) return;
-    const uint32_t gxx = xg.blc_x + x;
+    run_fft<true, E, false, kLanes, false>(buf, tw, ay, n, p, jt);
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(xg.gx) * xg.gy);
     const float ns = nsum[c], pk = peak[c];
-    const float cx = xg.corr_x[gxx];
-    const float sincx = xg.wproject ? xg.sinc[gxx] : 1.0f;
-    float *o = out + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
-    for (uint32_t y = jt; y < xg.ny; y += kLanes) {
-        const uint32_t gyy = xg.blc_y + y;
-        uint32_t i = gyy + n / 2;
-        if (i >= n) i -= n;
-        float re = buf[r * n + i].x * scale;
-        if ((gxx + gyy + xg.base) & 1) re = -re;
-        const float factor = cx * xg.corr_y[gyy];
-        float v;
-        if (!xg.wproject) {
-            v = re * factor;  // gridder.rs:1646-1657
-            v = v / ns / pk;  // lib.rs:68928-68929
-        } else {
-            const float sf = sincx * xg.sinc[gyy];
-            v = fabsf(sf) > 1.0e-6f ? re * (factor / sf) : 0.0f;  // gridder.rs:1685-1715
-            v = 2.0f * v / ns / pk;                                // lib.rs:70960-70961
+    for (int q = 0; q < 2; ++q) {
+        const uint32_t x = x0 + 2 * p + q;
+        if (x >= xg.nx) break;
+        const uint32_t gxx = xg.blc_x + x;
+        const float cx = xg.corr_x[gxx];
+        const float sincx = xg.wproject ? xg.sinc[gxx] : 1.0f;
+        float *o = out + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
+        for (uint32_t y = jt; y < xg.ny; y += kLanes) {
+            const uint32_t gyy = xg.blc_y + y;
+            uint32_t i = gyy + n / 2;
+            if (i >= n) i -= n;
+            const float2 z = buf[p * n + i];
+            float re = (q == 0 ? z.x : z.y) * scale;
+            if ((gxx + gyy + xg.base) & 1) re = -re;
+            const float factor = cx * xg.corr_y[gyy];
+            float v;
+            if (!xg.wproject) {
+                v = re * factor;  // gridder.rs:1646-1657
+                v = v / ns / pk;  // lib.rs:68928-68929
+            } else {
+                const float sf = sincx * xg.sinc[gyy];
+                v = fabsf(sf) > 1.0e-6f ? re * (factor / sf) : 0.0f;  // gridder.rs:1685-1715
+                v = 2.0f * v / ns / pk;                                // lib.rs:70960-70961
+            }
+            o[y] = v;
         }
-        o[y] = v;
     }
 }
 
@@ -341,18 +420,32 @@ std::vector<float2> twiddles(uint32_t n) {
     return t;
 }
 
+std::vector<float2> twiddle_differences(uint32_t n) {
+    std::vector<float2> t(64);
+    for (uint32_t m = 0; m < 64; ++m) {
+        const double a = -2.0 * M_PI * static_cast<double>(m) / static_cast<double>(n);
+        // cos(a) - 1 = -2 sin^2(a / 2), without cancellation
+        const double s = std::sin(0.5 * a);
+        t[m] = make_float2(static_cast<float>(-2.0 * s * s), static_cast<float>(std::sin(a)));
+    }
+    return t;
+}
+
 namespace {
 
 int e_bucket(uint32_t n) {
     const uint32_t per = (n + kLanes - 1) / kLanes;
     if (per <= 4) return 4;
     if (per <= 8) return 8;
+    if (per <= 10) return 10;
     if (per <= 12) return 12;
     if (per <= 16) return 16;
     return 0;
 }
 
-size_t tile_lds(uint32_t n) { return static_cast<size_t>(kBlock + 1) * n * sizeof(float2); }
+// rows tile + full twiddle table; column tile + split twiddle table
+size_t row_lds(uint32_t n) { return static_cast<size_t>(kPairs + 1) * n * sizeof(float2); }
+size_t col_lds(uint32_t n) { return (static_cast<size_t>(kBlock) * n + 64 + (n + 63) / 64) * sizeof(float2); }
 
 template <class K> void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024)
@@ -360,20 +453,21 @@ template <class K> void allow_lds(K kernel, size_t bytes) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
 }
 
-#define CHIP_XF_CASE(KERNEL, EV, GDIM, LDS, ST, ...)                                              \
+#define CHIP_XF_CASE(KERNEL, EV, GDIM, NT, LDS, ST, ...)                                          \
     case EV: {                                                                                     \
         allow_lds(KERNEL<EV>, LDS);                                                                \
-        hipLaunchKernelGGL((KERNEL<EV>), GDIM, dim3(kThreads), LDS, ST, __VA_ARGS__);               \
+        hipLaunchKernelGGL((KERNEL<EV>), GDIM, dim3(NT), LDS, ST, __VA_ARGS__);                     \
         break;                                                                                     \
     }
 
-#define CHIP_XF_LAUNCH(E, KERNEL, GDIM, LDS, ST, ...)                                             \
+#define CHIP_XF_LAUNCH(E, KERNEL, GDIM, NT, LDS, ST, ...)                                         \
     do {                                                                                           \
         switch (E) {                                                                               \
-            CHIP_XF_CASE(KERNEL, 4, GDIM, LDS, ST, __VA_ARGS__)                                    \
-            CHIP_XF_CASE(KERNEL, 8, GDIM, LDS, ST, __VA_ARGS__)                                    \
-            CHIP_XF_CASE(KERNEL, 12, GDIM, LDS, ST, __VA_ARGS__)                                   \
-            CHIP_XF_CASE(KERNEL, 16, GDIM, LDS, ST, __VA_ARGS__)                                   \
+            CHIP_XF_CASE(KERNEL, 4, GDIM, NT, LDS, ST, __VA_ARGS__)                                \
+            CHIP_XF_CASE(KERNEL, 8, GDIM, NT, LDS, ST, __VA_ARGS__)                                \
+            CHIP_XF_CASE(KERNEL, 10, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
+            CHIP_XF_CASE(KERNEL, 12, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
+            CHIP_XF_CASE(KERNEL, 16, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
             default: fail(CHIP_ERR_INVALID, "transform too long for one workgroup");              \
         }                                                                                          \
         CHIP_HIP(hipGetLastError());                                                               \
@@ -388,7 +482,7 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
         return false;
     p.e_x = e_bucket(p.ax.n);
     p.e_y = e_bucket(p.ay.n);
-    if (!p.e_x || !p.e_y || tile_lds(p.ax.n) > lds_limit || tile_lds(p.ay.n) > lds_limit) return false;
+    if (!p.e_x || !p.e_y || col_lds(p.ax.n) > lds_limit || row_lds(p.ay.n) > lds_limit) return false;
     XGeom &xg = p.xg;
     xg.nx = static_cast<uint32_t>(g.geom.nx);
     xg.ny = static_cast<uint32_t>(g.geom.ny);
@@ -398,13 +492,12 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.blc_y = static_cast<uint32_t>(g.blc_y);
     xg.base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
     xg.wproject = g.wproject ? 1u : 0u;
-    xg.nxp = (xg.nx + kBlock - 1) / kBlock * kBlock;
-    xg.nyb = (xg.gy + kBlock - 1) / kBlock;
+    xg.nxp = (xg.nx + kRowsPerGroup - 1) / kRowsPerGroup * kRowsPerGroup;
     xg.nyb_h = (xg.gy / 2 + 1 + kBlock - 1) / kBlock;
-    // the half spectrum and the inverse rows borrow the grid buffers
-    if (xg.nxp > xg.gx) return false;
-    xg.gstride = static_cast<uint64_t>(xg.nyb) * xg.gx * kBlock;
-    xg.hstride = static_cast<uint64_t>(xg.nyb) * xg.nxp * kBlock;
+    // the half spectra borrow the grid buffers
+    if (xg.nxp > 2 * xg.gx) return false;
+    xg.gstride = 2ull * xg.nyb_h * xg.gx * kBlock;
+    xg.hstride = static_cast<uint64_t>(xg.nyb_h) * xg.nxp * kBlock;
     xg.corr_x = g.d_corr_x.as<float>();
     xg.corr_y = g.d_corr_y.as<float>();
     xg.sinc = g.wproject ? g.d_sinc.as<float>() : nullptr;
@@ -412,28 +505,33 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
     p.n_chan = static_cast<uint32_t>(n_chan);
     std::vector<float2> tx = twiddles(p.ax.n), ty = twiddles(p.ay.n);
+    std::vector<float2> dx = twiddle_differences(p.ax.n), dy = twiddle_differences(p.ay.n);
+    tx.insert(tx.end(), dx.begin(), dx.end());
+    ty.insert(ty.end(), dy.begin(), dy.end());
     p.tw_x.alloc(tx.size() * sizeof(float2));
     p.tw_y.alloc(ty.size() * sizeof(float2));
     CHIP_HIP(hipMemcpy(p.tw_x.p, tx.data(), p.tw_x.bytes, hipMemcpyHostToDevice));
     CHIP_HIP(hipMemcpy(p.tw_y.p, ty.data(), p.tw_y.bytes, hipMemcpyHostToDevice));
     p.ax.tw = p.tw_x.as<float2>();
     p.ay.tw = p.tw_y.as<float2>();
+    p.ax.twd = p.ax.tw + p.ax.n;
+    p.ay.twd = p.ay.tw + p.ay.n;
     return true;
 }
 
 void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st) {
-    const dim3 rows(p.xg.nxp / kBlock, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
-    const size_t row_lds = tile_lds(p.ay.n), col_lds = tile_lds(p.ax.n);
-    CHIP_XF_LAUNCH(p.e_y, k_fwd_rows, rows, row_lds, st, p.xg, p.ay, model, half);
-    CHIP_XF_LAUNCH(p.e_x, k_fwd_cols, cols, col_lds, st, p.xg, p.ax, half, grid);
+    const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
+    const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
+    CHIP_XF_LAUNCH(p.e_y, k_fwd_rows, rows, kRowThreads, rl, st, p.xg, p.ay, model, half);
+    CHIP_XF_LAUNCH(p.e_x, k_fwd_cols, cols, kThreads, cl, st, p.xg, p.ax, half, grid);
 }
 
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows_buf, const float *nsum,
                       const float *peak, float *out, hipStream_t st) {
-    const dim3 rows(p.xg.nxp / kBlock, p.n_chan), cols(p.xg.nyb, p.n_chan);
-    const size_t row_lds = tile_lds(p.ay.n), col_lds = tile_lds(p.ax.n);
-    CHIP_XF_LAUNCH(p.e_x, k_inv_cols, cols, col_lds, st, p.xg, p.ax, grid, rows_buf);
-    CHIP_XF_LAUNCH(p.e_y, k_inv_rows, rows, row_lds, st, p.xg, p.ay, rows_buf, nsum, peak, out);
+    const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
+    const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
+    CHIP_XF_LAUNCH(p.e_x, k_inv_cols, cols, kThreads, cl, st, p.xg, p.ax, grid, rows_buf);
+    CHIP_XF_LAUNCH(p.e_y, k_inv_rows, rows, kRowThreads, rl, st, p.xg, p.ay, rows_buf, nsum, peak, out);
 }
 
 }  // namespace xf

@@ -1,11 +1,15 @@
 // Fused image <-> uv-grid transforms of the major cycle (xform.hip).
 //
-// Grid layout of the major-cycle workspace ("column blocked"): cell (x, y)
-// of channel c is element
-//     c * gstride + (y / 8) * (gx * 8) + x * 8 + (y % 8),
-// i.e. blocks of 8 adjacent y columns, each block stored x-major.  A column
-// FFT over one block then reads and writes one contiguous run, and a rows
-// pass over 8 adjacent x rows moves 8 x 8 contiguous values per y block.
+// Grid layout of the major-cycle workspace ("mirror-paired blocks"): with
+// hy = gy/2, column y <= hy lives in upper block y / 8 at lane y % 8, and
+// column y > hy is stored by its mirror index m = gy - y in lower block m / 8
+// at lane m % 8; each block holds its 8 lanes for every x, x-major:
+//     c * gstride + block * (gx * 8) + x * 8 + lane,
+// lower blocks following the nyb_h upper ones.  A column and its Hermitian
+// partner -y then sit at the same lane of paired blocks, so the forward pass
+// writes a transformed column and its mirror contiguously, and the inverse
+// pass forms the Hermitian part of a residual column from two contiguous
+// reads.
 #pragma once
 
 #include "internal.hpp"
@@ -16,8 +20,11 @@ namespace xf {
 constexpr uint32_t kMaxStages = 16;
 constexpr uint32_t kBlock = 8;     // y columns per block = transforms per workgroup
 constexpr uint32_t kLogBlock = 3;
-constexpr uint32_t kThreads = 1024;
+constexpr uint32_t kThreads = 1024;              // column-pass workgroup: 8 transforms
 constexpr uint32_t kLanes = kThreads / kBlock;  // threads per transform
+constexpr uint32_t kRowsPerGroup = 8;            // rows-pass workgroup: 8 rows as 4 transforms
+constexpr uint32_t kPairs = kRowsPerGroup / 2;
+constexpr uint32_t kRowThreads = kPairs * kLanes;
 
 // One transform axis: length n = product of radix[0..n_stages) (2, 3, 4, 5),
 // with the per-stage constants of the Stockham recursion.
@@ -29,6 +36,7 @@ struct Axis {
     uint32_t step[kMaxStages];    // n / (ns * radix): twiddle index stride
     float inv_ns[kMaxStages];     // 1 / ns, for the exact float quotient j / ns
     const float2 *tw;             // n twiddles exp(-2 pi i m / n)
+    const float2 *twd;            // 64 differences exp(-2 pi i m / n) - 1
 };
 
 struct XGeom {
@@ -36,11 +44,9 @@ struct XGeom {
     uint32_t base;      // gx/2 + gy/2
     uint32_t wproject;
     uint32_t nxp;       // nx rounded up to a multiple of 8
-    uint32_t nyb;       // ceil(gy / 8): y blocks of a grid
-    uint32_t nyb_h;     // ceil((gy/2 + 1) / 8): y blocks of the half spectrum
-    uint32_t _pad;
-    uint64_t gstride;   // elements per channel of a blocked grid: nyb * gx * 8
-    uint64_t hstride;   // elements per channel of the half spectrum / inverse rows: nyb * nxp * 8
+    uint32_t nyb_h;     // ceil((gy/2 + 1) / 8): upper (= lower) blocks of a grid
+    uint64_t gstride;   // elements per channel of a paired grid: 2 * nyb_h * gx * 8
+    uint64_t hstride;   // elements per channel of a half spectrum [nyb_h][nxp][8]
     const float *corr_x, *corr_y;
     const float *sinc;            // W products: sinc of max(gx, gy) on both axes
     const float *sinc_x, *sinc_y; // W model apodisation: per-axis sinc
@@ -56,6 +62,7 @@ struct XPlan {
 
 bool factor_axis(uint32_t n, Axis &ax);
 std::vector<float2> twiddles(uint32_t n);
+std::vector<float2> twiddle_differences(uint32_t n);
 
 // False when a grid length does not factor into 2, 3, 5, a transform does
 // not fit the LDS limit or the per-thread register budget, or the image
@@ -65,12 +72,13 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
 // Elements of one blocked grid cube (all channels).
 inline uint64_t grid_elements(const XPlan &p) { return static_cast<uint64_t>(p.n_chan) * p.xg.gstride; }
 
-// model [chan][nx][ny] -> half spectrum (in `half`) -> blocked grid: the raw
+// model [chan][nx][ny] -> half spectrum (in `half`) -> paired grid: the raw
 // (unshifted) FFT of the checkerboard-signed, apodised, zero-padded model.
 void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st);
 
-// blocked residual grid -> inverse rows (in `rows`) -> residual image
-// [chan][nx][ny], corrected and normalised (nsum, peak: n_chan device floats).
+// paired residual grid -> Hermitian half of the x-inverse (in `rows`) ->
+// residual image [chan][nx][ny], corrected and normalised (nsum, peak: n_chan
+// device floats).
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows, const float *nsum,
                       const float *peak, float *out, hipStream_t st);
 
