@@ -206,6 +206,65 @@ def measured_traffic(workload):
         return None
 
 
+def _profile_csv(directory, suffix):
+    import glob
+    found = glob.glob(os.path.join(directory, "**", "*" + suffix), recursive=True)
+    if not found:
+        raise SystemExit(f"no *{suffix} under {directory}")
+    import csv
+    with open(found[0]) as f:
+        return list(csv.DictReader(f))
+
+
+def reduce_trace(directory, out_txt, command, top=40):
+    """rocprofv3 --kernel-trace --stats output -> short per-kernel table."""
+    rows = _profile_csv(directory, "kernel_stats.csv")
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    lines = [f"command: {command}", "",
+             f"{'calls':>7} {'avg_us':>10} {'total_ms':>10} {'pct':>6}  kernel"]
+    for r in rows[:top]:
+        lines.append(f"{int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:10.2f} "
+                     f"{float(r['TotalDurationNs']) / 1e6:10.2f} {float(r['Percentage']):6.2f}  {r['Name'][:140]}")
+    with open(out_txt, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def reduce_traffic(fetch_dir, write_dir, kernel, out_json, out_txt, commands):
+    """Two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) ->
+    HBM bytes per launch of `kernel`, each counter scaled by its reading for
+    the 1 GiB device copies that --calibrate-copy runs."""
+    res, lines = {}, [f"command: {c}" for c in commands] + [""]
+    for name, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        per = {}
+        for r in _profile_csv(d, "counter_collection.csv"):
+            if r["Counter_Name"] == counter:
+                per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        copies = sorted(v for k, vals in per.items() if "copy" in k.lower() for v in vals)[-3:]
+        if not copies:
+            raise SystemExit("no copy kernel dispatches: run the profiled bench with --calibrate-copy")
+        cal = sum(copies) / len(copies)
+        matches = [(k, v) for k, v in per.items() if kernel in k]
+        if not matches:
+            raise SystemExit(f"no dispatches of {kernel}")
+        kname, vals = max(matches, key=lambda kv: len(kv[1]))
+        raw = sum(vals) / len(vals)
+        scale = 1024.0 * 1024.0 / cal
+        res[name] = dict(counter=counter, kernel=kname[:160], dispatches=len(vals), kib_per_dispatch=raw,
+                         copy_1gib_kib=cal, scale=scale, bytes_per_dispatch=raw * scale * 1024.0)
+        lines.append(f"{counter}: {kname[:110]}")
+        lines.append(f"  {len(vals)} dispatches, mean {raw:.1f} KiB/dispatch; a 1 GiB copy reads as {cal:.1f} KiB "
+                     f"-> scale {scale:.4f}; scaled {raw * scale / 1024 / 1024:.4f} GiB/dispatch")
+    traffic = res["fetch"]["bytes_per_dispatch"] + res["write"]["bytes_per_dispatch"]
+    lines.append(f"HBM traffic per launch (scaled fetch + write): {traffic / 1e9:.4f} GB")
+    with open(out_json, "w") as f:
+        json.dump(dict(traffic_bytes_per_launch=traffic, fetch=res["fetch"], write=res["write"],
+                       commands=list(commands)), f, indent=1)
+    with open(out_txt, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,7 +280,19 @@ def main():
                     help="also time the step with hipFFT 2-D plans (reported beside value)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--reduce-trace", nargs=3, metavar=("DIR", "OUT_TXT", "COMMAND"),
+                    help="summarise a rocprofv3 --kernel-trace --stats run (no GPU work)")
+    ap.add_argument("--reduce-traffic", nargs=7,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL", "OUT_JSON", "OUT_TXT", "FETCH_CMD", "WRITE_CMD"),
+                    help="reduce rocprofv3 FETCH_SIZE / WRITE_SIZE runs to bytes per launch (no GPU work)")
     args = ap.parse_args()
+    if args.reduce_trace:
+        reduce_trace(*args.reduce_trace)
+        return
+    if args.reduce_traffic:
+        a = args.reduce_traffic
+        reduce_traffic(a[0], a[1], a[2], a[3], a[4], a[5:7])
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every GPU box here has one, and the sharded path "
               f"is described, not built (asked for --gpus {args.gpus})", file=sys.stderr)
