@@ -117,7 +117,7 @@ def make_cube(torch, wl, seed, dev):
     gen = torch.Generator(device=dev)
     gen.manual_seed(seed)
     flag = torch.rand((n_rows, n_chan), generator=gen, device=dev) < torch.tensor(frac, device=dev)[None, :]
-    weight = briggs_weights(torch, U, scale, flag, nx, cell, robust=0.0)
+    weight = torch.ones((n_rows, n_chan), dtype=torch.float32, device=dev)  # natural; Briggs below
     model = torch.zeros((n_chan, nx, nx), dtype=torch.float32, device=dev)
     # CLEAN-like point components at the source positions (x = l/cell + nx/2, y = ny/2 - m/cell)
     xs = np.clip(np.round(lpix + nx / 2).astype(int), 0, nx - 1)
@@ -126,34 +126,6 @@ def make_cube(torch, wl, seed, dev):
     return dict(uvw=U, scale=torch.tensor(scale, dtype=torch.float64, device=dev), vis=vis, flag=flag,
                 weight=weight, model=model, cell=cell, nx=nx, n_rows=n_rows, n_chan=n_chan,
                 freq=freq, max_uv=max_uv, max_w=float(np.max(np.abs(uvw[:, 2])) * scale[-1]))
-
-
-def briggs_weights(torch, U, scale, flag, nx, cell, robust):
-    """Briggs weights per channel from a density grid of natural weights on the
-    image-sized uv grid (cube convention: nearest cell, Hermitian mirror),
-    f2 = (5 10^-R)^2 / (sum d^2 / sum d), w = 1 / (d f2 + 1)
-    (weighting.rs:2440-2445, :2758-2777).  Data preparation, not timed."""
-    n_rows, n_chan = flag.shape
-    out = torch.empty((n_rows, n_chan), dtype=torch.float32, device=U.device)
-    uscale = nx * cell
-    for c in range(n_chan):
-        u = U[:, 0] * scale[c] * uscale
-        v = U[:, 1] * scale[c] * uscale
-        ok = ~flag[:, c]
-        dens = torch.zeros(nx * nx, dtype=torch.float32, device=U.device)
-        for sgn in (1.0, -1.0):
-            x = torch.round(sgn * u + nx / 2).long()
-            y = torch.round(-sgn * v + nx / 2).long()
-            inb = ok & (x > 0) & (y > 0) & (x < nx) & (y < nx)
-            dens.index_add_(0, (x * nx + y)[inb], torch.ones(int(inb.sum()), dtype=torch.float32, device=U.device))
-        sw = dens.double().sum()
-        s2 = (dens.double() ** 2).sum()
-        f2 = float((5.0 * 10.0 ** (-robust)) ** 2 / (s2 / sw)) if s2 > 0 else 0.0
-        x = torch.round(u + nx / 2).long().clamp(0, nx - 1)
-        y = torch.round(-v + nx / 2).long().clamp(0, nx - 1)
-        d = dens[x * nx + y]
-        out[:, c] = 1.0 / (d * np.float32(f2) + 1.0)
-    return out
 
 
 def cpu_baseline(pkg, g, data, cpu_seconds, log):
@@ -328,6 +300,15 @@ def main():
     sup, _, _ = g.wproject_tables()
     log(f"W kernels: {g.info.planes} planes, sampling {g.info.sampling}, width {g.info.width}, "
         f"supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
+    # Briggs robust 0 from one density grid per channel (library, cube convention)
+    natural = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
+    t0 = time.perf_counter()
+    wts, f2, _ = g.density_weights(natural, pkg.WEIGHT_BRIGGS, 0.0, per_plane=True, want_density=False)
+    torch.cuda.synchronize()
+    data["weight"] = wts.view(data["n_rows"], data["n_chan"])
+    del natural, wts
+    log(f"Briggs robust 0 weights from {n_chan} density grids ({time.perf_counter() - t0:.2f} s), "
+        f"f2 {f2.min():.3g}..{f2.max():.3g}")
     batch = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
     n_unflagged = int((~data["flag"]).sum())
     t0 = time.perf_counter()

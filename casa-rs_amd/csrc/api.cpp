@@ -213,6 +213,17 @@ int chip_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, float *
     });
 }
 
+int chip_density_weights(chip_gridder *g, const chip_samples *s, uint32_t mode, float robust,
+                         uint32_t per_plane, float *weight_out_dev, float *f2_host, float *density_dev,
+                         char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        density_weights(*g, *s, mode, robust, per_plane, weight_out_dev, f2_host, density_dev);
+    });
+}
+
 int chip_vset_create(chip_gridder *g, const chip_samples *s, chip_vset **out, char **err) {
     return guarded(err, [&] {
         if (!g || !s || !out) fail(CHIP_ERR_INVALID, "NULL argument");
@@ -326,7 +337,7 @@ int chip_grid_plain(chip_gridder *g, const chip_samples *s, uint32_t mode, uint3
 }
 
 int chip_degrid_plain(chip_gridder *g, const chip_samples *s, uint32_t precision,
-                      const void *model_dev, float *vis_out_dev, char **err) {
+                      const void *model_dev, void *vis_out_dev, char **err) {
     return guarded(err, [&] {
         if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
         validate_source(*s);

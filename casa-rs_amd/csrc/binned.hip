@@ -1,9 +1,11 @@
 // Binned FP32 gridding and degridding over the resident visibility stream.
 //
 // One workgroup per work item: a run of groups (samples with identical
-// plans) of one channel whose anchor cells fall in one 32x32 uv tile.  The
-// workgroup owns an LDS tile of (32 + 2*halo)^2 cells (halo = largest
-// half-support), accumulates into it, and finally adds the non-zero LDS cells
+// plans) of one channel whose anchor cells fall in one uv tile (32x32; 16x16
+// or 8x8 when a large W support would not fit the LDS).  The workgroup owns
+// an LDS tile of (tile + 2*halo)^2 cells (halo = largest half-support; above
+// 64 KB the kernel asks for the device's full LDS per workgroup), accumulates
+// into it, and finally adds the non-zero LDS cells
 // to the global grid with global float atomics (few per tile).  Degridding
 // reads a model tile staged in LDS.
 //
@@ -69,6 +71,12 @@ template <int MODE, bool WP>
 static size_t binned_lds_bytes(const GridParams &gp) {
     const size_t tap_floats = WP ? 0 : ((static_cast<size_t>(gp.tap_rows) * kTaps + 3) & ~size_t(3));
     return (tap_floats + static_cast<size_t>(LdsArrays<MODE>::n) * gp.L * gp.Ls) * sizeof(float);
+}
+
+size_t binned_tile_lds(const chip_gridder &g, uint32_t tile, uint32_t halo) {
+    const size_t L = tile + 2 * halo, Ls = L | 1u;
+    const size_t tap_floats = g.wproject ? 0 : ((static_cast<size_t>(g.os + 1) * kTaps + 3) & ~size_t(3));
+    return (tap_floats + 4 * L * Ls) * sizeof(float);
 }
 
 constexpr int kSlots = 2;  // taps per lane kept in registers (footprints up to 128 taps)
@@ -382,10 +390,13 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
                             const float2 *model, float *psf, float *acc, float2 *vis_out,
                             hipStream_t st) {
     const size_t lds = binned_lds_bytes<MODE, WP>(gp);
-    if (lds > 64 * 1024)
+    if (lds > lds_limit(vs.device))
         fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
                                    std::to_string(gp.halo) + ")");
     if (vs.n_items == 0) return;
+    if (lds > 64 * 1024)
+        CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     // Events on the launch stream bracket the kernel (chip_vset_stats).
     CHIP_HIP(hipEventRecord(vs.ev_start, st));
     hipLaunchKernelGGL((k_binned<MODE, WP>), dim3(static_cast<unsigned>(vs.n_items)), dim3(256),

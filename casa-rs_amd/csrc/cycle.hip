@@ -99,12 +99,6 @@ struct CycleWorkspace {
 
 namespace {
 
-size_t lds_limit(int device) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0)
-        return 64 * 1024;
-    return static_cast<size_t>(v);
-}
 
 // Forward + inverse transforms of the fused plan, in ms.  Writes scratch
 // values into the workspace and `out` (overwritten afterwards).

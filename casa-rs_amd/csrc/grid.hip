@@ -30,7 +30,7 @@ __device__ __forceinline__ void gadd(T *p, T v) {
 template <typename T, int MODE, bool WP>
 __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t chan,
                                                GridParams gp, const T *model, T *psf, T *acc,
-                                               float2 *vis_out, double *sums) {
+                                               T *vis_out, double *sums) {
     __shared__ double red[3][4];
     const uint64_t row = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     double nsum = 0.0, rsum = 0.0, cnt = 0.0;
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
         bool ok = sample_accepted(pp, smp, pl);
         const uint32_t flags = ok ? sample_flags(pp, smp) : 0u;
         if (ok && MODE == kModeResidual && !(flags & kResidOkBit)) ok = false;
-        if (MODE == kModeDegrid && !ok) vis_out[i] = make_float2(0.0f, 0.0f);
+        if (MODE == kModeDegrid && !ok) vis_out[2 * i] = vis_out[2 * i + 1] = T(0);
         if (ok) {
             const float gwf = grid_weight(pp, smp);
             if (pp.wproject) {
@@ -69,7 +69,8 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
                     }
                 }
                 if (MODE == kModeDegrid) {
-                    vis_out[i] = make_float2(static_cast<float>(pr), static_cast<float>(pi));
+                    vis_out[2 * i] = static_cast<T>(pr);
+                    vis_out[2 * i + 1] = static_cast<T>(pi);
                 } else {
                     T vr, vi;
                     if (MODE == kModeResidual) {
@@ -115,7 +116,8 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
                     }
                 }
                 if (MODE == kModeDegrid) {
-                    vis_out[i] = make_float2(static_cast<float>(pr), static_cast<float>(pi));
+                    vis_out[2 * i] = static_cast<T>(pr);
+                    vis_out[2 * i + 1] = static_cast<T>(pi);
                 } else {
                     T vr, vi;
                     if (MODE == kModeResidual) {
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
 
 template <typename T, int MODE>
 static void launch_plain_t(chip_gridder &g, const Source &src, const void *model, void *psf,
-                           void *acc, float *vis_out, double *sums) {
+                           void *acc, void *vis_out, double *sums) {
     const GridParams gp = grid_params(g, 32, 0, 1);
     const PlanParams pp = plan_params(g);
     hipStream_t st = g.ctx->stream;
@@ -176,11 +178,11 @@ static void launch_plain_t(chip_gridder &g, const Source &src, const void *model
         if (g.wproject)
             hipLaunchKernelGGL((k_plain<T, MODE, true>), dim3(blocks), dim3(256), 0, st, pp, src, c,
                                gp, static_cast<const T *>(model), static_cast<T *>(psf),
-                               static_cast<T *>(acc), reinterpret_cast<float2 *>(vis_out), sums);
+                               static_cast<T *>(acc), static_cast<T *>(vis_out), sums);
         else
             hipLaunchKernelGGL((k_plain<T, MODE, false>), dim3(blocks), dim3(256), 0, st, pp, src,
                                c, gp, static_cast<const T *>(model), static_cast<T *>(psf),
-                               static_cast<T *>(acc), reinterpret_cast<float2 *>(vis_out), sums);
+                               static_cast<T *>(acc), static_cast<T *>(vis_out), sums);
         CHIP_HIP(hipGetLastError());
     }
 }
@@ -231,7 +233,7 @@ void grid_plain(chip_gridder &g, const Source &src, uint32_t mode, uint32_t prec
 }
 
 void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,
-                  float *out) {
+                  void *out) {
     if (!model || !out) fail(CHIP_ERR_INVALID, "degrid needs a model grid and an output");
     if (precision == CHIP_F64)
         launch_plain_t<double, kModeDegrid>(g, src, model, nullptr, nullptr, out, nullptr);

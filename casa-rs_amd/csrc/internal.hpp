@@ -174,13 +174,28 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
 void grid_plain(chip_gridder &g, const Source &src, uint32_t mode, uint32_t precision,
                 const void *model, void *psf, void *dirty, chip_sums *sums);
 void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,
-                  float *out);
+                  void *out);
 
 void finish_products(chip_gridder &g, uint32_t precision, uint64_t batch, void *psf, void *dirty,
                      const double *nsumwt, float *psf_out, float *res_out, float *peaks);
 void model_to_grid(chip_gridder &g, uint32_t precision, uint64_t batch, const float *model,
                    void *grid_out);
 void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *image_out);
+void density_weights(chip_gridder &g, const chip_samples &cs, uint32_t mode, float robust, uint32_t per_plane,
+                     float *weight_out, float *f2_host, float *density_out);
+
+// Largest dynamic LDS one workgroup may use on `device` (160 KB on gfx950).
+inline size_t lds_limit(int device) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0)
+        return 64 * 1024;
+    return static_cast<size_t>(v);
+}
+
+// LDS bytes of the binned kernel's tile for the most demanding mode
+// (four float arrays of (tile + 2 halo) x ((tile + 2 halo) | 1) cells, plus
+// the standard tap table).
+size_t binned_tile_lds(const chip_gridder &g, uint32_t tile, uint32_t halo);
 
 // Major-cycle workspace (cycle.hip): grids and FFT plans kept with the set.
 struct CycleWorkspace;

@@ -316,7 +316,13 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     vs.n_samples = n;
     vs.n_chan = src.n_chan;
     vs.halo = g.wproject ? g.max_support : static_cast<uint32_t>(kSupport);
+    // 32x32 tiles; smaller tiles when a large W support would not leave the
+    // (tile + 2 halo)^2 LDS tile inside one workgroup's LDS
+    const size_t lds = lds_limit(vs.device);
     vs.tile = 32;
+    while (vs.tile > 8 && binned_tile_lds(g, vs.tile, vs.halo) > lds) vs.tile /= 2;
+    if (binned_tile_lds(g, vs.tile, vs.halo) > lds)
+        fail(CHIP_ERR_INVALID, "W support " + std::to_string(vs.halo) + " too large for an LDS tile");
     vs.tiles_x = static_cast<uint32_t>((g.gx + vs.tile - 1) / vs.tile);
     vs.tiles_y = static_cast<uint32_t>((g.gy + vs.tile - 1) / vs.tile);
     const uint64_t n_tiles = static_cast<uint64_t>(vs.tiles_x) * vs.tiles_y;

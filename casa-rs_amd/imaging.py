@@ -225,6 +225,21 @@ class StandardGridder:
                  ker.ctypes.data_as(ctypes.c_void_p), nt.ctypes.data_as(ctypes.c_void_p))
         return sup, ker, nt
 
+    def density_weights(self, batch, mode=abi.WEIGHT_BRIGGS, robust=0.0, per_plane=True, want_density=True):
+        """Uniform / Briggs weights of every sample (input order) from density
+        grids of image size (weighting.rs:1180-1232).  Returns (weights, f2,
+        density): float32 weights on the device, f2 per density plane, and the
+        density grids (planes, nx, ny) or None."""
+        s, keep = batch.to_samples()
+        planes = s.n_chan if per_plane else 1
+        dev = self.ctx.torch_device
+        out = torch.empty(s.n_rows * s.n_chan, dtype=torch.float32, device=dev)
+        dens = torch.empty((planes, self.nx, self.ny), dtype=torch.float32, device=dev) if want_density else None
+        f2 = np.zeros(planes, np.float32)
+        abi.call("chip_density_weights", self.handle, ctypes.byref(s), int(mode), ctypes.c_float(robust),
+                 1 if per_plane else 0, _ptr(out), f2.ctypes.data_as(ctypes.c_void_p), _ptr(dens))
+        return out, f2, dens
+
     def plan(self, batch):
         """Per-sample FP64 plans (int32 (n, 6)) and W normalisations."""
         s, keep = batch.to_samples()
@@ -251,8 +266,10 @@ class StandardGridder:
         return [x.as_dict() for x in sums]
 
     def degrid_plain(self, batch, model, precision=abi.F64):
+        """Plain per-channel degridding: complex128 results for F64, complex64 for F32."""
         s, keep = batch.to_samples()
-        out = torch.empty(s.n_rows * s.n_chan, dtype=torch.complex64, device=self.ctx.torch_device)
+        dtype = torch.complex128 if precision == abi.F64 else torch.complex64
+        out = torch.empty(s.n_rows * s.n_chan, dtype=dtype, device=self.ctx.torch_device)
         abi.call("chip_degrid_plain", self.handle,ctypes.byref(s), precision, _ptr(model), _ptr(out))
         return out
 

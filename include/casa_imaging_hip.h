@@ -45,6 +45,10 @@ extern "C" {
 #define CHIP_MODE_DIRTY 1    /* PSF + dirty (V*w) grids */
 #define CHIP_MODE_RESIDUAL 2 /* degrid model grid, grid (V - model)*w */
 
+/* Density weighting modes of chip_density_weights (WeightingMode, types.rs:145-160). */
+#define CHIP_WEIGHT_UNIFORM 1
+#define CHIP_WEIGHT_BRIGGS 2
+
 /* Transform strategies of chip_vset_major_cycle. */
 #define CHIP_CYCLE_AUTO (-1)
 #define CHIP_CYCLE_HIPFFT 0
@@ -156,6 +160,23 @@ int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t exp
 int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels,
                          float *norm_table, char **err);
 
+/* ---- density weighting -------------------------------------------------- */
+/* Uniform / Briggs reweighting from a density grid of image size
+ * (apply_weighting_to_owned_batches_by_sample_groups, weighting.rs:1180-1232;
+ * density build :1747-1856; reweight :2501-2592; cells gridder.rs:1888-1934).
+ * per_plane != 0 (WeightDensityMode::PerPlane): one density grid per channel,
+ * cube convention (Double coordinates rounded; built with the conjugate
+ * samples, looked up without); per_plane == 0: one combined grid over every
+ * sample, VisImagingWeight convention (Float coordinates truncated).
+ * weight_out_dev: one float per sample in input order: w / d (uniform) or
+ * w / (d * f2 + 1) (Briggs, f2 = (5 * 10^-robust)^2 / (sum d^2 / sum d));
+ * samples off the density grid or with a non-positive weight get 0; flagged
+ * or non-gridable samples keep their weight.  f2_host (may be NULL): one value
+ * per density plane; density_dev (may be NULL): planes x nx x ny floats. */
+int chip_density_weights(chip_gridder *g, const chip_samples *s, uint32_t mode, float robust,
+                         uint32_t per_plane, float *weight_out_dev, float *f2_host, float *density_dev,
+                         char **err);
+
 /* ---- FP64 planning ------------------------------------------------------ */
 /* Per-sample plan, bit-exact with the reference CPU planner:
  *  standard (no W attached): plan_positive_taps (gridder.rs:557-566, :1958-1974)
@@ -240,8 +261,10 @@ int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *shape, do
 int chip_grid_plain(chip_gridder *g, const chip_samples *s, uint32_t mode, uint32_t precision,
                     const void *model_dev, void *psf_dev, void *dirty_dev, chip_sums *sums_host,
                     char **err);
+/* Plain degridding; vis_out_dev is complex128 for CHIP_F64 (FP64 MACs end to
+ * end), complex64 for CHIP_F32; rejected samples get 0. */
 int chip_degrid_plain(chip_gridder *g, const chip_samples *s, uint32_t precision,
-                      const void *model_dev, float *vis_out_dev, char **err);
+                      const void *model_dev, void *vis_out_dev, char **err);
 
 /* ---- grid <-> image ------------------------------------------------------ */
 /* Dirty products (dirty_grids_to_psf_and_residual_owned, lib.rs:48987-49131;

@@ -141,6 +141,18 @@ def _declare(L):
     L.co_w_degrid.argtypes = [ctypes.POINTER(_WProj), _U64, _P, _P, _P, _P, _P]
     L.co_direct_adjoint.restype = None
     L.co_direct_adjoint.argtypes = [_U64, _U64, _D, _D, _U64, _P, _P, _P, _P, ctypes.c_int, _P]
+    L.co_density_cell.restype = ctypes.c_int
+    L.co_density_cell.argtypes = [_U64, _U64, _D, _D, ctypes.c_int, _D, _D, _P, _P]
+    L.co_density_grid.restype = None
+    L.co_density_grid.argtypes = [_U64, _U64, _D, _D, ctypes.c_int, ctypes.c_int, _U64, _P, _P, _P, _P, _P, _P]
+    L.co_briggs_f2.restype = ctypes.c_float
+    L.co_briggs_f2.argtypes = [_U64, _P, ctypes.c_float, ctypes.c_int]
+    L.co_density_reweight.restype = None
+    L.co_density_reweight.argtypes = [_U64, _U64, _D, _D, ctypes.c_int, ctypes.c_int, _U64, _P, _P, _P, _P, _P,
+                                      _P, _P, _P]
+    L.co_direct_adjoint_window.restype = None
+    L.co_direct_adjoint_window.argtypes = [_U64, _U64, _D, _D, _U64, _U64, _U64, _U64, _U64, _P, _P, _P, _P,
+                                           ctypes.c_int, _P]
     L.co_direct_predict.restype = None
     L.co_direct_predict.argtypes = [_U64, _U64, _D, _D, _P, _U64, _P, _P, _P, _P]
 
@@ -647,6 +659,16 @@ def direct_adjoint(nx, ny, cell_x, cell_y, u, v, w, values, threads=None, image=
     return image
 
 
+def direct_adjoint_window(nx, ny, cell_x, cell_y, x0, y0, wx, wy, u, v, w, values, threads=None):
+    """direct_adjoint restricted to the pixels [x0, x0+wx) x [y0, y0+wy)."""
+    u, v, w = _f64(u), _f64(v), _f64(w)
+    vals = _c64_as_f32(values)
+    image = np.zeros((wx, wy), np.float32)
+    lib().co_direct_adjoint_window(nx, ny, cell_x, cell_y, x0, y0, wx, wy, len(u), _ptr(u), _ptr(v),
+                                   _ptr(w), _ptr(vals), threads or default_threads(), _ptr(image))
+    return image
+
+
 def direct_predict(nx, ny, cell_x, cell_y, model, u, v, w):
     """direct_predict_visibility for every sample (lib.rs:73870-73887)."""
     u, v, w = _f64(u), _f64(v), _f64(w)
@@ -681,3 +703,55 @@ def direct_psf_and_residual(nx, ny, cell_x, cell_y, u, v, w, weight, sumwt_facto
     img = direct_adjoint(nx, ny, cell_x, cell_y, u[ok], v[ok], w[ok], resid, threads)
     img = ((img / np.float32(nsum)) / peak).astype(np.float32)
     return psf, img, peak, nsum
+
+
+# ---------------------------------------------------------------------------
+# density weighting (weighting.rs)
+# ---------------------------------------------------------------------------
+
+VIS_IMAGING_WEIGHT = 0   # DensityCellConvention::VisImagingWeight (Float ops, truncation)
+CUBE_BRIGGS = 1          # CubeBriggsWeightorDensity / Lookup (Double, round half away)
+WEIGHT_UNIFORM = 1
+WEIGHT_BRIGGS = 2
+
+
+def density_cell(nx, ny, cell_x, cell_y, convention, u, v):
+    """weight_density_cell_anchor (gridder.rs:1888-1934): (x, y) or None."""
+    x, y = ctypes.c_int64(), ctypes.c_int64()
+    ok = lib().co_density_cell(nx, ny, cell_x, cell_y, convention, float(u), float(v), ctypes.byref(x),
+                               ctypes.byref(y))
+    return (x.value, y.value) if ok else None
+
+
+def density_weights(nx, ny, cell_x, cell_y, u, v, weight, mode, robust=0.0, plane=None, n_planes=1,
+                    usable=None):
+    """Uniform / Briggs reweighting of every sample (weighting.rs:1180-1232).
+
+    plane None: one combined density (VisImagingWeight convention, conjugates
+    included); else per-plane densities (cube convention: build with
+    conjugates, look up without).  Returns (weights, density, f2)."""
+    u, v = _f64(u), _f64(v)
+    weight = _f32(weight)
+    n = len(u)
+    per_plane = plane is not None
+    conv = CUBE_BRIGGS if per_plane else VIS_IMAGING_WEIGHT
+    pl = None if plane is None else np.ascontiguousarray(plane, dtype=np.uint32)
+    us = None if usable is None else np.ascontiguousarray(usable, dtype=np.uint8)
+    planes = n_planes if per_plane else 1
+    density = np.zeros((planes, nx, ny), np.float32)
+    lib().co_density_grid(nx, ny, cell_x, cell_y, conv, 1, n, _ptr(u), _ptr(v), _ptr(weight),
+                          None if pl is None else _ptr(pl), None if us is None else _ptr(us), _ptr(density))
+    f2 = np.ones(planes, np.float32)
+    if mode == WEIGHT_BRIGGS:
+        for p in range(planes):
+            f2[p] = lib().co_briggs_f2(nx * ny, _ptr(density[p]), ctypes.c_float(robust), 1)
+    out = np.empty(n, np.float32)
+    lib().co_density_reweight(nx, ny, cell_x, cell_y, conv, mode, n, _ptr(u), _ptr(v), _ptr(weight),
+                              None if pl is None else _ptr(pl), None if us is None else _ptr(us),
+                              _ptr(density), _ptr(f2), _ptr(out))
+    return out, density, f2
+
+
+def briggs_f2(density, robust, includes_conjugates=True):
+    d = np.ascontiguousarray(density, np.float32)
+    return float(lib().co_briggs_f2(d.size, _ptr(d), ctypes.c_float(robust), 1 if includes_conjugates else 0))

@@ -700,7 +700,8 @@ void co_w_degrid(const co_wproj *p, uint64_t n, const double *u, const double *v
 typedef struct {
     uint64_t nx, ny;
     double cell_x, cell_y;
-    uint64_t begin, end; /* pixel range */
+    uint64_t x0, y0, wy;  /* window origin and height: pixel p is (x0 + p / wy, y0 + p % wy) */
+    uint64_t begin, end; /* pixel range of the window */
     uint64_t n;
     const double *u, *v, *w;
     const float *value; /* complex per sample */
@@ -723,7 +724,7 @@ static void *co_dft_adjoint_worker(void *arg) {
     co_dft_job *j = (co_dft_job *)arg;
     const double TAU = 6.28318530717958647692;
     for (uint64_t p = j->begin; p < j->end; ++p) {
-        uint64_t x = p / j->ny, y = p % j->ny;
+        uint64_t x = j->x0 + p / j->wy, y = j->y0 + p % j->wy;
         double l, m, nm1;
         co_pixel_lmn(j->nx, j->ny, j->cell_x, j->cell_y, x, y, &l, &m, &nm1);
         float acc = j->image[p];
@@ -738,17 +739,21 @@ static void *co_dft_adjoint_worker(void *arg) {
     return NULL;
 }
 
-void co_direct_adjoint(uint64_t nx, uint64_t ny, double cell_x, double cell_y, uint64_t n,
-                       const double *u, const double *v, const double *w, const float *value,
-                       int threads, float *image) {
+/* The pixels of the window [x0, x0 + wx) x [y0, y0 + wy) of an nx x ny image;
+ * image holds wx * wy values, y fastest. */
+void co_direct_adjoint_window(uint64_t nx, uint64_t ny, double cell_x, double cell_y, uint64_t x0,
+                              uint64_t y0, uint64_t wx, uint64_t wy, uint64_t n, const double *u,
+                              const double *v, const double *w, const float *value, int threads,
+                              float *image) {
     if (threads < 1) threads = 1;
-    uint64_t pixels = nx * ny;
+    uint64_t pixels = wx * wy;
     uint64_t chunk = (pixels + threads - 1) / threads;
     co_dft_job *jobs = (co_dft_job *)calloc(threads, sizeof(co_dft_job));
     pthread_t *tids = (pthread_t *)calloc(threads, sizeof(pthread_t));
     for (int t = 0; t < threads; ++t) {
         co_dft_job *j = &jobs[t];
         j->nx = nx; j->ny = ny; j->cell_x = cell_x; j->cell_y = cell_y;
+        j->x0 = x0; j->y0 = y0; j->wy = wy;
         j->begin = (uint64_t)t * chunk;
         j->end = j->begin + chunk < pixels ? j->begin + chunk : pixels;
         if (j->begin > pixels) j->begin = pixels;
@@ -758,6 +763,12 @@ void co_direct_adjoint(uint64_t nx, uint64_t ny, double cell_x, double cell_y, u
     for (int t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
     free(jobs);
     free(tids);
+}
+
+void co_direct_adjoint(uint64_t nx, uint64_t ny, double cell_x, double cell_y, uint64_t n,
+                       const double *u, const double *v, const double *w, const float *value,
+                       int threads, float *image) {
+    co_direct_adjoint_window(nx, ny, cell_x, cell_y, 0, 0, nx, ny, n, u, v, w, value, threads, image);
 }
 
 /* direct_predict_visibility, lib.rs:73870-73887: components are the non-zero
@@ -780,5 +791,120 @@ void co_direct_predict(uint64_t nx, uint64_t ny, double cell_x, double cell_y, c
         }
         out[2 * i] = re;
         out[2 * i + 1] = im;
+    }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Density (uniform / Briggs) weighting                                      */
+/* ------------------------------------------------------------------------- */
+
+/* weight_density_cell_anchor, gridder.rs:1888-1934, with the density origin
+ * and scale of StandardGridder::new_with_padding_factor (gridder.rs:421-441):
+ * center = n / 2, scale = n * cell, limit = n.
+ * convention 0 (VisImagingWeight): u, v, scale and origin as Float, the product
+ *   and the sum as separate Float operations, x from -u, then truncation;
+ * convention 1 (cube Briggs): x = u*scale + center, y = -v*scale + center in
+ *   Double, rounded half away from zero.
+ * Accepted cells satisfy 0 < x < nx and 0 < y < ny.  Returns 1 if accepted. */
+int co_density_cell(uint64_t nx, uint64_t ny, double cell_x, double cell_y, int convention, double u,
+                    double v, int64_t *ax, int64_t *ay) {
+    const double cx = (double)nx / 2.0, cy = (double)ny / 2.0;
+    const double su = (double)nx * cell_x, sv = (double)ny * cell_y;
+    double x, y;
+    if (convention == 0) {
+        const float uf = (float)u, vf = (float)v;
+        const float xs = (-uf) * (float)su;
+        const float ys = vf * (float)sv;
+        const float xf = xs + (float)cx;
+        const float yf = ys + (float)cy;
+        x = (double)xf;
+        y = (double)yf;
+    } else {
+        x = u * su + cx;
+        y = -v * sv + cy;
+    }
+    if (!(isfinite(x) && isfinite(y))) return 0;
+    if (convention != 0) {
+        x = round(x);
+        y = round(y);
+    }
+    /* Rust's `as isize` truncates toward zero and saturates: anything this far
+     * out is rejected by the bounds below either way */
+    if (x >= 9.0e18 || x <= -9.0e18 || y >= 9.0e18 || y <= -9.0e18) return 0;
+    const int64_t X = (int64_t)x, Y = (int64_t)y;
+    if (X <= 0 || Y <= 0 || X >= (int64_t)nx || Y >= (int64_t)ny) return 0;
+    *ax = X;
+    *ay = Y;
+    return 1;
+}
+
+/* accumulate_density_grid_serial / accumulate_density_sample_serial
+ * (weighting.rs:1747-1856): samples in order, weight added to the sample's
+ * cell and, with mirror, to the cell of (-u, -v), in Float.  plane[i] selects
+ * the density plane of sample i (NULL: plane 0); usable[i] == 0 skips it. */
+void co_density_grid(uint64_t nx, uint64_t ny, double cell_x, double cell_y, int convention, int mirror,
+                     uint64_t n, const double *u, const double *v, const float *weight,
+                     const uint32_t *plane, const uint8_t *usable, float *density) {
+    for (uint64_t i = 0; i < n; ++i) {
+        if (usable && !usable[i]) continue;
+        const float w = weight[i];
+        if (!(isfinite(w) && w > 0.0f)) continue;
+        float *d = density + (plane ? (uint64_t)plane[i] * nx * ny : 0);
+        int64_t x, y;
+        if (co_density_cell(nx, ny, cell_x, cell_y, convention, u[i], v[i], &x, &y)) d[x * ny + y] += w;
+        if (mirror && co_density_cell(nx, ny, cell_x, cell_y, convention, -u[i], -v[i], &x, &y))
+            d[x * ny + y] += w;
+    }
+}
+
+/* Briggs f2 of one density plane (weighting.rs:1211-1224, :2758-2777):
+ * (5 * 10^-robust)^2 / (sum of squares of the positive cells / total), the
+ * total being the plane's sum, or twice it when the density holds no
+ * conjugate samples (robust_density_weight_sum_for_f2, :1493-1501). */
+float co_briggs_f2(uint64_t cells, const float *density, float robust, int includes_conjugates) {
+    double sum = 0.0, sumsq = 0.0;
+    for (uint64_t i = 0; i < cells; ++i) {
+        sum += (double)density[i];
+        if (density[i] > 0.0f) sumsq += (double)density[i] * (double)density[i];
+    }
+    const double total = includes_conjugates ? sum : 2.0 * sum;
+    if (!(sumsq > 0.0 && total > 0.0)) return 0.0f;
+    const double s = 5.0 * pow(10.0, -(double)robust);
+    return (float)(s * s / (sumsq / total));
+}
+
+/* reweight_owned_batch_uniform_in_place / _briggs_in_place
+ * (weighting.rs:2501-2592) with casa_briggs_denominator (:2440-2443):
+ * uniform w / d, Briggs w / (d * f2 + 1) as separate Float operations; a
+ * sample off the density grid, or with a non-positive weight or density,
+ * gets 0.  mode 1 uniform, 2 Briggs; f2 per plane. */
+void co_density_reweight(uint64_t nx, uint64_t ny, double cell_x, double cell_y, int convention, int mode,
+                         uint64_t n, const double *u, const double *v, const float *weight,
+                         const uint32_t *plane, const uint8_t *usable, const float *density,
+                         const float *f2, float *out) {
+    for (uint64_t i = 0; i < n; ++i) {
+        const float w = weight[i];
+        if (usable && !usable[i]) {
+            out[i] = w;
+            continue;
+        }
+        const uint64_t p = plane ? plane[i] : 0;
+        int64_t x, y;
+        if (!co_density_cell(nx, ny, cell_x, cell_y, convention, u[i], v[i], &x, &y)) {
+            out[i] = 0.0f;
+            continue;
+        }
+        const float d = density[p * nx * ny + (uint64_t)x * ny + (uint64_t)y];
+        if (!(isfinite(w) && w > 0.0f && isfinite(d) && d > 0.0f)) {
+            out[i] = 0.0f;
+            continue;
+        }
+        if (mode == 1) {
+            out[i] = w / d;
+        } else {
+            const float product = d * f2[p];
+            const float denominator = product + 1.0f;
+            out[i] = w / denominator;
+        }
     }
 }

@@ -24,7 +24,8 @@ def test_header_constants_match_binding(pkg):
     text = open(HEADER).read()
     defs = dict(re.findall(r"#define (CHIP_[A-Z0-9_]+) \(?(-?\d+)\)?", text))
     for name in ("MODE_PSF", "MODE_DIRTY", "MODE_RESIDUAL", "F32", "F64", "PAD_COMPOSITE_1_2",
-                 "PAD_COMPOSITE_1_3", "PAD_NONE", "CYCLE_AUTO", "CYCLE_HIPFFT", "CYCLE_FUSED"):
+                 "PAD_COMPOSITE_1_3", "PAD_NONE", "CYCLE_AUTO", "CYCLE_HIPFFT", "CYCLE_FUSED",
+                 "WEIGHT_UNIFORM", "WEIGHT_BRIGGS"):
         assert int(defs["CHIP_" + name]) == getattr(pkg.abi, name), name
 
 

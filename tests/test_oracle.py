@@ -242,3 +242,94 @@ def test_direct_dft_matches_gridded_dirty_image(oracle_mod):
     assert rms < 3e-2
     px = np.unravel_index(np.argmax(res), res.shape)
     assert px == (nx // 2 + 5, nx // 2 + 3)
+
+
+def test_direct_adjoint_window_matches_full_image(oracle_mod):
+    """The windowed direct adjoint is the same sum restricted to a window."""
+    o = oracle_mod
+    rng = np.random.default_rng(13)
+    n, nx, ny = 500, 24, 20
+    cell = np.deg2rad(30.0 / 3600.0)
+    u, v, w = rng.uniform(-800, 800, n), rng.uniform(-800, 800, n), rng.uniform(-50, 50, n)
+    vals = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64)
+    full = o.direct_adjoint(nx, ny, cell, cell, u, v, w, vals)
+    win = o.direct_adjoint_window(nx, ny, cell, cell, 5, 3, 11, 9, u, v, w, vals)
+    assert np.array_equal(win, full[5:16, 3:12])
+
+
+# ---------------------------------------------------------------------------
+# density weighting: cell literals and the f2 formula from the reference tests
+# ---------------------------------------------------------------------------
+
+def _f32_bits(b):
+    return float(np.array([b], np.uint32).view(np.float32)[0])
+
+
+def test_density_cells_match_reference_literals(oracle_mod):
+    o = oracle_mod
+    vis, cube = o.VIS_IMAGING_WEIGHT, o.CUBE_BRIGGS
+    du = dv = 1.0 / (32.0 * 1.0e-4)
+    # density_cells_truncate_to_image_weight_grid (gridder.rs:5190-5214)
+    cell = lambda conv, u, v: o.density_cell(32, 32, 1.0e-4, 1.0e-4, conv, u, v)
+    assert cell(vis, 0.0, 0.0) == (16, 16)
+    assert cell(vis, 0.99 * du, -0.99 * dv) == (15, 15)
+    assert cell(vis, 1.01 * du, -1.01 * dv) == (14, 14)
+    assert cell(vis, -1.01 * du, 1.01 * dv) == (17, 17)
+    # cube_density_cells_round_to_nearest_weight_grid_location (:5293-5317)
+    assert cell(cube, 0.49 * du, -0.49 * dv) == (16, 16)
+    assert cell(cube, 0.51 * du, -0.51 * dv) == (17, 17)
+    # visibility_weight_density_cell_uses_casa_separate_float_operations (:5217-5242):
+    # a fused multiply-add would give cell 2
+    c = 0.08 / 206_264.806_247
+    assert o.density_cell(1280, 1280, c, c, vis, _f32_bits(0x499CA142), 0.0) == (3, 640)
+    # ..._uses_casa_float_scale_and_origin (:5244-5262) and
+    # ..._narrows_uv_before_float_transform (:5264-5290)
+    c = np.deg2rad(0.6 / 3600.0)
+    assert o.density_cell(4096, 4096, c, c, vis, 0.0, _f32_bits(0x4767CA0C)) == (2048, 2755)
+    assert o.density_cell(4096, 4096, c, c, vis, 1_091.081_732_177_734_5, 0.0) == (2035, 2048)
+    # cells on the edge row/column 0 are rejected (anchor must be > 0)
+    assert cell(cube, -16.0 * du, 0.0) is None
+
+
+def test_briggs_f2_matches_reference_literal(oracle_mod):
+    # cube_briggs_f2_and_denominator_follow_casa_formula (weighting.rs:3454-3458)
+    d = np.array([[0.0, 2.0], [3.0, 0.0]], np.float32)
+    assert abs(oracle_mod.briggs_f2(d, 0.0) - 9.615385) < 1e-5
+    # robust shifts f2 by 100^-robust; the total doubles without conjugates
+    assert abs(oracle_mod.briggs_f2(d, 1.0) / oracle_mod.briggs_f2(d, 0.0) - 0.01) < 1e-6
+    assert abs(oracle_mod.briggs_f2(d, 0.0, includes_conjugates=False) / 9.615385 - 2.0) < 1e-6
+
+
+def test_density_weights_brute_force(oracle_mod):
+    """Uniform and Briggs reweighting against a plain numpy restatement."""
+    o = oracle_mod
+    rng = np.random.default_rng(21)
+    nx, cell, n = 40, 1.0e-3, 3000
+    umax = 0.5 / cell
+    u, v = rng.uniform(-umax, umax, n), rng.uniform(-umax, umax, n)
+    w = rng.choice([0.5, 1.0, 2.0], n).astype(np.float32)
+    w[::97] = 0.0
+    plane = rng.integers(0, 3, n).astype(np.uint32)
+    for mode in (o.WEIGHT_UNIFORM, o.WEIGHT_BRIGGS):
+        got, dens, f2 = o.density_weights(nx, nx, cell, cell, u, v, w, mode, robust=0.5, plane=plane, n_planes=3)
+        ref_d = np.zeros((3, nx, nx), np.float64)
+        cells = [o.density_cell(nx, nx, cell, cell, o.CUBE_BRIGGS, a, b) for a, b in zip(u, v)]
+        mir = [o.density_cell(nx, nx, cell, cell, o.CUBE_BRIGGS, -a, -b) for a, b in zip(u, v)]
+        for i in range(n):
+            if w[i] > 0:
+                for cc in (cells[i], mir[i]):
+                    if cc:
+                        ref_d[plane[i]][cc] += w[i]
+        assert np.array_equal(dens, ref_d.astype(np.float32))  # sums of 0.5, 1, 2 are exact
+        for p in range(3):
+            pos = ref_d[p][ref_d[p] > 0]
+            f2_ref = (5.0 * 10 ** -0.5) ** 2 / (np.sum(pos ** 2) / np.sum(ref_d[p])) if mode == o.WEIGHT_BRIGGS else 1.0
+            assert abs(f2[p] - f2_ref) <= 1e-6 * f2_ref
+        for i in range(n):
+            c = cells[i]
+            if c is None or w[i] <= 0:
+                assert got[i] == 0.0
+                continue
+            d = np.float32(dens[plane[i]][c])
+            ref = w[i] / d if mode == o.WEIGHT_UNIFORM else w[i] / (d * f2[plane[i]] + np.float32(1.0))
+            assert got[i] == np.float32(ref)
