@@ -10,10 +10,11 @@ from . import abi
 from .abi import (CYCLE_AUTO, CYCLE_FUSED, CYCLE_HIPFFT, F32, F64, MODE_DIRTY, WEIGHT_BRIGGS, WEIGHT_UNIFORM, MODE_PSF, MODE_RESIDUAL, ImagingError, InvalidRequest,
                   NoUsableSamples, Normalization, Unsupported)
 from .imaging import (Context, RowChannelBatch, StandardGridder, VisibilityBatch, VisibilitySet,
-                      default_context)
+                      default_context, phase_rotation, rotate_rows)
 
 __all__ = [
     "abi", "CYCLE_AUTO", "CYCLE_FUSED", "CYCLE_HIPFFT", "WEIGHT_BRIGGS", "WEIGHT_UNIFORM", "F32", "F64", "MODE_DIRTY", "MODE_PSF", "MODE_RESIDUAL", "ImagingError",
     "InvalidRequest", "NoUsableSamples", "Normalization", "Unsupported", "Context",
     "RowChannelBatch", "StandardGridder", "VisibilityBatch", "VisibilitySet", "default_context",
+    "phase_rotation", "rotate_rows",
 ]

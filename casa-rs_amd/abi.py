@@ -41,7 +41,7 @@ EXPORTS = (
     "chip_ctx_create", "chip_ctx_destroy", "chip_ctx_set_stream", "chip_error_free", "chip_version",
     "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
     "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
-    "chip_density_weights",
+    "chip_density_weights", "chip_phase_rotation", "chip_rotate_rows",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
     "chip_vset_cycle_strategy",
@@ -157,6 +157,8 @@ def load():
         "chip_gridder_attach_wproject": (ctypes.c_int, [_P, ctypes.c_double, _U32, _U32, _PP]),
         "chip_wproject_tables": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
         "chip_plan": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _P, _PP]),
+        "chip_phase_rotation": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
+        "chip_rotate_rows": (ctypes.c_int, [_P, _U64, _U64, _P, _P, _P, _P, _P, _P, _P, _PP]),
         "chip_density_weights": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, ctypes.c_float, _U32, _P, _P,
                                                 _P, _PP]),
         "chip_vset_create": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _PP, _PP]),

@@ -1,5 +1,6 @@
 // extern "C" entry points of libcasa_imaging_hip.so.  Each wrapper converts
 // exceptions into the status / message convention of the header.
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 
@@ -210,6 +211,28 @@ int chip_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, float *
         set_device(g->ctx);
         launch_plan_raw(*g, make_source(*s), plan_dev, norm_dev, g->ctx->stream);
         CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int chip_phase_rotation(const double *source_dir, const double *target_dir, double *uvrot, double *phrot,
+                        char **err) {
+    return guarded(err, [&] {
+        if (!source_dir || !target_dir || !uvrot || !phrot) fail(CHIP_ERR_INVALID, "NULL argument");
+        for (int k = 0; k < 2; ++k)
+            if (!std::isfinite(source_dir[k]) || !std::isfinite(target_dir[k]))
+                fail(CHIP_ERR_INVALID, "directions must be finite");
+        phase_rotation(source_dir, target_dir, uvrot, phrot);
+    });
+}
+
+int chip_rotate_rows(chip_ctx *ctx, uint64_t n_rows, uint64_t n_chan, const double *uvrot, const double *phrot,
+                     const double *uvw_m_dev, const double *freq_hz_dev, float *vis_dev, double *uvw_out_dev,
+                     double *phase_shift_m_dev, char **err) {
+    return guarded(err, [&] {
+        if (!ctx) fail(CHIP_ERR_INVALID, "NULL context");
+        set_device(ctx);
+        rotate_rows(*ctx, n_rows, n_chan, uvrot, phrot, uvw_m_dev, freq_hz_dev, vis_dev, uvw_out_dev,
+                    phase_shift_m_dev);
     });
 }
 

@@ -183,6 +183,9 @@ void model_to_grid(chip_gridder &g, uint32_t precision, uint64_t batch, const fl
 void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *image_out);
 void density_weights(chip_gridder &g, const chip_samples &cs, uint32_t mode, float robust, uint32_t per_plane,
                      float *weight_out, float *f2_host, float *density_out);
+void phase_rotation(const double src[2], const double tgt[2], double uvrot[9], double phrot[3]);
+void rotate_rows(chip_ctx &ctx, uint64_t n_rows, uint64_t n_chan, const double uvrot[9], const double phrot[3],
+                 const double *uvw, const double *freq, float *vis, double *uvw_out, double *shift_out);
 
 // Largest dynamic LDS one workgroup may use on `device` (160 KB on gfx950).
 inline size_t lds_limit(int device) {

@@ -3,6 +3,7 @@
 // per gridder on the CPU (a few thousand values); the per-sample work is on
 // the GPU.  Each formula keeps the reference's precision and operation order
 // because the tap weights and corrections enter the results bit for bit.
+#include <array>
 #include <cmath>
 
 #include "internal.hpp"
@@ -154,6 +155,55 @@ uint32_t find_w_support(const std::vector<float> &mag, uint32_t conv, uint32_t s
     const float q = static_cast<float>(trial) / static_cast<float>(sampling) + 0.5f;
     const int64_t sup = static_cast<int64_t>(std::floor(q)) + 1;
     return static_cast<uint32_t>(std::min(sup, cap));
+}
+
+
+
+// ---- phase-centre rotation (casars-imager/src/lib.rs:48030-48159) ----
+
+namespace {
+
+using M3 = std::array<std::array<double, 3>, 3>;
+
+M3 mat3_mul(const M3 &a, const M3 &b) {
+    M3 t{};
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) t[r][c] = a[r][0] * b[0][c] + a[r][1] * b[1][c] + a[r][2] * b[2][c];
+    return t;
+}
+
+// gridft_axis_rotation: about X (axis 0) or Z (axis 1)
+M3 axis_rotation(double angle, int axis) {
+    const double s = std::sin(angle), c = std::cos(angle);
+    if (axis == 0) return M3{{{1.0, 0.0, 0.0}, {0.0, c, -s}, {0.0, s, c}}};
+    return M3{{{c, -s, 0.0}, {s, c, 0.0}, {0.0, 0.0, 1.0}}};
+}
+
+// gridft_euler_rotation of two operations from the identity
+M3 euler2(double a1, int ax1, double a2, int ax2) {
+    M3 m{{{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}}};
+    m = mat3_mul(m, axis_rotation(a1, ax1));
+    return mat3_mul(m, axis_rotation(a2, ax2));
+}
+
+constexpr double kHalfPi = 1.57079632679489661923;
+
+M3 target_rotation(const double tgt[2]) { return euler2(kHalfPi - tgt[1], 0, -(tgt[0] - kHalfPi), 1); }
+
+}  // namespace
+
+void phase_rotation(const double src[2], const double tgt[2], double uvrot[9], double phrot[3]) {
+    const M3 rot1 = euler2(-(kHalfPi - src[0]), 1, src[1] - kHalfPi, 0);
+    const M3 rot3 = target_rotation(tgt);
+    const M3 p = mat3_mul(rot3, rot1);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) uvrot[3 * r + c] = p[c][r];  // transpose
+    const double tc[3] = {std::cos(tgt[1]) * std::cos(tgt[0]), std::cos(tgt[1]) * std::sin(tgt[0]),
+                          std::sin(tgt[1])};
+    const double sc[3] = {std::cos(src[1]) * std::cos(src[0]), std::cos(src[1]) * std::sin(src[0]),
+                          std::sin(src[1])};
+    const double d[3] = {tc[0] - sc[0], tc[1] - sc[1], tc[2] - sc[2]};
+    for (int r = 0; r < 3; ++r) phrot[r] = rot3[r][0] * d[0] + rot3[r][1] * d[1] + rot3[r][2] * d[2];
 }
 
 }  // namespace chip

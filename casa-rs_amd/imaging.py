@@ -75,6 +75,40 @@ class Context:
 _default_ctx: Context | None = None
 
 
+def phase_rotation(source_dir, target_dir):
+    """(uvrot (3, 3), phrot (3,)) of gridft_uvw_rotation_matrix and
+    uvw_phase_rotation_vector_from_angles (casars-imager/src/lib.rs:48038-48090),
+    computed on the host in FP64; directions (ra, dec) in radians."""
+    src = np.ascontiguousarray(source_dir, np.float64)
+    tgt = np.ascontiguousarray(target_dir, np.float64)
+    rot = np.empty(9, np.float64)
+    ph = np.empty(3, np.float64)
+    abi.call("chip_phase_rotation", src.ctypes.data_as(ctypes.c_void_p), tgt.ctypes.data_as(ctypes.c_void_p),
+             rot.ctypes.data_as(ctypes.c_void_p), ph.ctypes.data_as(ctypes.c_void_p))
+    return rot.reshape(3, 3), ph
+
+
+def rotate_rows(uvw_m, freq_hz, vis, uvrot, phrot, ctx: "Context | None" = None):
+    """Phase-centre rotation of a row stream on the GPU (FP64): returns the
+    rotated uvw (rows, 3) and the phase shift per row; ``vis`` (rows, chan)
+    complex64 on the device is rotated in place (lib.rs:47064-47069, 248-259)."""
+    ctx = ctx or default_context()
+    dev = ctx.torch_device
+    uvw = uvw_m.to(device=dev, dtype=torch.float64).contiguous()
+    f = torch.as_tensor(freq_hz, dtype=torch.float64, device=dev).contiguous()
+    if vis is not None and (vis.dtype != torch.complex64 or not vis.is_contiguous() or vis.device != dev):
+        raise abi.InvalidRequest("vis must be a contiguous complex64 device tensor (rotated in place)")
+    n_rows = uvw.shape[0]
+    n_chan = f.shape[0]
+    out = torch.empty_like(uvw)
+    shift = torch.empty(n_rows, dtype=torch.float64, device=dev)
+    rot = np.ascontiguousarray(uvrot, np.float64).ravel()
+    ph = np.ascontiguousarray(phrot, np.float64).ravel()
+    abi.call("chip_rotate_rows", ctx.handle, n_rows, n_chan, rot.ctypes.data_as(ctypes.c_void_p),
+             ph.ctypes.data_as(ctypes.c_void_p), _ptr(uvw), _ptr(f), _ptr(vis), _ptr(out), _ptr(shift))
+    return out, shift
+
+
 def default_context() -> Context:
     global _default_ctx
     if _default_ctx is None:

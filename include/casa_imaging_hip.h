@@ -160,6 +160,24 @@ int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t exp
 int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels,
                          float *norm_table, char **err);
 
+/* ---- phase-centre rotation of a row stream ------------------------------- */
+/* Host FP64: uvrot = gridft_uvw_rotation_matrix(source_dir, target_dir)
+ * (casars-imager/src/lib.rs:48074-48090, row-major 3x3) and phrot =
+ * uvw_phase_rotation_vector_from_angles(source_dir, target_dir) (:48038-48055);
+ * directions are (ra, dec) in radians.  The imager passes (image phase
+ * centre, row phase centre) (:47952-47956). */
+int chip_phase_rotation(const double *source_dir, const double *target_dir, double *uvrot, double *phrot,
+                        char **err);
+/* Device FP64 (lib.rs:47064-47069; casa-imaging lib.rs:248-259): per row
+ * uvw' = uvw * uvrot (row vector) into uvw_out_dev (may be NULL, may alias
+ * uvw_m_dev) and shift = phrot . uvw' into phase_shift_m_dev (may be NULL);
+ * per (row, chan), if vis_dev is not NULL, vis *= (cos phi, sin phi) in
+ * Float with phi = -2 pi shift f / c (skipped when shift or f is 0).  vis_dev
+ * is complex64 [row][chan], freq_hz_dev n_chan values. */
+int chip_rotate_rows(chip_ctx *ctx, uint64_t n_rows, uint64_t n_chan, const double *uvrot, const double *phrot,
+                     const double *uvw_m_dev, const double *freq_hz_dev, float *vis_dev, double *uvw_out_dev,
+                     double *phase_shift_m_dev, char **err);
+
 /* ---- density weighting -------------------------------------------------- */
 /* Uniform / Briggs reweighting from a density grid of image size
  * (apply_weighting_to_owned_batches_by_sample_groups, weighting.rs:1180-1232;

@@ -150,6 +150,12 @@ def _declare(L):
     L.co_density_reweight.restype = None
     L.co_density_reweight.argtypes = [_U64, _U64, _D, _D, ctypes.c_int, ctypes.c_int, _U64, _P, _P, _P, _P, _P,
                                       _P, _P, _P]
+    L.co_uvw_rotation_matrix.restype = None
+    L.co_uvw_rotation_matrix.argtypes = [_P, _P, _P]
+    L.co_phase_rotation_vector.restype = None
+    L.co_phase_rotation_vector.argtypes = [_P, _P, _P]
+    L.co_rotate_rows.restype = None
+    L.co_rotate_rows.argtypes = [_U64, _U64, _P, _P, _P, _P, _P, _P, _P]
     L.co_direct_adjoint_window.restype = None
     L.co_direct_adjoint_window.argtypes = [_U64, _U64, _D, _D, _U64, _U64, _U64, _U64, _U64, _P, _P, _P, _P,
                                            ctypes.c_int, _P]
@@ -755,3 +761,33 @@ def density_weights(nx, ny, cell_x, cell_y, u, v, weight, mode, robust=0.0, plan
 def briggs_f2(density, robust, includes_conjugates=True):
     d = np.ascontiguousarray(density, np.float32)
     return float(lib().co_briggs_f2(d.size, _ptr(d), ctypes.c_float(robust), 1 if includes_conjugates else 0))
+
+
+# ---------------------------------------------------------------------------
+# phase-centre rotation (casars-imager/src/lib.rs:48030-48159, casa-imaging lib.rs:248-259)
+# ---------------------------------------------------------------------------
+
+def phase_rotation(source_dir, target_dir):
+    """(uvrot 3x3, phrot 3): gridft_uvw_rotation_matrix(source, target) and
+    uvw_phase_rotation_vector_from_angles(source, target); directions (ra, dec) rad."""
+    src = np.ascontiguousarray(source_dir, np.float64)
+    tgt = np.ascontiguousarray(target_dir, np.float64)
+    rot = np.empty(9, np.float64)
+    ph = np.empty(3, np.float64)
+    lib().co_uvw_rotation_matrix(_ptr(src), _ptr(tgt), _ptr(rot))
+    lib().co_phase_rotation_vector(_ptr(src), _ptr(tgt), _ptr(ph))
+    return rot.reshape(3, 3), ph
+
+
+def rotate_rows(uvrot, phrot, uvw_m, freq_hz, vis):
+    """Rotated uvw (rows, 3), phase shift (rows,) and phase-rotated vis (rows, chan)."""
+    uvw = np.ascontiguousarray(uvw_m, np.float64)
+    f = np.ascontiguousarray(freq_hz, np.float64)
+    v = np.array(vis, dtype=np.complex64, order="C", copy=True)
+    n_rows, n_chan = v.shape
+    out = np.empty_like(uvw)
+    shift = np.empty(n_rows, np.float64)
+    lib().co_rotate_rows(n_rows, n_chan, _ptr(np.ascontiguousarray(uvrot, np.float64)),
+                         _ptr(np.ascontiguousarray(phrot, np.float64)), _ptr(uvw), _ptr(f),
+                         _ptr(v.view(np.float32)), _ptr(out), _ptr(shift))
+    return out, shift, v

@@ -908,3 +908,90 @@ void co_density_reweight(uint64_t nx, uint64_t ny, double cell_x, double cell_y,
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Phase-centre rotation (casars-imager/src/lib.rs:47064-47069,              */
+/* :48030-48159; casa-imaging/src/lib.rs:248-259)                            */
+/* ------------------------------------------------------------------------- */
+
+static void co_mat3_mul(const double a[3][3], const double b[3][3], double out[3][3]) {
+    double t[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) t[r][c] = a[r][0] * b[0][c] + a[r][1] * b[1][c] + a[r][2] * b[2][c];
+    memcpy(out, t, sizeof(t));
+}
+
+/* gridft_axis_rotation: axis 0 = X, 1 = Z */
+static void co_axis_rotation(double angle, int axis, double m[3][3]) {
+    const double s = sin(angle), c = cos(angle);
+    if (axis == 0) {
+        double x[3][3] = {{1.0, 0.0, 0.0}, {0.0, c, -s}, {0.0, s, c}};
+        memcpy(m, x, sizeof(x));
+    } else {
+        double z[3][3] = {{c, -s, 0.0}, {s, c, 0.0}, {0.0, 0.0, 1.0}};
+        memcpy(m, z, sizeof(z));
+    }
+}
+
+/* gridft_euler_rotation of two operations, starting from the identity */
+static void co_euler2(double a1, int ax1, double a2, int ax2, double out[3][3]) {
+    double m[3][3] = {{1.0, 0.0, 0.0}, {0.0, 1.0, 0.0}, {0.0, 0.0, 1.0}}, r[3][3];
+    co_axis_rotation(a1, ax1, r);
+    co_mat3_mul(m, r, m);
+    co_axis_rotation(a2, ax2, r);
+    co_mat3_mul(m, r, m);
+    memcpy(out, m, sizeof(m));
+}
+
+static const double CO_HALF_PI = 1.57079632679489661923;
+
+/* rot3 of the target direction (X then Z) */
+static void co_target_rotation(const double tgt[2], double out[3][3]) {
+    co_euler2(CO_HALF_PI - tgt[1], 0, -(tgt[0] - CO_HALF_PI), 1, out);
+}
+
+/* gridft_uvw_rotation_matrix (:48074-48090): transpose(rot3 * rot1), row-major out[9] */
+void co_uvw_rotation_matrix(const double src[2], const double tgt[2], double *out) {
+    double rot1[3][3], rot3[3][3], p[3][3];
+    co_euler2(-(CO_HALF_PI - src[0]), 1, src[1] - CO_HALF_PI, 0, rot1);
+    co_target_rotation(tgt, rot3);
+    co_mat3_mul(rot3, rot1, p);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[3 * r + c] = p[c][r];
+}
+
+/* direction_cosines_from_angles / uvw_phase_rotation_vector_from_angles (:48030-48055) */
+void co_phase_rotation_vector(const double src[2], const double tgt[2], double *out) {
+    double rot3[3][3];
+    co_target_rotation(tgt, rot3);
+    const double tc[3] = {cos(tgt[1]) * cos(tgt[0]), cos(tgt[1]) * sin(tgt[0]), sin(tgt[1])};
+    const double sc[3] = {cos(src[1]) * cos(src[0]), cos(src[1]) * sin(src[0]), sin(src[1])};
+    const double d[3] = {tc[0] - sc[0], tc[1] - sc[1], tc[2] - sc[2]};
+    for (int r = 0; r < 3; ++r) out[r] = rot3[r][0] * d[0] + rot3[r][1] * d[1] + rot3[r][2] * d[2];
+}
+
+/* Apply per row: uvw' = uvw * uvrot (row vector), shift = phrot . uvw'; per
+ * (row, chan): phase_rotate_visibility with phase = -TAU * shift * f / c,
+ * phasor cast to Float, complex Float multiply.  vis [row][chan] in place. */
+void co_rotate_rows(uint64_t n_rows, uint64_t n_chan, const double *uvrot, const double *phrot,
+                    const double *uvw, const double *freq, float *vis, double *uvw_out, double *shift_out) {
+    const double TAU = 6.28318530717958647692, C = 299792458.0;
+    for (uint64_t r = 0; r < n_rows; ++r) {
+        const double *x = uvw + 3 * r;
+        double o[3];
+        for (int k = 0; k < 3; ++k) o[k] = x[0] * uvrot[k] + x[1] * uvrot[3 + k] + x[2] * uvrot[6 + k];
+        const double shift = phrot[0] * o[0] + phrot[1] * o[1] + phrot[2] * o[2];
+        if (uvw_out) memcpy(uvw_out + 3 * r, o, sizeof(o));
+        if (shift_out) shift_out[r] = shift;
+        for (uint64_t c = 0; c < n_chan; ++c) {
+            if (shift == 0.0 || freq[c] == 0.0) continue;
+            const double phase = -TAU * shift * freq[c] / C;
+            const float pr = (float)cos(phase), pi = (float)sin(phase);
+            float *v = vis + 2 * (r * n_chan + c);
+            const float re = v[0] * pr - v[1] * pi;
+            const float im = v[0] * pi + v[1] * pr;
+            v[0] = re;
+            v[1] = im;
+        }
+    }
+}

@@ -78,3 +78,16 @@ def test_invalid_geometry_is_rejected_before_device_work(pkg):
     assert rc == pkg.abi.CHIP_ERR_INVALID
     assert b"NULL" in ctypes.cast(err, ctypes.c_char_p).value
     L.chip_error_free(err)
+
+
+def test_phase_rotation_host_matches_oracle(pkg, oracle_mod):
+    """chip_phase_rotation runs on the host (no device needed): FP64 with the
+    reference's operation order, bit-exact with the oracle."""
+    import numpy as np
+    rng = np.random.default_rng(8)
+    for _ in range(50):
+        a = (rng.uniform(0, 2 * np.pi), rng.uniform(-1.5, 1.5))
+        b = (rng.uniform(0, 2 * np.pi), rng.uniform(-1.5, 1.5))
+        R, ph = pkg.phase_rotation(a, b)
+        oR, oph = oracle_mod.phase_rotation(a, b)
+        assert np.array_equal(R, oR) and np.array_equal(ph, oph)

@@ -333,3 +333,50 @@ def test_density_weights_brute_force(oracle_mod):
             d = np.float32(dens[plane[i]][c])
             ref = w[i] / d if mode == o.WEIGHT_UNIFORM else w[i] / (d * f2[plane[i]] + np.float32(1.0))
             assert got[i] == np.float32(ref)
+
+
+# ---------------------------------------------------------------------------
+# phase-centre rotation
+# ---------------------------------------------------------------------------
+
+def _uvw_toward(b, ra, dec):
+    """uvw of baseline b (equatorial XYZ) toward (ra, dec), MS sign convention
+    (u, v opposite to the textbook u = b . (-sin ra, cos ra, 0))."""
+    u = np.array([-np.sin(ra), np.cos(ra), 0.0])
+    v = np.array([-np.sin(dec) * np.cos(ra), -np.sin(dec) * np.sin(ra), np.cos(dec)])
+    s = np.array([np.cos(dec) * np.cos(ra), np.cos(dec) * np.sin(ra), np.sin(dec)])
+    return np.array([-(b @ u), -(b @ v), b @ s])
+
+
+def test_phase_rotation_reprojects_uvw(oracle_mod):
+    """gridft_uvw_rotation_matrix(a, b) maps uvw toward a onto uvw toward b and
+    the phase vector gives uvw' . phrot = w_b - w_a (the geometric delay)."""
+    o = oracle_mod
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        a = (rng.uniform(0, 2 * np.pi), rng.uniform(-1.2, 1.2))
+        bdir = (a[0] + rng.uniform(-0.05, 0.05), a[1] + rng.uniform(-0.05, 0.05))
+        R, ph = o.phase_rotation(a, bdir)
+        assert np.max(np.abs(R @ R.T - np.eye(3))) < 1e-15
+        base = rng.normal(size=(50, 3)) * 3000.0
+        uvw_a = np.array([_uvw_toward(x, *a) for x in base])
+        uvw_b = np.array([_uvw_toward(x, *bdir) for x in base])
+        out, shift, _ = o.rotate_rows(R, ph, uvw_a, [1.0e9], np.ones((50, 1), np.complex64))
+        assert np.max(np.abs(out - uvw_b)) < 1e-9
+        assert np.max(np.abs(shift - (uvw_b[:, 2] - uvw_a[:, 2]))) < 1e-9
+    R, ph = o.phase_rotation((0.3, 0.5), (0.3, 0.5))
+    assert np.max(np.abs(R - np.eye(3))) < 1e-15 and np.all(ph == 0.0)
+
+
+def test_phase_rotate_visibility_literal(oracle_mod):
+    """phase_rotate_visibility_applies_geometric_phasor (casa-imaging lib.rs:78080-78090)."""
+    o = oracle_mod
+    R, ph = np.eye(3), np.array([0.0, 0.0, 1.0])  # shift = w
+    f = 1.0e9
+    quarter = 299_792_458.0 / (4.0 * f)
+    uvw = np.array([[0.0, 0.0, quarter], [0.0, 0.0, 0.0]])
+    _, shift, vis = o.rotate_rows(R, ph, uvw, [f], np.ones((2, 1), np.complex64))
+    assert abs(vis[0, 0].real) < 1e-6 and abs(vis[0, 0].imag + 1.0) < 1e-6
+    assert vis[1, 0] == 1.0  # zero shift leaves the sample unchanged
+    _, _, vis = o.rotate_rows(R, ph, uvw, [0.0], np.ones((2, 1), np.complex64))
+    assert np.all(vis == 1.0)  # zero frequency too
