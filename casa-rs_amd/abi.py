@@ -32,6 +32,13 @@ CYCLE_FUSED = 1
 F32 = 0
 F64 = 1
 
+AW_OK = 0
+AW_NON_FINITE_COORDINATE = 1
+AW_OUTSIDE_GRID = 2
+AW_KERNEL_INDEX_OUTSIDE_CELL = 3
+AW_INVALID_NORMALIZATION = 4
+AW_NO_CELL = 5
+
 PAD_COMPOSITE_1_2 = 0
 PAD_COMPOSITE_1_3 = 1
 PAD_NONE = 2
@@ -47,6 +54,8 @@ EXPORTS = (
     "chip_vset_cycle_strategy",
     "chip_grid_plain", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
+    "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_aw_cells", "chip_aw_plan",
+    "chip_airy_table",
 )
 
 
@@ -96,7 +105,7 @@ class GridderInfo(ctypes.Structure):
         ("oversampling", ctypes.c_uint32), ("wproject", ctypes.c_uint32),
         ("planes", ctypes.c_uint32), ("sampling", ctypes.c_uint32),
         ("width", ctypes.c_uint32), ("max_support", ctypes.c_uint32),
-        ("norm_radius", ctypes.c_uint32), ("_pad", ctypes.c_uint32),
+        ("norm_radius", ctypes.c_uint32), ("aw", ctypes.c_uint32),
         ("w_scale", ctypes.c_double),
     ]
 
@@ -121,6 +130,24 @@ class Samples(ctypes.Structure):
         ("vis_dev", ctypes.c_void_p), ("weight_dev", ctypes.c_void_p),
         ("sumwt_factor_dev", ctypes.c_void_p), ("gridable_dev", ctypes.c_void_p),
         ("flag_dev", ctypes.c_void_p),
+        ("cf_cell_dev", ctypes.c_void_p), ("pointing_dev", ctypes.c_void_p),
+    ]
+
+
+class CfCell(ctypes.Structure):
+    _fields_ = [
+        ("shape_x", ctypes.c_uint64), ("shape_y", ctypes.c_uint64),
+        ("x_support", ctypes.c_uint32), ("y_support", ctypes.c_uint32),
+        ("offset", ctypes.c_uint64),
+    ]
+
+
+class AiryParams(ctypes.Structure):
+    _fields_ = [
+        ("dish_m", ctypes.c_double), ("blockage_m", ctypes.c_double),
+        ("freq_hz", ctypes.c_double), ("pa_rad", ctypes.c_double),
+        ("sampling", ctypes.c_uint32), ("n_w", ctypes.c_uint32),
+        ("max_abs_w", ctypes.c_double),
     ]
 
 
@@ -176,6 +203,12 @@ def load():
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),
         "chip_model_to_grid": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _PP]),
         "chip_grid_to_model_adjoint": (ctypes.c_int, [_P, _U64, _P, _P, _PP]),
+        "chip_gridder_attach_aw": (ctypes.c_int, [_P, _U32, ctypes.POINTER(CfCell), _P, _U64, _U32, _U32, _P,
+                                                  ctypes.c_double, _PP]),
+        "chip_gridder_attach_aw_airy": (ctypes.c_int, [_P, ctypes.POINTER(AiryParams), _U32, _P, _PP]),
+        "chip_aw_cells": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
+        "chip_aw_plan": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _P, _P, _PP]),
+        "chip_airy_table": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, _P, _P, _PP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

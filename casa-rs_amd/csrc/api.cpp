@@ -158,6 +158,7 @@ int chip_gridder_info_get(const chip_gridder *g, chip_gridder_info *out, char **
         r.planes = g->planes; r.sampling = g->sampling; r.width = g->width;
         r.max_support = g->max_support;
         r.norm_radius = static_cast<uint32_t>(g->norm_radius);
+        r.aw = g->aw ? 1u : 0u;
         r.w_scale = g->w_scale;
         *out = r;
     });
@@ -191,11 +192,71 @@ int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t exp
     });
 }
 
+int chip_gridder_attach_aw(chip_gridder *g, uint32_t n_cells, const chip_cf_cell *cells,
+                           const float *pixels_dev, uint64_t n_pixels, uint32_t sampling,
+                           uint32_t n_pointings, const double *phase_gradients, double w_increment,
+                           char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        set_device(g->ctx);
+        attach_aw(*g, n_cells, cells, pixels_dev, n_pixels, sampling, n_pointings, phase_gradients, w_increment);
+    });
+}
+
+int chip_gridder_attach_aw_airy(chip_gridder *g, const chip_airy_params *p, uint32_t n_pointings,
+                                const double *phase_gradients, char **err) {
+    return guarded(err, [&] {
+        if (!g || !p) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        attach_aw_airy(*g, *p, n_pointings, phase_gradients);
+    });
+}
+
+int chip_aw_cells(const chip_gridder *g, uint32_t *n_cells, chip_cf_cell *cells, uint64_t *n_pixels,
+                  float *pixels_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        if (!g->aw) fail(CHIP_ERR_INVALID, "no AW atlas attached");
+        set_device(g->ctx);
+        if (n_cells) *n_cells = g->aw_cells;
+        if (n_pixels) *n_pixels = g->aw_n_pixels;
+        if (cells) std::memcpy(cells, g->aw_meta.data(), g->aw_meta.size() * sizeof(chip_cf_cell));
+        if (pixels_dev) {
+            CHIP_HIP(hipMemcpyAsync(pixels_dev, g->d_aw_pixels.p, g->d_aw_pixels.bytes, hipMemcpyDeviceToDevice,
+                                    g->ctx->stream));
+            CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+        }
+    });
+}
+
+int chip_aw_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, float *norm32_dev,
+                 double *norm64_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s || !plan_dev) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        aw_plan_raw(*g, make_source(*s), plan_dev, norm32_dev, norm64_dev);
+        CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int chip_airy_table(double dish_m, double blockage_m, float *values, double *max_radius, char **err) {
+    return guarded(err, [&] {
+        if (!(std::isfinite(dish_m) && dish_m > 0.0 && std::isfinite(blockage_m) && blockage_m >= 0.0))
+            fail(CHIP_ERR_INVALID, "dish and blockage diameters must be finite, dish positive");
+        if (values) {
+            const std::vector<float> t = airy_table(dish_m, blockage_m);
+            std::memcpy(values, t.data(), t.size() * sizeof(float));
+        }
+        if (max_radius) *max_radius = airy_max_radius(dish_m, blockage_m);
+    });
+}
+
 int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels, float *norm_table,
                          char **err) {
     return guarded(err, [&] {
         if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
-        if (!g->wproject) fail(CHIP_ERR_INVALID, "no W projection attached");
+        if (!g->wproject || g->aw) fail(CHIP_ERR_INVALID, "no W projection attached");
         set_device(g->ctx);
         if (supports) std::memcpy(supports, g->supports.data(), g->supports.size() * sizeof(int32_t));
         if (norm_table) std::memcpy(norm_table, g->norm_table.data(), g->norm_table.size() * sizeof(float));

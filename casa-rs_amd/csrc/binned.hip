@@ -58,6 +58,7 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
     gp.compact = g.wproject ? g.d_compact.as<float2>() : nullptr;
     gp.supports = g.wproject ? g.d_supports.as<int32_t>() : nullptr;
+    gp.aw_norm = g.aw ? g.d_aw_tnorm.as<float2>() : nullptr;
     return gp;
 }
 
@@ -233,6 +234,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
         const float2 *ktap = nullptr;
         const float *wxr = nullptr, *wyr = nullptr;
         float csign = 1.0f;
+        float2 awn = make_float2(1.0f, 0.0f);  // AWProject normalisation (degrid divides by its conjugate)
         if (!WP) {
             side = kTaps;
             wxr = taps + (pk.y & 0xffu) * kTaps;
@@ -247,6 +249,9 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
                     static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R))) *
                        gp.tap_stride;
             csign = (flags & kConjBit) ? -1.0f : 1.0f;
+            if (kModel && gp.aw_norm)
+                awn = gp.aw_norm[static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
+                                 static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R))];
         }
         const int T = side * side;
         const int bx = WP ? ax - (side >> 1) : ax;  // standard anchors are already x_start
@@ -268,6 +273,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
                 }
                 pr = wave_sum_f(pr);
                 pi = wave_sum_f(pi);
+                if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
                 if (MODE == kModeDegrid) {
                     for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
                     continue;
@@ -340,6 +346,7 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
             }
             pr = wave_sum_f(pr);
             pi = wave_sum_f(pi);
+            if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
             if (MODE == kModeDegrid) {
                 // every sample of the group has the same prediction
                 for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);

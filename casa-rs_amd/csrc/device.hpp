@@ -24,6 +24,7 @@ constexpr uint32_t kResidOkBit = 1u << 29;
 struct Plan {
     int32_t x, y, sx, sy, plane, conj;
     float norm;
+    double mag;  // AWProject: |grid_normalization|
 };
 
 // One work item: a run of groups (identical plans) of one (channel, tile).
@@ -40,12 +41,15 @@ struct SampleIn {
     float2 vis;
     bool usable;  // gridable and not flagged
     uint32_t chan;
+    uint64_t index, row;
 };
 
 __device__ __forceinline__ SampleIn load_sample(const Source &s, uint64_t i) {
     SampleIn r;
     const uint64_t row = s.n_chan == 1 ? i : i / s.n_chan;
     r.chan = s.n_chan == 1 ? 0u : static_cast<uint32_t>(i - row * s.n_chan);
+    r.index = i;
+    r.row = row;
     if (s.uvw_m) {
         const double scale = s.lambda_scale[r.chan];
         r.u = s.uvw_m[3 * row] * scale;
@@ -141,6 +145,68 @@ __device__ __forceinline__ bool plan_sample(const PlanParams &p, double u, doubl
     return true;
 }
 
+// awproject_sample_geometry (gridder.rs:2699-2748): anchor, sub-cell offset,
+// grid and cell bounds.  Returns a CHIP_AW_* code.
+__device__ __forceinline__ int aw_geometry(const PlanParams &p, const AwCell &c, double u, double v, double w,
+                                           int64_t &lx, int64_t &ly, int64_t &ox, int64_t &oy) {
+#pragma clang fp contract(off)
+    const double pos_x = u / p.du + p.half_gx;
+    const double pos_y = -v / p.dv + p.half_gy;
+    if (!(isfinite(pos_x) && isfinite(pos_y) && isfinite(w))) return CHIP_AW_NON_FINITE_COORDINATE;
+    if (pos_x > 4.0e18 || pos_x < -4.0e18 || pos_y > 4.0e18 || pos_y < -4.0e18) return CHIP_AW_OUTSIDE_GRID;
+    lx = static_cast<int64_t>(round(pos_x));
+    ly = static_cast<int64_t>(round(pos_y));
+    const double s = static_cast<double>(p.sampling);
+    ox = static_cast<int64_t>(round((static_cast<double>(lx) - pos_x) * s));
+    oy = static_cast<int64_t>(round((static_cast<double>(ly) - pos_y) * s));
+    if (lx - c.xs < 0 || ly - c.ys < 0 || lx + c.xs >= static_cast<int64_t>(p.gx) ||
+        ly + c.ys >= static_cast<int64_t>(p.gy))
+        return CHIP_AW_OUTSIDE_GRID;
+    const int64_t si = p.sampling;
+    if (c.cx - c.xs * si + ox < 0 || c.cy - c.ys * si + oy < 0 || c.cx + c.xs * si + ox >= c.shape_x ||
+        c.cy + c.ys * si + oy >= c.shape_y)
+        return CHIP_AW_KERNEL_INDEX_OUTSIDE_CELL;
+    return CHIP_AW_OK;
+}
+
+// The sample's CF cell: the caller's selection, or the W plane
+// round(sqrt(WIncr |w|)) clamped to the cells (awprojection.rs:1239-1246).
+__device__ __forceinline__ uint32_t aw_cell_of(const PlanParams &p, const SampleIn &s) {
+    if (p.cf_cell) return p.cf_cell[s.index];
+    if (p.aw_cells <= 1 || p.aw_w_incr == 0.0) return 0u;
+    double r = round(sqrt(p.aw_w_incr * fabs(s.w)));
+    const double top = static_cast<double>(p.aw_cells - 1);
+    if (!(r >= 0.0)) r = 0.0;
+    if (r > top) r = top;
+    return static_cast<uint32_t>(r);
+}
+
+// AWProject plan of one sample: geometry, then the (cell, offset)
+// normalisation's validity (plan_sample_detailed, gridder.rs:2253-2311).
+// The tap table folds in the pointing and the w sign, so conj stays 0.
+__device__ __forceinline__ int aw_plan_sample(const PlanParams &p, const SampleIn &s, Plan &out) {
+    const uint32_t cell = aw_cell_of(p, s);
+    const uint32_t pt = p.pointing ? p.pointing[s.row] : 0u;
+    if (cell >= p.aw_cells || pt >= p.aw_pointings) return CHIP_AW_NO_CELL;
+    const AwCell c = p.aw_cell[cell];
+    int64_t lx, ly, ox, oy;
+    const int code = aw_geometry(p, c, s.u, s.v, s.w, lx, ly, ox, oy);
+    if (code != CHIP_AW_OK) return code;
+    const int64_t R = p.norm_radius, axis = 2 * R + 1;
+    if (ox > R || ox < -R || oy > R || oy < -R) return CHIP_AW_KERNEL_INDEX_OUTSIDE_CELL;
+    const double mag = p.aw_mag[(static_cast<int64_t>(cell) * axis + (ox + R)) * axis + (oy + R)];
+    if (!(mag > 0.0)) return CHIP_AW_INVALID_NORMALIZATION;
+    out.x = static_cast<int32_t>(lx);
+    out.y = static_cast<int32_t>(ly);
+    out.sx = static_cast<int32_t>(ox);
+    out.sy = static_cast<int32_t>(oy);
+    out.plane = static_cast<int32_t>((cell * p.aw_pointings + pt) * 2u + (s.w > 0.0 ? 1u : 0u));
+    out.conj = 0;
+    out.mag = mag;
+    out.norm = static_cast<float>(mag);
+    return CHIP_AW_OK;
+}
+
 // Validity filters ahead of planning.
 //  standard: execution.rs:18270-18300 (gridable, weight and sumwt_factor
 //            finite and positive);
@@ -154,6 +220,7 @@ __device__ __forceinline__ bool sample_accepted(const PlanParams &p, const Sampl
               isfinite(s.w)))
             return false;
     }
+    if (p.aw) return aw_plan_sample(p, s, plan) == CHIP_AW_OK;
     return plan_sample(p, s.u, s.v, s.w, plan);
 }
 
@@ -222,7 +289,24 @@ struct GridParams {
     const float2 *kernels;   // planes x width x width
     const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest
     const int32_t *supports;
+    // AWProject: per (table, offset) Complex32 normalisation with the table's
+    // w sign; degridded values are divided by its conjugate.  NULL otherwise.
+    const float2 *aw_norm;
 };
+
+// num-complex 0.4 division z / conj(n): ((a c + b d), (b c - a d)) / (c^2 + d^2)
+// with c = n.re, d = -n.im (degrid_sample_planned, gridder.rs:2359-2371).
+template <typename T>
+__device__ __forceinline__ void div_conj(T &re, T &im, T nr, T ni) {
+#pragma clang fp contract(off)
+    const T c = nr, d = -ni;
+    const T ns = c * c + d * d;
+    const T a = re, b = im;
+    const T r = a * c + b * d;
+    const T i = b * c - a * d;
+    re = r / ns;
+    im = i / ns;
+}
 
 __device__ __forceinline__ uint64_t cell_offset(const GridParams &gp, uint32_t x, uint32_t y) {
     if (!gp.paired) return static_cast<uint64_t>(x) * gp.xstride + y;

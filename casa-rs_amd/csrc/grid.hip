@@ -44,7 +44,10 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
         if (MODE == kModeDegrid && !ok) vis_out[2 * i] = vis_out[2 * i + 1] = T(0);
         if (ok) {
             const float gwf = grid_weight(pp, smp);
-            if (pp.wproject) {
+            if (pp.aw) {
+                nsum = static_cast<double>(smp.weight) * pl.mag;
+                rsum = nsum;
+            } else if (pp.wproject) {
                 nsum = 2.0 * static_cast<double>(smp.weight) * static_cast<double>(pl.norm);
                 rsum = static_cast<double>(smp.weight) * static_cast<double>(smp.sf);
             } else {
@@ -93,6 +96,70 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
                             }
                         }
                     }
+                }
+            } else if (pp.aw) {
+                // AWProject (gridder.rs:2318-2371): phased taps from the tap
+                // table; contribution value*tap as Complex32 added to the T
+                // grid; degrid sum of conj(tap)*grid in tap order divided by
+                // conj(normalization).
+#pragma clang fp contract(off)
+                const int R = gp.norm_radius;
+                const int n_off = static_cast<int>(gp.n_off);
+                const int S = gp.supports[pl.plane];
+                const int side = 2 * S + 1;
+                const AwCell c = pp.aw_cell[static_cast<uint32_t>(pl.plane) / (2u * pp.aw_pointings)];
+                const uint64_t oi = (static_cast<uint64_t>(pl.plane) * n_off + (pl.sx + R)) * n_off + (pl.sy + R);
+                const float2 *tab = gp.compact + oi * gp.tap_stride;
+                const float2 nm = gp.aw_norm[oi];
+                T pr = 0, pi = 0;
+                if ((MODE == kModeResidual || MODE == kModeDegrid) && model) {
+                    for (int iy = -c.ys; iy <= c.ys; ++iy)
+                        for (int ix = -c.xs; ix <= c.xs; ++ix) {
+                            const float2 t = tab[(ix + S) * side + (iy + S)];
+                            const T cr = static_cast<T>(t.x), ci = -static_cast<T>(t.y);
+                            const uint64_t gi = 2 * (base + static_cast<uint64_t>(pl.x + ix) * gp.gy + (pl.y + iy));
+                            const T gr = model[gi], gim = model[gi + 1];
+                            const T ar = cr * gr - ci * gim;
+                            const T ai = cr * gim + ci * gr;
+                            pr += ar;
+                            pi += ai;
+                        }
+                    div_conj<T>(pr, pi, static_cast<T>(nm.x), static_cast<T>(nm.y));
+                }
+                if (MODE == kModeDegrid) {
+                    vis_out[2 * i] = pr;
+                    vis_out[2 * i + 1] = pi;
+                } else {
+                    // value: PSF weight; dirty V*weight; residual (V - model)*weight, in f32
+                    const float wf = gwf;
+                    float vr, vi;
+                    if (MODE == kModeResidual) {
+                        const float rr = smp.vis.x - static_cast<float>(pr);
+                        const float ri = smp.vis.y - static_cast<float>(pi);
+                        vr = rr * wf;
+                        vi = ri * wf;
+                    } else {
+                        vr = smp.vis.x * wf;
+                        vi = smp.vis.y * wf;
+                    }
+                    const bool dirty_on = MODE == kModeResidual || (MODE == kModeDirty && (flags & kVisOkBit));
+                    for (int iy = -c.ys; iy <= c.ys; ++iy)
+                        for (int ix = -c.xs; ix <= c.xs; ++ix) {
+                            const float2 t = tab[(ix + S) * side + (iy + S)];
+                            const uint64_t gi = 2 * (base + static_cast<uint64_t>(pl.x + ix) * gp.gy + (pl.y + iy));
+                            if (MODE == kModePsf || MODE == kModeDirty) {
+                                const float cr = wf * t.x - 0.0f * t.y;
+                                const float ci = wf * t.y + 0.0f * t.x;
+                                gadd(psf + gi, static_cast<T>(cr));
+                                gadd(psf + gi + 1, static_cast<T>(ci));
+                            }
+                            if (dirty_on) {
+                                const float cr = vr * t.x - vi * t.y;
+                                const float ci = vr * t.y + vi * t.x;
+                                gadd(acc + gi, static_cast<T>(cr));
+                                gadd(acc + gi + 1, static_cast<T>(ci));
+                            }
+                        }
                 }
             } else {
                 const int s_ = static_cast<int>(gp.sampling);
@@ -170,7 +237,7 @@ template <typename T, int MODE>
 static void launch_plain_t(chip_gridder &g, const Source &src, const void *model, void *psf,
                            void *acc, void *vis_out, double *sums) {
     const GridParams gp = grid_params(g, 32, 0, 1);
-    const PlanParams pp = plan_params(g);
+    const PlanParams pp = plan_params(g, src);
     hipStream_t st = g.ctx->stream;
     const unsigned blocks = static_cast<unsigned>((src.n_rows + 255) / 256);
     if (blocks == 0) return;

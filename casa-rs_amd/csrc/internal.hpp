@@ -117,6 +117,23 @@ struct chip_gridder {
     std::vector<float> sinc;        // build_sinc_axis(max(gx, gy), sampling)
     std::vector<float> sinc_x, sinc_y;  // per-axis sinc used by model apodisation
     chip::DevBuf d_sinc, d_sinc_x, d_sinc_y;
+
+    // AWProject CF atlas (aw.hip).  When attached, wproject is set too and the
+    // W tap-table fields describe the atlas's tap tables: "plane" t =
+    // (cell * pointings + pointing) * 2 + conjugate, supports[t] the cell's
+    // larger half-support (tables are square, zero outside the cell's own
+    // x/y supports), d_compact the phased taps.
+    bool aw = false;
+    uint32_t aw_cells = 0, aw_pointings = 1;
+    double aw_w_incr = 0.0;
+    std::vector<chip_cf_cell> aw_meta;
+    uint64_t aw_n_pixels = 0;
+    chip::DevBuf d_aw_pixels;  // complex64 copy of the cells
+    chip::DevBuf d_aw_cell;    // AwCell per cell
+    chip::DevBuf d_aw_n32;     // cells x (2R+1)^2 float2: unconjugated Complex32 normalisation
+    chip::DevBuf d_aw_n64;     // cells x (2R+1)^2 double2: unconjugated Complex64 normalisation
+    chip::DevBuf d_aw_mag;     // cells x (2R+1)^2 double: |grid_normalization|, 0 where invalid
+    chip::DevBuf d_aw_tnorm;   // tables x (2R+1)^2 float2: normalisation with the table's w sign
 };
 
 namespace chip {
@@ -150,6 +167,22 @@ struct PlanParams {
     int32_t norm_radius;
     const int32_t *supports;  // device
     const float *norm_table;  // device
+    // AWProject (aw = 1): per-cell geometry, per (cell, offset) |grid_normalization|
+    uint32_t aw;
+    uint32_t aw_cells, aw_pointings;
+    double aw_w_incr;
+    const struct AwCell *aw_cell;
+    const double *aw_mag;
+    const uint32_t *cf_cell;   // per sample, or NULL (W-plane selection)
+    const uint32_t *pointing;  // per row, or NULL
+};
+
+// Device view of one CF cell.
+struct AwCell {
+    int64_t shape_x, shape_y;
+    int32_t xs, ys;      // half-supports
+    int32_t cx, cy;      // aw_kernel_origin of each axis
+    uint64_t offset;     // first pixel (complex64 elements)
 };
 
 PlanParams plan_params(const chip_gridder &g);
@@ -161,8 +194,11 @@ struct Source {
     const float2 *vis;
     const float *weight, *sumwt_factor;
     const uint8_t *gridable, *flag;
+    const uint32_t *cf_cell, *pointing;
 };
 Source make_source(const chip_samples &s);
+// plan_params with the source's per-sample AWProject selections
+PlanParams plan_params(const chip_gridder &g, const Source &src);
 void validate_source(const chip_samples &s);
 
 void launch_plan_raw(const chip_gridder &g, const Source &src, int32_t *plan, float *norm,
@@ -184,6 +220,13 @@ void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *
 void density_weights(chip_gridder &g, const chip_samples &cs, uint32_t mode, float robust, uint32_t per_plane,
                      float *weight_out, float *f2_host, float *density_out);
 void phase_rotation(const double src[2], const double tgt[2], double uvrot[9], double phrot[3]);
+void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, const float *pixels_dev,
+               uint64_t n_pixels, uint32_t sampling, uint32_t n_pointings, const double *gradients,
+               double w_increment);
+void attach_aw_airy(chip_gridder &g, const chip_airy_params &p, uint32_t n_pointings, const double *gradients);
+void aw_plan_raw(chip_gridder &g, const Source &src, int32_t *plan, float *n32, double *n64);
+double airy_max_radius(double dish, double blockage);
+std::vector<float> airy_table(double dish, double blockage);
 void rotate_rows(chip_ctx &ctx, uint64_t n_rows, uint64_t n_chan, const double uvrot[9], const double phrot[3],
                  const double *uvw, const double *freq, float *vis, double *uvw_out, double *shift_out);
 

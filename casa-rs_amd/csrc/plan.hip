@@ -36,7 +36,22 @@ PlanParams plan_params(const chip_gridder &g) {
     p.w_scale = g.w_scale;
     p.norm_radius = g.norm_radius;
     p.supports = g.wproject ? g.d_supports.as<int32_t>() : nullptr;
-    p.norm_table = g.wproject ? g.d_norm_table.as<float>() : nullptr;
+    p.norm_table = g.wproject && !g.aw ? g.d_norm_table.as<float>() : nullptr;
+    p.aw = g.aw ? 1u : 0u;
+    p.aw_cells = g.aw_cells;
+    p.aw_pointings = g.aw_pointings;
+    p.aw_w_incr = g.aw_w_incr;
+    p.aw_cell = g.aw ? g.d_aw_cell.as<AwCell>() : nullptr;
+    p.aw_mag = g.aw ? g.d_aw_mag.as<double>() : nullptr;
+    p.cf_cell = nullptr;
+    p.pointing = nullptr;
+    return p;
+}
+
+PlanParams plan_params(const chip_gridder &g, const Source &src) {
+    PlanParams p = plan_params(g);
+    p.cf_cell = src.cf_cell;
+    p.pointing = src.pointing;
     return p;
 }
 
@@ -61,6 +76,7 @@ Source make_source(const chip_samples &s) {
     r.vis = reinterpret_cast<const float2 *>(s.vis_dev);
     r.weight = s.weight_dev; r.sumwt_factor = s.sumwt_factor_dev;
     r.gridable = s.gridable_dev; r.flag = s.flag_dev;
+    r.cf_cell = s.cf_cell_dev; r.pointing = s.pointing_dev;
     return r;
 }
 
@@ -90,7 +106,7 @@ void launch_plan_raw(const chip_gridder &g, const Source &src, int32_t *plan, fl
                      hipStream_t st) {
     const uint64_t n = src.n_rows * src.n_chan;
     if (n == 0) return;
-    hipLaunchKernelGGL(k_plan_raw, dim3(blocks_for(n, 256)), dim3(256), 0, st, plan_params(g),
+    hipLaunchKernelGGL(k_plan_raw, dim3(blocks_for(n, 256)), dim3(256), 0, st, plan_params(g, src),
                        src, plan, norm);
     CHIP_HIP(hipGetLastError());
 }
@@ -233,7 +249,13 @@ __global__ __launch_bounds__(256) void k_build_stream(PlanParams p, Source s, ui
         wt[pidx] = gw;
         index[pidx] = i;
         double f[kSumFields];
-        if (p.wproject) {
+        if (p.aw) {
+            // AWProject: weight * |grid_normalization| (lib.rs:26535, one hand)
+            f[0] = static_cast<double>(smp.weight) * pl.mag;
+            f[1] = f[0];
+            const AwCell c = p.aw_cell[static_cast<uint32_t>(pl.plane) / (2u * p.aw_pointings)];
+            f[3] = static_cast<double>(2 * c.xs + 1) * static_cast<double>(2 * c.ys + 1);
+        } else if (p.wproject) {
             // w_project_accumulation_for_sample (lib.rs:71460-71470)
             f[0] = 2.0 * static_cast<double>(smp.weight) * static_cast<double>(pl.norm);
             f[1] = static_cast<double>(smp.weight) * static_cast<double>(smp.sf);
@@ -355,7 +377,7 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     DevBuf keys(n * 8), vals(n * 4), keys_s(n * 8), vals_s(n * 4);
     DevBuf d_nvalid(8);
     CHIP_HIP(hipMemsetAsync(d_nvalid.p, 0, 8, st));
-    const PlanParams pp = plan_params(g);
+    const PlanParams pp = plan_params(g, src);
     hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, pp, src, b,
                        keys.as<uint64_t>(), vals.as<uint32_t>(), d_nvalid.as<unsigned long long>());
     CHIP_HIP(hipGetLastError());

@@ -109,6 +109,22 @@ def rotate_rows(uvw_m, freq_hz, vis, uvrot, phrot, ctx: "Context | None" = None)
     return out, shift
 
 
+def airy_table(dish_m, blockage_m):
+    """AiryVoltageLookup table (10,000 f32 voltages) and its maximum radius (arcmin GHz)."""
+    t = np.empty(10000, np.float32)
+    mr = ctypes.c_double()
+    abi.call("chip_airy_table", float(dish_m), float(blockage_m), t.ctypes.data_as(ctypes.c_void_p),
+             ctypes.byref(mr))
+    return t, mr.value
+
+
+def mosaic_phase_gradient(nx, ny, pixel_offset, sampling):
+    """mosaic_projector_phase_gradient_rad_per_sample_from_pixel_offset
+    (lib.rs:56657-56671): -offset * TAU / (n * sampling) per axis."""
+    tau = 2.0 * np.pi
+    return (-pixel_offset[0] * tau / (nx * sampling), -pixel_offset[1] * tau / (ny * sampling))
+
+
 def default_context() -> Context:
     global _default_ctx
     if _default_ctx is None:
@@ -127,6 +143,8 @@ class VisibilityBatch:
     visibility: torch.Tensor  # complex64
     sumwt_factor: Optional[torch.Tensor] = None
     gridable: Optional[torch.Tensor] = None  # bool / uint8
+    cf_cell: Optional[torch.Tensor] = None   # AWProject: CF cell per sample (uint32 / int)
+    pointing: Optional[torch.Tensor] = None  # AWProject: pointing per sample (uint32 / int)
 
     def to_samples(self):
         keep = []
@@ -145,8 +163,10 @@ class VisibilityBatch:
         wt = dev(self.weight, torch.float32)
         sf = dev(self.sumwt_factor, torch.float32)
         g = dev(self.gridable, torch.uint8)
+        cc = dev(self.cf_cell, torch.int32)
+        pt = dev(self.pointing, torch.int32)
         s = abi.Samples(len(u), 1, _ptr(u), _ptr(v), _ptr(w), None, None, _ptr(vis), _ptr(wt),
-                        _ptr(sf), _ptr(g), None)
+                        _ptr(sf), _ptr(g), None, _ptr(cc), _ptr(pt))
         return s, keep
 
 
@@ -160,6 +180,8 @@ class RowChannelBatch:
     weight: torch.Tensor        # (rows, chan) float32
     flag: Optional[torch.Tensor] = None  # (rows, chan) bool / uint8
     sumwt_factor: Optional[torch.Tensor] = None
+    cf_cell: Optional[torch.Tensor] = None   # AWProject: (rows, chan) CF cell
+    pointing: Optional[torch.Tensor] = None  # AWProject: (rows,) pointing
 
     @staticmethod
     def lambda_scales(freq_hz) -> np.ndarray:
@@ -189,8 +211,10 @@ class RowChannelBatch:
         wt = dev(self.weight, torch.float32)
         f = dev(self.flag, torch.uint8)
         sf = dev(self.sumwt_factor, torch.float32)
+        cc = dev(self.cf_cell, torch.int32)
+        pt = dev(self.pointing, torch.int32)
         s = abi.Samples(self.n_rows, self.n_chan, None, None, None, _ptr(uvw), _ptr(ls), _ptr(vis),
-                        _ptr(wt), _ptr(sf), None, _ptr(f))
+                        _ptr(wt), _ptr(sf), None, _ptr(f), _ptr(cc), _ptr(pt))
         return s, keep
 
 
@@ -258,6 +282,64 @@ class StandardGridder:
         abi.call("chip_wproject_tables", self.handle,sup.ctypes.data_as(ctypes.c_void_p),
                  ker.ctypes.data_as(ctypes.c_void_p), nt.ctypes.data_as(ctypes.c_void_p))
         return sup, ker, nt
+
+    def attach_aw(self, cells, sampling, phase_gradients=((0.0, 0.0),), w_increment=0.0):
+        """AwProjector over a CF atlas (gridder.rs:2194-2247).  cells: list of
+        (pixels complex64 (shape_x, shape_y) indexed (x, y), x_support,
+        y_support); phase_gradients: one (x, y) pair per pointing, radians per
+        oversampled CF pixel; w_increment: the cache's WIncr for W-plane cell
+        selection when a batch carries no cf_cell."""
+        dev = self.ctx.torch_device
+        meta = (abi.CfCell * len(cells))()
+        flat, off = [], 0
+        for k, (pix, xs, ys) in enumerate(cells):
+            pix = torch.as_tensor(pix).to(device=dev, dtype=torch.complex64).contiguous()
+            meta[k] = abi.CfCell(pix.shape[0], pix.shape[1], int(xs), int(ys), off)
+            off += pix.numel()
+            flat.append(pix.reshape(-1))
+        pixels = torch.cat(flat)
+        g = np.ascontiguousarray(phase_gradients, np.float64).reshape(-1)
+        abi.call("chip_gridder_attach_aw", self.handle, len(cells), meta, _ptr(pixels), int(pixels.numel()),
+                 int(sampling), len(g) // 2, g.ctypes.data_as(ctypes.c_void_p), float(w_increment))
+        self._refresh()
+        return self
+
+    def attach_aw_airy(self, dish_m, blockage_m, freq_hz, n_w, max_abs_w, sampling=8, pa_rad=0.0,
+                       phase_gradients=((0.0, 0.0),)):
+        """Build Airy A-term x W CF cells on the device and attach them."""
+        p = abi.AiryParams(float(dish_m), float(blockage_m), float(freq_hz), float(pa_rad), int(sampling),
+                           int(n_w), float(max_abs_w))
+        g = np.ascontiguousarray(phase_gradients, np.float64).reshape(-1)
+        abi.call("chip_gridder_attach_aw_airy", self.handle, ctypes.byref(p), len(g) // 2,
+                 g.ctypes.data_as(ctypes.c_void_p))
+        self._refresh()
+        return self
+
+    def aw_cells(self):
+        """The attached atlas: list of (pixels complex64 (shape_x, shape_y), x_support, y_support)."""
+        n = ctypes.c_uint32()
+        npx = ctypes.c_uint64()
+        abi.call("chip_aw_cells", self.handle, ctypes.byref(n), None, ctypes.byref(npx), None)
+        meta = (abi.CfCell * n.value)()
+        pix = torch.empty(npx.value, dtype=torch.complex64, device=self.ctx.torch_device)
+        abi.call("chip_aw_cells", self.handle, None, meta, None, _ptr(pix))
+        out = []
+        for c in meta:
+            p = pix[c.offset:c.offset + c.shape_x * c.shape_y].reshape(c.shape_x, c.shape_y)
+            out.append((p, c.x_support, c.y_support))
+        return out
+
+    def aw_plan(self, batch):
+        """plan_sample_detailed per sample: int32 (n, 6) (loc_x, loc_y, off_x,
+        off_y, conjugate, code), normalization complex64, grid_normalization complex128."""
+        s, keep = batch.to_samples()
+        n = s.n_rows * s.n_chan
+        dev = self.ctx.torch_device
+        plan = torch.empty((n, 6), dtype=torch.int32, device=dev)
+        n32 = torch.empty(n, dtype=torch.complex64, device=dev)
+        n64 = torch.empty(n, dtype=torch.complex128, device=dev)
+        abi.call("chip_aw_plan", self.handle, ctypes.byref(s), _ptr(plan), _ptr(n32), _ptr(n64))
+        return plan, n32, n64
 
     def density_weights(self, batch, mode=abi.WEIGHT_BRIGGS, robust=0.0, per_plane=True, want_density=True):
         """Uniform / Briggs weights of every sample (input order) from density

@@ -86,7 +86,8 @@ typedef struct {
     uint32_t width;        /* W kernel quarter-plane width */
     uint32_t max_support;  /* largest W half-support */
     uint32_t norm_radius;  /* (sampling + 1) / 2 */
-    uint32_t _pad;
+    uint32_t aw;           /* 1 once an AWProject CF atlas is attached (then wproject = 1 too,
+                              planes = tap tables: cells x pointings x 2) */
     double w_scale;
 } chip_gridder_info;
 
@@ -119,7 +120,46 @@ typedef struct {
     const float *sumwt_factor_dev;
     const uint8_t *gridable_dev;
     const uint8_t *flag_dev;
+    /* AWProject only (both optional): the CF cell of every sample, as chosen
+     * by the caller's cache lookup (AwConvolutionFunctionCache::
+     * select_key_for_sample, awprojection.rs:1117-1180); NULL selects the cell
+     * as the W plane round(sqrt(w_increment * |w|)) clamped to the cells
+     * (awprojection.rs:1239-1246).  pointing_dev: per row, index into the
+     * atlas's phase gradients; NULL = pointing 0. */
+    const uint32_t *cf_cell_dev;
+    const uint32_t *pointing_dev;
 } chip_samples;
+
+/* One AWProject convolution-function cell (AwConvolutionFunctionKernelMetadata,
+ * awprojection.rs:97-130, with its pixels AwConvolutionFunctionCell::imaging,
+ * :141-149): stored pixel shape, half-supports in grid pixels, and the index
+ * (in complex64 elements) of its first pixel in the atlas pixel array.  Pixels
+ * are (x, y) with y fastest, origin at aw_kernel_origin (gridder.rs:2809-2815):
+ * n/2 for even n, n/2 + 1 for odd n. */
+typedef struct {
+    uint64_t shape_x, shape_y;
+    uint32_t x_support, y_support;
+    uint64_t offset;
+} chip_cf_cell;
+
+/* Airy A-term CF construction (chip_gridder_attach_aw_airy). */
+typedef struct {
+    double dish_m;      /* PrimaryBeamModel::Airy dish_diameter_m */
+    double blockage_m;  /* blockage_diameter_m (0 = unblocked) */
+    double freq_hz;     /* frequency the voltage pattern is evaluated at */
+    double pa_rad;      /* parallactic-angle rotation of the sky coordinates */
+    uint32_t sampling;  /* CF oversampling (even, 2..126) */
+    uint32_t n_w;       /* W planes: w_k = k^2 / w_scale, as chip_gridder_attach_wproject */
+    double max_abs_w;   /* largest |w| in wavelengths: w_scale = (n_w - 1)^2 / (1.05 max|w|) */
+} chip_airy_params;
+
+/* AWProject plan rejections (AwProjectSamplePlanRejection, gridder.rs:2133-2138). */
+#define CHIP_AW_OK 0
+#define CHIP_AW_NON_FINITE_COORDINATE 1
+#define CHIP_AW_OUTSIDE_GRID 2
+#define CHIP_AW_KERNEL_INDEX_OUTSIDE_CELL 3
+#define CHIP_AW_INVALID_NORMALIZATION 4
+#define CHIP_AW_NO_CELL 5 /* cf_cell_dev / pointing_dev index outside the atlas (no reference counterpart) */
 
 /* ---- context ------------------------------------------------------------ */
 int chip_ctx_create(int device, chip_ctx **out, char **err);
@@ -159,6 +199,51 @@ int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t exp
  * norm_table[planes][2R+1][2R+1].  Any pointer may be NULL. */
 int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels,
                          float *norm_table, char **err);
+
+/* ---- AWProject CF atlas -------------------------------------------------- */
+/* Attach n_cells CF cells sharing one oversampling (AwProjector::new,
+ * gridder.rs:2194-2247: every cell must hold origin +- support*sampling) and
+ * n_pointings pointing phase gradients (host, 2 doubles per pointing: radians
+ * per oversampled CF pixel along x and y, the projector's
+ * phase_gradient_rad_per_sample; mosaic_projector_phase_gradient_rad_per_
+ * sample_from_pixel_offset, lib.rs:56657-56671, derives them).  pixels_dev
+ * holds n_pixels complex64 values and is copied.  The library precomputes,
+ * per (cell, pointing, w-sign, sub-cell offset), the phased taps of
+ * for_each_grid_tap (gridder.rs:2636-2666) and the f32 / f64 normalisations
+ * of plan_sample_detailed (:2253-2311); afterwards the gridder grids, degrids
+ * and plans through AWProject with every entry point that takes samples.
+ * w_increment (the cache's WIncr) selects cells when chip_samples.cf_cell_dev
+ * is NULL.  Grid weight and sums: a sample grids value*weight (PSF: weight)
+ * with its phased taps; normalization_sumwt and reported_sumwt accumulate
+ * weight * |grid_normalization| (lib.rs:26535, one hand per sample). */
+int chip_gridder_attach_aw(chip_gridder *g, uint32_t n_cells, const chip_cf_cell *cells,
+                           const float *pixels_dev, uint64_t n_pixels, uint32_t sampling,
+                           uint32_t n_pointings, const double *phase_gradients, double w_increment,
+                           char **err);
+/* Build the CF cells of an Airy A-term on the device and attach them: per W
+ * plane k, the FFT of the screen corr(l) corr(m) PB(l', m') exp(2 pi i w_k
+ * (sqrt(1 - l^2 - m^2) - 1)) with PB = V^2 the Airy power pattern
+ * (AiryVoltageLookup::evaluate_offsets, lib.rs:57505-57532, at freq_hz) of
+ * the sky coordinates (l', m') rotated by pa_rad, on the W-projection screen
+ * geometry (gridder.rs:3434-3470), normalised by the plane-zero peak and
+ * integral (gridder.rs:3485-3550); each plane becomes a full-plane cell of
+ * even shape 2*(S*sampling + R + 1), S its support from the 1e-3 threshold. */
+int chip_gridder_attach_aw_airy(chip_gridder *g, const chip_airy_params *p, uint32_t n_pointings,
+                                const double *phase_gradients, char **err);
+/* Read back the attached atlas: cells (n_cells entries) and, when
+ * pixels_dev != NULL, the pixels (complex64, n_pixels).  Either may be NULL. */
+int chip_aw_cells(const chip_gridder *g, uint32_t *n_cells, chip_cf_cell *cells, uint64_t *n_pixels,
+                  float *pixels_dev, char **err);
+/* plan_sample_detailed per sample (no weight / flag filters): plan_dev
+ * 6 int32 per sample (loc_x, loc_y, off_x, off_y, conjugate, CHIP_AW_* code),
+ * norm32_dev 2 floats (normalization, Complex32), norm64_dev 2 doubles
+ * (grid_normalization, Complex64); either norm output may be NULL. */
+int chip_aw_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, float *norm32_dev,
+                 double *norm64_dev, char **err);
+/* CASA PBMath1DAiry table of AiryVoltageLookup::new (lib.rs:57479-57503):
+ * 10,000 voltage values and the table's maximum radius (arcmin * GHz,
+ * casa_airy_max_radius_arcmin_ghz, lib.rs:57592-57605).  Host only. */
+int chip_airy_table(double dish_m, double blockage_m, float *values, double *max_radius, char **err);
 
 /* ---- phase-centre rotation of a row stream ------------------------------- */
 /* Host FP64: uvrot = gridft_uvw_rotation_matrix(source_dir, target_dir)

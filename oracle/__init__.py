@@ -161,6 +161,28 @@ def _declare(L):
                                            ctypes.c_int, _P]
     L.co_direct_predict.restype = None
     L.co_direct_predict.argtypes = [_U64, _U64, _D, _D, _P, _U64, _P, _P, _P, _P]
+    L.co_aw_kernel_origin.restype = _U64
+    L.co_aw_kernel_origin.argtypes = [_U64]
+    L.co_aw_plan.restype = ctypes.c_int
+    L.co_aw_plan.argtypes = [_U64, _U64, _D, _D, _U64, _P, _P, _D, _D, _D, _P, _P, _P]
+    L.co_aw_taps.restype = None
+    L.co_aw_taps.argtypes = [_U64, _P, _P, _P, _P, _P]
+    L.co_aw_grid_f64.restype = None
+    L.co_aw_grid_f64.argtypes = [_U64, _U64, _D, _D, _U64, _P, _P, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.co_aw_degrid.restype = None
+    L.co_aw_degrid.argtypes = [_U64, _U64, _D, _D, _U64, _P, _P, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P]
+    L.co_airy_max_radius.restype = _D
+    L.co_airy_max_radius.argtypes = [_D, _D]
+    L.co_airy_from_index.restype = ctypes.c_float
+    L.co_airy_from_index.argtypes = [_D, _D, _D, _D]
+    L.co_airy_table.restype = None
+    L.co_airy_table.argtypes = [_D, _D, _P]
+    L.co_airy_offsets.restype = ctypes.c_float
+    L.co_airy_offsets.argtypes = [_D, _D, _D, _D, _D, _P]
+    L.co_airy_radius.restype = ctypes.c_float
+    L.co_airy_radius.argtypes = [_D, _D, _D, _D]
+    L.co_airy_offsets_many.restype = None
+    L.co_airy_offsets_many.argtypes = [_U64, _P, _P, _D, _D, _D, _P, _P]
 
 
 def _ptr(a):
@@ -791,3 +813,127 @@ def rotate_rows(uvrot, phrot, uvw_m, freq_hz, vis):
                          _ptr(np.ascontiguousarray(phrot, np.float64)), _ptr(uvw), _ptr(f),
                          _ptr(v.view(np.float32)), _ptr(out), _ptr(shift))
     return out, shift, v
+
+
+# ---------------------------------------------------------------------------
+# AWProject CF cells (gridder.rs:2157-2815) and the Airy A-term (lib.rs:57436-57605)
+# ---------------------------------------------------------------------------
+
+AW_OK = 0
+AW_NON_FINITE_COORDINATE = 1
+AW_OUTSIDE_GRID = 2
+AW_KERNEL_INDEX_OUTSIDE_CELL = 3
+AW_INVALID_NORMALIZATION = 4
+
+
+class _CfCell(ctypes.Structure):
+    _fields_ = [("shape_x", ctypes.c_uint64), ("shape_y", ctypes.c_uint64),
+                ("x_support", ctypes.c_uint64), ("y_support", ctypes.c_uint64),
+                ("offset", ctypes.c_uint64)]
+
+
+def aw_kernel_origin(length: int) -> int:
+    return int(lib().co_aw_kernel_origin(length))
+
+
+class AwAtlas:
+    """CF cells sharing one oversampling: each cell is (pixels complex64
+    [shape_x, shape_y] indexed (x, y), x_support, y_support)."""
+
+    def __init__(self, cells, sampling):
+        self.sampling = int(sampling)
+        self.cells = [(np.ascontiguousarray(p, np.complex64), int(xs), int(ys)) for p, xs, ys in cells]
+        arr = (_CfCell * len(self.cells))()
+        off = 0
+        for k, (p, xs, ys) in enumerate(self.cells):
+            arr[k] = _CfCell(p.shape[0], p.shape[1], xs, ys, off)
+            off += p.size
+        self.table = arr
+        self.pixels = np.concatenate([p.ravel() for p, _, _ in self.cells]) if self.cells else np.zeros(0, np.complex64)
+        self.offsets = np.array([arr[k].offset for k in range(len(self.cells))], np.uint64)
+
+    def _cell_ptr(self, k):
+        return ctypes.cast(ctypes.byref(self.table, k * ctypes.sizeof(_CfCell)), ctypes.c_void_p)
+
+
+def aw_plan(gx, gy, du, dv, atlas, cell, u, v, w):
+    """plan_sample_detailed for one sample: (status, (loc_x, loc_y, off_x,
+    off_y, conj), norm complex64, grid_normalization complex128)."""
+    geo = np.zeros(5, np.int64)
+    n32 = np.zeros(2, np.float32)
+    n64 = np.zeros(2, np.float64)
+    rc = lib().co_aw_plan(gx, gy, du, dv, atlas.sampling, atlas._cell_ptr(cell), _ptr(atlas.pixels.view(np.float32)),
+                          u, v, w, _ptr(geo), _ptr(n32), _ptr(n64))
+    return int(rc), tuple(int(x) for x in geo), np.complex64(complex(n32[0], n32[1])), complex(n64[0], n64[1])
+
+
+def aw_taps(atlas, cell, geo, gradient):
+    """Phased taps of a plan, complex64 [2*ys+1][2*xs+1] (iy outer)."""
+    _, xs, ys = atlas.cells[cell]
+    out = np.zeros((2 * ys + 1, 2 * xs + 1), np.complex64)
+    g = np.ascontiguousarray(gradient, np.float64)
+    lib().co_aw_taps(atlas.sampling, atlas._cell_ptr(cell), _ptr(atlas.pixels.view(np.float32)),
+                     _ptr(np.asarray(geo, np.int64)), _ptr(g), _ptr(out.view(np.float32)))
+    return out
+
+
+def _aw_stream(atlas, gradients, u, v, w, cell_of, pointing_of):
+    n = len(u)
+    return (n, _f64(u), _f64(v), _f64(w), np.ascontiguousarray(cell_of, np.uint32),
+            np.ascontiguousarray(np.zeros(n) if pointing_of is None else pointing_of, np.uint32),
+            np.ascontiguousarray(gradients, np.float64).reshape(-1))
+
+
+def aw_grid_f64(gx, gy, du, dv, atlas, gradients, u, v, w, cell_of, values, pointing_of=None):
+    """Sequential grid_sample_planned_f64 over the samples: complex128 grid
+    [gx, gy] and the per-sample plan status."""
+    n, u, v, w, c, p, g = _aw_stream(atlas, gradients, u, v, w, cell_of, pointing_of)
+    vals = _c64_as_f32(values)
+    grid = np.zeros((gx, gy), np.complex128)
+    st = np.zeros(n, np.int32)
+    lib().co_aw_grid_f64(gx, gy, du, dv, atlas.sampling, ctypes.cast(atlas.table, ctypes.c_void_p),
+                         _ptr(atlas.pixels.view(np.float32)), _ptr(g), n, _ptr(u), _ptr(v), _ptr(w), _ptr(c),
+                         _ptr(p), _ptr(vals), _ptr(grid.view(np.float64)), _ptr(st))
+    return grid, st
+
+
+def aw_degrid(gx, gy, du, dv, atlas, gradients, u, v, w, cell_of, model, pointing_of=None):
+    """degrid_sample_planned per sample from a complex64 grid: complex64 values and plan status."""
+    n, u, v, w, c, p, g = _aw_stream(atlas, gradients, u, v, w, cell_of, pointing_of)
+    m = _c64_as_f32(model)
+    out = np.zeros(n, np.complex64)
+    st = np.zeros(n, np.int32)
+    lib().co_aw_degrid(gx, gy, du, dv, atlas.sampling, ctypes.cast(atlas.table, ctypes.c_void_p),
+                       _ptr(atlas.pixels.view(np.float32)), _ptr(g), n, _ptr(u), _ptr(v), _ptr(w), _ptr(c),
+                       _ptr(p), _ptr(m), _ptr(out.view(np.float32)), _ptr(st))
+    return out, st
+
+
+def airy_max_radius(dish, blockage) -> float:
+    return float(lib().co_airy_max_radius(dish, blockage))
+
+
+def airy_from_index(index, max_radius, dish, blockage) -> np.float32:
+    return np.float32(lib().co_airy_from_index(float(index), max_radius, dish, blockage))
+
+
+def airy_table(dish, blockage) -> np.ndarray:
+    out = np.empty(10000, np.float32)
+    lib().co_airy_table(dish, blockage, _ptr(out))
+    return out
+
+
+def airy_radius(radius, freq, dish, blockage) -> np.float32:
+    return np.float32(lib().co_airy_radius(radius, freq, dish, blockage))
+
+
+def airy_offsets(l, m, freq, dish, blockage, table=None):
+    """AiryVoltageLookup::evaluate_offsets over arrays of (l, m) radians."""
+    if table is None:
+        table = airy_table(dish, blockage)
+    l = _f64(l)
+    m = _f64(np.broadcast_to(m, l.shape))
+    out = np.empty(l.shape, np.float32)
+    lib().co_airy_offsets_many(l.size, _ptr(l), _ptr(np.ascontiguousarray(m)), freq, dish, blockage,
+                               _ptr(np.ascontiguousarray(table, np.float32)), _ptr(out))
+    return out

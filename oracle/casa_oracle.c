@@ -995,3 +995,270 @@ void co_rotate_rows(uint64_t n_rows, uint64_t n_chan, const double *uvrot, const
         }
     }
 }
+
+/* ------------------------------------------------------------------------- */
+/* AWProject convolution-function cells and the Airy A-term                   */
+/* ------------------------------------------------------------------------- */
+
+/* aw_kernel_origin, gridder.rs:2809-2815 */
+uint64_t co_aw_kernel_origin(uint64_t length) { return length % 2 == 0 ? length / 2 : length / 2 + 1; }
+
+/* One CF cell: stored shape, supports and the offset (in complex elements) of
+ * its pixels [x][y] (y fastest) in a shared pixel array. */
+typedef struct {
+    uint64_t shape_x, shape_y;
+    uint64_t x_support, y_support;
+    uint64_t offset;
+} co_cf_cell;
+
+/* awproject_sample_geometry, gridder.rs:2699-2748.  out = loc_x, loc_y,
+ * off_x, off_y, conjugate.  Returns 0, or the AwProjectSamplePlanRejection
+ * (gridder.rs:2133-2138 order): 1 NonFiniteCoordinate, 2 OutsideGrid,
+ * 3 KernelIndexOutsideCell. */
+int co_aw_geometry(uint64_t gx, uint64_t gy, double du, double dv, uint64_t sampling,
+                   const co_cf_cell *cell, double u, double v, double w, int64_t *out) {
+    const double pos_x = u / du + (double)gx / 2.0;
+    const double pos_y = -v / dv + (double)gy / 2.0;
+    if (!(isfinite(pos_x) && isfinite(pos_y) && isfinite(w))) return 1;
+    /* |pos| beyond 4e18 saturates the reference's cast and is off-grid either way */
+    if (fabs(pos_x) > 4.0e18 || fabs(pos_y) > 4.0e18) return 2;
+    const int64_t loc_x = (int64_t)round(pos_x), loc_y = (int64_t)round(pos_y);
+    const double s = (double)sampling;
+    const int64_t off_x = (int64_t)round(((double)loc_x - pos_x) * s);
+    const int64_t off_y = (int64_t)round(((double)loc_y - pos_y) * s);
+    const int64_t xs = (int64_t)cell->x_support, ys = (int64_t)cell->y_support;
+    if (loc_x - xs < 0 || loc_y - ys < 0 || loc_x + xs >= (int64_t)gx || loc_y + ys >= (int64_t)gy) return 2;
+    const int64_t cx = (int64_t)co_aw_kernel_origin(cell->shape_x);
+    const int64_t cy = (int64_t)co_aw_kernel_origin(cell->shape_y);
+    const int64_t si = (int64_t)sampling;
+    if (cx - xs * si + off_x < 0 || cy - ys * si + off_y < 0 || cx + xs * si + off_x >= (int64_t)cell->shape_x ||
+        cy + ys * si + off_y >= (int64_t)cell->shape_y)
+        return 3;
+    out[0] = loc_x;
+    out[1] = loc_y;
+    out[2] = off_x;
+    out[3] = off_y;
+    out[4] = w > 0.0 ? 1 : 0;
+    return 0;
+}
+
+/* casa_aw_axis_phase / casa_aw_phase_gradient_from_axes, gridder.rs:2758-2766:
+ * each axis phasor in f64 narrowed to f32, the product in f64 narrowed to f32. */
+static void co_aw_phasor(double phase_x, double phase_y, float *out) {
+    const float xr = (float)cos(phase_x), xi = (float)sin(phase_x);
+    const float yr = (float)cos(phase_y), yi = (float)sin(phase_y);
+    const double re = (double)xr * (double)yr - (double)xi * (double)yi;
+    const double im = (double)xr * (double)yi + (double)xi * (double)yr;
+    out[0] = (float)re;
+    out[1] = (float)im;
+}
+
+/* plan_sample_detailed, gridder.rs:2253-2311: geometry, then the f32 and f64
+ * sums of the (conjugated when w > 0) unphased taps, iy outer, ix inner.
+ * Returns the rejection code, 4 = InvalidNormalization.  norm32 = 2 floats,
+ * norm64 = 2 doubles. */
+int co_aw_plan(uint64_t gx, uint64_t gy, double du, double dv, uint64_t sampling, const co_cf_cell *cell,
+               const float *pixels, double u, double v, double w, int64_t *geo, float *norm32, double *norm64) {
+    const int rc = co_aw_geometry(gx, gy, du, dv, sampling, cell, u, v, w, geo);
+    if (rc) return rc;
+    const int64_t xs = (int64_t)cell->x_support, ys = (int64_t)cell->y_support, si = (int64_t)sampling;
+    const int64_t cx = (int64_t)co_aw_kernel_origin(cell->shape_x), cy = (int64_t)co_aw_kernel_origin(cell->shape_y);
+    const float *px = pixels + 2 * cell->offset;
+    float nr = 0.0f, ni = 0.0f;
+    double dr = 0.0, di = 0.0;
+    for (int64_t iy = -ys; iy <= ys; ++iy) {
+        const int64_t ky = cy + iy * si + geo[3];
+        for (int64_t ix = -xs; ix <= xs; ++ix) {
+            const int64_t kx = cx + ix * si + geo[2];
+            const float *t = px + 2 * ((uint64_t)kx * cell->shape_y + (uint64_t)ky);
+            const float tr = t[0], ti = geo[4] ? -t[1] : t[1];
+            nr += tr;
+            ni += ti;
+            dr += (double)tr;
+            di += (double)ti;
+        }
+    }
+    norm32[0] = nr;
+    norm32[1] = ni;
+    norm64[0] = dr;
+    norm64[1] = di;
+    if (!(isfinite(nr) && isfinite(ni) && (nr != 0.0f || ni != 0.0f) && isfinite(dr) && isfinite(di) &&
+          (dr != 0.0 || di != 0.0)))
+        return 4;
+    return 0;
+}
+
+/* for_each_grid_tap, gridder.rs:2636-2666: the phased taps of a plan, in
+ * (iy outer, ix inner) order: raw CF pixel, conjugated when w > 0, times the
+ * pointing phasor of (ix*s + off_x, iy*s + off_y), Complex32 product. */
+void co_aw_taps(uint64_t sampling, const co_cf_cell *cell, const float *pixels, const int64_t *geo,
+                const double *gradient, float *out) {
+    const int64_t xs = (int64_t)cell->x_support, ys = (int64_t)cell->y_support, si = (int64_t)sampling;
+    const int64_t cx = (int64_t)co_aw_kernel_origin(cell->shape_x), cy = (int64_t)co_aw_kernel_origin(cell->shape_y);
+    const float *px = pixels + 2 * cell->offset;
+    uint64_t k = 0;
+    for (int64_t iy = -ys; iy <= ys; ++iy) {
+        const int64_t ycoord = iy * si + geo[3];
+        for (int64_t ix = -xs; ix <= xs; ++ix) {
+            const int64_t xcoord = ix * si + geo[2];
+            const float *t = px + 2 * ((uint64_t)(cx + xcoord) * cell->shape_y + (uint64_t)(cy + ycoord));
+            const float tr = t[0], ti = geo[4] ? -t[1] : t[1];
+            float ph[2];
+            co_aw_phasor((double)xcoord * gradient[0], (double)ycoord * gradient[1], ph);
+            out[2 * k] = tr * ph[0] - ti * ph[1];
+            out[2 * k + 1] = tr * ph[1] + ti * ph[0];
+            ++k;
+        }
+    }
+}
+
+/* Sequential AW gridding over samples in order (grid_sample_planned_f64,
+ * gridder.rs:2318-2343): contribution = value * tap as Complex32, added to a
+ * Complex64 grid [gx][gy].  Per sample: cell index, pointing gradient
+ * (2 doubles per pointing), value (complex64).  Rejected samples are skipped;
+ * status[i] receives the plan code. */
+void co_aw_grid_f64(uint64_t gx, uint64_t gy, double du, double dv, uint64_t sampling, const co_cf_cell *cells,
+                    const float *pixels, const double *gradients, uint64_t n, const double *u, const double *v,
+                    const double *w, const uint32_t *cell_of, const uint32_t *pointing_of, const float *values,
+                    double *grid, int32_t *status) {
+    float *taps = NULL;
+    uint64_t cap = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const co_cf_cell *c = cells + cell_of[i];
+        int64_t geo[5];
+        float n32[2];
+        double n64[2];
+        const int rc = co_aw_plan(gx, gy, du, dv, sampling, c, pixels, u[i], v[i], w[i], geo, n32, n64);
+        if (status) status[i] = rc;
+        if (rc) continue;
+        const uint64_t T = (2 * c->x_support + 1) * (2 * c->y_support + 1);
+        if (T > cap) {
+            free(taps);
+            taps = (float *)malloc(2 * T * sizeof(float));
+            cap = T;
+        }
+        co_aw_taps(sampling, c, pixels, geo, gradients + 2 * pointing_of[i], taps);
+        const float vr = values[2 * i], vi = values[2 * i + 1];
+        const int64_t xs = (int64_t)c->x_support, ys = (int64_t)c->y_support;
+        uint64_t k = 0;
+        for (int64_t iy = -ys; iy <= ys; ++iy)
+            for (int64_t ix = -xs; ix <= xs; ++ix, ++k) {
+                const float tr = taps[2 * k], ti = taps[2 * k + 1];
+                const float cr = vr * tr - vi * ti, ci = vr * ti + vi * tr;
+                double *g = grid + 2 * ((uint64_t)(geo[0] + ix) * gy + (uint64_t)(geo[1] + iy));
+                g[0] += (double)cr;
+                g[1] += (double)ci;
+            }
+    }
+    free(taps);
+}
+
+/* degrid_sample_planned, gridder.rs:2359-2371: Complex32 sum of
+ * conj(tap) * grid in tap order, divided by conj(normalization) with
+ * num-complex 0.4's division ((a c + b d) / |z|^2, (b c - a d) / |z|^2).
+ * model: complex64 [gx][gy].  Rejected samples give 0 and their status. */
+void co_aw_degrid(uint64_t gx, uint64_t gy, double du, double dv, uint64_t sampling, const co_cf_cell *cells,
+                  const float *pixels, const double *gradients, uint64_t n, const double *u, const double *v,
+                  const double *w, const uint32_t *cell_of, const uint32_t *pointing_of, const float *model,
+                  float *out, int32_t *status) {
+    float *taps = NULL;
+    uint64_t cap = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const co_cf_cell *c = cells + cell_of[i];
+        int64_t geo[5];
+        float n32[2];
+        double n64[2];
+        const int rc = co_aw_plan(gx, gy, du, dv, sampling, c, pixels, u[i], v[i], w[i], geo, n32, n64);
+        if (status) status[i] = rc;
+        out[2 * i] = out[2 * i + 1] = 0.0f;
+        if (rc) continue;
+        const uint64_t T = (2 * c->x_support + 1) * (2 * c->y_support + 1);
+        if (T > cap) {
+            free(taps);
+            taps = (float *)malloc(2 * T * sizeof(float));
+            cap = T;
+        }
+        co_aw_taps(sampling, c, pixels, geo, gradients + 2 * pointing_of[i], taps);
+        const int64_t xs = (int64_t)c->x_support, ys = (int64_t)c->y_support;
+        float ar = 0.0f, ai = 0.0f;
+        uint64_t k = 0;
+        for (int64_t iy = -ys; iy <= ys; ++iy)
+            for (int64_t ix = -xs; ix <= xs; ++ix, ++k) {
+                const float tr = taps[2 * k], ti = -taps[2 * k + 1];
+                const float *g = model + 2 * ((uint64_t)(geo[0] + ix) * gy + (uint64_t)(geo[1] + iy));
+                const float pr = tr * g[0] - ti * g[1];
+                const float pi = tr * g[1] + ti * g[0];
+                ar += pr;
+                ai += pi;
+            }
+        const float c_ = n32[0], d_ = -n32[1];
+        const float ns = c_ * c_ + d_ * d_;
+        const float re = ar * c_ + ai * d_;
+        const float im = ai * c_ - ar * d_;
+        out[2 * i] = re / ns;
+        out[2 * i + 1] = im / ns;
+    }
+    free(taps);
+}
+
+/* casa_airy_max_radius_arcmin_ghz, lib.rs:57592-57605 */
+double co_airy_max_radius(double dish, double blockage) {
+    if (fabs(dish - 10.7) <= 1.0e-6 && fabs(blockage - 0.75) <= 1.0e-6) return 250.0;
+    if (fabs(dish - 6.25) <= 1.0e-6 && fabs(blockage - 0.75) <= 1.0e-6) return 500.0;
+    return 1.784 * 60.0;
+}
+
+/* airy_voltage_pattern_from_table_index, lib.rs:57557-57578 (j1: the C
+ * library's Bessel function of the first kind, order one). */
+float co_airy_from_index(double index, double max_radius, double dish, double blockage) {
+    const double n1 = 9999.0;
+    const double xmax = max_radius * 7.016 / (1.566 * 60.0) * dish / 24.5;
+    const double x = index * xmax / n1;
+    if (fabs(x) <= 2.220446049250313e-16) return 1.0f;
+    if (blockage <= 0.0) return (float)(2.0 * j1(x) / x);
+    const double area_ratio = (dish / blockage) * (dish / blockage);
+    const double area_norm = area_ratio - 1.0;
+    const double lr = dish / blockage;
+    return (float)((area_ratio * 2.0 * j1(x) / x - 2.0 * j1(x * lr) / (x * lr)) / area_norm);
+}
+
+/* AiryVoltageLookup::new, lib.rs:57479-57503: 10,000 entries. */
+void co_airy_table(double dish, double blockage, float *out) {
+    const double mr = co_airy_max_radius(dish, blockage);
+    for (int k = 0; k < 10000; ++k) out[k] = co_airy_from_index((double)k, mr, dish, blockage);
+}
+
+static int64_t co_airy_index(double radius_arcmin_ghz, double max_radius) {
+    double idx = floor(radius_arcmin_ghz * 9999.0 / max_radius);
+    if (idx < 0.0) idx = 0.0;
+    if (idx > 9999.0) idx = 9999.0;
+    return (int64_t)idx;
+}
+
+/* AiryVoltageLookup::evaluate_offsets, lib.rs:57505-57532: degrees narrowed
+ * to f32, radius in f32, arcmin*GHz narrowed to f32, truncated table index. */
+float co_airy_offsets(double l, double m, double freq, double dish, double blockage, const float *table) {
+    const double DEG = 57.29577951308232; /* f64::to_degrees multiplier, 180/pi */
+    if (!(isfinite(l) && isfinite(m) && isfinite(freq) && freq > 0.0)) return 0.0f;
+    const float ld = (float)(l * DEG), md = (float)(m * DEG);
+    const float rd = sqrtf(ld * ld + md * md);
+    const double r = (double)(float)((double)rd * 60.0 * (freq / 1.0e9));
+    const double mr = co_airy_max_radius(dish, blockage);
+    if (r > mr) return 0.0f;
+    return table[co_airy_index(r, mr)];
+}
+
+/* airy_voltage_pattern, lib.rs:57436-57453: one radius in radians. */
+float co_airy_radius(double radius, double freq, double dish, double blockage) {
+    const double DEG = 57.29577951308232;
+    if (!(isfinite(radius) && radius >= 0.0 && isfinite(freq) && freq > 0.0)) return 0.0f;
+    const double r = (double)(float)(radius * DEG * 60.0 * (freq / 1.0e9));
+    const double mr = co_airy_max_radius(dish, blockage);
+    if (r > mr) return 0.0f;
+    return co_airy_from_index((double)co_airy_index(r, mr), mr, dish, blockage);
+}
+
+void co_airy_offsets_many(uint64_t n, const double *l, const double *m, double freq, double dish, double blockage,
+                          const float *table, float *out) {
+    for (uint64_t i = 0; i < n; ++i) out[i] = co_airy_offsets(l[i], m[i], freq, dish, blockage, table);
+}

@@ -25,7 +25,8 @@ def test_header_constants_match_binding(pkg):
     defs = dict(re.findall(r"#define (CHIP_[A-Z0-9_]+) \(?(-?\d+)\)?", text))
     for name in ("MODE_PSF", "MODE_DIRTY", "MODE_RESIDUAL", "F32", "F64", "PAD_COMPOSITE_1_2",
                  "PAD_COMPOSITE_1_3", "PAD_NONE", "CYCLE_AUTO", "CYCLE_HIPFFT", "CYCLE_FUSED",
-                 "WEIGHT_UNIFORM", "WEIGHT_BRIGGS"):
+                 "WEIGHT_UNIFORM", "WEIGHT_BRIGGS", "AW_OK", "AW_NON_FINITE_COORDINATE", "AW_OUTSIDE_GRID",
+                 "AW_KERNEL_INDEX_OUTSIDE_CELL", "AW_INVALID_NORMALIZATION", "AW_NO_CELL"):
         assert int(defs["CHIP_" + name]) == getattr(pkg.abi, name), name
 
 
