@@ -42,6 +42,17 @@ WORKLOADS = {
     # reduced cube for quick checks (same geometry, fewer rows and channels)
     "cube16": dict(hours=0.25, dt=2.0, n_chan=16, f0=1.0e9, f1=2.0e9, nx=1024, planes=32,
                    sampling=None, n_src=200, flag_max=0.6, arm_m=21_000.0),
+    # AW-projection mosaic: 7 pointings on a hexagon at half-power spacing,
+    # RR/RL/LR/LL, Airy A-term (25 m dish, 2 m blockage), 6 h at 10 s,
+    # 16 channels 1.0-1.5 GHz, 2048^2 grid (1707^2 image), 64 W planes,
+    # oversampling 8, 100 Gaussian sources, per-pointing flags U(0, 0.4)
+    "mosaic7": dict(kind="mosaic", hours=6.0, dt=10.0, n_chan=16, n_corr=4, f0=1.0e9, f1=1.5e9, nx=1707,
+                    planes=64, sampling=8, n_src=100, flag_max=0.4, arm_m=2_500.0, pointings=7,
+                    dish=25.0, blockage=2.0),
+    # reduced mosaic for quick checks (same geometry, fewer rows)
+    "mosaic7s": dict(kind="mosaic", hours=1.0, dt=10.0, n_chan=4, n_corr=4, f0=1.0e9, f1=1.5e9, nx=1707,
+                     planes=64, sampling=8, n_src=100, flag_max=0.4, arm_m=2_500.0, pointings=7,
+                     dish=25.0, blockage=2.0),
 }
 
 
@@ -126,6 +137,141 @@ def make_cube(torch, wl, seed, dev):
     return dict(uvw=U, scale=torch.tensor(scale, dtype=torch.float64, device=dev), vis=vis, flag=flag,
                 weight=weight, model=model, cell=cell, nx=nx, n_rows=n_rows, n_chan=n_chan,
                 freq=freq, max_uv=max_uv, max_w=float(np.max(np.abs(uvw[:, 2])) * scale[-1]))
+
+
+def airy_power(torch, r, freq, dish, blockage):
+    """Airy power pattern for data synthesis, in the form of the voltage table
+    the A-term uses: V = (a^2 jinc(x) - jinc(a x)) / (a^2 - 1), a = dish /
+    blockage, x = pi dish sin(r) / lambda, jinc(z) = 2 J1(z) / z."""
+    a = dish / blockage
+    x = torch.clamp(torch.pi * dish * torch.sin(r) * freq / C, min=1e-9)
+
+    def jinc(z):
+        return 2.0 * torch.special.bessel_j1(z) / z
+
+    v = (a * a * jinc(x) - jinc(x * a)) / (a * a - 1.0)
+    return v * v
+
+
+def make_mosaic(torch, wl, seed, dev):
+    """7-pointing AW mosaic on the GPU: rows cycle through the pointings in
+    time; every row carries 16 channels x 4 correlations as 64 planes
+    (plane = chan * 4 + corr).  Visibilities of a pointing are the sky
+    attenuated by that pointing's Airy power pattern, phase-referenced to the
+    common phase centre."""
+    rng = np.random.default_rng(seed)
+    enu = y_array(rng, arm_m=wl["arm_m"])
+    uvw = uvw_tracks(enu, 34.0, 30.0, wl["hours"], wl["dt"])
+    n_base = len(enu) * (len(enu) - 1) // 2
+    n_rows = uvw.shape[0]
+    n_t = n_rows // n_base
+    nc, ncorr = wl["n_chan"], wl["n_corr"]
+    freq = np.linspace(wl["f0"], wl["f1"], nc)
+    plane_freq = np.repeat(freq, ncorr)
+    scale = plane_freq / C
+    max_uv = float(np.max(np.hypot(uvw[:, 0], uvw[:, 1])) * freq[-1] / C)
+    cell = 1.0 / (3.0 * max_uv)
+    nx = wl["nx"]
+    # pointings: centre + hexagon at half the power-beam FWHM of the band centre
+    fwhm = 1.02 * (C / (0.5 * (wl["f0"] + wl["f1"]))) / wl["dish"]
+    sep = 0.5 * fwhm
+    pl = [(0.0, 0.0)] + [(sep * np.cos(np.pi / 3 * k), sep * np.sin(np.pi / 3 * k)) for k in range(6)]
+    pl = np.asarray(pl[: wl["pointings"]])
+    pointing_of_t = np.arange(n_t) % len(pl)
+    pointing = np.repeat(pointing_of_t, n_base).astype(np.int32)
+    ns = wl["n_src"]
+    flux = np.exp(rng.uniform(np.log(0.01), np.log(1.0), ns))
+    half = 0.4 * nx * cell
+    l = rng.uniform(-half, half, ns)
+    m = rng.uniform(-half, half, ns)
+    sfw = rng.uniform(1.0, 10.0, ns) / (2.0 * np.sqrt(2.0 * np.log(2.0))) * cell
+    n1 = np.sqrt(1.0 - l * l - m * m) - 1.0
+    U = torch.tensor(uvw, dtype=torch.float64, device=dev)
+    P = torch.tensor(pointing, dtype=torch.int64, device=dev)
+    L_ = torch.tensor(l, dtype=torch.float64, device=dev)
+    M_ = torch.tensor(m, dtype=torch.float64, device=dev)
+    N_ = torch.tensor(n1, dtype=torch.float64, device=dev)
+    S_ = torch.tensor(flux, dtype=torch.float64, device=dev)
+    G_ = torch.tensor(2.0 * np.pi ** 2 * sfw ** 2, dtype=torch.float64, device=dev)
+    PL = torch.tensor(pl, dtype=torch.float64, device=dev)
+    # distance of every source from every pointing centre
+    dr = torch.sqrt((L_[None, :] - PL[:, 0:1]) ** 2 + (M_[None, :] - PL[:, 1:2]) ** 2)
+    pol = torch.tensor(0.05 * np.exp(1j * rng.uniform(0, 2 * np.pi, ns)), dtype=torch.complex128, device=dev)
+    vis = torch.empty((n_rows, nc * ncorr), dtype=torch.complex64, device=dev)
+    for c in range(nc):
+        f = freq[c]
+        pb = airy_power(torch, dr, f, wl["dish"], wl["blockage"])  # (pointings, sources)
+        u = U[:, 0:1] * (f / C)
+        v = U[:, 1:2] * (f / C)
+        w = U[:, 2:3] * (f / C)
+        ph = -2.0 * np.pi * (u * L_ + v * M_ + w * N_)
+        amp = S_ * torch.exp(-G_ * (u * u + v * v)) * pb[P]
+        e = torch.complex(torch.cos(ph), torch.sin(ph))
+        par = (amp * e).sum(1)
+        crs = (amp * e * pol).sum(1)
+        vis[:, 4 * c + 0] = par.to(torch.complex64)
+        vis[:, 4 * c + 1] = crs.to(torch.complex64)
+        vis[:, 4 * c + 2] = crs.conj().to(torch.complex64)
+        vis[:, 4 * c + 3] = par.to(torch.complex64)
+    frac = rng.uniform(0.0, wl["flag_max"], len(pl))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    flag = torch.rand((n_rows, nc * ncorr), generator=gen, device=dev) < \
+        torch.tensor(frac, device=dev)[P][:, None]
+    weight = torch.ones((n_rows, nc * ncorr), dtype=torch.float32, device=dev)
+    model = torch.zeros((nc * ncorr, nx, nx), dtype=torch.float32, device=dev)
+    xs = np.clip(np.round(l / cell + nx / 2).astype(int), 0, nx - 1)
+    ys = np.clip(np.round(nx / 2 - m / cell).astype(int), 0, nx - 1)
+    for k in range(nc * ncorr):
+        if k % 4 in (0, 3):
+            model[k, xs, ys] = torch.tensor(0.5 * flux, dtype=torch.float32, device=dev)
+    return dict(uvw=U, scale=torch.tensor(scale, dtype=torch.float64, device=dev), vis=vis, flag=flag,
+                weight=weight, model=model, cell=cell, nx=nx, n_rows=n_rows, n_chan=nc * ncorr,
+                freq=plane_freq, max_uv=max_uv, max_w=float(np.max(np.abs(uvw[:, 2])) * freq[-1] / C),
+                pointing=torch.tensor(pointing, dtype=torch.int32, device=dev), pointings=pl)
+
+
+def cpu_baseline_aw(pkg, g, data, cpu_seconds, log):
+    """Oracle AW residual refresh (degrid from a complex64 model grid, then
+    grid (V - p) w into a Complex64 grid, sequential as the reference's
+    AwProjector, one core) on a bounded sample of rows of one plane."""
+    import oracle
+
+    cells = [(c.cpu().numpy(), xs, ys) for c, xs, ys in g.aw_cells()]
+    atlas = oracle.AwAtlas(cells, g.info.sampling)
+    grads = data["gradients"]
+    uvw = data["uvw"].cpu().numpy()
+    scale = data["scale"].cpu().numpy()
+    pointing = data["pointing"].cpu().numpy()
+    gx, gy, du, dv = g.info.gx, g.info.gy, g.info.du, g.info.dv
+    w_incr = g.info.w_scale
+    c = 0
+    model = g.model_to_grid(data["model"][c])[0].cpu().numpy()
+    keep = ~data["flag"][:, c].cpu().numpy()
+    rows = np.nonzero(keep)[0]
+    rng = np.random.default_rng(1)
+    rng.shuffle(rows)
+    nvis, spent, chunk = 0, 0.0, 2000
+    grid = None
+    for k in range(0, len(rows), chunk):
+        r = np.sort(rows[k:k + chunk])
+        u, v, w = (uvw[r, j] * scale[c] for j in range(3))
+        cell_of = np.clip(np.round(np.sqrt(w_incr * np.abs(w))), 0, len(cells) - 1).astype(np.uint32)
+        vis = data["vis"][r, c].cpu().numpy()
+        wt = data["weight"][r, c].cpu().numpy()
+        t0 = time.perf_counter()
+        pred, _ = oracle.aw_degrid(gx, gy, du, dv, atlas, grads, u, v, w, cell_of, model, pointing[r])
+        vals = ((vis - pred) * wt).astype(np.complex64)
+        gk, _ = oracle.aw_grid_f64(gx, gy, du, dv, atlas, grads, u, v, w, cell_of, vals, pointing[r])
+        spent += time.perf_counter() - t0
+        grid = gk if grid is None else grid
+        nvis += len(r)
+        if spent >= cpu_seconds:
+            break
+    log(f"cpu baseline (AW): {nvis} vis in {spent:.2f} s on 1 thread")
+    return {"value": nvis / spent / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
+            "sample": f"oracle AWProject residual refresh (degrid + grid into Complex64 grids, sequential as "
+                      f"the reference's AwProjector), {nvis} unflagged visibilities of plane 0, FFTs excluded"}
 
 
 def cpu_baseline(pkg, g, data, cpu_seconds, log):
@@ -286,8 +432,9 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("calibration copy: 3 x 1 GiB device copies done")
+    mosaic = wl.get("kind") == "mosaic"
     t0 = time.perf_counter()
-    data = make_cube(torch, wl, args.seed, dev)
+    data = (make_mosaic if mosaic else make_cube)(torch, wl, args.seed, dev)
     torch.cuda.synchronize()
     log(f"data: {data['n_rows']} rows x {data['n_chan']} chan, cell {np.rad2deg(data['cell']) * 3600:.4f} arcsec, "
         f"max|uv| {data['max_uv']:.0f}, max|w| {data['max_w']:.0f} ({time.perf_counter() - t0:.1f} s)")
@@ -295,21 +442,35 @@ def main():
     nx, cell, n_chan = data["nx"], data["cell"], data["n_chan"]
     g = pkg.StandardGridder(nx, nx, cell, cell)
     t0 = time.perf_counter()
-    g.attach_wproject(data["max_w"], planes=wl["planes"], sampling=wl["sampling"])
-    kernel_build_s = time.perf_counter() - t0
-    sup, _, _ = g.wproject_tables()
-    log(f"W kernels: {g.info.planes} planes, sampling {g.info.sampling}, width {g.info.width}, "
-        f"supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
-    # Briggs robust 0 from one density grid per channel (library, cube convention)
-    natural = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
-    t0 = time.perf_counter()
-    wts, f2, _ = g.density_weights(natural, pkg.WEIGHT_BRIGGS, 0.0, per_plane=True, want_density=False)
-    torch.cuda.synchronize()
-    data["weight"] = wts.view(data["n_rows"], data["n_chan"])
-    del natural, wts
-    log(f"Briggs robust 0 weights from {n_chan} density grids ({time.perf_counter() - t0:.2f} s), "
-        f"f2 {f2.min():.3g}..{f2.max():.3g}")
-    batch = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
+    if mosaic:
+        # pointing phase gradients (lib.rs:56657-56671) from each pointing's
+        # pixel offset; CF cells from the Airy power pattern at the band centre
+        grads = [pkg.mosaic_phase_gradient(g.info.gx, g.info.gy, (lp / cell, -mp / cell), wl["sampling"])
+                 for lp, mp in data["pointings"]]
+        data["gradients"] = np.asarray(grads)
+        g.attach_aw_airy(wl["dish"], wl["blockage"], 0.5 * (wl["f0"] + wl["f1"]), wl["planes"], data["max_w"],
+                         sampling=wl["sampling"], phase_gradients=grads)
+        kernel_build_s = time.perf_counter() - t0
+        sup = np.array([c[1] for c in g.aw_cells()])
+        log(f"AW CF atlas: {len(sup)} Airy x W cells x {len(grads)} pointings = {g.info.planes} tap tables, "
+            f"sampling {g.info.sampling}, supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
+    else:
+        g.attach_wproject(data["max_w"], planes=wl["planes"], sampling=wl["sampling"])
+        kernel_build_s = time.perf_counter() - t0
+        sup, _, _ = g.wproject_tables()
+        log(f"W kernels: {g.info.planes} planes, sampling {g.info.sampling}, width {g.info.width}, "
+            f"supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
+        # Briggs robust 0 from one density grid per channel (library, cube convention)
+        natural = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
+        t0 = time.perf_counter()
+        wts, f2, _ = g.density_weights(natural, pkg.WEIGHT_BRIGGS, 0.0, per_plane=True, want_density=False)
+        torch.cuda.synchronize()
+        data["weight"] = wts.view(data["n_rows"], data["n_chan"])
+        del natural, wts
+        log(f"Briggs robust 0 weights from {n_chan} density grids ({time.perf_counter() - t0:.2f} s), "
+            f"f2 {f2.min():.3g}..{f2.max():.3g}")
+    batch = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"],
+                                pointing=data.get("pointing"))
     n_unflagged = int((~data["flag"]).sum())
     t0 = time.perf_counter()
     vs = g.visibility_set(batch)
@@ -450,7 +611,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(pkg, g, data, args.cpu_seconds, log)
+            cpu = (cpu_baseline_aw if mosaic else cpu_baseline)(pkg, g, data, args.cpu_seconds, log)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline failed: {e!r}")
 
@@ -468,11 +629,16 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: W-projection residual refresh (degrid+grid+FFTs), "
-                        f"{data['n_rows']} rows x {n_chan} channels, {n_unflagged} unflagged vis, "
-                        f"{nx}^2 image / {g.info.gx}^2 grid, {g.info.planes} W planes, sampling {g.info.sampling}",
+            "workload": (f"{args.workload}: AW-projection mosaic residual refresh (degrid+grid+FFTs), "
+                         f"{len(data['pointings'])} pointings, {data['n_rows']} rows x {n_chan} planes "
+                         f"(channels x correlations), {n_unflagged} unflagged vis, {nx}^2 image / {g.info.gx}^2 grid, "
+                         f"{len(sup)} W planes of Airy CFs, sampling {g.info.sampling}" if mosaic else
+                         f"{args.workload}: W-projection residual refresh (degrid+grid+FFTs), "
+                         f"{data['n_rows']} rows x {n_chan} channels, {n_unflagged} unflagged vis, "
+                         f"{nx}^2 image / {g.info.gx}^2 grid, {g.info.planes} W planes, sampling {g.info.sampling}"),
             "n_rows": data["n_rows"], "n_chan": n_chan, "n_vis_unflagged": n_unflagged,
-            "n_vis_gridded": n_valid, "grid": g.info.gx, "image": nx, "w_planes": g.info.planes,
+            "n_vis_gridded": n_valid, "grid": g.info.gx, "image": nx,
+            "w_planes": len(sup) if mosaic else g.info.planes,
             "w_sampling": g.info.sampling, "w_support_range": [int(sup.min()), int(sup.max())],
             "grid_point_updates_per_pass": st["taps"], "plan_bin_ms": plan_s * 1e3,
             "w_kernel_build_ms": kernel_build_s * 1e3, "parallelism": "single GPU",

@@ -80,10 +80,11 @@ size_t binned_tile_lds(const chip_gridder &g, uint32_t tile, uint32_t halo) {
     return (tap_floats + 4 * L * Ls) * sizeof(float);
 }
 
-constexpr int kSlots = 2;  // taps per lane kept in registers (footprints up to 128 taps)
+constexpr int kBinnedThreads = 512;  // 8 wavefronts per work item: more groups in flight per CU
+constexpr int kBig = 8;    // taps per lane per chunk of the large-footprint path (512 per chunk)
 
-template <int MODE, bool WP>
-__global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
+template <int MODE, bool WP, int KS>
+__global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
                                                 const uint32_t *__restrict__ gstart,
                                                 const uint2 *__restrict__ plan,
                                                 const float2 *__restrict__ vis,
@@ -148,43 +149,65 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
     }
     __syncthreads();
 
-    // Footprint state of this wavefront (uniform): anchor of the first tap
-    // and footprint side; per lane, for slots f = lane + 64*s: tap row/column,
-    // LDS cell, register accumulators and staged model values.
-    int cur_bx = INT_MIN, cur_by = 0, cur_side = 0, cur_T = 0;
-    int tx_s[kSlots], ty_s[kSlots], cell_s[kSlots];
-    float accr[kSlots], acci[kSlots], pacr[kSlots], paci[kSlots], mre[kSlots], mim[kSlots];
+    // Register slots (KS > 0).  With M = 2 halo + 1, slot s = lane + 64 k of
+    // a wavefront owns the LDS-tile cells (x, y) with x = px and y = py
+    // modulo M, (px, py) = (s / M, s % M): every footprint (side <= M) has
+    // exactly one cell per slot.  A slot accumulates in registers while the
+    // taps of successive groups land on the same cell (and keeps that cell's
+    // model value staged), and spills to the LDS tile with one atomic per
+    // component when its cell changes.  Groups are sorted by anchor cell, so
+    // consecutive footprints overlap and most slots keep their cell
+    // (J. W. Romein, "An efficient work-distribution strategy for gridding
+    // radio-telescope data on GPUs", ICS 2012).
+    constexpr int NS = KS > 0 ? KS : 1;
+    const int Mside = 2 * static_cast<int>(gp.halo) + 1;
+    const float invM = 1.0f / static_cast<float>(Mside);
+    int pxy[NS], cell_k[NS];
+    float accr[NS], acci[NS], pacr[NS], paci[NS], mre[NS], mim[NS];
 #pragma unroll
-    for (int s = 0; s < kSlots; ++s) {
-        tx_s[s] = ty_s[s] = 0;
-        cell_s[s] = 0;
-        accr[s] = acci[s] = pacr[s] = paci[s] = mre[s] = mim[s] = 0.0f;
+    for (int k = 0; k < NS; ++k) {
+        const int sl = lane + 64 * k;
+        const int px = static_cast<int>((static_cast<float>(sl) + 0.5f) * invM);
+        pxy[k] = (KS > 0 && sl < Mside * Mside) ? (px | ((sl - px * Mside) << 16)) : -1;
+        cell_k[k] = -1;
+        accr[k] = acci[k] = pacr[k] = paci[k] = mre[k] = mim[k] = 0.0f;
     }
-    auto flush_regs = [&]() {
-        if (cur_side == 0) return;
-#pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-            if (lane + 64 * s < cur_T) {
-                if (kPsf) {
-                    if (pacr[s] != 0.0f) atomicAdd(P_re + cell_s[s], pacr[s]);
-                    if (WP && paci[s] != 0.0f) atomicAdd(P_im + cell_s[s], paci[s]);
-                }
-                if (kAcc) {
-                    if (accr[s] != 0.0f) atomicAdd(A_re + cell_s[s], accr[s]);
-                    if (acci[s] != 0.0f) atomicAdd(A_im + cell_s[s], acci[s]);
-                }
+    auto spill = [&](int k) {
+        if (cell_k[k] >= 0) {
+            if (kPsf) {
+                if (pacr[k] != 0.0f) atomicAdd(P_re + cell_k[k], pacr[k]);
+                if (WP && paci[k] != 0.0f) atomicAdd(P_im + cell_k[k], paci[k]);
             }
-            accr[s] = acci[s] = pacr[s] = paci[s] = 0.0f;
+            if (kAcc) {
+                if (accr[k] != 0.0f) atomicAdd(A_re + cell_k[k], accr[k]);
+                if (acci[k] != 0.0f) atomicAdd(A_im + cell_k[k], acci[k]);
+            }
         }
+        accr[k] = acci[k] = pacr[k] = paci[k] = 0.0f;
     };
 
     const int R = gp.norm_radius;
     const uint32_t per = (it.count + nwaves - 1) / nwaves;
     const uint32_t g_begin = it.start + static_cast<uint32_t>(wave) * per;
     const uint32_t g_end = min(it.start + it.count, g_begin + per);
+    // The next group's bounds and plan are loaded one iteration ahead, so the
+    // dependent chain gstart -> plan -> tap table of a group overlaps the
+    // previous group's work instead of stalling every group.
+    uint32_t n_s0 = 0, n_s1 = 0;
+    uint2 n_pk = make_uint2(0u, 0u);
+    if (g_begin < g_end) {
+        n_s0 = gstart[g_begin];
+        n_s1 = gstart[g_begin + 1];
+        n_pk = plan[n_s0];
+    }
     for (uint32_t grp = g_begin; grp < g_end; ++grp) {
-        const uint32_t s0 = gstart[grp], s1 = gstart[grp + 1];
-        const uint2 pk = plan[s0];
+        const uint32_t s0 = n_s0, s1 = n_s1;
+        const uint2 pk = n_pk;
+        if (grp + 1 < g_end) {
+            n_s0 = s1;
+            n_s1 = gstart[grp + 2];
+            n_pk = plan[s1];
+        }
         const int ax = static_cast<int>(pk.x & 0xffffu) - x0;
         const int ay = static_cast<int>(pk.x >> 16) - y0;
         const uint32_t flags = pk.y;
@@ -257,19 +280,46 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
         const int bx = WP ? ax - (side >> 1) : ax;  // standard anchors are already x_start
         const int by = WP ? ay - (side >> 1) : ay;
 
-        if (T > 64 * kSlots) {
-            // Large footprint: no register caching, taps go straight to LDS.
-            flush_regs();
-            cur_side = 0;
+        if (KS == 0) {
+            // Supports above the register slots: taps straight to LDS, in
+            // chunks of 64 * kBig taps whose loads every lane issues together
+            // and keeps in registers for both the degrid and the grid pass
+            // when the footprint fits one chunk.
+            const float inv = 1.0f / static_cast<float>(side);
+            const int n_chunks = (T + 64 * kBig - 1) / (64 * kBig);
+            float2 tv[kBig];
+            auto load_chunk = [&](int ch) {
+#pragma unroll
+                for (int k = 0; k < kBig; ++k) {
+                    const int f = ch * 64 * kBig + 64 * k + lane;
+                    float2 c = make_float2(0.0f, 0.0f);
+                    if (f < T) {
+                        if (WP) {
+                            c = ktap[f];
+                        } else {
+                            const int tx = static_cast<int>((static_cast<float>(f) + 0.5f) * inv);
+                            c = make_float2(wxr[tx] * wyr[f - tx * side], 0.0f);
+                        }
+                    }
+                    tv[k] = c;
+                }
+            };
             float pr = 0.0f, pi = 0.0f;
             if (kModel) {
-                for (int f = lane; f < T; f += 64) {
-                    const int tx = f / side, ty = f - tx * side;
-                    const int cell = (bx + tx) * Ls + (by + ty);
-                    const float2 c = WP ? ktap[f] : make_float2(wxr[tx] * wyr[ty], 0.0f);
-                    const float cr = c.x, ci = -(c.y * csign);
-                    pr += cr * M_re[cell] - ci * M_im[cell];
-                    pi += cr * M_im[cell] + ci * M_re[cell];
+                for (int ch = 0; ch < n_chunks; ++ch) {
+                    load_chunk(ch);
+#pragma unroll
+                    for (int k = 0; k < kBig; ++k) {
+                        const int f = ch * 64 * kBig + 64 * k + lane;
+                        if (f < T) {
+                            const int tx = static_cast<int>((static_cast<float>(f) + 0.5f) * inv);
+                            const int cell = (bx + tx) * Ls + (by + f - tx * side);
+                            const float cr = tv[k].x, ci = -(tv[k].y * csign);
+                            const float mr = M_re[cell], mi = M_im[cell];
+                            pr += cr * mr - ci * mi;
+                            pi += cr * mi + ci * mr;
+                        }
+                    }
                 }
                 pr = wave_sum_f(pr);
                 pi = wave_sum_f(pi);
@@ -280,70 +330,67 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
                 }
             }
             if (MODE == kModeResidual) { vr = vr - pr * w; vi = vi - pi * w; }
-            for (int f = lane; f < T; f += 64) {
-                const int tx = f / side, ty = f - tx * side;
-                const int cell = (bx + tx) * Ls + (by + ty);
-                const float2 c = WP ? ktap[f] : make_float2(wxr[tx] * wyr[ty], 0.0f);
-                const float cr = c.x, ci = c.y * csign;
-                if (kPsf) {
-                    atomicAdd(P_re + cell, w * cr);
-                    if (WP) atomicAdd(P_im + cell, w * ci);
-                }
-                if (kAcc) {
-                    atomicAdd(A_re + cell, vr * cr - vi * ci);
-                    atomicAdd(A_im + cell, vr * ci + vi * cr);
+            for (int ch = 0; ch < n_chunks; ++ch) {
+                if (!(kModel && n_chunks == 1)) load_chunk(ch);
+#pragma unroll
+                for (int k = 0; k < kBig; ++k) {
+                    const int f = ch * 64 * kBig + 64 * k + lane;
+                    if (f < T) {
+                        const int tx = static_cast<int>((static_cast<float>(f) + 0.5f) * inv);
+                        const int cell = (bx + tx) * Ls + (by + f - tx * side);
+                        const float cr = tv[k].x, ci = tv[k].y * csign;
+                        if (kPsf) {
+                            atomicAdd(P_re + cell, w * cr);
+                            if (WP) atomicAdd(P_im + cell, w * ci);
+                        }
+                        if (kAcc) {
+                            atomicAdd(A_re + cell, vr * cr - vi * ci);
+                            atomicAdd(A_im + cell, vr * ci + vi * cr);
+                        }
+                    }
                 }
             }
             continue;
         }
 
-        if (bx != cur_bx || by != cur_by || side != cur_side) {
-            // The footprint moved: spill the register accumulators to LDS and
-            // set up the new footprint (and its model values).
-            flush_regs();
-            cur_bx = bx;
-            cur_by = by;
-            cur_side = side;
-            cur_T = T;
-            const float inv = 1.0f / static_cast<float>(side);
+        // Register-slot path: move every slot whose cell is in this
+        // footprint onto it, then accumulate.
+        float cr[NS], ci[NS];
+        float pr = 0.0f, pi = 0.0f;
 #pragma unroll
-            for (int s = 0; s < kSlots; ++s) {
-                const int f = lane + 64 * s;
-                const int tx = static_cast<int>((static_cast<float>(f) + 0.5f) * inv);
-                tx_s[s] = tx;
-                ty_s[s] = f - tx * side;
-                cell_s[s] = (bx + tx) * Ls + (by + ty_s[s]);
+        for (int k = 0; k < NS; ++k) {
+            cr[k] = ci[k] = 0.0f;
+            if (pxy[k] < 0) continue;
+            const int px = pxy[k] & 0xffff, py = pxy[k] >> 16;
+            // (px - bx) mod M and (py - by) mod M, bx, by >= 0
+            const int qx = static_cast<int>((static_cast<float>(bx - px + Mside - 1) + 0.5f) * invM);
+            const int qy = static_cast<int>((static_cast<float>(by - py + Mside - 1) + 0.5f) * invM);
+            const int i = px + qx * Mside - bx;
+            const int j = py + qy * Mside - by;
+            if (i >= side || j >= side) continue;
+            const int cell = (bx + i) * Ls + (by + j);
+            if (cell != cell_k[k]) {
+                spill(k);
+                cell_k[k] = cell;
                 if (kModel) {
-                    const bool on = f < T;
-                    mre[s] = on ? M_re[cell_s[s]] : 0.0f;
-                    mim[s] = on ? M_im[cell_s[s]] : 0.0f;
+                    mre[k] = M_re[cell];
+                    mim[k] = M_im[cell];
                 }
             }
-        }
-        // This group's taps.
-        float cr[kSlots], ci[kSlots];
-#pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
-            const int f = lane + 64 * s;
-            cr[s] = ci[s] = 0.0f;
-            if (f < T) {
-                if (WP) {
-                    const float2 c = ktap[f];
-                    cr[s] = c.x;
-                    ci[s] = c.y * csign;
-                } else {
-                    cr[s] = wxr[tx_s[s]] * wyr[ty_s[s]];
-                }
+            if (WP) {
+                const float2 c = ktap[i * side + j];
+                cr[k] = c.x;
+                ci[k] = c.y * csign;
+            } else {
+                cr[k] = wxr[i] * wyr[j];
+            }
+            if (kModel) {
+                // conj(cwt) * model
+                pr += cr[k] * mre[k] + ci[k] * mim[k];
+                pi += cr[k] * mim[k] - ci[k] * mre[k];
             }
         }
         if (kModel) {
-            float pr = 0.0f, pi = 0.0f;
-#pragma unroll
-            for (int s = 0; s < kSlots; ++s) {
-                // conj(cwt) * model
-                pr += cr[s] * mre[s] + ci[s] * mim[s];
-                pi += cr[s] * mim[s] - ci[s] * mre[s];
-            }
             pr = wave_sum_f(pr);
             pi = wave_sum_f(pi);
             if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
@@ -357,18 +404,19 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
             vi = vi - pi * w;
         }
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s) {
+        for (int k = 0; k < NS; ++k) {
             if (kPsf) {
-                pacr[s] += w * cr[s];
-                paci[s] += w * ci[s];
+                pacr[k] += w * cr[k];
+                paci[k] += w * ci[k];
             }
             if (kAcc) {
-                accr[s] += vr * cr[s] - vi * ci[s];
-                acci[s] += vr * ci[s] + vi * cr[s];
+                accr[k] += vr * cr[k] - vi * ci[k];
+                acci[k] += vr * ci[k] + vi * cr[k];
             }
         }
     }
-    flush_regs();
+#pragma unroll
+    for (int k = 0; k < NS; ++k) spill(k);
     if (MODE == kModeDegrid) return;
     __syncthreads();
 
@@ -392,6 +440,35 @@ __global__ __launch_bounds__(256) void k_binned(GridParams gp, const WorkItem *_
     }
 }
 
+template <int MODE, bool WP, int KS>
+static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *values, const float2 *model,
+                            float *psf, float *acc, float2 *vis_out, size_t lds, hipStream_t st) {
+    if (lds > 64 * 1024)
+        CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP, KS>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    // Events on the launch stream bracket the kernel (chip_vset_stats).
+    CHIP_HIP(hipEventRecord(vs.ev_start, st));
+    hipLaunchKernelGGL((k_binned<MODE, WP, KS>), dim3(static_cast<unsigned>(vs.n_items)), dim3(kBinnedThreads),
+                       lds, st, gp, vs.items.as<WorkItem>(), vs.gstart.as<uint32_t>(),
+                       vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
+                       vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
+    CHIP_HIP(hipGetLastError());
+    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
+    vs.timed = true;
+}
+
+// Register slots per lane for a tile halo: ceil((2 halo + 1)^2 / 64) rounded
+// up to an instantiated count, 0 (LDS-atomic chunks) beyond 9.
+static int register_slots(uint32_t halo) {
+    const uint32_t m = 2 * halo + 1, cells = m * m;
+    if (cells <= 64) return 1;
+    if (cells <= 128) return 2;
+    if (cells <= 256) return 4;
+    if (cells <= 384) return 6;
+    if (cells <= 576) return 9;
+    return 0;
+}
+
 template <int MODE, bool WP>
 static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
                             const float2 *model, float *psf, float *acc, float2 *vis_out,
@@ -401,18 +478,20 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
         fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
                                    std::to_string(gp.halo) + ")");
     if (vs.n_items == 0) return;
-    if (lds > 64 * 1024)
-        CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    // Events on the launch stream bracket the kernel (chip_vset_stats).
-    CHIP_HIP(hipEventRecord(vs.ev_start, st));
-    hipLaunchKernelGGL((k_binned<MODE, WP>), dim3(static_cast<unsigned>(vs.n_items)), dim3(256),
-                       lds, st, gp, vs.items.as<WorkItem>(), vs.gstart.as<uint32_t>(),
-                       vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
-                       vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
-    CHIP_HIP(hipGetLastError());
-    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
-    vs.timed = true;
+    const int ks = register_slots(gp.halo);
+    if constexpr (!WP) {
+        if (ks != 1) fail(CHIP_ERR_INVALID, "standard gridder halo must be the 7x7 support");
+        launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st);
+    } else {
+        switch (ks) {
+            case 1: launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            case 2: launch_binned_k<MODE, WP, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            case 4: launch_binned_k<MODE, WP, 4>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            case 6: launch_binned_k<MODE, WP, 6>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            case 9: launch_binned_k<MODE, WP, 9>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            default: launch_binned_k<MODE, WP, 0>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+        }
+    }
 }
 
 template <int MODE>
