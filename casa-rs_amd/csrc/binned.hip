@@ -190,44 +190,103 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     const uint32_t per = (it.count + nwaves - 1) / nwaves;
     const uint32_t g_begin = it.start + static_cast<uint32_t>(wave) * per;
     const uint32_t g_end = min(it.start + it.count, g_begin + per);
-    // The next group's bounds and plan are loaded one iteration ahead, so the
-    // dependent chain gstart -> plan -> tap table of a group overlaps the
-    // previous group's work instead of stalling every group.
-    uint32_t n_s0 = 0, n_s1 = 0;
-    uint2 n_pk = make_uint2(0u, 0u);
-    if (g_begin < g_end) {
-        n_s0 = gstart[g_begin];
-        n_s1 = gstart[g_begin + 1];
-        n_pk = plan[n_s0];
-    }
-    for (uint32_t grp = g_begin; grp < g_end; ++grp) {
-        const uint32_t s0 = n_s0, s1 = n_s1;
-        const uint2 pk = n_pk;
-        if (grp + 1 < g_end) {
-            n_s0 = s1;
-            n_s1 = gstart[grp + 2];
-            n_pk = plan[s1];
-        }
+
+    // Footprint of a group from its packed plan (wavefront-uniform).
+    struct Foot {
+        int side, bx, by;
+        const float2 *ktap;
+        const float *wxr, *wyr;
+        float csign;
+        float2 awn;  // AWProject normalisation: degridded values are divided by its conjugate
+    };
+    auto footprint = [&](const uint2 pk) {
+        Foot f;
         const int ax = static_cast<int>(pk.x & 0xffffu) - x0;
         const int ay = static_cast<int>(pk.x >> 16) - y0;
-        const uint32_t flags = pk.y;
+        f.ktap = nullptr;
+        f.wxr = f.wyr = nullptr;
+        f.csign = 1.0f;
+        f.awn = make_float2(1.0f, 0.0f);
+        if (!WP) {
+            f.side = kTaps;
+            f.wxr = taps + (pk.y & 0xffu) * kTaps;
+            f.wyr = taps + ((pk.y >> 8) & 0xffu) * kTaps;
+            f.bx = ax;  // standard anchors are already x_start
+            f.by = ay;
+        } else {
+            const int ox = static_cast<int>(pk.y & 0xffu) - 128;
+            const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
+            const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
+            f.side = 2 * gp.supports[plane] + 1;
+            const uint64_t po = static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
+                                static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R));
+            f.ktap = gp.compact + po * gp.tap_stride;
+            f.csign = (pk.y & kConjBit) ? -1.0f : 1.0f;
+            if (kModel && gp.aw_norm) f.awn = gp.aw_norm[po];
+            f.bx = ax - (f.side >> 1);
+            f.by = ay - (f.side >> 1);
+        }
+        return f;
+    };
+
+    // Per-lane inputs of one group, loaded a group ahead: the lane's sample
+    // (index s0 + lane) and, per register slot, the tap and LDS cell of the
+    // slot's footprint cell (cell -1: the slot is outside the footprint).
+    struct GroupData {
+        float2 v;     // visibility (ADJOINT: the value)
+        float wj;
+        uint32_t fj;  // plan flags of the sample
+        uint32_t idx; // original index (DEGRID)
+        float2 tap[NS];
+        int cell[NS];
+    };
+    auto load_group = [&](uint32_t s0, uint32_t s1, const Foot &f, GroupData &d) {
+        const uint32_t j = s0 + lane;
+        d.v = make_float2(0.0f, 0.0f);
+        d.wj = 0.0f;
+        d.fj = 0u;
+        d.idx = 0u;
+        if (j < s1) {
+            if (MODE == kModeAdjoint) {
+                d.v = values[index[j]];
+            } else if (MODE == kModeDegrid) {
+                d.idx = index[j];
+            } else {
+                d.fj = plan[j].y;
+                d.wj = wt[j];
+                d.v = vis[j];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            d.cell[k] = -1;
+            d.tap[k] = make_float2(0.0f, 0.0f);
+            if (KS == 0 || pxy[k] < 0) continue;
+            const int px = pxy[k] & 0xffff, py = pxy[k] >> 16;
+            // (px - bx) mod M and (py - by) mod M, bx, by >= 0
+            const int qx = static_cast<int>((static_cast<float>(f.bx - px + Mside - 1) + 0.5f) * invM);
+            const int qy = static_cast<int>((static_cast<float>(f.by - py + Mside - 1) + 0.5f) * invM);
+            const int i = px + qx * Mside - f.bx;
+            const int jj = py + qy * Mside - f.by;
+            if (i >= f.side || jj >= f.side) continue;
+            d.cell[k] = (f.bx + i) * Ls + (f.by + jj);
+            d.tap[k] = WP ? f.ktap[i * f.side + jj] : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
+        }
+    };
+
+    auto process = [&](uint32_t s0, uint32_t s1, const uint2 pk, const Foot &f, const GroupData &d) {
+        const uint32_t count = s1 - s0;
         // Group sums, lane-parallel over the samples:
         //   PSF/DIRTY: w = sum w_i (all samples), v = sum w_i V_i (finite V)
         //   RESIDUAL:  w = sum w_i, v = sum w_i V_i over usable samples
         //   ADJOINT:   v = sum values_i
         float vr = 0.0f, vi = 0.0f, w = 0.0f;
         if (MODE != kModeDegrid) {
-            for (uint32_t j = s0 + lane; j < s1; j += 64) {
+            auto add = [&](uint32_t fj, float wj, float2 vj) {
                 if (MODE == kModeAdjoint) {
-                    const float2 a = values[index[j]];
-                    vr += a.x;
-                    vi += a.y;
-                    continue;
-                }
-                const uint32_t fj = plan[j].y;
-                const float wj = wt[j];
-                const float2 vj = vis[j];
-                if (MODE == kModeResidual) {
+                    vr += vj.x;
+                    vi += vj.y;
+                } else if (MODE == kModeResidual) {
                     if (fj & kResidOkBit) {
                         w += wj;
                         vr += vj.x * wj;
@@ -240,8 +299,13 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                         vi += vj.y * wj;
                     }
                 }
+            };
+            if (static_cast<uint32_t>(lane) < count) add(d.fj, d.wj, d.v);
+            for (uint32_t j = s0 + 64 + lane; j < s1; j += 64) {
+                if (MODE == kModeAdjoint) add(0u, 0.0f, values[index[j]]);
+                else add(plan[j].y, wt[j], vis[j]);
             }
-            if (s1 - s0 > 1) {
+            if (count > 1) {
                 vr = wave_sum_f(vr);
                 vi = wave_sum_f(vi);
                 if (MODE != kModeAdjoint) w = wave_sum_f(w);
@@ -250,35 +314,19 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 vi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vi)));
                 w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(w)));
             }
-            if (MODE == kModeResidual && w == 0.0f) continue;
+            if (MODE == kModeResidual && w == 0.0f) return;
         }
-        // Footprint geometry (uniform).
-        int side;
-        const float2 *ktap = nullptr;
-        const float *wxr = nullptr, *wyr = nullptr;
-        float csign = 1.0f;
-        float2 awn = make_float2(1.0f, 0.0f);  // AWProject normalisation (degrid divides by its conjugate)
-        if (!WP) {
-            side = kTaps;
-            wxr = taps + (pk.y & 0xffu) * kTaps;
-            wyr = taps + ((pk.y >> 8) & 0xffu) * kTaps;
-        } else {
-            const int ox = static_cast<int>(pk.y & 0xffu) - 128;
-            const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
-            const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
-            side = 2 * gp.supports[plane] + 1;
-            ktap = gp.compact +
-                   (static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
-                    static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R))) *
-                       gp.tap_stride;
-            csign = (flags & kConjBit) ? -1.0f : 1.0f;
-            if (kModel && gp.aw_norm)
-                awn = gp.aw_norm[static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
-                                 static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R))];
-        }
-        const int T = side * side;
-        const int bx = WP ? ax - (side >> 1) : ax;  // standard anchors are already x_start
-        const int by = WP ? ay - (side >> 1) : ay;
+        auto write_degrid = [&](float pr, float pi) {
+            // every sample of the group has the same prediction
+            if (static_cast<uint32_t>(lane) < count) vis_out[d.idx] = make_float2(pr, pi);
+            for (uint32_t j = s0 + 64 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
+        };
+        const int side = f.side, T = side * side, bx = f.bx, by = f.by;
+        const float2 *ktap = f.ktap;
+        const float *wxr = f.wxr, *wyr = f.wyr;
+        const float csign = f.csign;
+        const float2 awn = f.awn;
+        (void)pk;
 
         if (KS == 0) {
             // Supports above the register slots: taps straight to LDS, in
@@ -325,8 +373,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 pi = wave_sum_f(pi);
                 if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
                 if (MODE == kModeDegrid) {
-                    for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
-                    continue;
+                    write_degrid(pr, pi);
+                    return;
                 }
             }
             if (MODE == kModeResidual) { vr = vr - pr * w; vi = vi - pi * w; }
@@ -350,7 +398,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                     }
                 }
             }
-            continue;
+            return;
         }
 
         // Register-slot path: move every slot whose cell is in this
@@ -360,15 +408,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             cr[k] = ci[k] = 0.0f;
-            if (pxy[k] < 0) continue;
-            const int px = pxy[k] & 0xffff, py = pxy[k] >> 16;
-            // (px - bx) mod M and (py - by) mod M, bx, by >= 0
-            const int qx = static_cast<int>((static_cast<float>(bx - px + Mside - 1) + 0.5f) * invM);
-            const int qy = static_cast<int>((static_cast<float>(by - py + Mside - 1) + 0.5f) * invM);
-            const int i = px + qx * Mside - bx;
-            const int j = py + qy * Mside - by;
-            if (i >= side || j >= side) continue;
-            const int cell = (bx + i) * Ls + (by + j);
+            const int cell = d.cell[k];
+            if (cell < 0) continue;
             if (cell != cell_k[k]) {
                 spill(k);
                 cell_k[k] = cell;
@@ -377,13 +418,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                     mim[k] = M_im[cell];
                 }
             }
-            if (WP) {
-                const float2 c = ktap[i * side + j];
-                cr[k] = c.x;
-                ci[k] = c.y * csign;
-            } else {
-                cr[k] = wxr[i] * wyr[j];
-            }
+            cr[k] = d.tap[k].x;
+            ci[k] = d.tap[k].y * csign;
             if (kModel) {
                 // conj(cwt) * model
                 pr += cr[k] * mre[k] + ci[k] * mim[k];
@@ -395,9 +431,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             pi = wave_sum_f(pi);
             if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
             if (MODE == kModeDegrid) {
-                // every sample of the group has the same prediction
-                for (uint32_t j = s0 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
-                continue;
+                write_degrid(pr, pi);
+                return;
             }
             // sum_i (V_i - p) w_i = sum_i w_i V_i - p sum_i w_i
             vr = vr - pr * w;
@@ -413,6 +448,49 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 accr[k] += vr * cr[k] - vi * ci[k];
                 acci[k] += vr * ci[k] + vi * cr[k];
             }
+        }
+    };
+
+    // Software pipeline over the wavefront's groups: while group g is
+    // processed, the taps and samples of group g + 1 and the bounds and plan
+    // of group g + 2 are in flight, so the dependent chain gstart -> plan ->
+    // tap table of a group no longer stalls the wavefront.
+    if (g_begin < g_end) {
+        uint32_t c_s0 = gstart[g_begin], c_s1 = gstart[g_begin + 1];
+        uint2 c_pk = plan[c_s0];
+        Foot c_f = footprint(c_pk);
+        GroupData c_d;
+        load_group(c_s0, c_s1, c_f, c_d);
+        uint32_t n_s0 = 0, n_s1 = 0;
+        uint2 n_pk = make_uint2(0u, 0u);
+        if (g_begin + 1 < g_end) {
+            n_s0 = c_s1;
+            n_s1 = gstart[g_begin + 2];
+            n_pk = plan[n_s0];
+        }
+        for (uint32_t grp = g_begin; grp < g_end; ++grp) {
+            Foot n_f = c_f;
+            GroupData n_d = c_d;
+            uint32_t m_s0 = 0, m_s1 = 0;
+            uint2 m_pk = make_uint2(0u, 0u);
+            if (grp + 1 < g_end) {
+                n_f = footprint(n_pk);
+                load_group(n_s0, n_s1, n_f, n_d);
+                if (grp + 2 < g_end) {
+                    m_s0 = n_s1;
+                    m_s1 = gstart[grp + 3];
+                    m_pk = plan[m_s0];
+                }
+            }
+            process(c_s0, c_s1, c_pk, c_f, c_d);
+            c_s0 = n_s0;
+            c_s1 = n_s1;
+            c_pk = n_pk;
+            c_f = n_f;
+            c_d = n_d;
+            n_s0 = m_s0;
+            n_s1 = m_s1;
+            n_pk = m_pk;
         }
     }
 #pragma unroll
