@@ -149,26 +149,30 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     }
     __syncthreads();
 
-    // Register slots (KS > 0).  With M = 2 halo + 1, slot s = lane + 64 k of
-    // a wavefront owns the LDS-tile cells (x, y) with x = px and y = py
-    // modulo M, (px, py) = (s / M, s % M): every footprint (side <= M) has
-    // exactly one cell per slot.  A slot accumulates in registers while the
-    // taps of successive groups land on the same cell (and keeps that cell's
-    // model value staged), and spills to the LDS tile with one atomic per
-    // component when its cell changes.  Groups are sorted by anchor cell, so
-    // consecutive footprints overlap and most slots keep their cell
-    // (J. W. Romein, "An efficient work-distribution strategy for gridding
-    // radio-telescope data on GPUs", ICS 2012).
+    // Register slots (KS > 0).  With M = 2 halo + 1 and P = floor(64 / M),
+    // lane l < P M owns the LDS-tile cells (x, y) with y = l % M and
+    // x = l / M + k P (modulo M), one x residue per slot k: every footprint
+    // (side <= M) has exactly one cell per (lane, slot).  A slot accumulates
+    // in registers while the taps of successive groups land on the same cell
+    // (and keeps that cell's model value staged), and spills to the LDS tile
+    // with one atomic per component when its cell changes.  Groups are sorted
+    // by anchor cell, so consecutive footprints overlap and most slots keep
+    // their cell (J. W. Romein, "An efficient work-distribution strategy for
+    // gridding radio-telescope data on GPUs", ICS 2012).  A lane's y residue
+    // is fixed, so per group the y offset is computed once and the x offsets
+    // of its slots follow by steps of P.
     constexpr int NS = KS > 0 ? KS : 1;
     const int Mside = 2 * static_cast<int>(gp.halo) + 1;
     const float invM = 1.0f / static_cast<float>(Mside);
-    int pxy[NS], cell_k[NS];
+    const int Pl = 64 / Mside;
+    const int lane_pg = static_cast<int>((static_cast<float>(lane) + 0.5f) * invM);  // lane / M
+    const int lane_py = lane - lane_pg * Mside;
+    uint32_t slot_mask = 0u;  // bit k: slot k owns an x residue
+    int cell_k[NS];
     float accr[NS], acci[NS], pacr[NS], paci[NS], mre[NS], mim[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        const int sl = lane + 64 * k;
-        const int px = static_cast<int>((static_cast<float>(sl) + 0.5f) * invM);
-        pxy[k] = (KS > 0 && sl < Mside * Mside) ? (px | ((sl - px * Mside) << 16)) : -1;
+        if (KS > 0 && lane_pg < Pl && lane_pg + k * Pl < Mside) slot_mask |= 1u << k;
         cell_k[k] = -1;
         accr[k] = acci[k] = pacr[k] = paci[k] = mre[k] = mim[k] = 0.0f;
     }
@@ -257,19 +261,24 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 d.v = vis[j];
             }
         }
+        // (lane_py - by) mod M once, then x offsets (lane_pg + k P - bx) mod M
+        // by steps of P; bx, by >= 0.
+        const int qy = static_cast<int>((static_cast<float>(f.by - lane_py + Mside - 1) + 0.5f) * invM);
+        const int jj = lane_py + qy * Mside - f.by;
+        const int qx = static_cast<int>((static_cast<float>(f.bx - lane_pg + Mside - 1) + 0.5f) * invM);
+        int i = lane_pg + qx * Mside - f.bx;
+        const bool in_y = jj < f.side;
+        const int cy = f.by + jj;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             d.cell[k] = -1;
             d.tap[k] = make_float2(0.0f, 0.0f);
-            if (KS == 0 || pxy[k] < 0) continue;
-            const int px = pxy[k] & 0xffff, py = pxy[k] >> 16;
-            // (px - bx) mod M and (py - by) mod M, bx, by >= 0
-            const int qx = static_cast<int>((static_cast<float>(f.bx - px + Mside - 1) + 0.5f) * invM);
-            const int qy = static_cast<int>((static_cast<float>(f.by - py + Mside - 1) + 0.5f) * invM);
-            const int i = px + qx * Mside - f.bx;
-            const int jj = py + qy * Mside - f.by;
-            if (i >= f.side || jj >= f.side) continue;
-            d.cell[k] = (f.bx + i) * Ls + (f.by + jj);
+            if (k > 0) {
+                i += Pl;
+                if (i >= Mside) i -= Mside;
+            }
+            if (!((slot_mask >> k) & 1u) || !in_y || i >= f.side) continue;
+            d.cell[k] = (f.bx + i) * Ls + cy;
             d.tap[k] = WP ? f.ktap[i * f.side + jj] : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
         }
     };
@@ -535,15 +544,17 @@ static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *v
     vs.timed = true;
 }
 
-// Register slots per lane for a tile halo: ceil((2 halo + 1)^2 / 64) rounded
-// up to an instantiated count, 0 (LDS-atomic chunks) beyond 9.
+// Register slots per lane for a tile halo: the x residues per lane,
+// ceil(M / floor(64 / M)) with M = 2 halo + 1, rounded up to an instantiated
+// count; 0 (LDS-atomic chunks) beyond 7 (halo > 10).
 static int register_slots(uint32_t halo) {
-    const uint32_t m = 2 * halo + 1, cells = m * m;
-    if (cells <= 64) return 1;
-    if (cells <= 128) return 2;
-    if (cells <= 256) return 4;
-    if (cells <= 384) return 6;
-    if (cells <= 576) return 9;
+    const uint32_t m = 2 * halo + 1, p = 64 / m;
+    if (p == 0) return 0;
+    const uint32_t need = (m + p - 1) / p;  // x residues per lane
+    if (need <= 1) return 1;
+    if (need <= 2) return 2;
+    if (need <= 4) return 4;
+    if (need <= 7) return 7;
     return 0;
 }
 
@@ -565,8 +576,7 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
             case 1: launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
             case 2: launch_binned_k<MODE, WP, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
             case 4: launch_binned_k<MODE, WP, 4>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
-            case 6: launch_binned_k<MODE, WP, 6>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
-            case 9: launch_binned_k<MODE, WP, 9>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            case 7: launch_binned_k<MODE, WP, 7>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
             default: launch_binned_k<MODE, WP, 0>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
         }
     }
