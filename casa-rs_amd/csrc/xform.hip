@@ -224,15 +224,29 @@ __global__ __launch_bounds__(kRowThreads) void k_fwd_rows(XGeom xg, Axis ay, con
         f[q] = live[q] ? xg.corr_x[gxx[q]] : 0.0f;
         sxv[q] = live[q] && xg.wproject ? xg.sinc_x[gxx[q]] : 1.0f;
     }
-    for (uint32_t i = jt; i < n; i += kLanes) {
+    // Two phases: every global load of the thread is issued before the first
+    // use, so the ~E loads are in flight together instead of one at a time.
+    float mv[E][2], cyv[E], syv[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t i = jt + static_cast<uint32_t>(t) * kLanes;
+        const bool in = i < n && i >= xg.blc_y && i < xg.blc_y + xg.ny;
+        cyv[t] = in ? xg.corr_y[i] : 0.0f;
+        syv[t] = in && xg.wproject ? xg.sinc_y[i] : 1.0f;
+        for (int q = 0; q < 2; ++q) mv[t][q] = in && live[q] ? row[q][i - xg.blc_y] : 0.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t i = jt + static_cast<uint32_t>(t) * kLanes;
+        if (i >= n) break;
         float v[2] = {0.0f, 0.0f};
         if (i >= xg.blc_y && i < xg.blc_y + xg.ny) {
             for (int q = 0; q < 2; ++q) {
                 if (!live[q]) continue;
                 // factor evaluated left to right as the reference: cx*cy [*sx*sy]
-                float fac = f[q] * xg.corr_y[i];
-                if (xg.wproject) fac = fac * sxv[q] * xg.sinc_y[i];
-                float val = row[q][i - xg.blc_y] * fac;
+                float fac = f[q] * cyv[t];
+                if (xg.wproject) fac = fac * sxv[q] * syv[t];
+                float val = mv[t][q] * fac;
                 if ((gxx[q] + i) & 1) val = -val;
                 v[q] = val;
             }
@@ -271,11 +285,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
     load_split_twiddles(tw, ax, tid, kThreads);
     const float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
-    for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
-        const uint32_t i = e >> 3;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (i >= xg.blc_x && i < xg.blc_x + xg.nx) v = h[(i - xg.blc_x) * kBlock + (e & 7)];
-        buf[e] = v;
+    float2 hv[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads, i = e >> 3;
+        hv[t] = (e < n * kBlock && i >= xg.blc_x && i < xg.blc_x + xg.nx) ? h[(i - xg.blc_x) * kBlock + (e & 7)]
+                                                                         : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
+        if (e < n * kBlock) buf[e] = hv[t];
     }
     __syncthreads();
     run_fft<false, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
@@ -310,16 +330,23 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
     const float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
     const float2 *lo = up + static_cast<uint64_t>(xg.nyb_h) * bstride;
     const uint32_t hy = xg.gy / 2;
-    for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
+    float2 gv[E], mv[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
         const uint32_t kx = e >> 3, lane = e & 7, ky = yb * kBlock + lane;
-        float2 v = make_float2(0.0f, 0.0f);
-        if (ky <= hy) {
+        gv[t] = mv[t] = make_float2(0.0f, 0.0f);
+        if (e < n * kBlock && ky <= hy) {
             const uint32_t mx = kx == 0 ? 0u : n - kx;
-            const float2 g = up[e];
-            const float2 m = (ky == 0 || ky == hy) ? up[mx * kBlock + lane] : lo[mx * kBlock + lane];
-            v = make_float2(0.5f * (g.x + m.x), 0.5f * (g.y - m.y));
+            gv[t] = up[e];
+            mv[t] = (ky == 0 || ky == hy) ? up[mx * kBlock + lane] : lo[mx * kBlock + lane];
         }
-        buf[e] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
+        if (e < n * kBlock)
+            buf[e] = make_float2(0.5f * (gv[t].x + mv[t].x), 0.5f * (gv[t].y - mv[t].y));
     }
     __syncthreads();
     run_fft<true, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
@@ -346,14 +373,28 @@ __global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, con
     const float2 *h = rows + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
     const uint32_t rows_live = xg.nx - x0;  // rows of this group inside the image (>= 1)
-    for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += kRowThreads) {
+    // elements per thread: nyb_h * 32 / 512 <= (n / 16 + 1) / 16 + 1 <= E / 2 + 1
+    constexpr int EL = E / 2 + 1;
+    const uint32_t total = xg.nyb_h * (kPairs * kBlock);
+    float2 av[EL], bv[EL];
+#pragma unroll
+    for (int t = 0; t < EL; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kRowThreads;
+        const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
+        const uint32_t k = yb * kBlock + lane, ra = 2 * pp;
+        av[t] = bv[t] = make_float2(0.0f, 0.0f);
+        if (e < total && k <= hy) {
+            if (ra < rows_live) av[t] = h[half_at(xg, yb, x0 + ra, lane)];
+            if (ra + 1 < rows_live) bv[t] = h[half_at(xg, yb, x0 + ra + 1, lane)];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < EL; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kRowThreads;
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
         const uint32_t k = yb * kBlock + lane;
-        if (k > hy) continue;
-        const uint32_t ra = 2 * pp;
-        const float2 zero = make_float2(0.0f, 0.0f);
-        const float2 A = ra < rows_live ? h[half_at(xg, yb, x0 + ra, lane)] : zero;
-        const float2 B = ra + 1 < rows_live ? h[half_at(xg, yb, x0 + ra + 1, lane)] : zero;
+        if (e >= total || k > hy) continue;
+        const float2 A = av[t], B = bv[t];
         buf[pp * n + k] = make_float2(A.x - B.y, A.y + B.x);
         if (k > 0 && k < hy) buf[pp * n + n - k] = make_float2(A.x + B.y, B.x - A.y);
     }
