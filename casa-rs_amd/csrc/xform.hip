@@ -190,6 +190,24 @@ __device__ __forceinline__ void load_split_twiddles(float2 *tw, const Axis &ax, 
     for (uint32_t m = tid; m < 64 + nhi; m += nthreads) tw[m] = m < 64 ? ax.twd[m] : ax.tw[64 * (m - 64)];
 }
 
+// Full twiddle table of a rows pass (n <= E * kLanes, so at most E / 4 + 1
+// entries per thread), loads issued together.
+template <int E>
+__device__ __forceinline__ void load_row_twiddles(float2 *tw, const Axis &ay, uint32_t tid, uint32_t n) {
+    constexpr int ET = E / 4 + 1;
+    float2 v[ET];
+#pragma unroll
+    for (int t = 0; t < ET; ++t) {
+        const uint32_t m = tid + static_cast<uint32_t>(t) * kRowThreads;
+        v[t] = m < n ? ay.tw[m] : make_float2(0.0f, 0.0f);
+    }
+#pragma unroll
+    for (int t = 0; t < ET; ++t) {
+        const uint32_t m = tid + static_cast<uint32_t>(t) * kRowThreads;
+        if (m < n) tw[m] = v[t];
+    }
+}
+
 // ---------------------------------------------------------------- kernels
 // Rows passes: 8 rows per workgroup of kRowThreads, packed as 4 complex
 // transforms of two real rows each (row 2p real part, row 2p + 1 imaginary
@@ -210,7 +228,7 @@ __global__ __launch_bounds__(kRowThreads) void k_fwd_rows(XGeom xg, Axis ay, con
     float2 *tw = buf + kPairs * n;
     const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
     const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
-    for (uint32_t m = tid; m < n; m += kRowThreads) tw[m] = ay.tw[m];
+    load_row_twiddles<E>(tw, ay, tid, n);
     // z = a + i b for rows a = x0 + 2p, b = a + 1 (apodised, signed, zero-padded)
     float f[2], sxv[2];
     const float *row[2];
@@ -367,7 +385,7 @@ __global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, con
     float2 *tw = buf + kPairs * n;
     const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
     const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
-    for (uint32_t m = tid; m < n; m += kRowThreads) tw[m] = ay.tw[m];
+    load_row_twiddles<E>(tw, ay, tid, n);
     // Rows a = x0 + 2p and b = a + 1 are Hermitian in ky: Z[k] = A[k] + i B[k]
     // over the full length (A[-k] = conj A[k]), whose inverse is a + i b.
     const float2 *h = rows + static_cast<uint64_t>(c) * xg.hstride;
@@ -402,6 +420,14 @@ __global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, con
     run_fft<true, E, false, kLanes, false>(buf, tw, ay, n, p, jt);
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(xg.gx) * xg.gy);
     const float ns = nsum[c], pk = peak[c];
+    // y corrections of this thread's columns, loaded together (ny <= n <= E kLanes)
+    float cyv[E], syv[E];
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t y = jt + static_cast<uint32_t>(t) * kLanes, gyy = xg.blc_y + y;
+        cyv[t] = y < xg.ny ? xg.corr_y[gyy] : 0.0f;
+        syv[t] = y < xg.ny && xg.wproject ? xg.sinc[gyy] : 1.0f;
+    }
     for (int q = 0; q < 2; ++q) {
         const uint32_t x = x0 + 2 * p + q;
         if (x >= xg.nx) break;
@@ -409,20 +435,23 @@ __global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, con
         const float cx = xg.corr_x[gxx];
         const float sincx = xg.wproject ? xg.sinc[gxx] : 1.0f;
         float *o = out + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
-        for (uint32_t y = jt; y < xg.ny; y += kLanes) {
+#pragma unroll
+        for (int t = 0; t < E; ++t) {
+            const uint32_t y = jt + static_cast<uint32_t>(t) * kLanes;
+            if (y >= xg.ny) break;
             const uint32_t gyy = xg.blc_y + y;
             uint32_t i = gyy + n / 2;
             if (i >= n) i -= n;
             const float2 z = buf[p * n + i];
             float re = (q == 0 ? z.x : z.y) * scale;
             if ((gxx + gyy + xg.base) & 1) re = -re;
-            const float factor = cx * xg.corr_y[gyy];
+            const float factor = cx * cyv[t];
             float v;
             if (!xg.wproject) {
                 v = re * factor;  // gridder.rs:1646-1657
                 v = v / ns / pk;  // lib.rs:68928-68929
             } else {
-                const float sf = sincx * xg.sinc[gyy];
+                const float sf = sincx * syv[t];
                 v = fabsf(sf) > 1.0e-6f ? re * (factor / sf) : 0.0f;  // gridder.rs:1685-1715
                 v = 2.0f * v / ns / pk;                                // lib.rs:70960-70961
             }
