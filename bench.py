@@ -544,9 +544,33 @@ def main():
     rate32 = n_unflagged / t32 / 1e6
     log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
         f"binned kernel {np.mean(kernel_ms):.3f} ms")
+    sched = vs.schedule()
+    log(f"binned schedule: {sched['threads']} threads x {sched['groups_per_item']} groups per item "
+        f"(autotuned over {len(sched['timed'])} candidates: "
+        + ", ".join(f"{a}x{b} {ms:.3f} ms" for a, b, ms in sched["timed"]) + ")")
     result_extra.update(transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
                         transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
-                        transform_fwd_inv_ms=info["transform_ms"])
+                        transform_fwd_inv_ms=info["transform_ms"],
+                        binned_schedule={"threads": sched["threads"], "groups_per_item": sched["groups_per_item"],
+                                         "candidates_timed": len(sched["timed"])})
+    if info["strategy"] == pkg.CYCLE_FUSED:
+        # the same step replayed from a captured HIP graph (one launch per cycle)
+        vs.set_cycle_graph(True)
+        r_g = torch.empty_like(r32)
+        vs.major_cycle(model, nsum, peak32, out=r_g)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            vs.major_cycle(model, nsum, peak32, out=r_g)
+        torch.cuda.synchronize()
+        tg = (time.perf_counter() - t0) / args.steps
+        diff = float(torch.max(torch.abs(r_g - r32)) / torch.max(torch.abs(r32)))
+        result_extra.update(graph_ms_per_step=tg * 1e3, graph_mvis_s=n_unflagged / tg / 1e6,
+                            graph_vs_eager_max_diff=diff, graph_replays=vs.cycle_replays())
+        log(f"same step replayed from a HIP graph: {tg * 1e3:.2f} ms/step ({n_unflagged / tg / 1e6:.1f} Mvis/s); "
+            f"graph vs eager residual diff {diff:.2e} of peak")
+        vs.set_cycle_graph(False)
+        del r_g
     if info["strategy"] == pkg.CYCLE_FUSED and args.compare_hipfft:
         # the same step with hipFFT 2-D plans and separate placement / crop kernels
         r_h = torch.empty_like(r32)

@@ -47,11 +47,12 @@ PAD_NONE = 2
 EXPORTS = (
     "chip_ctx_create", "chip_ctx_destroy", "chip_ctx_set_stream", "chip_error_free", "chip_version",
     "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
-    "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_plan",
+    "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_gridder_wkernel_budget",
+    "chip_plan",
     "chip_density_weights", "chip_phase_rotation", "chip_rotate_rows",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
-    "chip_vset_cycle_strategy",
+    "chip_vset_cycle_strategy", "chip_vset_cycle_graph", "chip_vset_schedule", "chip_vset_schedule_info",
     "chip_grid_plain", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
     "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_aw_cells", "chip_aw_plan",
@@ -183,6 +184,7 @@ def load():
         "chip_max_abs_w": (ctypes.c_int, [_P, ctypes.POINTER(Samples), ctypes.POINTER(ctypes.c_double), _PP]),
         "chip_gridder_attach_wproject": (ctypes.c_int, [_P, ctypes.c_double, _U32, _U32, _PP]),
         "chip_wproject_tables": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
+        "chip_gridder_wkernel_budget": (ctypes.c_int, [_P, _U64, _P, _PP]),
         "chip_plan": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _P, _PP]),
         "chip_phase_rotation": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
         "chip_rotate_rows": (ctypes.c_int, [_P, _U64, _U64, _P, _P, _P, _P, _P, _P, _P, _PP]),
@@ -198,6 +200,9 @@ def load():
         "chip_vset_major_cycle": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
         "chip_vset_cycle_info": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
         "chip_vset_cycle_strategy": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
+        "chip_vset_cycle_graph": (ctypes.c_int, [_P, ctypes.c_int, _P, _PP]),
+        "chip_vset_schedule": (ctypes.c_int, [_P, _U32, _U32, _PP]),
+        "chip_vset_schedule_info": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
         "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
         "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),

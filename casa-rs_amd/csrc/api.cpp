@@ -15,6 +15,10 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
                       float *out);
 void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], double *transform_ms);
 void vset_cycle_strategy(chip_vset &vs, int strategy);
+void vset_cycle_graph(chip_vset &vs, int enable);
+void vset_schedule(chip_vset &vs, uint32_t threads, uint32_t gpi);
+uint64_t vset_cycle_replays(const chip_vset &vs);
+void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev);
 }  // namespace chip
 
 using namespace chip;
@@ -260,8 +264,23 @@ int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernel
         set_device(g->ctx);
         if (supports) std::memcpy(supports, g->supports.data(), g->supports.size() * sizeof(int32_t));
         if (norm_table) std::memcpy(norm_table, g->norm_table.data(), g->norm_table.size() * sizeof(float));
-        if (kernels)
-            CHIP_HIP(hipMemcpy(kernels, g->d_kernels.p, g->d_kernels.bytes, hipMemcpyDeviceToHost));
+        if (kernels) {
+            const uint64_t bytes = static_cast<uint64_t>(g->width) * g->width * g->planes * sizeof(float2);
+            DevBuf tmp(bytes);
+            wk_all_kernels(*g, tmp.as<float2>());
+            CHIP_HIP(hipMemcpy(kernels, tmp.p, bytes, hipMemcpyDeviceToHost));
+        }
+    });
+}
+
+int chip_gridder_wkernel_budget(chip_gridder *g, uint64_t max_table_bytes, uint32_t *chunk_planes, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        set_device(g->ctx);
+        g->wk_budget = max_table_bytes;
+        if (g->wproject && !g->aw)
+            build_wproject(*g, g->wk.max_abs_w, g->wk.explicit_planes, g->wk.sampling_arg);
+        if (chunk_planes) *chunk_planes = (g->wproject && !g->aw) ? g->wk_chunk : 0u;
     });
 }
 
@@ -390,6 +409,35 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
         if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
         set_device(vs->g->ctx);
         vset_major_cycle(*vs, model_dev, nsum, psf_peak, residual_out_dev);
+    });
+}
+
+int chip_vset_schedule(chip_vset *vs, uint32_t threads, uint32_t groups_per_item, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        set_device(vs->g->ctx);
+        vset_schedule(*vs, threads, groups_per_item);
+    });
+}
+
+int chip_vset_schedule_info(const chip_vset *vs, uint32_t *threads, uint32_t *groups_per_item, uint32_t *n_timed,
+                            double *timed, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        if (threads) *threads = vs->threads;
+        if (groups_per_item) *groups_per_item = vs->groups_per_item;
+        if (n_timed) *n_timed = static_cast<uint32_t>(vs->tune_ms.size() / 3);
+        if (timed) std::memcpy(timed, vs->tune_ms.data(), vs->tune_ms.size() * sizeof(double));
+    });
+}
+
+int chip_vset_cycle_graph(chip_vset *vs, int enable, uint64_t *replays, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        if (enable != -1 && enable != 0 && enable != 1) fail(CHIP_ERR_INVALID, "enable must be -1, 0 or 1");
+        set_device(vs->g->ctx);
+        if (replays) *replays = vset_cycle_replays(*vs);
+        if (enable != -1) vset_cycle_graph(*vs, enable);
     });
 }
 

@@ -384,6 +384,7 @@ void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, con
 
     g.wproject = true;
     g.aw = true;
+    g.wk_chunk = 0;  // the atlas is always resident
     g.aw_cells = n_cells;
     g.aw_pointings = n_pointings;
     g.aw_w_incr = w_increment;

@@ -58,6 +58,8 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
     gp.compact = g.wproject ? g.d_compact.as<float2>() : nullptr;
     gp.supports = g.wproject ? g.d_supports.as<int32_t>() : nullptr;
+    gp.plane_lo = 0;
+    gp.plane_hi = UINT32_MAX;
     gp.aw_norm = g.aw ? g.d_aw_tnorm.as<float2>() : nullptr;
     return gp;
 }
@@ -80,12 +82,14 @@ size_t binned_tile_lds(const chip_gridder &g, uint32_t tile, uint32_t halo) {
     return (tap_floats + 4 * L * Ls) * sizeof(float);
 }
 
-constexpr int kBinnedThreads = 512;  // 8 wavefronts per work item: more groups in flight per CU
+// Most threads per workgroup (8 wavefronts per work item); the schedule
+// autotuner may launch fewer (any multiple of 64).
+constexpr int kBinnedThreads = 512;
 constexpr int kBig = 8;    // taps per lane per chunk of the large-footprint path (512 per chunk)
 
 template <int MODE, bool WP, int KS>
 __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
-                                                const uint32_t *__restrict__ gstart,
+                                                const uint4 *__restrict__ grec,
                                                 const uint2 *__restrict__ plan,
                                                 const float2 *__restrict__ vis,
                                                 const float *__restrict__ wt,
@@ -222,7 +226,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
             const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
             f.side = 2 * gp.supports[plane] + 1;
-            const uint64_t po = static_cast<uint64_t>(plane) * gp.n_off * gp.n_off +
+            const uint64_t po = static_cast<uint64_t>(plane - static_cast<int>(gp.plane_lo)) * gp.n_off * gp.n_off +
                                 static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R));
             f.ktap = gp.compact + po * gp.tap_stride;
             f.csign = (pk.y & kConjBit) ? -1.0f : 1.0f;
@@ -460,36 +464,53 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         }
     };
 
-    // Software pipeline over the wavefront's groups: while group g is
-    // processed, the taps and samples of group g + 1 and the bounds and plan
-    // of group g + 2 are in flight, so the dependent chain gstart -> plan ->
-    // tap table of a group no longer stalls the wavefront.
+    // Group records (first sample, end, packed plan of the group) are read 64
+    // at a time with one coalesced vector load, lane l holding the record of
+    // group base + l, and the next 64 are prefetched a batch ahead.  A group's
+    // record moves to scalar registers with readlane, so its footprint is
+    // scalar arithmetic and no scalar load waits behind the LDS traffic of the
+    // group being processed.  Software pipeline: while group g is processed,
+    // the taps and samples of group g + 1 are in flight.
+    auto rec_load = [&](uint32_t base) {
+        const uint32_t g = base + static_cast<uint32_t>(lane);
+        return g < g_end ? grec[g] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto rec_at = [&](const uint4 &rb, uint32_t k) {
+        const int kk = static_cast<int>(k);
+        return make_uint4(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.x), kk)),
+                          static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.y), kk)),
+                          static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.z), kk)),
+                          static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.w), kk)));
+    };
     if (g_begin < g_end) {
-        uint32_t c_s0 = gstart[g_begin], c_s1 = gstart[g_begin + 1];
-        uint2 c_pk = plan[c_s0];
+        uint32_t base = g_begin;
+        uint4 rb = rec_load(base);
+        uint4 rn = rec_load(base + 64);
+        const uint4 c0 = rec_at(rb, 0);
+        uint32_t c_s0 = c0.x, c_s1 = c0.y;
+        uint2 c_pk = make_uint2(c0.z, c0.w);
         Foot c_f = footprint(c_pk);
         GroupData c_d;
         load_group(c_s0, c_s1, c_f, c_d);
-        uint32_t n_s0 = 0, n_s1 = 0;
-        uint2 n_pk = make_uint2(0u, 0u);
-        if (g_begin + 1 < g_end) {
-            n_s0 = c_s1;
-            n_s1 = gstart[g_begin + 2];
-            n_pk = plan[n_s0];
-        }
         for (uint32_t grp = g_begin; grp < g_end; ++grp) {
             Foot n_f = c_f;
             GroupData n_d = c_d;
-            uint32_t m_s0 = 0, m_s1 = 0;
-            uint2 m_pk = make_uint2(0u, 0u);
+            uint32_t n_s0 = 0, n_s1 = 0;
+            uint2 n_pk = make_uint2(0u, 0u);
             if (grp + 1 < g_end) {
+                uint32_t k = grp + 1 - base;
+                if (k == 64) {
+                    base += 64;
+                    rb = rn;
+                    rn = rec_load(base + 64);
+                    k = 0;
+                }
+                const uint4 nr = rec_at(rb, k);
+                n_s0 = nr.x;
+                n_s1 = nr.y;
+                n_pk = make_uint2(nr.z, nr.w);
                 n_f = footprint(n_pk);
                 load_group(n_s0, n_s1, n_f, n_d);
-                if (grp + 2 < g_end) {
-                    m_s0 = n_s1;
-                    m_s1 = gstart[grp + 3];
-                    m_pk = plan[m_s0];
-                }
             }
             process(c_s0, c_s1, c_pk, c_f, c_d);
             c_s0 = n_s0;
@@ -497,9 +518,6 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             c_pk = n_pk;
             c_f = n_f;
             c_d = n_d;
-            n_s0 = m_s0;
-            n_s1 = m_s1;
-            n_pk = m_pk;
         }
     }
 #pragma unroll
@@ -527,19 +545,58 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     }
 }
 
+// The stream is binned for the W-table residency the gridder had when the set
+// was created.
+static void check_wk_chunk(const chip_vset &vs) {
+    const chip_gridder &g = *vs.g;
+    const uint32_t now = (g.wproject && !g.aw) ? g.wk_chunk : 0u;
+    if (now != vs.wk_chunk)
+        fail(CHIP_ERR_INVALID, "the W kernel residency changed after the visibility set was created");
+}
+
 template <int MODE, bool WP, int KS>
 static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *values, const float2 *model,
-                            float *psf, float *acc, float2 *vis_out, size_t lds, hipStream_t st) {
+                            float *psf, float *acc, float2 *vis_out, size_t lds, hipStream_t st, bool timed) {
     if (lds > 64 * 1024)
         CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP, KS>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    // Events on the launch stream bracket the kernel (chip_vset_stats).
+    auto launch = [&](const GridParams &p, uint64_t first, uint64_t count) {
+        hipLaunchKernelGGL((k_binned<MODE, WP, KS>), dim3(static_cast<unsigned>(count)), dim3(vs.threads),
+                           lds, st, p, vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(),
+                           vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
+                           vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
+        CHIP_HIP(hipGetLastError());
+    };
+    if (vs.threads == 0 || vs.threads > kBinnedThreads || vs.threads % 64)
+        fail(CHIP_ERR_INVALID, "binned kernel threads must be a multiple of 64 up to 512");
+    if (!timed) {  // captured into a graph: kernels only
+        if (vs.wk_chunk) fail(CHIP_ERR_INVALID, "W tables recomputed on the fly cannot be captured");
+        launch(gp, 0, vs.n_items);
+        vs.timed = false;
+        return;
+    }
+    // Events on the launch stream bracket the kernel (chip_vset_stats); with
+    // W tables recomputed on the fly they bracket the recomputation too.
     CHIP_HIP(hipEventRecord(vs.ev_start, st));
-    hipLaunchKernelGGL((k_binned<MODE, WP, KS>), dim3(static_cast<unsigned>(vs.n_items)), dim3(kBinnedThreads),
-                       lds, st, gp, vs.items.as<WorkItem>(), vs.gstart.as<uint32_t>(),
-                       vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
-                       vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
-    CHIP_HIP(hipGetLastError());
+    if (!vs.wk_chunk) {
+        launch(gp, 0, vs.n_items);
+    } else {
+        chip_gridder &g = *vs.g;
+        WkScratch ws;
+        wk_prepare(g, ws);
+        GridParams p = gp;
+        for (size_t c = 0; c + 1 < vs.chunk_items.size(); ++c) {
+            const uint64_t first = vs.chunk_items[c], count = vs.chunk_items[c + 1] - first;
+            if (count == 0) continue;  // no samples in these planes: nothing to recompute
+            const uint32_t lo = static_cast<uint32_t>(c) * vs.wk_chunk;
+            wk_fill(g, ws, lo, std::min(g.planes, lo + vs.wk_chunk), p);
+            launch(p, first, count);
+        }
+        CHIP_HIP(hipEventRecord(vs.ev_stop, st));
+        CHIP_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+        vs.timed = true;
+        return;
+    }
     CHIP_HIP(hipEventRecord(vs.ev_stop, st));
     vs.timed = true;
 }
@@ -561,7 +618,7 @@ static int register_slots(uint32_t halo) {
 template <int MODE, bool WP>
 static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
                             const float2 *model, float *psf, float *acc, float2 *vis_out,
-                            hipStream_t st) {
+                            hipStream_t st, bool timed = true) {
     const size_t lds = binned_lds_bytes<MODE, WP>(gp);
     if (lds > lds_limit(vs.device))
         fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
@@ -570,14 +627,14 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
     const int ks = register_slots(gp.halo);
     if constexpr (!WP) {
         if (ks != 1) fail(CHIP_ERR_INVALID, "standard gridder halo must be the 7x7 support");
-        launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st);
+        launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed);
     } else {
         switch (ks) {
-            case 1: launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
-            case 2: launch_binned_k<MODE, WP, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
-            case 4: launch_binned_k<MODE, WP, 4>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
-            case 7: launch_binned_k<MODE, WP, 7>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
-            default: launch_binned_k<MODE, WP, 0>(vs, gp, values, model, psf, acc, vis_out, lds, st); break;
+            case 1: launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+            case 2: launch_binned_k<MODE, WP, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+            case 4: launch_binned_k<MODE, WP, 4>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+            case 7: launch_binned_k<MODE, WP, 7>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+            default: launch_binned_k<MODE, WP, 0>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
         }
     }
 }
@@ -586,6 +643,7 @@ template <int MODE>
 static void launch_binned(chip_vset &vs, const float2 *values, const float2 *model, float *psf,
                           float *acc, float2 *vis_out) {
     const chip_gridder &g = *vs.g;
+    check_wk_chunk(vs);
     const GridParams gp = grid_params(g, vs.tile, vs.halo, vs.tiles_y);
     if (vs.n_items > 0x7fffffffull) fail(CHIP_ERR_INVALID, "too many work items");
     if (g.wproject)
@@ -616,11 +674,14 @@ void vset_grid(chip_vset &vs, uint32_t mode, const float *model, float *psf, flo
 
 // Residual pass of the major cycle over caller-laid-out grids (see GridParams
 // cstride / paired / model_signed).
-void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res) {
+// (on stream st; timed = false while the major cycle is captured into a graph)
+void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res, hipStream_t st,
+                        bool timed) {
+    check_wk_chunk(vs);
     if (vs.g->wproject)
-        launch_binned_t<kModeResidual, true>(vs, gp, nullptr, model, nullptr, res, nullptr, vs.g->ctx->stream);
+        launch_binned_t<kModeResidual, true>(vs, gp, nullptr, model, nullptr, res, nullptr, st, timed);
     else
-        launch_binned_t<kModeResidual, false>(vs, gp, nullptr, model, nullptr, res, nullptr, vs.g->ctx->stream);
+        launch_binned_t<kModeResidual, false>(vs, gp, nullptr, model, nullptr, res, nullptr, st, timed);
 }
 
 void vset_degrid(chip_vset &vs, const float *model, float *vis_out) {

@@ -20,6 +20,7 @@
 //            2, 3, 5 and fits the LDS);
 //   hipFFT : rocFFT 2-D plans over [chan][x][y] with separate placement and
 //            crop kernels.
+#include <algorithm>
 #include <cmath>
 
 #include "device.hpp"
@@ -27,7 +28,9 @@
 
 namespace chip {
 
-void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res);
+void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res, hipStream_t st,
+                        bool timed);
+void vset_split_items(chip_vset &vs, uint32_t gpi);
 
 namespace {
 
@@ -84,7 +87,82 @@ __global__ void k_cycle_crop(CycleGeom cg, const float2 *__restrict__ raw, const
     out[(static_cast<uint64_t>(c) * cg.nx + x) * cg.ny + y] = v;
 }
 
+// Zero fill by a kernel rather than hipMemsetAsync, so that the major cycle
+// can be captured into a graph (memset nodes are avoided there).
+__global__ void k_zero(float4 *p, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t k = i; k < n; k += stride) p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+void zero_fill(void *p, uint64_t bytes, hipStream_t st) {
+    if (bytes % sizeof(float4)) fail(CHIP_ERR_INVALID, "zero fill of a buffer that is not 16-byte sized");
+    const uint64_t n = bytes / sizeof(float4);
+    if (n == 0) return;
+    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, static_cast<float4 *>(p), n);
+    CHIP_HIP(hipGetLastError());
+}
+
+// Schedule autotuner of the binned residual kernel: every (threads per
+// workgroup, groups per work item) candidate is launched twice on the major
+// cycle's own grids (the second launch is timed with the set's events) and
+// the fastest is kept.  The grids' contents do not matter: the residual grid
+// is cleared again before the real pass.
+void autotune_schedule(chip_vset &vs, const GridParams &gp, const float2 *M, float *Rg, hipStream_t st) {
+    static constexpr uint32_t kThreads[] = {256, 512};
+    static constexpr uint32_t kGpi[] = {256, 512, 1024, 2048};
+    vs.tune_ms.clear();
+    double best = -1.0;
+    uint32_t best_t = vs.threads, best_g = vs.groups_per_item;
+    for (uint32_t gpi : kGpi) {
+        vset_split_items(vs, gpi);
+        for (uint32_t t : kThreads) {
+            vs.threads = t;
+            float ms = 0.0f;
+            for (int rep = 0; rep < 2; ++rep) {
+                vset_residual_with(vs, gp, M, Rg, st, true);
+                CHIP_HIP(hipEventSynchronize(vs.ev_stop));
+                CHIP_HIP(hipEventElapsedTime(&ms, vs.ev_start, vs.ev_stop));
+            }
+            vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(t), static_cast<double>(gpi), ms});
+            if (best < 0.0 || ms < best) {
+                best = ms;
+                best_t = t;
+                best_g = gpi;
+            }
+        }
+    }
+    vset_split_items(vs, best_g);
+    vs.threads = best_t;
+    vs.schedule_tuned = true;
+}
+
 }  // namespace
+
+// A captured major cycle (fused strategy): the two forward passes, the zero
+// fill of the residual grid, the binned residual kernel and the two inverse
+// passes, replayed with one hipGraphLaunch.  Captured on a stream of its own
+// (the caller's stream may be the legacy default stream, which cannot be
+// captured); replayed on the context's stream.  The model and residual
+// pointers are baked in: a call with other pointers captures again.
+struct CycleGraph {
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const float *model = nullptr;
+    float *out = nullptr;
+    void reset() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~CycleGraph() {
+        reset();
+        if (cap) (void)hipStreamDestroy(cap);
+    }
+};
 
 struct CycleWorkspace {
     int strategy = -1;  // CHIP_CYCLE_HIPFFT or CHIP_CYCLE_FUSED once set up
@@ -92,6 +170,8 @@ struct CycleWorkspace {
     hipfftHandle p2d = nullptr;
     xf::XPlan xp;
     double transform_ms = -1.0;
+    CycleGraph graph;
+    uint64_t replays = 0;  // major cycles run by graph replay
     ~CycleWorkspace() {
         if (p2d) hipfftDestroy(p2d);
     }
@@ -183,6 +263,8 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
     }
     CycleWorkspace &ws = *vs.cycle;
     if (ws.p2d) CHIP_FFT(hipfftSetStream(ws.p2d, st));  // the context's stream may have changed
+    if (vs.cycle_graph && ws.strategy != CHIP_CYCLE_FUSED)
+        fail(CHIP_ERR_INVALID, "graph replay of the major cycle needs the fused transforms");
 
     std::vector<float> host(2 * nc);
     for (uint64_t c = 0; c < nc; ++c) {
@@ -203,11 +285,39 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
         gp.cstride = ws.xp.xg.gstride;
         gp.bstride = g.gx * xf::kBlock;
         gp.lower = static_cast<uint64_t>(ws.xp.xg.nyb_h) * gp.bstride;
+        if (!vs.schedule_fixed && !vs.schedule_tuned && vs.n_items)
+            autotune_schedule(vs, gp, M, reinterpret_cast<float *>(Rg), st);
         // the half spectrum borrows the residual grid, the inverse rows the model grid
-        xf::forward_model(ws.xp, model, Rg, M, st);
-        CHIP_HIP(hipMemsetAsync(Rg, 0, ws.res_grid.bytes, st));
-        vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg));
-        xf::inverse_residual(ws.xp, Rg, M, d_nsum, d_peak, out, st);
+        auto enqueue = [&](hipStream_t s, bool timed) {
+            xf::forward_model(ws.xp, model, Rg, M, s);
+            zero_fill(Rg, ws.res_grid.bytes, s);
+            vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), s, timed);
+            xf::inverse_residual(ws.xp, Rg, M, d_nsum, d_peak, out, s);
+        };
+        if (!vs.cycle_graph) {
+            enqueue(st, true);
+        } else {
+            CycleGraph &cg = ws.graph;
+            if (!cg.exec || cg.model != model || cg.out != out) {
+                cg.reset();
+                if (!cg.cap) CHIP_HIP(hipStreamCreateWithFlags(&cg.cap, hipStreamNonBlocking));
+                CHIP_HIP(hipStreamBeginCapture(cg.cap, hipStreamCaptureModeRelaxed));
+                try {
+                    enqueue(cg.cap, false);
+                } catch (...) {
+                    hipGraph_t partial = nullptr;
+                    (void)hipStreamEndCapture(cg.cap, &partial);
+                    if (partial) (void)hipGraphDestroy(partial);
+                    throw;
+                }
+                CHIP_HIP(hipStreamEndCapture(cg.cap, &cg.graph));
+                CHIP_HIP(hipGraphInstantiate(&cg.exec, cg.graph, nullptr, nullptr, 0));
+                cg.model = model;
+                cg.out = out;
+            }
+            CHIP_HIP(hipGraphLaunch(cg.exec, st));
+            ws.replays += 1;
+        }
     } else {
         CycleGeom cg{};
         cg.nx = static_cast<uint32_t>(g.geom.nx);
@@ -231,13 +341,41 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
         CHIP_HIP(hipGetLastError());
         CHIP_FFT(hipfftExecC2C(ws.p2d, reinterpret_cast<hipfftComplex *>(M),
                                reinterpret_cast<hipfftComplex *>(M), HIPFFT_FORWARD));
-        vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg));
+        if (!vs.schedule_fixed && !vs.schedule_tuned && vs.n_items) {
+            autotune_schedule(vs, gp, M, reinterpret_cast<float *>(Rg), st);
+            CHIP_HIP(hipMemsetAsync(Rg, 0, ws.res_grid.bytes, st));
+        }
+        vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), st, true);
         CHIP_FFT(hipfftExecC2C(ws.p2d, reinterpret_cast<hipfftComplex *>(Rg),
                                reinterpret_cast<hipfftComplex *>(Rg), HIPFFT_BACKWARD));
         hipLaunchKernelGGL(k_cycle_crop, img_grid, dim3(256), 0, st, cg, Rg, d_nsum, d_peak, out);
         CHIP_HIP(hipGetLastError());
     }
     CHIP_HIP(hipStreamSynchronize(st));
+}
+
+void vset_cycle_graph(chip_vset &vs, int enable) {
+    if (enable && vs.wk_chunk)
+        fail(CHIP_ERR_INVALID, "W tables recomputed on the fly cannot be captured into a graph");
+    vs.cycle_graph = enable != 0;
+    if (vs.cycle) vs.cycle->graph.reset();
+}
+
+uint64_t vset_cycle_replays(const chip_vset &vs) { return vs.cycle ? vs.cycle->replays : 0; }
+
+void vset_schedule(chip_vset &vs, uint32_t threads, uint32_t gpi) {
+    if (threads == 0 && gpi == 0) {  // autotune again at the next major cycle
+        vs.schedule_fixed = false;
+        vs.schedule_tuned = false;
+    } else {
+        if (threads == 0 || threads > 512 || threads % 64)
+            fail(CHIP_ERR_INVALID, "threads per workgroup must be a multiple of 64 up to 512");
+        if (gpi == 0 || gpi > (1u << 20)) fail(CHIP_ERR_INVALID, "groups per work item must be in 1 .. 2^20");
+        vset_split_items(vs, gpi);
+        vs.threads = threads;
+        vs.schedule_fixed = true;
+    }
+    if (vs.cycle) vs.cycle->graph.reset();
 }
 
 void vset_cycle_strategy(chip_vset &vs, int strategy) {

@@ -289,6 +289,10 @@ struct GridParams {
     const float2 *kernels;   // planes x width x width
     const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest
     const int32_t *supports;
+    // Planes [plane_lo, plane_hi) are present in kernels / compact, which
+    // start at plane_lo (W tables recomputed per chunk of planes); 0 and
+    // UINT32_MAX when every table is resident.
+    uint32_t plane_lo, plane_hi;
     // AWProject: per (table, offset) Complex32 normalisation with the table's
     // w sign; degridded values are divided by its conjugate.  NULL otherwise.
     const float2 *aw_norm;
@@ -316,6 +320,18 @@ __device__ __forceinline__ uint64_t cell_offset(const GridParams &gp, uint32_t x
 }
 
 GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint32_t tiles_y);
+
+// W tables recomputed on the fly (g.wk_chunk != 0): scratch for one chunk of
+// planes, then per chunk wk_fill recomputes planes [lo, hi) into it and points
+// gp's kernels / compact / plane range at them.
+struct WkScratch {
+    DevBuf screen, kernels, compact;
+};
+void wk_prepare(const chip_gridder &g, WkScratch &ws);
+void wk_fill(chip_gridder &g, WkScratch &ws, uint32_t lo, uint32_t hi, GridParams &gp);
+// Kernels (planes x width x width complex64) of every plane into kernels_dev,
+// recomputed when they are not resident.
+void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev);
 
 // Wavefront-wide sum of a float (64 lanes, all active); every lane receives
 // the same value.  DPP lane swaps inside each row of 16 (xor 1, xor 2,

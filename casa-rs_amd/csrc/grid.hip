@@ -39,9 +39,14 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
         const SampleIn smp = load_sample(s, i);
         Plan pl;
         bool ok = sample_accepted(pp, smp, pl);
+        // W tables recomputed per chunk of planes: samples of planes outside
+        // [plane_lo, plane_hi) belong to another launch
+        const bool other_chunk = WP && ok && (static_cast<uint32_t>(pl.plane) < gp.plane_lo ||
+                                              static_cast<uint32_t>(pl.plane) >= gp.plane_hi);
+        ok = ok && !other_chunk;
         const uint32_t flags = ok ? sample_flags(pp, smp) : 0u;
         if (ok && MODE == kModeResidual && !(flags & kResidOkBit)) ok = false;
-        if (MODE == kModeDegrid && !ok) vis_out[2 * i] = vis_out[2 * i + 1] = T(0);
+        if (MODE == kModeDegrid && !ok && !other_chunk) vis_out[2 * i] = vis_out[2 * i + 1] = T(0);
         if (ok) {
             const float gwf = grid_weight(pp, smp);
             if (pp.aw) {
@@ -165,7 +170,7 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
                 const int s_ = static_cast<int>(gp.sampling);
                 const int W = static_cast<int>(gp.width);
                 const int S = gp.supports[pl.plane];
-                const float2 *K = gp.kernels + static_cast<uint64_t>(pl.plane) * W * W;
+                const float2 *K = gp.kernels + static_cast<uint64_t>(pl.plane - static_cast<int>(gp.plane_lo)) * W * W;
                 const T csign = pl.conj ? T(-1) : T(1);
                 T pr = 0, pi = 0;
                 if ((MODE == kModeResidual || MODE == kModeDegrid) && model) {
@@ -236,22 +241,36 @@ __global__ __launch_bounds__(256) void k_plain(PlanParams pp, Source s, uint32_t
 template <typename T, int MODE>
 static void launch_plain_t(chip_gridder &g, const Source &src, const void *model, void *psf,
                            void *acc, void *vis_out, double *sums) {
-    const GridParams gp = grid_params(g, 32, 0, 1);
+    GridParams gp = grid_params(g, 32, 0, 1);
     const PlanParams pp = plan_params(g, src);
     hipStream_t st = g.ctx->stream;
     const unsigned blocks = static_cast<unsigned>((src.n_rows + 255) / 256);
     if (blocks == 0) return;
-    for (uint32_t c = 0; c < src.n_chan; ++c) {
-        if (g.wproject)
-            hipLaunchKernelGGL((k_plain<T, MODE, true>), dim3(blocks), dim3(256), 0, st, pp, src, c,
-                               gp, static_cast<const T *>(model), static_cast<T *>(psf),
-                               static_cast<T *>(acc), static_cast<T *>(vis_out), sums);
-        else
-            hipLaunchKernelGGL((k_plain<T, MODE, false>), dim3(blocks), dim3(256), 0, st, pp, src,
-                               c, gp, static_cast<const T *>(model), static_cast<T *>(psf),
-                               static_cast<T *>(acc), static_cast<T *>(vis_out), sums);
-        CHIP_HIP(hipGetLastError());
+    auto run = [&](const GridParams &p) {
+        for (uint32_t c = 0; c < src.n_chan; ++c) {
+            if (g.wproject)
+                hipLaunchKernelGGL((k_plain<T, MODE, true>), dim3(blocks), dim3(256), 0, st, pp, src, c,
+                                   p, static_cast<const T *>(model), static_cast<T *>(psf),
+                                   static_cast<T *>(acc), static_cast<T *>(vis_out), sums);
+            else
+                hipLaunchKernelGGL((k_plain<T, MODE, false>), dim3(blocks), dim3(256), 0, st, pp, src,
+                                   c, p, static_cast<const T *>(model), static_cast<T *>(psf),
+                                   static_cast<T *>(acc), static_cast<T *>(vis_out), sums);
+            CHIP_HIP(hipGetLastError());
+        }
+    };
+    if (!g.wproject || g.aw || !g.wk_chunk) {
+        run(gp);
+        return;
     }
+    // W tables recomputed on the fly: one pass per chunk of planes
+    WkScratch ws;
+    wk_prepare(g, ws);
+    for (uint32_t lo = 0; lo < g.planes; lo += g.wk_chunk) {
+        wk_fill(g, ws, lo, std::min(g.planes, lo + g.wk_chunk), gp);
+        run(gp);
+    }
+    CHIP_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
 }
 
 template <typename T>

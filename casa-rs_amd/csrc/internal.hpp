@@ -82,6 +82,22 @@ struct FftCache {
 
 }  // namespace chip
 
+namespace chip {
+// Everything the W tables of a WProjector are computed from (wkernel.hip).
+struct WkState {
+    uint32_t conv = 0, inner = 0, quarter = 0, planes = 0, sampling = 1;
+    int32_t R = 0;
+    double s0 = 0.0, s1 = 0.0, w_scale = 1.0;
+    uint32_t width = 0, stride = 0;  // common quarter-plane edge, compact taps per (plane, offset)
+    float peak0 = 0.0f, pb = 0.0f;   // plane-zero FFT peak and integral at integer taps
+    std::vector<uint32_t> crop;      // per plane: cropped quarter-plane edge
+    DevBuf d_corr;                   // screen correction axis (inner floats)
+    // attach arguments, to rebuild when the residency budget changes
+    double max_abs_w = 0.0;
+    uint32_t explicit_planes = 0, sampling_arg = 0;
+};
+}  // namespace chip
+
 struct chip_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;  // created with the context
@@ -114,6 +130,15 @@ struct chip_gridder {
     // lanes of a wavefront read one sample's taps with coalesced loads.
     chip::DevBuf d_compact;         // planes x (2R+1)^2 x tap_stride float2
     uint32_t tap_stride = 0;
+    // Residency of the W tables (d_kernels, d_compact).  wk_budget = 0 keeps
+    // them resident; otherwise, when they exceed wk_budget bytes, they are not
+    // stored: every call that grids or degrids recomputes them in chunks of
+    // wk_chunk planes (screen, FFT, crop, normalisation, gather) and runs the
+    // chunk's samples before the next chunk overwrites them.  wk_chunk = 0:
+    // resident.  wk holds what the recomputation needs.
+    uint64_t wk_budget = 0;
+    uint32_t wk_chunk = 0;
+    chip::WkState wk;
     std::vector<float> sinc;        // build_sinc_axis(max(gx, gy), sampling)
     std::vector<float> sinc_x, sinc_y;  // per-axis sinc used by model apodisation
     chip::DevBuf d_sinc, d_sinc_x, d_sinc_y;
@@ -259,13 +284,28 @@ struct chip_vset {
     uint32_t halo = 3;        // max half-support
     uint32_t tiles_x = 0, tiles_y = 0;
     uint64_t n_items = 0;
+    // W tables recomputed on the fly: the stream is binned by (plane chunk,
+    // channel, tile); chunk_items[c] is the first work item of chunk c (plus
+    // n_items at the end).  wk_chunk = 0: one chunk.
+    uint32_t wk_chunk = 0;
+    std::vector<uint32_t> chunk_items;
+    // bins (plane chunk, channel, tile): groups per bin and first group, kept
+    // so that the schedule autotuner can split the work items again
+    chip::DevBuf bin_groups, bin_first;
+    uint32_t n_bins = 0, n_tiles_all = 0;
+    // launch schedule of the binned kernel: threads per workgroup and groups
+    // per work item (chosen by the autotuner at the first major cycle unless
+    // fixed by chip_vset_schedule)
+    uint32_t threads = 512, groups_per_item = 1024;
+    bool schedule_fixed = false, schedule_tuned = false;
+    std::vector<double> tune_ms;  // timed candidates: (threads, groups per item, ms) triples
     // stream (sorted by channel, tile)
     chip::DevBuf plan;    // uint2 per sample
     chip::DevBuf vis;     // float2
     chip::DevBuf wt;      // float
     chip::DevBuf index;   // uint32 original sample index
     chip::DevBuf items;   // WorkItem (chan, tile, first group, group count)
-    chip::DevBuf gstart;  // uint32 first stream position of each group, plus n_valid
+    chip::DevBuf grec;    // uint4 per group: first stream position, end, packed plan (x, y)
     uint64_t n_groups = 0;
     std::vector<chip_sums> sums;
     uint64_t taps = 0;    // sum over accepted samples of (2S+1)^2
@@ -274,6 +314,7 @@ struct chip_vset {
     bool timed = false;
     std::shared_ptr<chip::CycleWorkspace> cycle;
     int cycle_strategy = CHIP_CYCLE_AUTO;
+    bool cycle_graph = false;  // major cycles replay a captured HIP graph
     chip::DevBuf cycle_norm;  // per channel: sumwt (f32), then psf peak
     ~chip_vset() {
         if (ev_start) (void)hipEventDestroy(ev_start);

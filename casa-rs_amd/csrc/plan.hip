@@ -12,8 +12,9 @@
 // adjacent identical plans, lib.rs:71646-71660; here every identical plan of
 // a bin is grouped, and the group's samples are summed exactly as the
 // per-sample sum would be: sum (V_i - p) w_i = sum w_i V_i - p sum w_i).
-// Groups of one bin are split into work items of at most kGroupsPerItem so
-// that one launch covers every channel with balanced workgroups.
+// Groups of one bin are split into work items of at most groups_per_item
+// (1024 unless the schedule autotuner picks another) so that one launch
+// covers every channel with balanced workgroups.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -22,7 +23,6 @@
 
 namespace chip {
 
-constexpr uint32_t kGroupsPerItem = 1024;  // groups of identical plans per work item (upper bound)
 
 PlanParams plan_params(const chip_gridder &g) {
     PlanParams p{};
@@ -148,7 +148,8 @@ double launch_max_abs_w(const chip_gridder &g, const Source &src, hipStream_t st
 
 // ---- binned stream ---------------------------------------------------------
 // Sort key (64 bits, most significant first):
-//   bin   = channel * n_tiles + uv tile of the anchor cell
+//   bin   = channel * n_tiles + uv tile of the anchor cell (with W tables
+//           recomputed per chunk of planes, the plane chunk comes first)
 //   cell  = anchor cell inside the tile (10 bits for 32x32 tiles)
 //   plan  = the rest of the plan: tap-row selectors (standard) or
 //           plane, conjugation and offsets (W)
@@ -159,7 +160,10 @@ struct BinParams {
     uint32_t tile;
     uint32_t tiles_x, tiles_y;
     uint32_t n_tiles;
-    uint32_t n_bins;        // n_chan * n_tiles (also the invalid bin)
+    uint32_t n_bins;        // chunks * n_chan * n_tiles (also the invalid bin)
+    uint32_t n_chan;
+    uint32_t chunk_planes;  // W tables recomputed per chunk of planes: bins
+                            // are (plane / chunk_planes, channel, tile); 0: one chunk
     uint32_t cell_bits;     // log2(tile * tile)
     uint32_t plan_bits;
     uint32_t n_sel;         // selector values per axis
@@ -189,7 +193,9 @@ __global__ __launch_bounds__(256) void k_keys(PlanParams p, Source s, BinParams 
             const uint32_t ax = static_cast<uint32_t>(p.wproject ? pl.x : pl.x + kSupport);
             const uint32_t ay = static_cast<uint32_t>(p.wproject ? pl.y : pl.y + kSupport);
             const uint32_t tile = (ax / b.tile) * b.tiles_y + (ay / b.tile);
-            const uint64_t bin = static_cast<uint64_t>(smp.chan) * b.n_tiles + tile;
+            uint64_t bin = static_cast<uint64_t>(smp.chan) * b.n_tiles + tile;
+            if (b.chunk_planes)
+                bin += static_cast<uint64_t>(static_cast<uint32_t>(pl.plane) / b.chunk_planes) * b.n_chan * b.n_tiles;
             const uint64_t cell = static_cast<uint64_t>((ax % b.tile) * b.tile + (ay % b.tile));
             key = (bin << (b.cell_bits + b.plan_bits)) | (cell << b.plan_bits);
             if (b.plan_bits) key |= plan_code(p, b, pl);
@@ -215,6 +221,16 @@ __global__ void k_group_starts(const uint64_t *keys, const uint32_t *flags, cons
     if (p >= n || !flags[p]) return;
     gstart[gid[p]] = static_cast<uint32_t>(p);
     atomicAdd(&bin_groups[keys[p] >> low_bits], 1u);
+}
+
+// One record per group: (first sample, end, packed plan), so that the binned
+// kernel reads a group's bounds and plan with one load.
+__global__ void k_group_records(const uint32_t *gstart, const uint2 *plan, uint32_t n_groups, uint4 *rec) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const uint32_t s0 = gstart[g], s1 = gstart[g + 1];
+    const uint2 pk = plan[s0];
+    rec[g] = make_uint4(s0, s1, pk.x, pk.y);
 }
 
 // Writes the packed stream in sorted order and accumulates per-channel sums
@@ -281,23 +297,24 @@ __global__ __launch_bounds__(256) void k_build_stream(PlanParams p, Source s, ui
     }
 }
 
-// Work items per bin: runs of at most kGroupsPerItem groups of one bin.
-__global__ void k_items_per_bin(const uint32_t *bin_groups, uint32_t n_bins, uint32_t *items_per_bin) {
+// Work items per bin: runs of at most gpi groups of one bin.
+__global__ void k_items_per_bin(const uint32_t *bin_groups, uint32_t n_bins, uint32_t gpi, uint32_t *items_per_bin) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bins) return;
-    items_per_bin[k] = (bin_groups[k] + kGroupsPerItem - 1) / kGroupsPerItem;
+    items_per_bin[k] = (bin_groups[k] + gpi - 1) / gpi;
 }
 
 __global__ void k_write_items(const uint32_t *bin_groups, const uint32_t *bin_first_group,
                               const uint32_t *item_start, uint32_t n_bins, uint32_t n_tiles,
-                              WorkItem *items) {
+                              uint32_t n_chan, uint32_t gpi, WorkItem *items) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bins) return;
     const uint32_t count = bin_groups[k];
     if (count == 0) return;
-    const uint32_t chan = k / n_tiles;
-    const uint32_t tile = k - chan * n_tiles;
-    const uint32_t nit = (count + kGroupsPerItem - 1) / kGroupsPerItem;
+    const uint32_t ct = k / n_tiles;  // chunk * n_chan + channel
+    const uint32_t chan = ct % n_chan;
+    const uint32_t tile = k - ct * n_tiles;
+    const uint32_t nit = (count + gpi - 1) / gpi;
     const uint32_t base = count / nit, extra = count % nit;  // spread evenly
     uint32_t pos = bin_first_group[k];
     for (uint32_t j = 0; j < nit; ++j) {
@@ -328,6 +345,37 @@ static uint32_t read_u32(const uint32_t *dev, hipStream_t st) {
     return v;
 }
 
+// (Re)splits every bin's groups into work items of at most gpi groups and
+// records the first item of each plane chunk.
+void vset_split_items(chip_vset &vs, uint32_t gpi) {
+    if (gpi == 0) fail(CHIP_ERR_INVALID, "groups per work item must be positive");
+    hipStream_t st = vs.g->ctx->stream;
+    vs.groups_per_item = gpi;
+    if (vs.n_valid == 0 || vs.n_bins == 0) return;
+    const uint32_t n_bins = vs.n_bins;
+    DevBuf ipb((static_cast<uint64_t>(n_bins) + 1) * 4), item_start((static_cast<uint64_t>(n_bins) + 1) * 4);
+    CHIP_HIP(hipMemsetAsync(ipb.p, 0, ipb.bytes, st));
+    hipLaunchKernelGGL(k_items_per_bin, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
+                       vs.bin_groups.as<uint32_t>(), n_bins, gpi, ipb.as<uint32_t>());
+    CHIP_HIP(hipGetLastError());
+    exclusive_sum(ipb.as<uint32_t>(), item_start.as<uint32_t>(), n_bins + 1, st);
+    std::vector<uint32_t> starts(static_cast<uint64_t>(n_bins) + 1);
+    CHIP_HIP(hipMemcpyAsync(starts.data(), item_start.p, starts.size() * 4, hipMemcpyDeviceToHost, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+    const uint32_t n_items = starts[n_bins];
+    vs.n_items = n_items;
+    vs.items.alloc(std::max<uint64_t>(n_items, 1) * sizeof(WorkItem));
+    hipLaunchKernelGGL(k_write_items, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
+                       vs.bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(),
+                       item_start.as<uint32_t>(), n_bins, vs.n_tiles_all,
+                       static_cast<uint32_t>(vs.n_chan), gpi, vs.items.as<WorkItem>());
+    CHIP_HIP(hipGetLastError());
+    // first work item of each plane chunk
+    const uint64_t per_chunk = static_cast<uint64_t>(vs.n_tiles_all) * vs.n_chan;
+    for (uint64_t c = 0; c < vs.chunk_items.size(); ++c) vs.chunk_items[c] = starts[c * per_chunk];
+    CHIP_HIP(hipStreamSynchronize(st));
+}
+
 // Builds the resident stream for a vset: FP64 plans, 64-bit radix sort by
 // (bin, anchor cell, plan), groups of identical plans, work items.
 void vset_build(chip_vset &vs, const chip_samples &cs) {
@@ -348,8 +396,11 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     vs.tiles_x = static_cast<uint32_t>((g.gx + vs.tile - 1) / vs.tile);
     vs.tiles_y = static_cast<uint32_t>((g.gy + vs.tile - 1) / vs.tile);
     const uint64_t n_tiles = static_cast<uint64_t>(vs.tiles_x) * vs.tiles_y;
-    const uint64_t n_bins64 = n_tiles * src.n_chan;
-    if (n_bins64 >= (1ull << 31)) fail(CHIP_ERR_INVALID, "too many (channel, tile) bins");
+    vs.wk_chunk = (g.wproject && !g.aw) ? g.wk_chunk : 0u;
+    const uint64_t n_chunks = vs.wk_chunk ? (g.planes + vs.wk_chunk - 1) / vs.wk_chunk : 1;
+    const uint64_t n_bins64 = n_tiles * src.n_chan * n_chunks;
+    if (n_bins64 >= (1ull << 31)) fail(CHIP_ERR_INVALID, "too many (plane chunk, channel, tile) bins");
+    vs.chunk_items.assign(n_chunks + 1, 0u);
     const uint32_t n_bins = static_cast<uint32_t>(n_bins64);
     vs.sums.assign(src.n_chan, chip_sums{});
     CHIP_HIP(hipEventCreate(&vs.ev_start));
@@ -362,6 +413,8 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     b.tiles_y = vs.tiles_y;
     b.n_tiles = static_cast<uint32_t>(n_tiles);
     b.n_bins = n_bins;
+    b.n_chan = static_cast<uint32_t>(src.n_chan);
+    b.chunk_planes = vs.wk_chunk;
     b.cell_bits = static_cast<uint32_t>(bits_for(static_cast<uint64_t>(vs.tile) * vs.tile - 1));
     b.n_sel = g.wproject ? static_cast<uint32_t>(2 * g.norm_radius + 1) : g.os + 1;
     const uint64_t max_code = g.wproject
@@ -441,38 +494,35 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     exclusive_sum(flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid + 1, st);
     const uint32_t n_groups = read_u32(gid.as<uint32_t>() + n_valid, st);
     vs.n_groups = n_groups;
-    vs.gstart.alloc((static_cast<uint64_t>(n_groups) + 1) * 4);
+    DevBuf gstart((static_cast<uint64_t>(n_groups) + 1) * 4);
     DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
     CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
     const uint32_t low_bits = b.cell_bits + b.plan_bits;
     hipLaunchKernelGGL(k_group_starts, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
                        keys_s.as<uint64_t>(), flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid,
-                       low_bits, vs.gstart.as<uint32_t>(), bin_groups.as<uint32_t>());
+                       low_bits, gstart.as<uint32_t>(), bin_groups.as<uint32_t>());
     CHIP_HIP(hipGetLastError());
     const uint32_t nv32 = static_cast<uint32_t>(n_valid);
-    CHIP_HIP(hipMemcpyAsync(vs.gstart.as<uint32_t>() + n_groups, &nv32, 4, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipMemcpyAsync(gstart.as<uint32_t>() + n_groups, &nv32, 4, hipMemcpyHostToDevice, st));
+    vs.grec.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(uint4));
+    if (n_groups) {
+        hipLaunchKernelGGL(k_group_records, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st,
+                           gstart.as<uint32_t>(), vs.plan.as<uint2>(), n_groups, vs.grec.as<uint4>());
+        CHIP_HIP(hipGetLastError());
+    }
+    CHIP_HIP(hipStreamSynchronize(st));
+    gstart.release();
     keys_s.release();
     flags.release();
     gid.release();
 
     // work items
-    DevBuf first_group((static_cast<uint64_t>(n_bins) + 1) * 4);
-    DevBuf ipb((static_cast<uint64_t>(n_bins) + 1) * 4), item_start((static_cast<uint64_t>(n_bins) + 1) * 4);
-    exclusive_sum(bin_groups.as<uint32_t>(), first_group.as<uint32_t>(), n_bins + 1, st);
-    CHIP_HIP(hipMemsetAsync(ipb.p, 0, ipb.bytes, st));
-    hipLaunchKernelGGL(k_items_per_bin, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
-                       bin_groups.as<uint32_t>(), n_bins, ipb.as<uint32_t>());
-    CHIP_HIP(hipGetLastError());
-    exclusive_sum(ipb.as<uint32_t>(), item_start.as<uint32_t>(), n_bins + 1, st);
-    const uint32_t n_items = read_u32(item_start.as<uint32_t>() + n_bins, st);
-    vs.n_items = n_items;
-    vs.items.alloc(std::max<uint64_t>(n_items, 1) * sizeof(WorkItem));
-    hipLaunchKernelGGL(k_write_items, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
-                       bin_groups.as<uint32_t>(), first_group.as<uint32_t>(),
-                       item_start.as<uint32_t>(), n_bins, static_cast<uint32_t>(n_tiles),
-                       vs.items.as<WorkItem>());
-    CHIP_HIP(hipGetLastError());
-    CHIP_HIP(hipStreamSynchronize(st));
+    vs.bin_first.alloc((static_cast<uint64_t>(n_bins) + 1) * 4);
+    exclusive_sum(bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(), n_bins + 1, st);
+    vs.bin_groups = std::move(bin_groups);
+    vs.n_bins = n_bins;
+    vs.n_tiles_all = static_cast<uint32_t>(n_tiles);
+    vset_split_items(vs, vs.groups_per_item);
 }
 
 }  // namespace chip

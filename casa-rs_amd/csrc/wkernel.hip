@@ -74,11 +74,10 @@ __global__ void k_peak(const float2 *t, float *peak) {
 // Crop plane b of the batch to crop x crop, divided by the plane-zero peak,
 // into the staging area (planes x stage_w x stage_w, zero padded).
 __global__ void k_crop(const float2 *t, uint32_t conv, uint32_t crop, uint32_t stage_w,
-                       const float *peak, float2 *stage) {
+                       float p, float2 *stage) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= crop * crop) return;
     const uint32_t x = i / crop, y = i - x * crop;
-    const float p = *peak;
     const float2 v = t[static_cast<uint64_t>(x) * conv + y];
     stage[static_cast<uint64_t>(x) * stage_w + y] = make_float2(v.x / p, v.y / p);
 }
@@ -147,6 +146,62 @@ __global__ void k_compact(const float2 *kernels, const int32_t *supports, uint32
 
 static unsigned nb(uint64_t n, unsigned t) { return static_cast<unsigned>((n + t - 1) / t); }
 
+namespace {
+
+// Screen of plane k and its forward FFT into `screen` (conv x conv).  One
+// plane per transform, so that a plane's values do not depend on which
+// planes are transformed with it.
+void screen_fft(chip_ctx &ctx, const WkState &wk, uint32_t k, float2 *screen, hipStream_t st) {
+    const uint64_t cells = static_cast<uint64_t>(wk.conv) * wk.conv;
+    CHIP_HIP(hipMemsetAsync(screen, 0, cells * sizeof(float2), st));
+    const ScreenParams sp{wk.conv, wk.inner, wk.s0, wk.s1, wk.w_scale, wk.planes, k};
+    const uint64_t work = static_cast<uint64_t>(wk.inner) * wk.inner;
+    hipLaunchKernelGGL(k_screen, dim3(nb(work, 256)), dim3(256), 0, st, sp, wk.d_corr.as<float>(), screen, 1u);
+    CHIP_HIP(hipGetLastError());
+    hipfftHandle h = ctx.fft.get(wk.conv, wk.conv, 1, HIPFFT_C2C, st);
+    CHIP_FFT(hipfftExecC2C(h, reinterpret_cast<hipfftComplex *>(screen), reinterpret_cast<hipfftComplex *>(screen),
+                           HIPFFT_FORWARD));
+}
+
+// Tables of planes [lo, hi): kernels (hi - lo) x width^2 (the cropped quarter
+// planes / plane-zero peak / plane-zero integral, zero padded), norm (may be
+// NULL) (hi - lo) x (2R+1)^2, compact (may be NULL) (hi - lo) x (2R+1)^2 x
+// stride.  d_sup: supports of every plane.  screen: one conv^2 plane.
+void wk_tables(chip_ctx &ctx, const WkState &wk, const int32_t *d_sup, uint32_t lo, uint32_t hi, float2 *screen,
+               float2 *kernels, float *norm, float2 *compact, hipStream_t st) {
+    const uint32_t n = hi - lo;
+    const uint64_t kcells = static_cast<uint64_t>(wk.width) * wk.width;
+    CHIP_HIP(hipMemsetAsync(kernels, 0, kcells * n * sizeof(float2), st));
+    for (uint32_t k = lo; k < hi; ++k) {
+        screen_fft(ctx, wk, k, screen, st);
+        const uint32_t c = wk.crop[k];
+        hipLaunchKernelGGL(k_crop, dim3(nb(static_cast<uint64_t>(c) * c, 256)), dim3(256), 0, st, screen, wk.conv,
+                           c, wk.width, wk.peak0, kernels + (k - lo) * kcells);
+        CHIP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_scale, dim3(nb(kcells * n, 256)), dim3(256), 0, st, kernels, kcells * n, wk.pb);
+    CHIP_HIP(hipGetLastError());
+    const uint32_t axis = static_cast<uint32_t>(2 * wk.R + 1);
+    if (norm) {
+        hipLaunchKernelGGL(k_norm_table, dim3(nb(static_cast<uint64_t>(n) * axis * axis, 128)), dim3(128), 0, st,
+                           kernels, d_sup + lo, n, wk.width, wk.sampling, wk.R, norm);
+        CHIP_HIP(hipGetLastError());
+    }
+    if (compact) {
+        const uint64_t n_compact = static_cast<uint64_t>(n) * axis * axis * wk.stride;
+        hipLaunchKernelGGL(k_compact, dim3(nb(n_compact, 256)), dim3(256), 0, st, kernels, d_sup + lo, n, wk.width,
+                           wk.sampling, wk.R, wk.stride, compact);
+        CHIP_HIP(hipGetLastError());
+    }
+}
+
+uint64_t wk_plane_bytes(const WkState &wk) {
+    const uint64_t axis = static_cast<uint64_t>(2 * wk.R + 1);
+    return (static_cast<uint64_t>(wk.width) * wk.width + axis * axis * wk.stride) * sizeof(float2);
+}
+
+}  // namespace
+
 void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
                     uint32_t sampling) {
     hipStream_t st = g.ctx->stream;
@@ -156,136 +211,126 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
     if (planes > 2047) fail(CHIP_ERR_INVALID, "at most 2047 W planes");
     const uint32_t s = sampling ? sampling : (planes > 1 ? 4u : 1u);  // gridder.rs:3413
     if (s > 127) fail(CHIP_ERR_INVALID, "W sampling must be at most 127");
-    const uint32_t conv = static_cast<uint32_t>(std::max(g.gx, g.gy));
-    const uint32_t inner = std::max<uint32_t>(conv / s, 2u);
-    const uint32_t quarter = conv / 2 - 1;
-    if (quarter < 2) fail(CHIP_ERR_INVALID, "grid too small for W projection");
-    const std::vector<float> corr = gridder_correction_axis(inner);
-    const double s0 = std::fabs(geo.cell_x) * s * static_cast<double>(g.gx) / conv;
-    const double s1 = std::fabs(geo.cell_y) * s * static_cast<double>(g.gy) / conv;
+    WkState wk;
+    wk.max_abs_w = max_abs_w;
+    wk.explicit_planes = explicit_planes;
+    wk.sampling_arg = sampling;
+    wk.planes = planes;
+    wk.sampling = s;
+    wk.conv = static_cast<uint32_t>(std::max(g.gx, g.gy));
+    wk.inner = std::max<uint32_t>(wk.conv / s, 2u);
+    wk.quarter = wk.conv / 2 - 1;
+    if (wk.quarter < 2) fail(CHIP_ERR_INVALID, "grid too small for W projection");
+    const std::vector<float> corr = gridder_correction_axis(wk.inner);
+    wk.s0 = std::fabs(geo.cell_x) * s * static_cast<double>(g.gx) / wk.conv;
+    wk.s1 = std::fabs(geo.cell_y) * s * static_cast<double>(g.gy) / wk.conv;
     const double maxw = w_max_uvw(geo, max_abs_w, explicit_planes != 0);
-    double w_scale = 1.0;
+    wk.w_scale = 1.0;
     if (planes > 1 && std::isfinite(maxw) && maxw > 0.0)
-        w_scale = static_cast<double>((planes - 1) * (planes - 1)) / maxw;
-    const int32_t R = static_cast<int32_t>((s + 1) / 2);
+        wk.w_scale = static_cast<double>((planes - 1) * (planes - 1)) / maxw;
+    wk.R = static_cast<int32_t>((s + 1) / 2);
+    const int32_t R = wk.R;
+    const uint32_t conv = wk.conv, quarter = wk.quarter;
+    wk.d_corr.alloc(wk.inner * sizeof(float));
+    CHIP_HIP(hipMemcpyAsync(wk.d_corr.p, corr.data(), wk.d_corr.bytes, hipMemcpyHostToDevice, st));
 
-    // Planes per FFT batch: keep the screens under ~4 GiB.
+    // Pass 1, plane by plane: plane-zero peak, each plane's support from the
+    // 1e-3 threshold on the two axes (gridder.rs:3832-3855), its crop, and
+    // the plane-zero integral at integer taps (pb_sum, gridder.rs:3991-4002,
+    // summed on the host in the reference order).
     const uint64_t plane_bytes = static_cast<uint64_t>(conv) * conv * sizeof(float2);
-    uint32_t per_batch = static_cast<uint32_t>(std::max<uint64_t>(1, (4ull << 30) / plane_bytes));
-    per_batch = std::min(per_batch, planes);
-    DevBuf screens(plane_bytes * per_batch);
-    DevBuf d_corr(inner * sizeof(float));
-    CHIP_HIP(hipMemcpyAsync(d_corr.p, corr.data(), d_corr.bytes, hipMemcpyHostToDevice, st));
+    DevBuf screen(plane_bytes);
     DevBuf d_peak(sizeof(float));
-    DevBuf d_mag(static_cast<uint64_t>(quarter) * per_batch * sizeof(float));
-
-    // Staging: each plane cropped to its own width, at most quarter.
-    const uint32_t stage_w_max = quarter;
-    std::vector<uint32_t> crop(planes), support(planes);
-    std::vector<DevBuf> stage(planes);
-    std::vector<float> mag(static_cast<uint64_t>(quarter) * per_batch);
-    ScreenParams sp{conv, inner, s0, s1, w_scale, planes, 0};
-    for (uint32_t first = 0; first < planes; first += per_batch) {
-        const uint32_t nbat = std::min(per_batch, planes - first);
-        CHIP_HIP(hipMemsetAsync(screens.p, 0, plane_bytes * nbat, st));
-        sp.first = first;
-        const uint64_t work = static_cast<uint64_t>(inner) * inner * nbat;
-        hipLaunchKernelGGL(k_screen, dim3(nb(work, 256)), dim3(256), 0, st, sp, d_corr.as<float>(),
-                           screens.as<float2>(), nbat);
-        CHIP_HIP(hipGetLastError());
-        hipfftHandle h = g.ctx->fft.get(conv, conv, nbat, HIPFFT_C2C, st);
-        CHIP_FFT(hipfftExecC2C(h, screens.as<hipfftComplex>(), screens.as<hipfftComplex>(),
-                               HIPFFT_FORWARD));
-        if (first == 0) {
-            hipLaunchKernelGGL(k_peak, dim3(1), dim3(64), 0, st, screens.as<float2>(), d_peak.as<float>());
+    DevBuf d_mag(static_cast<uint64_t>(quarter) * sizeof(float));
+    std::vector<uint32_t> support(planes);
+    wk.crop.assign(planes, 0);
+    std::vector<float> mag(quarter);
+    for (uint32_t k = 0; k < planes; ++k) {
+        screen_fft(*g.ctx, wk, k, screen.as<float2>(), st);
+        if (k == 0) {
+            hipLaunchKernelGGL(k_peak, dim3(1), dim3(64), 0, st, screen.as<float2>(), d_peak.as<float>());
             CHIP_HIP(hipGetLastError());
-            float peak = 0.0f;
-            CHIP_HIP(hipMemcpyAsync(&peak, d_peak.p, sizeof(float), hipMemcpyDeviceToHost, st));
+            CHIP_HIP(hipMemcpyAsync(&wk.peak0, d_peak.p, sizeof(float), hipMemcpyDeviceToHost, st));
             CHIP_HIP(hipStreamSynchronize(st));
-            if (!(peak > 0.0f)) fail(CHIP_ERR_NORMALIZATION, "wproject plane-zero kernel peak is zero");
+            if (!(wk.peak0 > 0.0f)) fail(CHIP_ERR_NORMALIZATION, "wproject plane-zero kernel peak is zero");
         }
-        hipLaunchKernelGGL(k_axis_mag, dim3(nb(static_cast<uint64_t>(quarter) * nbat, 256)), dim3(256), 0,
-                           st, screens.as<float2>(), conv, quarter, nbat, d_peak.as<float>(),
-                           d_mag.as<float>());
+        hipLaunchKernelGGL(k_axis_mag, dim3(nb(quarter, 256)), dim3(256), 0, st, screen.as<float2>(), conv, quarter,
+                           1u, d_peak.as<float>(), d_mag.as<float>());
         CHIP_HIP(hipGetLastError());
-        CHIP_HIP(hipMemcpyAsync(mag.data(), d_mag.p, static_cast<uint64_t>(quarter) * nbat * sizeof(float),
+        CHIP_HIP(hipMemcpyAsync(mag.data(), d_mag.p, static_cast<uint64_t>(quarter) * sizeof(float),
                                 hipMemcpyDeviceToHost, st));
         CHIP_HIP(hipStreamSynchronize(st));
-        for (uint32_t b = 0; b < nbat; ++b) {
-            const uint32_t k = first + b;
-            std::vector<float> m(mag.begin() + static_cast<uint64_t>(b) * quarter,
-                                 mag.begin() + static_cast<uint64_t>(b + 1) * quarter);
-            support[k] = find_w_support(m, conv, s);
-            uint64_t cw = static_cast<uint64_t>(support[k]) * s + R + 1;
-            cw = std::max<uint64_t>(std::min<uint64_t>(cw, quarter), 1);
-            crop[k] = static_cast<uint32_t>(cw);
-            // Taps read |i*s + off| <= support*s + R; the crop must hold them.
-            if (static_cast<uint64_t>(support[k]) * s + R + 1 > cw)
-                fail(CHIP_ERR_INVALID, "W kernel support exceeds the stored quarter plane");
-            stage[k].alloc(cw * cw * sizeof(float2));
-            hipLaunchKernelGGL(k_crop, dim3(nb(cw * cw, 256)), dim3(256), 0, st,
-                               screens.as<float2>() + static_cast<uint64_t>(b) * conv * conv, conv,
-                               crop[k], crop[k], d_peak.as<float>(), stage[k].as<float2>());
+        support[k] = find_w_support(mag, conv, s);
+        uint64_t cw = static_cast<uint64_t>(support[k]) * s + R + 1;
+        cw = std::max<uint64_t>(std::min<uint64_t>(cw, quarter), 1);
+        wk.crop[k] = static_cast<uint32_t>(cw);
+        // Taps read |i*s + off| <= support*s + R; the crop must hold them.
+        if (static_cast<uint64_t>(support[k]) * s + R + 1 > cw)
+            fail(CHIP_ERR_INVALID, "W kernel support exceeds the stored quarter plane");
+        if (k == 0) {
+            DevBuf c0(cw * cw * sizeof(float2));
+            hipLaunchKernelGGL(k_crop, dim3(nb(cw * cw, 256)), dim3(256), 0, st, screen.as<float2>(), conv,
+                               static_cast<uint32_t>(cw), static_cast<uint32_t>(cw), wk.peak0, c0.as<float2>());
             CHIP_HIP(hipGetLastError());
+            std::vector<float2> k0(cw * cw);
+            CHIP_HIP(hipMemcpyAsync(k0.data(), c0.p, c0.bytes, hipMemcpyDeviceToHost, st));
+            CHIP_HIP(hipStreamSynchronize(st));
+            const int64_t S = support[0];
+            float pb = 0.0f;
+            for (int64_t iy = -S; iy <= S; ++iy) {
+                const uint64_t ky = static_cast<uint64_t>(std::llabs(iy * static_cast<int64_t>(s)));
+                for (int64_t ix = -S; ix <= S; ++ix) {
+                    const uint64_t kx = static_cast<uint64_t>(std::llabs(ix * static_cast<int64_t>(s)));
+                    pb += k0[kx * cw + ky].x;
+                }
+            }
+            if (!(std::isfinite(pb) && pb > 0.0f))
+                fail(CHIP_ERR_NORMALIZATION, "wproject plane-zero kernel integral is non-positive");
+            wk.pb = pb;
         }
     }
-    (void)stage_w_max;
-    screens.release();
     uint32_t width = 0, max_support = 0;
     for (uint32_t k = 0; k < planes; ++k) {
-        width = std::max(width, crop[k]);
+        width = std::max(width, wk.crop[k]);
         max_support = std::max(max_support, support[k]);
     }
-    const uint64_t kcells = static_cast<uint64_t>(width) * width;
-    DevBuf kernels(kcells * planes * sizeof(float2));
-    CHIP_HIP(hipMemsetAsync(kernels.p, 0, kernels.bytes, st));
-    for (uint32_t k = 0; k < planes; ++k) {
-        CHIP_HIP(hipMemcpy2DAsync(kernels.as<float2>() + k * kcells, width * sizeof(float2),
-                                  stage[k].p, crop[k] * sizeof(float2), crop[k] * sizeof(float2),
-                                  crop[k], hipMemcpyDeviceToDevice, st));
-    }
-    // pb_sum: plane-zero sum of Re at integer taps (gridder.rs:3991-4002),
-    // summed on the host in the reference order from a copy of plane 0.
-    std::vector<float2> k0(kcells);
-    CHIP_HIP(hipMemcpyAsync(k0.data(), kernels.p, kcells * sizeof(float2), hipMemcpyDeviceToHost, st));
-    CHIP_HIP(hipStreamSynchronize(st));
-    stage.clear();
-    float pb = 0.0f;
-    {
-        const int64_t S = support[0];
-        for (int64_t iy = -S; iy <= S; ++iy) {
-            const uint64_t ky = static_cast<uint64_t>(std::llabs(iy * static_cast<int64_t>(s)));
-            for (int64_t ix = -S; ix <= S; ++ix) {
-                const uint64_t kx = static_cast<uint64_t>(std::llabs(ix * static_cast<int64_t>(s)));
-                pb += k0[kx * width + ky].x;
-            }
-        }
-    }
-    if (!(std::isfinite(pb) && pb > 0.0f))
-        fail(CHIP_ERR_NORMALIZATION, "wproject plane-zero kernel integral is non-positive");
-    hipLaunchKernelGGL(k_scale, dim3(nb(kcells * planes, 256)), dim3(256), 0, st,
-                       kernels.as<float2>(), kcells * planes, pb);
-    CHIP_HIP(hipGetLastError());
+    wk.width = width;
+    wk.stride = (2 * max_support + 1) * (2 * max_support + 1);
 
     DevBuf d_sup(planes * sizeof(int32_t));
     std::vector<int32_t> sup32(support.begin(), support.end());
     CHIP_HIP(hipMemcpyAsync(d_sup.p, sup32.data(), d_sup.bytes, hipMemcpyHostToDevice, st));
     const uint32_t axis = static_cast<uint32_t>(2 * R + 1);
     DevBuf d_norm(static_cast<uint64_t>(planes) * axis * axis * sizeof(float));
-    hipLaunchKernelGGL(k_norm_table, dim3(nb(static_cast<uint64_t>(planes) * axis * axis, 128)), dim3(128), 0,
-                       st, kernels.as<float2>(), d_sup.as<int32_t>(), planes, width, s, R,
-                       d_norm.as<float>());
-    CHIP_HIP(hipGetLastError());
+
+    // Pass 2: the tables, resident or (above the budget) one chunk at a time
+    // for the normalisation table only.
+    const uint64_t per_plane = wk_plane_bytes(wk);
+    const uint64_t resident = per_plane * planes;
+    uint32_t chunk = 0;
+    if (g.wk_budget && resident > g.wk_budget) {
+        const uint64_t avail = g.wk_budget > plane_bytes ? g.wk_budget - plane_bytes : 0;
+        chunk = static_cast<uint32_t>(std::min<uint64_t>(std::max<uint64_t>(avail / per_plane, 1), planes));
+    }
+    DevBuf kernels, compact;
+    if (!chunk) {
+        kernels.alloc(static_cast<uint64_t>(width) * width * planes * sizeof(float2));
+        compact.alloc(static_cast<uint64_t>(planes) * axis * axis * wk.stride * sizeof(float2));
+        wk_tables(*g.ctx, wk, d_sup.as<int32_t>(), 0, planes, screen.as<float2>(), kernels.as<float2>(),
+                  d_norm.as<float>(), compact.as<float2>(), st);
+    } else {
+        DevBuf kc(static_cast<uint64_t>(width) * width * chunk * sizeof(float2));
+        for (uint32_t lo = 0; lo < planes; lo += chunk) {
+            const uint32_t hi = std::min(planes, lo + chunk);
+            wk_tables(*g.ctx, wk, d_sup.as<int32_t>(), lo, hi, screen.as<float2>(), kc.as<float2>(),
+                      d_norm.as<float>() + static_cast<uint64_t>(lo) * axis * axis, nullptr, st);
+        }
+        CHIP_HIP(hipStreamSynchronize(st));
+    }
     std::vector<float> norm(static_cast<uint64_t>(planes) * axis * axis);
     CHIP_HIP(hipMemcpyAsync(norm.data(), d_norm.p, d_norm.bytes, hipMemcpyDeviceToHost, st));
-
-    const uint32_t stride = (2 * max_support + 1) * (2 * max_support + 1);
-    const uint64_t n_compact = static_cast<uint64_t>(planes) * axis * axis * stride;
-    DevBuf compact(n_compact * sizeof(float2));
-    hipLaunchKernelGGL(k_compact, dim3(nb(n_compact, 256)), dim3(256), 0, st, kernels.as<float2>(),
-                       d_sup.as<int32_t>(), planes, width, s, R, stride, compact.as<float2>());
-    CHIP_HIP(hipGetLastError());
     CHIP_HIP(hipStreamSynchronize(st));
+    screen.release();
 
     g.wproject = true;
     g.planes = planes;
@@ -293,14 +338,16 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
     g.width = width;
     g.max_support = max_support;
     g.norm_radius = R;
-    g.w_scale = w_scale;
+    g.w_scale = wk.w_scale;
     g.supports = sup32;
     g.norm_table = norm;
     g.d_kernels = std::move(kernels);
     g.d_compact = std::move(compact);
-    g.tap_stride = stride;
+    g.tap_stride = wk.stride;
     g.d_supports = std::move(d_sup);
     g.d_norm_table = std::move(d_norm);
+    g.wk_chunk = chunk;
+    g.wk = std::move(wk);
     g.sinc = sinc_axis(std::max(g.gx, g.gy), s);
     g.sinc_x = sinc_axis(g.gx, s);
     g.sinc_y = sinc_axis(g.gy, s);
@@ -310,6 +357,36 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
     CHIP_HIP(hipMemcpyAsync(g.d_sinc.p, g.sinc.data(), g.d_sinc.bytes, hipMemcpyHostToDevice, st));
     CHIP_HIP(hipMemcpyAsync(g.d_sinc_x.p, g.sinc_x.data(), g.d_sinc_x.bytes, hipMemcpyHostToDevice, st));
     CHIP_HIP(hipMemcpyAsync(g.d_sinc_y.p, g.sinc_y.data(), g.d_sinc_y.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+}
+
+void wk_prepare(const chip_gridder &g, WkScratch &ws) {
+    const WkState &wk = g.wk;
+    const uint64_t axis = static_cast<uint64_t>(2 * wk.R + 1);
+    ws.screen.alloc(static_cast<uint64_t>(wk.conv) * wk.conv * sizeof(float2));
+    ws.kernels.alloc(static_cast<uint64_t>(wk.width) * wk.width * g.wk_chunk * sizeof(float2));
+    ws.compact.alloc(axis * axis * wk.stride * g.wk_chunk * sizeof(float2));
+}
+
+void wk_fill(chip_gridder &g, WkScratch &ws, uint32_t lo, uint32_t hi, GridParams &gp) {
+    if (hi <= lo || hi - lo > g.wk_chunk || hi > g.planes) fail(CHIP_ERR_INVALID, "bad W plane chunk");
+    wk_tables(*g.ctx, g.wk, g.d_supports.as<int32_t>(), lo, hi, ws.screen.as<float2>(), ws.kernels.as<float2>(),
+              nullptr, ws.compact.as<float2>(), g.ctx->stream);
+    gp.kernels = ws.kernels.as<float2>();
+    gp.compact = ws.compact.as<float2>();
+    gp.plane_lo = lo;
+    gp.plane_hi = hi;
+}
+
+void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev) {
+    hipStream_t st = g.ctx->stream;
+    if (!g.wk_chunk) {
+        CHIP_HIP(hipMemcpyAsync(kernels_dev, g.d_kernels.p, g.d_kernels.bytes, hipMemcpyDeviceToDevice, st));
+    } else {
+        DevBuf screen(static_cast<uint64_t>(g.wk.conv) * g.wk.conv * sizeof(float2));
+        wk_tables(*g.ctx, g.wk, g.d_supports.as<int32_t>(), 0, g.planes, screen.as<float2>(), kernels_dev, nullptr,
+                  nullptr, st);
+    }
     CHIP_HIP(hipStreamSynchronize(st));
 }
 

@@ -272,6 +272,16 @@ class StandardGridder:
         self._refresh()
         return self
 
+    def set_wkernel_budget(self, max_table_bytes: int) -> int:
+        """Residency of the W tables: 0 keeps them resident; otherwise tables
+        above max_table_bytes are recomputed by every gridding / degridding
+        call, in chunks of planes that fit the budget.  Returns the planes per
+        chunk (0: resident).  Visibility sets must be created afterwards."""
+        chunk = ctypes.c_uint32()
+        abi.call("chip_gridder_wkernel_budget", self.handle, int(max_table_bytes), ctypes.byref(chunk))
+        self._refresh()
+        return chunk.value
+
     def wproject_tables(self):
         """(supports, kernels (planes, width, width) complex64, norm_table)."""
         i = self.info
@@ -502,6 +512,32 @@ class VisibilitySet:
     def set_cycle_strategy(self, strategy):
         """abi.CYCLE_AUTO, abi.CYCLE_HIPFFT or abi.CYCLE_FUSED for later major cycles."""
         abi.call("chip_vset_cycle_strategy", self.handle, int(strategy))
+
+    def set_schedule(self, threads: int = 0, groups_per_item: int = 0):
+        """Fix the binned kernel's launch schedule; (0, 0) autotunes again at the
+        next major cycle."""
+        abi.call("chip_vset_schedule", self.handle, int(threads), int(groups_per_item))
+
+    def schedule(self):
+        """Schedule in use and the autotuner's timed candidates [(threads, groups, ms)]."""
+        t, gpi, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        buf = (ctypes.c_double * 96)()
+        abi.call("chip_vset_schedule_info", self.handle, ctypes.byref(t), ctypes.byref(gpi), ctypes.byref(n), buf)
+        timed = [(int(buf[3 * k]), int(buf[3 * k + 1]), float(buf[3 * k + 2])) for k in range(n.value)]
+        return {"threads": t.value, "groups_per_item": gpi.value, "timed": timed}
+
+    def set_cycle_graph(self, enable: bool) -> int:
+        """Replay later major cycles from a captured HIP graph (fused strategy,
+        resident W tables).  Returns the replays run so far."""
+        n = ctypes.c_uint64()
+        abi.call("chip_vset_cycle_graph", self.handle, 1 if enable else 0, ctypes.byref(n))
+        return n.value
+
+    def cycle_replays(self) -> int:
+        """Major cycles run by graph replay so far."""
+        n = ctypes.c_uint64()
+        abi.call("chip_vset_cycle_graph", self.handle, -1, ctypes.byref(n))
+        return n.value
 
     def cycle_info(self):
         """Transform strategy in use, fused pass shape and the timed transform ms."""

@@ -195,6 +195,21 @@ int chip_max_abs_w(chip_gridder *g, const chip_samples *s, double *out, char **e
  * sampling = 0 selects the reference rule (4 if planes > 1 else 1, :3413). */
 int chip_gridder_attach_wproject(chip_gridder *g, double max_abs_w, uint32_t explicit_planes,
                                  uint32_t sampling, char **err);
+/* Residency of the W tables (WProjector::new, gridder.rs:3403-3586: the
+ * reference keeps them for the projector's lifetime).  max_table_bytes = 0
+ * (default) keeps them resident on the device.  Otherwise, when the tables of
+ * every plane (quarter planes plus the gathered per-offset taps) exceed
+ * max_table_bytes, they are not stored: each call that grids or degrids
+ * recomputes them (screen, FFT, crop, normalisation, gather) for chunks of
+ * planes whose tables, plus one FFT plane of scratch, fit in max_table_bytes,
+ * running each chunk's samples before the next chunk (memory is held only
+ * during the call).  Results are bit-identical to the resident tables.  Takes
+ * effect immediately (an attached W projector is rebuilt); visibility sets
+ * created before must be created again.  chunk_planes (may be NULL) receives
+ * the planes per chunk, 0 when resident.  The AW atlas is always resident.
+ * No reference counterpart. */
+int chip_gridder_wkernel_budget(chip_gridder *g, uint64_t max_table_bytes, uint32_t *chunk_planes,
+                                char **err);
 /* Host copies: supports[planes], kernels[planes][width][width] complex64,
  * norm_table[planes][2R+1][2R+1].  Any pointer may be NULL. */
 int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernels,
@@ -350,6 +365,33 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
  * No reference counterpart (the reference's DirtyProductFftPolicy picks its
  * host FFT backend, lib.rs:1280-1310). */
 int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
+/* Launch schedule of the binned kernel: threads per workgroup (a multiple
+ * of 64, at most 512) and groups of identical plans per work item (the
+ * splitting of each (channel, uv tile) bin).  Unless fixed here, the first
+ * major cycle times every candidate of {256, 512} threads x {256, 512, 1024,
+ * 2048} groups on its own grids and keeps the fastest (default before that:
+ * 512, 1024).  threads = groups_per_item = 0 asks for autotuning again at the
+ * next major cycle.  Results do not depend on the schedule beyond the order
+ * of float additions.  No reference counterpart (the Metal path dispatches
+ * fixed-size chunks, lib.rs:72180-72290). */
+int chip_vset_schedule(chip_vset *vs, uint32_t threads, uint32_t groups_per_item, char **err);
+/* Schedule in use and, after autotuning, the timed candidates: n_timed
+ * triples (threads, groups per item, ms) in timed (at most 32 triples; may be
+ * NULL).  Any pointer may be NULL. */
+int chip_vset_schedule_info(const chip_vset *vs, uint32_t *threads, uint32_t *groups_per_item, uint32_t *n_timed,
+                            double *timed, char **err);
+/* Graph replay of later major cycles: enable = 1 captures the next major
+ * cycle's kernels (forward passes, zero fill, binned residual kernel, inverse
+ * passes) into a HIP graph once and replays it with one launch per call, on
+ * the context's stream; a call with other model / residual pointers captures
+ * again.  Needs the fused strategy (CHIP_ERR_INVALID at the major cycle
+ * otherwise) and resident W tables (CHIP_ERR_INVALID here otherwise).  While
+ * replaying, chip_vset_stats reports no binned-kernel time (-1).  enable = 0
+ * returns to eager launches, -1 leaves the setting unchanged.  replays (may
+ * be NULL) receives the number of major cycles run by replay so far.  No
+ * reference counterpart (the Metal path encodes one command buffer per
+ * dispatch, lib.rs:72270-72290). */
+int chip_vset_cycle_graph(chip_vset *vs, int enable, uint64_t *replays, char **err);
 /* Strategy in use (-1 before the first major cycle); shape[3] of the fused
  * passes (columns per column-pass workgroup, its threads, rows per rows-pass
  * workgroup; zeros for hipFFT); the timed forward + inverse transform ms of
