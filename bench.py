@@ -393,6 +393,32 @@ def reduce_traffic(fetch_dir, write_dir, kernel, out_json, out_txt, commands):
     print("\n".join(lines))
 
 
+def reduce_counters(directory, kernel, out_txt, command, kernel_ms=None):
+    """rocprofv3 --pmc run -> per-counter mean over the dispatches of `kernel`,
+    with the FP32 operation rate when the FP32 instruction counters are there."""
+    per = {}
+    for r in _profile_csv(directory, "counter_collection.csv"):
+        if kernel in r["Kernel_Name"]:
+            per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    if not per:
+        raise SystemExit(f"no dispatches of {kernel}")
+    lines = [f"command: {command}", "", f"kernel: {kernel}"]
+    mean = {k: sum(v) / len(v) for k, v in per.items()}
+    for k in sorted(mean):
+        lines.append(f"  {k:28s} {mean[k]:16.4f}   ({len(per[k])} dispatches)")
+    f32 = ("SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_TRANS_F32")
+    if all(k in mean for k in f32):
+        ops = (2 * mean[f32[0]] + mean[f32[1]] + mean[f32[2]] + mean[f32[3]]) * 64
+        lines.append(f"  FP32 operations executed per dispatch: {ops:.4e}")
+        if kernel_ms:
+            tf = ops / (float(kernel_ms) * 1e-3) / 1e12
+            lines.append(f"  at {float(kernel_ms):.3f} ms per dispatch: {tf:.2f} TFLOP/s executed = "
+                         f"{100 * tf / FP32_PEAK_TFLOPS:.1f} % of the FP32 peak ({FP32_PEAK_TFLOPS} TFLOP/s)")
+    with open(out_txt, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -413,7 +439,12 @@ def main():
     ap.add_argument("--reduce-traffic", nargs=7,
                     metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL", "OUT_JSON", "OUT_TXT", "FETCH_CMD", "WRITE_CMD"),
                     help="reduce rocprofv3 FETCH_SIZE / WRITE_SIZE runs to bytes per launch (no GPU work)")
+    ap.add_argument("--reduce-counters", nargs="+", metavar="ARG",
+                    help="DIR KERNEL OUT_TXT COMMAND [KERNEL_MS]: summarise a rocprofv3 --pmc run (no GPU work)")
     args = ap.parse_args()
+    if args.reduce_counters:
+        reduce_counters(*args.reduce_counters)
+        return
     if args.reduce_trace:
         reduce_trace(*args.reduce_trace)
         return

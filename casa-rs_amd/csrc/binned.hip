@@ -90,6 +90,7 @@ constexpr int kBig = 8;    // taps per lane per chunk of the large-footprint pat
 template <int MODE, bool WP, int KS>
 __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
                                                 const uint4 *__restrict__ grec,
+                                                const float4 *__restrict__ gsum,
                                                 const uint2 *__restrict__ plan,
                                                 const float2 *__restrict__ vis,
                                                 const float *__restrict__ wt,
@@ -118,6 +119,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     constexpr bool kModel = MODE == kModeResidual || MODE == kModeDegrid;
     constexpr bool kAcc = MODE == kModeResidual || MODE == kModeDirty || MODE == kModeAdjoint;
     constexpr bool kPsf = MODE == kModePsf || MODE == kModeDirty;
+    // gridding modes read each group's precomputed sums instead of its samples
+    constexpr bool kSums = MODE == kModePsf || MODE == kModeDirty || MODE == kModeResidual;
 
     const int tid = threadIdx.x;
     const int nth = blockDim.x;
@@ -254,7 +257,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         d.wj = 0.0f;
         d.fj = 0u;
         d.idx = 0u;
-        if (j < s1) {
+        if (!kSums && j < s1) {
             if (MODE == kModeAdjoint) {
                 d.v = values[index[j]];
             } else if (MODE == kModeDegrid) {
@@ -287,14 +290,21 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         }
     };
 
-    auto process = [&](uint32_t s0, uint32_t s1, const uint2 pk, const Foot &f, const GroupData &d) {
+    auto process = [&](uint32_t s0, uint32_t s1, const uint2 pk, const Foot &f, const GroupData &d,
+                       const float4 gs) {
         const uint32_t count = s1 - s0;
-        // Group sums, lane-parallel over the samples:
+        // Group sums:
         //   PSF/DIRTY: w = sum w_i (all samples), v = sum w_i V_i (finite V)
         //   RESIDUAL:  w = sum w_i, v = sum w_i V_i over usable samples
-        //   ADJOINT:   v = sum values_i
+        //   (both precomputed per group when the stream was built)
+        //   ADJOINT:   v = sum values_i, lane-parallel over the samples
         float vr = 0.0f, vi = 0.0f, w = 0.0f;
-        if (MODE != kModeDegrid) {
+        if (kSums) {
+            w = gs.x;
+            vr = gs.y;
+            vi = gs.z;
+            if (MODE == kModeResidual && w == 0.0f) return;
+        } else if (MODE != kModeDegrid) {
             auto add = [&](uint32_t fj, float wj, float2 vj) {
                 if (MODE == kModeAdjoint) {
                     vr += vj.x;
@@ -464,9 +474,9 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         }
     };
 
-    // Group records (first sample, end, packed plan of the group) are read 64
-    // at a time with one coalesced vector load, lane l holding the record of
-    // group base + l, and the next 64 are prefetched a batch ahead.  A group's
+    // Group records (first sample, end, packed plan of the group) and, in the
+    // gridding modes, the group's sums are read 64 at a time with coalesced
+    // vector loads, lane l holding those of group base + l.  A group's
     // record moves to scalar registers with readlane, so its footprint is
     // scalar arithmetic and no scalar load waits behind the LDS traffic of the
     // group being processed.  Software pipeline: while group g is processed,
@@ -474,6 +484,16 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     auto rec_load = [&](uint32_t base) {
         const uint32_t g = base + static_cast<uint32_t>(lane);
         return g < g_end ? grec[g] : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto sum_load = [&](uint32_t base) {
+        const uint32_t g = base + static_cast<uint32_t>(lane);
+        return (kSums && g < g_end) ? gsum[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
+    auto sum_at = [&](const float4 &sb, uint32_t k) {
+        const int kk = static_cast<int>(k);
+        return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(sb.x), kk)),
+                           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sb.y), kk)),
+                           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sb.z), kk)), 0.0f);
     };
     auto rec_at = [&](const uint4 &rb, uint32_t k) {
         const int kk = static_cast<int>(k);
@@ -485,10 +505,11 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     if (g_begin < g_end) {
         uint32_t base = g_begin;
         uint4 rb = rec_load(base);
-        uint4 rn = rec_load(base + 64);
+        float4 sb = sum_load(base);
         const uint4 c0 = rec_at(rb, 0);
         uint32_t c_s0 = c0.x, c_s1 = c0.y;
         uint2 c_pk = make_uint2(c0.z, c0.w);
+        float4 c_g = kSums ? sum_at(sb, 0) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         Foot c_f = footprint(c_pk);
         GroupData c_d;
         load_group(c_s0, c_s1, c_f, c_d);
@@ -497,27 +518,30 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             GroupData n_d = c_d;
             uint32_t n_s0 = 0, n_s1 = 0;
             uint2 n_pk = make_uint2(0u, 0u);
+            float4 n_g = c_g;
             if (grp + 1 < g_end) {
                 uint32_t k = grp + 1 - base;
                 if (k == 64) {
                     base += 64;
-                    rb = rn;
-                    rn = rec_load(base + 64);
+                    rb = rec_load(base);
+                    sb = sum_load(base);
                     k = 0;
                 }
                 const uint4 nr = rec_at(rb, k);
                 n_s0 = nr.x;
                 n_s1 = nr.y;
                 n_pk = make_uint2(nr.z, nr.w);
+                if (kSums) n_g = sum_at(sb, k);
                 n_f = footprint(n_pk);
                 load_group(n_s0, n_s1, n_f, n_d);
             }
-            process(c_s0, c_s1, c_pk, c_f, c_d);
+            process(c_s0, c_s1, c_pk, c_f, c_d, c_g);
             c_s0 = n_s0;
             c_s1 = n_s1;
             c_pk = n_pk;
             c_f = n_f;
             c_d = n_d;
+            c_g = n_g;
         }
     }
 #pragma unroll
@@ -560,9 +584,12 @@ static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *v
     if (lds > 64 * 1024)
         CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP, KS>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    const float4 *gsum = MODE == kModeResidual                         ? vs.gsum_r.as<float4>()
+                         : (MODE == kModePsf || MODE == kModeDirty) ? vs.gsum_d.as<float4>()
+                                                                     : nullptr;
     auto launch = [&](const GridParams &p, uint64_t first, uint64_t count) {
         hipLaunchKernelGGL((k_binned<MODE, WP, KS>), dim3(static_cast<unsigned>(count)), dim3(vs.threads),
-                           lds, st, p, vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(),
+                           lds, st, p, vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(), gsum,
                            vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
                            vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
         CHIP_HIP(hipGetLastError());

@@ -306,6 +306,8 @@ struct chip_vset {
     chip::DevBuf index;   // uint32 original sample index
     chip::DevBuf items;   // WorkItem (chan, tile, first group, group count)
     chip::DevBuf grec;    // uint4 per group: first stream position, end, packed plan (x, y)
+    chip::DevBuf gsum_d;  // float4 per group: sum w, sum w V (finite V)      (PSF, DIRTY)
+    chip::DevBuf gsum_r;  // float4 per group: sum w, sum w V (usable samples) (RESIDUAL)
     uint64_t n_groups = 0;
     std::vector<chip_sums> sums;
     uint64_t taps = 0;    // sum over accepted samples of (2S+1)^2

@@ -233,6 +233,36 @@ __global__ void k_group_records(const uint32_t *gstart, const uint2 *plan, uint3
     rec[g] = make_uint4(s0, s1, pk.x, pk.y);
 }
 
+// Per-group sums of what the gridding modes add, computed once in FP64 over
+// the group's samples and rounded to FP32:
+//   dsum = (sum w_i, sum w_i V_i over finite V)      PSF and DIRTY
+//   rsum = (sum w_i, sum w_i V_i) over residual-usable samples   RESIDUAL
+// so that a gridding pass reads one record per group instead of its samples.
+__global__ void k_group_sums(const uint4 *grec, const uint2 *plan, const float2 *vis, const float *wt,
+                             uint32_t n_groups, float4 *dsum, float4 *rsum) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const uint4 r = grec[g];
+    double wa = 0.0, dr = 0.0, di = 0.0, wr = 0.0, rr = 0.0, ri = 0.0;
+    for (uint32_t j = r.x; j < r.y; ++j) {
+        const uint32_t f = plan[j].y;
+        const double w = wt[j];
+        const float2 v = vis[j];
+        wa += w;
+        if (f & kVisOkBit) {
+            dr += static_cast<double>(v.x) * w;
+            di += static_cast<double>(v.y) * w;
+        }
+        if (f & kResidOkBit) {
+            wr += w;
+            rr += static_cast<double>(v.x) * w;
+            ri += static_cast<double>(v.y) * w;
+        }
+    }
+    dsum[g] = make_float4(static_cast<float>(wa), static_cast<float>(dr), static_cast<float>(di), 0.0f);
+    rsum[g] = make_float4(static_cast<float>(wr), static_cast<float>(rr), static_cast<float>(ri), 0.0f);
+}
+
 // Writes the packed stream in sorted order and accumulates per-channel sums
 // (normalisation sumwt, reported sumwt, accepted count, tap count).  Sorted
 // samples of one block span few channels, so sums are first reduced in LDS
@@ -508,6 +538,14 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     if (n_groups) {
         hipLaunchKernelGGL(k_group_records, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st,
                            gstart.as<uint32_t>(), vs.plan.as<uint2>(), n_groups, vs.grec.as<uint4>());
+        CHIP_HIP(hipGetLastError());
+    }
+    vs.gsum_d.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));
+    vs.gsum_r.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));
+    if (n_groups) {
+        hipLaunchKernelGGL(k_group_sums, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st, vs.grec.as<uint4>(),
+                           vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(), n_groups,
+                           vs.gsum_d.as<float4>(), vs.gsum_r.as<float4>());
         CHIP_HIP(hipGetLastError());
     }
     CHIP_HIP(hipStreamSynchronize(st));

@@ -11,7 +11,8 @@
 * Differentiable imaging at size (1024^2 sky of Gaussians and a disk, 2M
   visibilities, 32 W planes): dot-product adjoint, central finite differences
   (eps 1e-3) on 64 seeded pixels of chi^2 + 1e-3 TV, the gradient against the
-  direct DFT on a 10k-visibility subset, and 200 Adam steps.
+  direct DFT on a 10k-visibility subset, and 200 Adam steps, with the W
+  kernels recomputed on the fly (no table kept between calls).
 """
 import importlib
 
@@ -200,6 +201,9 @@ def test_config5_differentiable_at_size(gpu, oracle_mod):
     cell = 1.0 / (3.0 * float(np.max(np.hypot(u, v))))
     g = gpu.StandardGridder(nx, nx, cell, cell)
     g.attach_wproject(float(np.max(np.abs(w))), planes=32)
+    # W kernels recomputed on the fly: no table is kept between calls, every
+    # degrid / grid recomputes them one plane at a time
+    assert g.set_wkernel_budget(1) == 1
     ones = torch.ones(n, dtype=torch.float32)
     b = gpu.VisibilityBatch(t(u), t(v), t(w), ones, torch.zeros(n, dtype=torch.complex64), ones)
     vs = g.visibility_set(b)
