@@ -53,6 +53,7 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.paired = 0;
     gp.hy = static_cast<uint32_t>(g.gy / 2);
     gp.model_signed = 0;
+    gp.model_half = 0;
     gp.sign_base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
     gp.taps = g.d_taps.as<float>();
     gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
@@ -142,7 +143,13 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             float2 m = make_float2(0.0f, 0.0f);
             if (model && ly < L && gxx >= 0 && gyy >= 0 && gxx < static_cast<int>(gp.gx) &&
                 gyy < static_cast<int>(gp.gy)) {
-                m = model[chan_off + cell_offset(gp, gxx, gyy)];
+                if (gp.model_half && gyy > static_cast<int>(gp.hy)) {
+                    const uint32_t mx = gxx == 0 ? 0u : gp.gx - static_cast<uint32_t>(gxx);
+                    m = model[chan_off + cell_offset(gp, mx, gp.gy - static_cast<uint32_t>(gyy))];
+                    m.y = -m.y;
+                } else {
+                    m = model[chan_off + cell_offset(gp, gxx, gyy)];
+                }
                 if (gp.model_signed && ((gxx + gyy + gp.sign_base) & 1)) {
                     m.x = -m.x;
                     m.y = -m.y;

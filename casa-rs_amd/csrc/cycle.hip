@@ -282,6 +282,7 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
 
     if (ws.strategy == CHIP_CYCLE_FUSED) {
         gp.paired = 1;
+        gp.model_half = 1;  // the forward columns pass stores y <= gy/2 only
         gp.cstride = ws.xp.xg.gstride;
         gp.bstride = g.gx * xf::kBlock;
         gp.lower = static_cast<uint64_t>(ws.xp.xg.nyb_h) * gp.bstride;

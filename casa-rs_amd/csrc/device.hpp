@@ -285,6 +285,9 @@ struct GridParams {
     // checkerboarded model, so (-1)^(x + y + sign_base) is applied on load.
     uint32_t model_signed;
     uint32_t sign_base;      // gx/2 + gy/2
+    // Major cycle: only the columns y <= hy of the (Hermitian) model grid are
+    // stored; G[x, y > hy] = conj G[(gx - x) mod gx, gy - y].
+    uint32_t model_half;
     const float *taps;       // (os+1) x 7
     const float2 *kernels;   // planes x width x width
     const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest

@@ -317,18 +317,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
     }
     __syncthreads();
     run_fft<false, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
-    // column ky into upper block yb; its mirror (row -kx, column gy - ky,
-    // i.e. m = ky) into lower block yb at the same lane
+    // column ky into upper block yb.  The columns ky > gy/2 are not stored:
+    // the transform of a real image is Hermitian, so the binned kernel reads
+    // G[kx, ky] as conj G[-kx, gy - ky] from the upper blocks.
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
     float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
-    float2 *lo = up + static_cast<uint64_t>(xg.nyb_h) * bstride;
     const uint32_t hy = xg.gy / 2;
     for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
-        const uint32_t kx = e >> 3, lane = e & 7, ky = yb * kBlock + lane;
-        if (ky > hy) continue;
-        const float2 v = buf[e];
-        up[e] = v;
-        if (ky > 0 && ky < hy) lo[(kx == 0 ? 0u : n - kx) * kBlock + lane] = make_float2(v.x, -v.y);
+        const uint32_t ky = yb * kBlock + (e & 7);
+        if (ky <= hy) up[e] = buf[e];
     }
 }
 
