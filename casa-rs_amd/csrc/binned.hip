@@ -54,6 +54,7 @@ GridParams grid_params(const chip_gridder &g, uint32_t tile, uint32_t halo, uint
     gp.hy = static_cast<uint32_t>(g.gy / 2);
     gp.model_signed = 0;
     gp.model_half = 0;
+    gp.acc_hermitian = 0;
     gp.sign_base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
     gp.taps = g.d_taps.as<float>();
     gp.kernels = g.wproject ? g.d_kernels.as<float2>() : nullptr;
@@ -570,8 +571,22 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         }
         if (kAcc) {
             const float r = A_re[c], i = A_im[c];
-            if (r != 0.0f) unsafeAtomicAdd(acc_out + gi, r);
-            if (i != 0.0f) unsafeAtomicAdd(acc_out + gi + 1, i);
+            if (!gp.acc_hermitian) {
+                if (r != 0.0f) unsafeAtomicAdd(acc_out + gi, r);
+                if (i != 0.0f) unsafeAtomicAdd(acc_out + gi + 1, i);
+            } else if (r != 0.0f || i != 0.0f) {
+                const uint32_t hy = gp.hy, ux = static_cast<uint32_t>(gxx), uy = static_cast<uint32_t>(gyy);
+                const uint32_t mx = ux == 0 ? 0u : gp.gx - ux;
+                if (uy <= hy) {
+                    unsafeAtomicAdd(acc_out + gi, r);
+                    unsafeAtomicAdd(acc_out + gi + 1, i);
+                }
+                if (uy == 0 || uy >= hy) {  // conj v to (-x, -y) when that is a stored column
+                    const uint64_t mi = 2 * (chan_off + cell_offset(gp, mx, uy == 0 ? 0u : gp.gy - uy));
+                    unsafeAtomicAdd(acc_out + mi, r);
+                    unsafeAtomicAdd(acc_out + mi + 1, -i);
+                }
+            }
         }
     }
 }

@@ -89,18 +89,26 @@ __global__ void k_cycle_crop(CycleGeom cg, const float2 *__restrict__ raw, const
 
 // Zero fill by a kernel rather than hipMemsetAsync, so that the major cycle
 // can be captured into a graph (memset nodes are avoided there).
-__global__ void k_zero(float4 *p, uint64_t n) {
+// Segment s of n_seg covers float4s [s * stride, s * stride + seg).
+__global__ void k_zero(float4 *p, uint64_t seg, uint64_t stride, uint64_t n_seg) {
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    for (uint64_t k = i; k < n; k += stride) p[k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t n = seg * n_seg;
+    for (uint64_t k = i; k < n; k += step) {
+        const uint64_t s = k / seg;
+        p[s * stride + (k - s * seg)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
 }
 
-void zero_fill(void *p, uint64_t bytes, hipStream_t st) {
-    if (bytes % sizeof(float4)) fail(CHIP_ERR_INVALID, "zero fill of a buffer that is not 16-byte sized");
-    const uint64_t n = bytes / sizeof(float4);
+// n_seg segments of seg_bytes, stride_bytes apart (16-byte multiples).
+void zero_fill(void *p, uint64_t seg_bytes, uint64_t stride_bytes, uint64_t n_seg, hipStream_t st) {
+    if (seg_bytes % sizeof(float4) || stride_bytes % sizeof(float4) || seg_bytes > stride_bytes)
+        fail(CHIP_ERR_INVALID, "zero fill of segments that are not 16-byte sized");
+    const uint64_t seg = seg_bytes / sizeof(float4), n = seg * n_seg;
     if (n == 0) return;
     const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, static_cast<float4 *>(p), n);
+    hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, static_cast<float4 *>(p), seg,
+                       stride_bytes / sizeof(float4), n_seg);
     CHIP_HIP(hipGetLastError());
 }
 
@@ -282,7 +290,8 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
 
     if (ws.strategy == CHIP_CYCLE_FUSED) {
         gp.paired = 1;
-        gp.model_half = 1;  // the forward columns pass stores y <= gy/2 only
+        gp.model_half = 1;     // the forward columns pass stores y <= gy/2 only
+        gp.acc_hermitian = 1;  // the inverse columns pass reads 2H, y <= gy/2
         gp.cstride = ws.xp.xg.gstride;
         gp.bstride = g.gx * xf::kBlock;
         gp.lower = static_cast<uint64_t>(ws.xp.xg.nyb_h) * gp.bstride;
@@ -291,7 +300,9 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
         // the half spectrum borrows the residual grid, the inverse rows the model grid
         auto enqueue = [&](hipStream_t s, bool timed) {
             xf::forward_model(ws.xp, model, Rg, M, s);
-            zero_fill(Rg, ws.res_grid.bytes, s);
+            // the upper blocks of every channel (the residual is kept as 2H there)
+            const uint64_t gs = ws.xp.xg.gstride * sizeof(float2);
+            zero_fill(Rg, gs / 2, gs, nc, s);
             vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), s, timed);
             xf::inverse_residual(ws.xp, Rg, M, d_nsum, d_peak, out, s);
         };

@@ -288,6 +288,11 @@ struct GridParams {
     // Major cycle: only the columns y <= hy of the (Hermitian) model grid are
     // stored; G[x, y > hy] = conj G[(gx - x) mod gx, gy - y].
     uint32_t model_half;
+    // Major cycle: the residual is accumulated as 2H, H = (G[k] + conj G[-k]) / 2
+    // its Hermitian part, in the columns y <= hy only: a cell (x, y > hy) adds
+    // conj v to (-x, gy - y); a cell of column 0 or hy adds v to itself and
+    // conj v to (-x, y).
+    uint32_t acc_hermitian;
     const float *taps;       // (os+1) x 7
     const float2 *kernels;   // planes x width x width
     const float2 *compact;   // planes x n_off^2 x tap_stride, tap order (ix, iy) with iy fastest

@@ -10,13 +10,13 @@
 //                 spectra; only ky in [0, gy/2] is kept (half spectrum).
 //   column pass : one block of 8 columns ky <= gy/2 per workgroup; rows
 //                 outside the image are zero -> FFT along x; writes column
-//                 ky and its Hermitian mirror -ky, so the full grid comes
-//                 from half of the columns.
+//                 ky only (the binned kernel reads the columns ky > gy/2 as
+//                 conjugate mirrors).
 // Inverse (centered inverse FFT + crop + correction, gridder.rs:1646-1715):
 // only Re of the inverse is used, which is the inverse of the Hermitian part
 // H = (G[k] + conj G[-k]) / 2 of the residual grid G.
-//   column pass : one block of 8 columns ky <= gy/2 per workgroup, forms H
-//                 from the column and its mirror, IFFT along x; keeps only
+//   column pass : one block of 8 columns ky <= gy/2 per workgroup of 2H
+//                 (accumulated so by the binned kernel), IFFT along x; keeps only
 //                 the nx rows the crop needs (shift theorem: output row
 //                 (blc_x + x + gx/2) mod gx).
 //   rows pass   : 8 rows per workgroup, each Hermitian in ky, two rows per
@@ -339,29 +339,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
     load_split_twiddles(tw, ax, tid, kThreads);
     // Hermitian part H[kx, ky] = (G[kx, ky] + conj G[-kx, -ky]) / 2 of the
-    // columns ky <= gy/2; its inverse transform is real and equals
-    // Re(centered inverse of G) after the shifts.
+    // columns ky <= gy/2, accumulated as 2H by the binned kernel; its inverse
+    // transform is real and equals Re(centered inverse of G) after the shifts.
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
     const float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
-    const float2 *lo = up + static_cast<uint64_t>(xg.nyb_h) * bstride;
     const uint32_t hy = xg.gy / 2;
-    float2 gv[E], mv[E];
+    float2 gv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
         const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
-        const uint32_t kx = e >> 3, lane = e & 7, ky = yb * kBlock + lane;
-        gv[t] = mv[t] = make_float2(0.0f, 0.0f);
-        if (e < n * kBlock && ky <= hy) {
-            const uint32_t mx = kx == 0 ? 0u : n - kx;
-            gv[t] = up[e];
-            mv[t] = (ky == 0 || ky == hy) ? up[mx * kBlock + lane] : lo[mx * kBlock + lane];
-        }
+        const uint32_t ky = yb * kBlock + (e & 7);
+        gv[t] = (e < n * kBlock && ky <= hy) ? up[e] : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int t = 0; t < E; ++t) {
         const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
-        if (e < n * kBlock)
-            buf[e] = make_float2(0.5f * (gv[t].x + mv[t].x), 0.5f * (gv[t].y - mv[t].y));
+        if (e < n * kBlock) buf[e] = make_float2(0.5f * gv[t].x, 0.5f * gv[t].y);
     }
     __syncthreads();
     run_fft<true, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
