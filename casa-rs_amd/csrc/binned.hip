@@ -205,7 +205,6 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         accr[k] = acci[k] = pacr[k] = paci[k] = 0.0f;
     };
 
-    const int R = gp.norm_radius;
     const uint32_t per = (it.count + nwaves - 1) / nwaves;
     const uint32_t g_begin = it.start + static_cast<uint32_t>(wave) * per;
     const uint32_t g_end = min(it.start + it.count, g_begin + per);
@@ -233,14 +232,13 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             f.bx = ax;  // standard anchors are already x_start
             f.by = ay;
         } else {
-            const int ox = static_cast<int>(pk.y & 0xffu) - 128;
-            const int oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
-            const int plane = static_cast<int>((pk.y >> 16) & 0x7ffu);
-            f.side = 2 * gp.supports[plane] + 1;
-            const uint64_t po = static_cast<uint64_t>(plane - static_cast<int>(gp.plane_lo)) * gp.n_off * gp.n_off +
-                                static_cast<uint64_t>((ox + R) * static_cast<int>(gp.n_off) + (oy + R));
-            f.ktap = gp.compact + po * gp.tap_stride;
-            f.csign = (pk.y & kConjBit) ? -1.0f : 1.0f;
+            // the group record's fourth word: tap-table index (25 bits),
+            // conjugation (bit 25), half-support (bits 26-31)
+            const uint32_t po = pk.y & 0x1ffffffu;
+            f.side = 2 * static_cast<int>(pk.y >> 26) + 1;
+            const uint64_t local = po - gp.plane_lo * gp.n_off * gp.n_off;  // tables of this chunk
+            f.ktap = gp.compact + local * gp.tap_stride;
+            f.csign = ((pk.y >> 25) & 1u) ? -1.0f : 1.0f;
             if (kModel && gp.aw_norm) f.awn = gp.aw_norm[po];
             f.bx = ax - (f.side >> 1);
             f.by = ay - (f.side >> 1);

@@ -225,12 +225,26 @@ __global__ void k_group_starts(const uint64_t *keys, const uint32_t *flags, cons
 
 // One record per group: (first sample, end, packed plan), so that the binned
 // kernel reads a group's bounds and plan with one load.
-__global__ void k_group_records(const uint32_t *gstart, const uint2 *plan, uint32_t n_groups, uint4 *rec) {
+// W / AWProject: the fourth word is not the packed plan but what the binned
+// kernel's footprint needs, so that it loads nothing else per group:
+// tap-table index po = table * n_off^2 + (off_x + R) n_off + (off_y + R)
+// (25 bits), conjugation (bit 25), half-support (bits 26-31).
+constexpr uint32_t kRecPoBits = 25;
+
+__global__ void k_group_records(const uint32_t *gstart, const uint2 *plan, uint32_t n_groups, const int32_t *supports,
+                                uint32_t n_off, int32_t R, uint4 *rec) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n_groups) return;
     const uint32_t s0 = gstart[g], s1 = gstart[g + 1];
     const uint2 pk = plan[s0];
-    rec[g] = make_uint4(s0, s1, pk.x, pk.y);
+    uint32_t w = pk.y;
+    if (supports) {
+        const int ox = static_cast<int>(pk.y & 0xffu) - 128, oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
+        const uint32_t table = (pk.y >> 16) & 0x7ffu;
+        const uint32_t po = (table * n_off + static_cast<uint32_t>(ox + R)) * n_off + static_cast<uint32_t>(oy + R);
+        w = po | ((pk.y & kConjBit) ? (1u << kRecPoBits) : 0u) | (static_cast<uint32_t>(supports[table]) << 26);
+    }
+    rec[g] = make_uint4(s0, s1, pk.x, w);
 }
 
 // Per-group sums of what the gridding modes add, computed once in FP64 over
@@ -536,8 +550,16 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     CHIP_HIP(hipMemcpyAsync(gstart.as<uint32_t>() + n_groups, &nv32, 4, hipMemcpyHostToDevice, st));
     vs.grec.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(uint4));
     if (n_groups) {
+        const uint32_t n_off = static_cast<uint32_t>(2 * g.norm_radius + 1);
+        if (g.wproject) {
+            if (g.max_support > 63) fail(CHIP_ERR_INVALID, "W support above 63 for the binned stream");
+            if (static_cast<uint64_t>(g.planes) * n_off * n_off >= (1ull << kRecPoBits))
+                fail(CHIP_ERR_INVALID, "too many W tap tables for the binned stream");
+        }
         hipLaunchKernelGGL(k_group_records, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st,
-                           gstart.as<uint32_t>(), vs.plan.as<uint2>(), n_groups, vs.grec.as<uint4>());
+                           gstart.as<uint32_t>(), vs.plan.as<uint2>(), n_groups,
+                           g.wproject ? g.d_supports.as<int32_t>() : nullptr, n_off,
+                           static_cast<int32_t>(g.norm_radius), vs.grec.as<uint4>());
         CHIP_HIP(hipGetLastError());
     }
     vs.gsum_d.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));

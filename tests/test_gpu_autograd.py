@@ -108,3 +108,49 @@ def test_gradient_against_direct_dft(gpu, oracle_mod):
     sl = slice(nx // 8, nx - nx // 8)
     err = np.max(np.abs(got[sl, sl] - ref[sl, sl])) / np.max(np.abs(ref[sl, sl]))
     assert err < 3e-2
+
+
+def test_gradient_against_major_cycle_residual_image(gpu):
+    """The chi^2 gradient of the autograd operator against the major cycle's
+    residual image (compute_residual_standard_tiled, lib.rs:68862-68932):
+    for L = sum w |A x - V|^2, dL/dx = -2 Re(A^H W (V - A x)) =
+    -2 gx gy sumwt psf_peak R(x), with R the residual image (the centered
+    inverse transform is F^H / (gx gy), and both paths apply the same grid
+    correction).  FP32 on both sides: 2e-4 of the gradient's peak."""
+    ag = importlib.import_module("casa_rs_amd.autograd")
+    nx, cell = 64, np.deg2rad(20.0 / 3600.0)
+    rng = np.random.default_rng(11)
+    n = 30000
+    umax = 0.4 / cell
+    u, v = rng.uniform(-umax, umax, n), rng.uniform(-umax, umax, n)
+    w = np.zeros(n)
+    g = gpu.StandardGridder(nx, nx, cell, cell)
+    wt = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    x_true = np.zeros((nx, nx), np.float32)
+    x_true[rng.integers(12, nx - 12, 8), rng.integers(12, nx - 12, 8)] = rng.uniform(0.2, 1.0, 8)
+    # data: the forward operator of x_true plus noise, through a set without data
+    zeros = torch.zeros(n, dtype=torch.complex64)
+    probe = g.visibility_set(gpu.VisibilityBatch(torch.from_numpy(u), torch.from_numpy(v), torch.from_numpy(w),
+                                                 torch.from_numpy(wt), zeros, torch.ones(n)))
+    V = (ag.DegridOperator(g, probe).forward_linear(torch.from_numpy(x_true).cuda()).cpu()
+         + 0.01 * torch.randn(n, dtype=torch.complex64))
+    vs = g.visibility_set(gpu.VisibilityBatch(torch.from_numpy(u), torch.from_numpy(v), torch.from_numpy(w),
+                                              torch.from_numpy(wt), V, torch.ones(n)))
+    op = ag.DegridOperator(g, vs)
+    psf = g.new_grids(1)
+    vs.grid(gpu.MODE_PSF, psf=psf)
+    nsum = vs.sums[0]["normalization_sumwt"]
+    _, _, peak = g.finish_products(psf, None, nsum)
+
+    x0 = torch.from_numpy(0.5 * x_true).cuda()
+    x = x0.clone().requires_grad_(True)
+    Vd, wd = V.cuda(), torch.from_numpy(wt).cuda()
+    r = op(x) - Vd
+    L = torch.sum(wd * (r.real ** 2 + r.imag ** 2))
+    L.backward()
+    grad = x.grad.double().cpu().numpy()
+    R = vs.major_cycle(x0[None], nsum, peak)[0].double().cpu().numpy()
+    expect = -2.0 * g.info.gx * g.info.gy * nsum * float(peak[0]) * R
+    err = float(np.max(np.abs(grad - expect)) / np.max(np.abs(expect)))
+    print(f"gradient vs -2 gx gy sumwt peak R: {err:.2e} of peak")
+    assert err < 2e-4

@@ -5,9 +5,11 @@
   * every schedule gives the same results: degridded visibilities are
     bit-identical (a group's sum does not depend on how groups are split into
     work items or spread over wavefronts), PSF grids agree to 1e-6 of their
-    peak, dirty grids of random-sign visibilities and residual images to 1e-5
-    (float atomics add in another order; the grid correction amplifies that
-    rounding at the image corners);
+    peak, dirty grids of random-sign visibilities to 1e-5 and their residual
+    images to 3e-5 of peak (float atomics add in another order; the data are
+    pure noise, so the residual image is a small difference of large grid
+    sums, and the grid correction amplifies the rounding at the image
+    corners);
   * schedules outside the kernel's limits are rejected.
 """
 import numpy as np
@@ -98,7 +100,7 @@ def test_every_schedule_gives_the_same_results(gpu, threads, gpi):
     r_ref = ref.major_cycle(t(img), nsum, peaks).cpu().numpy()
     r_vs = vs.major_cycle(t(img), nsum, peaks).cpu().numpy()
     print(f"schedule ({threads}, {gpi}): residual {rel(r_vs, r_ref):.2e} of peak")
-    assert rel(r_vs, r_ref) < 1e-5
+    assert rel(r_vs, r_ref) < 3e-5
     # fixed schedules are not autotuned
     assert vs.schedule()["timed"] == [] and vs.schedule()["threads"] == threads
 
