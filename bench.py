@@ -429,6 +429,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-minor", action="store_true", help="skip the Hogbom minor-cycle timing")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--compare-hipfft", type=int, default=1,
                     help="also time the step with hipFFT 2-D plans (reported beside value)")
@@ -646,6 +647,35 @@ def main():
         log(f"plain FP64 per-channel major cycle: {t64 * 1e3:.2f} ms/step, {rate64:.1f} Mvis/s "
             f"(batched FP32 is {rate32 / rate64:.2f}x); residual error {res_err:.2e} of peak")
         del mgrid64, res64
+        torch.cuda.empty_cache()
+
+    if not args.no_minor:
+        # Hogbom minor cycle on every plane's residual image with its own PSF
+        # (run_hogbom_minor_cycle, lib.rs:59640-59885): 100 components per plane
+        # (niter 99, CASA's inclusive budget), gain 0.1, no thresholds
+        res_c = r32.clone()
+        model_c = torch.zeros_like(res_c)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        upd, _, stop = pkg.run_hogbom_minor_cycle(res_c, p32, model_c, 0.1, 99)
+        torch.cuda.synchronize()
+        tm = time.perf_counter() - t0
+        mc = {"planes": n_chan, "components_per_plane": int(upd.max()), "ms": tm * 1e3,
+              "us_per_component_all_planes": tm * 1e6 / max(int(upd.max()), 1)}
+        if not args.no_cpu:
+            import oracle
+            t0 = time.perf_counter()
+            o_r, o_m, o_upd, _ = oracle.hogbom(r32[0].cpu().numpy(), p32[0].cpu().numpy(),
+                                               np.zeros((nx, nx), np.float32), 0.1, 100)
+            tc = time.perf_counter() - t0
+            same = bool(np.array_equal(o_r[0].view(np.uint32), res_c[0].cpu().numpy().view(np.uint32))
+                        and np.array_equal(o_m[0].view(np.uint32), model_c[0].cpu().numpy().view(np.uint32)))
+            mc.update(cpu_oracle_s_one_plane=tc, cpu_one_thread=True, plane0_bit_identical_to_oracle=same)
+        result_extra["minor_cycle"] = mc
+        log(f"Hogbom minor cycle: {int(upd.max())} components on each of {n_chan} planes in {tm * 1e3:.1f} ms"
+            + (f"; oracle one plane {mc['cpu_oracle_s_one_plane']:.2f} s, plane 0 bit-identical: "
+               f"{mc['plane0_bit_identical_to_oracle']}" if "cpu_oracle_s_one_plane" in mc else ""))
+        del res_c, model_c
         torch.cuda.empty_cache()
 
     # Roofline of the dominant kernel, k_binned<RESIDUAL, W> (degrid + grid of

@@ -7,14 +7,14 @@ It has no CPU fallback: without the built library or a HIP device, every
 entry point raises.
 """
 from . import abi
-from .abi import (AW_INVALID_NORMALIZATION, AW_KERNEL_INDEX_OUTSIDE_CELL, AW_NO_CELL,
+from .abi import (CLEAN_BUDGET, CLEAN_CYCLE, CLEAN_GLOBAL, CLEAN_NO_PIXELS, CLEAN_NSIGMA, AW_INVALID_NORMALIZATION, AW_KERNEL_INDEX_OUTSIDE_CELL, AW_NO_CELL,
                   AW_NON_FINITE_COORDINATE, AW_OK, AW_OUTSIDE_GRID, CYCLE_AUTO, CYCLE_FUSED,
                   CYCLE_HIPFFT, F32, F64, MODE_DIRTY, MODE_PSF, MODE_RESIDUAL, WEIGHT_BRIGGS,
                   WEIGHT_UNIFORM, ImagingError, InvalidRequest, NoUsableSamples, Normalization,
                   Unsupported)
 from .imaging import (Context, RowChannelBatch, StandardGridder, VisibilityBatch, VisibilitySet,
                       airy_table, default_context, mosaic_phase_gradient, phase_rotation,
-                      rotate_rows)
+                      rotate_rows, run_hogbom_minor_cycle)
 
 __all__ = [
     "abi", "CYCLE_AUTO", "CYCLE_FUSED", "CYCLE_HIPFFT", "WEIGHT_BRIGGS", "WEIGHT_UNIFORM", "F32", "F64", "MODE_DIRTY", "MODE_PSF", "MODE_RESIDUAL", "ImagingError",
@@ -22,5 +22,6 @@ __all__ = [
     "RowChannelBatch", "StandardGridder", "VisibilityBatch", "VisibilitySet", "default_context",
     "phase_rotation", "rotate_rows", "airy_table", "mosaic_phase_gradient",
     "AW_OK", "AW_NON_FINITE_COORDINATE", "AW_OUTSIDE_GRID", "AW_KERNEL_INDEX_OUTSIDE_CELL",
-    "AW_INVALID_NORMALIZATION", "AW_NO_CELL",
+    "AW_INVALID_NORMALIZATION", "AW_NO_CELL", "run_hogbom_minor_cycle", "CLEAN_BUDGET", "CLEAN_GLOBAL",
+    "CLEAN_NSIGMA", "CLEAN_CYCLE", "CLEAN_NO_PIXELS",
 ]

@@ -29,6 +29,13 @@ CYCLE_AUTO = -1
 CYCLE_HIPFFT = 0
 CYCLE_FUSED = 1
 
+# chip_hogbom stop codes (CleanStopReason order + 1; 0 = budget used up)
+CLEAN_BUDGET = 0
+CLEAN_GLOBAL = 1
+CLEAN_NSIGMA = 2
+CLEAN_CYCLE = 3
+CLEAN_NO_PIXELS = 6
+
 F32 = 0
 F64 = 1
 
@@ -56,7 +63,7 @@ EXPORTS = (
     "chip_grid_plain", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
     "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_aw_cells", "chip_aw_plan",
-    "chip_airy_table",
+    "chip_airy_table", "chip_hogbom",
 )
 
 
@@ -143,6 +150,10 @@ class CfCell(ctypes.Structure):
     ]
 
 
+class HogbomParams(ctypes.Structure):
+    _fields_ = [("gain", ctypes.c_float), ("threshold", ctypes.c_float), ("budget", ctypes.c_uint32)]
+
+
 class AiryParams(ctypes.Structure):
     _fields_ = [
         ("dish_m", ctypes.c_double), ("blockage_m", ctypes.c_double),
@@ -202,6 +213,8 @@ def load():
         "chip_vset_cycle_strategy": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
         "chip_vset_cycle_graph": (ctypes.c_int, [_P, ctypes.c_int, _P, _PP]),
         "chip_vset_schedule": (ctypes.c_int, [_P, _U32, _U32, _PP]),
+        "chip_hogbom": (ctypes.c_int, [_P, ctypes.POINTER(HogbomParams), _U64, _U64, _U64, _P, _U64, _U64, _P, _P,
+                                       _P, _P, _P, _P, _P, _PP]),
         "chip_vset_schedule_info": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
         "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
         "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),

@@ -508,4 +508,16 @@ int chip_grid_to_model_adjoint(chip_gridder *g, uint64_t batch, float *grid_dev,
     });
 }
 
+int chip_hogbom(chip_ctx *ctx, const chip_hogbom_params *params, uint64_t n_planes, uint64_t nx, uint64_t ny,
+                float *residual_dev, uint64_t kx, uint64_t ky, const float *psf_dev, float *model_dev,
+                const uint8_t *mask_dev, const float *cycle_threshold, const float *nsigma_threshold,
+                uint32_t *updates_host, int32_t *stop_host, char **err) {
+    return guarded(err, [&] {
+        if (!ctx || !params || !updates_host || !stop_host) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        hogbom(*ctx, *params, n_planes, nx, ny, residual_dev, kx, ky, psf_dev, model_dev, mask_dev, cycle_threshold,
+               nsigma_threshold, updates_host, stop_host);
+    });
+}
+
 }  // extern "C"

@@ -245,6 +245,9 @@ void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *
 void density_weights(chip_gridder &g, const chip_samples &cs, uint32_t mode, float robust, uint32_t per_plane,
                      float *weight_out, float *f2_host, float *density_out);
 void phase_rotation(const double src[2], const double tgt[2], double uvrot[9], double phrot[3]);
+void hogbom(chip_ctx &ctx, const chip_hogbom_params &hp, uint64_t n_planes, uint64_t nx, uint64_t ny,
+            float *residual, uint64_t kx, uint64_t ky, const float *psf, float *model, const uint8_t *mask,
+            const float *cycle_threshold, const float *nsigma_threshold, uint32_t *updates, int32_t *stop);
 void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, const float *pixels_dev,
                uint64_t n_pixels, uint32_t sampling, uint32_t n_pointings, const double *gradients,
                double w_increment);

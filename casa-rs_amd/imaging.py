@@ -125,6 +125,45 @@ def mosaic_phase_gradient(nx, ny, pixel_offset, sampling):
     return (-pixel_offset[0] * tau / (nx * sampling), -pixel_offset[1] * tau / (ny * sampling))
 
 
+def run_hogbom_minor_cycle(residual, psf, model, gain, niter, threshold=0.0, cycle_threshold=0.0,
+                           nsigma_threshold=0.0, mask=None, inclusive=True, ctx: "Context | None" = None):
+    """One Hogbom minor cycle on every plane, in place on the device
+    (run_hogbom_minor_cycle_cpu_with_progress, lib.rs:59748-59885).
+
+    residual, model: (planes, nx, ny) float32 device tensors, updated in
+    place; psf: (planes, kx, ky) or (kx, ky) float32 (centre at kx/2, ky/2);
+    cycle_threshold / nsigma_threshold: scalars or one per plane; mask:
+    (planes, nx, ny) or (nx, ny) bool (True = cleanable) or None; inclusive:
+    CASA's inclusive budget (niter + 1 components, hogbom_component_budget,
+    lib.rs:64541-64551).  Returns (components taken per plane, reported
+    iterations = min(components, niter), stop codes CLEAN_*)."""
+    ctx = ctx or default_context()
+    dev = ctx.torch_device
+    for name, t in (("residual", residual), ("model", model)):
+        if t.dtype != torch.float32 or t.device != dev or not t.is_contiguous():
+            raise abi.InvalidRequest(f"{name} must be a contiguous float32 device tensor (updated in place)")
+    res, mod = (residual[None], model[None]) if residual.dim() == 2 else (residual, model)
+    planes, nx, ny = res.shape
+    if tuple(mod.shape) != (planes, nx, ny):
+        raise abi.InvalidRequest("model and residual shapes differ")
+    k = psf.to(device=dev, dtype=torch.float32)
+    k = (k[None] if k.dim() == 2 else k).expand(planes, -1, -1).contiguous()
+    mk = None
+    if mask is not None:
+        mk = torch.as_tensor(mask).to(device=dev, dtype=torch.uint8)
+        mk = (mk[None] if mk.dim() == 2 else mk).expand(planes, nx, ny).contiguous()
+    cyc = np.ascontiguousarray(np.broadcast_to(np.asarray(cycle_threshold, np.float32), (planes,)))
+    nsg = np.ascontiguousarray(np.broadcast_to(np.asarray(nsigma_threshold, np.float32), (planes,)))
+    budget = int(niter) + (1 if inclusive else 0)
+    params = abi.HogbomParams(float(gain), float(threshold), budget)
+    upd = np.zeros(planes, np.uint32)
+    stop = np.zeros(planes, np.int32)
+    abi.call("chip_hogbom", ctx.handle, ctypes.byref(params), planes, nx, ny, _ptr(res), k.shape[1], k.shape[2],
+             _ptr(k), _ptr(mod), _ptr(mk), cyc.ctypes.data_as(ctypes.c_void_p), nsg.ctypes.data_as(ctypes.c_void_p),
+             upd.ctypes.data_as(ctypes.c_void_p), stop.ctypes.data_as(ctypes.c_void_p))
+    return upd, np.minimum(upd, int(niter)), stop
+
+
 def default_context() -> Context:
     global _default_ctx
     if _default_ctx is None:

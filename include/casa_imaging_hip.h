@@ -54,6 +54,14 @@ extern "C" {
 #define CHIP_CYCLE_HIPFFT 0
 #define CHIP_CYCLE_FUSED 1
 
+/* Stop codes of chip_hogbom: CleanStopReason (types.rs:2215-2230) in
+ * declaration order plus one; 0 = the component budget was used up. */
+#define CHIP_CLEAN_BUDGET 0
+#define CHIP_CLEAN_GLOBAL 1    /* GlobalThresholdReached */
+#define CHIP_CLEAN_NSIGMA 2    /* NsigmaThresholdReached */
+#define CHIP_CLEAN_CYCLE 3     /* CycleThresholdReached */
+#define CHIP_CLEAN_NO_PIXELS 6 /* NoCleanablePixels */
+
 #define CHIP_F32 0 /* binned, LDS-tiled, FP32 complex MACs, complex64 grids */
 #define CHIP_F64 1 /* plain per-channel launch, FP64 MACs and atomics, complex128 grids */
 
@@ -435,6 +443,35 @@ int chip_model_to_grid(chip_gridder *g, uint32_t precision, uint64_t batch, cons
  * overwritten.  image_out_dev: batch x nx x ny float. */
 int chip_grid_to_model_adjoint(chip_gridder *g, uint64_t batch, float *grid_dev,
                                float *image_out_dev, char **err);
+
+/* ---- minor cycle --------------------------------------------------------- */
+/* Hogbom minor-cycle controls (CleanConfig, types.rs): loop gain, global
+ * threshold (Jy/beam) and component budget (hogbom_component_budget,
+ * lib.rs:64541-64551: niter, plus one in CASA's inclusive mode). */
+typedef struct {
+    float gain;
+    float threshold;
+    uint32_t budget;
+} chip_hogbom_params;
+
+/* One Hogbom minor cycle on every plane of a cube, on the device
+ * (run_hogbom_minor_cycle_cpu_with_progress, lib.rs:59748-59885): per plane,
+ * until the budget or a stop rule (minor_cycle_stop_reason, lib.rs:74653-
+ * 74677, with the plane's cycle and nsigma thresholds): peak = largest |value|
+ * over unmasked pixels, ties to the smallest y*nx + x, the first unmasked
+ * pixel in x-major order kept when it is NaN (lib.rs:74064-74115); component
+ * = gain * peak; model[peak] += component; residual -= component * psf shifted
+ * to the peak with centre (kx/2, ky/2) over the overlap (lib.rs:74805-74868).
+ * Planes are bit-identical to the reference's CPU loop.  residual_dev,
+ * model_dev: n_planes x nx x ny float (updated in place); psf_dev: n_planes x
+ * kx x ky float; mask_dev: n_planes x nx x ny bytes (nonzero = cleanable) or
+ * NULL; cycle_threshold, nsigma_threshold: host arrays of n_planes;
+ * updates_host, stop_host (CHIP_CLEAN_*): host arrays of n_planes.  One
+ * workgroup per plane; synchronous. */
+int chip_hogbom(chip_ctx *ctx, const chip_hogbom_params *params, uint64_t n_planes, uint64_t nx, uint64_t ny,
+                float *residual_dev, uint64_t kx, uint64_t ky, const float *psf_dev, float *model_dev,
+                const uint8_t *mask_dev, const float *cycle_threshold, const float *nsigma_threshold,
+                uint32_t *updates_host, int32_t *stop_host, char **err);
 
 #ifdef __cplusplus
 }

@@ -182,6 +182,11 @@ def _declare(L):
     L.co_airy_radius.restype = ctypes.c_float
     L.co_airy_radius.argtypes = [_D, _D, _D, _D]
     L.co_airy_offsets_many.restype = None
+    L.co_minor_cycle_stop.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.co_minor_cycle_stop.restype = ctypes.c_int
+    L.co_hogbom.argtypes = [_U64, _U64, _U64, _P, _U64, _U64, _P, _P, _P, ctypes.c_float, ctypes.c_float, _P, _P,
+                            _U64, _P, _P]
+    L.co_hogbom.restype = None
     L.co_airy_offsets_many.argtypes = [_U64, _P, _P, _D, _D, _D, _P, _P]
 
 
@@ -937,3 +942,38 @@ def airy_offsets(l, m, freq, dish, blockage, table=None):
     lib().co_airy_offsets_many(l.size, _ptr(l), _ptr(np.ascontiguousarray(m)), freq, dish, blockage,
                                _ptr(np.ascontiguousarray(table, np.float32)), _ptr(out))
     return out
+
+
+# --------------------------------------------------------------------------
+# Hogbom minor cycle (lib.rs:59748-59885)
+# --------------------------------------------------------------------------
+
+STOP_NONE, STOP_GLOBAL, STOP_NSIGMA, STOP_CYCLE, STOP_NO_PIXELS = 0, 1, 2, 3, 6
+
+
+def minor_cycle_stop(peak_abs, threshold, cycle_threshold, nsigma_threshold) -> int:
+    """minor_cycle_stop_reason (lib.rs:74653-74677) as a stop code."""
+    return lib().co_minor_cycle_stop(float(peak_abs), float(threshold), float(cycle_threshold),
+                                     float(nsigma_threshold))
+
+
+def hogbom(residual, psf, model, gain, budget, threshold=0.0, cycle_threshold=0.0, nsigma_threshold=0.0,
+           mask=None):
+    """One Hogbom minor cycle on every plane of ``residual`` (planes, nx, ny),
+    in place on float32 copies.  Returns (residual, model, updates, stop codes)."""
+    r = np.array(residual, dtype=np.float32, copy=True, order="C")
+    m = np.array(model, dtype=np.float32, copy=True, order="C")
+    if r.ndim == 2:
+        r, m = r[None], m[None]
+    k = _f32(psf)
+    if k.ndim == 2:
+        k = k[None]
+    planes, nx, ny = r.shape
+    mk = None if mask is None else np.ascontiguousarray(np.broadcast_to(mask, r.shape), dtype=np.uint8)
+    ct = _f32(np.broadcast_to(np.asarray(cycle_threshold, np.float32), (planes,)))
+    ns = _f32(np.broadcast_to(np.asarray(nsigma_threshold, np.float32), (planes,)))
+    upd = np.zeros(planes, np.uint64)
+    stop = np.zeros(planes, np.int32)
+    lib().co_hogbom(planes, nx, ny, _ptr(r), k.shape[1], k.shape[2], _ptr(k), _ptr(m), _ptr(mk), float(gain),
+                    float(threshold), _ptr(ct), _ptr(ns), int(budget), _ptr(upd), _ptr(stop))
+    return r, m, upd, stop

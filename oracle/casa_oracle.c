@@ -1262,3 +1262,112 @@ void co_airy_offsets_many(uint64_t n, const double *l, const double *m, double f
                           const float *table, float *out) {
     for (uint64_t i = 0; i < n; ++i) out[i] = co_airy_offsets(l[i], m[i], freq, dish, blockage, table);
 }
+
+/* ------------------------------------------------------------------------- */
+/* Hogbom minor cycle (run_hogbom_minor_cycle_cpu_with_progress,             */
+/* lib.rs:59748-59885), one image plane at a time.                            */
+/* ------------------------------------------------------------------------- */
+
+/* Stop codes: CleanStopReason (types.rs:2215-2230) in declaration order,
+ * plus 1; 0 = the component budget was used up. */
+enum { CO_STOP_NONE = 0, CO_STOP_GLOBAL = 1, CO_STOP_NSIGMA = 2, CO_STOP_CYCLE = 3, CO_STOP_NO_PIXELS = 6 };
+
+/* threshold_reached_with_tolerance, lib.rs:74628-74635 */
+static int co_threshold_reached_with_tolerance(float peak_abs, float threshold) {
+    if (threshold <= 0.0f) return peak_abs <= threshold;
+    const float tol = fmaxf(0.01f * threshold, 2.0e-8f);
+    return peak_abs <= threshold || fabsf(peak_abs - threshold) <= tol;
+}
+
+/* minor_cycle_stop_reason, lib.rs:74653-74677 */
+int co_minor_cycle_stop(float peak_abs, float threshold, float cycle_threshold, float nsigma_threshold) {
+    const int floor_active = threshold > 0.0f && (0.01f * threshold) < 2.0e-8f && cycle_threshold <= threshold;
+    if (peak_abs < threshold || (floor_active && co_threshold_reached_with_tolerance(peak_abs, threshold)))
+        return CO_STOP_GLOBAL;
+    if (cycle_threshold >= nsigma_threshold) return peak_abs < cycle_threshold ? CO_STOP_CYCLE : CO_STOP_NONE;
+    if (nsigma_threshold > threshold) return peak_abs < nsigma_threshold ? CO_STOP_NSIGMA : CO_STOP_NONE;
+    return CO_STOP_NONE;
+}
+
+/* peak_location_masked_in_bounds_zero_tolerance over the whole plane,
+ * lib.rs:74064-74115: visited x-major; the first visited pixel seeds the
+ * best, a later one replaces it when |v| is larger, or equal with a smaller
+ * y-major key y*nx + x.  Returns 0 when no pixel is unmasked. */
+static int co_hclean_peak(const float *img, const uint8_t *mask, uint64_t nx, uint64_t ny, uint64_t *px,
+                          uint64_t *py, float *value) {
+    int found = 0;
+    float best = 0.0f;
+    uint64_t bkey = 0;
+    for (uint64_t x = 0; x < nx; ++x) {
+        for (uint64_t y = 0; y < ny; ++y) {
+            if (mask && !mask[x * ny + y]) continue;
+            const float v = img[x * ny + y];
+            const uint64_t key = y * nx + x;
+            if (!found || fabsf(v) > fabsf(best) || (fabsf(v) == fabsf(best) && key < bkey)) {
+                found = 1;
+                best = v;
+                bkey = key;
+                *px = x;
+                *py = y;
+            }
+        }
+    }
+    *value = best;
+    return found;
+}
+
+/* subtract_shifted_kernel_in_window over the full image, lib.rs:74805-74868:
+ * image[x, y] -= scale * kernel[x - px + kx/2, y - py + ky/2] where both
+ * overlap (one f32 product, then the subtraction). */
+static void co_subtract_shifted_psf(float *img, uint64_t nx, uint64_t ny, const float *psf, uint64_t kx,
+                                    uint64_t ky, uint64_t px, uint64_t py, float scale) {
+    const int64_t xmin = (int64_t)px - (int64_t)(kx / 2), ymin = (int64_t)py - (int64_t)(ky / 2);
+    const int64_t xmax = xmin + (int64_t)kx - 1, ymax = ymin + (int64_t)ky - 1;
+    if (nx == 0 || ny == 0 || xmax < 0 || ymax < 0 || xmin > (int64_t)nx - 1 || ymin > (int64_t)ny - 1) return;
+    const int64_t x0 = xmin > 0 ? xmin : 0, y0 = ymin > 0 ? ymin : 0;
+    const int64_t x1 = xmax < (int64_t)nx - 1 ? xmax : (int64_t)nx - 1;
+    const int64_t y1 = ymax < (int64_t)ny - 1 ? ymax : (int64_t)ny - 1;
+    for (int64_t x = x0; x <= x1; ++x)
+        for (int64_t y = y0; y <= y1; ++y) {
+            const float p = scale * psf[(uint64_t)(x - xmin) * ky + (uint64_t)(y - ymin)];
+            img[(uint64_t)x * ny + (uint64_t)y] = img[(uint64_t)x * ny + (uint64_t)y] - p;
+        }
+}
+
+/* One Hogbom minor cycle on each of n_planes planes (residual, model: nx*ny
+ * each; psf: kx*ky each; mask: nx*ny bytes per plane or NULL), lib.rs:59761-
+ * 59810: up to `budget` components (hogbom_component_budget, lib.rs:64541-
+ * 64551: niter, plus one in CASA's inclusive mode).  Writes the components
+ * taken and the stop code per plane. */
+void co_hogbom(uint64_t n_planes, uint64_t nx, uint64_t ny, float *residual, uint64_t kx, uint64_t ky,
+               const float *psf, float *model, const uint8_t *mask, float gain, float threshold,
+               const float *cycle_threshold, const float *nsigma_threshold, uint64_t budget, uint64_t *updates,
+               int32_t *stop) {
+    for (uint64_t c = 0; c < n_planes; ++c) {
+        float *r = residual + c * nx * ny;
+        float *m = model + c * nx * ny;
+        const float *k = psf + c * kx * ky;
+        const uint8_t *mk = mask ? mask + c * nx * ny : NULL;
+        uint64_t done = 0;
+        int32_t why = CO_STOP_NONE;
+        while (done < budget) {
+            uint64_t px = 0, py = 0;
+            float pv = 0.0f;
+            if (!co_hclean_peak(r, mk, nx, ny, &px, &py, &pv)) {
+                why = CO_STOP_NO_PIXELS;
+                break;
+            }
+            const int s = co_minor_cycle_stop(fabsf(pv), threshold, cycle_threshold[c], nsigma_threshold[c]);
+            if (s) {
+                why = s;
+                break;
+            }
+            const float comp = gain * pv;
+            m[px * ny + py] = m[px * ny + py] + comp;
+            co_subtract_shifted_psf(r, nx, ny, k, kx, ky, px, py, comp);
+            ++done;
+        }
+        updates[c] = done;
+        stop[c] = why;
+    }
+}
