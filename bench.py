@@ -577,13 +577,16 @@ def main():
     log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
         f"binned kernel {np.mean(kernel_ms):.3f} ms")
     sched = vs.schedule()
-    log(f"binned schedule: {sched['threads']} threads x {sched['groups_per_item']} groups per item "
-        f"(autotuned over {len(sched['timed'])} candidates: "
-        + ", ".join(f"{a}x{b} {ms:.3f} ms" for a, b, ms in sched["timed"]) + ")")
+    log(f"binned schedule: {sched['tile']}^2 uv tiles, groups by {('anchor', 'plan')[sched['order']]}, "
+        f"{sched['threads']} threads x {sched['groups_per_item']} groups per item (autotuned over "
+        f"{len(sched['timed'])} candidates tile/order/threads x groups: "
+        + ", ".join(f"{tl}/{od}/{a}x{b} {ms:.3f} ms" for tl, od, a, b, ms in sched["timed"]) + ")")
     result_extra.update(transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
                         transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
                         transform_fwd_inv_ms=info["transform_ms"],
-                        binned_schedule={"threads": sched["threads"], "groups_per_item": sched["groups_per_item"],
+                        binned_schedule={"tile": sched["tile"], "order": ("anchor", "plan")[sched["order"]],
+                                         "threads": sched["threads"],
+                                         "groups_per_item": sched["groups_per_item"],
                                          "candidates_timed": len(sched["timed"])})
     if info["strategy"] == pkg.CYCLE_FUSED:
         # the same step replayed from a captured HIP graph (one launch per cycle)

@@ -212,10 +212,10 @@ def load():
         "chip_vset_cycle_info": (ctypes.c_int, [_P, _P, _P, _P, _PP]),
         "chip_vset_cycle_strategy": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
         "chip_vset_cycle_graph": (ctypes.c_int, [_P, ctypes.c_int, _P, _PP]),
-        "chip_vset_schedule": (ctypes.c_int, [_P, _U32, _U32, _PP]),
+        "chip_vset_schedule": (ctypes.c_int, [_P, _U32, _U32, _U32, _U32, _PP]),
         "chip_hogbom": (ctypes.c_int, [_P, ctypes.POINTER(HogbomParams), _U64, _U64, _U64, _P, _U64, _U64, _P, _P,
                                        _P, _P, _P, _P, _P, _PP]),
-        "chip_vset_schedule_info": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
+        "chip_vset_schedule_info": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _PP]),
         "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
         "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),

@@ -16,7 +16,7 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
 void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], double *transform_ms);
 void vset_cycle_strategy(chip_vset &vs, int strategy);
 void vset_cycle_graph(chip_vset &vs, int enable);
-void vset_schedule(chip_vset &vs, uint32_t threads, uint32_t gpi);
+void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi);
 uint64_t vset_cycle_replays(const chip_vset &vs);
 void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev);
 }  // namespace chip
@@ -412,21 +412,24 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
     });
 }
 
-int chip_vset_schedule(chip_vset *vs, uint32_t threads, uint32_t groups_per_item, char **err) {
+int chip_vset_schedule(chip_vset *vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t groups_per_item,
+                       char **err) {
     return guarded(err, [&] {
         if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
         set_device(vs->g->ctx);
-        vset_schedule(*vs, threads, groups_per_item);
+        vset_schedule(*vs, tile, order, threads, groups_per_item);
     });
 }
 
-int chip_vset_schedule_info(const chip_vset *vs, uint32_t *threads, uint32_t *groups_per_item, uint32_t *n_timed,
-                            double *timed, char **err) {
+int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order, uint32_t *threads,
+                            uint32_t *groups_per_item, uint32_t *n_timed, double *timed, char **err) {
     return guarded(err, [&] {
         if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        if (tile) *tile = vs->tile;
+        if (order) *order = vs->order;
         if (threads) *threads = vs->threads;
         if (groups_per_item) *groups_per_item = vs->groups_per_item;
-        if (n_timed) *n_timed = static_cast<uint32_t>(vs->tune_ms.size() / 3);
+        if (n_timed) *n_timed = static_cast<uint32_t>(vs->tune_ms.size() / 5);
         if (timed) std::memcpy(timed, vs->tune_ms.data(), vs->tune_ms.size() * sizeof(double));
     });
 }

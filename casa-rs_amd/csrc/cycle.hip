@@ -18,8 +18,10 @@
 //            apodisation, Hermitian half-spectrum, crop and normalisation
 //            folded into the passes (default whenever the grid factors into
 //            2, 3, 5 and fits the LDS);
-//   hipFFT : rocFFT 2-D plans over [chan][x][y] with separate placement and
-//            crop kernels.
+//   hipFFT : rocFFT 2-D real-to-complex / complex-to-real plans over
+//            [chan][x][y] (the model is real; the residual image depends only
+//            on the Hermitian part of the residual grid) with separate
+//            placement and crop kernels.
 #include <algorithm>
 #include <cmath>
 
@@ -31,6 +33,7 @@ namespace chip {
 void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res, hipStream_t st,
                         bool timed);
 void vset_split_items(chip_vset &vs, uint32_t gpi);
+void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order);
 
 namespace {
 
@@ -44,9 +47,10 @@ struct CycleGeom {
     const float *sinc_x, *sinc_y;
 };
 
-// Model (chan, x, y) -> grid cell (blc + x, blc + y), times the apodisation
-// (gridder.rs:1554-1592) and (-1)^(gx + gy).  Grid: (ceil(ny/256), nx, chan).
-__global__ void k_cycle_place(CycleGeom cg, const float *__restrict__ model, float2 *grid) {
+// Model (chan, x, y) -> real grid cell (blc + x, blc + y), times the
+// apodisation (gridder.rs:1554-1592) and (-1)^(gx + gy).  Grid: (ceil(ny/256),
+// nx, chan).
+__global__ void k_cycle_place(CycleGeom cg, const float *__restrict__ model, float *grid) {
     const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t x = blockIdx.y;
     const uint32_t c = blockIdx.z;
@@ -56,12 +60,12 @@ __global__ void k_cycle_place(CycleGeom cg, const float *__restrict__ model, flo
     if (cg.wproject) f = f * cg.sinc_x[gxx] * cg.sinc_y[gyy];
     float v = model[(static_cast<uint64_t>(c) * cg.nx + x) * cg.ny + y] * f;
     if ((gxx + gyy) & 1) v = -v;
-    grid[c * cg.cstride + static_cast<uint64_t>(gxx) * cg.gy + gyy] = make_float2(v, 0.0f);
+    grid[c * cg.cstride + static_cast<uint64_t>(gxx) * cg.gy + gyy] = v;
 }
 
-// Inverse-transformed residual grid -> residual image (gridder.rs:1646-1657
+// Real inverse transform of 2H -> residual image (gridder.rs:1646-1657
 // standard, :1685-1715 W; normalisation lib.rs:68920-68932, :70960-70962).
-__global__ void k_cycle_crop(CycleGeom cg, const float2 *__restrict__ raw, const float *__restrict__ nsum,
+__global__ void k_cycle_crop(CycleGeom cg, const float *__restrict__ raw, const float *__restrict__ nsum,
                              const float *__restrict__ peak, float *out) {
     const uint32_t y = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t x = blockIdx.y;
@@ -72,7 +76,7 @@ __global__ void k_cycle_crop(CycleGeom cg, const float2 *__restrict__ raw, const
     if (sx >= cg.gx) sx -= cg.gx;
     if (sy >= cg.gy) sy -= cg.gy;
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(cg.gx) * cg.gy);
-    float re = raw[c * cg.cstride + static_cast<uint64_t>(sx) * cg.gy + sy].x * scale;
+    float re = (0.5f * raw[c * cg.cstride + static_cast<uint64_t>(sx) * cg.gy + sy]) * scale;
     if ((gxx + gyy + cg.base) & 1) re = -re;
     const float factor = cg.corr_x[gxx] * cg.corr_y[gyy];
     float v;
@@ -112,35 +116,61 @@ void zero_fill(void *p, uint64_t seg_bytes, uint64_t stride_bytes, uint64_t n_se
     CHIP_HIP(hipGetLastError());
 }
 
-// Schedule autotuner of the binned residual kernel: every (threads per
-// workgroup, groups per work item) candidate is launched twice on the major
-// cycle's own grids (the second launch is timed with the set's events) and
-// the fastest is kept.  The grids' contents do not matter: the residual grid
+// Schedule autotuner of the binned residual kernel: for both group orders
+// (anchor cell first, plan first) and every uv tile size that fits the LDS
+// with the set's halo (16, 32, 64 and the size the set was built with), the
+// stream is re-binned and every (threads per workgroup, groups per work item)
+// candidate is launched on the major cycle's own grids (timed with the set's
+// events); the fastest (tile, order, threads, groups) is kept, the earlier
+// candidate on a near tie.  make_gp builds the launch parameters for the
+// set's current tiling.  The grids' contents do not matter: the residual grid
 // is cleared again before the real pass.
-void autotune_schedule(chip_vset &vs, const GridParams &gp, const float2 *M, float *Rg, hipStream_t st) {
+template <class MakeGp>
+void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg, hipStream_t st) {
     static constexpr uint32_t kThreads[] = {256, 512};
     static constexpr uint32_t kGpi[] = {256, 512, 1024, 2048};
+    const chip_gridder &g = *vs.g;
+    std::vector<uint32_t> tiles{vs.tile};
+    for (uint32_t t : {16u, 32u, 64u})
+        if (t != vs.tile && binned_tile_lds(g, t, vs.halo) <= lds_limit(vs.device)) tiles.push_back(t);
     vs.tune_ms.clear();
     double best = -1.0;
-    uint32_t best_t = vs.threads, best_g = vs.groups_per_item;
-    for (uint32_t gpi : kGpi) {
-        vset_split_items(vs, gpi);
-        for (uint32_t t : kThreads) {
-            vs.threads = t;
-            float ms = 0.0f;
-            for (int rep = 0; rep < 2; ++rep) {
-                vset_residual_with(vs, gp, M, Rg, st, true);
-                CHIP_HIP(hipEventSynchronize(vs.ev_stop));
-                CHIP_HIP(hipEventElapsedTime(&ms, vs.ev_start, vs.ev_stop));
-            }
-            vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(t), static_cast<double>(gpi), ms});
-            if (best < 0.0 || ms < best) {
-                best = ms;
-                best_t = t;
-                best_g = gpi;
+    uint32_t best_tile = vs.tile, best_order = vs.order, best_t = vs.threads, best_g = vs.groups_per_item;
+    for (uint32_t order : {0u, 1u}) {
+        for (uint32_t tile : tiles) {
+            vset_rebin(vs, tile, order);
+            const GridParams gp = make_gp();
+            for (uint32_t gpi : kGpi) {
+                vset_split_items(vs, gpi);
+                for (uint32_t t : kThreads) {
+                    vs.threads = t;
+                    // a warm-up launch, then the fastest of up to three timed ones
+                    // (one when the first is far behind the best so far)
+                    float ms = -1.0f;
+                    for (int rep = 0; rep < 4; ++rep) {
+                        vset_residual_with(vs, gp, M, Rg, st, true);
+                        CHIP_HIP(hipEventSynchronize(vs.ev_stop));
+                        float m = 0.0f;
+                        CHIP_HIP(hipEventElapsedTime(&m, vs.ev_start, vs.ev_stop));
+                        if (rep == 0) continue;
+                        ms = ms < 0.0f ? m : std::min(ms, m);
+                        if (best > 0.0 && ms > 1.25 * best) break;
+                    }
+                    vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(tile), static_cast<double>(order),
+                                                         static_cast<double>(t), static_cast<double>(gpi), ms});
+                    // a later candidate must win by more than 1 % (timing noise)
+                    if (best < 0.0 || ms < 0.99 * best) {
+                        best = ms;
+                        best_tile = tile;
+                        best_order = order;
+                        best_t = t;
+                        best_g = gpi;
+                    }
+                }
             }
         }
     }
+    vset_rebin(vs, best_tile, best_order);
     vset_split_items(vs, best_g);
     vs.threads = best_t;
     vs.schedule_tuned = true;
@@ -175,13 +205,14 @@ struct CycleGraph {
 struct CycleWorkspace {
     int strategy = -1;  // CHIP_CYCLE_HIPFFT or CHIP_CYCLE_FUSED once set up
     DevBuf model_grid, res_grid;
-    hipfftHandle p2d = nullptr;
+    hipfftHandle r2c = nullptr, c2r = nullptr;
     xf::XPlan xp;
     double transform_ms = -1.0;
     CycleGraph graph;
     uint64_t replays = 0;  // major cycles run by graph replay
     ~CycleWorkspace() {
-        if (p2d) hipfftDestroy(p2d);
+        if (r2c) hipfftDestroy(r2c);
+        if (c2r) hipfftDestroy(c2r);
     }
 };
 
@@ -222,7 +253,9 @@ void cycle_setup(chip_vset &vs, CycleWorkspace &ws, const float *model, float *o
             fail(CHIP_ERR_INVALID,
                  "grid does not factor into 2, 3, 5 or exceeds one workgroup for fused transforms");
     }
-    const uint64_t elements = fused ? xf::grid_elements(ws.xp) : nc * g.gx * g.gy;
+    // hipFFT strategy: half-spectrum grids gx x (gy/2 + 1), whose memory also
+    // holds the real gx x gy planes of the placement and of the inverse
+    const uint64_t elements = fused ? xf::grid_elements(ws.xp) : nc * g.gx * (g.gy / 2 + 1);
     ws.model_grid.alloc(elements * sizeof(float2));
     ws.res_grid.alloc(elements * sizeof(float2));
     CHIP_HIP(hipMemsetAsync(ws.model_grid.p, 0, ws.model_grid.bytes, st));
@@ -234,12 +267,21 @@ void cycle_setup(chip_vset &vs, CycleWorkspace &ws, const float *model, float *o
     }
     if (g.gx * g.gy > 0x7fffffffull || nc > 0x7fffffffull)
         fail(CHIP_ERR_INVALID, "grid too large for a batched 2-D plan");
-    CHIP_FFT(hipfftCreate(&ws.p2d));
+    // real-to-complex forward of the real model, complex-to-real inverse of
+    // the Hermitian 2H: half the work of complex transforms
     int n[2] = {static_cast<int>(g.gx), static_cast<int>(g.gy)};
+    int real_em[2] = {static_cast<int>(g.gx), static_cast<int>(g.gy)};
+    int half_em[2] = {static_cast<int>(g.gx), static_cast<int>(g.gy / 2 + 1)};
+    const int real_dist = static_cast<int>(g.gx * g.gy), half_dist = static_cast<int>(g.gx * (g.gy / 2 + 1));
     size_t work = 0;
-    CHIP_FFT(hipfftMakePlanMany(ws.p2d, 2, n, nullptr, 1, static_cast<int>(g.gx * g.gy), nullptr, 1,
-                                static_cast<int>(g.gx * g.gy), HIPFFT_C2C, static_cast<int>(nc), &work));
-    CHIP_FFT(hipfftSetStream(ws.p2d, st));
+    CHIP_FFT(hipfftCreate(&ws.r2c));
+    CHIP_FFT(hipfftMakePlanMany(ws.r2c, 2, n, real_em, 1, real_dist, half_em, 1, half_dist, HIPFFT_R2C,
+                                static_cast<int>(nc), &work));
+    CHIP_FFT(hipfftCreate(&ws.c2r));
+    CHIP_FFT(hipfftMakePlanMany(ws.c2r, 2, n, half_em, 1, half_dist, real_em, 1, real_dist, HIPFFT_C2R,
+                                static_cast<int>(nc), &work));
+    CHIP_FFT(hipfftSetStream(ws.r2c, st));
+    CHIP_FFT(hipfftSetStream(ws.c2r, st));
     ws.strategy = CHIP_CYCLE_HIPFFT;
 }
 
@@ -270,7 +312,8 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
         vs.cycle = ws;
     }
     CycleWorkspace &ws = *vs.cycle;
-    if (ws.p2d) CHIP_FFT(hipfftSetStream(ws.p2d, st));  // the context's stream may have changed
+    if (ws.r2c) CHIP_FFT(hipfftSetStream(ws.r2c, st));  // the context's stream may have changed
+    if (ws.c2r) CHIP_FFT(hipfftSetStream(ws.c2r, st));
     if (vs.cycle_graph && ws.strategy != CHIP_CYCLE_FUSED)
         fail(CHIP_ERR_INVALID, "graph replay of the major cycle needs the fused transforms");
 
@@ -285,18 +328,30 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
 
     float2 *M = ws.model_grid.as<float2>();
     float2 *Rg = ws.res_grid.as<float2>();
-    GridParams gp = grid_params(g, vs.tile, vs.halo, vs.tiles_y);
-    gp.model_signed = 1;
+    const bool fused = ws.strategy == CHIP_CYCLE_FUSED;
+    auto make_gp = [&]() {
+        GridParams p = grid_params(g, vs.tile, vs.halo, vs.tiles_y);
+        p.model_signed = 1;
+        p.model_half = 1;     // both strategies store the model's columns y <= gy/2 only
+        p.acc_hermitian = 1;  // and accumulate the residual as 2H in those columns
+        if (fused) {
+            p.paired = 1;
+            p.cstride = ws.xp.xg.gstride;
+            p.bstride = g.gx * xf::kBlock;
+            p.lower = static_cast<uint64_t>(ws.xp.xg.nyb_h) * p.bstride;
+        } else {
+            p.xstride = g.gy / 2 + 1;  // hipFFT half spectra, x-major
+            p.cstride = g.gx * (g.gy / 2 + 1);
+        }
+        return p;
+    };
 
-    if (ws.strategy == CHIP_CYCLE_FUSED) {
-        gp.paired = 1;
-        gp.model_half = 1;     // the forward columns pass stores y <= gy/2 only
-        gp.acc_hermitian = 1;  // the inverse columns pass reads 2H, y <= gy/2
-        gp.cstride = ws.xp.xg.gstride;
-        gp.bstride = g.gx * xf::kBlock;
-        gp.lower = static_cast<uint64_t>(ws.xp.xg.nyb_h) * gp.bstride;
-        if (!vs.schedule_fixed && !vs.schedule_tuned && vs.n_items)
-            autotune_schedule(vs, gp, M, reinterpret_cast<float *>(Rg), st);
+    if (fused) {
+        if (!vs.schedule_fixed && !vs.schedule_tuned && vs.n_items) {
+            autotune_schedule(vs, make_gp, M, reinterpret_cast<float *>(Rg), st);
+            ws.graph.reset();  // the stream was re-binned
+        }
+        const GridParams gp = make_gp();
         // the half spectrum borrows the residual grid, the inverse rows the model grid
         auto enqueue = [&](hipStream_t s, bool timed) {
             xf::forward_model(ws.xp, model, Rg, M, s);
@@ -340,27 +395,28 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
         cg.blc_y = static_cast<uint32_t>(g.blc_y);
         cg.base = static_cast<uint32_t>(g.gx / 2 + g.gy / 2);
         cg.wproject = g.wproject ? 1u : 0u;
-        cg.cstride = g.gx * g.gy;
+        cg.cstride = g.gx * g.gy;  // real planes
         cg.corr_x = g.d_corr_x.as<float>();
         cg.corr_y = g.d_corr_y.as<float>();
         cg.sinc = g.wproject ? g.d_sinc.as<float>() : nullptr;
         cg.sinc_x = g.wproject ? g.d_sinc_x.as<float>() : nullptr;
         cg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
-        CHIP_HIP(hipMemsetAsync(M, 0, ws.model_grid.bytes, st));
-        CHIP_HIP(hipMemsetAsync(Rg, 0, ws.res_grid.bytes, st));
+        // real placement in the residual buffer -> R2C half spectrum in M;
+        // residual 2H accumulated in the residual buffer -> C2R real planes in M
+        float *Rr = reinterpret_cast<float *>(Rg), *Mr = reinterpret_cast<float *>(M);
+        const uint64_t real_bytes = nc * g.gx * g.gy * sizeof(float);
+        const uint64_t half_bytes = nc * g.gx * (g.gy / 2 + 1) * sizeof(float2);
+        CHIP_HIP(hipMemsetAsync(Rr, 0, real_bytes, st));
         const dim3 img_grid((cg.ny + 255) / 256, cg.nx, static_cast<unsigned>(nc));
-        hipLaunchKernelGGL(k_cycle_place, img_grid, dim3(256), 0, st, cg, model, M);
+        hipLaunchKernelGGL(k_cycle_place, img_grid, dim3(256), 0, st, cg, model, Rr);
         CHIP_HIP(hipGetLastError());
-        CHIP_FFT(hipfftExecC2C(ws.p2d, reinterpret_cast<hipfftComplex *>(M),
-                               reinterpret_cast<hipfftComplex *>(M), HIPFFT_FORWARD));
-        if (!vs.schedule_fixed && !vs.schedule_tuned && vs.n_items) {
-            autotune_schedule(vs, gp, M, reinterpret_cast<float *>(Rg), st);
-            CHIP_HIP(hipMemsetAsync(Rg, 0, ws.res_grid.bytes, st));
-        }
-        vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), st, true);
-        CHIP_FFT(hipfftExecC2C(ws.p2d, reinterpret_cast<hipfftComplex *>(Rg),
-                               reinterpret_cast<hipfftComplex *>(Rg), HIPFFT_BACKWARD));
-        hipLaunchKernelGGL(k_cycle_crop, img_grid, dim3(256), 0, st, cg, Rg, d_nsum, d_peak, out);
+        CHIP_FFT(hipfftExecR2C(ws.r2c, reinterpret_cast<hipfftReal *>(Rr), reinterpret_cast<hipfftComplex *>(M)));
+        if (!vs.schedule_fixed && !vs.schedule_tuned && vs.n_items)
+            autotune_schedule(vs, make_gp, M, reinterpret_cast<float *>(Rg), st);
+        CHIP_HIP(hipMemsetAsync(Rg, 0, half_bytes, st));
+        vset_residual_with(vs, make_gp(), M, reinterpret_cast<float *>(Rg), st, true);
+        CHIP_FFT(hipfftExecC2R(ws.c2r, reinterpret_cast<hipfftComplex *>(Rg), reinterpret_cast<hipfftReal *>(Mr)));
+        hipLaunchKernelGGL(k_cycle_crop, img_grid, dim3(256), 0, st, cg, Mr, d_nsum, d_peak, out);
         CHIP_HIP(hipGetLastError());
     }
     CHIP_HIP(hipStreamSynchronize(st));
@@ -375,14 +431,15 @@ void vset_cycle_graph(chip_vset &vs, int enable) {
 
 uint64_t vset_cycle_replays(const chip_vset &vs) { return vs.cycle ? vs.cycle->replays : 0; }
 
-void vset_schedule(chip_vset &vs, uint32_t threads, uint32_t gpi) {
-    if (threads == 0 && gpi == 0) {  // autotune again at the next major cycle
+void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi) {
+    if (tile == 0 && threads == 0 && gpi == 0) {  // autotune again at the next major cycle
         vs.schedule_fixed = false;
         vs.schedule_tuned = false;
     } else {
         if (threads == 0 || threads > 512 || threads % 64)
             fail(CHIP_ERR_INVALID, "threads per workgroup must be a multiple of 64 up to 512");
         if (gpi == 0 || gpi > (1u << 20)) fail(CHIP_ERR_INVALID, "groups per work item must be in 1 .. 2^20");
+        vset_rebin(vs, tile ? tile : vs.tile, order);
         vset_split_items(vs, gpi);
         vs.threads = threads;
         vs.schedule_fixed = true;

@@ -300,8 +300,9 @@ struct chip_vset {
     // per work item (chosen by the autotuner at the first major cycle unless
     // fixed by chip_vset_schedule)
     uint32_t threads = 512, groups_per_item = 1024;
+    uint32_t order = 0;  // groups of a bin by anchor cell (0) or by plan (1)
     bool schedule_fixed = false, schedule_tuned = false;
-    std::vector<double> tune_ms;  // timed candidates: (threads, groups per item, ms) triples
+    std::vector<double> tune_ms;  // timed candidates: (tile, order, threads, groups per item, ms)
     // stream (sorted by channel, tile)
     chip::DevBuf plan;    // uint2 per sample
     chip::DevBuf vis;     // float2

@@ -164,6 +164,9 @@ struct BinParams {
     uint32_t n_chan;
     uint32_t chunk_planes;  // W tables recomputed per chunk of planes: bins
                             // are (plane / chunk_planes, channel, tile); 0: one chunk
+    uint32_t plan_major;    // inside a bin, order groups by plan then anchor cell
+                            // (consecutive groups share their tap table); 0:
+                            // anchor cell then plan (consecutive footprints overlap)
     uint32_t cell_bits;     // log2(tile * tile)
     uint32_t plan_bits;
     uint32_t n_sel;         // selector values per axis
@@ -420,6 +423,178 @@ void vset_split_items(chip_vset &vs, uint32_t gpi) {
     CHIP_HIP(hipStreamSynchronize(st));
 }
 
+namespace {
+
+// Bins of a stream cut into tile x tile uv tiles.
+struct BinSetup {
+    BinParams b{};
+    bool merge = false;
+    int end_bit = 0;
+    uint32_t n_bins = 0;
+    uint64_t n_tiles = 0;
+};
+
+BinSetup bin_setup(chip_vset &vs, uint32_t tile) {
+    chip_gridder &g = *vs.g;
+    BinSetup s;
+    vs.tile = tile;
+    vs.tiles_x = static_cast<uint32_t>((g.gx + tile - 1) / tile);
+    vs.tiles_y = static_cast<uint32_t>((g.gy + tile - 1) / tile);
+    s.n_tiles = static_cast<uint64_t>(vs.tiles_x) * vs.tiles_y;
+    const uint64_t n_chunks = vs.wk_chunk ? (g.planes + vs.wk_chunk - 1) / vs.wk_chunk : 1;
+    const uint64_t n_bins64 = s.n_tiles * vs.n_chan * n_chunks;
+    if (n_bins64 >= (1ull << 31)) fail(CHIP_ERR_INVALID, "too many (plane chunk, channel, tile) bins");
+    vs.chunk_items.assign(n_chunks + 1, 0u);
+    s.n_bins = static_cast<uint32_t>(n_bins64);
+    BinParams &b = s.b;
+    b.tile = tile;
+    b.tiles_x = vs.tiles_x;
+    b.tiles_y = vs.tiles_y;
+    b.n_tiles = static_cast<uint32_t>(s.n_tiles);
+    b.n_bins = s.n_bins;
+    b.n_chan = static_cast<uint32_t>(vs.n_chan);
+    b.chunk_planes = vs.wk_chunk;
+    b.cell_bits = static_cast<uint32_t>(bits_for(static_cast<uint64_t>(tile) * tile - 1));
+    b.n_sel = g.wproject ? static_cast<uint32_t>(2 * g.norm_radius + 1) : g.os + 1;
+    const uint64_t max_code = g.wproject
+        ? (static_cast<uint64_t>(g.planes) * 2 * b.n_sel) * b.n_sel - 1
+        : static_cast<uint64_t>(b.n_sel) * b.n_sel - 1;
+    const int bin_bits = bits_for(s.n_bins);  // includes the invalid bin
+    const int plan_bits = bits_for(max_code);
+    s.merge = bin_bits + static_cast<int>(b.cell_bits) + plan_bits <= 64;
+    b.plan_bits = s.merge ? static_cast<uint32_t>(plan_bits) : 0u;
+    s.end_bit = bin_bits + static_cast<int>(b.cell_bits + b.plan_bits);
+    if (s.end_bit > 64) fail(CHIP_ERR_INVALID, "sort key exceeds 64 bits");
+    return s;
+}
+
+// Radix sort of (keys, vals) by the low end_bit bits into (keys_s, vals_s).
+void sort_pairs(DevBuf &keys, DevBuf &vals, DevBuf &keys_s, DevBuf &vals_s, uint64_t n, int end_bit, hipStream_t st) {
+    size_t temp_bytes = 0;
+    CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.as<uint64_t>(), keys_s.as<uint64_t>(),
+                                                vals.as<uint32_t>(), vals_s.as<uint32_t>(), static_cast<int>(n), 0,
+                                                end_bit, st));
+    DevBuf temp(std::max<size_t>(temp_bytes, 16));
+    CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, keys.as<uint64_t>(), keys_s.as<uint64_t>(),
+                                                vals.as<uint32_t>(), vals_s.as<uint32_t>(), static_cast<int>(n), 0,
+                                                end_bit, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+}
+
+// Groups of identical plans, their records and sums, and the work items, from
+// the stream sorted by keys_s.
+void build_groups(chip_vset &vs, const BinSetup &s, const DevBuf &keys_s) {
+    chip_gridder &g = *vs.g;
+    hipStream_t st = g.ctx->stream;
+    const uint64_t n_valid = vs.n_valid;
+    const uint32_t n_bins = s.n_bins;
+    DevBuf flags((n_valid + 1) * 4), gid((n_valid + 1) * 4);
+    CHIP_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, st));
+    hipLaunchKernelGGL(k_group_flags, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
+                       keys_s.as<uint64_t>(), n_valid, s.merge, flags.as<uint32_t>());
+    CHIP_HIP(hipGetLastError());
+    exclusive_sum(flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid + 1, st);
+    const uint32_t n_groups = read_u32(gid.as<uint32_t>() + n_valid, st);
+    vs.n_groups = n_groups;
+    DevBuf gstart((static_cast<uint64_t>(n_groups) + 1) * 4);
+    DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
+    CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
+    const uint32_t low_bits = s.b.cell_bits + s.b.plan_bits;
+    hipLaunchKernelGGL(k_group_starts, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
+                       keys_s.as<uint64_t>(), flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid,
+                       low_bits, gstart.as<uint32_t>(), bin_groups.as<uint32_t>());
+    CHIP_HIP(hipGetLastError());
+    const uint32_t nv32 = static_cast<uint32_t>(n_valid);
+    CHIP_HIP(hipMemcpyAsync(gstart.as<uint32_t>() + n_groups, &nv32, 4, hipMemcpyHostToDevice, st));
+    vs.grec.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(uint4));
+    if (n_groups) {
+        const uint32_t n_off = static_cast<uint32_t>(2 * g.norm_radius + 1);
+        if (g.wproject) {
+            if (g.max_support > 63) fail(CHIP_ERR_INVALID, "W support above 63 for the binned stream");
+            if (static_cast<uint64_t>(g.planes) * n_off * n_off >= (1ull << kRecPoBits))
+                fail(CHIP_ERR_INVALID, "too many W tap tables for the binned stream");
+        }
+        hipLaunchKernelGGL(k_group_records, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st,
+                           gstart.as<uint32_t>(), vs.plan.as<uint2>(), n_groups,
+                           g.wproject ? g.d_supports.as<int32_t>() : nullptr, n_off,
+                           static_cast<int32_t>(g.norm_radius), vs.grec.as<uint4>());
+        CHIP_HIP(hipGetLastError());
+    }
+    vs.gsum_d.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));
+    vs.gsum_r.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));
+    if (n_groups) {
+        hipLaunchKernelGGL(k_group_sums, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st, vs.grec.as<uint4>(),
+                           vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(), n_groups,
+                           vs.gsum_d.as<float4>(), vs.gsum_r.as<float4>());
+        CHIP_HIP(hipGetLastError());
+    }
+    CHIP_HIP(hipStreamSynchronize(st));
+    gstart.release();
+    flags.release();
+    gid.release();
+
+    // work items
+    vs.bin_first.alloc((static_cast<uint64_t>(n_bins) + 1) * 4);
+    exclusive_sum(bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(), n_bins + 1, st);
+    vs.bin_groups = std::move(bin_groups);
+    vs.n_bins = n_bins;
+    vs.n_tiles_all = static_cast<uint32_t>(s.n_tiles);
+    vset_split_items(vs, vs.groups_per_item);
+}
+
+// Sort keys of a stream already planned: (bin, anchor cell, plan) from the
+// packed plan and the sample's original index (channel = index mod n_chan),
+// the same key k_keys makes from the samples.
+__global__ void k_rekey(const uint2 *plan, const uint32_t *index, uint64_t n, BinParams b, uint32_t wproject,
+                        int32_t R, uint64_t *keys, uint32_t *vals) {
+    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint2 pk = plan[p];
+    const uint32_t chan = b.n_chan == 1 ? 0u : index[p] % b.n_chan;
+    const uint32_t ax = (pk.x & 0xffffu) + (wproject ? 0u : static_cast<uint32_t>(kSupport));
+    const uint32_t ay = (pk.x >> 16) + (wproject ? 0u : static_cast<uint32_t>(kSupport));
+    const uint32_t plane = (pk.y >> 16) & 0x7ffu;
+    const uint32_t tile = (ax / b.tile) * b.tiles_y + (ay / b.tile);
+    uint64_t bin = static_cast<uint64_t>(chan) * b.n_tiles + tile;
+    if (b.chunk_planes) bin += static_cast<uint64_t>(plane / b.chunk_planes) * b.n_chan * b.n_tiles;
+    const uint64_t cell = static_cast<uint64_t>((ax % b.tile) * b.tile + (ay % b.tile));
+    uint64_t key = bin << (b.cell_bits + b.plan_bits);
+    if (!b.plan_bits) {
+        key |= cell;
+    } else {
+        uint64_t code;
+        if (wproject) {
+            const uint64_t pc = static_cast<uint64_t>(plane) * 2 + ((pk.y & kConjBit) ? 1u : 0u);
+            const uint64_t sx = (pk.y & 0xffu) - 128u + static_cast<uint32_t>(R);
+            const uint64_t sy = ((pk.y >> 8) & 0xffu) - 128u + static_cast<uint32_t>(R);
+            code = (pc * b.n_sel + sx) * b.n_sel + sy;
+        } else {
+            code = static_cast<uint64_t>(pk.y & 0xffu) * b.n_sel + ((pk.y >> 8) & 0xffu);
+        }
+        key |= b.plan_major ? ((code << b.cell_bits) | cell) : ((cell << b.plan_bits) | code);
+    }
+    keys[p] = key;
+    vals[p] = static_cast<uint32_t>(p);
+}
+
+template <class T>
+__global__ void k_gather(const T *src, const uint32_t *perm, uint64_t n, T *dst) {
+    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[perm[p]];
+}
+
+template <class T>
+void gather(DevBuf &buf, const DevBuf &perm, uint64_t n, hipStream_t st) {
+    DevBuf out(std::max<uint64_t>(n, 1) * sizeof(T));
+    hipLaunchKernelGGL(k_gather<T>, dim3(blocks_for(n, 256)), dim3(256), 0, st, buf.as<T>(), perm.as<uint32_t>(), n,
+                       out.as<T>());
+    CHIP_HIP(hipGetLastError());
+    CHIP_HIP(hipStreamSynchronize(st));
+    buf = std::move(out);
+}
+
+}  // namespace
+
 // Builds the resident stream for a vset: FP64 plans, 64-bit radix sort by
 // (bin, anchor cell, plan), groups of identical plans, work items.
 void vset_build(chip_vset &vs, const chip_samples &cs) {
@@ -430,67 +605,28 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     vs.n_samples = n;
     vs.n_chan = src.n_chan;
     vs.halo = g.wproject ? g.max_support : static_cast<uint32_t>(kSupport);
+    vs.wk_chunk = (g.wproject && !g.aw) ? g.wk_chunk : 0u;
     // 32x32 tiles; smaller tiles when a large W support would not leave the
     // (tile + 2 halo)^2 LDS tile inside one workgroup's LDS
     const size_t lds = lds_limit(vs.device);
-    vs.tile = 32;
-    while (vs.tile > 8 && binned_tile_lds(g, vs.tile, vs.halo) > lds) vs.tile /= 2;
-    if (binned_tile_lds(g, vs.tile, vs.halo) > lds)
+    uint32_t tile = 32;
+    while (tile > 8 && binned_tile_lds(g, tile, vs.halo) > lds) tile /= 2;
+    if (binned_tile_lds(g, tile, vs.halo) > lds)
         fail(CHIP_ERR_INVALID, "W support " + std::to_string(vs.halo) + " too large for an LDS tile");
-    vs.tiles_x = static_cast<uint32_t>((g.gx + vs.tile - 1) / vs.tile);
-    vs.tiles_y = static_cast<uint32_t>((g.gy + vs.tile - 1) / vs.tile);
-    const uint64_t n_tiles = static_cast<uint64_t>(vs.tiles_x) * vs.tiles_y;
-    vs.wk_chunk = (g.wproject && !g.aw) ? g.wk_chunk : 0u;
-    const uint64_t n_chunks = vs.wk_chunk ? (g.planes + vs.wk_chunk - 1) / vs.wk_chunk : 1;
-    const uint64_t n_bins64 = n_tiles * src.n_chan * n_chunks;
-    if (n_bins64 >= (1ull << 31)) fail(CHIP_ERR_INVALID, "too many (plane chunk, channel, tile) bins");
-    vs.chunk_items.assign(n_chunks + 1, 0u);
-    const uint32_t n_bins = static_cast<uint32_t>(n_bins64);
+    const BinSetup bs = bin_setup(vs, tile);
     vs.sums.assign(src.n_chan, chip_sums{});
     CHIP_HIP(hipEventCreate(&vs.ev_start));
     CHIP_HIP(hipEventCreate(&vs.ev_stop));
     if (n == 0) return;
 
-    BinParams b{};
-    b.tile = vs.tile;
-    b.tiles_x = vs.tiles_x;
-    b.tiles_y = vs.tiles_y;
-    b.n_tiles = static_cast<uint32_t>(n_tiles);
-    b.n_bins = n_bins;
-    b.n_chan = static_cast<uint32_t>(src.n_chan);
-    b.chunk_planes = vs.wk_chunk;
-    b.cell_bits = static_cast<uint32_t>(bits_for(static_cast<uint64_t>(vs.tile) * vs.tile - 1));
-    b.n_sel = g.wproject ? static_cast<uint32_t>(2 * g.norm_radius + 1) : g.os + 1;
-    const uint64_t max_code = g.wproject
-        ? (static_cast<uint64_t>(g.planes) * 2 * b.n_sel) * b.n_sel - 1
-        : static_cast<uint64_t>(b.n_sel) * b.n_sel - 1;
-    const int bin_bits = bits_for(n_bins);  // includes the invalid bin
-    const int plan_bits = bits_for(max_code);
-    const bool merge = bin_bits + static_cast<int>(b.cell_bits) + plan_bits <= 64;
-    b.plan_bits = merge ? static_cast<uint32_t>(plan_bits) : 0u;
-    const int end_bit = bin_bits + static_cast<int>(b.cell_bits + b.plan_bits);
-    if (end_bit > 64) fail(CHIP_ERR_INVALID, "sort key exceeds 64 bits");
-
     DevBuf keys(n * 8), vals(n * 4), keys_s(n * 8), vals_s(n * 4);
     DevBuf d_nvalid(8);
     CHIP_HIP(hipMemsetAsync(d_nvalid.p, 0, 8, st));
     const PlanParams pp = plan_params(g, src);
-    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, pp, src, b,
+    hipLaunchKernelGGL(k_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, pp, src, bs.b,
                        keys.as<uint64_t>(), vals.as<uint32_t>(), d_nvalid.as<unsigned long long>());
     CHIP_HIP(hipGetLastError());
-    size_t temp_bytes = 0;
-    CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, temp_bytes, keys.as<uint64_t>(),
-                                                keys_s.as<uint64_t>(), vals.as<uint32_t>(),
-                                                vals_s.as<uint32_t>(), static_cast<int>(n), 0,
-                                                end_bit, st));
-    {
-        DevBuf temp(std::max<size_t>(temp_bytes, 16));
-        CHIP_HIP(hipcub::DeviceRadixSort::SortPairs(temp.p, temp_bytes, keys.as<uint64_t>(),
-                                                    keys_s.as<uint64_t>(), vals.as<uint32_t>(),
-                                                    vals_s.as<uint32_t>(), static_cast<int>(n), 0,
-                                                    end_bit, st));
-        CHIP_HIP(hipStreamSynchronize(st));
-    }
+    sort_pairs(keys, vals, keys_s, vals_s, n, bs.end_bit, st);
     keys.release();
     vals.release();
     unsigned long long nv_ll = 0;
@@ -528,61 +664,44 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     }
     vals_s.release();
     if (n_valid == 0) return;
+    build_groups(vs, bs, keys_s);
+}
 
-    // groups of identical plans
-    DevBuf flags((n_valid + 1) * 4), gid((n_valid + 1) * 4);
-    CHIP_HIP(hipMemsetAsync(flags.p, 0, flags.bytes, st));
-    hipLaunchKernelGGL(k_group_flags, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
-                       keys_s.as<uint64_t>(), n_valid, merge, flags.as<uint32_t>());
+// Cuts the stream into tile x tile uv tiles again, with the groups of a bin
+// ordered by anchor cell (order 0) or by plan (order 1): new keys from the
+// packed plans, a radix sort, the stream permuted, groups and work items
+// rebuilt.
+// Group membership does not change (identical plans share their anchor, so
+// they share a tile of any size); only the order of groups and their binning
+// do.
+void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order) {
+    chip_gridder &g = *vs.g;
+    if (order > 1) fail(CHIP_ERR_INVALID, "group order must be 0 (anchor cell first) or 1 (plan first)");
+    if (tile == vs.tile && order == vs.order) return;
+    if (tile < 8 || tile > 256 || (tile & (tile - 1)))
+        fail(CHIP_ERR_INVALID, "uv tile must be a power of two in 8 .. 256");
+    if (binned_tile_lds(g, tile, vs.halo) > lds_limit(vs.device))
+        fail(CHIP_ERR_INVALID, "uv tile of " + std::to_string(tile) + " does not fit the LDS with its halo");
+    hipStream_t st = g.ctx->stream;
+    BinSetup bs = bin_setup(vs, tile);
+    bs.b.plan_major = order;
+    vs.order = order;
+    const uint64_t n = vs.n_valid;
+    if (n == 0) return;
+    DevBuf keys(n * 8), vals(n * 4), keys_s(n * 8), vals_s(n * 4);
+    hipLaunchKernelGGL(k_rekey, dim3(blocks_for(n, 256)), dim3(256), 0, st, vs.plan.as<uint2>(),
+                       vs.index.as<uint32_t>(), n, bs.b, g.wproject ? 1u : 0u, static_cast<int32_t>(g.norm_radius),
+                       keys.as<uint64_t>(), vals.as<uint32_t>());
     CHIP_HIP(hipGetLastError());
-    exclusive_sum(flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid + 1, st);
-    const uint32_t n_groups = read_u32(gid.as<uint32_t>() + n_valid, st);
-    vs.n_groups = n_groups;
-    DevBuf gstart((static_cast<uint64_t>(n_groups) + 1) * 4);
-    DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
-    CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
-    const uint32_t low_bits = b.cell_bits + b.plan_bits;
-    hipLaunchKernelGGL(k_group_starts, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
-                       keys_s.as<uint64_t>(), flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid,
-                       low_bits, gstart.as<uint32_t>(), bin_groups.as<uint32_t>());
-    CHIP_HIP(hipGetLastError());
-    const uint32_t nv32 = static_cast<uint32_t>(n_valid);
-    CHIP_HIP(hipMemcpyAsync(gstart.as<uint32_t>() + n_groups, &nv32, 4, hipMemcpyHostToDevice, st));
-    vs.grec.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(uint4));
-    if (n_groups) {
-        const uint32_t n_off = static_cast<uint32_t>(2 * g.norm_radius + 1);
-        if (g.wproject) {
-            if (g.max_support > 63) fail(CHIP_ERR_INVALID, "W support above 63 for the binned stream");
-            if (static_cast<uint64_t>(g.planes) * n_off * n_off >= (1ull << kRecPoBits))
-                fail(CHIP_ERR_INVALID, "too many W tap tables for the binned stream");
-        }
-        hipLaunchKernelGGL(k_group_records, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st,
-                           gstart.as<uint32_t>(), vs.plan.as<uint2>(), n_groups,
-                           g.wproject ? g.d_supports.as<int32_t>() : nullptr, n_off,
-                           static_cast<int32_t>(g.norm_radius), vs.grec.as<uint4>());
-        CHIP_HIP(hipGetLastError());
-    }
-    vs.gsum_d.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));
-    vs.gsum_r.alloc(std::max<uint64_t>(n_groups, 1) * sizeof(float4));
-    if (n_groups) {
-        hipLaunchKernelGGL(k_group_sums, dim3(blocks_for(n_groups, 256)), dim3(256), 0, st, vs.grec.as<uint4>(),
-                           vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(), n_groups,
-                           vs.gsum_d.as<float4>(), vs.gsum_r.as<float4>());
-        CHIP_HIP(hipGetLastError());
-    }
-    CHIP_HIP(hipStreamSynchronize(st));
-    gstart.release();
-    keys_s.release();
-    flags.release();
-    gid.release();
-
-    // work items
-    vs.bin_first.alloc((static_cast<uint64_t>(n_bins) + 1) * 4);
-    exclusive_sum(bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(), n_bins + 1, st);
-    vs.bin_groups = std::move(bin_groups);
-    vs.n_bins = n_bins;
-    vs.n_tiles_all = static_cast<uint32_t>(n_tiles);
-    vset_split_items(vs, vs.groups_per_item);
+    sort_pairs(keys, vals, keys_s, vals_s, n, bs.end_bit, st);
+    keys.release();
+    vals.release();
+    gather<uint2>(vs.plan, vals_s, n, st);
+    gather<float2>(vs.vis, vals_s, n, st);
+    gather<float>(vs.wt, vals_s, n, st);
+    gather<uint32_t>(vs.index, vals_s, n, st);
+    vals_s.release();
+    build_groups(vs, bs, keys_s);
 }
 
 }  // namespace chip

@@ -552,18 +552,23 @@ class VisibilitySet:
         """abi.CYCLE_AUTO, abi.CYCLE_HIPFFT or abi.CYCLE_FUSED for later major cycles."""
         abi.call("chip_vset_cycle_strategy", self.handle, int(strategy))
 
-    def set_schedule(self, threads: int = 0, groups_per_item: int = 0):
-        """Fix the binned kernel's launch schedule; (0, 0) autotunes again at the
-        next major cycle."""
-        abi.call("chip_vset_schedule", self.handle, int(threads), int(groups_per_item))
+    def set_schedule(self, threads: int = 0, groups_per_item: int = 0, tile: int = 0, order: int = 0):
+        """Fix the binned kernel's launch schedule (tile 0 keeps the current uv
+        tile; order 0 groups by anchor cell, 1 by plan); all zeros autotune
+        again at the next major cycle."""
+        abi.call("chip_vset_schedule", self.handle, int(tile), int(order), int(threads), int(groups_per_item))
 
     def schedule(self):
-        """Schedule in use and the autotuner's timed candidates [(threads, groups, ms)]."""
-        t, gpi, n = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
-        buf = (ctypes.c_double * 96)()
-        abi.call("chip_vset_schedule_info", self.handle, ctypes.byref(t), ctypes.byref(gpi), ctypes.byref(n), buf)
-        timed = [(int(buf[3 * k]), int(buf[3 * k + 1]), float(buf[3 * k + 2])) for k in range(n.value)]
-        return {"threads": t.value, "groups_per_item": gpi.value, "timed": timed}
+        """Schedule in use and the autotuner's timed candidates
+        [(tile, order, threads, groups, ms)]."""
+        tile, order, t, gpi, n = (ctypes.c_uint32() for _ in range(5))
+        buf = (ctypes.c_double * 480)()
+        abi.call("chip_vset_schedule_info", self.handle, ctypes.byref(tile), ctypes.byref(order), ctypes.byref(t),
+                 ctypes.byref(gpi), ctypes.byref(n), buf)
+        timed = [(int(buf[5 * k]), int(buf[5 * k + 1]), int(buf[5 * k + 2]), int(buf[5 * k + 3]),
+                  float(buf[5 * k + 4])) for k in range(n.value)]
+        return {"tile": tile.value, "order": order.value, "threads": t.value, "groups_per_item": gpi.value,
+                "timed": timed}
 
     def set_cycle_graph(self, enable: bool) -> int:
         """Replay later major cycles from a captured HIP graph (fused strategy,

@@ -364,8 +364,9 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
                           const float *psf_peak, float *residual_out_dev, char **err);
 /* Transform strategy of later major cycles (frees the current workspace):
  *   CHIP_CYCLE_AUTO   fused when the grid allows it, else hipFFT (default);
- *   CHIP_CYCLE_HIPFFT hipFFT 2-D plans over [chan][x][y], separate placement
- *                     and crop kernels;
+ *   CHIP_CYCLE_HIPFFT hipFFT 2-D real-to-complex (model) and complex-to-real
+ *                     (Hermitian part of the residual grid) plans over
+ *                     [chan][x][y], separate placement and crop kernels;
  *   CHIP_CYCLE_FUSED  row / column FFT passes with placement, apodisation,
  *                     Hermitian half spectrum, crop and normalisation fused
  *                     in (grid lengths factoring into 2, 3, 5; CHIP_ERR_INVALID
@@ -373,21 +374,28 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
  * No reference counterpart (the reference's DirtyProductFftPolicy picks its
  * host FFT backend, lib.rs:1280-1310). */
 int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
-/* Launch schedule of the binned kernel: threads per workgroup (a multiple
- * of 64, at most 512) and groups of identical plans per work item (the
- * splitting of each (channel, uv tile) bin).  Unless fixed here, the first
- * major cycle times every candidate of {256, 512} threads x {256, 512, 1024,
- * 2048} groups on its own grids and keeps the fastest (default before that:
- * 512, 1024).  threads = groups_per_item = 0 asks for autotuning again at the
- * next major cycle.  Results do not depend on the schedule beyond the order
- * of float additions.  No reference counterpart (the Metal path dispatches
- * fixed-size chunks, lib.rs:72180-72290). */
-int chip_vset_schedule(chip_vset *vs, uint32_t threads, uint32_t groups_per_item, char **err);
+/* Launch schedule of the binned kernel: the uv tile edge (a power of two
+ * whose (tile + 2 halo)^2 LDS tile fits the device), the order of the groups
+ * inside a (channel, tile) bin (0: by anchor cell, so consecutive footprints
+ * overlap; 1: by plan, so consecutive groups share their tap table) -- both
+ * applied by re-binning the stream on the device, group membership unchanged
+ * -- threads per workgroup (a multiple of 64, at most 512) and groups of
+ * identical plans per work item (the splitting of each bin).  Unless fixed
+ * here, the first major cycle times every candidate of {anchor, plan order} x
+ * {the set's tile, 16, 32, 64 where they fit} x {256, 512} threads x {256,
+ * 512, 1024, 2048} groups on its own grids and keeps the fastest (default
+ * before that: 32 or the largest fitting tile, anchor order, 512, 1024).
+ * tile = 0 keeps the current tile; tile = threads = groups_per_item = 0 asks
+ * for autotuning again at the next major cycle.  Results do not depend on the
+ * schedule beyond the order of float additions.  No reference counterpart
+ * (the Metal path dispatches fixed-size chunks, lib.rs:72180-72290). */
+int chip_vset_schedule(chip_vset *vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t groups_per_item,
+                       char **err);
 /* Schedule in use and, after autotuning, the timed candidates: n_timed
- * triples (threads, groups per item, ms) in timed (at most 32 triples; may be
- * NULL).  Any pointer may be NULL. */
-int chip_vset_schedule_info(const chip_vset *vs, uint32_t *threads, uint32_t *groups_per_item, uint32_t *n_timed,
-                            double *timed, char **err);
+ * quintuples (tile, order, threads, groups per item, ms) in timed (at most 96
+ * quintuples; may be NULL).  Any pointer may be NULL. */
+int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order, uint32_t *threads,
+                            uint32_t *groups_per_item, uint32_t *n_timed, double *timed, char **err);
 /* Graph replay of later major cycles: enable = 1 captures the next major
  * cycle's kernels (forward passes, zero fill, binned residual kernel, inverse
  * passes) into a HIP graph once and replays it with one launch per call, on
