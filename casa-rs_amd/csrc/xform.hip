@@ -25,18 +25,24 @@
 //                 grid correction and normalisation are applied on the way
 //                 out.
 //
-// A workgroup holds 8 (columns) or 4 (row pairs) transforms in LDS, 128
-// threads per transform, and runs a mixed-radix (2, 3, 4, 5) Stockham
-// autosort FFT.
-// Stage with radix r after sub-transforms of length ns: butterfly j reads
-// elements j + t n/r, multiplies input t by exp(-+2 pi i k t / (ns r)) with
-// k = j mod ns, and writes output t to (j - k) r + k + t ns.  Every stage
-// reads its inputs into registers, synchronises, and writes its outputs
-// back, so one LDS buffer suffices.  Twiddles come from a table exp(-2 pi i
-// m / n) rounded once from FP64 and staged in LDS (for columns as two 64-way
-// split factors, so that two workgroups fit one CU).  Column tiles are
-// interleaved in LDS (element i of transform c at 8 i + c) to match the
-// paired grid layout; row tiles are row-major (c n + i).
+// A workgroup holds 8 (columns) or 4 (row pairs) transforms in LDS, 64
+// threads per transform, and runs a mixed-radix Stockham autosort FFT whose
+// stages pair two radices (25 = 5 x 5, 20, 16, 15, 12, 10, 9, 8, 6, or one
+// radix 2 .. 5 left over), so a length-1250 transform takes three passes
+// through LDS (25, 25, 2) instead of five.
+// Stage with radix R = r1 r2 after sub-transforms of length ns: butterfly j
+// reads elements j + t n/R, multiplies input t by exp(-+2 pi i k t / (ns R))
+// with k = j mod ns, takes their R-point DFT and writes output s to
+// (j - k) R + k + s ns.  The R-point DFT runs in registers as r1 DFTs of
+// r2 points, the constant twiddles exp(-+2 pi i t1 s2 / R), and r2 DFTs of
+// r1 points (t = t1 + r1 t2, s = s2 + r2 s1); the stage twiddle of input t
+// is applied as exp(-+2 pi i k r1 t2 / (ns R)) times exp(-+2 pi i k t1 /
+// (ns R)), both rounded once from FP64 and read from a per-stage table in
+// global memory (a few KB, cache resident), which keeps LDS for the data.
+// Every stage reads its inputs into registers, synchronises, and writes its
+// outputs back, so one LDS buffer suffices.  Column tiles are interleaved in
+// LDS (element i of transform c at 8 i + c) to match the paired grid layout;
+// row tiles are row-major (c n + i).
 #include <cmath>
 
 #include "device.hpp"
@@ -112,24 +118,74 @@ template <bool INV> __device__ __forceinline__ void bf5(float2 *a) {
 template <> __device__ __forceinline__ void butterfly<5, false>(float2 *a) { bf5<false>(a); }
 template <> __device__ __forceinline__ void butterfly<5, true>(float2 *a) { bf5<true>(a); }
 
-// Twiddle exp(-2 pi i m / n) from the LDS table: the full table, or (SPLIT)
-// hi + hi * d with hi = exp(-2 pi i 64 (m / 64) / n) and the small
-// d = exp(-2 pi i (m % 64) / n) - 1, both rounded once from FP64, which keeps
-// the product within about 0.7 ulp.
-template <bool SPLIT>
-__device__ __forceinline__ float2 twiddle(const float2 *tw, uint32_t m) {
-    if (!SPLIT) return tw[m];
-    const float2 hi = tw[64 + (m >> 6)], d = tw[m & 63];
-    return make_float2(fmaf(hi.x, d.x, fmaf(-hi.y, d.y, hi.x)), fmaf(hi.x, d.y, fmaf(hi.y, d.x, hi.y)));
+// exp(-2 pi i m / R) for m < R <= 32, evaluated at compile time (Taylor
+// series in FP64 on [-pi, pi], exact at multiples of a quarter turn).
+constexpr double kPi = 3.14159265358979323846;
+constexpr double taylor_sin(double x) {
+    double term = x, sum = x;
+    for (int i = 1; i < 30; ++i) {
+        term *= -x * x / ((2.0 * i) * (2.0 * i + 1.0));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double taylor_cos(double x) {
+    double term = 1.0, sum = 1.0;
+    for (int i = 1; i < 30; ++i) {
+        term *= -x * x / ((2.0 * i - 1.0) * (2.0 * i));
+        sum += term;
+    }
+    return sum;
+}
+struct Roots {
+    float re[32], im[32];
+};
+constexpr Roots make_roots(int R) {
+    Roots r{};
+    for (int m = 0; m < R; ++m) {
+        if ((4 * m) % R == 0) {  // 1, -i, -1, +i
+            const int q = 4 * m / R;
+            r.re[m] = q == 0 ? 1.0f : q == 2 ? -1.0f : 0.0f;
+            r.im[m] = q == 1 ? -1.0f : q == 3 ? 1.0f : 0.0f;
+            continue;
+        }
+        double a = -2.0 * kPi * m / R;
+        if (a < -kPi) a += 2.0 * kPi;
+        r.re[m] = static_cast<float>(taylor_cos(a));
+        r.im[m] = static_cast<float>(taylor_sin(a));
+    }
+    return r;
 }
 
-// One Stockham stage.  Thread (c, jt) handles butterflies j = jt + b * LANES
-// of transform c; MB bounds the butterflies per thread.
-template <int R, bool INV, int MB, bool ILV, int LANES, bool SPLIT>
-__device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n, uint32_t pitch,
-                                      uint32_t ns, uint32_t step, float inv_ns, uint32_t c, uint32_t jt) {
-    const uint32_t nb = n / R;
-    float2 v[MB][R];
+template <bool INV> __device__ __forceinline__ float2 conj_if(float2 w) { return INV ? make_float2(w.x, -w.y) : w; }
+
+// One Stockham stage of radix R1 * R2.  Thread (c, jt) handles butterflies
+// j = jt + b * LANES of transform c; MB bounds the butterflies per thread.
+// tw: this stage's twiddle table (read only when ns > 1); nb = n / R comes
+// from the stage table, so that addresses are not hoisted out of the stage
+// loop for every radix at once.
+template <int R1, int R2, bool INV, int MB, bool ILV, int LANES>
+__device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw, uint32_t nb, uint32_t pitch,
+                                      uint32_t ns, float inv_ns, uint32_t c, uint32_t jt) {
+    constexpr int R = R1 * R2, M = (R1 - 1) + (R2 - 1);
+    constexpr Roots kr = make_roots(R);
+    const bool twiddled = ns > 1;
+    float2 v[MB][R], w[MB][M];
+    uint32_t qv[MB], kv[MB];
+    // twiddle loads first: they travel while the LDS reads and the barrier run
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+        const uint32_t j = jt + b * LANES;
+        // j / ns, exact for n < 2^22: the fraction of (j + 1/2) / ns is at
+        // least 1/(2 ns) away from an integer, the float error far smaller
+        qv[b] = static_cast<uint32_t>((static_cast<float>(j) + 0.5f) * inv_ns);
+        kv[b] = j - qv[b] * ns;
+        if (twiddled && j < nb) {
+            const float2 *t = tw + static_cast<size_t>(kv[b]) * M;
+#pragma unroll
+            for (int m = 0; m < M; ++m) w[b][m] = t[m];
+        }
+    }
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
         const uint32_t j = jt + b * LANES;
@@ -145,25 +201,50 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n,
 #pragma unroll
     for (int b = 0; b < MB; ++b) {
         const uint32_t j = jt + b * LANES;
-        if (j < nb) {
-            // j / ns, exact for n < 2^22: the fraction of (j + 1/2) / ns is at
-            // least 1/(2 ns) away from an integer, the float error far smaller
-            const uint32_t q = static_cast<uint32_t>((static_cast<float>(j) + 0.5f) * inv_ns);
-            const uint32_t k = j - q * ns;
-            if (k != 0) {
+        if (j >= nb) continue;
+        if (twiddled) {
 #pragma unroll
-                for (int t = 1; t < R; ++t) {
-                    float2 w = twiddle<SPLIT>(tw, k * t * step);
-                    if (INV) w.y = -w.y;
-                    v[b][t] = cmul(v[b][t], w);
-                }
+            for (int t2 = 1; t2 < R2; ++t2) {
+                const float2 wt = conj_if<INV>(w[b][t2 - 1]);
+#pragma unroll
+                for (int t1 = 0; t1 < R1; ++t1) v[b][t1 + R1 * t2] = cmul(v[b][t1 + R1 * t2], wt);
             }
-            butterfly<R, INV>(v[b]);
-            const uint32_t base = q * ns * R + k;
 #pragma unroll
-            for (int t = 0; t < R; ++t) {
-                const uint32_t i = base + t * ns;
-                buf[ILV ? (i << kLogBlock) + c : c * pitch + i] = v[b][t];
+            for (int t1 = 1; t1 < R1; ++t1) {
+                const float2 wt = conj_if<INV>(w[b][R2 - 1 + t1 - 1]);
+#pragma unroll
+                for (int t2 = 0; t2 < R2; ++t2) v[b][t1 + R1 * t2] = cmul(v[b][t1 + R1 * t2], wt);
+            }
+        }
+        if constexpr (R2 > 1) {
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) {
+                float2 a[R2];
+#pragma unroll
+                for (int t2 = 0; t2 < R2; ++t2) a[t2] = v[b][t1 + R1 * t2];
+                butterfly<R2, INV>(a);
+#pragma unroll
+                for (int s2 = 0; s2 < R2; ++s2) v[b][t1 + R1 * s2] = a[s2];
+            }
+#pragma unroll
+            for (int t1 = 1; t1 < R1; ++t1)
+#pragma unroll
+                for (int s2 = 1; s2 < R2; ++s2) {
+                    const int m = (t1 * s2) % R;
+                    v[b][t1 + R1 * s2] = cmul(v[b][t1 + R1 * s2], make_float2(kr.re[m], INV ? -kr.im[m] : kr.im[m]));
+                }
+        }
+        const uint32_t base = qv[b] * ns * R + kv[b];
+#pragma unroll
+        for (int s2 = 0; s2 < R2; ++s2) {
+            float2 a[R1];
+#pragma unroll
+            for (int t1 = 0; t1 < R1; ++t1) a[t1] = v[b][t1 + R1 * s2];
+            butterfly<R1, INV>(a);
+#pragma unroll
+            for (int s1 = 0; s1 < R1; ++s1) {
+                const uint32_t i = base + (s2 + R2 * s1) * ns;
+                buf[ILV ? (i << kLogBlock) + c : c * pitch + i] = a[s1];
             }
         }
     }
@@ -171,64 +252,56 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *tw, uint32_t n,
 }
 
 // E: elements per thread per transform (n <= LANES E).
-template <bool INV, int E, bool ILV, int LANES, bool SPLIT>
-__device__ __forceinline__ void run_fft(float2 *buf, const float2 *tw, const Axis &ax, uint32_t pitch,
-                                        uint32_t c, uint32_t jt) {
+template <bool INV, int E, bool ILV, int LANES>
+__device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t pitch, uint32_t c, uint32_t jt) {
     for (uint32_t s = 0; s < ax.n_stages; ++s) {
-        const uint32_t r = ax.radix[s], ns = ax.ns[s], step = ax.step[s];
+        const uint32_t ns = ax.ns[s];
         const float inv = ax.inv_ns[s];
-        if (r == 5) stage<5, INV, (E + 4) / 5, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-        else if (r == 4) stage<4, INV, (E + 3) / 4, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-        else if (r == 3) stage<3, INV, (E + 2) / 3, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-        else stage<2, INV, (E + 1) / 2, ILV, LANES, SPLIT>(buf, tw, ax.n, pitch, ns, step, inv, c, jt);
-    }
-}
-
-// Split twiddle table: the 64 low differences, then ceil(n / 64) high entries.
-__device__ __forceinline__ void load_split_twiddles(float2 *tw, const Axis &ax, uint32_t tid, uint32_t nthreads) {
-    const uint32_t nhi = (ax.n + 63) / 64;
-    for (uint32_t m = tid; m < 64 + nhi; m += nthreads) tw[m] = m < 64 ? ax.twd[m] : ax.tw[64 * (m - 64)];
-}
-
-// Full twiddle table of a rows pass (n <= E * kLanes, so at most E / 4 + 1
-// entries per thread), loads issued together.
-template <int E>
-__device__ __forceinline__ void load_row_twiddles(float2 *tw, const Axis &ay, uint32_t tid, uint32_t n) {
-    constexpr int ET = E / 4 + 1;
-    float2 v[ET];
-#pragma unroll
-    for (int t = 0; t < ET; ++t) {
-        const uint32_t m = tid + static_cast<uint32_t>(t) * kRowThreads;
-        v[t] = m < n ? ay.tw[m] : make_float2(0.0f, 0.0f);
-    }
-#pragma unroll
-    for (int t = 0; t < ET; ++t) {
-        const uint32_t m = tid + static_cast<uint32_t>(t) * kRowThreads;
-        if (m < n) tw[m] = v[t];
+        const float2 *tw = ax.tw + ax.tw_off[s];
+        switch (ax.r1[s] * 8 + ax.r2[s]) {
+#define CHIP_XF_STAGE(A, B)                                                                          \
+    case A * 8 + B:                                                                                  \
+        stage<A, B, INV, (E + A * B - 1) / (A * B), ILV, LANES>(buf, tw, ax.nb[s], pitch, ns, inv, c, jt); \
+        break;
+            CHIP_XF_STAGE(5, 5)
+            CHIP_XF_STAGE(5, 3)
+            CHIP_XF_STAGE(5, 4)
+            CHIP_XF_STAGE(5, 2)
+            CHIP_XF_STAGE(3, 3)
+            CHIP_XF_STAGE(3, 4)
+            CHIP_XF_STAGE(3, 2)
+            CHIP_XF_STAGE(4, 4)
+            CHIP_XF_STAGE(4, 2)
+            CHIP_XF_STAGE(5, 1)
+            CHIP_XF_STAGE(3, 1)
+            CHIP_XF_STAGE(4, 1)
+            CHIP_XF_STAGE(2, 1)
+#undef CHIP_XF_STAGE
+            default: break;
+        }
     }
 }
 
 // ---------------------------------------------------------------- kernels
 // Rows passes: 8 rows per workgroup of kRowThreads, packed as 4 complex
 // transforms of two real rows each (row 2p real part, row 2p + 1 imaginary
-// part), row-major in LDS, then the full twiddle table; grid (nxp / 8,
-// n_chan).  Column passes: one block of 8 columns per workgroup of
-// kThreads, interleaved in LDS, then the split twiddle table (small enough
-// for two workgroups per CU); grid (nyb_h, n_chan).
+// part), row-major in LDS; grid (nxp / 8, n_chan).  Column passes: one block
+// of 8 columns per workgroup of kThreads, interleaved in LDS; grid (nyb_h,
+// n_chan).  LDS holds only the data (two column workgroups per CU up to
+// length 1280); waves_per_eu states the occupancy the LDS allows, so the
+// compiler may give a thread up to 128 VGPRs.
 
 __device__ __forceinline__ uint64_t half_at(const XGeom &xg, uint32_t yb, uint32_t x, uint32_t lane) {
     return static_cast<uint64_t>(yb) * xg.nxp * kBlock + x * kBlock + lane;
 }
 
 template <int E>
-__global__ __launch_bounds__(kRowThreads) void k_fwd_rows(XGeom xg, Axis ay, const float *__restrict__ model,
-                                                          float2 *__restrict__ half) {
+__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_fwd_rows(
+    XGeom xg, Axis ay, const float *__restrict__ model, float2 *__restrict__ half) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gy;
-    float2 *tw = buf + kPairs * n;
     const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
     const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
-    load_row_twiddles<E>(tw, ay, tid, n);
     // z = a + i b for rows a = x0 + 2p, b = a + 1 (apodised, signed, zero-padded)
     float f[2], sxv[2];
     const float *row[2];
@@ -242,37 +315,41 @@ __global__ __launch_bounds__(kRowThreads) void k_fwd_rows(XGeom xg, Axis ay, con
         f[q] = live[q] ? xg.corr_x[gxx[q]] : 0.0f;
         sxv[q] = live[q] && xg.wproject ? xg.sinc_x[gxx[q]] : 1.0f;
     }
-    // Two phases: every global load of the thread is issued before the first
-    // use, so the ~E loads are in flight together instead of one at a time.
-    float mv[E][2], cyv[E], syv[E];
+    // Two phases per chunk of CH elements: every global load of the chunk is
+    // issued before the first use, so they are in flight together.
+    constexpr int CH = E < 16 ? E : 16;
 #pragma unroll
-    for (int t = 0; t < E; ++t) {
-        const uint32_t i = jt + static_cast<uint32_t>(t) * kLanes;
-        const bool in = i < n && i >= xg.blc_y && i < xg.blc_y + xg.ny;
-        cyv[t] = in ? xg.corr_y[i] : 0.0f;
-        syv[t] = in && xg.wproject ? xg.sinc_y[i] : 1.0f;
-        for (int q = 0; q < 2; ++q) mv[t][q] = in && live[q] ? row[q][i - xg.blc_y] : 0.0f;
-    }
+    for (int t0 = 0; t0 < E; t0 += CH) {
+        float mv[CH][2], cyv[CH], syv[CH];
 #pragma unroll
-    for (int t = 0; t < E; ++t) {
-        const uint32_t i = jt + static_cast<uint32_t>(t) * kLanes;
-        if (i >= n) break;
-        float v[2] = {0.0f, 0.0f};
-        if (i >= xg.blc_y && i < xg.blc_y + xg.ny) {
-            for (int q = 0; q < 2; ++q) {
-                if (!live[q]) continue;
-                // factor evaluated left to right as the reference: cx*cy [*sx*sy]
-                float fac = f[q] * cyv[t];
-                if (xg.wproject) fac = fac * sxv[q] * syv[t];
-                float val = mv[t][q] * fac;
-                if ((gxx[q] + i) & 1) val = -val;
-                v[q] = val;
-            }
+        for (int u = 0; u < CH; ++u) {
+            const uint32_t i = jt + static_cast<uint32_t>(t0 + u) * kLanes;
+            const bool in = t0 + u < E && i < n && i >= xg.blc_y && i < xg.blc_y + xg.ny;
+            cyv[u] = in ? xg.corr_y[i] : 0.0f;
+            syv[u] = in && xg.wproject ? xg.sinc_y[i] : 1.0f;
+            for (int q = 0; q < 2; ++q) mv[u][q] = in && live[q] ? row[q][i - xg.blc_y] : 0.0f;
         }
-        buf[p * n + i] = make_float2(v[0], v[1]);
+#pragma unroll
+        for (int u = 0; u < CH; ++u) {
+            const uint32_t i = jt + static_cast<uint32_t>(t0 + u) * kLanes;
+            if (t0 + u >= E || i >= n) break;
+            float v[2] = {0.0f, 0.0f};
+            if (i >= xg.blc_y && i < xg.blc_y + xg.ny) {
+                for (int q = 0; q < 2; ++q) {
+                    if (!live[q]) continue;
+                    // factor evaluated left to right as the reference: cx*cy [*sx*sy]
+                    float fac = f[q] * cyv[u];
+                    if (xg.wproject) fac = fac * sxv[q] * syv[u];
+                    float val = mv[u][q] * fac;
+                    if ((gxx[q] + i) & 1) val = -val;
+                    v[q] = val;
+                }
+            }
+            buf[p * n + i] = make_float2(v[0], v[1]);
+        }
     }
     __syncthreads();
-    run_fft<false, E, false, kLanes, false>(buf, tw, ay, n, p, jt);
+    run_fft<false, E, false, kLanes>(buf, ay, n, p, jt);
     // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i, k <= gy/2,
     // into the half spectrum [yb][x][8]: 8 rows x 8 lanes contiguous per block
     float2 *h = half + static_cast<uint64_t>(c) * xg.hstride;
@@ -292,16 +369,13 @@ __global__ __launch_bounds__(kRowThreads) void k_fwd_rows(XGeom xg, Axis ay, con
     }
 }
 
-// Two workgroups per CU whenever the LDS allows it (transforms up to 1280).
 template <int E>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 10 ? 8 : 1))) void k_fwd_cols(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_fwd_cols(
     XGeom xg, Axis ax, const float2 *__restrict__ half, float2 *__restrict__ grid) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gx;
-    float2 *tw = buf + kBlock * n;
     const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
-    load_split_twiddles(tw, ax, tid, kThreads);
     const float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
     float2 hv[E];
 #pragma unroll
@@ -316,7 +390,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
         if (e < n * kBlock) buf[e] = hv[t];
     }
     __syncthreads();
-    run_fft<false, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
+    run_fft<false, E, true, kLanes>(buf, ax, 0, cc, jt);
     // column ky into upper block yb.  The columns ky > gy/2 are not stored:
     // the transform of a real image is Hermitian, so the binned kernel reads
     // G[kx, ky] as conj G[-kx, gy - ky] from the upper blocks.
@@ -330,26 +404,32 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
 }
 
 template <int E>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 10 ? 8 : 1))) void k_inv_cols(
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_inv_cols(
     XGeom xg, Axis ax, const float2 *__restrict__ grid, float2 *__restrict__ rows) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gx;
-    float2 *tw = buf + kBlock * n;
     const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
-    load_split_twiddles(tw, ax, tid, kThreads);
     // Hermitian part H[kx, ky] = (G[kx, ky] + conj G[-kx, -ky]) / 2 of the
     // columns ky <= gy/2, accumulated as 2H by the binned kernel; its inverse
     // transform is real and equals Re(centered inverse of G) after the shifts.
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
     const float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
     const uint32_t hy = xg.gy / 2;
+    // Loads from clamped in-block addresses, issued unconditionally so that
+    // they are all in flight together (a conditional load is compiled as a
+    // branch that waits for it); out-of-range values are then replaced by 0.
     float2 gv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
         const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
+        gv[t] = up[e < n * kBlock ? e : n * kBlock - 1];
+    }
+#pragma unroll
+    for (int t = 0; t < E; ++t) {
+        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
         const uint32_t ky = yb * kBlock + (e & 7);
-        gv[t] = (e < n * kBlock && ky <= hy) ? up[e] : make_float2(0.0f, 0.0f);
+        if (e >= n * kBlock || ky > hy) gv[t] = make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int t = 0; t < E; ++t) {
@@ -357,7 +437,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
         if (e < n * kBlock) buf[e] = make_float2(0.5f * gv[t].x, 0.5f * gv[t].y);
     }
     __syncthreads();
-    run_fft<true, E, true, kLanes, true>(buf, tw, ax, 0, cc, jt);
+    run_fft<true, E, true, kLanes>(buf, ax, 0, cc, jt);
     float2 *o = rows + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
     for (uint32_t e = tid; e < xg.nx * kBlock; e += kThreads) {
         uint32_t i = xg.blc_x + (e >> 3) + n / 2;
@@ -367,21 +447,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 1
 }
 
 template <int E>
-__global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, const float2 *__restrict__ rows,
-                                                          const float *__restrict__ nsum,
-                                                          const float *__restrict__ peak, float *__restrict__ out) {
+__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_inv_rows(
+    XGeom xg, Axis ay, const float2 *__restrict__ rows, const float *__restrict__ nsum,
+    const float *__restrict__ peak, float *__restrict__ out) {
     extern __shared__ float2 buf[];
     const uint32_t n = xg.gy;
-    float2 *tw = buf + kPairs * n;
     const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
     const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
-    load_row_twiddles<E>(tw, ay, tid, n);
     // Rows a = x0 + 2p and b = a + 1 are Hermitian in ky: Z[k] = A[k] + i B[k]
     // over the full length (A[-k] = conj A[k]), whose inverse is a + i b.
     const float2 *h = rows + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
     const uint32_t rows_live = xg.nx - x0;  // rows of this group inside the image (>= 1)
-    // elements per thread: nyb_h * 32 / 512 <= (n / 16 + 1) / 16 + 1 <= E / 2 + 1
+    // elements per thread: nyb_h * 32 / 256 <= (n / 16 + 1) / 8 + 1 <= E / 2 + 1
     constexpr int EL = E / 2 + 1;
     const uint32_t total = xg.nyb_h * (kPairs * kBlock);
     float2 av[EL], bv[EL];
@@ -407,7 +485,7 @@ __global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, con
         if (k > 0 && k < hy) buf[pp * n + n - k] = make_float2(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    run_fft<true, E, false, kLanes, false>(buf, tw, ay, n, p, jt);
+    run_fft<true, E, false, kLanes>(buf, ay, n, p, jt);
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(xg.gx) * xg.gy);
     const float ns = nsum[c], pk = peak[c];
     // y corrections of this thread's columns, loaded together (ny <= n <= E kLanes)
@@ -452,60 +530,61 @@ __global__ __launch_bounds__(kRowThreads) void k_inv_rows(XGeom xg, Axis ay, con
 
 // ---------------------------------------------------------------- host side
 
-bool factor_axis(uint32_t n, Axis &ax) {
+bool factor_axis(uint32_t n, Axis &ax, std::vector<float2> &table) {
     ax.n = n;
     ax.n_stages = 0;
-    const uint32_t order[4] = {5, 3, 4, 2};
-    uint32_t rest = n, ns = 1;
-    for (uint32_t p : order)
+    table.clear();
+    if (n == 0 || n >= (1u << 22)) return false;
+    // radices 5, 3, 4 (as many as divide), then at most one 2; consecutive
+    // radices pair into one stage
+    std::vector<uint32_t> f;
+    uint32_t rest = n;
+    for (uint32_t p : {5u, 3u, 4u, 2u})
         while (rest % p == 0) {
-            if (ax.n_stages >= kMaxStages) return false;
-            const uint32_t s = ax.n_stages++;
-            ax.radix[s] = p;
-            ax.ns[s] = ns;
-            ax.step[s] = n / (ns * p);
-            ax.inv_ns[s] = 1.0f / static_cast<float>(ns);
-            ns *= p;
+            f.push_back(p);
             rest /= p;
         }
-    return rest == 1 && n < (1u << 22);
-}
-
-std::vector<float2> twiddles(uint32_t n) {
-    std::vector<float2> t(n);
-    for (uint32_t m = 0; m < n; ++m) {
-        const double a = -2.0 * M_PI * static_cast<double>(m) / static_cast<double>(n);
-        t[m] = make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+    if (rest != 1) return false;
+    uint32_t ns = 1;
+    for (size_t i = 0; i < f.size(); i += 2) {
+        if (ax.n_stages >= kMaxStages) return false;
+        const uint32_t s = ax.n_stages++;
+        const uint32_t r1 = f[i], r2 = i + 1 < f.size() ? f[i + 1] : 1u, R = r1 * r2;
+        ax.r1[s] = r1;
+        ax.r2[s] = r2;
+        ax.ns[s] = ns;
+        ax.nb[s] = n / R;
+        ax.inv_ns[s] = 1.0f / static_cast<float>(ns);
+        ax.tw_off[s] = static_cast<uint32_t>(table.size());
+        if (ns > 1) {
+            const uint64_t N = static_cast<uint64_t>(ns) * R;
+            auto root = [&](uint64_t m) {  // exp(-2 pi i m / N)
+                const double a = -2.0 * M_PI * static_cast<double>(m % N) / static_cast<double>(N);
+                return make_float2(static_cast<float>(std::cos(a)), static_cast<float>(std::sin(a)));
+            };
+            for (uint32_t k = 0; k < ns; ++k) {
+                for (uint32_t t2 = 1; t2 < r2; ++t2) table.push_back(root(static_cast<uint64_t>(k) * r1 * t2));
+                for (uint32_t t1 = 1; t1 < r1; ++t1) table.push_back(root(static_cast<uint64_t>(k) * t1));
+            }
+        }
+        ns *= R;
     }
-    return t;
-}
-
-std::vector<float2> twiddle_differences(uint32_t n) {
-    std::vector<float2> t(64);
-    for (uint32_t m = 0; m < 64; ++m) {
-        const double a = -2.0 * M_PI * static_cast<double>(m) / static_cast<double>(n);
-        // cos(a) - 1 = -2 sin^2(a / 2), without cancellation
-        const double s = std::sin(0.5 * a);
-        t[m] = make_float2(static_cast<float>(-2.0 * s * s), static_cast<float>(std::sin(a)));
-    }
-    return t;
+    if (table.empty()) table.push_back(make_float2(1.0f, 0.0f));  // keep the table pointer valid
+    return true;
 }
 
 namespace {
 
 int e_bucket(uint32_t n) {
     const uint32_t per = (n + kLanes - 1) / kLanes;
-    if (per <= 4) return 4;
-    if (per <= 8) return 8;
-    if (per <= 10) return 10;
-    if (per <= 12) return 12;
-    if (per <= 16) return 16;
+    for (int e : {4, 8, 12, 16, 20, 24, 32})
+        if (per <= static_cast<uint32_t>(e)) return e;
     return 0;
 }
 
-// rows tile + full twiddle table; column tile + split twiddle table
-size_t row_lds(uint32_t n) { return static_cast<size_t>(kPairs + 1) * n * sizeof(float2); }
-size_t col_lds(uint32_t n) { return (static_cast<size_t>(kBlock) * n + 64 + (n + 63) / 64) * sizeof(float2); }
+// rows tile: 4 transforms; column tile: 8 interleaved transforms
+size_t row_lds(uint32_t n) { return static_cast<size_t>(kPairs) * n * sizeof(float2); }
+size_t col_lds(uint32_t n) { return static_cast<size_t>(kBlock) * n * sizeof(float2); }
 
 template <class K> void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024)
@@ -525,9 +604,11 @@ template <class K> void allow_lds(K kernel, size_t bytes) {
         switch (E) {                                                                               \
             CHIP_XF_CASE(KERNEL, 4, GDIM, NT, LDS, ST, __VA_ARGS__)                                \
             CHIP_XF_CASE(KERNEL, 8, GDIM, NT, LDS, ST, __VA_ARGS__)                                \
-            CHIP_XF_CASE(KERNEL, 10, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
             CHIP_XF_CASE(KERNEL, 12, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
             CHIP_XF_CASE(KERNEL, 16, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
+            CHIP_XF_CASE(KERNEL, 20, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
+            CHIP_XF_CASE(KERNEL, 24, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
+            CHIP_XF_CASE(KERNEL, 32, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
             default: fail(CHIP_ERR_INVALID, "transform too long for one workgroup");              \
         }                                                                                          \
         CHIP_HIP(hipGetLastError());                                                               \
@@ -538,7 +619,8 @@ template <class K> void allow_lds(K kernel, size_t bytes) {
 bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, XPlan &p) {
     if (g.gx % 2 || g.gy % 2) return false;
     if (n_chan == 0 || n_chan > 65535 || g.gx > 0xffffffu || g.gy > 0xffffffu) return false;
-    if (!factor_axis(static_cast<uint32_t>(g.gx), p.ax) || !factor_axis(static_cast<uint32_t>(g.gy), p.ay))
+    std::vector<float2> tx, ty;
+    if (!factor_axis(static_cast<uint32_t>(g.gx), p.ax, tx) || !factor_axis(static_cast<uint32_t>(g.gy), p.ay, ty))
         return false;
     p.e_x = e_bucket(p.ax.n);
     p.e_y = e_bucket(p.ay.n);
@@ -564,18 +646,12 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.sinc_x = g.wproject ? g.d_sinc_x.as<float>() : nullptr;
     xg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
     p.n_chan = static_cast<uint32_t>(n_chan);
-    std::vector<float2> tx = twiddles(p.ax.n), ty = twiddles(p.ay.n);
-    std::vector<float2> dx = twiddle_differences(p.ax.n), dy = twiddle_differences(p.ay.n);
-    tx.insert(tx.end(), dx.begin(), dx.end());
-    ty.insert(ty.end(), dy.begin(), dy.end());
     p.tw_x.alloc(tx.size() * sizeof(float2));
     p.tw_y.alloc(ty.size() * sizeof(float2));
     CHIP_HIP(hipMemcpy(p.tw_x.p, tx.data(), p.tw_x.bytes, hipMemcpyHostToDevice));
     CHIP_HIP(hipMemcpy(p.tw_y.p, ty.data(), p.tw_y.bytes, hipMemcpyHostToDevice));
     p.ax.tw = p.tw_x.as<float2>();
     p.ay.tw = p.tw_y.as<float2>();
-    p.ax.twd = p.ax.tw + p.ax.n;
-    p.ay.twd = p.ay.tw + p.ay.n;
     return true;
 }
 

@@ -20,23 +20,28 @@ namespace xf {
 constexpr uint32_t kMaxStages = 16;
 constexpr uint32_t kBlock = 8;     // y columns per block = transforms per workgroup
 constexpr uint32_t kLogBlock = 3;
-constexpr uint32_t kThreads = 1024;              // column-pass workgroup: 8 transforms
+constexpr uint32_t kThreads = 512;               // column-pass workgroup: 8 transforms
 constexpr uint32_t kLanes = kThreads / kBlock;  // threads per transform
 constexpr uint32_t kRowsPerGroup = 8;            // rows-pass workgroup: 8 rows as 4 transforms
 constexpr uint32_t kPairs = kRowsPerGroup / 2;
 constexpr uint32_t kRowThreads = kPairs * kLanes;
 
-// One transform axis: length n = product of radix[0..n_stages) (2, 3, 4, 5),
-// with the per-stage constants of the Stockham recursion.
+// One transform axis: length n = product over the stages of r1[s] * r2[s]
+// (r1 in 2..5, r2 in 1..5), with the per-stage constants of the Stockham
+// recursion.  A stage of radix r1 * r2 runs two radix passes in registers.
 struct Axis {
     uint32_t n;
     uint32_t n_stages;
-    uint32_t radix[kMaxStages];
+    uint32_t r1[kMaxStages];
+    uint32_t r2[kMaxStages];
     uint32_t ns[kMaxStages];      // product of the radices before stage s
-    uint32_t step[kMaxStages];    // n / (ns * radix): twiddle index stride
+    uint32_t nb[kMaxStages];      // butterflies of stage s: n / (r1 r2)
+    uint32_t tw_off[kMaxStages];  // first twiddle of stage s in tw
     float inv_ns[kMaxStages];     // 1 / ns, for the exact float quotient j / ns
-    const float2 *tw;             // n twiddles exp(-2 pi i m / n)
-    const float2 *twd;            // 64 differences exp(-2 pi i m / n) - 1
+    // Per stage with ns > 1 and W = exp(-2 pi i / (ns r1 r2)), for each k < ns:
+    // W^(k r1 t2) for t2 = 1 .. r2-1, then W^(k t1) for t1 = 1 .. r1-1, each
+    // rounded once from FP64.
+    const float2 *tw;
 };
 
 struct XGeom {
@@ -60,9 +65,9 @@ struct XPlan {
     DevBuf tw_x, tw_y;
 };
 
-bool factor_axis(uint32_t n, Axis &ax);
-std::vector<float2> twiddles(uint32_t n);
-std::vector<float2> twiddle_differences(uint32_t n);
+// Stages of a length-n axis and its twiddle table (false when n does not
+// factor into 2, 3, 5 or has too many stages).
+bool factor_axis(uint32_t n, Axis &ax, std::vector<float2> &table);
 
 // False when a grid length does not factor into 2, 3, 5, a transform does
 // not fit the LDS limit or the per-thread register budget, or the image
