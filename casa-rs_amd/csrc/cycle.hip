@@ -91,29 +91,52 @@ __global__ void k_cycle_crop(CycleGeom cg, const float *__restrict__ raw, const 
     out[(static_cast<uint64_t>(c) * cg.nx + x) * cg.ny + y] = v;
 }
 
-// Zero fill by a kernel rather than hipMemsetAsync, so that the major cycle
-// can be captured into a graph (memset nodes are avoided there).
-// Segment s of n_seg covers float4s [s * stride, s * stride + seg).
-__global__ void k_zero(float4 *p, uint64_t seg, uint64_t stride, uint64_t n_seg) {
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint64_t step = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    const uint64_t n = seg * n_seg;
-    for (uint64_t k = i; k < n; k += step) {
-        const uint64_t s = k / seg;
-        p[s * stride + (k - s * seg)] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+// uv-coverage mask of the fused layout (XGeom::touch): one thread per work
+// item marks the row chunks of the upper blocks that its tile and halo,
+// rectangle [xa, xb] x [ya, yb] of the full grid, reads from the model grid
+// or adds to the residual grid: cells y <= gy/2 in place, and cells y >= gy/2
+// or y == 0 at their Hermitian mirror ((gx - x) mod gx, (gy - y) mod gy),
+// where the binned kernel reads the model's conjugate and adds the residual's.
+struct TouchGeom {
+    uint32_t gx, gy, tile, halo, tiles_y, nyb_h, nxc;
+};
+
+__device__ void touch_mark(uint8_t *t, const TouchGeom &tg, uint32_t ylo, uint32_t yhi, uint32_t xlo,
+                           uint32_t xhi) {
+    for (uint32_t b = ylo / xf::kBlock; b <= yhi / xf::kBlock; ++b)
+        for (uint32_t ch = xlo / xf::kTouchRows; ch <= xhi / xf::kTouchRows; ++ch) t[b * tg.nxc + ch] = 1;
+}
+
+// rows x' = (gx - x) mod gx of x in [xa, xb]
+__device__ void touch_mark_mirror(uint8_t *t, const TouchGeom &tg, uint32_t ylo, uint32_t yhi, uint32_t xa,
+                                  uint32_t xb) {
+    if (xa == 0) {
+        touch_mark(t, tg, ylo, yhi, 0, 0);
+        if (xb >= 1) touch_mark(t, tg, ylo, yhi, tg.gx - xb, tg.gx - 1);
+    } else {
+        touch_mark(t, tg, ylo, yhi, tg.gx - xb, tg.gx - xa);
     }
 }
 
-// n_seg segments of seg_bytes, stride_bytes apart (16-byte multiples).
-void zero_fill(void *p, uint64_t seg_bytes, uint64_t stride_bytes, uint64_t n_seg, hipStream_t st) {
-    if (seg_bytes % sizeof(float4) || stride_bytes % sizeof(float4) || seg_bytes > stride_bytes)
-        fail(CHIP_ERR_INVALID, "zero fill of segments that are not 16-byte sized");
-    const uint64_t seg = seg_bytes / sizeof(float4), n = seg * n_seg;
-    if (n == 0) return;
-    const uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 8192);
-    hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, static_cast<float4 *>(p), seg,
-                       stride_bytes / sizeof(float4), n_seg);
-    CHIP_HIP(hipGetLastError());
+__global__ void k_touch(const WorkItem *__restrict__ items, uint64_t n_items, TouchGeom tg, uint8_t *touch) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    const WorkItem it = items[i];
+    const uint32_t tx = it.tile / tg.tiles_y, ty = it.tile - tx * tg.tiles_y;
+    const int64_t L = tg.tile + 2 * tg.halo;
+    const int64_t x0 = static_cast<int64_t>(tx) * tg.tile - tg.halo, y0 = static_cast<int64_t>(ty) * tg.tile - tg.halo;
+    const int64_t xa = x0 < 0 ? 0 : x0, ya = y0 < 0 ? 0 : y0;
+    const int64_t xb = std::min<int64_t>(x0 + L - 1, tg.gx - 1), yb = std::min<int64_t>(y0 + L - 1, tg.gy - 1);
+    if (xa > xb || ya > yb) return;
+    uint8_t *t = touch + static_cast<uint64_t>(it.chan) * tg.nyb_h * tg.nxc;
+    const int64_t hy = tg.gy / 2;
+    const uint32_t uxa = static_cast<uint32_t>(xa), uxb = static_cast<uint32_t>(xb);
+    if (ya <= hy) touch_mark(t, tg, static_cast<uint32_t>(ya), static_cast<uint32_t>(std::min(yb, hy)), uxa, uxb);
+    if (yb >= hy) {
+        const uint32_t ylo = static_cast<uint32_t>(tg.gy - yb), yhi = static_cast<uint32_t>(tg.gy - std::max(ya, hy));
+        touch_mark_mirror(t, tg, ylo, yhi, uxa, uxb);
+    }
+    if (ya == 0) touch_mark_mirror(t, tg, 0, 0, uxa, uxb);
 }
 
 // Schedule autotuner of the binned residual kernel: for both group orders
@@ -205,6 +228,8 @@ struct CycleGraph {
 struct CycleWorkspace {
     int strategy = -1;  // CHIP_CYCLE_HIPFFT or CHIP_CYCLE_FUSED once set up
     DevBuf model_grid, res_grid;
+    DevBuf touch;                        // fused: uv-coverage mask of the work items
+    uint64_t touch_version = ~0ull;      // items_version the mask was built for
     hipfftHandle r2c = nullptr, c2r = nullptr;
     xf::XPlan xp;
     double transform_ms = -1.0;
@@ -218,6 +243,21 @@ struct CycleWorkspace {
 
 namespace {
 
+
+void build_touch(const chip_vset &vs, CycleWorkspace &ws, hipStream_t st) {
+    xf::XGeom &xg = ws.xp.xg;
+    const uint64_t bytes = static_cast<uint64_t>(ws.xp.n_chan) * xg.nyb_h * xg.nxc;
+    if (ws.touch.bytes != bytes) ws.touch.alloc(bytes);
+    CHIP_HIP(hipMemsetAsync(ws.touch.p, 0, bytes, st));
+    if (vs.n_items) {
+        const TouchGeom tg{xg.gx, xg.gy, vs.tile, vs.halo, vs.tiles_y, xg.nyb_h, xg.nxc};
+        hipLaunchKernelGGL(k_touch, dim3(static_cast<unsigned>((vs.n_items + 255) / 256)), dim3(256), 0, st,
+                           vs.items.as<WorkItem>(), vs.n_items, tg, ws.touch.as<uint8_t>());
+        CHIP_HIP(hipGetLastError());
+    }
+    xg.touch = ws.touch.as<uint8_t>();
+    ws.touch_version = vs.items_version;
+}
 
 // Forward + inverse transforms of the fused plan, in ms.  Writes scratch
 // values into the workspace and `out` (overwritten afterwards).
@@ -262,6 +302,7 @@ void cycle_setup(chip_vset &vs, CycleWorkspace &ws, const float *model, float *o
     CHIP_HIP(hipMemsetAsync(ws.res_grid.p, 0, ws.res_grid.bytes, st));
     if (fused) {
         ws.strategy = CHIP_CYCLE_FUSED;
+        build_touch(vs, ws, st);
         ws.transform_ms = time_fused(vs, ws, model, out, st);
         return;
     }
@@ -351,13 +392,16 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
             autotune_schedule(vs, make_gp, M, reinterpret_cast<float *>(Rg), st);
             ws.graph.reset();  // the stream was re-binned
         }
+        if (ws.touch_version != vs.items_version) {
+            build_touch(vs, ws, st);
+            ws.graph.reset();
+        }
         const GridParams gp = make_gp();
         // the half spectrum borrows the residual grid, the inverse rows the model grid
         auto enqueue = [&](hipStream_t s, bool timed) {
             xf::forward_model(ws.xp, model, Rg, M, s);
-            // the upper blocks of every channel (the residual is kept as 2H there)
-            const uint64_t gs = ws.xp.xg.gstride * sizeof(float2);
-            zero_fill(Rg, gs / 2, gs, nc, s);
+            // the covered rows of the upper blocks (the residual is kept as 2H there)
+            xf::zero_touched(ws.xp, Rg, s);
             vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), s, timed);
             xf::inverse_residual(ws.xp, Rg, M, d_nsum, d_peak, out, s);
         };

@@ -287,6 +287,7 @@ struct chip_vset {
     uint32_t halo = 3;        // max half-support
     uint32_t tiles_x = 0, tiles_y = 0;
     uint64_t n_items = 0;
+    uint64_t items_version = 0;  // bumped whenever the work items are rewritten
     // W tables recomputed on the fly: the stream is binned by (plane chunk,
     // channel, tile); chunk_items[c] is the first work item of chunk c (plus
     // n_items at the end).  wk_chunk = 0: one chunk.

@@ -421,6 +421,7 @@ void vset_split_items(chip_vset &vs, uint32_t gpi) {
     const uint64_t per_chunk = static_cast<uint64_t>(vs.n_tiles_all) * vs.n_chan;
     for (uint64_t c = 0; c < vs.chunk_items.size(); ++c) vs.chunk_items[c] = starts[c * per_chunk];
     CHIP_HIP(hipStreamSynchronize(st));
+    vs.items_version += 1;
 }
 
 namespace {

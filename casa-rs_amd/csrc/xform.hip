@@ -51,6 +51,12 @@
 namespace chip {
 namespace xf {
 
+// Word 3 of a raw buffer resource on gfx9-family GPUs (no format, no
+// swizzle), and an offset beyond any buffer this file builds (loads there
+// return 0).
+constexpr int kBufferFlags = 0x00020000;
+constexpr uint32_t kBufferOut = 0x80000000u;
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -291,6 +297,23 @@ __device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t pi
 // length 1280); waves_per_eu states the occupancy the LDS allows, so the
 // compiler may give a thread up to 128 VGPRs.
 
+// Coverage mask of one upper block as two 64-bit words (row chunk ch at bit
+// ch of lo for ch < 64, else of hi), gathered by a wavefront ballot.
+struct RowMask {
+    uint64_t lo, hi;
+};
+__device__ __forceinline__ RowMask row_mask(const XGeom &xg, uint32_t c, uint32_t yb) {
+    if (!xg.touch) return {~0ull, ~0ull};
+    const uint8_t *t = xg.touch + (static_cast<uint64_t>(c) * xg.nyb_h + yb) * xg.nxc;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint8_t a = t[lane < xg.nxc ? lane : 0], b = t[lane + 64 < xg.nxc ? lane + 64 : 0];
+    return {__ballot(lane < xg.nxc && a != 0), __ballot(lane + 64 < xg.nxc && b != 0)};
+}
+__device__ __forceinline__ bool row_touched(const RowMask &m, uint32_t x) {
+    const uint32_t ch = x / kTouchRows;
+    return ((ch < 64 ? m.lo >> ch : m.hi >> (ch - 64)) & 1ull) != 0;
+}
+
 __device__ __forceinline__ uint64_t half_at(const XGeom &xg, uint32_t yb, uint32_t x, uint32_t lane) {
     return static_cast<uint64_t>(yb) * xg.nxp * kBlock + x * kBlock + lane;
 }
@@ -393,13 +416,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 2
     run_fft<false, E, true, kLanes>(buf, ax, 0, cc, jt);
     // column ky into upper block yb.  The columns ky > gy/2 are not stored:
     // the transform of a real image is Hermitian, so the binned kernel reads
-    // G[kx, ky] as conj G[-kx, gy - ky] from the upper blocks.
+    // G[kx, ky] as conj G[-kx, gy - ky] from the upper blocks.  Rows outside
+    // the uv coverage are not stored either: the binned kernel never reads them.
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
     float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
     const uint32_t hy = xg.gy / 2;
+    const RowMask rm = row_mask(xg, c, yb);
     for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
         const uint32_t ky = yb * kBlock + (e & 7);
-        if (ky <= hy) up[e] = buf[e];
+        if (ky <= hy && row_touched(rm, e >> kLogBlock)) up[e] = buf[e];
     }
 }
 
@@ -416,20 +441,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 2
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
     const float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
     const uint32_t hy = xg.gy / 2;
-    // Loads from clamped in-block addresses, issued unconditionally so that
-    // they are all in flight together (a conditional load is compiled as a
-    // branch that waits for it); out-of-range values are then replaced by 0.
+    // Buffer loads, issued unconditionally so that they are all in flight
+    // together (a conditional load is compiled as a branch that waits for
+    // it): an element outside the block, a column ky > gy/2 or a row outside
+    // the uv coverage (zero residual) gets an offset beyond the buffer's
+    // size, for which the load returns 0 without a memory access.
+    const RowMask rm = row_mask(xg, c, yb);
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2 *>(up), 0, static_cast<int>(n * kBlock * sizeof(float2)), kBufferFlags);
     float2 gv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
         const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
-        gv[t] = up[e < n * kBlock ? e : n * kBlock - 1];
-    }
-#pragma unroll
-    for (int t = 0; t < E; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
         const uint32_t ky = yb * kBlock + (e & 7);
-        if (e >= n * kBlock || ky > hy) gv[t] = make_float2(0.0f, 0.0f);
+        const bool live = e < n * kBlock && ky <= hy && row_touched(rm, e >> kLogBlock);
+        gv[t] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                               rs, live ? e * static_cast<uint32_t>(sizeof(float2)) : kBufferOut, 0, 0));
     }
 #pragma unroll
     for (int t = 0; t < E; ++t) {
@@ -526,6 +553,17 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
             o[y] = v;
         }
     }
+}
+
+// Zero fill of the covered rows of the upper blocks: grid (nyb_h, n_chan),
+// 256 threads; a row of a block is 8 float2 = 4 float4.
+__global__ __launch_bounds__(256) void k_zero_touched(XGeom xg, float4 *__restrict__ grid) {
+    const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    const RowMask rm = row_mask(xg, c, yb);
+    float4 *blk = grid + (static_cast<uint64_t>(c) * xg.gstride + static_cast<uint64_t>(yb) * xg.gx * kBlock) / 2;
+    const uint32_t n4 = xg.gx * (kBlock / 2);
+    for (uint32_t q = threadIdx.x; q < n4; q += 256)
+        if (row_touched(rm, q >> 2)) blk[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- host side
@@ -645,6 +683,9 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.sinc = g.wproject ? g.d_sinc.as<float>() : nullptr;
     xg.sinc_x = g.wproject ? g.d_sinc_x.as<float>() : nullptr;
     xg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
+    xg.touch = nullptr;
+    xg.nxc = (xg.gx + kTouchRows - 1) / kTouchRows;
+    if (xg.nxc > 128) return false;
     p.n_chan = static_cast<uint32_t>(n_chan);
     p.tw_x.alloc(tx.size() * sizeof(float2));
     p.tw_y.alloc(ty.size() * sizeof(float2));
@@ -668,6 +709,12 @@ void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows_buf, cons
     const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
     CHIP_XF_LAUNCH(p.e_x, k_inv_cols, cols, kThreads, cl, st, p.xg, p.ax, grid, rows_buf);
     CHIP_XF_LAUNCH(p.e_y, k_inv_rows, rows, kRowThreads, rl, st, p.xg, p.ay, rows_buf, nsum, peak, out);
+}
+
+void zero_touched(const XPlan &p, float2 *grid, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_touched, dim3(p.xg.nyb_h, p.n_chan), dim3(256), 0, st, p.xg,
+                       reinterpret_cast<float4 *>(grid));
+    CHIP_HIP(hipGetLastError());
 }
 
 }  // namespace xf

@@ -25,6 +25,7 @@ constexpr uint32_t kLanes = kThreads / kBlock;  // threads per transform
 constexpr uint32_t kRowsPerGroup = 8;            // rows-pass workgroup: 8 rows as 4 transforms
 constexpr uint32_t kPairs = kRowsPerGroup / 2;
 constexpr uint32_t kRowThreads = kPairs * kLanes;
+constexpr uint32_t kTouchRows = 16;              // x rows per chunk of the uv-coverage mask
 
 // One transform axis: length n = product over the stages of r1[s] * r2[s]
 // (r1 in 2..5, r2 in 1..5), with the per-stage constants of the Stockham
@@ -55,6 +56,13 @@ struct XGeom {
     const float *corr_x, *corr_y;
     const float *sinc;            // W products: sinc of max(gx, gy) on both axes
     const float *sinc_x, *sinc_y; // W model apodisation: per-axis sinc
+    // uv coverage: touch[(c * nyb_h + yb) * nxc + x / kTouchRows] != 0 when the
+    // binned kernel may read the model or add to the residual in rows
+    // [x, x + kTouchRows) of upper block yb; nullptr = every row.  The column
+    // passes store (forward) and load (inverse) only those rows: the model
+    // grid elsewhere is never read, the residual grid elsewhere is zero.
+    const uint8_t *touch;
+    uint32_t nxc;       // ceil(gx / kTouchRows) <= 128
 };
 
 struct XPlan {
@@ -86,6 +94,10 @@ void forward_model(const XPlan &p, const float *model, float2 *half, float2 *gri
 // device floats).
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows, const float *nsum,
                       const float *peak, float *out, hipStream_t st);
+
+// Zeroes the upper blocks of a paired grid cube in the rows the coverage
+// mask marks (every row without a mask).
+void zero_touched(const XPlan &p, float2 *grid, hipStream_t st);
 
 }  // namespace xf
 }  // namespace chip
