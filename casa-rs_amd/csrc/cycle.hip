@@ -98,17 +98,24 @@ __global__ void k_cycle_crop(CycleGeom cg, const float *__restrict__ raw, const 
 // or y == 0 at their Hermitian mirror ((gx - x) mod gx, (gy - y) mod gy),
 // where the binned kernel reads the model's conjugate and adds the residual's.
 struct TouchGeom {
-    uint32_t gx, gy, tile, halo, tiles_y, nyb_h, nxc;
+    uint32_t gx, gy, tile, halo, tiles_y, nyb_h, nxc, n_chan;
 };
 
-__device__ void touch_mark(uint8_t *t, const TouchGeom &tg, uint32_t ylo, uint32_t yhi, uint32_t xlo,
+// t: the channel's chunk flags [nyb_h][nxc], followed at tb by its block flags [nyb_h]
+struct TouchOut {
+    uint8_t *t, *tb;
+};
+
+__device__ void touch_mark(const TouchOut &o, const TouchGeom &tg, uint32_t ylo, uint32_t yhi, uint32_t xlo,
                            uint32_t xhi) {
-    for (uint32_t b = ylo / xf::kBlock; b <= yhi / xf::kBlock; ++b)
-        for (uint32_t ch = xlo / xf::kTouchRows; ch <= xhi / xf::kTouchRows; ++ch) t[b * tg.nxc + ch] = 1;
+    for (uint32_t b = ylo / xf::kBlock; b <= yhi / xf::kBlock; ++b) {
+        o.tb[b] = 1;
+        for (uint32_t ch = xlo / xf::kTouchRows; ch <= xhi / xf::kTouchRows; ++ch) o.t[b * tg.nxc + ch] = 1;
+    }
 }
 
 // rows x' = (gx - x) mod gx of x in [xa, xb]
-__device__ void touch_mark_mirror(uint8_t *t, const TouchGeom &tg, uint32_t ylo, uint32_t yhi, uint32_t xa,
+__device__ void touch_mark_mirror(const TouchOut &t, const TouchGeom &tg, uint32_t ylo, uint32_t yhi, uint32_t xa,
                                   uint32_t xb) {
     if (xa == 0) {
         touch_mark(t, tg, ylo, yhi, 0, 0);
@@ -128,7 +135,9 @@ __global__ void k_touch(const WorkItem *__restrict__ items, uint64_t n_items, To
     const int64_t xa = x0 < 0 ? 0 : x0, ya = y0 < 0 ? 0 : y0;
     const int64_t xb = std::min<int64_t>(x0 + L - 1, tg.gx - 1), yb = std::min<int64_t>(y0 + L - 1, tg.gy - 1);
     if (xa > xb || ya > yb) return;
-    uint8_t *t = touch + static_cast<uint64_t>(it.chan) * tg.nyb_h * tg.nxc;
+    const TouchOut t{touch + static_cast<uint64_t>(it.chan) * tg.nyb_h * tg.nxc,
+                     touch + static_cast<uint64_t>(tg.n_chan) * tg.nyb_h * tg.nxc +
+                         static_cast<uint64_t>(it.chan) * tg.nyb_h};
     const int64_t hy = tg.gy / 2;
     const uint32_t uxa = static_cast<uint32_t>(xa), uxb = static_cast<uint32_t>(xb);
     if (ya <= hy) touch_mark(t, tg, static_cast<uint32_t>(ya), static_cast<uint32_t>(std::min(yb, hy)), uxa, uxb);
@@ -246,16 +255,18 @@ namespace {
 
 void build_touch(const chip_vset &vs, CycleWorkspace &ws, hipStream_t st) {
     xf::XGeom &xg = ws.xp.xg;
-    const uint64_t bytes = static_cast<uint64_t>(ws.xp.n_chan) * xg.nyb_h * xg.nxc;
+    const uint64_t chunks = static_cast<uint64_t>(ws.xp.n_chan) * xg.nyb_h * xg.nxc;
+    const uint64_t bytes = chunks + static_cast<uint64_t>(ws.xp.n_chan) * xg.nyb_h;
     if (ws.touch.bytes != bytes) ws.touch.alloc(bytes);
     CHIP_HIP(hipMemsetAsync(ws.touch.p, 0, bytes, st));
     if (vs.n_items) {
-        const TouchGeom tg{xg.gx, xg.gy, vs.tile, vs.halo, vs.tiles_y, xg.nyb_h, xg.nxc};
+        const TouchGeom tg{xg.gx, xg.gy, vs.tile, vs.halo, vs.tiles_y, xg.nyb_h, xg.nxc, ws.xp.n_chan};
         hipLaunchKernelGGL(k_touch, dim3(static_cast<unsigned>((vs.n_items + 255) / 256)), dim3(256), 0, st,
                            vs.items.as<WorkItem>(), vs.n_items, tg, ws.touch.as<uint8_t>());
         CHIP_HIP(hipGetLastError());
     }
     xg.touch = ws.touch.as<uint8_t>();
+    xg.touch_blk = xg.touch + chunks;
     ws.touch_version = vs.items_version;
 }
 

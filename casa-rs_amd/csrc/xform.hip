@@ -375,10 +375,14 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     run_fft<false, E, false, kLanes>(buf, ay, n, p, jt);
     // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i, k <= gy/2,
     // into the half spectrum [yb][x][8]: 8 rows x 8 lanes contiguous per block
+    // (blocks of columns the uv coverage misses are not stored: the column
+    // pass skips them)
     float2 *h = half + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
+    const uint8_t *blk = xg.touch_blk ? xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h : nullptr;
     for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += kRowThreads) {
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
+        if (blk && !blk[yb]) continue;
         const uint32_t k = yb * kBlock + lane;
         float2 A = make_float2(0.0f, 0.0f), B = A;
         if (k <= hy) {
@@ -399,6 +403,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 2
     const uint32_t n = xg.gx;
     const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    if (xg.touch_blk && !xg.touch_blk[static_cast<uint64_t>(c) * xg.nyb_h + yb]) return;  // outside the uv coverage
     const float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
     float2 hv[E];
 #pragma unroll
@@ -435,6 +440,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 2
     const uint32_t n = xg.gx;
     const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    if (xg.touch_blk && !xg.touch_blk[static_cast<uint64_t>(c) * xg.nyb_h + yb]) return;  // outside the uv coverage
     // Hermitian part H[kx, ky] = (G[kx, ky] + conj G[-kx, -ky]) / 2 of the
     // columns ky <= gy/2, accumulated as 2H by the binned kernel; its inverse
     // transform is real and equals Re(centered inverse of G) after the shifts.
@@ -489,17 +495,24 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     // elements per thread: nyb_h * 32 / 256 <= (n / 16 + 1) / 8 + 1 <= E / 2 + 1
     constexpr int EL = E / 2 + 1;
     const uint32_t total = xg.nyb_h * (kPairs * kBlock);
+    // Buffer loads (see k_inv_cols); the blocks of columns the uv coverage
+    // misses were not transformed and read as zero.
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2 *>(h), 0, static_cast<int>(xg.hstride * sizeof(float2)), kBufferFlags);
+    const uint8_t *blk = xg.touch_blk ? xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h : nullptr;
     float2 av[EL], bv[EL];
 #pragma unroll
     for (int t = 0; t < EL; ++t) {
         const uint32_t e = tid + static_cast<uint32_t>(t) * kRowThreads;
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
         const uint32_t k = yb * kBlock + lane, ra = 2 * pp;
-        av[t] = bv[t] = make_float2(0.0f, 0.0f);
-        if (e < total && k <= hy) {
-            if (ra < rows_live) av[t] = h[half_at(xg, yb, x0 + ra, lane)];
-            if (ra + 1 < rows_live) bv[t] = h[half_at(xg, yb, x0 + ra + 1, lane)];
-        }
+        const bool live = e < total && k <= hy && (!blk || blk[yb < xg.nyb_h ? yb : 0] != 0);
+        const uint32_t oa = static_cast<uint32_t>(half_at(xg, yb, x0 + ra, lane) * sizeof(float2));
+        av[t] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                               rs, live && ra < rows_live ? oa : kBufferOut, 0, 0));
+        bv[t] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                               rs, live && ra + 1 < rows_live ? oa + kBlock * sizeof(float2) : kBufferOut,
+                                               0, 0));
     }
 #pragma unroll
     for (int t = 0; t < EL; ++t) {
@@ -684,6 +697,7 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.sinc_x = g.wproject ? g.d_sinc_x.as<float>() : nullptr;
     xg.sinc_y = g.wproject ? g.d_sinc_y.as<float>() : nullptr;
     xg.touch = nullptr;
+    xg.touch_blk = nullptr;
     xg.nxc = (xg.gx + kTouchRows - 1) / kTouchRows;
     if (xg.nxc > 128) return false;
     p.n_chan = static_cast<uint32_t>(n_chan);

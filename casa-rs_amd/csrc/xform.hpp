@@ -61,7 +61,10 @@ struct XGeom {
     // [x, x + kTouchRows) of upper block yb; nullptr = every row.  The column
     // passes store (forward) and load (inverse) only those rows: the model
     // grid elsewhere is never read, the residual grid elsewhere is zero.
-    const uint8_t *touch;
+    // touch_blk[c * nyb_h + yb] != 0 when any row of block yb is covered: the
+    // passes skip the other blocks (their columns are never read, their
+    // residual is zero).
+    const uint8_t *touch, *touch_blk;
     uint32_t nxc;       // ceil(gx / kTouchRows) <= 128
 };
 
