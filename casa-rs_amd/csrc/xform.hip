@@ -314,6 +314,28 @@ __device__ __forceinline__ bool row_touched(const RowMask &m, uint32_t x) {
     return ((ch < 64 ? m.lo >> ch : m.hi >> (ch - 64)) & 1ull) != 0;
 }
 
+// Block flags of one channel (touch_blk) as three 64-bit words (nyb_h <= 129),
+// gathered by a wavefront ballot; every block without a mask.
+struct BlockMask {
+    uint64_t w[3];
+};
+__device__ __forceinline__ BlockMask block_mask(const XGeom &xg, uint32_t c) {
+    if (!xg.touch_blk) return {{~0ull, ~0ull, ~0ull}};
+    const uint8_t *t = xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint8_t f[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) f[q] = t[lane + 64 * q < xg.nyb_h ? lane + 64 * q : 0];
+    BlockMask m;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) m.w[q] = __ballot(lane + 64 * q < xg.nyb_h && f[q] != 0);
+    return m;
+}
+__device__ __forceinline__ bool block_live(const BlockMask &m, uint32_t yb) {
+    const uint32_t q = yb >> 6;
+    return q < 3 && ((m.w[q] >> (yb & 63u)) & 1ull) != 0;
+}
+
 __device__ __forceinline__ uint64_t half_at(const XGeom &xg, uint32_t yb, uint32_t x, uint32_t lane) {
     return static_cast<uint64_t>(yb) * xg.nxp * kBlock + x * kBlock + lane;
 }
@@ -379,10 +401,10 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     // pass skips them)
     float2 *h = half + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
-    const uint8_t *blk = xg.touch_blk ? xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h : nullptr;
+    const BlockMask bm = block_mask(xg, c);
     for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += kRowThreads) {
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
-        if (blk && !blk[yb]) continue;
+        if (!block_live(bm, yb)) continue;
         const uint32_t k = yb * kBlock + lane;
         float2 A = make_float2(0.0f, 0.0f), B = A;
         if (k <= hy) {
@@ -499,14 +521,14 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     // misses were not transformed and read as zero.
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float2 *>(h), 0, static_cast<int>(xg.hstride * sizeof(float2)), kBufferFlags);
-    const uint8_t *blk = xg.touch_blk ? xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h : nullptr;
+    const BlockMask bm = block_mask(xg, c);
     float2 av[EL], bv[EL];
 #pragma unroll
     for (int t = 0; t < EL; ++t) {
         const uint32_t e = tid + static_cast<uint32_t>(t) * kRowThreads;
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
         const uint32_t k = yb * kBlock + lane, ra = 2 * pp;
-        const bool live = e < total && k <= hy && (!blk || blk[yb < xg.nyb_h ? yb : 0] != 0);
+        const bool live = e < total && k <= hy && block_live(bm, yb);
         const uint32_t oa = static_cast<uint32_t>(half_at(xg, yb, x0 + ra, lane) * sizeof(float2));
         av[t] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
                                                rs, live && ra < rows_live ? oa : kBufferOut, 0, 0));
@@ -526,22 +548,30 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     }
     __syncthreads();
     run_fft<true, E, false, kLanes>(buf, ay, n, p, jt);
+    // Output v = Re * scale * sign * (cx cy) / ns / pk (gridder.rs:1646-1657,
+    // lib.rs:68928-68929), with W projection v = 2 Re * scale * sign * (cx cy)
+    // / (sx sy) / ns / pk where |sx sy| > 1e-6, else 0 (gridder.rs:1685-1715,
+    // lib.rs:70960-70961).  The divisions are folded into per-row, per-column
+    // and per-channel factors (cx / sx, cy / sy, gain / (ns pk)), so a pixel
+    // costs a few products; this moves the result by a few ulp.
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(xg.gx) * xg.gy);
-    const float ns = nsum[c], pk = peak[c];
-    // y corrections of this thread's columns, loaded together (ny <= n <= E kLanes)
-    float cyv[E], syv[E];
+    const float chan_gain = (xg.wproject ? 2.0f : 1.0f) / nsum[c] / peak[c];
+    // y factors of this thread's columns, loaded together (ny <= n <= E kLanes)
+    float ryv[E], syv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
         const uint32_t y = jt + static_cast<uint32_t>(t) * kLanes, gyy = xg.blc_y + y;
-        cyv[t] = y < xg.ny ? xg.corr_y[gyy] : 0.0f;
+        const float cy = y < xg.ny ? xg.corr_y[gyy] : 0.0f;
         syv[t] = y < xg.ny && xg.wproject ? xg.sinc[gyy] : 1.0f;
+        ryv[t] = xg.wproject ? cy / syv[t] : cy;
     }
     for (int q = 0; q < 2; ++q) {
         const uint32_t x = x0 + 2 * p + q;
         if (x >= xg.nx) break;
         const uint32_t gxx = xg.blc_x + x;
-        const float cx = xg.corr_x[gxx];
         const float sincx = xg.wproject ? xg.sinc[gxx] : 1.0f;
+        const float rx = xg.wproject ? xg.corr_x[gxx] / sincx : xg.corr_x[gxx];
+        const float row_gain = scale * rx * chan_gain;
         float *o = out + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
 #pragma unroll
         for (int t = 0; t < E; ++t) {
@@ -551,18 +581,9 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
             uint32_t i = gyy + n / 2;
             if (i >= n) i -= n;
             const float2 z = buf[p * n + i];
-            float re = (q == 0 ? z.x : z.y) * scale;
-            if ((gxx + gyy + xg.base) & 1) re = -re;
-            const float factor = cx * cyv[t];
-            float v;
-            if (!xg.wproject) {
-                v = re * factor;  // gridder.rs:1646-1657
-                v = v / ns / pk;  // lib.rs:68928-68929
-            } else {
-                const float sf = sincx * syv[t];
-                v = fabsf(sf) > 1.0e-6f ? re * (factor / sf) : 0.0f;  // gridder.rs:1685-1715
-                v = 2.0f * v / ns / pk;                                // lib.rs:70960-70961
-            }
+            float v = (q == 0 ? z.x : z.y) * row_gain * ryv[t];
+            if ((gxx + gyy + xg.base) & 1) v = -v;
+            if (xg.wproject && !(fabsf(sincx * syv[t]) > 1.0e-6f)) v = 0.0f;
             o[y] = v;
         }
     }
@@ -699,7 +720,7 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.touch = nullptr;
     xg.touch_blk = nullptr;
     xg.nxc = (xg.gx + kTouchRows - 1) / kTouchRows;
-    if (xg.nxc > 128) return false;
+    if (xg.nxc > 128 || xg.nyb_h > 192) return false;
     p.n_chan = static_cast<uint32_t>(n_chan);
     p.tw_x.alloc(tx.size() * sizeof(float2));
     p.tw_y.alloc(ty.size() * sizeof(float2));
