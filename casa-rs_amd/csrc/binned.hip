@@ -335,8 +335,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 else add(plan[j].y, wt[j], vis[j]);
             }
             if (count > 1) {
-                vr = wave_sum_f(vr);
-                vi = wave_sum_f(vi);
+                wave_sum_f2(vr, vi);
                 if (MODE != kModeAdjoint) w = wave_sum_f(w);
             } else {
                 vr = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vr)));
@@ -398,8 +397,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                         }
                     }
                 }
-                pr = wave_sum_f(pr);
-                pi = wave_sum_f(pi);
+                wave_sum_f2(pr, pi);
                 if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
                 if (MODE == kModeDegrid) {
                     write_degrid(pr, pi);
@@ -456,8 +454,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             }
         }
         if (kModel) {
-            pr = wave_sum_f(pr);
-            pi = wave_sum_f(pi);
+            wave_sum_f2(pr, pi);
             if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
             if (MODE == kModeDegrid) {
                 write_degrid(pr, pi);

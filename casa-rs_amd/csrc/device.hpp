@@ -341,22 +341,45 @@ void wk_fill(chip_gridder &g, WkScratch &ws, uint32_t lo, uint32_t hi, GridParam
 // recomputed when they are not resident.
 void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev);
 
-// Wavefront-wide sum of a float (64 lanes, all active); every lane receives
-// the same value.  DPP lane swaps inside each row of 16 (xor 1, xor 2,
+// Wavefront-wide sum of a float (64 lanes, all active), returned as a
+// wavefront-uniform value.  DPP lane swaps inside each row of 16 (xor 1, xor 2,
 // half-row mirror, row mirror) leave the row sum in every lane of the row;
-// the four row sums are then read into scalars and added.
+// two row broadcasts then carry the row sums into the last row, whose lane
+// 63 holds the total.
 #define CHIP_DPP_ADD(v, ctrl) \
-    ((v) + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), (ctrl), 0xF, 0xF, false)))
+    ((v) + __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), (ctrl), 0xF, 0xF, true)))
+// row_bcast15 / row_bcast31 into the rows of row_mask; the other rows add
+// their own value (their result is not used: only lane 63 is read)
+#define CHIP_DPP_ADD_ROWS(v, ctrl, rows) \
+    ((v) + __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), (ctrl), (rows), 0xF, false)))
 __device__ __forceinline__ float wave_sum_f(float v) {
-    v = CHIP_DPP_ADD(v, 0xB1);   // quad_perm [1,0,3,2]
-    v = CHIP_DPP_ADD(v, 0x4E);   // quad_perm [2,3,0,1]
-    v = CHIP_DPP_ADD(v, 0x141);  // row_half_mirror
-    v = CHIP_DPP_ADD(v, 0x140);  // row_mirror
-    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
-    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
-    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
-    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return (r0 + r1) + (r2 + r3);
+    v = CHIP_DPP_ADD(v, 0xB1);             // quad_perm [1,0,3,2]
+    v = CHIP_DPP_ADD(v, 0x4E);             // quad_perm [2,3,0,1]
+    v = CHIP_DPP_ADD(v, 0x141);            // row_half_mirror
+    v = CHIP_DPP_ADD(v, 0x140);            // row_mirror: every lane holds its row's sum
+    v = CHIP_DPP_ADD_ROWS(v, 0x142, 0xA);  // row_bcast15: rows 1, 3 hold rows 0+1, 2+3
+    v = CHIP_DPP_ADD_ROWS(v, 0x143, 0x8);  // row_bcast31: row 3 holds the total
+    // (lanes of rows 0 - 2 now hold partial values; lane 63 is broadcast)
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+// Two wavefront sums with their steps interleaved (no DPP hazard waits
+// between the dependent steps of one sum).
+__device__ __forceinline__ void wave_sum_f2(float &a, float &b) {
+    a = CHIP_DPP_ADD(a, 0xB1);
+    b = CHIP_DPP_ADD(b, 0xB1);
+    a = CHIP_DPP_ADD(a, 0x4E);
+    b = CHIP_DPP_ADD(b, 0x4E);
+    a = CHIP_DPP_ADD(a, 0x141);
+    b = CHIP_DPP_ADD(b, 0x141);
+    a = CHIP_DPP_ADD(a, 0x140);
+    b = CHIP_DPP_ADD(b, 0x140);
+    a = CHIP_DPP_ADD_ROWS(a, 0x142, 0xA);
+    b = CHIP_DPP_ADD_ROWS(b, 0x142, 0xA);
+    a = CHIP_DPP_ADD_ROWS(a, 0x143, 0x8);
+    b = CHIP_DPP_ADD_ROWS(b, 0x143, 0x8);
+    a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
+    b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
 }
 
 // Wavefront-wide sum of a double (64 lanes).
