@@ -64,6 +64,7 @@ EXPORTS = (
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
     "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_aw_cells", "chip_aw_plan",
     "chip_airy_table", "chip_hogbom",
+    "chip_ingest_create", "chip_ingest_destroy", "chip_ingest_push", "chip_ingest_samples", "chip_ingest_download",
 )
 
 
@@ -150,6 +151,25 @@ class CfCell(ctypes.Structure):
     ]
 
 
+# row-stream ingestion: polarization and pair collapse (include/casa_imaging_hip.h)
+POL_EXPLICIT = 0
+POL_PAIR = 1
+PAIR_HALF_SUM = 0
+PAIR_HALF_DIFFERENCE = 1
+PAIR_POSITIVE_HALF_IMAG_DIFF = 2
+PAIR_NEGATIVE_HALF_IMAG_DIFF = 3
+
+
+class IngestParams(ctypes.Structure):
+    _fields_ = [
+        ("n_corr", ctypes.c_uint32), ("n_chan", ctypes.c_uint32), ("polarization", ctypes.c_int32),
+        ("corr", ctypes.c_uint32 * 2), ("transform", ctypes.c_int32), ("sumwt_factor", ctypes.c_float),
+        ("allow_psf_only", ctypes.c_int32), ("weight_spectrum", ctypes.c_int32), ("rotate", ctypes.c_int32),
+        ("uvrot", ctypes.c_double * 9), ("phrot", ctypes.c_double * 3),
+        ("max_rows", ctypes.c_uint64), ("block_rows", ctypes.c_uint64),
+    ]
+
+
 class HogbomParams(ctypes.Structure):
     _fields_ = [("gain", ctypes.c_float), ("threshold", ctypes.c_float), ("budget", ctypes.c_uint32)]
 
@@ -202,6 +222,11 @@ def load():
         "chip_density_weights": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, ctypes.c_float, _U32, _P, _P,
                                                 _P, _PP]),
         "chip_vset_create": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _PP, _PP]),
+        "chip_ingest_create": (ctypes.c_int, [_P, ctypes.POINTER(IngestParams), _P, _PP, _PP]),
+        "chip_ingest_destroy": (None, [_P]),
+        "chip_ingest_push": (ctypes.c_int, [_P, _U64, _P, _P, _P, _P, _P, _PP]),
+        "chip_ingest_samples": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _PP]),
+        "chip_ingest_download": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _PP]),
         "chip_vset_destroy": (None, [_P]),
         "chip_vset_sums": (ctypes.c_int, [_P, _P, _PP]),
         "chip_vset_stats": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _PP]),

@@ -14,6 +14,11 @@ void vset_grid_values(chip_vset &vs, const float *values, float *grid);
 void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, const float *peaks,
                       float *out);
 void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], double *transform_ms);
+void ingest_create(chip_ingest &ing, const chip_ingest_params &p, const double *freq_hz);
+void ingest_push(chip_ingest &ing, uint64_t n_rows, const double *uvw, const float *data, const uint8_t *flag,
+                 const float *weight, const uint8_t *row_flag);
+void ingest_samples(chip_ingest &ing, chip_samples &out);
+void ingest_download(chip_ingest &ing, double *uvw, float *vis, float *weight, float *sumwt_factor, uint8_t *flag);
 void vset_cycle_strategy(chip_vset &vs, int strategy);
 void vset_cycle_graph(chip_vset &vs, int enable);
 void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi);
@@ -313,6 +318,51 @@ int chip_rotate_rows(chip_ctx *ctx, uint64_t n_rows, uint64_t n_chan, const doub
         set_device(ctx);
         rotate_rows(*ctx, n_rows, n_chan, uvrot, phrot, uvw_m_dev, freq_hz_dev, vis_dev, uvw_out_dev,
                     phase_shift_m_dev);
+    });
+}
+
+int chip_ingest_create(chip_ctx *ctx, const chip_ingest_params *p, const double *freq_hz, chip_ingest **out,
+                       char **err) {
+    return guarded(err, [&] {
+        if (!ctx || !p || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        std::unique_ptr<chip_ingest> ing(new chip_ingest);
+        ing->ctx = ctx;
+        ing->device = ctx->device;
+        ingest_create(*ing, *p, freq_hz);
+        *out = ing.release();
+    });
+}
+
+void chip_ingest_destroy(chip_ingest *ing) {
+    if (!ing) return;
+    (void)hipSetDevice(ing->device);
+    delete ing;
+}
+
+int chip_ingest_push(chip_ingest *ing, uint64_t n_rows, const double *uvw_m, const float *data,
+                     const uint8_t *flag, const float *weight, const uint8_t *row_flag, char **err) {
+    return guarded(err, [&] {
+        if (!ing) fail(CHIP_ERR_INVALID, "NULL ingest");
+        set_device(ing->ctx);
+        ingest_push(*ing, n_rows, uvw_m, data, flag, weight, row_flag);
+    });
+}
+
+int chip_ingest_samples(chip_ingest *ing, chip_samples *out, char **err) {
+    return guarded(err, [&] {
+        if (!ing || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ing->ctx);
+        ingest_samples(*ing, *out);
+    });
+}
+
+int chip_ingest_download(chip_ingest *ing, double *uvw_m, float *vis, float *weight, float *sumwt_factor,
+                         uint8_t *flag, char **err) {
+    return guarded(err, [&] {
+        if (!ing) fail(CHIP_ERR_INVALID, "NULL ingest");
+        set_device(ing->ctx);
+        ingest_download(*ing, uvw_m, vis, weight, sumwt_factor, flag);
     });
 }
 

@@ -393,4 +393,25 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
+// Phase-centre rotation of one row (casars-imager/src/lib.rs:47064-47069):
+// uvw' = uvw * uvrot as a row vector, shift = phrot . uvw', FP64 in this
+// order without contraction.
+__device__ __forceinline__ void rotate_uvw_row(const double *uvrot, const double *phrot, const double x[3],
+                                               double o[3], double &shift) {
+#pragma clang fp contract(off)
+    for (int k = 0; k < 3; ++k) o[k] = x[0] * uvrot[k] + x[1] * uvrot[3 + k] + x[2] * uvrot[6 + k];
+    shift = phrot[0] * o[0] + phrot[1] * o[1] + phrot[2] * o[2];
+}
+
+// phase_rotate_visibility (casa-imaging lib.rs:248-259): phi = -2 pi shift
+// f / c in FP64, phasor (cos, sin) narrowed to Float, Float complex product;
+// unchanged when shift or f is 0.
+__device__ __forceinline__ float2 phase_rotate(float2 v, double shift, double f) {
+#pragma clang fp contract(off)
+    if (shift == 0.0 || f == 0.0) return v;
+    const double phase = -6.28318530717958647692 * shift * f / 299792458.0;
+    const float pr = static_cast<float>(cos(phase)), pi = static_cast<float>(sin(phase));
+    return make_float2(v.x * pr - v.y * pi, v.x * pi + v.y * pr);
+}
+
 }  // namespace chip

@@ -258,6 +258,17 @@ std::vector<float> airy_table(double dish, double blockage);
 void rotate_rows(chip_ctx &ctx, uint64_t n_rows, uint64_t n_chan, const double uvrot[9], const double phrot[3],
                  const double *uvw, const double *freq, float *vis, double *uvw_out, double *shift_out);
 
+// Row-stream ingestion (chip_ingest_*): the resident rows x channels
+// stream and two staging slots (pinned host + device) of block_rows rows.
+struct IngestSlot {
+    void *host = nullptr;        // pinned
+    DevBuf dev;
+    hipEvent_t copied = nullptr;    // the H2D copy from `host` completed
+    hipEvent_t consumed = nullptr;  // the conversion from `dev` completed
+    bool used = false;
+};
+void ingest_destroy_slots(IngestSlot *slots, int n);
+
 // Largest dynamic LDS one workgroup may use on `device` (160 KB on gfx950).
 inline size_t lds_limit(int device) {
     int v = 0;
@@ -277,6 +288,20 @@ struct CycleWorkspace;
 }  // namespace chip
 
 // Resident, binned visibility stream.
+struct chip_ingest {
+    chip_ctx *ctx = nullptr;
+    int device = 0;
+    chip_ingest_params p{};
+    uint64_t rows = 0;              // rows appended so far
+    chip::DevBuf lscale;            // n_chan freq / c
+    chip::DevBuf freq;              // n_chan Hz
+    chip::DevBuf uvw, vis, wt, sf, flag;
+    chip::IngestSlot slot[2];
+    int next = 0;
+    hipStream_t copy = nullptr;     // H2D copies (default priority)
+    ~chip_ingest();
+};
+
 struct chip_vset {
     chip_gridder *g = nullptr;
     int device = 0;  // kept so that destruction never dereferences g

@@ -20,13 +20,12 @@ struct RotParams {
 // One thread per row; uvw_out may alias uvw.
 __global__ __launch_bounds__(256) void k_rotate_uvw(RotParams p, const double *uvw, double *uvw_out,
                                                     double *shift) {
-#pragma clang fp contract(off)
     const uint64_t r = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= p.n_rows) return;
-    const double x0 = uvw[3 * r], x1 = uvw[3 * r + 1], x2 = uvw[3 * r + 2];
-    double o[3];
-    for (int k = 0; k < 3; ++k) o[k] = x0 * p.uvrot[k] + x1 * p.uvrot[3 + k] + x2 * p.uvrot[6 + k];
-    shift[r] = p.phrot[0] * o[0] + p.phrot[1] * o[1] + p.phrot[2] * o[2];
+    const double x[3] = {uvw[3 * r], uvw[3 * r + 1], uvw[3 * r + 2]};
+    double o[3], s;
+    rotate_uvw_row(p.uvrot, p.phrot, x, o, s);
+    shift[r] = s;
     if (uvw_out) {
         uvw_out[3 * r] = o[0];
         uvw_out[3 * r + 1] = o[1];
@@ -37,16 +36,10 @@ __global__ __launch_bounds__(256) void k_rotate_uvw(RotParams p, const double *u
 // One thread per sample (row, chan), channel fastest.
 __global__ __launch_bounds__(256) void k_rotate_vis(RotParams p, const double *shift, const double *freq,
                                                     float2 *vis) {
-#pragma clang fp contract(off)
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= p.n_rows * p.n_chan) return;
     const uint64_t r = i / p.n_chan, c = i - r * p.n_chan;
-    const double s = shift[r], f = freq[c];
-    if (s == 0.0 || f == 0.0) return;
-    const double phase = -6.28318530717958647692 * s * f / 299792458.0;
-    const float pr = static_cast<float>(cos(phase)), pi = static_cast<float>(sin(phase));
-    const float2 v = vis[i];
-    vis[i] = make_float2(v.x * pr - v.y * pi, v.x * pi + v.y * pr);
+    vis[i] = phase_rotate(vis[i], shift[r], freq[c]);
 }
 
 unsigned blocks_of(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }

@@ -257,6 +257,118 @@ class RowChannelBatch:
         return s, keep
 
 
+_PAIR_TRANSFORMS = {
+    "half_sum": abi.PAIR_HALF_SUM, "half_difference": abi.PAIR_HALF_DIFFERENCE,
+    "positive_half_imag_difference": abi.PAIR_POSITIVE_HALF_IMAG_DIFF,
+    "negative_half_imag_difference": abi.PAIR_NEGATIVE_HALF_IMAG_DIFF,
+}
+
+
+class RowIngest:
+    """Row-stream ingestion on the GPU (chip_ingest_*): host row blocks as
+    casars-imager reads them (uvw metres; DATA / FLAG [corr][chan]; WEIGHT per
+    corr or WEIGHT_SPECTRUM per (corr, chan); FLAG_ROW) become the rows x
+    channels samples of one imaging plane, behind two pinned staging buffers
+    so that reading block k + 1 overlaps the copy and conversion of block k.
+
+    ``corr``: the explicit correlation (StandardMfsVisibilityPolarization::
+    Explicit) or, with ``pair`` set to a StandardMfsPairCollapseTransform name
+    ("half_sum", "half_difference", "positive_half_imag_difference",
+    "negative_half_imag_difference"), the two correlations of a CollapsedPair
+    (execution.rs:2168-2315, types.rs:1593-1631).  ``rotation``: (uvrot,
+    phrot) from ``phase_rotation`` (each hand rotated before the collapse).
+    The object is itself a batch: ``VisibilitySet(gridder, ingest)`` and the
+    gridder methods accept it once every block is pushed."""
+
+    def __init__(self, n_corr, freq_hz, *, corr=(0,), pair=None, sumwt_factor=1.0, allow_psf_only=True,
+                 weight_spectrum=False, rotation=None, max_rows, block_rows, ctx: "Context | None" = None):
+        self.ctx = ctx or default_context()
+        f = np.ascontiguousarray(freq_hz, np.float64)
+        p = abi.IngestParams()
+        p.n_corr = int(n_corr)
+        p.n_chan = f.shape[0]
+        corr = tuple(int(c) for c in np.atleast_1d(corr))
+        if pair is None:
+            p.polarization = abi.POL_EXPLICIT
+            p.corr[0] = corr[0]
+        else:
+            if pair not in _PAIR_TRANSFORMS or len(corr) != 2:
+                raise abi.InvalidRequest("a pair needs two correlations and a known collapse transform")
+            p.polarization = abi.POL_PAIR
+            p.corr[0], p.corr[1] = corr
+            p.transform = _PAIR_TRANSFORMS[pair]
+        p.sumwt_factor = float(sumwt_factor)
+        p.allow_psf_only = 1 if allow_psf_only else 0
+        p.weight_spectrum = 1 if weight_spectrum else 0
+        if rotation is not None:
+            p.rotate = 1
+            rot, ph = (np.ascontiguousarray(a, np.float64).ravel() for a in rotation)
+            for k in range(9):
+                p.uvrot[k] = rot[k]
+            for k in range(3):
+                p.phrot[k] = ph[k]
+        p.max_rows = int(max_rows)
+        p.block_rows = int(block_rows)
+        self.n_corr, self.n_chan, self.weight_spectrum = p.n_corr, p.n_chan, bool(weight_spectrum)
+        self.freq_hz = f
+        h = ctypes.c_void_p()
+        abi.call("chip_ingest_create", self.ctx.handle, ctypes.byref(p), f.ctypes.data_as(ctypes.c_void_p),
+                 ctypes.byref(h))
+        self._h = h
+        self.n_rows = 0
+
+    def push(self, uvw_m, data, flag, weight, row_flag=None):
+        """Append one block of host rows (numpy): uvw_m (rows, 3) float64,
+        data (rows, corr, chan) complex64, flag (rows, corr, chan) bool,
+        weight (rows, corr) or (rows, corr, chan) float32, row_flag (rows,)."""
+        uvw = np.ascontiguousarray(uvw_m, np.float64)
+        n = uvw.shape[0]
+        d = np.ascontiguousarray(data, np.complex64)
+        fl = np.ascontiguousarray(flag, np.uint8)
+        w = np.ascontiguousarray(weight, np.float32)
+        shape = (n, self.n_corr, self.n_chan)
+        wshape = shape if self.weight_spectrum else (n, self.n_corr)
+        if uvw.shape != (n, 3) or d.shape != shape or fl.shape != shape or w.shape != wshape:
+            raise abi.InvalidRequest("block arrays do not match (rows, corr, chan)")
+        rf = None if row_flag is None else np.ascontiguousarray(row_flag, np.uint8)
+        if rf is not None and rf.shape != (n,):
+            raise abi.InvalidRequest("row_flag must have one value per row")
+        vp = ctypes.c_void_p
+        abi.call("chip_ingest_push", self._h, n, uvw.ctypes.data_as(vp), d.ctypes.data_as(vp),
+                 fl.ctypes.data_as(vp), w.ctypes.data_as(vp), None if rf is None else rf.ctypes.data_as(vp))
+        self.n_rows += n
+
+    def to_samples(self):
+        s = abi.Samples()
+        abi.call("chip_ingest_samples", self._h, ctypes.byref(s))
+        return s, [self]
+
+    def download(self):
+        """Host copies: uvw_m (rows, 3), visibility (rows, chan) complex64,
+        weight, sumwt_factor (rows, chan) float32, flag (rows, chan) bool."""
+        n, c = self.n_rows, self.n_chan
+        uvw = np.empty((n, 3), np.float64)
+        vis = np.empty((n, c), np.complex64)
+        wt = np.empty((n, c), np.float32)
+        sf = np.empty((n, c), np.float32)
+        fl = np.empty((n, c), np.uint8)
+        vp = ctypes.c_void_p
+        abi.call("chip_ingest_download", self._h, uvw.ctypes.data_as(vp), vis.ctypes.data_as(vp),
+                 wt.ctypes.data_as(vp), sf.ctypes.data_as(vp), fl.ctypes.data_as(vp))
+        return {"uvw_m": uvw, "visibility": vis, "weight": wt, "sumwt_factor": sf, "flag": fl.astype(bool)}
+
+    def close(self):
+        if self._h:
+            abi.load().chip_ingest_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class StandardGridder:
     """StandardGridder::new_with_padding_factor (gridder.rs:392-443) on the GPU.
 

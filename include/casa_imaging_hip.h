@@ -73,6 +73,7 @@ extern "C" {
 typedef struct chip_ctx chip_ctx;
 typedef struct chip_gridder chip_gridder;
 typedef struct chip_vset chip_vset;
+typedef struct chip_ingest chip_ingest;
 
 /* ImageGeometry (types.rs:885-925) plus the gridder's padding rule and tap-table
  * oversampling (the reference fixes 100, gridder.rs:407). */
@@ -285,6 +286,79 @@ int chip_phase_rotation(const double *source_dir, const double *target_dir, doub
 int chip_rotate_rows(chip_ctx *ctx, uint64_t n_rows, uint64_t n_chan, const double *uvrot, const double *phrot,
                      const double *uvw_m_dev, const double *freq_hz_dev, float *vis_dev, double *uvw_out_dev,
                      double *phase_shift_m_dev, char **err);
+
+/* ---- row-stream ingestion ------------------------------------------------ */
+/* Host row blocks (MeasurementSet rows as casars-imager reads them: uvw in
+ * metres, DATA / FLAG as [corr][chan], WEIGHT per corr or WEIGHT_SPECTRUM per
+ * (corr, chan), FLAG_ROW) turned on the GPU into the rows x channels samples
+ * of one imaging plane, with double-buffered host-to-device copies: the
+ * blocks the caller pushes go through two pinned staging buffers, so the
+ * caller reads block k + 1 while block k is copied and converted (the
+ * reference's two-live-block read-ahead, ARCHITECTURE.md:461-475).
+ * Per (row, chan) of a block (execution.rs:2168-2315):
+ *  - polarization CHIP_POL_EXPLICIT (StandardMfsVisibilityPolarization::
+ *    Explicit, types.rs:1611-1619): sample = corr[0]; flagged -> excluded;
+ *    a non-finite visibility adds to the PSF only when allow_psf_only, else
+ *    is excluded; weight = WEIGHT_SPECTRUM or WEIGHT of the corr;
+ *  - CHIP_POL_PAIR (CollapsedPair, types.rs:1620-1631): either corr flagged
+ *    -> excluded; visibility = collapse(v0, v1) (StandardMfsPairCollapse
+ *    Transform::collapse, types.rs:1593-1610, Float complex arithmetic);
+ *    non-finite -> excluded; both weights finite and > 0 or excluded;
+ *    weight = 0.5 (w0 + w1);
+ *  - FLAG_ROW set -> every sample of the row excluded;
+ *  - rotate != 0: uvw' = uvw * uvrot and each corr's visibility phase-rotated
+ *    before the collapse (chip_rotate_rows' arithmetic; casars-imager
+ *    lib.rs:19207-19211 rotates both hands of a pair);
+ *  - sumwt_factor on every sample; lambda scale freq / c per channel
+ *    (casars-imager/src/lib.rs:557).
+ * Excluded samples keep flag = 1 in the output. */
+#define CHIP_POL_EXPLICIT 0
+#define CHIP_POL_PAIR 1
+#define CHIP_PAIR_HALF_SUM 0                   /* (v0 + v1) * 0.5 */
+#define CHIP_PAIR_HALF_DIFFERENCE 1            /* (v0 - v1) * 0.5 */
+#define CHIP_PAIR_POSITIVE_HALF_IMAG_DIFF 2    /* (v0 - v1) * (0 + 0.5i) */
+#define CHIP_PAIR_NEGATIVE_HALF_IMAG_DIFF 3    /* (v0 - v1) * (0 - 0.5i) */
+
+typedef struct {
+    uint32_t n_corr;          /* correlations per row in the blocks */
+    uint32_t n_chan;          /* channels per row */
+    int32_t polarization;     /* CHIP_POL_* */
+    uint32_t corr[2];         /* explicit: corr[0]; pair: corr[0], corr[1] */
+    int32_t transform;        /* CHIP_PAIR_* (pair only) */
+    float sumwt_factor;       /* reported-sumwt multiplicity of every sample */
+    int32_t allow_psf_only;   /* explicit: non-finite visibilities still add to the PSF */
+    int32_t weight_spectrum;  /* weights per (corr, chan) instead of per corr */
+    int32_t rotate;           /* apply uvrot / phrot (chip_phase_rotation) */
+    double uvrot[9];
+    double phrot[3];
+    uint64_t max_rows;        /* capacity of the resident row stream */
+    uint64_t block_rows;      /* largest block one push accepts */
+} chip_ingest_params;
+
+/* freq_hz: host, n_chan values.  Allocates the resident stream (max_rows x
+ * n_chan samples) and two pinned staging buffers of block_rows rows. */
+int chip_ingest_create(chip_ctx *ctx, const chip_ingest_params *p, const double *freq_hz, chip_ingest **out,
+                       char **err);
+void chip_ingest_destroy(chip_ingest *ing);
+/* Host arrays of one block of n_rows <= block_rows rows: uvw_m [row][3],
+ * data complex64 [row][corr][chan], flag [row][corr][chan] (nonzero =
+ * flagged), weight [row][corr] or [row][corr][chan], row_flag [row] (NULL =
+ * none).  Returns once the block is in a staging buffer and its copy and
+ * conversion are queued (it waits only for the copy of block k - 2); the
+ * rows are appended after those of earlier pushes.  Exceeding max_rows is an
+ * error (CHIP_ERR_INVALID) and appends nothing.  No reference counterpart as
+ * a function (the reference converts rows on the CPU). */
+int chip_ingest_push(chip_ingest *ing, uint64_t n_rows, const double *uvw_m, const float *data,
+                     const uint8_t *flag, const float *weight, const uint8_t *row_flag, char **err);
+/* Waits for every pushed block and describes the resident stream as rows x
+ * channels samples (chip_samples: uvw_m, lambda_scale, vis, weight,
+ * sumwt_factor, flag); the pointers stay valid until the next push or
+ * chip_ingest_destroy. */
+int chip_ingest_samples(chip_ingest *ing, chip_samples *out, char **err);
+/* Host copies of the resident stream (any pointer may be NULL): uvw_m
+ * [rows][3], vis complex64 / weight / sumwt_factor / flag [rows][chan]. */
+int chip_ingest_download(chip_ingest *ing, double *uvw_m, float *vis, float *weight, float *sumwt_factor,
+                         uint8_t *flag, char **err);
 
 /* ---- density weighting -------------------------------------------------- */
 /* Uniform / Briggs reweighting from a density grid of image size

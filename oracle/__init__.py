@@ -820,6 +820,44 @@ def rotate_rows(uvrot, phrot, uvw_m, freq_hz, vis):
     return out, shift, v
 
 
+PAIR_TRANSFORMS = {"half_sum": 0, "half_difference": 1, "positive_half_imag_difference": 2,
+                   "negative_half_imag_difference": 3}
+
+
+def ingest_rows(uvw_m, data, flag, weight, freq_hz, corr=(0,), pair=None, sumwt_factor=1.0,
+                allow_psf_only=True, row_flag=None, rotation=None):
+    """Rows x channels samples of one plane from row blocks (co_ingest_rows:
+    execution.rs:2168-2315, types.rs:1593-1631).  data / flag (rows, corr,
+    chan); weight (rows, corr) or (rows, corr, chan).  Returns uvw (rows, 3),
+    visibility (rows, chan) complex64, weight, sumwt_factor, flag (bool)."""
+    uvw = np.ascontiguousarray(uvw_m, np.float64)
+    d = np.ascontiguousarray(data, np.complex64)
+    n, n_corr, n_chan = d.shape
+    fl = np.ascontiguousarray(flag, np.uint8)
+    w = np.ascontiguousarray(weight, np.float32)
+    spectrum = w.ndim == 3
+    f = np.ascontiguousarray(freq_hz, np.float64)
+    corr = tuple(int(c) for c in np.atleast_1d(corr))
+    c1 = corr[1] if pair is not None else 0
+    rf = None if row_flag is None else np.ascontiguousarray(row_flag, np.uint8)
+    rot = np.zeros(9) if rotation is None else np.ascontiguousarray(rotation[0], np.float64).ravel()
+    ph = np.zeros(3) if rotation is None else np.ascontiguousarray(rotation[1], np.float64).ravel()
+    uo = np.empty((n, 3), np.float64)
+    vo = np.empty((n, n_chan), np.complex64)
+    wo = np.empty((n, n_chan), np.float32)
+    so = np.empty((n, n_chan), np.float32)
+    fo = np.empty((n, n_chan), np.uint8)
+    L = lib()
+    L.co_ingest_rows(ctypes.c_uint64(n), ctypes.c_uint32(n_corr), ctypes.c_uint32(n_chan),
+                     ctypes.c_int(0 if pair is None else 1), ctypes.c_uint32(corr[0]), ctypes.c_uint32(c1),
+                     ctypes.c_int(PAIR_TRANSFORMS[pair] if pair is not None else 0), ctypes.c_float(sumwt_factor),
+                     ctypes.c_int(1 if allow_psf_only else 0), ctypes.c_int(1 if spectrum else 0),
+                     ctypes.c_int(0 if rotation is None else 1), _ptr(rot), _ptr(ph), _ptr(f), _ptr(uvw),
+                     _ptr(d.view(np.float32)), _ptr(fl), _ptr(w), None if rf is None else _ptr(rf), _ptr(uo),
+                     _ptr(vo.view(np.float32)), _ptr(wo), _ptr(so), _ptr(fo))
+    return {"uvw_m": uo, "visibility": vo, "weight": wo, "sumwt_factor": so, "flag": fo.astype(bool)}
+
+
 # ---------------------------------------------------------------------------
 # AWProject CF cells (gridder.rs:2157-2815) and the Airy A-term (lib.rs:57436-57605)
 # ---------------------------------------------------------------------------

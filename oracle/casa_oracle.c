@@ -996,6 +996,95 @@ void co_rotate_rows(uint64_t n_rows, uint64_t n_chan, const double *uvrot, const
     }
 }
 
+/* Row-stream ingestion of one imaging plane: the routed-sample builder
+ * (execution.rs:2168-2315) per (row, chan) of row blocks with DATA / FLAG
+ * [corr][chan] and WEIGHT [corr] or WEIGHT_SPECTRUM [corr][chan]:
+ * pol 0 (Explicit, types.rs:1611-1619): excluded when flagged, or when the
+ * visibility is non-finite and allow_psf_only == 0; weight of the corr;
+ * pol 1 (CollapsedPair, types.rs:1620-1631): excluded when either hand is
+ * flagged, the collapse (types.rs:1593-1610) is non-finite or either weight
+ * is not finite and positive; weight 0.5 (w0 + w1).  row_flag excludes the
+ * row.  rotate: uvw' and each hand's phase rotation as co_rotate_rows,
+ * before the collapse (casars-imager lib.rs:19207-19211).  Outputs per
+ * sample (row, chan): vis complex64, weight, sumwt_factor, flag (1 =
+ * excluded); uvw_out per row. */
+static void co_collapse(int t, const float *a, const float *b, float *out) {
+    if (t == 0) {
+        out[0] = (a[0] + b[0]) * 0.5f;
+        out[1] = (a[1] + b[1]) * 0.5f;
+        return;
+    }
+    const float dr = a[0] - b[0], di = a[1] - b[1];
+    if (t == 1) {
+        out[0] = dr * 0.5f;
+        out[1] = di * 0.5f;
+        return;
+    }
+    const float im = t == 2 ? 0.5f : -0.5f; /* Complex32::new(0, +-0.5), full complex product */
+    out[0] = dr * 0.0f - di * im;
+    out[1] = dr * im + di * 0.0f;
+}
+
+static void co_phase_rotate(float *v, double shift, double f) {
+    if (shift == 0.0 || f == 0.0) return;
+    const double phase = -6.28318530717958647692 * shift * f / 299792458.0;
+    const float pr = (float)cos(phase), pi = (float)sin(phase);
+    const float re = v[0] * pr - v[1] * pi, im = v[0] * pi + v[1] * pr;
+    v[0] = re;
+    v[1] = im;
+}
+
+void co_ingest_rows(uint64_t n_rows, uint32_t n_corr, uint32_t n_chan, int pol, uint32_t c0, uint32_t c1,
+                    int transform, float sumwt_factor, int allow_psf_only, int spectrum, int rotate,
+                    const double *uvrot, const double *phrot, const double *freq, const double *uvw,
+                    const float *data, const uint8_t *flag, const float *weight, const uint8_t *row_flag,
+                    double *uvw_out, float *vis_out, float *wt_out, float *sf_out, uint8_t *flag_out) {
+    const uint64_t per = (uint64_t)n_corr * n_chan;
+    for (uint64_t r = 0; r < n_rows; ++r) {
+        const double *x = uvw + 3 * r;
+        double o[3] = {x[0], x[1], x[2]}, shift = 0.0;
+        if (rotate) {
+            for (int k = 0; k < 3; ++k) o[k] = x[0] * uvrot[k] + x[1] * uvrot[3 + k] + x[2] * uvrot[6 + k];
+            shift = phrot[0] * o[0] + phrot[1] * o[1] + phrot[2] * o[2];
+        }
+        memcpy(uvw_out + 3 * r, o, sizeof(o));
+        for (uint32_t c = 0; c < n_chan; ++c) {
+            const uint64_t i0 = r * per + (uint64_t)c0 * n_chan + c, i1 = r * per + (uint64_t)c1 * n_chan + c;
+            const float w0 = spectrum ? weight[i0] : weight[r * n_corr + c0];
+            const float w1 = spectrum ? weight[i1] : weight[r * n_corr + c1];
+            int excluded = row_flag && row_flag[r];
+            float v[2];
+            float w;
+            if (pol == 0) {
+                excluded = excluded || flag[i0];
+                v[0] = data[2 * i0];
+                v[1] = data[2 * i0 + 1];
+                if (!(isfinite(v[0]) && isfinite(v[1])) && !allow_psf_only) excluded = 1;
+                if (rotate) co_phase_rotate(v, shift, freq[c]);
+                w = w0;
+            } else {
+                excluded = excluded || flag[i0] || flag[i1];
+                float a[2] = {data[2 * i0], data[2 * i0 + 1]}, b[2] = {data[2 * i1], data[2 * i1 + 1]};
+                if (rotate) {
+                    co_phase_rotate(a, shift, freq[c]);
+                    co_phase_rotate(b, shift, freq[c]);
+                }
+                co_collapse(transform, a, b, v);
+                if (!(isfinite(v[0]) && isfinite(v[1])) ||
+                    !(isfinite(w0) && w0 > 0.0f && isfinite(w1) && w1 > 0.0f))
+                    excluded = 1;
+                w = 0.5f * (w0 + w1);
+            }
+            const uint64_t o_i = r * n_chan + c;
+            vis_out[2 * o_i] = v[0];
+            vis_out[2 * o_i + 1] = v[1];
+            wt_out[o_i] = w;
+            sf_out[o_i] = sumwt_factor;
+            flag_out[o_i] = (uint8_t)(excluded ? 1 : 0);
+        }
+    }
+}
+
 /* ------------------------------------------------------------------------- */
 /* AWProject convolution-function cells and the Airy A-term                   */
 /* ------------------------------------------------------------------------- */
