@@ -431,6 +431,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-minor", action="store_true", help="skip the Hogbom minor-cycle timing")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--schedule", default="", metavar="TILE,ORDER,THREADS,GROUPS",
+                    help="fix the binned kernel's schedule instead of autotuning it (profiling runs)")
     ap.add_argument("--compare-hipfft", type=int, default=1,
                     help="also time the step with hipFFT 2-D plans (reported beside value)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -560,6 +562,10 @@ def main():
         kernel_ms.append(vs.stats()["last_kernel_ms"])
         return r32
 
+    if args.schedule:
+        # a fixed binned-kernel schedule (profiling runs: no autotuner launches)
+        tile, order, threads, gpi = (int(x) for x in args.schedule.split(","))
+        vs.set_schedule(threads, gpi, tile=tile, order=order)
     for _ in range(args.warmup):
         step32()
     info = vs.cycle_info()
@@ -578,9 +584,10 @@ def main():
         f"binned kernel {np.mean(kernel_ms):.3f} ms")
     sched = vs.schedule()
     log(f"binned schedule: {sched['tile']}^2 uv tiles, groups by {('anchor', 'plan')[sched['order']]}, "
-        f"{sched['threads']} threads x {sched['groups_per_item']} groups per item (autotuned over "
-        f"{len(sched['timed'])} candidates tile/order/threads x groups: "
-        + ", ".join(f"{tl}/{od}/{a}x{b} {ms:.3f} ms" for tl, od, a, b, ms in sched["timed"]) + ")")
+        f"{sched['threads']} threads x {sched['groups_per_item']} groups per item ("
+        + ("fixed by --schedule" if args.schedule else
+           f"autotuned over {len(sched['timed'])} candidates tile/order/threads x groups: "
+           + ", ".join(f"{tl}/{od}/{a}x{b} {ms:.3f} ms" for tl, od, a, b, ms in sched["timed"])) + ")")
     result_extra.update(transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
                         transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
                         transform_fwd_inv_ms=info["transform_ms"],
