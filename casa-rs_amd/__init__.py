@@ -12,7 +12,8 @@ from .abi import (CLEAN_BUDGET, CLEAN_CYCLE, CLEAN_GLOBAL, CLEAN_NO_PIXELS, CLEA
                   CYCLE_HIPFFT, F32, F64, MODE_DIRTY, MODE_PSF, MODE_RESIDUAL, WEIGHT_BRIGGS,
                   WEIGHT_UNIFORM, ImagingError, InvalidRequest, NoUsableSamples, Normalization,
                   Unsupported)
-from .imaging import (Context, RowChannelBatch, RowIngest, StandardGridder, VisibilityBatch, VisibilitySet,
+from .imaging import (Context, RowChannelBatch, RowIngest, StandardGridder, StandardMfsModelPredictor,
+                      VisibilityBatch, VisibilitySet,
                       airy_table, default_context, mosaic_phase_gradient, phase_rotation,
                       rotate_rows, run_hogbom_minor_cycle)
 
@@ -23,5 +24,5 @@ __all__ = [
     "phase_rotation", "rotate_rows", "airy_table", "mosaic_phase_gradient",
     "AW_OK", "AW_NON_FINITE_COORDINATE", "AW_OUTSIDE_GRID", "AW_KERNEL_INDEX_OUTSIDE_CELL",
     "AW_INVALID_NORMALIZATION", "AW_NO_CELL", "run_hogbom_minor_cycle", "CLEAN_BUDGET", "CLEAN_GLOBAL",
-    "CLEAN_NSIGMA", "CLEAN_CYCLE", "CLEAN_NO_PIXELS", "RowIngest",
+    "CLEAN_NSIGMA", "CLEAN_CYCLE", "CLEAN_NO_PIXELS", "RowIngest", "StandardMfsModelPredictor",
 ]

@@ -61,7 +61,7 @@ EXPORTS = (
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
     "chip_vset_cycle_strategy", "chip_vset_cycle_graph", "chip_vset_schedule", "chip_vset_schedule_info",
     "chip_grid_plain", "chip_degrid_plain",
-    "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint",
+    "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint", "chip_predict_model",
     "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_aw_cells", "chip_aw_plan",
     "chip_airy_table", "chip_hogbom",
     "chip_ingest_create", "chip_ingest_destroy", "chip_ingest_push", "chip_ingest_samples", "chip_ingest_download",
@@ -246,6 +246,7 @@ def load():
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),
         "chip_model_to_grid": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _PP]),
         "chip_grid_to_model_adjoint": (ctypes.c_int, [_P, _U64, _P, _P, _PP]),
+        "chip_predict_model": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _U64, _P, _PP]),
         "chip_gridder_attach_aw": (ctypes.c_int, [_P, _U32, ctypes.POINTER(CfCell), _P, _U64, _U32, _U32, _P,
                                                   ctypes.c_double, _PP]),
         "chip_gridder_attach_aw_airy": (ctypes.c_int, [_P, ctypes.POINTER(AiryParams), _U32, _P, _PP]),

@@ -144,6 +144,9 @@ int chip_gridder_create(chip_ctx *ctx, const chip_geometry *geo, chip_gridder **
         CHIP_HIP(hipMemcpyAsync(g->d_taps.p, g->taps.data(), g->d_taps.bytes, hipMemcpyHostToDevice, st));
         CHIP_HIP(hipMemcpyAsync(g->d_corr_x.p, g->corr_x.data(), g->d_corr_x.bytes, hipMemcpyHostToDevice, st));
         CHIP_HIP(hipMemcpyAsync(g->d_corr_y.p, g->corr_y.data(), g->d_corr_y.bytes, hipMemcpyHostToDevice, st));
+        const std::vector<float> raw = tap_weights_raw(g->os);
+        g->d_taps_raw.alloc(raw.size() * sizeof(float));
+        CHIP_HIP(hipMemcpyAsync(g->d_taps_raw.p, raw.data(), g->d_taps_raw.bytes, hipMemcpyHostToDevice, st));
         CHIP_HIP(hipStreamSynchronize(st));
         *out = g.release();
     });
@@ -528,6 +531,16 @@ int chip_degrid_plain(chip_gridder *g, const chip_samples *s, uint32_t precision
         validate_source(*s);
         set_device(g->ctx);
         degrid_plain(*g, make_source(*s), precision, model_dev, vis_out_dev);
+        CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
+    });
+}
+
+int chip_predict_model(chip_gridder *g, const chip_samples *s, const float *model_grids_dev,
+                       uint64_t n_model_grids, float *vis_out_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        predict_model(*g, *s, model_grids_dev, n_model_grids, vis_out_dev);
         CHIP_HIP(hipStreamSynchronize(g->ctx->stream));
     });
 }

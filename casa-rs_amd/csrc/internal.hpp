@@ -116,6 +116,7 @@ struct chip_gridder {
     std::vector<float> taps;          // (os+1) x 7
     std::vector<float> corr_x, corr_y;
     chip::DevBuf d_taps, d_corr_x, d_corr_y;
+    chip::DevBuf d_taps_raw;          // (os+1) x 7 before normalisation (model predictor)
 
     // W projection
     bool wproject = false;
@@ -173,6 +174,7 @@ std::vector<float> correction_axis(uint64_t n);
 std::vector<float> gridder_correction_axis(uint64_t n);
 std::vector<float> sinc_axis(uint64_t n, uint64_t sampling);
 std::vector<float> tap_weights(uint32_t os);
+std::vector<float> tap_weights_raw(uint32_t os);
 uint32_t suggested_w_plane_count(const chip_geometry &g, double max_abs_w);
 double w_max_uvw(const chip_geometry &g, double max_abs_w, bool explicit_planes);
 uint32_t find_w_support(const std::vector<float> &axis_mag_over_peak, uint32_t conv,
@@ -242,6 +244,7 @@ void finish_products(chip_gridder &g, uint32_t precision, uint64_t batch, void *
 void model_to_grid(chip_gridder &g, uint32_t precision, uint64_t batch, const float *model,
                    void *grid_out);
 void grid_to_model_adjoint(chip_gridder &g, uint64_t batch, float *grid, float *image_out);
+void predict_model(chip_gridder &g, const chip_samples &s, const float *grids, uint64_t n_grids, float *out);
 void density_weights(chip_gridder &g, const chip_samples &cs, uint32_t mode, float robust, uint32_t per_plane,
                      float *weight_out, float *f2_host, float *density_out);
 void phase_rotation(const double src[2], const double tgt[2], double uvrot[9], double phrot[3]);

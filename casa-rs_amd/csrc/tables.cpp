@@ -128,6 +128,27 @@ std::vector<float> tap_weights(uint32_t os) {
     return rows;
 }
 
+// The same per-offset 7-tap rows before their normalisation: the weights
+// sample_taps_unnormalized looks up (gridder.rs:1997-2023).  Row r is offset
+// r - os/2.
+std::vector<float> tap_weights_raw(uint32_t os) {
+    const uint64_t len = static_cast<uint64_t>(os) * (kSupport + 1);
+    std::vector<float> table(len, 0.0f);
+    for (uint64_t i = 0; i < static_cast<uint64_t>(os) * kSupport; ++i) {
+        const double d = static_cast<double>(i) / (static_cast<double>(kSupport) * os);
+        table[i] = spheroidal_kernel(d * kSupport, static_cast<double>(kSupport));
+    }
+    const int64_t half = os / 2;
+    std::vector<float> rows;
+    rows.reserve((2 * half + 1) * kTaps);
+    for (int64_t off = -half; off <= half; ++off)
+        for (int t = 0; t < kTaps; ++t) {
+            const int64_t lookup = std::llabs(static_cast<int64_t>(t - kSupport) * os + off);
+            rows.push_back(static_cast<uint64_t>(lookup) < len ? table[lookup] : 0.0f);
+        }
+    return rows;
+}
+
 // gridder.rs:3822-3830
 uint32_t suggested_w_plane_count(const chip_geometry &g, double max_abs_w) {
     if (!(std::isfinite(max_abs_w) && max_abs_w > 0.0)) return 1;

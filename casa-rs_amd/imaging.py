@@ -597,6 +597,68 @@ class StandardGridder:
             pass
 
 
+class StandardMfsModelPredictor:
+    """StandardMfsModelPredictor (lib.rs:3375-3404) on the GPU: a gridder with
+    CASA's composite padding 1.3 and oversampling 100
+    (new_with_casa_composite_padding, gridder.rs:379-383), the model grid
+    centered_fft2(apodize_model(model)) made on the device when the model has
+    a non-zero pixel, and predict() -- the tap-sum-normalised degrid of
+    degrid_sample_product_planned_sectdgrid (gridder.rs:1227-1266).
+
+    ``model``: one image (nx, ny) or one per channel (n_chan, nx, ny) for
+    rows x channels input."""
+
+    def __init__(self, nx, ny, cell_x, cell_y, model, ctx: Context | None = None):
+        self.gridder = StandardGridder(nx, ny, cell_x, cell_y, padding_mode=abi.PAD_COMPOSITE_1_3,
+                                       oversampling=100, ctx=ctx)
+        model = torch.as_tensor(model).to(device=self.gridder.ctx.torch_device, dtype=torch.float32)
+        if model.dim() == 2:
+            model = model[None]
+        self.has_components = bool(torch.any(model != 0))
+        self.model_grid = self.gridder.model_to_grid(model) if self.has_components else None
+
+    @property
+    def n_grids(self):
+        return 0 if self.model_grid is None else self.model_grid.shape[0]
+
+    def predict(self, u_lambda=None, v_lambda=None, *, uvw_m=None, lambda_scale=None, model_grid=None):
+        """Model visibilities (complex64 tensor), one per sample: per-sample
+        wavelengths (u_lambda, v_lambda) or rows x channels (uvw_m (rows, 3)
+        metres, lambda_scale (chan,) freq/c; sample (row, chan) at row * n_chan
+        + chan).  ``model_grid`` overrides the predictor's own grid(s)."""
+        g = self.gridder
+        dev = g.ctx.torch_device
+        keep = []
+
+        def put(t):
+            t = torch.as_tensor(t).to(device=dev, dtype=torch.float64).contiguous()
+            keep.append(t)
+            return t
+
+        if uvw_m is not None:
+            uvw, ls = put(uvw_m), put(lambda_scale)
+            n_rows, n_chan = uvw.shape[0], ls.shape[0]
+            s = abi.Samples(n_rows, n_chan, None, None, None, _ptr(uvw), _ptr(ls), None, None, None, None,
+                            None, None, None)
+        else:
+            u, v = put(u_lambda), put(v_lambda)
+            n_rows, n_chan = u.shape[0], 1
+            s = abi.Samples(n_rows, 1, _ptr(u), _ptr(v), None, None, None, None, None, None, None, None,
+                            None, None)
+        out = torch.zeros(n_rows * n_chan, dtype=torch.complex64, device=dev)
+        grid = self.model_grid if model_grid is None else model_grid
+        if grid is None or n_rows * n_chan == 0:
+            return out  # a model without components predicts zeros (lib.rs:3388-3390)
+        grid = grid.to(device=dev, dtype=torch.complex64).contiguous()
+        if grid.dim() == 2:
+            grid = grid[None]
+        abi.call("chip_predict_model", g.handle, ctypes.byref(s), _ptr(grid), grid.shape[0], _ptr(out))
+        return out
+
+    def close(self):
+        self.gridder.close()
+
+
 class VisibilitySet:
     """Resident binned stream: FP64 plan + integer sort once, grid/degrid many times."""
 

@@ -526,6 +526,25 @@ int chip_model_to_grid(chip_gridder *g, uint32_t precision, uint64_t batch, cons
 int chip_grid_to_model_adjoint(chip_gridder *g, uint64_t batch, float *grid_dev,
                                float *image_out_dev, char **err);
 
+/* ---- MODEL_DATA prediction ---------------------------------------------- */
+/* Model visibilities of the standard MFS model predictor
+ * (StandardMfsModelPredictor::predict, lib.rs:3387-3403), the reference's
+ * MODEL_DATA writer: per sample, the 7x7 spheroid taps before normalisation
+ * (sample_taps_unnormalized, gridder.rs:1997-2023) read from the model grid
+ * and divided by their sum (degrid_sample_product_planned_sectdgrid,
+ * gridder.rs:1227-1266); 0 for a sample whose taps leave the grid or whose
+ * coordinates are not finite (the predictor's fallback).  The gridder is a
+ * standard one (CHIP_ERR_INVALID with W or AW kernels attached); the
+ * reference's predictor is StandardGridder::new_with_casa_composite_padding
+ * (CHIP_PAD_COMPOSITE_1_3, oversampling 100) and its model grid
+ * centered_fft2(apodize_model(model)), i.e. chip_model_to_grid with CHIP_F32
+ * (lib.rs:3376-3384).  Samples: u_dev / v_dev per sample (n_chan = 1) or
+ * uvw_m_dev with lambda_scale_dev; vis, weight, flags and w are not read.
+ * model_grids_dev: n_model_grids complex64 grids, 1 (shared) or n_chan (one
+ * per channel).  vis_out_dev: complex64, one per sample in input order. */
+int chip_predict_model(chip_gridder *g, const chip_samples *s, const float *model_grids_dev,
+                       uint64_t n_model_grids, float *vis_out_dev, char **err);
+
 /* ---- minor cycle --------------------------------------------------------- */
 /* Hogbom minor-cycle controls (CleanConfig, types.rs): loop gain, global
  * threshold (Jy/beam) and component budget (hogbom_component_budget,

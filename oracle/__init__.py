@@ -127,6 +127,8 @@ def _declare(L):
                               ctypes.c_int, ctypes.c_int, _P, _P, ctypes.POINTER(_Sums)]
     L.co_std_degrid.restype = None
     L.co_std_degrid.argtypes = [ctypes.POINTER(_Std), _P, _U64, _P, _P, _P, _P]
+    L.co_std_predict_many.restype = None
+    L.co_std_predict_many.argtypes = [ctypes.POINTER(_Std), _U64, _P, _P, _P, _P]
     L.co_w_plane_sum.restype = ctypes.c_float
     L.co_w_plane_sum.argtypes = [_P, _U64, ctypes.c_int64, _U64]
     L.co_w_sample_normalization.restype = ctypes.c_float
@@ -363,6 +365,17 @@ class StandardGridder:
                             _ptr(m), _ptr(out.view(np.float32)))
         return out
 
+    def predict_sectdgrid(self, u, v, model_grid) -> np.ndarray:
+        """degrid_sample_product_planned_sectdgrid (gridder.rs:1227-1266) per
+        sample, 0 where the sample has no taps (StandardMfsModelPredictor::
+        predict, lib.rs:3387-3403)."""
+        u, v = _f64(u), _f64(v)
+        m = _c64_as_f32(model_grid)
+        out = np.empty(len(u), np.complex64)
+        lib().co_std_predict_many(ctypes.byref(self._s), len(u), _ptr(u), _ptr(v), _ptr(m),
+                                  _ptr(out.view(np.float32)))
+        return out
+
     def apodize_model(self, model: np.ndarray) -> np.ndarray:
         """gridder.rs:1554-1570: model x correction placed at the image origin."""
         out = np.zeros((self.gx, self.gy), np.complex64)
@@ -424,6 +437,29 @@ class StandardGridder:
         res = self.corrected_image_from_grid_f64(centered_ifft2(dirty_grid))
         res = ((res / nsum) / peak).astype(np.float32)
         return psf, res, peak
+
+
+class StandardMfsModelPredictor:
+    """StandardMfsModelPredictor (lib.rs:3375-3404): a gridder with CASA's
+    composite padding 1.3 (new_with_casa_composite_padding, gridder.rs:379-383)
+    and the reference's oversampling 100, the model grid
+    centered_fft2(apodize_model(model)) when the model has a non-zero pixel,
+    and predict(u, v) by the tap-sum-normalised degrid."""
+
+    def __init__(self, nx, ny, cell_x, cell_y, model, oversampling=100):
+        self.gridder = StandardGridder(nx, ny, cell_x, cell_y, padding_mode=StandardGridder.PAD_COMPOSITE_1_3,
+                                       oversampling=oversampling)
+        model = np.asarray(model, np.float32)
+        self.model_grid = self.gridder.model_grid(model) if np.any(np.abs(model) > 0) else None
+
+    def predict(self, u, v, model_grid=None):
+        """Predicted visibilities (complex64); ``model_grid`` overrides the
+        predictor's own (e.g. a grid made elsewhere, to compare the degrid
+        alone)."""
+        grid = self.model_grid if model_grid is None else model_grid
+        if grid is None:
+            return np.zeros(len(u), np.complex64)
+        return self.gridder.predict_sectdgrid(u, v, grid)
 
 
 # --------------------------------------------------------------------------
@@ -507,6 +543,9 @@ class WProjector:
     norm_radius: int = 0
     norm_table: np.ndarray = field(default_factory=lambda: np.zeros(0, np.float32))
     support_margin: list = field(default_factory=list)
+    # Sign of the screen phase.  +1 is the reference's (gridder.rs:3453); a
+    # test sets -1 to plant a sign mistake and show that its checks catch it.
+    screen_sign: float = 1.0
 
     def __post_init__(self):
         g = self.gridder
@@ -539,7 +578,7 @@ class WProjector:
         peak0 = None
         for k in range(planes):
             w = float(k * k) / self.w_scale if planes > 1 else 0.0
-            screen = w_screen(conv, inner, s0, s1, w, corr)
+            screen = w_screen(conv, inner, s0, s1, self.screen_sign * w, corr)
             t = np.fft.fft2(screen.astype(np.complex128)).astype(np.complex64)
             pk = np.float32(np.abs(t[0, 0]))
             if peak0 is None:
@@ -681,9 +720,12 @@ class WProjector:
 # Direct DFT (lib.rs:73709-73910)
 # --------------------------------------------------------------------------
 
-def direct_adjoint(nx, ny, cell_x, cell_y, u, v, w, values, threads=None, image=None):
-    """accumulate_direct_adjoint over samples: image += 2 Re(value e^{i phase})."""
-    u, v, w = _f64(u), _f64(v), _f64(w)
+def direct_adjoint(nx, ny, cell_x, cell_y, u, v, w, values, threads=None, image=None, w_sign=1.0):
+    """accumulate_direct_adjoint over samples: image += 2 Re(value e^{i phase}).
+
+    ``w_sign`` multiplies w (+1 is the reference's phase, lib.rs:73909-73910;
+    -1 evaluates the transform in the opposite w convention)."""
+    u, v, w = _f64(u), _f64(v), _f64(np.asarray(w, np.float64) * w_sign)
     vals = _c64_as_f32(values)
     if image is None:
         image = np.zeros((nx, ny), np.float32)
@@ -713,9 +755,12 @@ def direct_predict(nx, ny, cell_x, cell_y, model, u, v, w):
 
 
 def direct_psf_and_residual(nx, ny, cell_x, cell_y, u, v, w, weight, sumwt_factor, vis,
-                            model=None, gridable=None, threads=None):
-    """compute_psf_direct + compute_residual_direct (lib.rs:73709-73823)."""
+                            model=None, gridable=None, threads=None, w_sign=1.0):
+    """compute_psf_direct + compute_residual_direct (lib.rs:73709-73823).
+
+    ``w_sign`` multiplies w before the transform (+1 is the reference's)."""
     n = len(u)
+    w = np.asarray(w, np.float64) * w_sign
     weight = _f32(weight)
     sumwt_factor = _f32(sumwt_factor)
     vis = np.asarray(vis, np.complex64)

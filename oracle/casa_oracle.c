@@ -233,6 +233,70 @@ void co_std_plan_many(const co_std *g, uint64_t n, const double *u, const double
     }
 }
 
+/* sample_taps_unnormalized, gridder.rs:1997-2023: indices and raw spheroid
+ * weights of the 7 taps of one axis, or 0 when the coordinate is not finite,
+ * a tap leaves [0, size) or the weights sum to zero. */
+static int co_taps_unnormalized(double coordinate, uint64_t size, uint64_t os, const float *table,
+                                uint64_t len, int64_t *indices, float *weights) {
+    if (!isfinite(coordinate)) return 0;
+    if (coordinate > 4.0e18 || coordinate < -4.0e18) return 0;
+    int64_t anchor = (int64_t)round(coordinate);
+    int64_t offset = (int64_t)round(((double)anchor - coordinate) * (double)os);
+    float norm = 0.0f;
+    for (int tap = 0; tap < CO_TAPS; ++tap) {
+        int64_t index = anchor - CO_SUPPORT + tap;
+        if (index < 0 || index >= (int64_t)size) return 0;
+        int64_t delta = index - anchor;
+        int64_t lookup = delta * (int64_t)os + offset;
+        if (lookup < 0) lookup = -lookup;
+        float weight = (uint64_t)lookup < len ? table[lookup] : 0.0f;
+        indices[tap] = index;
+        weights[tap] = weight;
+        norm += weight;
+    }
+    return norm > 0.0f;
+}
+
+/* StandardMfsModelPredictor::predict (lib.rs:3387-3403) over a model grid
+ * (complex64, x-major): degrid_sample_product_planned_sectdgrid
+ * (gridder.rs:1227-1266) -- sum of grid * (wx * wy) and of wx * wy, x taps
+ * outer, y taps inner, then value / norm -- or, when that is None, the
+ * normalised product taps of plan_sample, or 0 when the sample has no plan.
+ * With the spheroid table every in-grid tap set has a positive sum, so the
+ * second branch returns the first branch's rejections as 0 (plan_sample
+ * rejects the same out-of-grid and non-finite coordinates). */
+void co_std_predict_many(const co_std *g, uint64_t n, const double *u, const double *v,
+                         const float *grid, float *out) {
+    uint64_t len = g->os * (CO_SUPPORT + 1);
+    float *table = (float *)malloc(len * sizeof(float));
+    co_std_kernel_table(g->os, table);
+    for (uint64_t i = 0; i < n; ++i) {
+        int64_t xi[CO_TAPS], yi[CO_TAPS];
+        float xw[CO_TAPS], yw[CO_TAPS];
+        float re = 0.0f, im = 0.0f;
+        if (co_taps_unnormalized(co_std_coord_x(g, u[i]), g->gx, g->os, table, len, xi, xw) &&
+            co_taps_unnormalized(co_std_coord_y(g, v[i]), g->gy, g->os, table, len, yi, yw)) {
+            float vr = 0.0f, vi = 0.0f, norm = 0.0f;
+            for (int a = 0; a < CO_TAPS; ++a) {
+                for (int b = 0; b < CO_TAPS; ++b) {
+                    float weight = xw[a] * yw[b];
+                    uint64_t idx = (uint64_t)xi[a] * g->gy + (uint64_t)yi[b];
+                    vr += grid[2 * idx] * weight;
+                    vi += grid[2 * idx + 1] * weight;
+                    norm += weight;
+                }
+            }
+            if (norm > 0.0f && isfinite(norm)) {
+                re = vr / norm;
+                im = vi / norm;
+            }
+        }
+        out[2 * i] = re;
+        out[2 * i + 1] = im;
+    }
+    free(table);
+}
+
 typedef struct {
     double normalization_sumwt;
     double reported_sumwt;

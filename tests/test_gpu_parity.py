@@ -215,13 +215,29 @@ def test_w_kernels_match_oracle(gpu, oracle_mod, nx, arcsec, planes, sampling, m
     assert info.planes == wp.planes and info.sampling == wp.sampling
     assert info.w_scale == wp.w_scale
     sup, ker, nt = g.wproject_tables()
+    # a support may differ only where the deciding value lies within 1e-5 of
+    # the 1e-3 threshold (FP32 against FP64 FFT rounding)
     margin_ok = np.asarray(wp.support_margin) > 1e-5
     assert np.array_equal(sup[margin_ok], wp.supports[margin_ok])
-    if np.array_equal(sup, wp.supports):
-        assert ker.shape == wp.kernels.shape
-        # FP32 FFT (GPU) against FP64 FFT (oracle), relative to the kernel peak
-        assert rel(ker, wp.kernels) < 2e-5
-        assert np.max(np.abs(nt - wp.norm_table)) < 2e-5 * np.max(np.abs(wp.norm_table))
+    # plane 0 fixes the common scale (its integral normalises every plane)
+    assert sup[0] == wp.supports[0]
+    same = np.nonzero(sup == wp.supports)[0]
+    assert len(same) >= len(sup) - 1
+    s = g.info.sampling
+    R = (s + 1) // 2
+    quarter = max(g.info.gx, g.info.gy) // 2 - 1
+    ax = 2 * R + 1
+    peak = np.max(np.abs(wp.kernels))
+    nt_g = nt.reshape(len(sup), ax * ax)
+    nt_o = wp.norm_table.reshape(len(sup), ax * ax)
+    for k in same:
+        # FP32 FFT (GPU) against FP64 FFT (oracle), relative to the kernel
+        # peak, over the plane's cropped footprint and its zero padding
+        cw = max(min(int(sup[k]) * s + R + 1, quarter), 1)
+        w_common = min(ker.shape[1], wp.kernels.shape[1])
+        assert np.max(np.abs(ker[k, :w_common, :w_common] - wp.kernels[k, :w_common, :w_common])) < 2e-5 * peak
+        assert not np.any(ker[k, cw:, :]) and not np.any(ker[k, :, cw:])
+        assert np.max(np.abs(nt_g[k] - nt_o[k])) < 2e-5 * np.max(np.abs(nt_o))
 
 
 @pytest.mark.parametrize("nx,arcsec,planes,sampling,maxw", W_CASES)
