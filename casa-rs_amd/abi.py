@@ -60,7 +60,7 @@ EXPORTS = (
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
     "chip_vset_cycle_strategy", "chip_vset_cycle_graph", "chip_vset_schedule", "chip_vset_schedule_info",
-    "chip_grid_plain", "chip_degrid_plain",
+    "chip_grid_plain", "chip_grid_plain_noalloc", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint", "chip_predict_model",
     "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_aw_cells", "chip_aw_plan",
     "chip_airy_table", "chip_hogbom",
@@ -242,6 +242,8 @@ def load():
                                        _P, _P, _P, _P, _P, _PP]),
         "chip_vset_schedule_info": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _PP]),
         "chip_grid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P, _PP]),
+        "chip_grid_plain_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _U32, _P, _P, _P, _P,
+                                                   _PP]),
         "chip_degrid_plain": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _U32, _P, _P, _PP]),
         "chip_finish_products": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _P, _P, _P, _P, _PP]),
         "chip_model_to_grid": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _PP]),

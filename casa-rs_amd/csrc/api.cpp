@@ -75,6 +75,7 @@ int chip_ctx_create(int device, chip_ctx **out, char **err) {
         std::unique_ptr<chip_ctx> c(new chip_ctx);
         c->device = device;
         CHIP_HIP(hipSetDevice(device));
+        noalloc_scratch_init(device);  // device address of the no-allocation scratch
         CHIP_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
         c->stream = c->own_stream;
         *out = c.release();
@@ -521,6 +522,17 @@ int chip_grid_plain(chip_gridder *g, const chip_samples *s, uint32_t mode, uint3
         validate_source(*s);
         set_device(g->ctx);
         grid_plain(*g, make_source(*s), mode, precision, model_dev, psf_dev, dirty_dev, sums_host);
+    });
+}
+
+int chip_grid_plain_noalloc(chip_gridder *g, const chip_samples *s, uint32_t mode, uint32_t precision,
+                            const void *model_dev, void *psf_dev, void *dirty_dev, chip_sums *sums_host,
+                            char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        validate_source(*s);
+        set_device(g->ctx);
+        grid_plain_noalloc(*g, make_source(*s), mode, precision, model_dev, psf_dev, dirty_dev, sums_host);
     });
 }
 

@@ -11,9 +11,51 @@
 //             (execution.rs:18306-18325, gridder.rs:1315-1396);
 //   W:        cwt = K[|ix*s+off_x|][|iy*s+off_y|], conjugated when w > 0;
 //             grid += value*cwt; degrid += conj(cwt)*grid (gridder.rs:3680-3747).
+#include <mutex>
+
 #include "device.hpp"
 
+// Scratch of chip_grid_plain_noalloc: the per-channel FP64 sums
+// (normalisation sum of weights, reported sum, gridded count) that
+// chip_grid_plain keeps in a buffer it allocates per call.  Every GPU the
+// library runs on holds one copy until the process ends; calls on one GPU
+// share it (overlapping calls race).  Host code never passes this name to a
+// kernel: it is a host-side stand-in; the device address comes from
+// hipGetSymbolAddress, per device, at context creation.
+static __device__ __align__(256) double scratch[3 * CHIP_NOALLOC_MAX_CHAN];
+
 namespace chip {
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+__global__ __launch_bounds__(256) void k_zero_f64(double *p, uint64_t n) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+}  // namespace
+
+// Device address of `scratch` on `device` (the current device), looked up
+// once per device.
+void noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) fail(CHIP_ERR_DEVICE, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        fail(CHIP_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<double *>(ptr);
+}
+
+static double *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    double *p = (device >= 0 && device < kMaxDevices) ? g_scratch_dev[device] : nullptr;
+    if (!p) fail(CHIP_ERR_DEVICE, "scratch of the no-allocation path not set up on this device");
+    return p;
+}
 
 // ---------------------------------------------------------------------------
 // Plain per-channel kernels.
@@ -316,6 +358,46 @@ void grid_plain(chip_gridder &g, const Source &src, uint32_t mode, uint32_t prec
             sums_host[c].skipped = src.n_rows - sums_host[c].gridded;
         }
     }
+}
+
+// chip_grid_plain with its sums in the device's `scratch` (no allocation):
+// the sums are cleared by a kernel (capturable), the grids are the caller's,
+// and the sums come back through a fixed host buffer on the stack.
+void grid_plain_noalloc(chip_gridder &g, const Source &src, uint32_t mode, uint32_t precision,
+                        const void *model, void *psf, void *dirty, chip_sums *sums_host) {
+    if (src.n_chan > CHIP_NOALLOC_MAX_CHAN)
+        fail(CHIP_ERR_INVALID, "the no-allocation plain gridder takes at most " +
+                                   std::to_string(CHIP_NOALLOC_MAX_CHAN) + " channels per call");
+    if (g.wproject && !g.aw && g.wk_chunk)
+        fail(CHIP_ERR_INVALID,
+             "the no-allocation plain gridder needs resident W tables (recomputing them needs scratch)");
+    if (precision != CHIP_F64 && precision != CHIP_F32) fail(CHIP_ERR_INVALID, "unknown precision");
+    hipStream_t st = g.ctx->stream;
+    double *sums = noalloc_scratch(g.device);
+    const uint64_t n = 3 * src.n_chan;
+    hipLaunchKernelGGL(k_zero_f64, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, st, sums, n);
+    CHIP_HIP(hipGetLastError());
+    if (precision == CHIP_F64)
+        grid_plain_t<double>(g, src, mode, model, psf, dirty, sums);
+    else
+        grid_plain_t<float>(g, src, mode, model, psf, dirty, sums);
+    constexpr uint64_t kChunk = 128;  // channels per copy into the stack buffer
+    double hs[3 * kChunk];
+    for (uint64_t c0 = 0; c0 < src.n_chan; c0 += kChunk) {
+        const uint64_t nc = std::min<uint64_t>(kChunk, src.n_chan - c0);
+        CHIP_HIP(hipMemcpyAsync(hs, sums + 3 * c0, 3 * nc * sizeof(double), hipMemcpyDeviceToHost, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+        if (sums_host) {
+            for (uint64_t k = 0; k < nc; ++k) {
+                chip_sums &o = sums_host[c0 + k];
+                o.normalization_sumwt = hs[3 * k];
+                o.reported_sumwt = hs[3 * k + 1];
+                o.gridded = static_cast<uint64_t>(hs[3 * k + 2]);
+                o.skipped = src.n_rows - o.gridded;
+            }
+        }
+    }
+    CHIP_HIP(hipStreamSynchronize(st));
 }
 
 void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,

@@ -238,6 +238,9 @@ void grid_plain(chip_gridder &g, const Source &src, uint32_t mode, uint32_t prec
                 const void *model, void *psf, void *dirty, chip_sums *sums);
 void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,
                   void *out);
+void noalloc_scratch_init(int device);
+void grid_plain_noalloc(chip_gridder &g, const Source &src, uint32_t mode, uint32_t precision,
+                        const void *model, void *psf, void *dirty, chip_sums *sums);
 
 void finish_products(chip_gridder &g, uint32_t precision, uint64_t batch, void *psf, void *dirty,
                      const double *nsumwt, float *psf_out, float *res_out, float *peaks);

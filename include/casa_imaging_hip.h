@@ -496,6 +496,19 @@ int chip_vset_cycle_info(const chip_vset *vs, int *strategy, uint32_t *shape, do
 int chip_grid_plain(chip_gridder *g, const chip_samples *s, uint32_t mode, uint32_t precision,
                     const void *model_dev, void *psf_dev, void *dirty_dev, chip_sums *sums_host,
                     char **err);
+/* chip_grid_plain without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  No counterpart in the reference.  Same arguments,
+ * grids and results; the per-channel FP64 sums live in a fixed device array
+ * of the library (3 doubles per channel, CHIP_NOALLOC_MAX_CHAN channels at
+ * most; more is CHIP_ERR_INVALID), cleared by a kernel at every call, so
+ * calls that overlap on one GPU race on it.  W tables must be resident
+ * (CHIP_ERR_INVALID under a chip_gridder_wkernel_budget that recomputes
+ * them).  Synchronous, like chip_grid_plain. */
+#define CHIP_NOALLOC_MAX_CHAN 4096
+int chip_grid_plain_noalloc(chip_gridder *g, const chip_samples *s, uint32_t mode, uint32_t precision,
+                            const void *model_dev, void *psf_dev, void *dirty_dev, chip_sums *sums_host,
+                            char **err);
 /* Plain degridding; vis_out_dev is complex128 for CHIP_F64 (FP64 MACs end to
  * end), complex64 for CHIP_F32; rejected samples get 0. */
 int chip_degrid_plain(chip_gridder *g, const chip_samples *s, uint32_t precision,
