@@ -139,6 +139,12 @@ def _declare(L):
     L.co_w_grid.restype = None
     L.co_w_grid.argtypes = [ctypes.POINTER(_WProj), _U64, _P, _P, _P, _P, _P, _P, _P, _P,
                             ctypes.c_int, ctypes.c_int, _P, _P, ctypes.POINTER(_Sums)]
+    L.co_w_grid64.restype = None
+    L.co_w_grid64.argtypes = [ctypes.POINTER(_WProj), _U64, _P, _P, _P, _P, _P, _P, _P, _P,
+                              ctypes.c_int, ctypes.c_int, _P, _P, ctypes.POINTER(_Sums)]
+    L.co_aw_grid64.restype = None
+    L.co_aw_grid64.argtypes = [_U64, _U64, _D, _D, _U64, _P, _P, _P, _U64, _P, _P, _P, _P, _P, _P, _P, _P,
+                               _P, ctypes.c_int, _P, _P, _P]
     L.co_w_degrid.restype = None
     L.co_w_degrid.argtypes = [ctypes.POINTER(_WProj), _U64, _P, _P, _P, _P, _P]
     L.co_direct_adjoint.restype = None
@@ -682,6 +688,39 @@ class WProjector:
         return psf, dirty, Sums(sums.normalization_sumwt, sums.reported_sumwt, sums.gridded,
                                 sums.skipped)
 
+    def grid64(self, u, v, w, weight, sumwt_factor, vis, gridable=None, mode=1, model=None,
+               threads=None, psf=None, dirty=None):
+        """``grid`` accumulated in FP64 (complex128 grids, f64 products; the
+        model grid, for mode 2, complex128): the FP64 reference FP32 results
+        are measured against, not the reference's Complex32 W grids."""
+        n = len(u)
+        u, v, w = _f64(u), _f64(v), _f64(w)
+        weight, sumwt_factor = _f32(weight), _f32(sumwt_factor)
+        vis_f = _c64_as_f32(vis)
+        g = None if gridable is None else np.ascontiguousarray(gridable, dtype=np.uint8)
+        shape = (self.gridder.gx, self.gridder.gy)
+        if psf is None:
+            psf = np.zeros(shape, np.complex128)
+        if dirty is None:
+            dirty = np.zeros(shape, np.complex128)
+        m = None if model is None else np.ascontiguousarray(model, np.complex128).view(np.float64)
+        sums = _Sums()
+        lib().co_w_grid64(ctypes.byref(self._s), n, _ptr(u), _ptr(v), _ptr(w), _ptr(weight),
+                          _ptr(sumwt_factor), _ptr(g), _ptr(vis_f), _ptr(m), mode,
+                          threads or default_threads(), _ptr(psf.view(np.float64)),
+                          _ptr(dirty.view(np.float64)), ctypes.byref(sums))
+        return psf, dirty, Sums(sums.normalization_sumwt, sums.reported_sumwt, sums.gridded,
+                                sums.skipped)
+
+    def model_grid64(self, model):
+        """model_grid with the transform kept in FP64 (complex128)."""
+        return w_model_grid64(self.gridder, self.sampling, model)
+
+    def dirty_products64(self, psf_grid, dirty_grid, sums: Sums, psf_peak=None):
+        """dirty_products in FP64 end to end (see w_dirty_products64)."""
+        return w_dirty_products64(self.gridder, self.sampling, psf_grid, dirty_grid,
+                                  sums.normalization_sumwt, psf_peak)
+
     def degrid(self, u, v, w, model):
         u, v, w = _f64(u), _f64(v), _f64(w)
         m = _c64_as_f32(model)
@@ -714,6 +753,42 @@ class WProjector:
         res = g.corrected_w_project_image_from_grid(raw_res, s)
         res = (((np.float32(2.0) * res) / nsum) / peak).astype(np.float32)
         return psf, res, peak
+
+
+def w_model_grid64(gridder, sampling, model):
+    """centered_fft2(apodize_w_project_model(model)) (lib.rs:70909-70913) with
+    the transform kept in FP64 (complex128)."""
+    return centered_fft2(gridder.apodize_w_project_model(model, sampling))
+
+
+def w_dirty_products64(gridder, sampling, psf_grid, dirty_grid, nsum, psf_peak=None):
+    """The W-projection dirty products (lib.rs:71009-71039, gridder.rs:1685-1715)
+    in FP64 end to end: f64 transform, correction cx*cy/sinc (the f32
+    correction and sinc values of the reference, 0 where |sinc| <= 1e-6),
+    2 v / nsum, PSF peak, residual / peak.  With ``psf_grid`` None,
+    ``psf_peak`` normalises the residual.  Returns (psf, residual, peak)."""
+    g = gridder
+    sinc = sinc_axis(max(g.gx, g.gy), sampling).astype(np.float64)
+    bx, by = g.blc
+    nx, ny = g.nx, g.ny
+    cx = g.correction_x[bx:bx + nx].astype(np.float64)
+    cy = g.correction_y[by:by + ny].astype(np.float64)
+    sf = sinc[bx:bx + nx][:, None] * sinc[by:by + ny][None, :]
+    ok = np.abs(sf) > 1e-6
+    factor = np.where(ok, (cx[:, None] * cy[None, :]) / np.where(ok, sf, 1.0), 0.0)
+
+    def image(grid):
+        return centered_ifft2(np.asarray(grid, np.complex128))[bx:bx + nx, by:by + ny].real * factor
+
+    psf = None
+    if psf_grid is not None:
+        psf = 2.0 * image(psf_grid) / float(nsum)
+        peak = float(np.max(np.abs(psf)))
+        psf = psf / peak
+    else:
+        peak = float(psf_peak)
+    res = 2.0 * image(dirty_grid) / float(nsum) / peak
+    return psf, res, peak
 
 
 # --------------------------------------------------------------------------
@@ -995,6 +1070,27 @@ def aw_degrid(gx, gy, du, dv, atlas, gradients, u, v, w, cell_of, model, pointin
                        _ptr(atlas.pixels.view(np.float32)), _ptr(g), n, _ptr(u), _ptr(v), _ptr(w), _ptr(c),
                        _ptr(p), _ptr(m), _ptr(out.view(np.float32)), _ptr(st))
     return out, st
+
+
+def aw_grid64(gx, gy, du, dv, atlas, gradients, u, v, w, cell_of, vis, weight, mode=1, model=None,
+              pointing_of=None, usable=None):
+    """co_aw_grid64: the AW gridding in FP64 throughout (complex128 PSF and
+    dirty / residual grids; ``model`` complex128 for mode 2).  Returns (psf,
+    dirty, sum of w |normalisation|, gridded count)."""
+    n, u, v, w, c, p, g = _aw_stream(atlas, gradients, u, v, w, cell_of, pointing_of)
+    vis_f = _c64_as_f32(vis)
+    wt = _f32(weight)
+    us = None if usable is None else np.ascontiguousarray(usable, np.uint8)
+    psf = np.zeros((gx, gy), np.complex128) if mode != 2 else None
+    dirty = np.zeros((gx, gy), np.complex128) if mode != 0 else None
+    m = None if model is None else np.ascontiguousarray(model, np.complex128).view(np.float64)
+    sums = np.zeros(2, np.float64)
+    lib().co_aw_grid64(gx, gy, du, dv, atlas.sampling, ctypes.cast(atlas.table, ctypes.c_void_p),
+                       _ptr(atlas.pixels.view(np.float32)), _ptr(g), n, _ptr(u), _ptr(v), _ptr(w), _ptr(c),
+                       _ptr(p), _ptr(vis_f), _ptr(wt), _ptr(us), _ptr(m), mode,
+                       _ptr(None if psf is None else psf.view(np.float64)),
+                       _ptr(None if dirty is None else dirty.view(np.float64)), _ptr(sums))
+    return psf, dirty, float(sums[0]), int(sums[1])
 
 
 def airy_max_radius(dish, blockage) -> float:

@@ -725,6 +725,130 @@ void co_w_grid(const co_wproj *p, uint64_t n, const double *u, const double *v, 
     free(tids);
 }
 
+/* The W path of co_w_grid accumulated in FP64 throughout: Complex64 grids, the
+ * (Complex32) kernel taps widened to f64, V * w, the degridded model and
+ * (V - p) * w in f64, with an f64 model grid.  Not a restatement of the
+ * reference's Complex32 W grids (lib.rs:71472-71540): this is the FP64
+ * reference that FP32 results are measured against; plans, validity filters
+ * and sums are co_w_grid's. */
+typedef struct {
+    const co_wproj *p;
+    uint64_t begin, end;
+    const double *u, *v, *w;
+    const float *weight, *sumwt_factor, *vis;
+    const uint8_t *gridable;
+    const double *model;
+    int mode;
+    double *psf, *dirty;
+    co_sums sums;
+} co_w_job64;
+
+static void *co_w_worker64(void *arg) {
+    co_w_job64 *j = (co_w_job64 *)arg;
+    const co_wproj *p = j->p;
+    const uint64_t W = p->width, stride = p->gy;
+    const int64_t s = (int64_t)p->sampling;
+    for (uint64_t i = j->begin; i < j->end; ++i) {
+        const float *vis = j->vis + 2 * i;
+        if ((j->gridable && !j->gridable[i]) ||
+            !co_w_sample_valid(j->weight[i], j->sumwt_factor[i], vis, j->u[i], j->v[i], j->w[i])) {
+            j->sums.skipped++;
+            continue;
+        }
+        int64_t plan[6];
+        float nrm;
+        if (!co_w_plan(p, j->u[i], j->v[i], j->w[i], plan, &nrm)) { j->sums.skipped++; continue; }
+        const double weight = (double)j->weight[i];
+        j->sums.normalization_sumwt += 2.0 * weight * (double)nrm;
+        j->sums.reported_sumwt += weight * (double)j->sumwt_factor[i];
+        j->sums.gridded++;
+        const int64_t S = p->support[plan[4]];
+        const float *K = p->kernels + 2 * (uint64_t)plan[4] * W * W;
+        const double csign = plan[5] ? -1.0 : 1.0;
+        double val_r, val_i;
+        if (j->mode == 2) {
+            double pr = 0.0, pi = 0.0;
+            if (j->model) {
+                for (int64_t iy = -S; iy <= S; ++iy) {
+                    int64_t ky = iy * s + plan[3]; if (ky < 0) ky = -ky;
+                    for (int64_t ix = -S; ix <= S; ++ix) {
+                        int64_t kx = ix * s + plan[2]; if (kx < 0) kx = -kx;
+                        double cr = (double)K[2 * (kx * W + ky)];
+                        double ci = -csign * (double)K[2 * (kx * W + ky) + 1]; /* conj(cwt) */
+                        uint64_t idx = (uint64_t)(plan[0] + ix) * stride + (uint64_t)(plan[1] + iy);
+                        double gr = j->model[2 * idx], gi = j->model[2 * idx + 1];
+                        pr += cr * gr - ci * gi;
+                        pi += cr * gi + ci * gr;
+                    }
+                }
+            }
+            val_r = ((double)vis[0] - pr) * weight;
+            val_i = ((double)vis[1] - pi) * weight;
+        } else {
+            val_r = (double)vis[0] * weight;
+            val_i = (double)vis[1] * weight;
+        }
+        for (int64_t iy = -S; iy <= S; ++iy) {
+            int64_t ky = iy * s + plan[3]; if (ky < 0) ky = -ky;
+            for (int64_t ix = -S; ix <= S; ++ix) {
+                int64_t kx = ix * s + plan[2]; if (kx < 0) kx = -kx;
+                double cr = (double)K[2 * (kx * W + ky)];
+                double ci = csign * (double)K[2 * (kx * W + ky) + 1];
+                uint64_t idx = (uint64_t)(plan[0] + ix) * stride + (uint64_t)(plan[1] + iy);
+                if (j->mode != 2) {
+                    j->psf[2 * idx] += weight * cr;
+                    j->psf[2 * idx + 1] += weight * ci;
+                }
+                if (j->mode != 0) {
+                    j->dirty[2 * idx] += val_r * cr - val_i * ci;
+                    j->dirty[2 * idx + 1] += val_r * ci + val_i * cr;
+                }
+            }
+        }
+    }
+    return NULL;
+}
+
+void co_w_grid64(const co_wproj *p, uint64_t n, const double *u, const double *v, const double *w,
+                 const float *weight, const float *sumwt_factor, const uint8_t *gridable,
+                 const float *vis, const double *model, int mode, int threads, double *psf,
+                 double *dirty, co_sums *sums) {
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > n) threads = n > 0 ? (int)n : 1;
+    uint64_t cells = p->gx * p->gy * 2;
+    uint64_t chunk = (n + threads - 1) / threads;
+    co_w_job64 *jobs = (co_w_job64 *)calloc(threads, sizeof(co_w_job64));
+    pthread_t *tids = (pthread_t *)calloc(threads, sizeof(pthread_t));
+    for (int t = 0; t < threads; ++t) {
+        co_w_job64 *j = &jobs[t];
+        j->p = p; j->u = u; j->v = v; j->w = w; j->weight = weight; j->sumwt_factor = sumwt_factor;
+        j->vis = vis; j->gridable = gridable; j->model = model; j->mode = mode;
+        j->begin = (uint64_t)t * chunk;
+        j->end = j->begin + chunk < n ? j->begin + chunk : n;
+        if (j->begin > n) j->begin = n;
+        if (threads == 1) { j->psf = psf; j->dirty = dirty; }
+        else {
+            j->psf = mode != 2 ? (double *)calloc(cells, sizeof(double)) : NULL;
+            j->dirty = mode != 0 ? (double *)calloc(cells, sizeof(double)) : NULL;
+        }
+        pthread_create(&tids[t], NULL, co_w_worker64, j);
+    }
+    for (int t = 0; t < threads; ++t) pthread_join(tids[t], NULL);
+    for (int t = 0; t < threads; ++t) {
+        co_w_job64 *j = &jobs[t];
+        sums->normalization_sumwt += j->sums.normalization_sumwt;
+        sums->reported_sumwt += j->sums.reported_sumwt;
+        sums->gridded += j->sums.gridded;
+        sums->skipped += j->sums.skipped;
+        if (threads > 1) {
+            if (j->psf) { co_add_f64(psf, j->psf, cells); free(j->psf); }
+            if (j->dirty) { co_add_f64(dirty, j->dirty, cells); free(j->dirty); }
+        }
+    }
+    free(jobs);
+    free(tids);
+}
+
 /* WProjector::degrid_sample_planned, gridder.rs:3726-3747.  NaN when rejected. */
 void co_w_degrid(const co_wproj *p, uint64_t n, const double *u, const double *v, const double *w,
                  const float *model, float *out) {
@@ -1350,6 +1474,83 @@ void co_aw_degrid(uint64_t gx, uint64_t gy, double du, double dv, uint64_t sampl
         const float im = ai * c_ - ar * d_;
         out[2 * i] = re / ns;
         out[2 * i + 1] = im / ns;
+    }
+    free(taps);
+}
+
+/* The AW gridding of co_aw_grid_f64 / co_aw_degrid in FP64 throughout: the
+ * phased Complex32 taps widened to f64, prediction sum of conj(tap) * model
+ * over an f64 model grid divided by conj of the f64 normalisation, value
+ * (V - p) * w (mode 2), V * w (mode 1, with w * tap into the PSF grid) or w
+ * (mode 0) times the tap, all in f64 into f64 grids.  The FP64 reference that
+ * FP32 results are measured against; plans are co_aw_plan's.  Samples with
+ * usable = 0, a non-positive / non-finite weight or a non-finite V are
+ * skipped (the W-path acceptance rule).  sums[0] += w |n64| (the sum of
+ * weights the device accumulates per accepted sample), sums[1] counts the
+ * gridded samples. */
+void co_aw_grid64(uint64_t gx, uint64_t gy, double du, double dv, uint64_t sampling, const co_cf_cell *cells,
+                  const float *pixels, const double *gradients, uint64_t n, const double *u, const double *v,
+                  const double *w, const uint32_t *cell_of, const uint32_t *pointing_of, const float *vis,
+                  const float *weight, const uint8_t *usable, const double *model, int mode, double *psf,
+                  double *dirty, double *sums) {
+    float *taps = NULL;
+    uint64_t cap = 0;
+    for (uint64_t i = 0; i < n; ++i) {
+        const float wt = weight[i];
+        if ((usable && !usable[i]) || !(isfinite(wt) && wt > 0.0f)) continue;
+        if (!(isfinite(vis[2 * i]) && isfinite(vis[2 * i + 1]))) continue;
+        const co_cf_cell *c = cells + cell_of[i];
+        int64_t geo[5];
+        float n32[2];
+        double n64[2];
+        if (co_aw_plan(gx, gy, du, dv, sampling, c, pixels, u[i], v[i], w[i], geo, n32, n64)) continue;
+        const uint64_t T = (2 * c->x_support + 1) * (2 * c->y_support + 1);
+        if (T > cap) {
+            free(taps);
+            taps = (float *)malloc(2 * T * sizeof(float));
+            cap = T;
+        }
+        co_aw_taps(sampling, c, pixels, geo, gradients + 2 * pointing_of[i], taps);
+        const double dw = (double)wt;
+        sums[0] += dw * hypot(n64[0], n64[1]);
+        sums[1] += 1.0;
+        const int64_t xs = (int64_t)c->x_support, ys = (int64_t)c->y_support;
+        double val_r = 0.0, val_i = 0.0;
+        if (mode == 2) {
+            double ar = 0.0, ai = 0.0;
+            if (model) {
+                uint64_t k = 0;
+                for (int64_t iy = -ys; iy <= ys; ++iy)
+                    for (int64_t ix = -xs; ix <= xs; ++ix, ++k) {
+                        const double tr = taps[2 * k], ti = -(double)taps[2 * k + 1];
+                        const double *g = model + 2 * ((uint64_t)(geo[0] + ix) * gy + (uint64_t)(geo[1] + iy));
+                        ar += tr * g[0] - ti * g[1];
+                        ai += tr * g[1] + ti * g[0];
+                    }
+            }
+            /* / conj(n64) */
+            const double cr = n64[0], ci = -n64[1], ns = cr * cr + ci * ci;
+            const double pr = (ar * cr + ai * ci) / ns, pi = (ai * cr - ar * ci) / ns;
+            val_r = ((double)vis[2 * i] - pr) * dw;
+            val_i = ((double)vis[2 * i + 1] - pi) * dw;
+        } else if (mode == 1) {
+            val_r = (double)vis[2 * i] * dw;
+            val_i = (double)vis[2 * i + 1] * dw;
+        }
+        uint64_t k = 0;
+        for (int64_t iy = -ys; iy <= ys; ++iy)
+            for (int64_t ix = -xs; ix <= xs; ++ix, ++k) {
+                const double tr = taps[2 * k], ti = taps[2 * k + 1];
+                const uint64_t idx = 2 * ((uint64_t)(geo[0] + ix) * gy + (uint64_t)(geo[1] + iy));
+                if (mode != 2 && psf) {
+                    psf[idx] += dw * tr;
+                    psf[idx + 1] += dw * ti;
+                }
+                if (mode != 0 && dirty) {
+                    dirty[idx] += val_r * tr - val_i * ti;
+                    dirty[idx + 1] += val_r * ti + val_i * tr;
+                }
+            }
     }
     free(taps);
 }
