@@ -34,6 +34,7 @@ if ROOT not in sys.path:
 
 C = 299_792_458.0
 HBM_PEAK_GBS = 8000.0
+PASS_NAMES = ("k_fwd_rows", "k_fwd_cols", "k_zero_touched", "k_binned", "k_inv_cols", "k_inv_rows")
 FP32_PEAK_TFLOPS = 157.3
 
 WORKLOADS = {
@@ -44,20 +45,30 @@ WORKLOADS = {
                    sampling=None, n_src=200, flag_max=0.6, arm_m=21_000.0),
     # W-projection drop-in at size: same array, 8 h at 2 s (5.05 M rows) x 16
     # channels 1.0-2.0 GHz (80.9 M visibilities), 3413^2 image on a 4096^2
-    # grid, 128 W planes, W oversampling 8, natural weights, no flags
+    # grid at cell 1/(3 max|uv|), 128 explicit W planes (spanning w up to
+    # 0.25/cell, CASA's rule for an explicit plane count, which here is the
+    # data's max|w|), W oversampling 8, natural weights, no flags; supports 4..7
     "w4096": dict(hours=8.0, dt=2.0, n_chan=16, f0=1.0e9, f1=2.0e9, nx=3413, planes=128,
                   sampling=8, n_src=200, flag_max=0.0, arm_m=21_000.0, weighting="natural"),
+    # the same data imaged over a 2.4x wider field (cell 1/(0.93 max|uv|)),
+    # where the W term is strong: the reference's automatic plane count
+    # (floor(1.05 max|w| sin(half field)) = 1456 planes over the data's w
+    # range) and supports growing from 4 to 31 (9 to 63 px); the ~10 % of
+    # samples whose footprint would leave the grid are skipped by the planner
+    "w4096wide": dict(hours=8.0, dt=2.0, n_chan=16, f0=1.0e9, f1=2.0e9, nx=3413, planes=None,
+                      sampling=8, n_src=200, flag_max=0.0, arm_m=21_000.0, weighting="natural", cell_k=0.93),
     # AW-projection mosaic: 7 pointings on a hexagon at half-power spacing,
     # RR/RL/LR/LL, Airy A-term (25 m dish, 2 m blockage), 6 h at 10 s,
     # 16 channels 1.0-1.5 GHz, 2048^2 grid (1707^2 image), 64 W planes,
-    # oversampling 8, 100 Gaussian sources, per-pointing flags U(0, 0.4)
+    # oversampling 8, 100 Gaussian sources, per-pointing flags U(0, 0.4); CF
+    # cells at 8 frequencies (one per channel pair), nearest per sample
     "mosaic7": dict(kind="mosaic", hours=6.0, dt=10.0, n_chan=16, n_corr=4, f0=1.0e9, f1=1.5e9, nx=1707,
                     planes=64, sampling=8, n_src=100, flag_max=0.4, arm_m=2_500.0, pointings=7,
-                    dish=25.0, blockage=2.0),
+                    dish=25.0, blockage=2.0, cf_freqs=8),
     # reduced mosaic for quick checks (same geometry, fewer rows)
     "mosaic7s": dict(kind="mosaic", hours=1.0, dt=10.0, n_chan=4, n_corr=4, f0=1.0e9, f1=1.5e9, nx=1707,
                      planes=64, sampling=8, n_src=100, flag_max=0.4, arm_m=2_500.0, pointings=7,
-                     dish=25.0, blockage=2.0),
+                     dish=25.0, blockage=2.0, cf_freqs=2),
 }
 
 
@@ -104,7 +115,7 @@ def make_cube(torch, wl, seed, dev):
     freq = np.linspace(wl["f0"], wl["f1"], n_chan)
     scale = freq / C
     max_uv = float(np.max(np.hypot(uvw[:, 0], uvw[:, 1])) * scale[-1])
-    cell = 1.0 / (3.0 * max_uv)
+    cell = 1.0 / (wl.get("cell_k", 3.0) * max_uv)
     nx = wl["nx"]
     # sources: Gaussians inside the inner 80% of the field
     ns = wl["n_src"]
@@ -126,7 +137,10 @@ def make_cube(torch, wl, seed, dev):
         u = U[:, 0:1] * scale[c]
         v = U[:, 1:2] * scale[c]
         w = U[:, 2:3] * scale[c]
-        ph = -2.0 * np.pi * (u * L_ + v * M_ + w * N_)
+        # w enters with the sign the W projector images (WProjector and
+        # AwProjector apply e^{-2 pi i w (n-1)}, the opposite of the
+        # reference's direct mode), so that the sky is in focus
+        ph = -2.0 * np.pi * (u * L_ + v * M_ - w * N_)
         amp = S_ * torch.exp(-G_ * (u * u + v * v))
         vis[:, c] = torch.complex((amp * torch.cos(ph)).sum(1), (amp * torch.sin(ph)).sum(1)).to(torch.complex64)
     frac = rng.uniform(0.0, wl["flag_max"], n_chan)
@@ -209,7 +223,7 @@ def make_mosaic(torch, wl, seed, dev):
         u = U[:, 0:1] * (f / C)
         v = U[:, 1:2] * (f / C)
         w = U[:, 2:3] * (f / C)
-        ph = -2.0 * np.pi * (u * L_ + v * M_ + w * N_)
+        ph = -2.0 * np.pi * (u * L_ + v * M_ - w * N_)  # the W projector's w sign, as make_cube
         amp = S_ * torch.exp(-G_ * (u * u + v * v)) * pb[P]
         e = torch.complex(torch.cos(ph), torch.sin(ph))
         par = (amp * e).sum(1)
@@ -236,12 +250,29 @@ def make_mosaic(torch, wl, seed, dev):
                 pointing=torch.tensor(pointing, dtype=torch.int32, device=dev), pointings=pl)
 
 
+def native_oracle(log):
+    """Route the oracle through a -march=native build for this host (the
+    portable build when the compiler is not there); returns (build, cpu)."""
+    import tempfile
+
+    import oracle
+
+    try:
+        oracle.use_native_build(tempfile.mkdtemp(prefix="oracle_native_"))
+        build = "gcc -O3 -march=native -ffp-contract=off"
+    except Exception as e:  # the baseline still runs, on the portable build
+        log(f"native oracle build failed ({e!r}); timing the portable build")
+        build = "gcc -O3 -march=x86-64-v2 -ffp-contract=off"
+    return build, oracle.cpu_model()
+
+
 def cpu_baseline_aw(pkg, g, data, cpu_seconds, log):
     """Oracle AW residual refresh (degrid from a complex64 model grid, then
     grid (V - p) w into a Complex64 grid, sequential as the reference's
     AwProjector, one core) on a bounded sample of rows of one plane."""
     import oracle
 
+    build, cpu_name = native_oracle(log)
     cells = [(c.cpu().numpy(), xs, ys) for c, xs, ys in g.aw_cells()]
     atlas = oracle.AwAtlas(cells, g.info.sampling)
     grads = data["gradients"]
@@ -261,7 +292,7 @@ def cpu_baseline_aw(pkg, g, data, cpu_seconds, log):
     for k in range(0, len(rows), chunk):
         r = np.sort(rows[k:k + chunk])
         u, v, w = (uvw[r, j] * scale[c] for j in range(3))
-        cell_of = np.clip(np.round(np.sqrt(w_incr * np.abs(w))), 0, len(cells) - 1).astype(np.uint32)
+        cell_of = data["cf_cell"][r, c].cpu().numpy().astype(np.uint32)
         vis = data["vis"][r, c].cpu().numpy()
         wt = data["weight"][r, c].cpu().numpy()
         t0 = time.perf_counter()
@@ -273,8 +304,9 @@ def cpu_baseline_aw(pkg, g, data, cpu_seconds, log):
         nvis += len(r)
         if spent >= cpu_seconds:
             break
-    log(f"cpu baseline (AW): {nvis} vis in {spent:.2f} s on 1 thread")
-    return {"value": nvis / spent / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port",
+    log(f"cpu baseline (AW): {nvis} vis in {spent:.2f} s on 1 thread ({cpu_name}, {build})")
+    return {"value": nvis / spent / 1e6, "unit": "Mvis/s", "cores": 1, "kind": "port", "cpu_model": cpu_name,
+            "build": build,
             "sample": f"oracle AWProject residual refresh (degrid + grid into Complex64 grids, sequential as "
                       f"the reference's AwProjector), {nvis} unflagged visibilities of plane 0, FFTs excluded"}
 
@@ -285,6 +317,9 @@ def cpu_baseline(pkg, g, data, cpu_seconds, log):
     import torch
     import oracle
 
+    build, cpu_name = native_oracle(log)
+    # every host thread of the run, up to 16: the host share of one GPU on
+    # the machines this runs on
     threads = min(16, len(os.sched_getaffinity(0)))
     nx, cell = data["nx"], data["cell"]
     og = oracle.StandardGridder(nx, nx, cell, cell)
@@ -313,24 +348,90 @@ def cpu_baseline(pkg, g, data, cpu_seconds, log):
                 break
         if spent >= cpu_seconds:
             break
-    log(f"cpu baseline: {nvis} vis of {chans} channels in {spent:.2f} s on {threads} threads")
-    return {"value": nvis / spent / 1e6, "unit": "Mvis/s", "cores": threads, "kind": "port",
+    log(f"cpu baseline: {nvis} vis of {chans} channels in {spent:.2f} s on {threads} threads ({cpu_name}, {build})")
+    return {"value": nvis / spent / 1e6, "unit": "Mvis/s", "cores": threads, "kind": "port", "cpu_model": cpu_name,
+            "build": build, "host_threads": len(os.sched_getaffinity(0)),
             "sample": f"oracle W-projection residual refresh (degrid+grid, FP32 grids as the "
                       f"reference) of {nvis} unflagged visibilities from {chans} of {data['n_chan']} channels "
                       f"(in blocks of {chunk} rows), FFTs excluded"}
 
 
+def oracle_fp64_check(g, data, mosaic, channels, psf32, dirty32, res32, model=None, log=log, threads=None):
+    """The batched FP32 images of the sampled ``channels`` (planes of a mosaic)
+    against the FP64 oracle on the same inputs: complex128 grids with FP64
+    products (W: WProjector.grid64 over the device's W tables; AW: aw_grid64
+    over the device's CF cells and phase gradients), FP64 model grids and FP64
+    transforms.  psf32 / dirty32: the FP32 dirty products (n_chan, nx, nx);
+    res32: the major-cycle residual of ``model`` (default data["model"]).
+    Errors are max |FP32 - FP64| over the image, relative to the FP64 image's
+    peak.  Returns a dict of per-channel and maximum errors."""
+    import oracle
+
+    t0 = time.perf_counter()
+    nx, cell = data["nx"], data["cell"]
+    model = data["model"] if model is None else model
+    uvw = data["uvw"].cpu().numpy()
+    scale = data["scale"].cpu().numpy()
+    og = oracle.StandardGridder(nx, nx, cell, cell)
+    s = g.info.sampling
+    threads = threads or min(16, len(os.sched_getaffinity(0)))
+    # private FP64 grids per thread: keep them within a few GB
+    threads = max(1, min(threads, int(4e9 // (2 * 16 * g.info.gx * g.info.gy))))
+    if mosaic:
+        cells = [(c.cpu().numpy(), xs, ys) for c, xs, ys in g.aw_cells()]
+        atlas = oracle.AwAtlas(cells, s)
+        pointing = data["pointing"].cpu().numpy().astype(np.uint32)
+    else:
+        sup, ker, nt = g.wproject_tables()
+        wp = oracle.WProjector.from_tables(og, g.info.planes, s, g.info.w_scale, ker, sup, nt)
+    out = {"channels": [int(c) for c in channels], "dirty": [], "psf": [], "residual": [], "gridded": []}
+    for c in channels:
+        u, v, w = (uvw[:, k] * scale[c] for k in range(3))
+        vis = data["vis"][:, c].cpu().numpy()
+        wt = data["weight"][:, c].cpu().numpy()
+        usable = ~data["flag"][:, c].cpu().numpy()
+        m = model[c].cpu().numpy()
+        m64 = oracle.w_model_grid64(og, s, m)
+        if mosaic:
+            cell_of = data["cf_cell"][:, c].cpu().numpy().astype(np.uint32)
+            gd = (g.info.gx, g.info.gy, g.info.du, g.info.dv, atlas, data["gradients"], u, v, w, cell_of)
+            p64, d64, nsum, count = oracle.aw_grid64(*gd, vis, wt, mode=1, pointing_of=pointing, usable=usable)
+            _, r64, _, _ = oracle.aw_grid64(*gd, vis, wt, mode=2, model=m64, pointing_of=pointing, usable=usable)
+        else:
+            ones = np.ones_like(wt)
+            p64, d64, sm = wp.grid64(u, v, w, wt, ones, vis, gridable=usable, mode=1, threads=threads)
+            _, r64, _ = wp.grid64(u, v, w, wt, ones, vis, gridable=usable, mode=2, model=m64, threads=threads)
+            nsum, count = sm.normalization_sumwt, sm.gridded
+        psf, dirty, peak = oracle.w_dirty_products64(og, s, p64, d64, nsum)
+        _, res, _ = oracle.w_dirty_products64(og, s, None, r64, nsum, psf_peak=peak)
+        del p64, d64, r64
+
+        def err(a, b):
+            return float(np.max(np.abs(np.asarray(a, np.float64) - b)) / np.max(np.abs(b)))
+
+        out["dirty"].append(err(dirty32[c].cpu().numpy(), dirty))
+        out["psf"].append(float(np.max(np.abs(psf32[c].cpu().numpy() - psf))))
+        out["residual"].append(err(res32[c].cpu().numpy(), res))
+        out["gridded"].append(int(count))
+    out.update(dirty_err=max(out["dirty"]), psf_err=max(out["psf"]), residual_err=max(out["residual"]),
+               seconds=time.perf_counter() - t0, threads=threads)
+    log(f"FP64 oracle on {'planes' if mosaic else 'channels'} {out['channels']} ({out['seconds']:.1f} s, "
+        f"{threads} threads): dirty {out['dirty_err']:.2e}, residual {out['residual_err']:.2e} of peak, "
+        f"PSF {out['psf_err']:.2e}")
+    return out
+
+
 def measured_traffic(workload):
-    """HBM bytes per launch of k_binned<RESIDUAL, W> from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes, each scaled by the same counter's reading
-    for a device copy of known size (profiles/k_binned_residual_<workload>_traffic.json),
-    or None when no such measurement is committed."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"k_binned_residual_{workload}_traffic.json")
+    """HBM bytes per dispatch of each kernel of the step, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes, each counter scaled by its reading for a
+    device copy of known size (profiles/step_traffic_<workload>.json, written
+    by --reduce-traffic): {kernel: bytes}, or None when no measurement is
+    committed."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"step_traffic_{workload}.json")
     try:
         with open(path) as f:
-            return float(json.load(f)["traffic_bytes_per_launch"])
-    except (OSError, KeyError, ValueError):
+            return {k: float(v["bytes_per_dispatch"]) for k, v in json.load(f)["kernels"].items()}
+    except (OSError, KeyError, ValueError, TypeError):
         return None
 
 
@@ -345,24 +446,35 @@ def _profile_csv(directory, suffix):
 
 
 def reduce_trace(directory, out_txt, command, top=40):
-    """rocprofv3 --kernel-trace --stats output -> short per-kernel table."""
+    """rocprofv3 --kernel-trace --stats output -> short per-kernel table
+    (calls, mean, median and total device time, share)."""
     rows = _profile_csv(directory, "kernel_stats.csv")
     rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    med = {}
+    try:
+        for r in _profile_csv(directory, "kernel_trace.csv"):
+            med.setdefault(r["Kernel_Name"], []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    except SystemExit:
+        pass
     lines = [f"command: {command}", "",
-             f"{'calls':>7} {'avg_us':>10} {'total_ms':>10} {'pct':>6}  kernel"]
+             f"{'calls':>7} {'avg_us':>10} {'median_us':>10} {'total_ms':>10} {'pct':>6}  kernel"]
     for r in rows[:top]:
-        lines.append(f"{int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:10.2f} "
+        d = med.get(r["Name"])
+        m = f"{float(np.median(d)) / 1e3:10.2f}" if d else f"{'-':>10}"
+        lines.append(f"{int(r['Calls']):7d} {float(r['AverageNs']) / 1e3:10.2f} {m} "
                      f"{float(r['TotalDurationNs']) / 1e6:10.2f} {float(r['Percentage']):6.2f}  {r['Name'][:140]}")
     with open(out_txt, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
 
-def reduce_traffic(fetch_dir, write_dir, kernel, out_json, out_txt, commands):
+def reduce_traffic(fetch_dir, write_dir, out_json, out_txt, commands):
     """Two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) ->
-    HBM bytes per launch of `kernel`, each counter scaled by its reading for
-    the 1 GiB device copies that --calibrate-copy runs."""
+    HBM bytes per dispatch of every kernel of the step, each counter scaled by
+    its reading for the 1 GiB device copies that --calibrate-copy runs."""
     res, lines = {}, [f"command: {c}" for c in commands] + [""]
+    scales = {}
+    per_counter = {}
     for name, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
         per = {}
         for r in _profile_csv(d, "counter_collection.csv"):
@@ -372,22 +484,29 @@ def reduce_traffic(fetch_dir, write_dir, kernel, out_json, out_txt, commands):
         if not copies:
             raise SystemExit("no copy kernel dispatches: run the profiled bench with --calibrate-copy")
         cal = sum(copies) / len(copies)
-        matches = [(k, v) for k, v in per.items() if kernel in k]
-        if not matches:
-            raise SystemExit(f"no dispatches of {kernel}")
-        kname, vals = max(matches, key=lambda kv: len(kv[1]))
-        raw = sum(vals) / len(vals)
-        scale = 1024.0 * 1024.0 / cal
-        res[name] = dict(counter=counter, kernel=kname[:160], dispatches=len(vals), kib_per_dispatch=raw,
-                         copy_1gib_kib=cal, scale=scale, bytes_per_dispatch=raw * scale * 1024.0)
-        lines.append(f"{counter}: {kname[:110]}")
-        lines.append(f"  {len(vals)} dispatches, mean {raw:.1f} KiB/dispatch; a 1 GiB copy reads as {cal:.1f} KiB "
-                     f"-> scale {scale:.4f}; scaled {raw * scale / 1024 / 1024:.4f} GiB/dispatch")
-    traffic = res["fetch"]["bytes_per_dispatch"] + res["write"]["bytes_per_dispatch"]
-    lines.append(f"HBM traffic per launch (scaled fetch + write): {traffic / 1e9:.4f} GB")
+        scales[name] = 1024.0 * 1024.0 / cal
+        per_counter[name] = per
+        lines.append(f"{counter}: a 1 GiB copy reads as {cal:.1f} KiB -> scale {scales[name]:.4f}")
+    kernels = {}
+    for k in PASS_NAMES:
+        entry = {}
+        for name, counter in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+            matches = [(kn, v) for kn, v in per_counter[name].items() if k + "<" in kn or k + "(" in kn]
+            if not matches:
+                break
+            kname, vals = max(matches, key=lambda kv: len(kv[1]))
+            raw = sum(vals) / len(vals)
+            entry[name] = dict(counter=counter, kernel=kname[:160], dispatches=len(vals), kib_per_dispatch=raw,
+                               bytes_per_dispatch=raw * scales[name] * 1024.0)
+        if len(entry) == 2:
+            entry["bytes_per_dispatch"] = entry["fetch"]["bytes_per_dispatch"] + entry["write"]["bytes_per_dispatch"]
+            kernels[k] = entry
+            lines.append(f"{k}: {entry['fetch']['dispatches']} dispatches, fetch "
+                         f"{entry['fetch']['bytes_per_dispatch'] / 1e9:.4f} GB + write "
+                         f"{entry['write']['bytes_per_dispatch'] / 1e9:.4f} GB = "
+                         f"{entry['bytes_per_dispatch'] / 1e9:.4f} GB per dispatch")
     with open(out_json, "w") as f:
-        json.dump(dict(traffic_bytes_per_launch=traffic, fetch=res["fetch"], write=res["write"],
-                       commands=list(commands)), f, indent=1)
+        json.dump(dict(kernels=kernels, scales=scales, commands=list(commands)), f, indent=1)
     with open(out_txt, "w") as f:
         f.write("\n".join(lines) + "\n")
     print("\n".join(lines))
@@ -419,6 +538,67 @@ def reduce_counters(directory, kernel, out_txt, command, kernel_ms=None):
     print("\n".join(lines))
 
 
+def setup_workload(pkg, torch, name, seed, dev, log=log):
+    """The workload ``name`` on the device: synthetic data (``make_cube`` or
+    ``make_mosaic``), a gridder with its W kernels or AW Airy atlas attached,
+    Briggs robust 0 weights for the cubes that ask for them, and the
+    RowChannelBatch the visibility set is built from."""
+    wl = WORKLOADS[name]
+    mosaic = wl.get("kind") == "mosaic"
+    t0 = time.perf_counter()
+    data = (make_mosaic if mosaic else make_cube)(torch, wl, seed, dev)
+    torch.cuda.synchronize()
+    log(f"data: {data['n_rows']} rows x {data['n_chan']} chan, cell {np.rad2deg(data['cell']) * 3600:.4f} arcsec, "
+        f"max|uv| {data['max_uv']:.0f}, max|w| {data['max_w']:.0f} ({time.perf_counter() - t0:.1f} s)")
+
+    nx, cell, n_chan = data["nx"], data["cell"], data["n_chan"]
+    g = pkg.StandardGridder(nx, nx, cell, cell)
+    t0 = time.perf_counter()
+    if mosaic:
+        # pointing phase gradients (lib.rs:56657-56671) from each pointing's
+        # pixel offset; CF cells from the Airy power pattern at cf_freqs CF
+        # frequencies across the band (the centres of equal sub-bands), each
+        # sample using the nearest (awprojection.rs:1127-1180)
+        grads = [pkg.mosaic_phase_gradient(g.info.gx, g.info.gy, (lp / cell, -mp / cell), wl["sampling"])
+                 for lp, mp in data["pointings"]]
+        data["gradients"] = np.asarray(grads)
+        chan_freq = np.linspace(wl["f0"], wl["f1"], wl["n_chan"])
+        nf = wl.get("cf_freqs", 1)
+        cf_freqs = chan_freq.reshape(nf, -1).mean(axis=1)
+        g.attach_aw_airy(wl["dish"], wl["blockage"], cf_freqs, wl["planes"], data["max_w"],
+                         sampling=wl["sampling"], phase_gradients=grads)
+        kernel_build_s = time.perf_counter() - t0
+        sup = np.array([c[1] for c in g.aw_cells()])
+        data["cf_freqs"] = cf_freqs
+        log(f"AW CF atlas: {len(cf_freqs)} frequencies x {wl['planes']} W planes = {len(sup)} Airy x W cells x "
+            f"{len(grads)} pointings = {g.info.planes} tap tables, sampling {g.info.sampling}, supports "
+            f"{sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
+    else:
+        g.attach_wproject(data["max_w"], planes=wl["planes"], sampling=wl["sampling"])
+        kernel_build_s = time.perf_counter() - t0
+        sup, _, _ = g.wproject_tables()
+        log(f"W kernels: {g.info.planes} planes, sampling {g.info.sampling}, width {g.info.width}, "
+            f"supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
+    if not mosaic and wl.get("weighting", "briggs") == "briggs":
+        # Briggs robust 0 from one density grid per channel (library, cube convention)
+        natural = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
+        t0 = time.perf_counter()
+        wts, f2, _ = g.density_weights(natural, pkg.WEIGHT_BRIGGS, 0.0, per_plane=True, want_density=False)
+        torch.cuda.synchronize()
+        data["weight"] = wts.view(data["n_rows"], data["n_chan"])
+        del natural, wts
+        log(f"Briggs robust 0 weights from {n_chan} density grids ({time.perf_counter() - t0:.2f} s), "
+            f"f2 {f2.min():.3g}..{f2.max():.3g}")
+    batch = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"],
+                                pointing=data.get("pointing"))
+    if mosaic:
+        data["cf_cell"] = g.aw_select_cells(batch)
+        batch.cf_cell = data["cf_cell"]
+    n_unflagged = int((~data["flag"]).sum())
+    return dict(wl=wl, mosaic=mosaic, data=data, g=g, batch=batch, sup=sup, kernel_build_s=kernel_build_s,
+                n_unflagged=n_unflagged)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +610,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-minor", action="store_true", help="skip the Hogbom minor-cycle timing")
+    ap.add_argument("--oracle-channels", type=int, default=None,
+                    help="channels (planes) checked against the CPU FP64 oracle (default 2, w4096 1; 0: none)")
     ap.add_argument("--seed", type=int, default=2026)
     ap.add_argument("--schedule", default="", metavar="TILE,ORDER,THREADS,GROUPS",
                     help="fix the binned kernel's schedule instead of autotuning it (profiling runs)")
@@ -439,9 +621,10 @@ def main():
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
     ap.add_argument("--reduce-trace", nargs=3, metavar=("DIR", "OUT_TXT", "COMMAND"),
                     help="summarise a rocprofv3 --kernel-trace --stats run (no GPU work)")
-    ap.add_argument("--reduce-traffic", nargs=7,
-                    metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL", "OUT_JSON", "OUT_TXT", "FETCH_CMD", "WRITE_CMD"),
-                    help="reduce rocprofv3 FETCH_SIZE / WRITE_SIZE runs to bytes per launch (no GPU work)")
+    ap.add_argument("--reduce-traffic", nargs=6,
+                    metavar=("FETCH_DIR", "WRITE_DIR", "OUT_JSON", "OUT_TXT", "FETCH_CMD", "WRITE_CMD"),
+                    help="reduce rocprofv3 FETCH_SIZE / WRITE_SIZE runs to bytes per dispatch of every step kernel "
+                         "(no GPU work)")
     ap.add_argument("--reduce-counters", nargs="+", metavar="ARG",
                     help="DIR KERNEL OUT_TXT COMMAND [KERNEL_MS]: summarise a rocprofv3 --pmc run (no GPU work)")
     args = ap.parse_args()
@@ -453,8 +636,10 @@ def main():
         return
     if args.reduce_traffic:
         a = args.reduce_traffic
-        reduce_traffic(a[0], a[1], a[2], a[3], a[4], a[5:7])
+        reduce_traffic(a[0], a[1], a[2], a[3], a[4:6])
         return
+    if args.oracle_channels is None:
+        args.oracle_channels = 1 if args.workload.startswith("w4096") else 2
     if args.gpus != 1:
         print(f"bench.py measures one GPU: every GPU box here has one, and the sharded path "
               f"is described, not built (asked for --gpus {args.gpus})", file=sys.stderr)
@@ -476,47 +661,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
         log("calibration copy: 3 x 1 GiB device copies done")
-    mosaic = wl.get("kind") == "mosaic"
-    t0 = time.perf_counter()
-    data = (make_mosaic if mosaic else make_cube)(torch, wl, args.seed, dev)
-    torch.cuda.synchronize()
-    log(f"data: {data['n_rows']} rows x {data['n_chan']} chan, cell {np.rad2deg(data['cell']) * 3600:.4f} arcsec, "
-        f"max|uv| {data['max_uv']:.0f}, max|w| {data['max_w']:.0f} ({time.perf_counter() - t0:.1f} s)")
-
+    ws = setup_workload(pkg, torch, args.workload, args.seed, dev)
+    data, g, batch, sup = ws["data"], ws["g"], ws["batch"], ws["sup"]
+    mosaic, kernel_build_s, n_unflagged = ws["mosaic"], ws["kernel_build_s"], ws["n_unflagged"]
     nx, cell, n_chan = data["nx"], data["cell"], data["n_chan"]
-    g = pkg.StandardGridder(nx, nx, cell, cell)
-    t0 = time.perf_counter()
-    if mosaic:
-        # pointing phase gradients (lib.rs:56657-56671) from each pointing's
-        # pixel offset; CF cells from the Airy power pattern at the band centre
-        grads = [pkg.mosaic_phase_gradient(g.info.gx, g.info.gy, (lp / cell, -mp / cell), wl["sampling"])
-                 for lp, mp in data["pointings"]]
-        data["gradients"] = np.asarray(grads)
-        g.attach_aw_airy(wl["dish"], wl["blockage"], 0.5 * (wl["f0"] + wl["f1"]), wl["planes"], data["max_w"],
-                         sampling=wl["sampling"], phase_gradients=grads)
-        kernel_build_s = time.perf_counter() - t0
-        sup = np.array([c[1] for c in g.aw_cells()])
-        log(f"AW CF atlas: {len(sup)} Airy x W cells x {len(grads)} pointings = {g.info.planes} tap tables, "
-            f"sampling {g.info.sampling}, supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
-    else:
-        g.attach_wproject(data["max_w"], planes=wl["planes"], sampling=wl["sampling"])
-        kernel_build_s = time.perf_counter() - t0
-        sup, _, _ = g.wproject_tables()
-        log(f"W kernels: {g.info.planes} planes, sampling {g.info.sampling}, width {g.info.width}, "
-            f"supports {sup.min()}..{sup.max()} ({kernel_build_s * 1e3:.1f} ms)")
-    if not mosaic and wl.get("weighting", "briggs") == "briggs":
-        # Briggs robust 0 from one density grid per channel (library, cube convention)
-        natural = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"])
-        t0 = time.perf_counter()
-        wts, f2, _ = g.density_weights(natural, pkg.WEIGHT_BRIGGS, 0.0, per_plane=True, want_density=False)
-        torch.cuda.synchronize()
-        data["weight"] = wts.view(data["n_rows"], data["n_chan"])
-        del natural, wts
-        log(f"Briggs robust 0 weights from {n_chan} density grids ({time.perf_counter() - t0:.2f} s), "
-            f"f2 {f2.min():.3g}..{f2.max():.3g}")
-    batch = pkg.RowChannelBatch(data["uvw"], data["scale"], data["vis"], data["weight"], data["flag"],
-                                pointing=data.get("pointing"))
-    n_unflagged = int((~data["flag"]).sum())
     t0 = time.perf_counter()
     vs = g.visibility_set(batch)
     torch.cuda.synchronize()
@@ -553,64 +701,104 @@ def main():
     torch.cuda.empty_cache()
 
     model = data["model"]
-    kernel_ms = []
+    n_vis = st["n_valid"]  # visibilities the planner accepted: the throughput numerator
+    kernel_ms, step_s = [], []
     r32 = torch.empty((n_chan, nx, nx), dtype=torch.float32, device=dev)
 
-    def step32():
-        # model -> apodise + FFT -> degrid + grid (V - p) w -> inverse FFT -> residual image
+    def step32(record=True):
+        # model -> apodise + FFT -> degrid + grid (V - p) w -> inverse FFT ->
+        # residual image; the call returns once its work completed
+        t = time.perf_counter()
         vs.major_cycle(model, nsum, peak32, out=r32)
-        kernel_ms.append(vs.stats()["last_kernel_ms"])
+        if record:
+            step_s.append(time.perf_counter() - t)
+            kernel_ms.append(vs.stats()["last_kernel_ms"])
         return r32
 
     if args.schedule:
         # a fixed binned-kernel schedule (profiling runs: no autotuner launches)
         tile, order, threads, gpi = (int(x) for x in args.schedule.split(","))
         vs.set_schedule(threads, gpi, tile=tile, order=order)
-    for _ in range(args.warmup):
-        step32()
+    # the first major cycle sets up the workspace, times the transform shapes
+    # and (unless fixed) autotunes the binned schedule
+    first_s = None
+    for k in range(args.warmup):
+        t0 = time.perf_counter()
+        step32(record=False)
+        if k == 0:
+            first_s = time.perf_counter() - t0
     info = vs.cycle_info()
     log(f"major-cycle transforms: {'fused row/column passes' if info['strategy'] == pkg.CYCLE_FUSED else 'hipFFT 2-D plans'}"
         f", column block {info['columns']} x {info['column_threads']} threads, {info['rows_per_group']} rows per "
         f"rows group, timed forward+inverse {info['transform_ms']:.2f} ms")
-    kernel_ms.clear()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step32()
     torch.cuda.synchronize()
     t32 = (time.perf_counter() - t0) / args.steps
-    rate32 = n_unflagged / t32 / 1e6
-    log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; "
-        f"binned kernel {np.mean(kernel_ms):.3f} ms")
+    rate32 = n_vis / t32 / 1e6
+    if first_s is None:  # no warm-up: the first timed step was the first cycle
+        first_s = step_s[0]
+    log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; steps "
+        f"{min(step_s) * 1e3:.2f}..{max(step_s) * 1e3:.2f} ms; binned kernel {np.mean(kernel_ms):.3f} ms")
     sched = vs.schedule()
     log(f"binned schedule: {sched['tile']}^2 uv tiles, groups by {('anchor', 'plan')[sched['order']]}, "
         f"{sched['threads']} threads x {sched['groups_per_item']} groups per item ("
         + ("fixed by --schedule" if args.schedule else
            f"autotuned over {len(sched['timed'])} candidates tile/order/threads x groups: "
            + ", ".join(f"{tl}/{od}/{a}x{b} {ms:.3f} ms" for tl, od, a, b, ms in sched["timed"])) + ")")
-    result_extra.update(transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
-                        transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
-                        transform_fwd_inv_ms=info["transform_ms"],
-                        binned_schedule={"tile": sched["tile"], "order": ("anchor", "plan")[sched["order"]],
-                                         "threads": sched["threads"],
-                                         "groups_per_item": sched["groups_per_item"],
-                                         "candidates_timed": len(sched["timed"])})
+    cycles = 10  # a typical number of major cycles per imaging run
+    amortised = (plan_s + first_s + (cycles - 1) * t32) / cycles
+    result_extra.update(
+        step_ms={"min": min(step_s) * 1e3, "median": float(np.median(step_s)) * 1e3, "max": max(step_s) * 1e3},
+        plan_inclusive={"plan_bin_ms": plan_s * 1e3, "first_cycle_ms": first_s * 1e3,
+                        "first_cycle_mvis_s": n_vis / (plan_s + first_s) / 1e6,
+                        "cycles": cycles, "amortised_ms_per_cycle": amortised * 1e3,
+                        "amortised_mvis_s": n_vis / amortised / 1e6},
+        groups={"n_groups": st["n_groups"], "vis_per_group": n_vis / max(st["n_groups"], 1),
+                "n_work_items": st["n_items"]},
+        transforms="fused" if info["strategy"] == pkg.CYCLE_FUSED else "hipfft",
+        transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
+        transform_fwd_inv_ms=info["transform_ms"],
+        binned_schedule={"tile": sched["tile"], "order": ("anchor", "plan")[sched["order"]],
+                         "threads": sched["threads"], "groups_per_item": sched["groups_per_item"],
+                         "candidates_timed": len(sched["timed"])})
+    pass_ms = None
     if info["strategy"] == pkg.CYCLE_FUSED:
-        # the same step replayed from a captured HIP graph (one launch per cycle)
+        # device time of every pass of the step, from events between the
+        # passes, over a few more cycles (not part of the timed steps)
+        vs.cycle_passes(1)
+        acc = []
+        for _ in range(max(3, min(args.steps, 5))):
+            step32(record=False)
+            acc.append(vs.cycle_passes(-1))
+        vs.cycle_passes(0)
+        pass_ms = np.mean(np.asarray(acc), axis=0)
+        log("passes (ms): " + ", ".join(f"{n} {m:.3f}" for n, m in zip(PASS_NAMES, pass_ms)))
+
+        # the same step replayed from a captured HIP graph (one launch per
+        # cycle), after as many warm-up replays as the eager steps had
         vs.set_cycle_graph(True)
         r_g = torch.empty_like(r32)
-        vs.major_cycle(model, nsum, peak32, out=r_g)
+        for _ in range(max(1, args.warmup)):
+            vs.major_cycle(model, nsum, peak32, out=r_g)
         torch.cuda.synchronize()
+        graph_s = []
         t0 = time.perf_counter()
         for _ in range(args.steps):
+            t1 = time.perf_counter()
             vs.major_cycle(model, nsum, peak32, out=r_g)
+            graph_s.append(time.perf_counter() - t1)
         torch.cuda.synchronize()
         tg = (time.perf_counter() - t0) / args.steps
         diff = float(torch.max(torch.abs(r_g - r32)) / torch.max(torch.abs(r32)))
-        result_extra.update(graph_ms_per_step=tg * 1e3, graph_mvis_s=n_unflagged / tg / 1e6,
+        result_extra.update(graph_ms_per_step=tg * 1e3, graph_mvis_s=n_vis / tg / 1e6,
+                            graph_step_ms={"min": min(graph_s) * 1e3, "median": float(np.median(graph_s)) * 1e3,
+                                           "max": max(graph_s) * 1e3},
                             graph_vs_eager_max_diff=diff, graph_replays=vs.cycle_replays())
-        log(f"same step replayed from a HIP graph: {tg * 1e3:.2f} ms/step ({n_unflagged / tg / 1e6:.1f} Mvis/s); "
-            f"graph vs eager residual diff {diff:.2e} of peak")
+        log(f"same step replayed from a HIP graph: {tg * 1e3:.2f} ms/step ({n_vis / tg / 1e6:.1f} Mvis/s; steps "
+            f"{min(graph_s) * 1e3:.2f}..{max(graph_s) * 1e3:.2f} ms); graph vs eager residual diff {diff:.2e} of peak")
         vs.set_cycle_graph(False)
         del r_g
     if info["strategy"] == pkg.CYCLE_FUSED and args.compare_hipfft:
@@ -631,6 +819,14 @@ def main():
         del r_h
         torch.cuda.empty_cache()
 
+    if args.oracle_channels > 0:
+        # the batched FP32 dirty and residual images of sampled channels
+        # against the CPU FP64 oracle on the same inputs (outside the timed
+        # region)
+        k = min(args.oracle_channels, n_chan)
+        pick = np.sort(np.random.default_rng(args.seed + 1).choice(n_chan, k, replace=False))
+        result_extra["oracle_fp64"] = oracle_fp64_check(g, data, mosaic, pick, p32, d32, r32)
+
     if not args.no_fp64:
         torch.cuda.empty_cache()
         mgrid64 = g.new_grids(n_chan, pkg.F64)
@@ -649,7 +845,7 @@ def main():
             r64 = step64()
         torch.cuda.synchronize()
         t64 = (time.perf_counter() - t0) / args.fp64_steps
-        rate64 = n_unflagged / t64 / 1e6
+        rate64 = n_vis / t64 / 1e6
         rpk = torch.amax(torch.abs(r64), dim=(1, 2))
         res_err = float(torch.max(torch.amax(torch.abs(r32 - r64), dim=(1, 2)) / torch.clamp(rpk, min=1e-30)))
         result_extra.update(fp64_plain_mvis_s=rate64, fp64_plain_ms_per_step=t64 * 1e3,
@@ -689,14 +885,15 @@ def main():
         torch.cuda.empty_cache()
 
     # Roofline of the dominant kernel, k_binned<RESIDUAL, W> (degrid + grid of
-    # one major cycle).  Algorithmic bytes per launch: 13 B per unflagged
+    # one major cycle).  Algorithmic bytes per launch: 13 B per gridded
     # visibility (vis 8 + weight 4 + flag 1) + 24 B per row (uvw f64) + the
     # model grid read once and the residual grid written once (8 B per cell
     # each).  Algorithmic flops: 16 per tap (complex degrid MAC 8 + grid MAC 8).
     kms = float(np.mean(kernel_ms))
-    n_valid = st["n_valid"]
-    cells = n_chan * g.info.gx * g.info.gy
-    bytes_alg = n_unflagged * (8 + 4 + 1) + data["n_rows"] * 24 + cells * 8 * 2
+    gx, gy = g.info.gx, g.info.gy
+    hy = gy // 2 + 1  # Hermitian half-spectrum columns
+    cells = n_chan * gx * gy
+    bytes_alg = n_vis * (8 + 4 + 1) + data["n_rows"] * 24 + cells * 8 * 2
     flops_alg = st["taps"] * 16.0
     gbs = bytes_alg / (kms * 1e-3) / 1e9
     tfl = flops_alg / (kms * 1e-3) / 1e12
@@ -705,14 +902,37 @@ def main():
     if hbm_time >= flop_time:
         roof = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
     else:
-        # compute-bound: priced at the FP32 peak (157.3 TFLOP/s); the kernel
-        # runs on the vector ALUs, whose FP32 peak is the same figure
-        roof = {"bound": "mfma", "engine": "valu_fp32", "achieved": tfl, "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": tfl / FP32_PEAK_TFLOPS}
-    roof["traffic"] = measured_traffic(args.workload)
-    roof.update(kernel="k_binned<RESIDUAL, W>", kernel_ms=kms, algorithmic_bytes=bytes_alg,
+        # compute-bound: the FP32 vector ALUs (MFMA is not used), priced at
+        # the FP32 vector peak of 157.3 TFLOP/s
+        roof = {"bound": "valu_fp32", "achieved": tfl, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tfl / FP32_PEAK_TFLOPS}
+    traffic = measured_traffic(args.workload)
+    roof["traffic"] = traffic.get("k_binned") if traffic else None
+    roof.update(kernel="k_binned<RESIDUAL, W>", kernel_ms=kms, kernel_ms_source="HIP events around each launch on "
+                "its stream, mean over the timed steps", algorithmic_bytes=bytes_alg,
                 algorithmic_flops=flops_alg, fp32_tflops=tfl, hbm_gbs=gbs,
-                hbm_frac=gbs / HBM_PEAK_GBS, fp32_frac=tfl / FP32_PEAK_TFLOPS)
+                hbm_frac=gbs / HBM_PEAK_GBS, fp32_frac=tfl / FP32_PEAK_TFLOPS,
+                measured_hbm_gbs=roof["traffic"] / (kms * 1e-3) / 1e9 if roof["traffic"] else None)
+    if pass_ms is not None:
+        # every pass of the fused step: algorithmic bytes (each input read
+        # once, each output written once, the whole half spectrum of every
+        # channel, no uv-coverage pruning) / event-timed device ms
+        img = n_chan * nx * nx * 4
+        half_img = n_chan * nx * hy * 8
+        half_grid = n_chan * gx * hy * 8
+        alg = {"k_fwd_rows": img + half_img, "k_fwd_cols": half_img + half_grid, "k_zero_touched": half_grid,
+               "k_binned": bytes_alg, "k_inv_cols": half_grid + half_img, "k_inv_rows": half_img + img}
+        passes = []
+        for name, ms in zip(PASS_NAMES, pass_ms):
+            a = alg[name] / (ms * 1e-3) / 1e9
+            t = traffic.get(name) if traffic else None
+            passes.append({"kernel": name, "ms": float(ms), "algorithmic_bytes": int(alg[name]), "achieved_gbs": a,
+                           "frac": a / HBM_PEAK_GBS, "traffic": t,
+                           "measured_hbm_gbs": t / (ms * 1e-3) / 1e9 if t else None})
+        roof["passes"] = passes
+        step_alg = sum(alg.values())
+        roof["step"] = {"ms": t32 * 1e3, "passes_ms": float(np.sum(pass_ms)), "algorithmic_bytes": int(step_alg),
+                        "achieved_gbs": step_alg / t32 / 1e9, "frac": step_alg / t32 / 1e9 / HBM_PEAK_GBS}
 
     cpu = None
     if not args.no_cpu:
@@ -737,13 +957,15 @@ def main():
         "config": {
             "workload": (f"{args.workload}: AW-projection mosaic residual refresh (degrid+grid+FFTs), "
                          f"{len(data['pointings'])} pointings, {data['n_rows']} rows x {n_chan} planes "
-                         f"(channels x correlations), {n_unflagged} unflagged vis, {nx}^2 image / {g.info.gx}^2 grid, "
-                         f"{len(sup)} W planes of Airy CFs, sampling {g.info.sampling}" if mosaic else
+                         f"(channels x correlations), {n_unflagged} unflagged vis ({n_vis} gridded), {nx}^2 image / "
+                         f"{g.info.gx}^2 grid, {len(sup)} W planes of Airy CFs, sampling {g.info.sampling}" if mosaic else
                          f"{args.workload}: W-projection residual refresh (degrid+grid+FFTs), "
-                         f"{data['n_rows']} rows x {n_chan} channels, {n_unflagged} unflagged vis, "
-                         f"{nx}^2 image / {g.info.gx}^2 grid, {g.info.planes} W planes, sampling {g.info.sampling}"),
+                         f"{data['n_rows']} rows x {n_chan} channels, {n_unflagged} unflagged vis ({n_vis} gridded), "
+                         f"{nx}^2 image / {g.info.gx}^2 grid, cell 1/({WORKLOADS[args.workload].get('cell_k', 3.0)} "
+                         f"max|uv|), {g.info.planes} W planes with supports {int(sup.min())}..{int(sup.max())}, "
+                         f"sampling {g.info.sampling}"),
             "n_rows": data["n_rows"], "n_chan": n_chan, "n_vis_unflagged": n_unflagged,
-            "n_vis_gridded": n_valid, "grid": g.info.gx, "image": nx,
+            "n_vis_gridded": n_vis, "grid": g.info.gx, "image": nx,
             "w_planes": len(sup) if mosaic else g.info.planes,
             "w_sampling": g.info.sampling, "w_support_range": [int(sup.min()), int(sup.max())],
             "grid_point_updates_per_pass": st["taps"], "plan_bin_ms": plan_s * 1e3,
@@ -752,7 +974,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out["value_counts"] = "visibilities the planner accepted (gridded), per second of the timed steps"
     out.update(result_extra)
+    # taps of every accepted sample; a group of identical plans costs one
+    # footprint on the device, so the kernel executes fewer
     out["grid_point_updates_per_s"] = st["taps"] / t32
     print(json.dumps(out))
 

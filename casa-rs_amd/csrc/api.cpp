@@ -23,6 +23,7 @@ void vset_cycle_strategy(chip_vset &vs, int strategy);
 void vset_cycle_graph(chip_vset &vs, int enable);
 void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi);
 uint64_t vset_cycle_replays(const chip_vset &vs);
+void vset_cycle_passes(chip_vset &vs, int enable, double *pass_ms);
 void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev);
 }  // namespace chip
 
@@ -221,7 +222,25 @@ int chip_gridder_attach_aw_airy(chip_gridder *g, const chip_airy_params *p, uint
     return guarded(err, [&] {
         if (!g || !p) fail(CHIP_ERR_INVALID, "NULL argument");
         set_device(g->ctx);
-        attach_aw_airy(*g, *p, n_pointings, phase_gradients);
+        attach_aw_airy(*g, *p, 1, &p->freq_hz, n_pointings, phase_gradients);
+    });
+}
+
+int chip_gridder_attach_aw_airy_freqs(chip_gridder *g, const chip_airy_params *p, uint32_t n_freq,
+                                      const double *freq_hz, uint32_t n_pointings, const double *phase_gradients,
+                                      char **err) {
+    return guarded(err, [&] {
+        if (!g || !p) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        attach_aw_airy(*g, *p, n_freq, freq_hz, n_pointings, phase_gradients);
+    });
+}
+
+int chip_aw_select_cells(chip_gridder *g, const chip_samples *s, uint32_t *cf_cell_dev, char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        aw_select_cells(*g, *s, cf_cell_dev);
     });
 }
 
@@ -485,6 +504,14 @@ int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order
         if (groups_per_item) *groups_per_item = vs->groups_per_item;
         if (n_timed) *n_timed = static_cast<uint32_t>(vs->tune_ms.size() / 5);
         if (timed) std::memcpy(timed, vs->tune_ms.data(), vs->tune_ms.size() * sizeof(double));
+    });
+}
+
+int chip_vset_cycle_passes(chip_vset *vs, int enable, double *pass_ms, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        if (enable < -1 || enable > 1) fail(CHIP_ERR_INVALID, "enable must be -1, 0 or 1");
+        vset_cycle_passes(*vs, enable, pass_ms);
     });
 }
 

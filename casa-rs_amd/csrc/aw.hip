@@ -259,7 +259,7 @@ void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, con
     if (!(std::isfinite(w_increment) && w_increment >= 0.0))
         fail(CHIP_ERR_INVALID, "w_increment must be finite and non-negative");
     const uint64_t n_tables = static_cast<uint64_t>(n_cells) * n_pointings * 2;
-    if (n_tables > 2047) fail(CHIP_ERR_INVALID, "cells x pointings x 2 must be at most 2047 tap tables");
+    if (n_tables > kMaxTables) fail(CHIP_ERR_INVALID, "cells x pointings x 2 must be at most 8191 tap tables");
     const int64_t s = sampling;
     std::vector<AwCell> dc(n_cells);
     int32_t smax = 0;
@@ -417,16 +417,21 @@ void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, con
     CHIP_HIP(hipStreamSynchronize(st));
 }
 
-void attach_aw_airy(chip_gridder &g, const chip_airy_params &ap, uint32_t n_pointings, const double *gradients) {
+void attach_aw_airy(chip_gridder &g, const chip_airy_params &ap, uint32_t n_freq, const double *freqs,
+                    uint32_t n_pointings, const double *gradients) {
     if (g.wproject) fail(CHIP_ERR_INVALID, "a W projection or AW atlas is already attached");
     if (!(std::isfinite(ap.dish_m) && ap.dish_m > 0.0 && std::isfinite(ap.blockage_m) && ap.blockage_m >= 0.0 &&
           ap.blockage_m < ap.dish_m))
         fail(CHIP_ERR_INVALID, "Airy dish / blockage diameters must satisfy 0 <= blockage < dish");
-    if (!(std::isfinite(ap.freq_hz) && ap.freq_hz > 0.0)) fail(CHIP_ERR_INVALID, "frequency must be positive");
+    if (n_freq == 0 || !freqs) fail(CHIP_ERR_INVALID, "at least one CF frequency is needed");
+    for (uint32_t k = 0; k < n_freq; ++k)
+        if (!(std::isfinite(freqs[k]) && freqs[k] > 0.0)) fail(CHIP_ERR_INVALID, "frequency must be positive");
     if (!std::isfinite(ap.pa_rad)) fail(CHIP_ERR_INVALID, "parallactic angle must be finite");
     if (ap.sampling < 2 || ap.sampling > 126 || ap.sampling % 2) fail(CHIP_ERR_INVALID, "sampling must be even, 2..126");
     if (ap.n_w == 0 || ap.n_w > 1023) fail(CHIP_ERR_INVALID, "n_w must be 1..1023");
     if (!(std::isfinite(ap.max_abs_w) && ap.max_abs_w >= 0.0)) fail(CHIP_ERR_INVALID, "max_abs_w must be finite");
+    if (static_cast<uint64_t>(n_freq) * ap.n_w * n_pointings * 2 > kMaxTables)
+        fail(CHIP_ERR_INVALID, "frequencies x W planes x pointings x 2 must be at most 8191 tap tables");
     hipStream_t st = g.ctx->stream;
     const chip_geometry &geo = g.geom;
     const uint32_t s = ap.sampling, planes = ap.n_w;
@@ -449,7 +454,6 @@ void attach_aw_airy(chip_gridder &g, const chip_airy_params &ap, uint32_t n_poin
     a.planes = planes;
     a.cos_pa = std::cos(ap.pa_rad);
     a.sin_pa = std::sin(ap.pa_rad);
-    a.freq = ap.freq_hz;
     a.max_radius = airy_max_radius(ap.dish_m, ap.blockage_m);
 
     const uint64_t plane_bytes = static_cast<uint64_t>(conv) * conv * sizeof(float2);
@@ -459,76 +463,136 @@ void attach_aw_airy(chip_gridder &g, const chip_airy_params &ap, uint32_t n_poin
     DevBuf d_peak(sizeof(float)), d_mag(static_cast<uint64_t>(quarter) * per_batch * sizeof(float));
     CHIP_HIP(hipMemcpyAsync(d_corr.p, corr.data(), d_corr.bytes, hipMemcpyHostToDevice, st));
     CHIP_HIP(hipMemcpyAsync(d_table.p, table.data(), d_table.bytes, hipMemcpyHostToDevice, st));
-    std::vector<uint32_t> support(planes), shape(planes);
-    std::vector<DevBuf> cellbuf(planes);
+    const uint32_t n_cells = n_freq * planes;  // cell f * planes + k: frequency f, W plane k
+    std::vector<uint32_t> support(n_cells), shape(n_cells);
+    std::vector<DevBuf> cellbuf(n_cells);
     std::vector<float> mag(static_cast<uint64_t>(quarter) * per_batch);
-    for (uint32_t first = 0; first < planes; first += per_batch) {
-        const uint32_t nbat = std::min(per_batch, planes - first);
-        CHIP_HIP(hipMemsetAsync(screens.p, 0, plane_bytes * nbat, st));
-        a.first = first;
-        const uint64_t work = static_cast<uint64_t>(2 * (inner / 2)) * (2 * (inner / 2)) * nbat;
-        hipLaunchKernelGGL(k_aw_screen, dim3(nb(work, 256)), dim3(256), 0, st, a, d_corr.as<float>(),
-                           d_table.as<float>(), screens.as<float2>(), nbat);
-        CHIP_HIP(hipGetLastError());
-        hipfftHandle h = g.ctx->fft.get(conv, conv, nbat, HIPFFT_C2C, st);
-        CHIP_FFT(hipfftExecC2C(h, screens.as<hipfftComplex>(), screens.as<hipfftComplex>(), HIPFFT_FORWARD));
-        if (first == 0) {
-            hipLaunchKernelGGL(k_aw_peak, dim3(1), dim3(64), 0, st, screens.as<float2>(), d_peak.as<float>());
+    for (uint32_t f = 0; f < n_freq; ++f) {
+        a.freq = freqs[f];
+        for (uint32_t first = 0; first < planes; first += per_batch) {
+            const uint32_t nbat = std::min(per_batch, planes - first);
+            CHIP_HIP(hipMemsetAsync(screens.p, 0, plane_bytes * nbat, st));
+            a.first = first;
+            const uint64_t work = static_cast<uint64_t>(2 * (inner / 2)) * (2 * (inner / 2)) * nbat;
+            hipLaunchKernelGGL(k_aw_screen, dim3(nb(work, 256)), dim3(256), 0, st, a, d_corr.as<float>(),
+                               d_table.as<float>(), screens.as<float2>(), nbat);
             CHIP_HIP(hipGetLastError());
-            float peak = 0.0f;
-            CHIP_HIP(hipMemcpyAsync(&peak, d_peak.p, sizeof(float), hipMemcpyDeviceToHost, st));
+            hipfftHandle h = g.ctx->fft.get(conv, conv, nbat, HIPFFT_C2C, st);
+            CHIP_FFT(hipfftExecC2C(h, screens.as<hipfftComplex>(), screens.as<hipfftComplex>(), HIPFFT_FORWARD));
+            if (first == 0) {  // this frequency's plane-zero peak
+                hipLaunchKernelGGL(k_aw_peak, dim3(1), dim3(64), 0, st, screens.as<float2>(), d_peak.as<float>());
+                CHIP_HIP(hipGetLastError());
+                float peak = 0.0f;
+                CHIP_HIP(hipMemcpyAsync(&peak, d_peak.p, sizeof(float), hipMemcpyDeviceToHost, st));
+                CHIP_HIP(hipStreamSynchronize(st));
+                if (!(std::isfinite(peak) && peak > 0.0f))
+                    fail(CHIP_ERR_NORMALIZATION, "AW plane-zero kernel peak is zero or non-finite");
+            }
+            hipLaunchKernelGGL(k_aw_axis_mag, dim3(nb(static_cast<uint64_t>(quarter) * nbat, 256)), dim3(256), 0,
+                               st, screens.as<float2>(), conv, quarter, nbat, d_peak.as<float>(), d_mag.as<float>());
+            CHIP_HIP(hipGetLastError());
+            CHIP_HIP(hipMemcpyAsync(mag.data(), d_mag.p, static_cast<uint64_t>(quarter) * nbat * sizeof(float),
+                                    hipMemcpyDeviceToHost, st));
             CHIP_HIP(hipStreamSynchronize(st));
-            if (!(std::isfinite(peak) && peak > 0.0f))
-                fail(CHIP_ERR_NORMALIZATION, "AW plane-zero kernel peak is zero or non-finite");
-        }
-        hipLaunchKernelGGL(k_aw_axis_mag, dim3(nb(static_cast<uint64_t>(quarter) * nbat, 256)), dim3(256), 0, st,
-                           screens.as<float2>(), conv, quarter, nbat, d_peak.as<float>(), d_mag.as<float>());
-        CHIP_HIP(hipGetLastError());
-        CHIP_HIP(hipMemcpyAsync(mag.data(), d_mag.p, static_cast<uint64_t>(quarter) * nbat * sizeof(float),
-                                hipMemcpyDeviceToHost, st));
-        CHIP_HIP(hipStreamSynchronize(st));
-        for (uint32_t b = 0; b < nbat; ++b) {
-            const uint32_t k = first + b;
-            std::vector<float> m(mag.begin() + static_cast<uint64_t>(b) * quarter,
-                                 mag.begin() + static_cast<uint64_t>(b + 1) * quarter);
-            support[k] = find_w_support(m, conv, s);
-            const uint64_t sh = 2 * (static_cast<uint64_t>(support[k]) * s + R + 1);
-            if (sh > conv) fail(CHIP_ERR_INVALID, "AW kernel support exceeds the screen");
-            shape[k] = static_cast<uint32_t>(sh);
-            cellbuf[k].alloc(sh * sh * sizeof(float2));
-            hipLaunchKernelGGL(k_aw_crop, dim3(nb(sh * sh, 256)), dim3(256), 0, st,
-                               screens.as<float2>() + static_cast<uint64_t>(b) * conv * conv, conv, shape[k],
-                               d_peak.as<float>(), cellbuf[k].as<float2>());
-            CHIP_HIP(hipGetLastError());
+            for (uint32_t b = 0; b < nbat; ++b) {
+                const uint32_t k = f * planes + first + b;
+                std::vector<float> m(mag.begin() + static_cast<uint64_t>(b) * quarter,
+                                     mag.begin() + static_cast<uint64_t>(b + 1) * quarter);
+                support[k] = find_w_support(m, conv, s);
+                const uint64_t sh = 2 * (static_cast<uint64_t>(support[k]) * s + R + 1);
+                if (sh > conv) fail(CHIP_ERR_INVALID, "AW kernel support exceeds the screen");
+                shape[k] = static_cast<uint32_t>(sh);
+                cellbuf[k].alloc(sh * sh * sizeof(float2));
+                hipLaunchKernelGGL(k_aw_crop, dim3(nb(sh * sh, 256)), dim3(256), 0, st,
+                                   screens.as<float2>() + static_cast<uint64_t>(b) * conv * conv, conv, shape[k],
+                                   d_peak.as<float>(), cellbuf[k].as<float2>());
+                CHIP_HIP(hipGetLastError());
+            }
         }
     }
     screens.release();
-    // pack the cells, divide by the plane-zero integral at integer taps
-    std::vector<chip_cf_cell> cells(planes);
+    // pack the cells; each frequency's cells are divided by that frequency's
+    // plane-zero integral at integer taps
+    std::vector<chip_cf_cell> cells(n_cells);
     uint64_t total = 0;
-    for (uint32_t k = 0; k < planes; ++k) {
+    for (uint32_t k = 0; k < n_cells; ++k) {
         cells[k] = chip_cf_cell{shape[k], shape[k], support[k], support[k], total};
         total += static_cast<uint64_t>(shape[k]) * shape[k];
     }
     DevBuf pixels(total * sizeof(float2));
-    for (uint32_t k = 0; k < planes; ++k)
+    for (uint32_t k = 0; k < n_cells; ++k)
         CHIP_HIP(hipMemcpyAsync(pixels.as<float2>() + cells[k].offset, cellbuf[k].p, cellbuf[k].bytes,
                                 hipMemcpyDeviceToDevice, st));
-    std::vector<float2> c0(static_cast<uint64_t>(shape[0]) * shape[0]);
-    CHIP_HIP(hipMemcpyAsync(c0.data(), pixels.p, c0.size() * sizeof(float2), hipMemcpyDeviceToHost, st));
     CHIP_HIP(hipStreamSynchronize(st));
     cellbuf.clear();
-    float pb = 0.0f;
-    {
-        const int64_t S = support[0], o = shape[0] / 2, n = shape[0];
-        for (int64_t iy = -S; iy <= S; ++iy)
-            for (int64_t ix = -S; ix <= S; ++ix) pb += c0[(o + ix * s) * n + (o + iy * s)].x;
+    for (uint32_t f = 0; f < n_freq; ++f) {
+        const chip_cf_cell &c = cells[f * planes];
+        std::vector<float2> c0(static_cast<uint64_t>(c.shape_x) * c.shape_y);
+        CHIP_HIP(hipMemcpyAsync(c0.data(), pixels.as<float2>() + c.offset, c0.size() * sizeof(float2),
+                                hipMemcpyDeviceToHost, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+        float pb = 0.0f;
+        {
+            const int64_t S = c.x_support, o = static_cast<int64_t>(c.shape_x) / 2, n = static_cast<int64_t>(c.shape_x);
+            for (int64_t iy = -S; iy <= S; ++iy)
+                for (int64_t ix = -S; ix <= S; ++ix) pb += c0[(o + ix * s) * n + (o + iy * s)].x;
+        }
+        if (!(std::isfinite(pb) && pb > 0.0f))
+            fail(CHIP_ERR_NORMALIZATION, "AW plane-zero kernel integral is non-positive");
+        const uint64_t end = f + 1 < n_freq ? cells[(f + 1) * planes].offset : total;
+        hipLaunchKernelGGL(k_aw_scale, dim3(nb(end - c.offset, 256)), dim3(256), 0, st, pixels.as<float2>() + c.offset,
+                           end - c.offset, pb);
+        CHIP_HIP(hipGetLastError());
     }
-    if (!(std::isfinite(pb) && pb > 0.0f)) fail(CHIP_ERR_NORMALIZATION, "AW plane-zero kernel integral is non-positive");
-    hipLaunchKernelGGL(k_aw_scale, dim3(nb(total, 256)), dim3(256), 0, st, pixels.as<float2>(), total, pb);
+    CHIP_HIP(hipStreamSynchronize(st));
+    attach_aw(g, n_cells, cells.data(), pixels.as<float>(), total, s, n_pointings, gradients, w_scale);
+    g.aw_n_w = planes;
+    g.aw_cf_freq.assign(freqs, freqs + n_freq);
+}
+
+// CF cell of every sample (rows x channels input): the CF frequency nearest
+// to the sample's (freq = lambda_scale * c) times the W planes per frequency,
+// plus the W plane round(sqrt(WIncr |w|)) clamped to those planes
+// (select_key_for_sample, awprojection.rs:1127-1180, without the conjugate-
+// beam and Mueller keys of a CASA cache).
+__global__ void k_aw_select(Source s, double w_incr, uint32_t n_w, const double *cf_freq, uint32_t n_freq,
+                            uint32_t *out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= s.n_rows * s.n_chan) return;
+    const uint64_t row = s.n_chan == 1 ? i : i / s.n_chan;
+    const uint32_t chan = s.n_chan == 1 ? 0u : static_cast<uint32_t>(i - row * s.n_chan);
+    const double ls = s.lambda_scale[chan];
+    const double freq = ls * 299792458.0;
+    uint32_t best = 0;
+    double bd = fabs(cf_freq[0] - freq);
+    for (uint32_t k = 1; k < n_freq; ++k) {
+        const double d = fabs(cf_freq[k] - freq);
+        if (d < bd) { bd = d; best = k; }
+    }
+    const double w = s.uvw_m[3 * row + 2] * ls;
+    double r = (n_w > 1 && w_incr != 0.0) ? round(sqrt(w_incr * fabs(w))) : 0.0;
+    const double top = static_cast<double>(n_w - 1);
+    if (!(r >= 0.0)) r = 0.0;
+    if (r > top) r = top;
+    out[i] = best * n_w + static_cast<uint32_t>(r);
+}
+
+void aw_select_cells(chip_gridder &g, const chip_samples &cs, uint32_t *out) {
+    if (!g.aw || g.aw_cf_freq.empty())
+        fail(CHIP_ERR_INVALID, "no Airy CF atlas with CF frequencies attached");
+    if (!cs.uvw_m_dev || !cs.lambda_scale_dev || cs.n_chan == 0)
+        fail(CHIP_ERR_INVALID, "CF selection needs rows x channels input (uvw_m, lambda_scale)");
+    if (!out) fail(CHIP_ERR_INVALID, "NULL output");
+    const uint64_t n = cs.n_rows * cs.n_chan;
+    if (n == 0) return;
+    hipStream_t st = g.ctx->stream;
+    DevBuf d_f(g.aw_cf_freq.size() * sizeof(double));
+    CHIP_HIP(hipMemcpyAsync(d_f.p, g.aw_cf_freq.data(), d_f.bytes, hipMemcpyHostToDevice, st));
+    const Source src = make_source(cs);
+    hipLaunchKernelGGL(k_aw_select, dim3(nb(n, 256)), dim3(256), 0, st, src, g.aw_w_incr, g.aw_n_w,
+                       d_f.as<double>(), static_cast<uint32_t>(g.aw_cf_freq.size()), out);
     CHIP_HIP(hipGetLastError());
     CHIP_HIP(hipStreamSynchronize(st));
-    attach_aw(g, planes, cells.data(), pixels.as<float>(), total, s, n_pointings, gradients, w_scale);
 }
 
 void aw_plan_raw(chip_gridder &g, const Source &src, int32_t *plan, float *n32, double *n64) {

@@ -168,13 +168,21 @@ void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg
     vs.tune_ms.clear();
     double best = -1.0;
     uint32_t best_tile = vs.tile, best_order = vs.order, best_t = vs.threads, best_g = vs.groups_per_item;
+    // Tuning time is bounded: launches above kLong ms skip the warm-up and
+    // are timed once, and no new candidate starts after kBudget ms of
+    // launches (the candidates are ordered so that the likely best come first).
+    constexpr double kLong = 20.0, kBudget = 3000.0;
+    double spent = 0.0;
     for (uint32_t order : {0u, 1u}) {
         for (uint32_t tile : tiles) {
+            if (spent > kBudget) break;
             vset_rebin(vs, tile, order);
             const GridParams gp = make_gp();
             for (uint32_t gpi : kGpi) {
+                if (spent > kBudget) break;
                 vset_split_items(vs, gpi);
                 for (uint32_t t : kThreads) {
+                    if (spent > kBudget) break;
                     vs.threads = t;
                     // a warm-up launch, then the fastest of up to three timed ones
                     // (one when the first is far behind the best so far)
@@ -184,9 +192,10 @@ void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg
                         CHIP_HIP(hipEventSynchronize(vs.ev_stop));
                         float m = 0.0f;
                         CHIP_HIP(hipEventElapsedTime(&m, vs.ev_start, vs.ev_stop));
-                        if (rep == 0) continue;
+                        spent += m;
+                        if (rep == 0 && m < kLong) continue;
                         ms = ms < 0.0f ? m : std::min(ms, m);
-                        if (best > 0.0 && ms > 1.25 * best) break;
+                        if (m >= kLong || (best > 0.0 && ms > 1.25 * best)) break;
                     }
                     vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(tile), static_cast<double>(order),
                                                          static_cast<double>(t), static_cast<double>(gpi), ms});
@@ -244,9 +253,15 @@ struct CycleWorkspace {
     double transform_ms = -1.0;
     CycleGraph graph;
     uint64_t replays = 0;  // major cycles run by graph replay
+    // per-pass timing of eager fused cycles (chip_vset_cycle_passes): events
+    // before, between and after the six passes, and the last cycle's ms
+    hipEvent_t pev[7] = {};
+    double pass_ms[6] = {-1.0, -1.0, -1.0, -1.0, -1.0, -1.0};
     ~CycleWorkspace() {
         if (r2c) hipfftDestroy(r2c);
         if (c2r) hipfftDestroy(c2r);
+        for (hipEvent_t &e : pev)
+            if (e) (void)hipEventDestroy(e);
     }
 };
 
@@ -416,7 +431,25 @@ void vset_major_cycle(chip_vset &vs, const float *model, const double *nsum, con
             vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), s, timed);
             xf::inverse_residual(ws.xp, Rg, M, d_nsum, d_peak, out, s);
         };
-        if (!vs.cycle_graph) {
+        if (!vs.cycle_graph && vs.cycle_passes) {
+            for (hipEvent_t &e : ws.pev)
+                if (!e) CHIP_HIP(hipEventCreate(&e));
+            CHIP_HIP(hipEventRecord(ws.pev[0], st));
+            xf::forward_model(ws.xp, model, Rg, M, st, ws.pev[1]);
+            CHIP_HIP(hipEventRecord(ws.pev[2], st));
+            xf::zero_touched(ws.xp, Rg, st);
+            CHIP_HIP(hipEventRecord(ws.pev[3], st));
+            vset_residual_with(vs, gp, M, reinterpret_cast<float *>(Rg), st, true);
+            CHIP_HIP(hipEventRecord(ws.pev[4], st));
+            xf::inverse_residual(ws.xp, Rg, M, d_nsum, d_peak, out, st, ws.pev[5]);
+            CHIP_HIP(hipEventRecord(ws.pev[6], st));
+            CHIP_HIP(hipEventSynchronize(ws.pev[6]));
+            for (int k = 0; k < 6; ++k) {
+                float ms = 0.0f;
+                CHIP_HIP(hipEventElapsedTime(&ms, ws.pev[k], ws.pev[k + 1]));
+                ws.pass_ms[k] = ms;
+            }
+        } else if (!vs.cycle_graph) {
             enqueue(st, true);
         } else {
             CycleGraph &cg = ws.graph;
@@ -485,6 +518,12 @@ void vset_cycle_graph(chip_vset &vs, int enable) {
 }
 
 uint64_t vset_cycle_replays(const chip_vset &vs) { return vs.cycle ? vs.cycle->replays : 0; }
+
+void vset_cycle_passes(chip_vset &vs, int enable, double *pass_ms) {
+    if (enable == 0 || enable == 1) vs.cycle_passes = enable == 1;
+    if (pass_ms)
+        for (int k = 0; k < 6; ++k) pass_ms[k] = vs.cycle ? vs.cycle->pass_ms[k] : -1.0;
+}
 
 void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi) {
     if (tile == 0 && threads == 0 && gpi == 0) {  // autotune again at the next major cycle

@@ -13,13 +13,19 @@ namespace chip {
 //      bits 16-31 y anchor (standard: y_start; W: loc_y)
 //   y: bits 0-7   x selector (standard: tap row wix; W: off_x + 128)
 //      bits 8-15  y selector
-//      bits 16-26 W plane
+//      bits 16-26 W plane / tap table, low 11 bits
 //      bit 27     conjugate kernel (w > 0)
 //      bit 28     visibility finite (contributes to dirty grids)
 //      bit 29     usable for residual refresh
+//      bits 30-31 W plane / tap table, high 2 bits (tables up to 8191)
 constexpr uint32_t kConjBit = 1u << 27;
 constexpr uint32_t kVisOkBit = 1u << 28;
 constexpr uint32_t kResidOkBit = 1u << 29;
+constexpr uint32_t kMaxTables = 8191;
+
+__host__ __device__ __forceinline__ uint32_t packed_table(uint32_t y) {
+    return ((y >> 16) & 0x7ffu) | ((y >> 30) << 11);
+}
 
 struct Plan {
     int32_t x, y, sx, sy, plane, conj;
@@ -235,7 +241,8 @@ __device__ __forceinline__ uint2 pack_plan(const PlanParams &p, const Plan &pl, 
         sx = static_cast<uint32_t>(pl.sx);
         sy = static_cast<uint32_t>(pl.sy);
     }
-    r.y = (sx & 0xffu) | ((sy & 0xffu) << 8) | ((static_cast<uint32_t>(pl.plane) & 0x7ffu) << 16) |
+    const uint32_t table = static_cast<uint32_t>(pl.plane);
+    r.y = (sx & 0xffu) | ((sy & 0xffu) << 8) | ((table & 0x7ffu) << 16) | ((table >> 11) << 30) |
           (pl.conj ? kConjBit : 0u) | flags;
     return r;
 }

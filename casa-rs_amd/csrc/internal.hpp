@@ -160,6 +160,10 @@ struct chip_gridder {
     chip::DevBuf d_aw_n64;     // cells x (2R+1)^2 double2: unconjugated Complex64 normalisation
     chip::DevBuf d_aw_mag;     // cells x (2R+1)^2 double: |grid_normalization|, 0 where invalid
     chip::DevBuf d_aw_tnorm;   // tables x (2R+1)^2 float2: normalisation with the table's w sign
+    // Airy atlas built on the device: W planes per CF frequency and the CF
+    // frequencies (cell f * aw_n_w + k is frequency f, W plane k)
+    uint32_t aw_n_w = 0;
+    std::vector<double> aw_cf_freq;
 };
 
 namespace chip {
@@ -257,7 +261,9 @@ void hogbom(chip_ctx &ctx, const chip_hogbom_params &hp, uint64_t n_planes, uint
 void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, const float *pixels_dev,
                uint64_t n_pixels, uint32_t sampling, uint32_t n_pointings, const double *gradients,
                double w_increment);
-void attach_aw_airy(chip_gridder &g, const chip_airy_params &p, uint32_t n_pointings, const double *gradients);
+void attach_aw_airy(chip_gridder &g, const chip_airy_params &p, uint32_t n_freq, const double *freqs,
+                    uint32_t n_pointings, const double *gradients);
+void aw_select_cells(chip_gridder &g, const chip_samples &cs, uint32_t *out);
 void aw_plan_raw(chip_gridder &g, const Source &src, int32_t *plan, float *n32, double *n64);
 double airy_max_radius(double dish, double blockage);
 std::vector<float> airy_table(double dish, double blockage);
@@ -353,6 +359,7 @@ struct chip_vset {
     std::shared_ptr<chip::CycleWorkspace> cycle;
     int cycle_strategy = CHIP_CYCLE_AUTO;
     bool cycle_graph = false;  // major cycles replay a captured HIP graph
+    bool cycle_passes = false;  // eager fused cycles time each pass with events
     chip::DevBuf cycle_norm;  // per channel: sumwt (f32), then psf peak
     ~chip_vset() {
         if (ev_start) (void)hipEventDestroy(ev_start);

@@ -243,7 +243,7 @@ __global__ void k_group_records(const uint32_t *gstart, const uint2 *plan, uint3
     uint32_t w = pk.y;
     if (supports) {
         const int ox = static_cast<int>(pk.y & 0xffu) - 128, oy = static_cast<int>((pk.y >> 8) & 0xffu) - 128;
-        const uint32_t table = (pk.y >> 16) & 0x7ffu;
+        const uint32_t table = packed_table(pk.y);
         const uint32_t po = (table * n_off + static_cast<uint32_t>(ox + R)) * n_off + static_cast<uint32_t>(oy + R);
         w = po | ((pk.y & kConjBit) ? (1u << kRecPoBits) : 0u) | (static_cast<uint32_t>(supports[table]) << 26);
     }
@@ -554,7 +554,7 @@ __global__ void k_rekey(const uint2 *plan, const uint32_t *index, uint64_t n, Bi
     const uint32_t chan = b.n_chan == 1 ? 0u : index[p] % b.n_chan;
     const uint32_t ax = (pk.x & 0xffffu) + (wproject ? 0u : static_cast<uint32_t>(kSupport));
     const uint32_t ay = (pk.x >> 16) + (wproject ? 0u : static_cast<uint32_t>(kSupport));
-    const uint32_t plane = (pk.y >> 16) & 0x7ffu;
+    const uint32_t plane = packed_table(pk.y);
     const uint32_t tile = (ax / b.tile) * b.tiles_y + (ay / b.tile);
     uint64_t bin = static_cast<uint64_t>(chan) * b.n_tiles + tile;
     if (b.chunk_planes) bin += static_cast<uint64_t>(plane / b.chunk_planes) * b.n_chan * b.n_tiles;

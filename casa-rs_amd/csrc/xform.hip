@@ -731,18 +731,21 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     return true;
 }
 
-void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st) {
+void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st,
+                   hipEvent_t between) {
     const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
     const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
     CHIP_XF_LAUNCH(p.e_y, k_fwd_rows, rows, kRowThreads, rl, st, p.xg, p.ay, model, half);
+    if (between) CHIP_HIP(hipEventRecord(between, st));
     CHIP_XF_LAUNCH(p.e_x, k_fwd_cols, cols, kThreads, cl, st, p.xg, p.ax, half, grid);
 }
 
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows_buf, const float *nsum,
-                      const float *peak, float *out, hipStream_t st) {
+                      const float *peak, float *out, hipStream_t st, hipEvent_t between) {
     const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
     const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
     CHIP_XF_LAUNCH(p.e_x, k_inv_cols, cols, kThreads, cl, st, p.xg, p.ax, grid, rows_buf);
+    if (between) CHIP_HIP(hipEventRecord(between, st));
     CHIP_XF_LAUNCH(p.e_y, k_inv_rows, rows, kRowThreads, rl, st, p.xg, p.ay, rows_buf, nsum, peak, out);
 }
 

@@ -90,13 +90,16 @@ inline uint64_t grid_elements(const XPlan &p) { return static_cast<uint64_t>(p.n
 
 // model [chan][nx][ny] -> half spectrum (in `half`) -> paired grid: the raw
 // (unshifted) FFT of the checkerboard-signed, apodised, zero-padded model.
-void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st);
+// `between` (may be null) is recorded on st between the two passes.
+void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st,
+                   hipEvent_t between = nullptr);
 
 // paired residual grid -> Hermitian half of the x-inverse (in `rows`) ->
 // residual image [chan][nx][ny], corrected and normalised (nsum, peak: n_chan
 // device floats).
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows, const float *nsum,
-                      const float *peak, float *out, hipStream_t st);
+                      const float *peak, float *out, hipStream_t st,
+                      hipEvent_t between = nullptr);
 
 // Zeroes the upper blocks of a paired grid cube in the rows the coverage
 // mask marks (every row without a mask).

@@ -467,14 +467,30 @@ class StandardGridder:
 
     def attach_aw_airy(self, dish_m, blockage_m, freq_hz, n_w, max_abs_w, sampling=8, pa_rad=0.0,
                        phase_gradients=((0.0, 0.0),)):
-        """Build Airy A-term x W CF cells on the device and attach them."""
-        p = abi.AiryParams(float(dish_m), float(blockage_m), float(freq_hz), float(pa_rad), int(sampling),
+        """Build Airy A-term x W CF cells on the device and attach them.
+        ``freq_hz``: one frequency, or a sequence of CF frequencies (cell f *
+        n_w + k; samples then select theirs with aw_select_cells)."""
+        freqs = np.atleast_1d(np.asarray(freq_hz, np.float64))
+        p = abi.AiryParams(float(dish_m), float(blockage_m), float(freqs[0]), float(pa_rad), int(sampling),
                            int(n_w), float(max_abs_w))
         g = np.ascontiguousarray(phase_gradients, np.float64).reshape(-1)
-        abi.call("chip_gridder_attach_aw_airy", self.handle, ctypes.byref(p), len(g) // 2,
-                 g.ctypes.data_as(ctypes.c_void_p))
+        if np.ndim(freq_hz) == 0:
+            abi.call("chip_gridder_attach_aw_airy", self.handle, ctypes.byref(p), len(g) // 2,
+                     g.ctypes.data_as(ctypes.c_void_p))
+        else:
+            freqs = np.ascontiguousarray(freqs)
+            abi.call("chip_gridder_attach_aw_airy_freqs", self.handle, ctypes.byref(p), len(freqs),
+                     freqs.ctypes.data_as(ctypes.c_void_p), len(g) // 2, g.ctypes.data_as(ctypes.c_void_p))
         self._refresh()
         return self
+
+    def aw_select_cells(self, batch):
+        """CF cell of every sample of a RowChannelBatch (nearest CF frequency,
+        W plane): int32 device tensor (rows, chan) for batch.cf_cell."""
+        s, keep = batch.to_samples()
+        out = torch.empty((s.n_rows, s.n_chan), dtype=torch.int32, device=self.ctx.torch_device)
+        abi.call("chip_aw_select_cells", self.handle, ctypes.byref(s), _ptr(out))
+        return out
 
     def aw_cells(self):
         """The attached atlas: list of (pixels complex64 (shape_x, shape_y), x_support, y_support)."""
@@ -750,6 +766,15 @@ class VisibilitySet:
         n = ctypes.c_uint64()
         abi.call("chip_vset_cycle_graph", self.handle, 1 if enable else 0, ctypes.byref(n))
         return n.value
+
+    def cycle_passes(self, enable: int = -1):
+        """Per-pass device ms of the last timed eager fused major cycle:
+        (forward rows, forward columns, zero fill, binned kernel, inverse
+        columns, inverse rows); enable 1 / 0 turns the timing of later cycles
+        on / off, -1 leaves it."""
+        ms = (ctypes.c_double * 6)()
+        abi.call("chip_vset_cycle_passes", self.handle, int(enable), ms)
+        return list(ms)
 
     def cycle_replays(self) -> int:
         """Major cycles run by graph replay so far."""

@@ -254,6 +254,22 @@ int chip_gridder_attach_aw(chip_gridder *g, uint32_t n_cells, const chip_cf_cell
  * even shape 2*(S*sampling + R + 1), S its support from the 1e-3 threshold. */
 int chip_gridder_attach_aw_airy(chip_gridder *g, const chip_airy_params *p, uint32_t n_pointings,
                                 const double *phase_gradients, char **err);
+/* The Airy CF cells of chip_gridder_attach_aw_airy at n_freq CF frequencies
+ * (p->freq_hz is not read): cell f * n_w + k is frequency freq_hz[f], W plane
+ * k, each frequency normalised by its own plane-zero peak and integral; an
+ * AWProject cache with a frequency axis (AwConvolutionFunctionCache,
+ * awprojection.rs:920-1180).  Samples select their cell through cf_cell_dev
+ * (chip_aw_select_cells).  n_freq * n_w * pointings * 2 <= 8191 tap tables. */
+int chip_gridder_attach_aw_airy_freqs(chip_gridder *g, const chip_airy_params *p, uint32_t n_freq,
+                                      const double *freq_hz, uint32_t n_pointings, const double *phase_gradients,
+                                      char **err);
+/* CF cell of every sample of a rows x channels input for an atlas from
+ * chip_gridder_attach_aw_airy(_freqs): the CF frequency nearest to the
+ * sample's (lambda_scale * c) times n_w, plus the W plane round(sqrt(WIncr
+ * |w|)) clamped to n_w - 1 (select_key_for_sample, awprojection.rs:1127-1180,
+ * without a CASA cache's conjugate-beam and Mueller keys).  cf_cell_dev:
+ * uint32 per sample, for chip_samples.cf_cell_dev. */
+int chip_aw_select_cells(chip_gridder *g, const chip_samples *s, uint32_t *cf_cell_dev, char **err);
 /* Read back the attached atlas: cells (n_cells entries) and, when
  * pixels_dev != NULL, the pixels (complex64, n_pixels).  Either may be NULL. */
 int chip_aw_cells(const chip_gridder *g, uint32_t *n_cells, chip_cf_cell *cells, uint64_t *n_pixels,
@@ -457,8 +473,11 @@ int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
  * identical plans per work item (the splitting of each bin).  Unless fixed
  * here, the first major cycle times every candidate of {anchor, plan order} x
  * {the set's tile, 16, 32, 64 where they fit} x {256, 512} threads x {256,
- * 512, 1024, 2048} groups on its own grids and keeps the fastest (default
- * before that: 32 or the largest fitting tile, anchor order, 512, 1024).
+ * 512, 1024, 2048} groups on its own grids, in that order, and keeps the
+ * fastest; it starts no new candidate after 3 s of timed launches, and a
+ * candidate whose launch takes over 20 ms is timed once without a warm-up
+ * (default before that: 32 or the largest fitting tile, anchor order, 512,
+ * 1024).
  * tile = 0 keeps the current tile; tile = threads = groups_per_item = 0 asks
  * for autotuning again at the next major cycle.  Results do not depend on the
  * schedule beyond the order of float additions.  No reference counterpart
@@ -482,6 +501,15 @@ int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order
  * reference counterpart (the Metal path encodes one command buffer per
  * dispatch, lib.rs:72270-72290). */
 int chip_vset_cycle_graph(chip_vset *vs, int enable, uint64_t *replays, char **err);
+/* Per-pass timing of later eager major cycles with the fused strategy:
+ * enable = 1 records events before, between and after the six passes (a few
+ * microseconds per cycle), 0 stops, -1 leaves the setting unchanged.
+ * pass_ms (6 doubles, may be NULL) receives the last timed cycle's device ms
+ * of: forward rows, forward columns, zero fill, binned residual kernel,
+ * inverse columns, inverse rows (-1 before one was timed).  No reference
+ * counterpart (the reference's ImagingStageTimings, lib.rs:3496-3503, time
+ * host stages). */
+int chip_vset_cycle_passes(chip_vset *vs, int enable, double *pass_ms, char **err);
 /* Strategy in use (-1 before the first major cycle); shape[3] of the fused
  * passes (columns per column-pass workgroup, its threads, rows per rows-pass
  * workgroup; zeros for hipFFT); the timed forward + inverse transform ms of

@@ -58,6 +58,33 @@ def lib():
     return _lib
 
 
+def use_native_build(directory) -> str:
+    """Compile the oracle for this host (-march=native, same floating-point
+    flags) into ``directory`` and route every later call through it; used to
+    time the CPU baseline on the host that runs it.  Returns the path."""
+    import subprocess
+
+    global _lib
+    subprocess.run(["make", "-s", "-C", _HERE, "native", f"NATIVE_DIR={directory}"], check=True)
+    path = os.path.join(directory, "libcasa_oracle_native.so")
+    L = ctypes.CDLL(path)
+    _declare(L)
+    _lib = L
+    return path
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 class _Std(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_uint64), ("ny", ctypes.c_uint64),

@@ -276,3 +276,38 @@ def test_airy_cells_match_numpy(gpu, oracle_mod, pa):
     core = c0[1:, 1:]
     assert np.max(np.abs(core - core[::-1, ::-1])) <= 1e-4 * np.max(np.abs(core))
     assert np.max(np.abs(c0.imag)) <= 1e-4 * np.max(np.abs(c0))
+
+
+def test_airy_frequency_atlas_and_cell_selection(gpu):
+    # an atlas at several CF frequencies holds, cell for cell, the atlases
+    # built at each frequency alone (cell f * n_w + k), and every sample of a
+    # rows x channels batch selects the nearest CF frequency and its W plane
+    # round(sqrt(WIncr |w|)) (select_key_for_sample, awprojection.rs:1127-1180)
+    nx, cell, s, n_w, max_w = 96, np.deg2rad(60.0 / 3600.0), 4, 5, 800.0
+    freqs = np.array([1.05e9, 1.2e9, 1.35e9])
+    multi = gpu.StandardGridder(nx, nx, cell, cell)
+    multi.attach_aw_airy(25.0, 2.0, freqs, n_w, max_w, sampling=s)
+    cells = multi.aw_cells()
+    assert len(cells) == len(freqs) * n_w
+    for f, fr in enumerate(freqs):
+        one = gpu.StandardGridder(nx, nx, cell, cell)
+        one.attach_aw_airy(25.0, 2.0, float(fr), n_w, max_w, sampling=s)
+        for k, (c, xs, ys) in enumerate(one.aw_cells()):
+            m, mx, my = cells[f * n_w + k]
+            assert (mx, my) == (xs, ys)
+            assert torch.equal(m, c)
+    rng = np.random.default_rng(3)
+    n_rows = 500
+    chan = np.linspace(1.0e9, 1.4e9, 7)
+    ls = chan / 299_792_458.0
+    uvw = np.stack([rng.uniform(-300, 300, n_rows), rng.uniform(-300, 300, n_rows),
+                    rng.uniform(-250, 250, n_rows)], axis=1)
+    batch = gpu.RowChannelBatch(torch.from_numpy(uvw), torch.from_numpy(ls),
+                                torch.zeros((n_rows, len(chan)), dtype=torch.complex64),
+                                torch.ones((n_rows, len(chan)), dtype=torch.float32))
+    got = multi.aw_select_cells(batch).cpu().numpy()
+    nearest = np.argmin(np.abs(freqs[None, :] - (ls * 299_792_458.0)[:, None]), axis=1)
+    w = uvw[:, 2:3] * ls[None, :]
+    plane = np.clip(np.round(np.sqrt(multi.info.w_scale * np.abs(w))), 0, n_w - 1).astype(np.int64)
+    assert np.array_equal(got, nearest[None, :] * n_w + plane)
+    assert len(np.unique(got)) > n_w
