@@ -12,8 +12,9 @@ from .abi import (CLEAN_BUDGET, CLEAN_CYCLE, CLEAN_GLOBAL, CLEAN_NO_PIXELS, CLEA
                   CYCLE_HIPFFT, F32, F64, MODE_DIRTY, MODE_PSF, MODE_RESIDUAL, WEIGHT_BRIGGS,
                   WEIGHT_UNIFORM, ImagingError, InvalidRequest, NoUsableSamples, Normalization,
                   Unsupported)
-from .imaging import (Context, RowChannelBatch, RowIngest, StandardGridder, StandardMfsModelPredictor,
-                      VisibilityBatch, VisibilitySet,
+from .imaging import (PACKED_W_SAMPLE, Context, HostBuffer, RowChannelBatch, RowIngest, StandardGridder,
+                      StandardMfsModelPredictor, VisibilityBatch, VisibilitySet, grid_wproject_samples_host,
+                      pack_wproject_samples,
                       airy_table, default_context, mosaic_phase_gradient, phase_rotation,
                       rotate_rows, run_hogbom_minor_cycle)
 
@@ -25,4 +26,5 @@ __all__ = [
     "AW_OK", "AW_NON_FINITE_COORDINATE", "AW_OUTSIDE_GRID", "AW_KERNEL_INDEX_OUTSIDE_CELL",
     "AW_INVALID_NORMALIZATION", "AW_NO_CELL", "run_hogbom_minor_cycle", "CLEAN_BUDGET", "CLEAN_GLOBAL",
     "CLEAN_NSIGMA", "CLEAN_CYCLE", "CLEAN_NO_PIXELS", "RowIngest", "StandardMfsModelPredictor",
+    "PACKED_W_SAMPLE", "HostBuffer", "grid_wproject_samples_host", "pack_wproject_samples",
 ]

@@ -55,7 +55,8 @@ EXPORTS = (
     "chip_ctx_create", "chip_ctx_destroy", "chip_ctx_set_stream", "chip_error_free", "chip_version",
     "chip_gridder_create", "chip_gridder_destroy", "chip_gridder_info_get", "chip_gridder_tables",
     "chip_max_abs_w", "chip_gridder_attach_wproject", "chip_wproject_tables", "chip_gridder_wkernel_budget",
-    "chip_plan",
+    "chip_plan", "chip_buffer_alloc", "chip_buffer_free", "chip_buffer_upload", "chip_buffer_download",
+    "chip_buffer_zero", "chip_gridder_attach_wproject_tables", "chip_grid_wproject_samples",
     "chip_density_weights", "chip_phase_rotation", "chip_rotate_rows",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
@@ -251,6 +252,14 @@ def load():
         "chip_model_to_grid": (ctypes.c_int, [_P, _U32, _U64, _P, _P, _PP]),
         "chip_grid_to_model_adjoint": (ctypes.c_int, [_P, _U64, _P, _P, _PP]),
         "chip_predict_model": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _U64, _P, _PP]),
+        "chip_buffer_alloc": (ctypes.c_int, [_P, _U64, _PP, _PP]),
+        "chip_buffer_free": (None, [_P, _P]),
+        "chip_buffer_upload": (ctypes.c_int, [_P, _P, _P, _U64, _PP]),
+        "chip_buffer_download": (ctypes.c_int, [_P, _P, _P, _U64, _PP]),
+        "chip_buffer_zero": (ctypes.c_int, [_P, _P, _U64, _PP]),
+        "chip_gridder_attach_wproject_tables": (ctypes.c_int, [_P, _U32, _U32, ctypes.c_double, _U32, _P, _P, _P,
+                                                               _PP]),
+        "chip_grid_wproject_samples": (ctypes.c_int, [_P, _U64, _P, _P, _P, _P, _U32, _PP]),
         "chip_gridder_attach_aw": (ctypes.c_int, [_P, _U32, ctypes.POINTER(CfCell), _P, _U64, _U32, _U32, _P,
                                                   ctypes.c_double, _PP]),
         "chip_gridder_attach_aw_airy": (ctypes.c_int, [_P, ctypes.POINTER(AiryParams), _U32, _P, _PP]),

@@ -301,9 +301,71 @@ int chip_wproject_tables(const chip_gridder *g, int32_t *supports, float *kernel
     });
 }
 
+int chip_gridder_attach_wproject_tables(chip_gridder *g, uint32_t planes, uint32_t sampling, double w_scale,
+                                        uint32_t width, const int32_t *supports, const float *kernels,
+                                        const float *norm_table, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        set_device(g->ctx);
+        attach_wproject_tables(*g, planes, sampling, w_scale, width, supports, kernels, norm_table);
+    });
+}
+
+int chip_grid_wproject_samples(chip_gridder *g, uint64_t n, const void *samples_dev, const float *model_dev,
+                               float *psf_dev, float *residual_dev, uint32_t mode, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        set_device(g->ctx);
+        grid_wproject_samples(*g, n, samples_dev, model_dev, psf_dev, residual_dev, mode);
+    });
+}
+
+int chip_buffer_alloc(chip_ctx *ctx, uint64_t bytes, void **dev_out, char **err) {
+    return guarded(err, [&] {
+        if (!ctx || !dev_out) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        *dev_out = nullptr;
+        if (bytes) CHIP_HIP(hipMalloc(dev_out, bytes));
+    });
+}
+
+void chip_buffer_free(chip_ctx *ctx, void *dev) {
+    if (!ctx || !dev) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipFree(dev);
+}
+
+int chip_buffer_upload(chip_ctx *ctx, void *dev, const void *host, uint64_t bytes, char **err) {
+    return guarded(err, [&] {
+        if (!ctx || (bytes && (!dev || !host))) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        if (bytes) CHIP_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+        CHIP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int chip_buffer_download(chip_ctx *ctx, void *host, const void *dev, uint64_t bytes, char **err) {
+    return guarded(err, [&] {
+        if (!ctx || (bytes && (!dev || !host))) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        if (bytes) CHIP_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        CHIP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int chip_buffer_zero(chip_ctx *ctx, void *dev, uint64_t bytes, char **err) {
+    return guarded(err, [&] {
+        if (!ctx || (bytes && !dev)) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(ctx);
+        if (bytes) CHIP_HIP(hipMemsetAsync(dev, 0, bytes, ctx->stream));
+        CHIP_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 int chip_gridder_wkernel_budget(chip_gridder *g, uint64_t max_table_bytes, uint32_t *chunk_planes, char **err) {
     return guarded(err, [&] {
         if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        if (g->wk_external) fail(CHIP_ERR_INVALID, "W tables given by the caller stay resident");
         set_device(g->ctx);
         g->wk_budget = max_table_bytes;
         if (g->wproject && !g->aw)

@@ -400,6 +400,124 @@ void grid_plain_noalloc(chip_gridder &g, const Source &src, uint32_t mode, uint3
     CHIP_HIP(hipStreamSynchronize(st));
 }
 
+// ---------------------------------------------------------------------------
+// Packed, pre-planned W samples (the reference's Metal seam input).
+// ---------------------------------------------------------------------------
+// WProjectMetalSample (lib.rs:71629-71635, packed by
+// w_project_metal_sample_from_plan, :71708-71754): loc_x | loc_y << 16;
+// plane (10 bits) | support << 10 (5 bits) | conjugate << 15 |
+// (off_x + 128) << 16 | (off_y + 128) << 24; weight; visibility.
+struct PackedWSample {
+    uint32_t loc, pso;
+    float weight, vr, vi;
+};
+static_assert(sizeof(PackedWSample) == 20, "the reference's packed sample is 20 bytes");
+
+struct SeamParams {
+    uint32_t gx, gy, width, sampling, planes, mode;
+    uint64_t n;
+    const float2 *kernels;
+};
+
+// One wavefront per sample, lanes over the taps (iy outer, ix inner): the
+// arithmetic of W_PROJECT_METAL_SHADER (lib.rs:72656-72724) -- cwt =
+// K[|ix s + off_x|][|iy s + off_y|], conjugated when flagged; prediction sum
+// of conj(cwt) * model; PSF += weight * cwt; residual += weight * ((V - p) *
+// cwt).  Samples whose footprint leaves the grid or the kernel are counted in
+// *rejected and not gridded.
+__global__ __launch_bounds__(256) void k_seam(SeamParams p, const PackedWSample *samples, const float2 *model,
+                                              float *psf, float *res, unsigned int *rejected) {
+    const uint64_t i = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (i >= p.n) return;
+    const PackedWSample sm = samples[i];
+    const int lx = static_cast<int>(sm.loc & 0xffffu), ly = static_cast<int>(sm.loc >> 16);
+    const uint32_t plane = sm.pso & 0x3ffu;
+    const int S = static_cast<int>((sm.pso >> 10) & 0x1fu);
+    const bool conj = (sm.pso >> 15) & 1u;
+    const int ox = static_cast<int>((sm.pso >> 16) & 0xffu) - 128, oy = static_cast<int>(sm.pso >> 24) - 128;
+    const int s = static_cast<int>(p.sampling);
+    const bool ok = plane < p.planes && lx - S >= 0 && ly - S >= 0 && lx + S < static_cast<int>(p.gx) &&
+                    ly + S < static_cast<int>(p.gy) && S * s + (ox < 0 ? -ox : ox) < static_cast<int>(p.width) &&
+                    S * s + (oy < 0 ? -oy : oy) < static_cast<int>(p.width);
+    if (!ok) {
+        if (lane == 0) atomicAdd(rejected, 1u);
+        return;
+    }
+    const int side = 2 * S + 1, T = side * side;
+    const float2 *K = p.kernels + static_cast<uint64_t>(plane) * p.width * p.width;
+    auto tap = [&](int t, int &cell) {
+        const int jy = t / side, jx = t - jy * side;
+        const int iy = jy - S, ix = jx - S;
+        const int kx = abs(ix * s + ox), ky = abs(iy * s + oy);
+        float2 c = K[static_cast<uint64_t>(kx) * p.width + ky];
+        if (conj) c.y = -c.y;
+        cell = (lx + ix) * static_cast<int>(p.gy) + (ly + iy);
+        return c;
+    };
+    float rr = sm.vr, ri = sm.vi;
+    if (p.mode == CHIP_MODE_RESIDUAL && model) {
+        float pr = 0.0f, pi = 0.0f;
+        for (int t = lane; t < T; t += 64) {
+            int cell;
+            const float2 c = tap(t, cell);
+            const float2 m = model[cell];
+            pr += c.x * m.x + c.y * m.y;
+            pi += c.x * m.y - c.y * m.x;
+        }
+        wave_sum_f2(pr, pi);
+        rr = sm.vr - pr;
+        ri = sm.vi - pi;
+    }
+    const bool wpsf = p.mode == CHIP_MODE_PSF || p.mode == CHIP_MODE_DIRTY;
+    const bool wres = p.mode == CHIP_MODE_DIRTY || p.mode == CHIP_MODE_RESIDUAL;
+    for (int t = lane; t < T; t += 64) {
+        int cell;
+        const float2 c = tap(t, cell);
+        if (wpsf) {
+            unsafeAtomicAdd(psf + 2 * static_cast<uint64_t>(cell), sm.weight * c.x);
+            unsafeAtomicAdd(psf + 2 * static_cast<uint64_t>(cell) + 1, sm.weight * c.y);
+        }
+        if (wres) {
+            unsafeAtomicAdd(res + 2 * static_cast<uint64_t>(cell), sm.weight * (rr * c.x - ri * c.y));
+            unsafeAtomicAdd(res + 2 * static_cast<uint64_t>(cell) + 1, sm.weight * (rr * c.y + ri * c.x));
+        }
+    }
+}
+
+void grid_wproject_samples(chip_gridder &g, uint64_t n, const void *samples, const float *model, float *psf,
+                           float *residual, uint32_t mode) {
+    if (!g.wproject || g.aw) fail(CHIP_ERR_INVALID, "packed W samples need a W projection (not AW)");
+    if (g.wk_chunk) fail(CHIP_ERR_INVALID, "packed W samples need resident W tables");
+    if (mode > CHIP_MODE_RESIDUAL) fail(CHIP_ERR_INVALID, "unknown grid mode");
+    if ((mode != CHIP_MODE_RESIDUAL && !psf) || (mode != CHIP_MODE_PSF && !residual))
+        fail(CHIP_ERR_INVALID, "the mode's grids are required");
+    if (n == 0) return;
+    if (!samples) fail(CHIP_ERR_INVALID, "NULL samples");
+    if (n > (1ull << 31)) fail(CHIP_ERR_INVALID, "at most 2^31 packed samples per call");
+    SeamParams p{};
+    p.gx = static_cast<uint32_t>(g.gx);
+    p.gy = static_cast<uint32_t>(g.gy);
+    p.width = g.width;
+    p.sampling = g.sampling;
+    p.planes = g.planes;
+    p.mode = mode;
+    p.n = n;
+    p.kernels = g.d_kernels.as<float2>();
+    hipStream_t st = g.ctx->stream;
+    DevBuf rej(sizeof(unsigned int));
+    CHIP_HIP(hipMemsetAsync(rej.p, 0, rej.bytes, st));
+    hipLaunchKernelGGL(k_seam, dim3(static_cast<unsigned>((n * 64 + 255) / 256)), dim3(256), 0, st, p,
+                       static_cast<const PackedWSample *>(samples), reinterpret_cast<const float2 *>(model), psf,
+                       residual, rej.as<unsigned int>());
+    CHIP_HIP(hipGetLastError());
+    unsigned int bad = 0;
+    CHIP_HIP(hipMemcpyAsync(&bad, rej.p, sizeof(bad), hipMemcpyDeviceToHost, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+    if (bad)
+        fail(CHIP_ERR_INVALID, std::to_string(bad) + " packed samples lie outside the grid or the W kernels");
+}
+
 void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,
                   void *out) {
     if (!model || !out) fail(CHIP_ERR_INVALID, "degrid needs a model grid and an output");

@@ -139,6 +139,7 @@ struct chip_gridder {
     // resident.  wk holds what the recomputation needs.
     uint64_t wk_budget = 0;
     uint32_t wk_chunk = 0;
+    bool wk_external = false;  // tables given by the caller (chip_gridder_attach_wproject_tables)
     chip::WkState wk;
     std::vector<float> sinc;        // build_sinc_axis(max(gx, gy), sampling)
     std::vector<float> sinc_x, sinc_y;  // per-axis sinc used by model apodisation
@@ -243,6 +244,10 @@ void grid_plain(chip_gridder &g, const Source &src, uint32_t mode, uint32_t prec
 void degrid_plain(chip_gridder &g, const Source &src, uint32_t precision, const void *model,
                   void *out);
 void noalloc_scratch_init(int device);
+void attach_wproject_tables(chip_gridder &g, uint32_t planes, uint32_t sampling, double w_scale, uint32_t width,
+                            const int32_t *supports, const float *kernels_host, const float *norm_host);
+void grid_wproject_samples(chip_gridder &g, uint64_t n, const void *samples, const float *model, float *psf,
+                           float *residual, uint32_t mode);
 void grid_plain_noalloc(chip_gridder &g, const Source &src, uint32_t mode, uint32_t precision,
                         const void *model, void *psf, void *dirty, chip_sums *sums);
 

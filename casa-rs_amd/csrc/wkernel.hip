@@ -360,6 +360,89 @@ void build_wproject(chip_gridder &g, double max_abs_w, uint32_t explicit_planes,
     CHIP_HIP(hipStreamSynchronize(st));
 }
 
+// WProjector over tables the caller built (the reference's own projector:
+// WProjector::flattened_kernel_weights, kernel_support, w_scale,
+// gridder.rs:3640-3678): the quarter-plane kernels are uploaded as they are,
+// the per-offset gathered taps are made on the device, and the per-(plane,
+// offset) normalisation table is taken from the caller or, when NULL,
+// computed on the device from the kernels (compute_w_project_sample_
+// normalization, gridder.rs:3782-3799).
+void attach_wproject_tables(chip_gridder &g, uint32_t planes, uint32_t sampling, double w_scale, uint32_t width,
+                            const int32_t *supports, const float *kernels_host, const float *norm_host) {
+    if (g.wproject) fail(CHIP_ERR_INVALID, "W projection already attached");
+    if (planes == 0 || planes > 2047) fail(CHIP_ERR_INVALID, "planes must be 1..2047");
+    if (sampling == 0 || sampling > 64) fail(CHIP_ERR_INVALID, "sampling must be 1..64");
+    if (!(std::isfinite(w_scale) && w_scale > 0.0)) fail(CHIP_ERR_INVALID, "w_scale must be finite and positive");
+    if (!supports || !kernels_host) fail(CHIP_ERR_INVALID, "NULL supports or kernels");
+    if (width == 0 || width > 4096) fail(CHIP_ERR_INVALID, "kernel width must be 1..4096");
+    const int32_t R = static_cast<int32_t>((sampling + 1) / 2);
+    uint32_t max_support = 0;
+    for (uint32_t k = 0; k < planes; ++k) {
+        if (supports[k] <= 0 || supports[k] > 63) fail(CHIP_ERR_INVALID, "W supports must be 1..63");
+        // taps read |i*s + off| <= support*s + R (gridder.rs:3686-3692)
+        if (static_cast<uint64_t>(supports[k]) * sampling + R >= width)
+            fail(CHIP_ERR_INVALID, "a W support reaches beyond the kernel width");
+        max_support = std::max(max_support, static_cast<uint32_t>(supports[k]));
+    }
+    hipStream_t st = g.ctx->stream;
+    const uint32_t axis = static_cast<uint32_t>(2 * R + 1);
+    const uint32_t stride = (2 * max_support + 1) * (2 * max_support + 1);
+    const uint64_t kcells = static_cast<uint64_t>(width) * width;
+    DevBuf kernels(kcells * planes * sizeof(float2));
+    DevBuf compact(static_cast<uint64_t>(planes) * axis * axis * stride * sizeof(float2));
+    DevBuf d_sup(planes * sizeof(int32_t));
+    DevBuf d_norm(static_cast<uint64_t>(planes) * axis * axis * sizeof(float));
+    CHIP_HIP(hipMemcpyAsync(kernels.p, kernels_host, kernels.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipMemcpyAsync(d_sup.p, supports, d_sup.bytes, hipMemcpyHostToDevice, st));
+    if (norm_host) {
+        CHIP_HIP(hipMemcpyAsync(d_norm.p, norm_host, d_norm.bytes, hipMemcpyHostToDevice, st));
+    } else {
+        hipLaunchKernelGGL(k_norm_table, dim3(nb(static_cast<uint64_t>(planes) * axis * axis, 128)), dim3(128), 0, st,
+                           kernels.as<float2>(), d_sup.as<int32_t>(), planes, width, sampling, R, d_norm.as<float>());
+        CHIP_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_compact, dim3(nb(static_cast<uint64_t>(planes) * axis * axis * stride, 256)), dim3(256), 0, st,
+                       kernels.as<float2>(), d_sup.as<int32_t>(), planes, width, sampling, R, stride,
+                       compact.as<float2>());
+    CHIP_HIP(hipGetLastError());
+    std::vector<float> norm(static_cast<uint64_t>(planes) * axis * axis);
+    CHIP_HIP(hipMemcpyAsync(norm.data(), d_norm.p, d_norm.bytes, hipMemcpyDeviceToHost, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+    g.wproject = true;
+    g.planes = planes;
+    g.sampling = sampling;
+    g.width = width;
+    g.max_support = max_support;
+    g.norm_radius = R;
+    g.w_scale = w_scale;
+    g.supports.assign(supports, supports + planes);
+    g.norm_table = norm;
+    g.d_kernels = std::move(kernels);
+    g.d_compact = std::move(compact);
+    g.tap_stride = stride;
+    g.d_supports = std::move(d_sup);
+    g.d_norm_table = std::move(d_norm);
+    g.wk_chunk = 0;
+    g.wk_external = true;
+    g.wk = WkState{};
+    g.wk.planes = planes;
+    g.wk.sampling = sampling;
+    g.wk.R = R;
+    g.wk.width = width;
+    g.wk.stride = stride;
+    g.wk.w_scale = w_scale;
+    g.sinc = sinc_axis(std::max(g.gx, g.gy), sampling);
+    g.sinc_x = sinc_axis(g.gx, sampling);
+    g.sinc_y = sinc_axis(g.gy, sampling);
+    g.d_sinc.alloc(g.sinc.size() * sizeof(float));
+    g.d_sinc_x.alloc(g.sinc_x.size() * sizeof(float));
+    g.d_sinc_y.alloc(g.sinc_y.size() * sizeof(float));
+    CHIP_HIP(hipMemcpyAsync(g.d_sinc.p, g.sinc.data(), g.d_sinc.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipMemcpyAsync(g.d_sinc_x.p, g.sinc_x.data(), g.d_sinc_x.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipMemcpyAsync(g.d_sinc_y.p, g.sinc_y.data(), g.d_sinc_y.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+}
+
 void wk_prepare(const chip_gridder &g, WkScratch &ws) {
     const WkState &wk = g.wk;
     const uint64_t axis = static_cast<uint64_t>(2 * wk.R + 1);

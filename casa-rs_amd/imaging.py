@@ -423,6 +423,19 @@ class StandardGridder:
         self._refresh()
         return self
 
+    def attach_wproject_tables(self, kernels, supports, w_scale, sampling, norm_table=None):
+        """WProjector over the caller's tables (host numpy arrays: kernels
+        (planes, width, width) complex64, supports int32, w_scale, sampling;
+        norm_table optional), copied to the device."""
+        k = np.ascontiguousarray(kernels, np.complex64)
+        sup = np.ascontiguousarray(supports, np.int32)
+        nt = None if norm_table is None else np.ascontiguousarray(norm_table, np.float32)
+        abi.call("chip_gridder_attach_wproject_tables", self.handle, k.shape[0], int(sampling), float(w_scale),
+                 k.shape[1], sup.ctypes.data_as(ctypes.c_void_p), k.ctypes.data_as(ctypes.c_void_p),
+                 None if nt is None else nt.ctypes.data_as(ctypes.c_void_p))
+        self._refresh()
+        return self
+
     def set_wkernel_budget(self, max_table_bytes: int) -> int:
         """Residency of the W tables: 0 keeps them resident; otherwise tables
         above max_table_bytes are recomputed by every gridding / degridding
@@ -673,6 +686,101 @@ class StandardMfsModelPredictor:
 
     def close(self):
         self.gridder.close()
+
+
+# ---------------------------------------------------------------------------
+# Host arrays (what a caller that links only the library does: no torch)
+# ---------------------------------------------------------------------------
+
+# WProjectMetalSample (lib.rs:71629-71635), 20 bytes
+PACKED_W_SAMPLE = np.dtype([("loc", "<u4"), ("plane_support_offsets", "<u4"), ("weight", "<f4"),
+                            ("visibility_re", "<f4"), ("visibility_im", "<f4")])
+
+
+def pack_wproject_samples(loc_x, loc_y, off_x, off_y, plane, conj, support, weight, vis):
+    """w_project_metal_sample_from_plan (lib.rs:71708-71754) over arrays."""
+    out = np.zeros(len(loc_x), PACKED_W_SAMPLE)
+    out["loc"] = np.asarray(loc_x, np.uint32) | (np.asarray(loc_y, np.uint32) << 16)
+    out["plane_support_offsets"] = (np.asarray(plane, np.uint32) | (np.asarray(support, np.uint32) << 10)
+                                    | (np.asarray(conj, np.uint32) << 15)
+                                    | ((np.asarray(off_x, np.int64) + 128).astype(np.uint32) << 16)
+                                    | ((np.asarray(off_y, np.int64) + 128).astype(np.uint32) << 24))
+    out["weight"] = weight
+    v = np.asarray(vis, np.complex64)
+    out["visibility_re"] = v.real
+    out["visibility_im"] = v.imag
+    return out
+
+
+class HostBuffer:
+    """Device memory moved to and from numpy arrays through chip_buffer_*."""
+
+    def __init__(self, ctx: Context, nbytes: int):
+        self.ctx, self.nbytes = ctx, int(nbytes)
+        p = ctypes.c_void_p()
+        abi.call("chip_buffer_alloc", ctx.handle, self.nbytes, ctypes.byref(p))
+        self.ptr = p
+
+    @classmethod
+    def of(cls, ctx, array):
+        a = np.ascontiguousarray(array)
+        b = cls(ctx, a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        abi.call("chip_buffer_upload", self.ctx.handle, self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes)
+
+    def download(self, array):
+        assert array.flags.c_contiguous and array.nbytes <= self.nbytes
+        abi.call("chip_buffer_download", self.ctx.handle, array.ctypes.data_as(ctypes.c_void_p), self.ptr,
+                 array.nbytes)
+        return array
+
+    def zero(self):
+        abi.call("chip_buffer_zero", self.ctx.handle, self.ptr, self.nbytes)
+
+    def free(self):
+        if self.ptr:
+            abi.load().chip_buffer_free(self.ctx.handle, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def grid_wproject_samples_host(gridder, samples, mode, model=None, psf=None, residual=None):
+    """accumulate_w_project_grid_metal_samples (lib.rs:71863-71871) over host
+    arrays: ``samples`` of PACKED_W_SAMPLE, ``model`` / ``psf`` / ``residual``
+    complex64 numpy grids (gx, gy); psf and residual are accumulated into in
+    place.  The path a crate linking only the library takes: device buffers,
+    upload, chip_grid_wproject_samples, download."""
+    ctx = gridder.ctx
+    samples = np.ascontiguousarray(samples, PACKED_W_SAMPLE)
+    bufs = []
+    try:
+        sb = HostBuffer.of(ctx, samples) if len(samples) else None
+        bufs.append(sb)
+        mb = HostBuffer.of(ctx, np.ascontiguousarray(model, np.complex64)) if model is not None else None
+        bufs.append(mb)
+        pb = HostBuffer.of(ctx, np.ascontiguousarray(psf, np.complex64)) if psf is not None else None
+        rb = HostBuffer.of(ctx, np.ascontiguousarray(residual, np.complex64)) if residual is not None else None
+        bufs += [pb, rb]
+        abi.call("chip_grid_wproject_samples", gridder.handle, len(samples), sb.ptr if sb else None,
+                 mb.ptr if mb else None, pb.ptr if pb else None, rb.ptr if rb else None, int(mode))
+        if pb is not None:
+            pb.download(psf)
+        if rb is not None:
+            rb.download(residual)
+    finally:
+        for b in bufs:
+            if b is not None:
+                b.free()
+    return psf, residual
 
 
 class VisibilitySet:

@@ -393,6 +393,51 @@ int chip_density_weights(chip_gridder *g, const chip_samples *s, uint32_t mode, 
                          uint32_t per_plane, float *weight_out_dev, float *f2_host, float *density_dev,
                          char **err);
 
+/* ---- host arrays: device buffers ---------------------------------------- */
+/* A crate that links only this library moves the caller's host arrays
+ * (borrowed for the call, as newBufferWithBytesNoCopy does in the Metal seam,
+ * lib.rs:72046-72064) through these: device memory of the context's device,
+ * copies and zero fill on the context's stream, each call synchronous.  No
+ * reference counterpart beyond that Metal buffer handling. */
+int chip_buffer_alloc(chip_ctx *ctx, uint64_t bytes, void **dev_out, char **err);
+void chip_buffer_free(chip_ctx *ctx, void *dev);
+int chip_buffer_upload(chip_ctx *ctx, void *dev, const void *host, uint64_t bytes, char **err);
+int chip_buffer_download(chip_ctx *ctx, void *host, const void *dev, uint64_t bytes, char **err);
+int chip_buffer_zero(chip_ctx *ctx, void *dev, uint64_t bytes, char **err);
+
+/* ---- the Metal W-projection seam, one to one ------------------------------ */
+/* WProjector over the caller's own tables (the reference's projector:
+ * flattened_kernel_weights planes x width x width complex64 quarter planes,
+ * kernel_support per plane, w_scale, sampling; gridder.rs:3640-3678), host
+ * arrays copied at the call.  norm_table (planes x (2R+1)^2 floats, R =
+ * (sampling + 1) / 2, WProjector::sample_normalization, gridder.rs:3759-3779)
+ * may be NULL: it is then computed on the device from the kernels.  Tables
+ * stay resident (chip_gridder_wkernel_budget refuses them). */
+int chip_gridder_attach_wproject_tables(chip_gridder *g, uint32_t planes, uint32_t sampling, double w_scale,
+                                        uint32_t width, const int32_t *supports, const float *kernels,
+                                        const float *norm_table, char **err);
+/* WProjectMetalSample (lib.rs:71629-71635): 20 bytes, packed by
+ * w_project_metal_sample_from_plan (lib.rs:71708-71754). */
+typedef struct {
+    uint32_t loc;                   /* loc_x | loc_y << 16 */
+    uint32_t plane_support_offsets; /* plane (10 bits) | support << 10 (5) | conjugate << 15 |
+                                       (off_x + 128) << 16 | (off_y + 128) << 24 */
+    float weight;
+    float visibility_re, visibility_im;
+} chip_wproject_sample;
+/* accumulate_w_project_grid_metal_samples(projector, samples, model_grid,
+ * psf_grid, residual_grid, mode) (lib.rs:71863-71871, the non-mac stub
+ * :71825-71838): n packed, pre-planned samples (device array of
+ * chip_wproject_sample) gridded into psf_dev / residual_dev (complex64 x-major
+ * grids, accumulated: the Metal path's chunk grids start at zero) with the
+ * arithmetic of W_PROJECT_METAL_SHADER (lib.rs:72656-72724): mode PSF adds
+ * weight * cwt to psf_dev, DIRTY adds that and weight * (V * cwt) to
+ * residual_dev, RESIDUAL adds weight * ((V - p) * cwt) with p the degrid of
+ * model_dev (NULL: zero model).  CHIP_ERR_INVALID when a sample's footprint
+ * leaves the grid or the kernel (the other samples are gridded). */
+int chip_grid_wproject_samples(chip_gridder *g, uint64_t n, const void *samples_dev, const float *model_dev,
+                               float *psf_dev, float *residual_dev, uint32_t mode, char **err);
+
 /* ---- FP64 planning ------------------------------------------------------ */
 /* Per-sample plan, bit-exact with the reference CPU planner:
  *  standard (no W attached): plan_positive_taps (gridder.rs:557-566, :1958-1974)
