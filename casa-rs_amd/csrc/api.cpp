@@ -24,6 +24,7 @@ void vset_cycle_graph(chip_vset &vs, int enable);
 void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi);
 uint64_t vset_cycle_replays(const chip_vset &vs);
 void vset_cycle_passes(chip_vset &vs, int enable, double *pass_ms);
+void vset_item_order(chip_vset &vs, uint32_t order);
 void wk_all_kernels(const chip_gridder &g, float2 *kernels_dev);
 }  // namespace chip
 
@@ -566,6 +567,15 @@ int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order
         if (groups_per_item) *groups_per_item = vs->groups_per_item;
         if (n_timed) *n_timed = static_cast<uint32_t>(vs->tune_ms.size() / 5);
         if (timed) std::memcpy(timed, vs->tune_ms.data(), vs->tune_ms.size() * sizeof(double));
+    });
+}
+
+int chip_vset_item_order(chip_vset *vs, uint32_t order, uint32_t *sibling, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        set_device(vs->g->ctx);
+        if (order != UINT32_MAX) vset_item_order(*vs, order);
+        if (sibling) *sibling = vs->sibling;
     });
 }
 

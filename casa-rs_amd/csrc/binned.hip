@@ -102,6 +102,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                                                 float *acc_out, float2 *vis_out) {
     extern __shared__ float lds[];
     const WorkItem it = items[blockIdx.x];
+    if (it.count == 0) return;  // padding of the XCD-aligned item order
     const int L = static_cast<int>(gp.L);
     const int Ls = static_cast<int>(gp.Ls);
     const int cells = L * Ls;

@@ -129,6 +129,7 @@ __global__ void k_touch(const WorkItem *__restrict__ items, uint64_t n_items, To
     const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
     const WorkItem it = items[i];
+    if (it.count == 0) return;  // padding item
     const uint32_t tx = it.tile / tg.tiles_y, ty = it.tile - tx * tg.tiles_y;
     const int64_t L = tg.tile + 2 * tg.halo;
     const int64_t x0 = static_cast<int64_t>(tx) * tg.tile - tg.halo, y0 = static_cast<int64_t>(ty) * tg.tile - tg.halo;

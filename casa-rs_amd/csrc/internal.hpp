@@ -344,6 +344,14 @@ struct chip_vset {
     // fixed by chip_vset_schedule)
     uint32_t threads = 512, groups_per_item = 1024;
     uint32_t order = 0;  // groups of a bin by anchor cell (0) or by plan (1)
+    // Planes whose samples share their plans (rows x channels input with equal
+    // lambda scales in runs of `sibling`, e.g. the correlations of one
+    // channel): with item_order 1 their work items of one (tile, split) run
+    // on one XCD at the same time, so that the tap tables they read again
+    // come from that XCD's L2 (padding items with no groups keep the XCD
+    // round robin aligned).
+    uint32_t sibling = 1;
+    uint32_t item_order = 0;
     bool schedule_fixed = false, schedule_tuned = false;
     std::vector<double> tune_ms;  // timed candidates: (tile, order, threads, groups per item, ms)
     // stream (sorted by channel, tile)

@@ -421,7 +421,55 @@ void vset_split_items(chip_vset &vs, uint32_t gpi) {
     const uint64_t per_chunk = static_cast<uint64_t>(vs.n_tiles_all) * vs.n_chan;
     for (uint64_t c = 0; c < vs.chunk_items.size(); ++c) vs.chunk_items[c] = starts[c * per_chunk];
     CHIP_HIP(hipStreamSynchronize(st));
+    if (vs.item_order == 1 && vs.sibling > 1 && vs.chunk_items.size() == 2 && n_items) {
+        // Sets of sibling work items: the items of one (sibling group, tile,
+        // split) of the G planes of the group.  Set k goes to XCD k mod 8;
+        // each XCD's queue holds its sets' items one after the other, and
+        // the queues are interleaved so that dispatch position p (XCD p mod
+        // 8, workgroups are dealt to the XCDs round robin) takes the next
+        // item of queue p mod 8; shorter queues are padded with items of no
+        // groups.
+        constexpr uint32_t kXcd = 8;
+        std::vector<WorkItem> items(n_items);
+        CHIP_HIP(hipMemcpyAsync(items.data(), vs.items.p, n_items * sizeof(WorkItem), hipMemcpyDeviceToHost, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+        const uint32_t G = vs.sibling, n_tiles = vs.n_tiles_all;
+        std::vector<std::vector<WorkItem>> queue(kXcd);
+        uint64_t set = 0;
+        for (uint32_t grp = 0; grp * G < vs.n_chan; ++grp) {
+            for (uint32_t t = 0; t < n_tiles; ++t) {
+                uint32_t most = 0;
+                for (uint32_t m = 0; m < G; ++m) {
+                    const uint64_t b = static_cast<uint64_t>(grp * G + m) * n_tiles + t;
+                    most = std::max(most, starts[b + 1] - starts[b]);
+                }
+                for (uint32_t r = 0; r < most; ++r, ++set) {
+                    std::vector<WorkItem> &q = queue[set % kXcd];
+                    for (uint32_t m = 0; m < G; ++m) {
+                        const uint64_t b = static_cast<uint64_t>(grp * G + m) * n_tiles + t;
+                        if (r < starts[b + 1] - starts[b]) q.push_back(items[starts[b] + r]);
+                    }
+                }
+            }
+        }
+        size_t longest = 0;
+        for (const auto &q : queue) longest = std::max(longest, q.size());
+        std::vector<WorkItem> order(longest * kXcd, WorkItem{0u, 0u, 0u, 0u});
+        for (uint32_t x = 0; x < kXcd; ++x)
+            for (size_t k = 0; k < queue[x].size(); ++k) order[k * kXcd + x] = queue[x][k];
+        vs.n_items = order.size();
+        vs.items.alloc(vs.n_items * sizeof(WorkItem));
+        CHIP_HIP(hipMemcpyAsync(vs.items.p, order.data(), vs.n_items * sizeof(WorkItem), hipMemcpyHostToDevice, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+        vs.chunk_items[1] = static_cast<uint32_t>(vs.n_items);
+    }
     vs.items_version += 1;
+}
+
+void vset_item_order(chip_vset &vs, uint32_t order) {
+    if (order > 1) fail(CHIP_ERR_INVALID, "item order must be 0 or 1");
+    vs.item_order = order;
+    vset_split_items(vs, vs.groups_per_item);
 }
 
 namespace {
@@ -605,6 +653,23 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     const uint64_t n = src.n_rows * src.n_chan;
     vs.n_samples = n;
     vs.n_chan = src.n_chan;
+    vs.sibling = 1;
+    if (src.uvw_m && src.n_chan > 1) {
+        // runs of planes with bit-identical lambda scales share their plans
+        std::vector<double> ls(src.n_chan);
+        CHIP_HIP(hipMemcpyAsync(ls.data(), src.lambda_scale, ls.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+        uint32_t run = 1;
+        while (run < src.n_chan && ls[run] == ls[0]) ++run;
+        bool regular = run > 1 && src.n_chan % run == 0;
+        for (uint64_t c = 0; regular && c < src.n_chan; ++c) {
+            const uint64_t first = c - c % run;  // first plane of c's run
+            if (ls[c] != ls[first]) regular = false;
+            if (c == first && c > 0 && ls[c] == ls[c - 1]) regular = false;  // runs of exactly `run`
+        }
+        if (regular) vs.sibling = run;
+    }
+    vs.item_order = vs.sibling > 1 ? 1u : 0u;  // sibling planes together on one XCD
     vs.halo = g.wproject ? g.max_support : static_cast<uint32_t>(kSupport);
     vs.wk_chunk = (g.wproject && !g.aw) ? g.wk_chunk : 0u;
     // 32x32 tiles; smaller tiles when a large W support would not leave the

@@ -875,6 +875,14 @@ class VisibilitySet:
         abi.call("chip_vset_cycle_graph", self.handle, 1 if enable else 0, ctypes.byref(n))
         return n.value
 
+    def item_order(self, order=None):
+        """Work-item launch order (0 by plane and tile; 1 sibling planes of
+        equal lambda scale together on one XCD); None leaves it.  Returns the
+        sibling run length of the set (1: none)."""
+        sib = ctypes.c_uint32()
+        abi.call("chip_vset_item_order", self.handle, 0xFFFFFFFF if order is None else int(order), ctypes.byref(sib))
+        return sib.value
+
     def cycle_passes(self, enable: int = -1):
         """Per-pass device ms of the last timed eager fused major cycle:
         (forward rows, forward columns, zero fill, binned kernel, inverse

@@ -546,6 +546,18 @@ int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order
  * reference counterpart (the Metal path encodes one command buffer per
  * dispatch, lib.rs:72270-72290). */
 int chip_vset_cycle_graph(chip_vset *vs, int enable, uint64_t *replays, char **err);
+/* Launch order of the binned kernel's work items.  order 0: by (plane,
+ * tile); order 1: when the set's planes come in runs of equal lambda scale
+ * (the correlations of one channel of a rows x channels input share every
+ * plan and tap table), the items of one (run, tile, split) are placed so that
+ * they run together on one XCD (workgroups are dealt to the 8 XCDs round
+ * robin; empty padding items keep the alignment), and the tap tables one
+ * reads again come from that XCD's L2 (the default when such runs exist;
+ * mosaic7's binned kernel 16.5 -> 15.9 ms).  order UINT32_MAX leaves the order.
+ * sibling (may be NULL) receives the run length found when the set was
+ * created (1: none).  Results do not depend on the order beyond the order of
+ * float additions.  No reference counterpart. */
+int chip_vset_item_order(chip_vset *vs, uint32_t order, uint32_t *sibling, char **err);
 /* Per-pass timing of later eager major cycles with the fused strategy:
  * enable = 1 records events before, between and after the six passes (a few
  * microseconds per cycle), 0 stops, -1 leaves the setting unchanged.
