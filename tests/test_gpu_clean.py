@@ -6,8 +6,10 @@ residual and model planes, component counts and stop codes must equal the
 oracle's (run_hogbom_minor_cycle_cpu_with_progress, lib.rs:59748-59885) bit
 for bit: on the reference's own literal case, on cubes with masks, plateaus
 of equal |value|, PSFs smaller and larger than the image, per-plane cycle and
-nsigma thresholds, a NaN at the search seed, no cleanable pixel, and on
-1024^2 planes with a beam-like PSF.
+nsigma thresholds, a NaN at the search seed, no cleanable pixel, on 1024^2
+planes with a beam-like PSF, and on one large plane, which the GPU splits
+into many row slices (peak candidates reduced across slices, slices the PSF
+window misses kept from the previous component).
 """
 import numpy as np
 import pytest
@@ -114,6 +116,32 @@ def test_full_size_planes(gpu, oracle_mod):
     g, o = run_both(gpu, oracle_mod, res, np.broadcast_to(psf, (planes, n, n)), 0.1, 59, cyc=0.0, nsig=0.0)
     assert_same(g, o)
     assert all(int(u) == 60 for u in g[2])
+
+
+@pytest.mark.parametrize("nan_seed", [False, True])
+def test_single_large_plane(gpu, oracle_mod, nan_seed):
+    """One 2048 x 1536 plane (1024 row slices of two rows), a 65 x 49 PSF patch,
+    a mask whose first unmasked pixel lies beyond 10^4 pixels, and plateaus
+    of equal |value| spread over many slices."""
+    rng = np.random.default_rng(21 + nan_seed)
+    nx, ny = 2048, 1536
+    res = np.round(rng.normal(size=(1, nx, ny)) * 4.0).astype(np.float32) / 4.0
+    mask = rng.uniform(size=(1, nx, ny)) < 0.8
+    mask.reshape(-1)[:10007] = False
+    seed = int(np.flatnonzero(mask.reshape(-1))[0])
+    if nan_seed:
+        res.reshape(-1)[seed] = np.nan
+    psf = rng.uniform(-0.1, 0.1, size=(65, 49)).astype(np.float32)
+    psf[32, 24] = 1.0
+    g, o = run_both(gpu, oracle_mod, res, psf[None], 0.1, 199, threshold=0.01, mask=mask)
+    r, m, upd, _, stop = g
+    o_r, o_m, o_upd, o_stop = o
+    assert np.array_equal(upd.astype(np.uint64), o_upd) and np.array_equal(stop, o_stop)
+    for a, b in ((r, o_r), (m, o_m)):
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+        assert np.array_equal(np.nan_to_num(a).view(np.uint32), np.nan_to_num(b).view(np.uint32))
+    if not nan_seed:
+        assert int(upd[0]) == 200
 
 
 def test_rejects_bad_arguments(gpu):
