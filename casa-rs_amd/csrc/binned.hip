@@ -89,6 +89,18 @@ size_t binned_tile_lds(const chip_gridder &g, uint32_t tile, uint32_t halo) {
 constexpr int kBinnedThreads = 512;
 constexpr int kBig = 8;    // taps per lane per chunk of the large-footprint path (512 per chunk)
 
+// z / conj(n) as z * conj(conj(n)) / |n|^2 with one hardware reciprocal of
+// |n|^2 (about 1 ulp) in place of two divisions: the binned kernels are
+// bound by instruction issue and their sums are not bit-reproducible anyway
+// (atomic order); the plain gridders keep the reference's division.
+__device__ __forceinline__ void div_conj_rcp(float &re, float &im, float nr, float ni) {
+    const float c = nr, d = -ni;
+    const float inv = __builtin_amdgcn_rcpf(c * c + d * d);
+    const float a = re, b = im;
+    re = (a * c + b * d) * inv;
+    im = (b * c - a * d) * inv;
+}
+
 template <int MODE, bool WP, int KS>
 __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
                                                 const uint4 *__restrict__ grec,
@@ -178,6 +190,10 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // is fixed, so per group the y offset is computed once and the x offsets
     // of its slots follow by steps of P.
     constexpr int NS = KS > 0 ? KS : 1;
+    // Many slots (large supports): incremental slot offsets and fused
+    // multiply-add chains issue fewer instructions; with one or two slots the
+    // plain forms measured faster.
+    constexpr bool kChains = NS > 2;
     const int Mside = 2 * static_cast<int>(gp.halo) + 1;
     const float invM = 1.0f / static_cast<float>(Mside);
     const int Pl = 64 / Mside;
@@ -283,17 +299,36 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         int i = lane_pg + qx * Mside - f.bx;
         const bool in_y = jj < f.side;
         const int cy = f.by + jj;
+        // the slot's LDS cell and tap byte offset, advanced with i (no
+        // per-slot multiplies); taps load from the group's table base plus a
+        // 32-bit offset
+        int cell = (f.bx + i) * Ls + cy;
+        uint32_t toff = static_cast<uint32_t>(i * f.side + jj) * 8u;
+        const int step_c = Pl * Ls, wrap_c = Mside * Ls;
+        const uint32_t step_t = static_cast<uint32_t>(Pl * f.side) * 8u, wrap_t = static_cast<uint32_t>(Mside * f.side) * 8u;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             d.cell[k] = -1;
             d.tap[k] = make_float2(0.0f, 0.0f);
             if (k > 0) {
                 i += Pl;
-                if (i >= Mside) i -= Mside;
+                cell += step_c;
+                toff += step_t;
+                if (i >= Mside) {
+                    i -= Mside;
+                    cell -= wrap_c;
+                    toff -= wrap_t;
+                }
             }
             if (!((slot_mask >> k) & 1u) || !in_y || i >= f.side) continue;
-            d.cell[k] = (f.bx + i) * Ls + cy;
-            d.tap[k] = WP ? f.ktap[i * f.side + jj] : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
+            if (kChains) {
+                d.cell[k] = cell;
+                d.tap[k] = WP ? *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(f.ktap) + toff)
+                              : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
+            } else {  // (few slots: the multiplies cost less than the carried offsets)
+                d.cell[k] = (f.bx + i) * Ls + cy;
+                d.tap[k] = WP ? f.ktap[i * f.side + jj] : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
+            }
         }
     };
 
@@ -399,7 +434,10 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                     }
                 }
                 wave_sum_f2(pr, pi);
-                if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
+                if (WP && gp.aw_norm) {
+                if (kChains) div_conj_rcp(pr, pi, awn.x, awn.y);
+                else div_conj<float>(pr, pi, awn.x, awn.y);
+            }
                 if (MODE == kModeDegrid) {
                     write_degrid(pr, pi);
                     return;
@@ -449,14 +487,25 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             cr[k] = d.tap[k].x;
             ci[k] = d.tap[k].y * csign;
             if (kModel) {
-                // conj(cwt) * model
-                pr += cr[k] * mre[k] + ci[k] * mim[k];
-                pi += cr[k] * mim[k] - ci[k] * mre[k];
+                // conj(cwt) * model (many slots: fused multiply-add chains,
+                // fewer instructions; few slots: shorter dependency chains)
+                if (kChains) {
+                    pr = fmaf(cr[k], mre[k], pr);
+                    pr = fmaf(ci[k], mim[k], pr);
+                    pi = fmaf(cr[k], mim[k], pi);
+                    pi = fmaf(-ci[k], mre[k], pi);
+                } else {
+                    pr += cr[k] * mre[k] + ci[k] * mim[k];
+                    pi += cr[k] * mim[k] - ci[k] * mre[k];
+                }
             }
         }
         if (kModel) {
             wave_sum_f2(pr, pi);
-            if (WP && gp.aw_norm) div_conj<float>(pr, pi, awn.x, awn.y);
+            if (WP && gp.aw_norm) {
+                if (kChains) div_conj_rcp(pr, pi, awn.x, awn.y);
+                else div_conj<float>(pr, pi, awn.x, awn.y);
+            }
             if (MODE == kModeDegrid) {
                 write_degrid(pr, pi);
                 return;
@@ -468,10 +517,15 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
             if (kPsf) {
-                pacr[k] += w * cr[k];
-                paci[k] += w * ci[k];
+                pacr[k] = fmaf(w, cr[k], pacr[k]);
+                paci[k] = fmaf(w, ci[k], paci[k]);
             }
-            if (kAcc) {
+            if (kAcc && kChains) {  // value * tap
+                accr[k] = fmaf(vr, cr[k], accr[k]);
+                accr[k] = fmaf(-vi, ci[k], accr[k]);
+                acci[k] = fmaf(vr, ci[k], acci[k]);
+                acci[k] = fmaf(vi, cr[k], acci[k]);
+            } else if (kAcc) {
                 accr[k] += vr * cr[k] - vi * ci[k];
                 acci[k] += vr * ci[k] + vi * cr[k];
             }
