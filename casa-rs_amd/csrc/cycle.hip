@@ -559,7 +559,7 @@ void vset_cycle_info(const chip_vset &vs, int *strategy, uint32_t shape[3], doub
     *strategy = vs.cycle->strategy;
     *transform_ms = vs.cycle->transform_ms;
     if (vs.cycle->strategy == CHIP_CYCLE_FUSED) {
-        shape[0] = xf::kBlock;
+        shape[0] = 1u << vs.cycle->xp.logb_x;
         shape[1] = xf::kThreads;
         shape[2] = xf::kRowsPerGroup;
     }

@@ -44,6 +44,7 @@
 // LDS (element i of transform c at 8 i + c) to match the paired grid layout;
 // row tiles are row-major (c n + i).
 #include <cmath>
+#include <type_traits>
 
 #include "device.hpp"
 #include "xform.hpp"
@@ -170,7 +171,7 @@ template <bool INV> __device__ __forceinline__ float2 conj_if(float2 w) { return
 // tw: this stage's twiddle table (read only when ns > 1); nb = n / R comes
 // from the stage table, so that addresses are not hoisted out of the stage
 // loop for every radix at once.
-template <int R1, int R2, bool INV, int MB, bool ILV, int LANES>
+template <int R1, int R2, bool INV, int MB, bool ILV, int LANES, int LOGB>
 __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw, uint32_t nb, uint32_t pitch,
                                       uint32_t ns, float inv_ns, uint32_t c, uint32_t jt) {
     constexpr int R = R1 * R2, M = (R1 - 1) + (R2 - 1);
@@ -199,7 +200,7 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw
 #pragma unroll
             for (int t = 0; t < R; ++t) {
                 const uint32_t i = j + t * nb;
-                v[b][t] = buf[ILV ? (i << kLogBlock) + c : c * pitch + i];
+                v[b][t] = buf[ILV ? (i << LOGB) + c : c * pitch + i];
             }
         }
     }
@@ -250,15 +251,16 @@ __device__ __forceinline__ void stage(float2 *buf, const float2 *__restrict__ tw
 #pragma unroll
             for (int s1 = 0; s1 < R1; ++s1) {
                 const uint32_t i = base + (s2 + R2 * s1) * ns;
-                buf[ILV ? (i << kLogBlock) + c : c * pitch + i] = a[s1];
+                buf[ILV ? (i << LOGB) + c : c * pitch + i] = a[s1];
             }
         }
     }
     __syncthreads();
 }
 
-// E: elements per thread per transform (n <= LANES E).
-template <bool INV, int E, bool ILV, int LANES>
+// E: elements per thread per transform (n <= LANES E); interleaved tiles
+// hold 2^LOGB transforms (element i of transform c at (i << LOGB) + c).
+template <bool INV, int E, bool ILV, int LANES, int LOGB = kLogBlock>
 __device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t pitch, uint32_t c, uint32_t jt) {
     for (uint32_t s = 0; s < ax.n_stages; ++s) {
         const uint32_t ns = ax.ns[s];
@@ -267,7 +269,7 @@ __device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t pi
         switch (ax.r1[s] * 8 + ax.r2[s]) {
 #define CHIP_XF_STAGE(A, B)                                                                          \
     case A * 8 + B:                                                                                  \
-        stage<A, B, INV, (E + A * B - 1) / (A * B), ILV, LANES>(buf, tw, ax.nb[s], pitch, ns, inv, c, jt); \
+        stage<A, B, INV, (E + A * B - 1) / (A * B), ILV, LANES, LOGB>(buf, tw, ax.nb[s], pitch, ns, inv, c, jt); \
         break;
             CHIP_XF_STAGE(5, 5)
             CHIP_XF_STAGE(5, 3)
@@ -289,63 +291,78 @@ __device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t pi
 }
 
 // ---------------------------------------------------------------- kernels
-// Rows passes: 8 rows per workgroup of kRowThreads, packed as 4 complex
+// Rows passes: 8 rows per workgroup of 4 LANES threads, packed as 4 complex
 // transforms of two real rows each (row 2p real part, row 2p + 1 imaginary
 // part), row-major in LDS; grid (nxp / 8, n_chan).  Column passes: one block
-// of 8 columns per workgroup of kThreads, interleaved in LDS; grid (nyb_h,
+// of 8 columns (4 for axes above 2048 points) per workgroup of 512 threads,
+// interleaved in LDS; grid (nyb_h x columns-per-block split,
 // n_chan).  LDS holds only the data (two column workgroups per CU up to
 // length 1280); waves_per_eu states the occupancy the LDS allows, so the
 // compiler may give a thread up to 128 VGPRs.
 
-// Coverage mask of one upper block as two 64-bit words (row chunk ch at bit
-// ch of lo for ch < 64, else of hi), gathered by a wavefront ballot.
+// Coverage mask of one upper block as kRowWords 64-bit words (row chunk ch
+// at bit ch % 64 of word ch / 64), gathered by wavefront ballots.
 struct RowMask {
-    uint64_t lo, hi;
+    uint64_t w[kRowWords];
 };
 __device__ __forceinline__ RowMask row_mask(const XGeom &xg, uint32_t c, uint32_t yb) {
-    if (!xg.touch) return {~0ull, ~0ull};
+    RowMask m;
+    if (!xg.touch) {
+#pragma unroll
+        for (uint32_t q = 0; q < kRowWords; ++q) m.w[q] = ~0ull;
+        return m;
+    }
     const uint8_t *t = xg.touch + (static_cast<uint64_t>(c) * xg.nyb_h + yb) * xg.nxc;
     const uint32_t lane = threadIdx.x & 63u;
-    const uint8_t a = t[lane < xg.nxc ? lane : 0], b = t[lane + 64 < xg.nxc ? lane + 64 : 0];
-    return {__ballot(lane < xg.nxc && a != 0), __ballot(lane + 64 < xg.nxc && b != 0)};
+    uint8_t f[kRowWords];
+#pragma unroll
+    for (uint32_t q = 0; q < kRowWords; ++q) f[q] = t[lane + 64 * q < xg.nxc ? lane + 64 * q : 0];
+#pragma unroll
+    for (uint32_t q = 0; q < kRowWords; ++q) m.w[q] = __ballot(lane + 64 * q < xg.nxc && f[q] != 0);
+    return m;
 }
 __device__ __forceinline__ bool row_touched(const RowMask &m, uint32_t x) {
     const uint32_t ch = x / kTouchRows;
-    return ((ch < 64 ? m.lo >> ch : m.hi >> (ch - 64)) & 1ull) != 0;
+    return ch < 64 * kRowWords && ((m.w[ch >> 6] >> (ch & 63u)) & 1ull) != 0;
 }
 
-// Block flags of one channel (touch_blk) as three 64-bit words (nyb_h <= 129),
-// gathered by a wavefront ballot; every block without a mask.
+// Block flags of one channel (touch_blk) as kBlockWords 64-bit words,
+// gathered by wavefront ballots; every block without a mask.
 struct BlockMask {
-    uint64_t w[3];
+    uint64_t w[kBlockWords];
 };
 __device__ __forceinline__ BlockMask block_mask(const XGeom &xg, uint32_t c) {
-    if (!xg.touch_blk) return {{~0ull, ~0ull, ~0ull}};
+    BlockMask m;
+    if (!xg.touch_blk) {
+#pragma unroll
+        for (uint32_t q = 0; q < kBlockWords; ++q) m.w[q] = ~0ull;
+        return m;
+    }
     const uint8_t *t = xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h;
     const uint32_t lane = threadIdx.x & 63u;
-    uint8_t f[3];
+    uint8_t f[kBlockWords];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) f[q] = t[lane + 64 * q < xg.nyb_h ? lane + 64 * q : 0];
-    BlockMask m;
+    for (uint32_t q = 0; q < kBlockWords; ++q) f[q] = t[lane + 64 * q < xg.nyb_h ? lane + 64 * q : 0];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) m.w[q] = __ballot(lane + 64 * q < xg.nyb_h && f[q] != 0);
+    for (uint32_t q = 0; q < kBlockWords; ++q) m.w[q] = __ballot(lane + 64 * q < xg.nyb_h && f[q] != 0);
     return m;
 }
 __device__ __forceinline__ bool block_live(const BlockMask &m, uint32_t yb) {
     const uint32_t q = yb >> 6;
-    return q < 3 && ((m.w[q] >> (yb & 63u)) & 1ull) != 0;
+    return q < kBlockWords && ((m.w[q] >> (yb & 63u)) & 1ull) != 0;
 }
 
 __device__ __forceinline__ uint64_t half_at(const XGeom &xg, uint32_t yb, uint32_t x, uint32_t lane) {
     return static_cast<uint64_t>(yb) * xg.nxp * kBlock + x * kBlock + lane;
 }
 
-template <int E>
-__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_fwd_rows(
+template <int E, int LANES>
+__global__ __launch_bounds__(kPairs * LANES) __attribute__((amdgpu_waves_per_eu(E <= 20 && LANES == 64 ? 4 : 2))) void k_fwd_rows(
     XGeom xg, Axis ay, const float *__restrict__ model, float2 *__restrict__ half) {
     extern __shared__ float2 buf[];
+    constexpr uint32_t NT = kPairs * LANES;
     const uint32_t n = xg.gy;
-    const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
+    const uint32_t tid = threadIdx.x, p = tid / LANES, jt = tid % LANES;
     const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
     // z = a + i b for rows a = x0 + 2p, b = a + 1 (apodised, signed, zero-padded)
     float f[2], sxv[2];
@@ -368,7 +385,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
         float mv[CH][2], cyv[CH], syv[CH];
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            const uint32_t i = jt + static_cast<uint32_t>(t0 + u) * kLanes;
+            const uint32_t i = jt + static_cast<uint32_t>(t0 + u) * LANES;
             const bool in = t0 + u < E && i < n && i >= xg.blc_y && i < xg.blc_y + xg.ny;
             cyv[u] = in ? xg.corr_y[i] : 0.0f;
             syv[u] = in && xg.wproject ? xg.sinc_y[i] : 1.0f;
@@ -376,7 +393,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
         }
 #pragma unroll
         for (int u = 0; u < CH; ++u) {
-            const uint32_t i = jt + static_cast<uint32_t>(t0 + u) * kLanes;
+            const uint32_t i = jt + static_cast<uint32_t>(t0 + u) * LANES;
             if (t0 + u >= E || i >= n) break;
             float v[2] = {0.0f, 0.0f};
             if (i >= xg.blc_y && i < xg.blc_y + xg.ny) {
@@ -394,7 +411,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
         }
     }
     __syncthreads();
-    run_fft<false, E, false, kLanes>(buf, ay, n, p, jt);
+    run_fft<false, E, false, LANES>(buf, ay, n, p, jt);
     // A[k] = (Z[k] + conj Z[-k]) / 2, B[k] = (Z[k] - conj Z[-k]) / 2i, k <= gy/2,
     // into the half spectrum [yb][x][8]: 8 rows x 8 lanes contiguous per block
     // (blocks of columns the uv coverage misses are not stored: the column
@@ -402,7 +419,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     float2 *h = half + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
     const BlockMask bm = block_mask(xg, c);
-    for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += kRowThreads) {
+    for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += NT) {
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
         if (!block_live(bm, yb)) continue;
         const uint32_t k = yb * kBlock + lane;
@@ -418,29 +435,34 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     }
 }
 
-template <int E>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_fwd_cols(
+// Column passes: a workgroup holds BLK = 2^LOGB columns of an 8-column block
+// (all 8, or half of them for long axes: workgroup blockIdx.x is block
+// blockIdx.x / SUB, columns col0 .. col0 + BLK - 1 of it), interleaved in LDS.
+template <int E, int LANES, int LOGB>
+__global__ __launch_bounds__(LANES << LOGB) __attribute__((amdgpu_waves_per_eu(E <= 20 && LANES == 64 ? 4 : 2))) void k_fwd_cols(
     XGeom xg, Axis ax, const float2 *__restrict__ half, float2 *__restrict__ grid) {
     extern __shared__ float2 buf[];
+    constexpr uint32_t BLK = 1u << LOGB, NT = LANES << LOGB, SUB = kBlock / BLK;
     const uint32_t n = xg.gx;
-    const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
-    const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    const uint32_t tid = threadIdx.x, cc = tid % BLK, jt = tid / BLK;
+    const uint32_t yb = blockIdx.x / SUB, col0 = (blockIdx.x % SUB) * BLK, c = blockIdx.y;
     if (xg.touch_blk && !xg.touch_blk[static_cast<uint64_t>(c) * xg.nyb_h + yb]) return;  // outside the uv coverage
     const float2 *h = half + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
     float2 hv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads, i = e >> 3;
-        hv[t] = (e < n * kBlock && i >= xg.blc_x && i < xg.blc_x + xg.nx) ? h[(i - xg.blc_x) * kBlock + (e & 7)]
-                                                                         : make_float2(0.0f, 0.0f);
+        const uint32_t e = tid + static_cast<uint32_t>(t) * NT, i = e >> LOGB;
+        hv[t] = (e < n * BLK && i >= xg.blc_x && i < xg.blc_x + xg.nx)
+                    ? h[(i - xg.blc_x) * kBlock + col0 + (e & (BLK - 1))]
+                    : make_float2(0.0f, 0.0f);
     }
 #pragma unroll
     for (int t = 0; t < E; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
-        if (e < n * kBlock) buf[e] = hv[t];
+        const uint32_t e = tid + static_cast<uint32_t>(t) * NT;
+        if (e < n * BLK) buf[e] = hv[t];
     }
     __syncthreads();
-    run_fft<false, E, true, kLanes>(buf, ax, 0, cc, jt);
+    run_fft<false, E, true, LANES, LOGB>(buf, ax, 0, cc, jt);
     // column ky into upper block yb.  The columns ky > gy/2 are not stored:
     // the transform of a real image is Hermitian, so the binned kernel reads
     // G[kx, ky] as conj G[-kx, gy - ky] from the upper blocks.  Rows outside
@@ -449,19 +471,21 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 2
     float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
     const uint32_t hy = xg.gy / 2;
     const RowMask rm = row_mask(xg, c, yb);
-    for (uint32_t e = tid; e < n * kBlock; e += kThreads) {
-        const uint32_t ky = yb * kBlock + (e & 7);
-        if (ky <= hy && row_touched(rm, e >> kLogBlock)) up[e] = buf[e];
+    for (uint32_t e = tid; e < n * BLK; e += NT) {
+        const uint32_t x = e >> LOGB, lane = col0 + (e & (BLK - 1));
+        const uint32_t ky = yb * kBlock + lane;
+        if (ky <= hy && row_touched(rm, x)) up[x * kBlock + lane] = buf[e];
     }
 }
 
-template <int E>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_inv_cols(
+template <int E, int LANES, int LOGB>
+__global__ __launch_bounds__(LANES << LOGB) __attribute__((amdgpu_waves_per_eu(E <= 20 && LANES == 64 ? 4 : 2))) void k_inv_cols(
     XGeom xg, Axis ax, const float2 *__restrict__ grid, float2 *__restrict__ rows) {
     extern __shared__ float2 buf[];
+    constexpr uint32_t BLK = 1u << LOGB, NT = LANES << LOGB, SUB = kBlock / BLK;
     const uint32_t n = xg.gx;
-    const uint32_t tid = threadIdx.x, cc = tid % kBlock, jt = tid / kBlock;
-    const uint32_t yb = blockIdx.x, c = blockIdx.y;
+    const uint32_t tid = threadIdx.x, cc = tid % BLK, jt = tid / BLK;
+    const uint32_t yb = blockIdx.x / SUB, col0 = (blockIdx.x % SUB) * BLK, c = blockIdx.y;
     if (xg.touch_blk && !xg.touch_blk[static_cast<uint64_t>(c) * xg.nyb_h + yb]) return;  // outside the uv coverage
     // Hermitian part H[kx, ky] = (G[kx, ky] + conj G[-kx, -ky]) / 2 of the
     // columns ky <= gy/2, accumulated as 2H by the binned kernel; its inverse
@@ -480,41 +504,47 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(E <= 2
     float2 gv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
-        const uint32_t ky = yb * kBlock + (e & 7);
-        const bool live = e < n * kBlock && ky <= hy && row_touched(rm, e >> kLogBlock);
+        const uint32_t e = tid + static_cast<uint32_t>(t) * NT;
+        const uint32_t x = e >> LOGB, lane = col0 + (e & (BLK - 1));
+        const uint32_t ky = yb * kBlock + lane;
+        const bool live = e < n * BLK && ky <= hy && row_touched(rm, x);
         gv[t] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
-                                               rs, live ? e * static_cast<uint32_t>(sizeof(float2)) : kBufferOut, 0, 0));
+                                               rs, live ? (x * kBlock + lane) * static_cast<uint32_t>(sizeof(float2))
+                                                        : kBufferOut,
+                                               0, 0));
     }
 #pragma unroll
     for (int t = 0; t < E; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kThreads;
-        if (e < n * kBlock) buf[e] = make_float2(0.5f * gv[t].x, 0.5f * gv[t].y);
+        const uint32_t e = tid + static_cast<uint32_t>(t) * NT;
+        if (e < n * BLK) buf[e] = make_float2(0.5f * gv[t].x, 0.5f * gv[t].y);
     }
     __syncthreads();
-    run_fft<true, E, true, kLanes>(buf, ax, 0, cc, jt);
+    run_fft<true, E, true, LANES, LOGB>(buf, ax, 0, cc, jt);
     float2 *o = rows + static_cast<uint64_t>(c) * xg.hstride + static_cast<uint64_t>(yb) * xg.nxp * kBlock;
-    for (uint32_t e = tid; e < xg.nx * kBlock; e += kThreads) {
-        uint32_t i = xg.blc_x + (e >> 3) + n / 2;
+    for (uint32_t e = tid; e < xg.nx * BLK; e += NT) {
+        const uint32_t x = e >> LOGB, lane = e & (BLK - 1);
+        uint32_t i = xg.blc_x + x + n / 2;
         if (i >= n) i -= n;
-        o[e] = buf[(i << kLogBlock) + (e & 7)];
+        o[x * kBlock + col0 + lane] = buf[(i << LOGB) + lane];
     }
 }
 
-template <int E>
-__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <= 20 ? 4 : 2))) void k_inv_rows(
+template <int E, int LANES>
+__global__ __launch_bounds__(kPairs * LANES) __attribute__((amdgpu_waves_per_eu(E <= 20 && LANES == 64 ? 4 : 2))) void k_inv_rows(
     XGeom xg, Axis ay, const float2 *__restrict__ rows, const float *__restrict__ nsum,
     const float *__restrict__ peak, float *__restrict__ out) {
     extern __shared__ float2 buf[];
+    constexpr uint32_t NT = kPairs * LANES;
     const uint32_t n = xg.gy;
-    const uint32_t tid = threadIdx.x, p = tid / kLanes, jt = tid % kLanes;
+    const uint32_t tid = threadIdx.x, p = tid / LANES, jt = tid % LANES;
     const uint32_t x0 = blockIdx.x * kRowsPerGroup, c = blockIdx.y;
     // Rows a = x0 + 2p and b = a + 1 are Hermitian in ky: Z[k] = A[k] + i B[k]
     // over the full length (A[-k] = conj A[k]), whose inverse is a + i b.
     const float2 *h = rows + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
     const uint32_t rows_live = xg.nx - x0;  // rows of this group inside the image (>= 1)
-    // elements per thread: nyb_h * 32 / 256 <= (n / 16 + 1) / 8 + 1 <= E / 2 + 1
+    // elements per thread: ceil(nyb_h * 32 / NT) <= E / 2 + 1 (checked when
+    // the transforms are planned)
     constexpr int EL = E / 2 + 1;
     const uint32_t total = xg.nyb_h * (kPairs * kBlock);
     // Buffer loads (see k_inv_cols); the blocks of columns the uv coverage
@@ -525,7 +555,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     float2 av[EL], bv[EL];
 #pragma unroll
     for (int t = 0; t < EL; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kRowThreads;
+        const uint32_t e = tid + static_cast<uint32_t>(t) * NT;
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
         const uint32_t k = yb * kBlock + lane, ra = 2 * pp;
         const bool live = e < total && k <= hy && block_live(bm, yb);
@@ -538,7 +568,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     }
 #pragma unroll
     for (int t = 0; t < EL; ++t) {
-        const uint32_t e = tid + static_cast<uint32_t>(t) * kRowThreads;
+        const uint32_t e = tid + static_cast<uint32_t>(t) * NT;
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
         const uint32_t k = yb * kBlock + lane;
         if (e >= total || k > hy) continue;
@@ -547,7 +577,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
         if (k > 0 && k < hy) buf[pp * n + n - k] = make_float2(A.x + B.y, B.x - A.y);
     }
     __syncthreads();
-    run_fft<true, E, false, kLanes>(buf, ay, n, p, jt);
+    run_fft<true, E, false, LANES>(buf, ay, n, p, jt);
     // Output v = Re * scale * sign * (cx cy) / ns / pk (gridder.rs:1646-1657,
     // lib.rs:68928-68929), with W projection v = 2 Re * scale * sign * (cx cy)
     // / (sx sy) / ns / pk where |sx sy| > 1e-6, else 0 (gridder.rs:1685-1715,
@@ -556,11 +586,11 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
     // costs a few products; this moves the result by a few ulp.
     const float scale = 1.0f / static_cast<float>(static_cast<uint64_t>(xg.gx) * xg.gy);
     const float chan_gain = (xg.wproject ? 2.0f : 1.0f) / nsum[c] / peak[c];
-    // y factors of this thread's columns, loaded together (ny <= n <= E kLanes)
+    // y factors of this thread's columns, loaded together (ny <= n <= E LANES)
     float ryv[E], syv[E];
 #pragma unroll
     for (int t = 0; t < E; ++t) {
-        const uint32_t y = jt + static_cast<uint32_t>(t) * kLanes, gyy = xg.blc_y + y;
+        const uint32_t y = jt + static_cast<uint32_t>(t) * LANES, gyy = xg.blc_y + y;
         const float cy = y < xg.ny ? xg.corr_y[gyy] : 0.0f;
         syv[t] = y < xg.ny && xg.wproject ? xg.sinc[gyy] : 1.0f;
         ryv[t] = xg.wproject ? cy / syv[t] : cy;
@@ -575,7 +605,7 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(E <
         float *o = out + (static_cast<uint64_t>(c) * xg.nx + x) * xg.ny;
 #pragma unroll
         for (int t = 0; t < E; ++t) {
-            const uint32_t y = jt + static_cast<uint32_t>(t) * kLanes;
+            const uint32_t y = jt + static_cast<uint32_t>(t) * LANES;
             if (y >= xg.ny) break;
             const uint32_t gyy = xg.blc_y + y;
             uint32_t i = gyy + n / 2;
@@ -647,16 +677,23 @@ bool factor_axis(uint32_t n, Axis &ax, std::vector<float2> &table) {
 
 namespace {
 
-int e_bucket(uint32_t n) {
-    const uint32_t per = (n + kLanes - 1) / kLanes;
-    for (int e : {4, 8, 12, 16, 20, 24, 32})
-        if (per <= static_cast<uint32_t>(e)) return e;
+// Per-thread element budget of an axis of n points with `lanes` threads per
+// transform (the instantiated budgets; 0: none fits).
+int e_bucket(uint32_t n, uint32_t lanes) {
+    const uint32_t per = (n + lanes - 1) / lanes;
+    if (lanes == kLanes) {
+        for (int e : {4, 8, 12, 16, 20, 24, 32})
+            if (per <= static_cast<uint32_t>(e)) return e;
+    } else {
+        for (int e : {20, 24, 32})
+            if (per <= static_cast<uint32_t>(e)) return e;
+    }
     return 0;
 }
 
-// rows tile: 4 transforms; column tile: 8 interleaved transforms
+// rows tile: 4 transforms; column tile: 2^logb interleaved transforms
 size_t row_lds(uint32_t n) { return static_cast<size_t>(kPairs) * n * sizeof(float2); }
-size_t col_lds(uint32_t n) { return static_cast<size_t>(kBlock) * n * sizeof(float2); }
+size_t col_lds(uint32_t n, uint32_t logb) { return (static_cast<size_t>(1) << logb) * n * sizeof(float2); }
 
 template <class K> void allow_lds(K kernel, size_t bytes) {
     if (bytes > 64 * 1024)
@@ -664,25 +701,37 @@ template <class K> void allow_lds(K kernel, size_t bytes) {
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(bytes)));
 }
 
-#define CHIP_XF_CASE(KERNEL, EV, GDIM, NT, LDS, ST, ...)                                          \
-    case EV: {                                                                                     \
-        allow_lds(KERNEL<EV>, LDS);                                                                \
-        hipLaunchKernelGGL((KERNEL<EV>), GDIM, dim3(NT), LDS, ST, __VA_ARGS__);                     \
-        break;                                                                                     \
-    }
+template <int V> using IC = std::integral_constant<int, V>;
 
-#define CHIP_XF_LAUNCH(E, KERNEL, GDIM, NT, LDS, ST, ...)                                         \
+// f(E, LANES) with the axis's element budget and lanes per transform as
+// compile-time constants: the instantiated combinations only.
+template <class F> void dispatch_axis(int e, uint32_t lanes, F f) {
+    if (lanes == kLanes) {
+        switch (e) {
+            case 4: f(IC<4>{}, IC<kLanes>{}); return;
+            case 8: f(IC<8>{}, IC<kLanes>{}); return;
+            case 12: f(IC<12>{}, IC<kLanes>{}); return;
+            case 16: f(IC<16>{}, IC<kLanes>{}); return;
+            case 20: f(IC<20>{}, IC<kLanes>{}); return;
+            case 24: f(IC<24>{}, IC<kLanes>{}); return;
+            case 32: f(IC<32>{}, IC<kLanes>{}); return;
+            default: break;
+        }
+    } else if (lanes == kLongLanes) {
+        switch (e) {
+            case 20: f(IC<20>{}, IC<kLongLanes>{}); return;
+            case 24: f(IC<24>{}, IC<kLongLanes>{}); return;
+            case 32: f(IC<32>{}, IC<kLongLanes>{}); return;
+            default: break;
+        }
+    }
+    fail(CHIP_ERR_INVALID, "transform too long for one workgroup");
+}
+
+#define CHIP_XF_GO(KEXPR, GDIM, NT, LDS, ST, ...)                                                 \
     do {                                                                                           \
-        switch (E) {                                                                               \
-            CHIP_XF_CASE(KERNEL, 4, GDIM, NT, LDS, ST, __VA_ARGS__)                                \
-            CHIP_XF_CASE(KERNEL, 8, GDIM, NT, LDS, ST, __VA_ARGS__)                                \
-            CHIP_XF_CASE(KERNEL, 12, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
-            CHIP_XF_CASE(KERNEL, 16, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
-            CHIP_XF_CASE(KERNEL, 20, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
-            CHIP_XF_CASE(KERNEL, 24, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
-            CHIP_XF_CASE(KERNEL, 32, GDIM, NT, LDS, ST, __VA_ARGS__)                               \
-            default: fail(CHIP_ERR_INVALID, "transform too long for one workgroup");              \
-        }                                                                                          \
+        allow_lds(KEXPR, LDS);                                                                     \
+        hipLaunchKernelGGL(KEXPR, GDIM, dim3(NT), LDS, ST, __VA_ARGS__);                           \
         CHIP_HIP(hipGetLastError());                                                               \
     } while (0)
 
@@ -694,9 +743,14 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     std::vector<float2> tx, ty;
     if (!factor_axis(static_cast<uint32_t>(g.gx), p.ax, tx) || !factor_axis(static_cast<uint32_t>(g.gy), p.ay, ty))
         return false;
-    p.e_x = e_bucket(p.ax.n);
-    p.e_y = e_bucket(p.ay.n);
-    if (!p.e_x || !p.e_y || col_lds(p.ax.n) > lds_limit || row_lds(p.ay.n) > lds_limit) return false;
+    // axes up to 2048 points: 64 threads per transform, 8 columns per column
+    // workgroup; up to 4096: 128 threads, 4 columns
+    p.lanes_x = p.ax.n > kLanes * 32 ? kLongLanes : kLanes;
+    p.lanes_y = p.ay.n > kLanes * 32 ? kLongLanes : kLanes;
+    p.logb_x = p.lanes_x == kLanes ? kLogBlock : kLongLogBlock;
+    p.e_x = e_bucket(p.ax.n, p.lanes_x);
+    p.e_y = e_bucket(p.ay.n, p.lanes_y);
+    if (!p.e_x || !p.e_y || col_lds(p.ax.n, p.logb_x) > lds_limit || row_lds(p.ay.n) > lds_limit) return false;
     XGeom &xg = p.xg;
     xg.nx = static_cast<uint32_t>(g.geom.nx);
     xg.ny = static_cast<uint32_t>(g.geom.ny);
@@ -720,7 +774,12 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
     xg.touch = nullptr;
     xg.touch_blk = nullptr;
     xg.nxc = (xg.gx + kTouchRows - 1) / kTouchRows;
-    if (xg.nxc > 128 || xg.nyb_h > 192) return false;
+    if (xg.nxc > 64 * kRowWords || xg.nyb_h > 64 * kBlockWords) return false;
+    // the inverse rows pass loads its half-spectrum elements in E / 2 + 1 steps
+    const uint64_t rows_nt = static_cast<uint64_t>(kPairs) * p.lanes_y;
+    if ((static_cast<uint64_t>(xg.nyb_h) * kPairs * kBlock + rows_nt - 1) / rows_nt >
+        static_cast<uint64_t>(p.e_y / 2 + 1))
+        return false;
     p.n_chan = static_cast<uint32_t>(n_chan);
     p.tw_x.alloc(tx.size() * sizeof(float2));
     p.tw_y.alloc(ty.size() * sizeof(float2));
@@ -733,20 +792,34 @@ bool plan_transforms(const chip_gridder &g, uint64_t n_chan, size_t lds_limit, X
 
 void forward_model(const XPlan &p, const float *model, float2 *half, float2 *grid, hipStream_t st,
                    hipEvent_t between) {
-    const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
-    const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
-    CHIP_XF_LAUNCH(p.e_y, k_fwd_rows, rows, kRowThreads, rl, st, p.xg, p.ay, model, half);
+    const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h * (kBlock >> p.logb_x), p.n_chan);
+    const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n, p.logb_x);
+    dispatch_axis(p.e_y, p.lanes_y, [&](auto E, auto L) {
+        constexpr int e = decltype(E)::value, l = decltype(L)::value;
+        CHIP_XF_GO((k_fwd_rows<e, l>), rows, kPairs * l, rl, st, p.xg, p.ay, model, half);
+    });
     if (between) CHIP_HIP(hipEventRecord(between, st));
-    CHIP_XF_LAUNCH(p.e_x, k_fwd_cols, cols, kThreads, cl, st, p.xg, p.ax, half, grid);
+    dispatch_axis(p.e_x, p.lanes_x, [&](auto E, auto L) {
+        constexpr int e = decltype(E)::value, l = decltype(L)::value;
+        constexpr int b = l == kLanes ? kLogBlock : kLongLogBlock;
+        CHIP_XF_GO((k_fwd_cols<e, l, b>), cols, l << b, cl, st, p.xg, p.ax, half, grid);
+    });
 }
 
 void inverse_residual(const XPlan &p, const float2 *grid, float2 *rows_buf, const float *nsum,
                       const float *peak, float *out, hipStream_t st, hipEvent_t between) {
-    const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h, p.n_chan);
-    const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n);
-    CHIP_XF_LAUNCH(p.e_x, k_inv_cols, cols, kThreads, cl, st, p.xg, p.ax, grid, rows_buf);
+    const dim3 rows(p.xg.nxp / kRowsPerGroup, p.n_chan), cols(p.xg.nyb_h * (kBlock >> p.logb_x), p.n_chan);
+    const size_t rl = row_lds(p.ay.n), cl = col_lds(p.ax.n, p.logb_x);
+    dispatch_axis(p.e_x, p.lanes_x, [&](auto E, auto L) {
+        constexpr int e = decltype(E)::value, l = decltype(L)::value;
+        constexpr int b = l == kLanes ? kLogBlock : kLongLogBlock;
+        CHIP_XF_GO((k_inv_cols<e, l, b>), cols, l << b, cl, st, p.xg, p.ax, grid, rows_buf);
+    });
     if (between) CHIP_HIP(hipEventRecord(between, st));
-    CHIP_XF_LAUNCH(p.e_y, k_inv_rows, rows, kRowThreads, rl, st, p.xg, p.ay, rows_buf, nsum, peak, out);
+    dispatch_axis(p.e_y, p.lanes_y, [&](auto E, auto L) {
+        constexpr int e = decltype(E)::value, l = decltype(L)::value;
+        CHIP_XF_GO((k_inv_rows<e, l>), rows, kPairs * l, rl, st, p.xg, p.ay, rows_buf, nsum, peak, out);
+    });
 }
 
 void zero_touched(const XPlan &p, float2 *grid, hipStream_t st) {

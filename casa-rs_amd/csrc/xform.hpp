@@ -26,6 +26,13 @@ constexpr uint32_t kRowsPerGroup = 8;            // rows-pass workgroup: 8 rows 
 constexpr uint32_t kPairs = kRowsPerGroup / 2;
 constexpr uint32_t kRowThreads = kPairs * kLanes;
 constexpr uint32_t kTouchRows = 16;              // x rows per chunk of the uv-coverage mask
+constexpr uint32_t kRowWords = 4;                // 64-bit words of a block's row-chunk mask: gx <= 4096
+constexpr uint32_t kBlockWords = 5;              // 64-bit words of a channel's block mask: nyb_h <= 320
+// Axes longer than 2048 points (up to 4096): 128 threads per transform, and
+// column workgroups of 4 columns (half an 8-column block) so that the tile
+// fits the LDS.
+constexpr uint32_t kLongLanes = 128;
+constexpr uint32_t kLongLogBlock = 2;
 
 // One transform axis: length n = product over the stages of r1[s] * r2[s]
 // (r1 in 2..5, r2 in 1..5), with the per-stage constants of the Stockham
@@ -73,6 +80,8 @@ struct XPlan {
     XGeom xg{};
     uint32_t n_chan = 0;
     int e_x = 0, e_y = 0;  // per-thread element budgets of the x and y transforms
+    uint32_t lanes_x = kLanes, lanes_y = kLanes;  // threads per transform of each axis
+    uint32_t logb_x = kLogBlock;                   // log2 columns per column-pass workgroup
     DevBuf tw_x, tw_y;
 };
 
