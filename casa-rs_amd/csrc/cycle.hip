@@ -33,7 +33,7 @@ namespace chip {
 void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res, hipStream_t st,
                         bool timed);
 void vset_split_items(chip_vset &vs, uint32_t gpi);
-void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order);
+void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order, bool force = false);
 
 namespace {
 
@@ -149,19 +149,20 @@ __global__ void k_touch(const WorkItem *__restrict__ items, uint64_t n_items, To
     if (ya == 0) touch_mark_mirror(t, tg, 0, 0, uxa, uxb);
 }
 
-// Schedule autotuner of the binned residual kernel: for both group orders
-// (anchor cell first, plan first) and every uv tile size that fits the LDS
-// with the set's halo (16, 32, 64 and the size the set was built with), the
-// stream is re-binned and every (threads per workgroup, groups per work item)
-// candidate is launched on the major cycle's own grids (timed with the set's
-// events); the fastest (tile, order, threads, groups) is kept, the earlier
-// candidate on a near tie.  make_gp builds the launch parameters for the
-// set's current tiling.  The grids' contents do not matter: the residual grid
-// is cleared again before the real pass.
+// Schedule autotuner of the binned residual kernel: for every uv tile size
+// that fits the LDS with the set's halo (the size the set was built with,
+// then 16, 32, 64) with groups ordered by anchor cell, and then for the
+// fastest tile with groups ordered by plan, the stream is re-binned and
+// every (threads per workgroup, groups per work item) candidate is launched
+// on the major cycle's own grids (timed with the set's events); the fastest
+// (tile, order, threads, groups) is kept, the earlier candidate on a near
+// tie.  make_gp builds the launch parameters for the set's current tiling.
+// The grids' contents do not matter: the residual grid is cleared again
+// before the real pass.
 template <class MakeGp>
 void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg, hipStream_t st) {
     static constexpr uint32_t kThreads[] = {256, 512};
-    static constexpr uint32_t kGpi[] = {256, 512, 1024, 2048};
+    static constexpr uint32_t kGpi[] = {256, 512, 1024};
     const chip_gridder &g = *vs.g;
     std::vector<uint32_t> tiles{vs.tile};
     for (uint32_t t : {16u, 32u, 64u})
@@ -175,6 +176,7 @@ void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg
     constexpr double kLong = 20.0, kBudget = 3000.0;
     double spent = 0.0;
     for (uint32_t order : {0u, 1u}) {
+        if (order == 1) tiles.assign(1, best_tile);  // plan order at the fastest tile only
         for (uint32_t tile : tiles) {
             if (spent > kBudget) break;
             vset_rebin(vs, tile, order);

@@ -594,12 +594,8 @@ void build_groups(chip_vset &vs, const BinSetup &s, const DevBuf &keys_s) {
 // Sort keys of a stream already planned: (bin, anchor cell, plan) from the
 // packed plan and the sample's original index (channel = index mod n_chan),
 // the same key k_keys makes from the samples.
-__global__ void k_rekey(const uint2 *plan, const uint32_t *index, uint64_t n, BinParams b, uint32_t wproject,
-                        int32_t R, uint64_t *keys, uint32_t *vals) {
-    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    const uint2 pk = plan[p];
-    const uint32_t chan = b.n_chan == 1 ? 0u : index[p] % b.n_chan;
+__device__ __forceinline__ uint64_t rekey_one(const uint2 pk, uint32_t chan, const BinParams &b, uint32_t wproject,
+                                              int32_t R) {
     const uint32_t ax = (pk.x & 0xffffu) + (wproject ? 0u : static_cast<uint32_t>(kSupport));
     const uint32_t ay = (pk.x >> 16) + (wproject ? 0u : static_cast<uint32_t>(kSupport));
     const uint32_t plane = packed_table(pk.y);
@@ -622,8 +618,33 @@ __global__ void k_rekey(const uint2 *plan, const uint32_t *index, uint64_t n, Bi
         }
         key |= b.plan_major ? ((code << b.cell_bits) | cell) : ((cell << b.plan_bits) | code);
     }
-    keys[p] = key;
+    return key;
+}
+
+__global__ void k_rekey(const uint2 *plan, const uint32_t *index, uint64_t n, BinParams b, uint32_t wproject,
+                        int32_t R, uint64_t *keys, uint32_t *vals) {
+    const uint64_t p = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const uint32_t chan = b.n_chan == 1 ? 0u : index[p] % b.n_chan;
+    keys[p] = rekey_one(plan[p], chan, b, wproject, R);
     vals[p] = static_cast<uint32_t>(p);
+}
+
+// The same key per group, from the packed plan of its first sample.
+__global__ void k_rekey_groups(const uint4 *grec, const uint2 *plan, const uint32_t *index, uint64_t n_groups,
+                               BinParams b, uint32_t wproject, int32_t R, uint64_t *keys, uint32_t *vals) {
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n_groups) return;
+    const uint32_t s0 = grec[g].x;
+    const uint32_t chan = b.n_chan == 1 ? 0u : index[s0] % b.n_chan;
+    keys[g] = rekey_one(plan[s0], chan, b, wproject, R);
+    vals[g] = static_cast<uint32_t>(g);
+}
+
+__global__ void k_bin_counts(const uint64_t *keys_s, uint64_t n, uint32_t low_bits, uint32_t *bin_groups) {
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    atomicAdd(&bin_groups[keys_s[g] >> low_bits], 1u);
 }
 
 template <class T>
@@ -734,16 +755,16 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
 }
 
 // Cuts the stream into tile x tile uv tiles again, with the groups of a bin
-// ordered by anchor cell (order 0) or by plan (order 1): new keys from the
-// packed plans, a radix sort, the stream permuted, groups and work items
-// rebuilt.
-// Group membership does not change (identical plans share their anchor, so
-// they share a tile of any size); only the order of groups and their binning
-// do.
-void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order) {
+// ordered by anchor cell (order 0) or by plan (order 1).  Group membership
+// does not change (identical plans share their anchor, so they share a tile
+// of any size); only the order of groups and their binning do, so the groups
+// are sorted by their new keys (from the packed plan of their first sample)
+// and their records and sums permuted, then bins and work items rebuilt.
+// force: the samples are sorted again and the groups rebuilt from them.
+void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order, bool force) {
     chip_gridder &g = *vs.g;
     if (order > 1) fail(CHIP_ERR_INVALID, "group order must be 0 (anchor cell first) or 1 (plan first)");
-    if (tile == vs.tile && order == vs.order) return;
+    if (tile == vs.tile && order == vs.order && !force) return;
     if (tile < 8 || tile > 256 || (tile & (tile - 1)))
         fail(CHIP_ERR_INVALID, "uv tile must be a power of two in 8 .. 256");
     if (binned_tile_lds(g, tile, vs.halo) > lds_limit(vs.device))
@@ -754,6 +775,38 @@ void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order) {
     vs.order = order;
     const uint64_t n = vs.n_valid;
     if (n == 0) return;
+    if (vs.n_groups > 0 && !force) {
+        // Groups keep their members (identical plans share their anchor, so
+        // they share a tile of any size): sort the groups, not the samples,
+        // by their new keys and permute their records and sums; each group's
+        // samples stay where they are.
+        const uint64_t ng = vs.n_groups;
+        DevBuf keys(ng * 8), vals(ng * 4), keys_s(ng * 8), vals_s(ng * 4);
+        hipLaunchKernelGGL(k_rekey_groups, dim3(blocks_for(ng, 256)), dim3(256), 0, st, vs.grec.as<uint4>(),
+                           vs.plan.as<uint2>(), vs.index.as<uint32_t>(), ng, bs.b, g.wproject ? 1u : 0u,
+                           static_cast<int32_t>(g.norm_radius), keys.as<uint64_t>(), vals.as<uint32_t>());
+        CHIP_HIP(hipGetLastError());
+        sort_pairs(keys, vals, keys_s, vals_s, ng, bs.end_bit, st);
+        keys.release();
+        vals.release();
+        gather<uint4>(vs.grec, vals_s, ng, st);
+        gather<float4>(vs.gsum_d, vals_s, ng, st);
+        gather<float4>(vs.gsum_r, vals_s, ng, st);
+        vals_s.release();
+        const uint32_t n_bins = bs.n_bins;
+        DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
+        CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
+        hipLaunchKernelGGL(k_bin_counts, dim3(blocks_for(ng, 256)), dim3(256), 0, st, keys_s.as<uint64_t>(), ng,
+                           bs.b.cell_bits + bs.b.plan_bits, bin_groups.as<uint32_t>());
+        CHIP_HIP(hipGetLastError());
+        vs.bin_first.alloc((static_cast<uint64_t>(n_bins) + 1) * 4);
+        exclusive_sum(bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(), n_bins + 1, st);
+        vs.bin_groups = std::move(bin_groups);
+        vs.n_bins = n_bins;
+        vs.n_tiles_all = static_cast<uint32_t>(bs.n_tiles);
+        vset_split_items(vs, vs.groups_per_item);
+        return;
+    }
     DevBuf keys(n * 8), vals(n * 4), keys_s(n * 8), vals_s(n * 4);
     hipLaunchKernelGGL(k_rekey, dim3(blocks_for(n, 256)), dim3(256), 0, st, vs.plan.as<uint2>(),
                        vs.index.as<uint32_t>(), n, bs.b, g.wproject ? 1u : 0u, static_cast<int32_t>(g.norm_radius),

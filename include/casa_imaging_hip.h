@@ -516,10 +516,12 @@ int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
  * applied by re-binning the stream on the device, group membership unchanged
  * -- threads per workgroup (a multiple of 64, at most 512) and groups of
  * identical plans per work item (the splitting of each bin).  Unless fixed
- * here, the first major cycle times every candidate of {anchor, plan order} x
- * {the set's tile, 16, 32, 64 where they fit} x {256, 512} threads x {256,
- * 512, 1024, 2048} groups on its own grids, in that order, and keeps the
- * fastest; it starts no new candidate after 3 s of timed launches, and a
+ * here, the first major cycle times every candidate of {the set's tile, 16,
+ * 32, 64 where they fit} x {256, 512} threads x {256, 512, 1024} groups with
+ * anchor order, then the same threads x groups with plan order at the
+ * fastest tile, on its own grids, and keeps the fastest (re-binning sorts
+ * the groups, not the samples); it starts no new candidate after 3 s of
+ * timed launches, and a
  * candidate whose launch takes over 20 ms is timed once without a warm-up
  * (default before that: 32 or the largest fitting tile, anchor order, 512,
  * 1024).

@@ -66,8 +66,16 @@ def test_first_major_cycle_keeps_the_fastest_candidate(gpu):
     vs.major_cycle(model, nsum, peaks)
     sch = vs.schedule()
     timed = sch["timed"]
-    assert sorted((tl, od, a, b) for tl, od, a, b, _ in timed) == sorted(
-        (tl, od, a, b) for tl in (16, 32, 64) for od in (0, 1) for a in (256, 512) for b in (256, 512, 1024, 2048))
+    # anchor order at every tile, then plan order at the fastest tile
+    anchor = [x for x in timed if x[1] == 0]
+    plan = [x for x in timed if x[1] == 1]
+    assert sorted((tl, a, b) for tl, _, a, b, _ in anchor) == sorted(
+        (tl, a, b) for tl in (16, 32, 64) for a in (256, 512) for b in (256, 512, 1024))
+    plan_tile = {x[0] for x in plan}
+    assert len(plan_tile) == 1 and sorted((a, b) for _, _, a, b, _ in plan) == sorted(
+        (a, b) for a in (256, 512) for b in (256, 512, 1024))
+    fastest_anchor = min(x[4] for x in anchor)
+    assert any(x[0] in plan_tile and x[4] <= fastest_anchor / 0.99 + 1e-9 for x in anchor)
     assert all(ms > 0 for *_, ms in timed)
     chosen = [x for x in timed if x[:4] == (sch["tile"], sch["order"], sch["threads"], sch["groups_per_item"])]
     assert len(chosen) == 1 and chosen[0][4] <= min(x[4] for x in timed) / 0.99 + 1e-9
