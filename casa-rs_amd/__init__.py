@@ -12,7 +12,8 @@ from .abi import (CLEAN_BUDGET, CLEAN_CYCLE, CLEAN_GLOBAL, CLEAN_NO_PIXELS, CLEA
                   CYCLE_HIPFFT, F32, F64, MODE_DIRTY, MODE_PSF, MODE_RESIDUAL, WEIGHT_BRIGGS,
                   WEIGHT_UNIFORM, ImagingError, InvalidRequest, NoUsableSamples, Normalization,
                   Unsupported)
-from .imaging import (PACKED_W_SAMPLE, Context, HostBuffer, RowChannelBatch, RowIngest, StandardGridder,
+from .imaging import (PACKED_W_SAMPLE, AwConvolutionFunctionCache, AwConvolutionFunctionInventory,
+                      AwConvolutionFunctionKey, Context, HostBuffer, RowChannelBatch, RowIngest, StandardGridder,
                       StandardMfsModelPredictor, VisibilityBatch, VisibilitySet, grid_wproject_samples_host,
                       pack_wproject_samples,
                       airy_table, default_context, mosaic_phase_gradient, phase_rotation,
@@ -27,4 +28,5 @@ __all__ = [
     "AW_INVALID_NORMALIZATION", "AW_NO_CELL", "run_hogbom_minor_cycle", "CLEAN_BUDGET", "CLEAN_GLOBAL",
     "CLEAN_NSIGMA", "CLEAN_CYCLE", "CLEAN_NO_PIXELS", "RowIngest", "StandardMfsModelPredictor",
     "PACKED_W_SAMPLE", "HostBuffer", "grid_wproject_samples_host", "pack_wproject_samples",
+    "AwConvolutionFunctionCache", "AwConvolutionFunctionInventory", "AwConvolutionFunctionKey",
 ]

@@ -17,6 +17,7 @@ CHIP_ERR_INVALID = 1
 CHIP_ERR_NO_SAMPLES = 2
 CHIP_ERR_NORMALIZATION = 3
 CHIP_ERR_DEVICE = 4
+CHIP_ERR_CF_CACHE = 5
 
 MODE_PSF = 0
 MODE_DIRTY = 1
@@ -67,6 +68,9 @@ EXPORTS = (
     "chip_aw_select_cells", "chip_aw_cells", "chip_aw_plan",
     "chip_airy_table", "chip_hogbom",
     "chip_ingest_create", "chip_ingest_destroy", "chip_ingest_push", "chip_ingest_samples", "chip_ingest_download",
+    "chip_cfcache_open", "chip_cfcache_close", "chip_cfcache_info_get", "chip_cfcache_keys", "chip_cfcache_metadata",
+    "chip_cfcache_nearest_key", "chip_cfcache_select_key", "chip_cfcache_load", "chip_gridder_attach_aw_cache",
+    "chip_cfcache_select_cells",
 )
 
 
@@ -92,11 +96,16 @@ class Unsupported(ImagingError):
     code = CHIP_ERR_DEVICE
 
 
+class ConvolutionFunctionCache(ImagingError):
+    code = CHIP_ERR_CF_CACHE
+
+
 _ERRORS = {
     CHIP_ERR_INVALID: InvalidRequest,
     CHIP_ERR_NO_SAMPLES: NoUsableSamples,
     CHIP_ERR_NORMALIZATION: Normalization,
     CHIP_ERR_DEVICE: Unsupported,
+    CHIP_ERR_CF_CACHE: ConvolutionFunctionCache,
 }
 
 
@@ -194,6 +203,31 @@ _U32 = ctypes.c_uint32
 _U64 = ctypes.c_uint64
 
 
+class CfKey(ctypes.Structure):
+    _fields_ = [("frequency_hz", ctypes.c_double), ("w_value_lambda", ctypes.c_double),
+                ("mueller_element", ctypes.c_int32), ("parallactic_angle_deg", ctypes.c_double)]
+
+
+class CfKernelMeta(ctypes.Structure):
+    _fields_ = [
+        ("shape_x", ctypes.c_uint64), ("shape_y", ctypes.c_uint64),
+        ("sampling", ctypes.c_uint64), ("x_support", ctypes.c_uint64), ("y_support", ctypes.c_uint64),
+        ("uv_reference_value", ctypes.c_double * 2), ("uv_reference_pixel", ctypes.c_double * 2),
+        ("uv_increment", ctypes.c_double * 2), ("uv_pc", ctypes.c_double * 4),
+        ("diameter_m", ctypes.c_double), ("conjugate_frequency_hz", ctypes.c_double),
+        ("w_increment", ctypes.c_double),
+        ("conjugate_polarization", ctypes.c_int32), ("polarization", ctypes.c_int32),
+        ("rotationally_symmetric", ctypes.c_int32),
+        ("telescope_name", ctypes.c_char * 64), ("band_name", ctypes.c_char * 64),
+    ]
+
+
+class CfCacheInfo(ctypes.Structure):
+    _fields_ = [("paired_cells", ctypes.c_uint64), ("n_freq", ctypes.c_uint32), ("n_w", ctypes.c_uint32),
+                ("n_mueller", ctypes.c_uint32), ("n_pa", ctypes.c_uint32), ("w_increment", ctypes.c_double),
+                ("metadata_fingerprint", ctypes.c_uint64)]
+
+
 def load():
     """Load the library (raises if it has not been built)."""
     global _lib
@@ -270,6 +304,22 @@ def load():
         "chip_aw_cells": (ctypes.c_int, [_P, _P, _P, _P, _P, _PP]),
         "chip_aw_plan": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, _P, _P, _PP]),
         "chip_airy_table": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, _P, _P, _PP]),
+        "chip_cfcache_open": (ctypes.c_int, [ctypes.c_char_p, _PP, _PP]),
+        "chip_cfcache_close": (None, [_P]),
+        "chip_cfcache_info_get": (ctypes.c_int, [_P, ctypes.POINTER(CfCacheInfo), _P, _P, _P, _P, _PP]),
+        "chip_cfcache_keys": (ctypes.c_int, [_P, _P, _PP]),
+        "chip_cfcache_metadata": (ctypes.c_int, [_P, _U64, ctypes.POINTER(CfKernelMeta),
+                                                 ctypes.POINTER(CfKernelMeta), _PP]),
+        "chip_cfcache_nearest_key": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_int32,
+                                                    ctypes.c_double, ctypes.POINTER(ctypes.c_int64), _PP]),
+        "chip_cfcache_select_key": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                   ctypes.c_int32, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                   ctypes.POINTER(ctypes.c_int64), _PP]),
+        "chip_cfcache_load": (ctypes.c_int, [_P, _U64, _P, _P, _PP]),
+        "chip_gridder_attach_aw_cache": (ctypes.c_int, [_P, _P, _U32, _P, _PP]),
+        "chip_cfcache_select_cells": (ctypes.c_int, [_P, ctypes.POINTER(Samples), _P, ctypes.c_double,
+                                                     ctypes.c_int32, _P, ctypes.c_double, ctypes.c_int,
+                                                     ctypes.c_int, _P, _PP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

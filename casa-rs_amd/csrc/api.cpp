@@ -1,5 +1,6 @@
 // extern "C" entry points of libcasa_imaging_hip.so.  Each wrapper converts
 // exceptions into the status / message convention of the header.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -58,6 +59,39 @@ int guarded(char **err, F &&f) {
 }
 
 void set_device(const chip_ctx *ctx) { CHIP_HIP(hipSetDevice(ctx->device)); }
+
+}  // namespace
+
+namespace {
+
+const CfCache &cache_of(const chip_cfcache *c) {
+    if (!c) fail(CHIP_ERR_INVALID, "NULL CF cache");
+    return c->c;
+}
+
+void copy_meta(const CfKernelMeta &m, chip_cf_kernel_meta *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->shape_x = m.shape[0];
+    o->shape_y = m.shape[1];
+    o->sampling = m.sampling;
+    o->x_support = m.x_support;
+    o->y_support = m.y_support;
+    for (int k = 0; k < 2; ++k) {
+        o->uv_reference_value[k] = m.uv[k];
+        o->uv_reference_pixel[k] = m.uv[2 + k];
+        o->uv_increment[k] = m.uv[4 + k];
+    }
+    for (int k = 0; k < 4; ++k) o->uv_pc[k] = m.uv[6 + k];
+    o->diameter_m = m.diameter_m;
+    o->conjugate_frequency_hz = m.conj_freq_hz;
+    o->w_increment = m.w_increment;
+    o->conjugate_polarization = m.conj_poln;
+    o->polarization = m.polarization;
+    o->rotationally_symmetric = m.rotationally_symmetric ? 1 : 0;
+    std::strncpy(o->telescope_name, m.telescope.c_str(), sizeof(o->telescope_name) - 1);
+    std::strncpy(o->band_name, m.band.c_str(), sizeof(o->band_name) - 1);
+}
 
 }  // namespace
 
@@ -282,6 +316,114 @@ int chip_airy_table(double dish_m, double blockage_m, float *values, double *max
             std::memcpy(values, t.data(), t.size() * sizeof(float));
         }
         if (max_radius) *max_radius = airy_max_radius(dish_m, blockage_m);
+    });
+}
+
+// ---- CASA CF caches (cfcache.cpp, aw.hip) ----
+
+int chip_cfcache_open(const char *path, chip_cfcache **out, char **err) {
+    return guarded(err, [&] {
+        if (!path || !out) fail(CHIP_ERR_INVALID, "NULL argument");
+        *out = nullptr;
+        auto c = std::make_unique<chip_cfcache>();
+        cf_cache_open(c->c, path);
+        *out = c.release();
+    });
+}
+
+void chip_cfcache_close(chip_cfcache *c) { delete c; }
+
+int chip_cfcache_info_get(const chip_cfcache *c, chip_cfcache_info *info, double *freqs, double *w_values,
+                          int32_t *muellers, double *pas, char **err) {
+    return guarded(err, [&] {
+        const CfCache &x = cache_of(c);
+        if (info) {
+            info->paired_cells = x.keys.size();
+            info->n_freq = static_cast<uint32_t>(x.freqs.size());
+            info->n_w = static_cast<uint32_t>(x.w_values.size());
+            info->n_mueller = static_cast<uint32_t>(x.muellers.size());
+            info->n_pa = static_cast<uint32_t>(x.pas.size());
+            info->w_increment = x.w_increment;
+            info->metadata_fingerprint = x.fingerprint;
+        }
+        if (freqs) std::copy(x.freqs.begin(), x.freqs.end(), freqs);
+        if (w_values) std::copy(x.w_values.begin(), x.w_values.end(), w_values);
+        if (muellers) std::copy(x.muellers.begin(), x.muellers.end(), muellers);
+        if (pas) std::copy(x.pas.begin(), x.pas.end(), pas);
+    });
+}
+
+int chip_cfcache_keys(const chip_cfcache *c, chip_cf_key *keys, char **err) {
+    return guarded(err, [&] {
+        const CfCache &x = cache_of(c);
+        if (!keys) fail(CHIP_ERR_INVALID, "NULL output");
+        for (size_t k = 0; k < x.keys.size(); ++k)
+            keys[k] = chip_cf_key{x.keys[k].frequency_hz, x.keys[k].w_value_lambda, x.keys[k].mueller_element,
+                                  x.keys[k].parallactic_angle_deg};
+    });
+}
+
+int chip_cfcache_metadata(const chip_cfcache *c, uint64_t index, chip_cf_kernel_meta *imaging,
+                          chip_cf_kernel_meta *weight, char **err) {
+    return guarded(err, [&] {
+        const CfCache &x = cache_of(c);
+        if (index >= x.keys.size()) fail(CHIP_ERR_INVALID, "CF cell index out of range");
+        copy_meta(x.imaging[index], imaging);
+        copy_meta(x.weight[index], weight);
+    });
+}
+
+int chip_cfcache_nearest_key(const chip_cfcache *c, double frequency_hz, double w_value_lambda, int32_t mueller,
+                             double parallactic_angle_deg, int64_t *index, char **err) {
+    return guarded(err, [&] {
+        const CfCache &x = cache_of(c);
+        if (!index) fail(CHIP_ERR_INVALID, "NULL output");
+        *index = cf_nearest_key(x, frequency_hz, w_value_lambda, mueller, parallactic_angle_deg);
+    });
+}
+
+int chip_cfcache_select_key(const chip_cfcache *c, double sample_frequency_hz, double reference_frequency_hz,
+                            double w_lambda, int32_t mueller, double parallactic_angle_deg, int conjugate_beams,
+                            int prediction, int64_t *index, char **err) {
+    return guarded(err, [&] {
+        const CfCache &x = cache_of(c);
+        if (!index) fail(CHIP_ERR_INVALID, "NULL output");
+        *index = cf_select_key(x, sample_frequency_hz, reference_frequency_hz, w_lambda, mueller,
+                               parallactic_angle_deg, conjugate_beams != 0, prediction != 0);
+    });
+}
+
+int chip_cfcache_load(const chip_cfcache *c, uint64_t index, float *imaging, float *weight, char **err) {
+    return guarded(err, [&] {
+        const CfCache &x = cache_of(c);
+        if (index >= x.keys.size()) fail(CHIP_ERR_INVALID, "CF cell index out of range");
+        // both cells are read (and checked) before either output is written
+        const std::vector<float> im = cf_read_kernel_pixels(x.imaging[index]);
+        const std::vector<float> wt = cf_read_kernel_pixels(x.weight[index]);
+        if (imaging) std::memcpy(imaging, im.data(), im.size() * sizeof(float));
+        if (weight) std::memcpy(weight, wt.data(), wt.size() * sizeof(float));
+    });
+}
+
+int chip_gridder_attach_aw_cache(chip_gridder *g, const chip_cfcache *c, uint32_t n_pointings,
+                                 const double *phase_gradients, char **err) {
+    return guarded(err, [&] {
+        if (!g) fail(CHIP_ERR_INVALID, "NULL gridder");
+        const CfCache &x = cache_of(c);
+        set_device(g->ctx);
+        attach_aw_cache(*g, x, n_pointings, phase_gradients);
+    });
+}
+
+int chip_cfcache_select_cells(chip_gridder *g, const chip_samples *s, const double *chan_freq_hz,
+                              double reference_frequency_hz, int32_t mueller, const double *pa_deg_dev,
+                              double pa_deg, int conjugate_beams, int prediction, uint32_t *cf_cell_dev,
+                              char **err) {
+    return guarded(err, [&] {
+        if (!g || !s) fail(CHIP_ERR_INVALID, "NULL argument");
+        set_device(g->ctx);
+        cf_select_cells(*g, *s, chan_freq_hz, reference_frequency_hz, mueller, pa_deg_dev, pa_deg,
+                        conjugate_beams, prediction, cf_cell_dev);
     });
 }
 

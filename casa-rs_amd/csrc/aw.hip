@@ -595,6 +595,161 @@ void aw_select_cells(chip_gridder &g, const chip_samples &cs, uint32_t *out) {
     CHIP_HIP(hipStreamSynchronize(st));
 }
 
+// Every imaging cell of a CASA CF cache, in key order, as the gridder's AW
+// atlas (AwProjector::new per selected cell, gridder.rs:2194-2232): one
+// oversampling for all cells, the cache's WIncr, and the cache's selection
+// axes kept for cf_select_cells.
+void attach_aw_cache(chip_gridder &g, const CfCache &c, uint32_t n_pointings, const double *gradients) {
+    if (c.keys.empty()) fail(CHIP_ERR_INVALID, "empty CF cache");
+    const uint64_t sampling = c.imaging[0].sampling;
+    for (const CfKernelMeta &m : c.imaging)
+        if (m.sampling != sampling)
+            fail(CHIP_ERR_INVALID, "CF cells of one atlas share one oversampling: " + m.path + " differs");
+    if (sampling > 127) fail(CHIP_ERR_INVALID, "AW sampling must be 1..127");
+    if (c.keys.size() > 0xffffffffull) fail(CHIP_ERR_INVALID, "too many CF cells");
+    std::vector<chip_cf_cell> cells(c.keys.size());
+    std::vector<float> host;
+    for (size_t k = 0; k < c.keys.size(); ++k) {
+        const CfKernelMeta &m = c.imaging[k];
+        if (m.x_support > 1024 || m.y_support > 1024)
+            fail(CHIP_ERR_INVALID, "CF cell " + m.path + " support exceeds 1024");
+        std::vector<float> px = cf_read_kernel_pixels(m);
+        cells[k] = chip_cf_cell{m.shape[0], m.shape[1], static_cast<uint32_t>(m.x_support),
+                                static_cast<uint32_t>(m.y_support), host.size() / 2};
+        host.insert(host.end(), px.begin(), px.end());
+    }
+    DevBuf pixels(host.size() * sizeof(float));
+    hipStream_t st = g.ctx->stream;
+    CHIP_HIP(hipMemcpyAsync(pixels.p, host.data(), pixels.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipStreamSynchronize(st));
+    attach_aw(g, static_cast<uint32_t>(cells.size()), cells.data(), pixels.as<float>(), host.size() / 2,
+              static_cast<uint32_t>(sampling), n_pointings, gradients, c.w_increment);
+    g.aw_from_cache = true;
+    g.aw_cache_axes = static_cast<const CfAxes &>(c);
+}
+
+// CASA CFBuffer selection of a CF cache cell per sample (select_key_for_sample
+// and select_key_for_prediction_sample, awprojection.rs:1106-1300), the same
+// FP64 steps as the host cf_select_key: conjugate-beam frequency map, Mueller
+// element m or 15 - m by the sign of w, W plane round(sqrt(WIncr |w|)),
+// nearest frequency and circularly nearest PA with ties to the smaller bit
+// pattern.  Samples without a cell get 0xffffffff (rejected by the planner).
+struct CfSelectArgs {
+    const double *chan_freq;  // per channel, or NULL: lambda_scale * c
+    double f_ref, pa_const;
+    const double *pa_row;     // per row, or NULL: pa_const
+    int32_t mueller, conjugate_beams, prediction;
+    const double *freqs, *pas;
+    const int32_t *muellers;
+    const uint32_t *cell_of;
+    uint32_t nf, nw, nm, np;
+    double w_incr;
+};
+
+__device__ inline double cf_circular(double v) {
+    double r = fmod(v + 180.0, 360.0);
+    if (r < 0.0) r += 360.0;
+    return r - 180.0;
+}
+
+__global__ __launch_bounds__(256) void k_cf_select(Source s, CfSelectArgs a, uint32_t *out) {
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= s.n_rows * s.n_chan) return;
+    const uint64_t row = s.n_chan == 1 ? i : i / s.n_chan;
+    const uint32_t chan = s.n_chan == 1 ? 0u : static_cast<uint32_t>(i - row * s.n_chan);
+    const double ls = s.lambda_scale[chan];
+    const double f = a.chan_freq ? a.chan_freq[chan] : ls * 299792458.0;
+    const double w = s.uvw_m[3 * row + 2] * ls;
+    const double pa = a.pa_row ? a.pa_row[row] : a.pa_const;
+    uint32_t cell = 0xffffffffu;
+    bool ok = isfinite(f) && f > 0.0 && isfinite(w) && isfinite(pa);
+    double req = f;
+    if (ok && !a.prediction) {
+        ok = isfinite(a.f_ref) && a.f_ref > 0.0;
+        if (ok && a.conjugate_beams) {
+            const double rad = 2.0 * a.f_ref * a.f_ref - f * f;
+            ok = isfinite(rad) && rad > 0.0;
+            req = ok ? sqrt(rad) : 0.0;
+        }
+    }
+    if (ok) {
+        const bool direct = a.prediction ? !(w > 0.0) : (w > 0.0);
+        const int64_t m = direct ? static_cast<int64_t>(a.mueller) : 15 - static_cast<int64_t>(a.mueller);
+        uint32_t mi = a.nm;
+        for (uint32_t k = 0; k < a.nm; ++k)
+            if (a.muellers[k] == m) mi = k;
+        if (mi < a.nm) {
+            uint32_t wi = 0;
+            if (a.nw > 1 && a.w_incr != 0.0) {
+                double r = round(sqrt(a.w_incr * fabs(w)));
+                r = fmin(fmax(r, 0.0), static_cast<double>(a.nw - 1));
+                wi = static_cast<uint32_t>(r);
+            }
+            uint32_t fi = 0;
+            for (uint32_t k = 1; k < a.nf; ++k) {
+                const double x = fabs(a.freqs[k] - req), y = fabs(a.freqs[fi] - req);
+                if (x < y || (x == y && static_cast<uint64_t>(__double_as_longlong(a.freqs[k])) <
+                                            static_cast<uint64_t>(__double_as_longlong(a.freqs[fi]))))
+                    fi = k;
+            }
+            uint32_t pi = 0;
+            for (uint32_t k = 1; k < a.np; ++k) {
+                const double x = fabs(cf_circular(a.pas[k] - pa)), y = fabs(cf_circular(a.pas[pi] - pa));
+                if (x < y || (x == y && static_cast<uint64_t>(__double_as_longlong(a.pas[k])) <
+                                            static_cast<uint64_t>(__double_as_longlong(a.pas[pi]))))
+                    pi = k;
+            }
+            cell = a.cell_of[((static_cast<uint64_t>(fi) * a.nw + wi) * a.nm + mi) * a.np + pi];
+        }
+    }
+    out[i] = cell;
+}
+
+void cf_select_cells(chip_gridder &g, const chip_samples &cs, const double *chan_freq_hz, double f_ref,
+                     int32_t mueller, const double *pa_deg_dev, double pa_deg, int conjugate_beams, int prediction,
+                     uint32_t *out) {
+    if (!g.aw || !g.aw_from_cache) fail(CHIP_ERR_INVALID, "no CF-cache atlas attached");
+    if (!cs.uvw_m_dev || !cs.lambda_scale_dev || cs.n_chan == 0)
+        fail(CHIP_ERR_INVALID, "CF selection needs rows x channels input (uvw_m, lambda_scale)");
+    if (!out) fail(CHIP_ERR_INVALID, "NULL output");
+    const uint64_t n = cs.n_rows * cs.n_chan;
+    if (n == 0) return;
+    const CfAxes &x = g.aw_cache_axes;
+    hipStream_t st = g.ctx->stream;
+    const size_t nd = x.freqs.size() + x.pas.size() + (chan_freq_hz ? cs.n_chan : 0);
+    std::vector<double> hd;
+    hd.reserve(nd);
+    hd.insert(hd.end(), x.freqs.begin(), x.freqs.end());
+    hd.insert(hd.end(), x.pas.begin(), x.pas.end());
+    if (chan_freq_hz) hd.insert(hd.end(), chan_freq_hz, chan_freq_hz + cs.n_chan);
+    DevBuf d_d(nd * sizeof(double)), d_m(x.muellers.size() * sizeof(int32_t)),
+        d_c(x.cell_of.size() * sizeof(uint32_t));
+    CHIP_HIP(hipMemcpyAsync(d_d.p, hd.data(), d_d.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipMemcpyAsync(d_m.p, x.muellers.data(), d_m.bytes, hipMemcpyHostToDevice, st));
+    CHIP_HIP(hipMemcpyAsync(d_c.p, x.cell_of.data(), d_c.bytes, hipMemcpyHostToDevice, st));
+    CfSelectArgs a{};
+    a.freqs = d_d.as<double>();
+    a.pas = a.freqs + x.freqs.size();
+    a.chan_freq = chan_freq_hz ? a.pas + x.pas.size() : nullptr;
+    a.f_ref = f_ref;
+    a.pa_const = pa_deg;
+    a.pa_row = pa_deg_dev;
+    a.mueller = mueller;
+    a.conjugate_beams = conjugate_beams != 0;
+    a.prediction = prediction != 0;
+    a.muellers = d_m.as<int32_t>();
+    a.cell_of = d_c.as<uint32_t>();
+    a.nf = static_cast<uint32_t>(x.freqs.size());
+    a.nw = static_cast<uint32_t>(x.w_values.size());
+    a.nm = static_cast<uint32_t>(x.muellers.size());
+    a.np = static_cast<uint32_t>(x.pas.size());
+    a.w_incr = x.w_increment;
+    const Source src = make_source(cs);
+    hipLaunchKernelGGL(k_cf_select, dim3(nb(n, 256)), dim3(256), 0, st, src, a, out);
+    CHIP_HIP(hipGetLastError());
+    CHIP_HIP(hipStreamSynchronize(st));
+}
+
 void aw_plan_raw(chip_gridder &g, const Source &src, int32_t *plan, float *n32, double *n64) {
     if (!g.aw) fail(CHIP_ERR_INVALID, "no AW atlas attached");
     const uint64_t n = src.n_rows * src.n_chan;

@@ -98,6 +98,60 @@ struct WkState {
 };
 }  // namespace chip
 
+namespace chip {
+
+// One CASA CF-cache cell key (AwConvolutionFunctionKey, awprojection.rs:36-62).
+struct CfKey {
+    double frequency_hz = 0.0, w_value_lambda = 0.0;
+    int32_t mueller_element = 0;
+    double parallactic_angle_deg = 0.0;
+};
+
+// AwConvolutionFunctionKernelMetadata (awprojection.rs:97-128); uv holds
+// crval, crpix, cdelt (two each) and the row-major PC matrix.
+struct CfKernelMeta {
+    std::string path, relative;
+    uint64_t shape[2] = {0, 0};
+    uint64_t sampling = 0, x_support = 0, y_support = 0;
+    double uv[10] = {};
+    std::string telescope, band;
+    double diameter_m = 0.0, conj_freq_hz = 0.0;
+    int32_t conj_poln = 0, polarization = 0;
+    double w_increment = 0.0;
+    bool rotationally_symmetric = false;
+};
+
+// Selection axes of a validated cache: sorted unique frequencies, W values,
+// Mueller elements and PAs, and the cell of every index combination.
+struct CfAxes {
+    std::vector<double> freqs, w_values, pas;
+    std::vector<int32_t> muellers;
+    std::vector<uint32_t> cell_of;  // ((f * n_w + w) * n_mueller + m) * n_pa + p
+    double w_increment = 0.0;
+};
+
+// A validated, indexed cache (AwConvolutionFunctionCache, awprojection.rs:920-1045);
+// cells are in key order (frequency, W, Mueller, PA bit patterns).
+struct CfCache : CfAxes {
+    std::string root;
+    std::vector<CfKey> keys;
+    std::vector<CfKernelMeta> imaging, weight;
+    uint64_t fingerprint = 0;
+};
+
+void cf_cache_open(CfCache &c, const std::string &root);
+CfKernelMeta cf_read_kernel_metadata(const std::string &dir, const std::string &relative, CfKey &key);
+std::vector<float> cf_read_kernel_pixels(const CfKernelMeta &m);
+int64_t cf_nearest_key(const CfCache &c, double f, double w, int32_t mueller, double pa);
+int64_t cf_select_key(const CfCache &c, double f, double f_ref, double w, int32_t mueller, double pa,
+                      bool conjugate_beams, bool prediction);
+
+}  // namespace chip
+
+struct chip_cfcache {
+    chip::CfCache c;
+};
+
 struct chip_ctx {
     int device = 0;
     hipStream_t own_stream = nullptr;  // created with the context
@@ -165,6 +219,9 @@ struct chip_gridder {
     // frequencies (cell f * aw_n_w + k is frequency f, W plane k)
     uint32_t aw_n_w = 0;
     std::vector<double> aw_cf_freq;
+    // Atlas attached from a CASA CF cache: its selection axes (cells in key order)
+    bool aw_from_cache = false;
+    chip::CfAxes aw_cache_axes;
 };
 
 namespace chip {
@@ -269,6 +326,10 @@ void attach_aw(chip_gridder &g, uint32_t n_cells, const chip_cf_cell *cells, con
 void attach_aw_airy(chip_gridder &g, const chip_airy_params &p, uint32_t n_freq, const double *freqs,
                     uint32_t n_pointings, const double *gradients);
 void aw_select_cells(chip_gridder &g, const chip_samples &cs, uint32_t *out);
+void attach_aw_cache(chip_gridder &g, const CfCache &c, uint32_t n_pointings, const double *gradients);
+void cf_select_cells(chip_gridder &g, const chip_samples &cs, const double *chan_freq_hz, double f_ref,
+                     int32_t mueller, const double *pa_deg_dev, double pa_deg, int conjugate_beams, int prediction,
+                     uint32_t *out);
 void aw_plan_raw(chip_gridder &g, const Source &src, int32_t *plan, float *n32, double *n64);
 double airy_max_radius(double dish, double blockage);
 std::vector<float> airy_table(double dish, double blockage);

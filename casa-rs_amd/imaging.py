@@ -505,6 +505,38 @@ class StandardGridder:
         abi.call("chip_aw_select_cells", self.handle, ctypes.byref(s), _ptr(out))
         return out
 
+    def attach_aw_cache(self, cache: "AwConvolutionFunctionCache", phase_gradients=((0.0, 0.0),)):
+        """Every imaging cell of a CASA CF cache, in key order, as the AW atlas
+        (atlas cell k = cache.keys()[k])."""
+        g = np.ascontiguousarray(phase_gradients, np.float64).reshape(-1)
+        abi.call("chip_gridder_attach_aw_cache", self.handle, cache.handle, len(g) // 2,
+                 g.ctypes.data_as(ctypes.c_void_p))
+        self._refresh()
+        return self
+
+    def cf_select_cells(self, batch, mueller_element, *, chan_freq_hz=None, reference_frequency_hz=0.0,
+                        parallactic_angle_deg=0.0, conjugate_beams=False, prediction=False):
+        """Per-sample CF-cache cell of a RowChannelBatch on the device
+        (select_key_for_sample / select_key_for_prediction_sample,
+        awprojection.rs:1106-1300): int32 device tensor (rows, chan), -1 where
+        no cell is selected.  parallactic_angle_deg: a scalar or one value per
+        row (degrees)."""
+        s, keep = batch.to_samples()
+        dev = self.ctx.torch_device
+        out = torch.empty((s.n_rows, s.n_chan), dtype=torch.int32, device=dev)
+        pa_dev, pa_const = None, 0.0
+        if np.ndim(parallactic_angle_deg) == 0:
+            pa_const = float(parallactic_angle_deg)
+        else:
+            pa_dev = torch.as_tensor(np.asarray(parallactic_angle_deg, np.float64), device=dev).contiguous()
+            keep = (keep, pa_dev)
+        cf = None if chan_freq_hz is None else np.ascontiguousarray(chan_freq_hz, np.float64)
+        abi.call("chip_cfcache_select_cells", self.handle, ctypes.byref(s),
+                 None if cf is None else cf.ctypes.data_as(ctypes.c_void_p), float(reference_frequency_hz),
+                 int(mueller_element), _ptr(pa_dev), pa_const, int(bool(conjugate_beams)), int(bool(prediction)),
+                 _ptr(out))
+        return out
+
     def aw_cells(self):
         """The attached atlas: list of (pixels complex64 (shape_x, shape_y), x_support, y_support)."""
         n = ctypes.c_uint32()
@@ -617,6 +649,146 @@ class StandardGridder:
     def close(self):
         if self._h:
             abi.load().chip_gridder_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass(frozen=True)
+class AwConvolutionFunctionKey:
+    """AwConvolutionFunctionKey (awprojection.rs:36-62)."""
+    frequency_hz: float
+    w_value_lambda: float
+    mueller_element: int
+    parallactic_angle_deg: float
+
+
+@dataclass(frozen=True)
+class AwConvolutionFunctionInventory:
+    """AwConvolutionFunctionInventory (awprojection.rs:165-178)."""
+    paired_cells: int
+    frequencies_hz: list
+    w_values_lambda: list
+    mueller_elements: list
+    parallactic_angles_deg: list
+
+
+class AwConvolutionFunctionCache:
+    """A read-only CASA AWProject CF cache (AwConvolutionFunctionCache,
+    awprojection.rs:920-1412): CFS_*.im / WTCFS_*.im casacore images, indexed
+    and validated by the library's reader.  Host only."""
+
+    def __init__(self, handle, path):
+        self._h = handle
+        self.path = path
+
+    @classmethod
+    def open(cls, path) -> "AwConvolutionFunctionCache":
+        h = ctypes.c_void_p()
+        abi.call("chip_cfcache_open", str(path).encode(), ctypes.byref(h))
+        return cls(h, str(path))
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise abi.InvalidRequest("CF cache is closed")
+        return self._h
+
+    def _info(self):
+        info = abi.CfCacheInfo()
+        abi.call("chip_cfcache_info_get", self.handle, ctypes.byref(info), None, None, None, None)
+        return info
+
+    def inventory(self) -> AwConvolutionFunctionInventory:
+        i = self._info()
+        f = np.empty(i.n_freq, np.float64)
+        w = np.empty(i.n_w, np.float64)
+        m = np.empty(i.n_mueller, np.int32)
+        p = np.empty(i.n_pa, np.float64)
+        abi.call("chip_cfcache_info_get", self.handle, None, f.ctypes.data_as(ctypes.c_void_p),
+                 w.ctypes.data_as(ctypes.c_void_p), m.ctypes.data_as(ctypes.c_void_p),
+                 p.ctypes.data_as(ctypes.c_void_p))
+        return AwConvolutionFunctionInventory(int(i.paired_cells), f.tolist(), w.tolist(),
+                                              [int(x) for x in m], p.tolist())
+
+    @property
+    def w_increment(self) -> float:
+        return float(self._info().w_increment)
+
+    @property
+    def metadata_fingerprint(self) -> int:
+        return int(self._info().metadata_fingerprint)
+
+    def keys(self):
+        n = int(self._info().paired_cells)
+        ks = (abi.CfKey * n)()
+        abi.call("chip_cfcache_keys", self.handle, ks)
+        return [AwConvolutionFunctionKey(k.frequency_hz, k.w_value_lambda, int(k.mueller_element),
+                                         k.parallactic_angle_deg) for k in ks]
+
+    def index(self, key: AwConvolutionFunctionKey) -> int:
+        for k, y in enumerate(self.keys()):
+            if y == key:
+                return k
+        raise abi.ConvolutionFunctionCache(f"{self.path}: no exact CF cell for key {key}")
+
+    def metadata(self, index: int):
+        """(imaging, weight) AwConvolutionFunctionKernelMetadata as dicts."""
+        out = []
+        im, wt = abi.CfKernelMeta(), abi.CfKernelMeta()
+        abi.call("chip_cfcache_metadata", self.handle, int(index), ctypes.byref(im), ctypes.byref(wt))
+        for m in (im, wt):
+            out.append(dict(shape=(m.shape_x, m.shape_y), sampling=m.sampling, x_support=m.x_support,
+                            y_support=m.y_support, uv_reference_value=tuple(m.uv_reference_value),
+                            uv_reference_pixel=tuple(m.uv_reference_pixel), uv_increment=tuple(m.uv_increment),
+                            uv_pc=tuple(m.uv_pc), diameter_m=m.diameter_m,
+                            conjugate_frequency_hz=m.conjugate_frequency_hz, w_increment=m.w_increment,
+                            conjugate_polarization=m.conjugate_polarization, polarization=m.polarization,
+                            rotationally_symmetric=bool(m.rotationally_symmetric),
+                            telescope_name=m.telescope_name.decode(), band_name=m.band_name.decode()))
+        return tuple(out)
+
+    def _key_or_none(self, idx):
+        return None if idx < 0 else self.keys()[idx]
+
+    def nearest_key(self, frequency_hz, w_value_lambda, mueller_element, parallactic_angle_deg):
+        r = ctypes.c_int64()
+        abi.call("chip_cfcache_nearest_key", self.handle, float(frequency_hz), float(w_value_lambda),
+                 int(mueller_element), float(parallactic_angle_deg), ctypes.byref(r))
+        return self._key_or_none(r.value)
+
+    def select_key_for_sample(self, sample_frequency_hz, image_reference_frequency_hz, w_lambda, mueller_element,
+                              parallactic_angle_deg, conjugate_beams):
+        r = ctypes.c_int64()
+        abi.call("chip_cfcache_select_key", self.handle, float(sample_frequency_hz),
+                 float(image_reference_frequency_hz), float(w_lambda), int(mueller_element),
+                 float(parallactic_angle_deg), int(bool(conjugate_beams)), 0, ctypes.byref(r))
+        return self._key_or_none(r.value)
+
+    def select_key_for_prediction_sample(self, sample_frequency_hz, w_lambda, mueller_element,
+                                         parallactic_angle_deg):
+        r = ctypes.c_int64()
+        abi.call("chip_cfcache_select_key", self.handle, float(sample_frequency_hz), 0.0, float(w_lambda),
+                 int(mueller_element), float(parallactic_angle_deg), 0, 1, ctypes.byref(r))
+        return self._key_or_none(r.value)
+
+    def load(self, key):
+        """(imaging, weight) complex64 pixels indexed (x, y) of a key or cell index."""
+        idx = key if isinstance(key, (int, np.integer)) else self.index(key)
+        im, wt = self.metadata(idx)
+        a = np.empty(im["shape"], np.complex64)
+        b = np.empty(wt["shape"], np.complex64)
+        abi.call("chip_cfcache_load", self.handle, int(idx), a.ctypes.data_as(ctypes.c_void_p),
+                 b.ctypes.data_as(ctypes.c_void_p))
+        return a, b
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            abi.load().chip_cfcache_close(self._h)
             self._h = None
 
     def __del__(self):

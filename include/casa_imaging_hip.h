@@ -39,6 +39,7 @@ extern "C" {
 #define CHIP_ERR_NO_SAMPLES 2
 #define CHIP_ERR_NORMALIZATION 3
 #define CHIP_ERR_DEVICE 4
+#define CHIP_ERR_CF_CACHE 5 /* ImagingError::ConvolutionFunctionCache (error.rs): unreadable or inconsistent CF cache */
 
 /* Grid modes of accumulate_w_project_grid_metal_samples (lib.rs:71688-71690). */
 #define CHIP_MODE_PSF 0      /* PSF grid only */
@@ -74,6 +75,7 @@ typedef struct chip_ctx chip_ctx;
 typedef struct chip_gridder chip_gridder;
 typedef struct chip_vset chip_vset;
 typedef struct chip_ingest chip_ingest;
+typedef struct chip_cfcache chip_cfcache;
 
 /* ImageGeometry (types.rs:885-925) plus the gridder's padding rule and tap-table
  * oversampling (the reference fixes 100, gridder.rs:407). */
@@ -284,6 +286,84 @@ int chip_aw_plan(chip_gridder *g, const chip_samples *s, int32_t *plan_dev, floa
  * 10,000 voltage values and the table's maximum radius (arcmin * GHz,
  * casa_airy_max_radius_arcmin_ghz, lib.rs:57592-57605).  Host only. */
 int chip_airy_table(double dish_m, double blockage_m, float *values, double *max_radius, char **err);
+
+/* ---- CASA AWProject CF caches (CFS_*.im / WTCFS_*.im PagedImages) -------- */
+/* Scientific key of one cache cell (AwConvolutionFunctionKey,
+ * awprojection.rs:36-62): spectral reference frequency, miscinfo WValue,
+ * MuellerElement and ParallacticAngle. */
+typedef struct {
+    double frequency_hz;
+    double w_value_lambda;
+    int32_t mueller_element;
+    double parallactic_angle_deg;
+} chip_cf_key;
+
+/* AwConvolutionFunctionKernelMetadata (awprojection.rs:97-128).  The UU/VV
+ * affine coordinate is world = crval + cdelt * PC * (pixel - crpix) with
+ * uv_pc row-major; names longer than 63 bytes are truncated. */
+typedef struct {
+    uint64_t shape_x, shape_y;
+    uint64_t sampling, x_support, y_support;
+    double uv_reference_value[2], uv_reference_pixel[2], uv_increment[2], uv_pc[4];
+    double diameter_m, conjugate_frequency_hz, w_increment;
+    int32_t conjugate_polarization, polarization, rotationally_symmetric;
+    char telescope_name[64], band_name[64];
+} chip_cf_kernel_meta;
+
+/* AwConvolutionFunctionInventory and the identity's fingerprint
+ * (awprojection.rs:165-190, :1820-1972). */
+typedef struct {
+    uint64_t paired_cells;
+    uint32_t n_freq, n_w, n_mueller, n_pa;
+    double w_increment;
+    uint64_t metadata_fingerprint;
+} chip_cfcache_info;
+
+/* AwConvolutionFunctionCache::open (awprojection.rs:920-1045): index and
+ * validate every CFS_*.im / WTCFS_*.im casacore image in `path` (pairs,
+ * metadata, a complete frequency x W x Mueller x PA product, W values
+ * k^2 / WIncr).  Host only; no device needed.  Errors: CHIP_ERR_CF_CACHE. */
+int chip_cfcache_open(const char *path, chip_cfcache **out, char **err);
+void chip_cfcache_close(chip_cfcache *c);
+/* The inventory: sorted unique frequencies (n_freq), W values (n_w), Mueller
+ * elements (n_mueller) and PAs (n_pa); any array may be NULL. */
+int chip_cfcache_info_get(const chip_cfcache *c, chip_cfcache_info *info, double *freqs, double *w_values,
+                          int32_t *muellers, double *pas, char **err);
+/* keys() (awprojection.rs:1058-1061): paired_cells keys in key order
+ * (frequency, W, Mueller, PA bit patterns).  Cell index k below is keys[k]. */
+int chip_cfcache_keys(const chip_cfcache *c, chip_cf_key *keys, char **err);
+/* metadata(key) (:1063-1069): imaging and weight metadata of cell `index`. */
+int chip_cfcache_metadata(const chip_cfcache *c, uint64_t index, chip_cf_kernel_meta *imaging,
+                          chip_cf_kernel_meta *weight, char **err);
+/* nearest_key (:1071-1104).  *index = -1 where the reference returns None. */
+int chip_cfcache_nearest_key(const chip_cfcache *c, double frequency_hz, double w_value_lambda, int32_t mueller,
+                             double parallactic_angle_deg, int64_t *index, char **err);
+/* select_key_for_sample (prediction = 0, :1106-1172) or
+ * select_key_for_prediction_sample (prediction != 0; reference_frequency_hz
+ * and conjugate_beams unused, :1174-1210).  *index = -1 for None. */
+int chip_cfcache_select_key(const chip_cfcache *c, double sample_frequency_hz, double reference_frequency_hz,
+                            double w_lambda, int32_t mueller, double parallactic_angle_deg, int conjugate_beams,
+                            int prediction, int64_t *index, char **err);
+/* load(key) (:1380-1412): host pixels of cell `index`, complex64 (x, y) with
+ * y fastest: imaging shape_x * shape_y values, weight likewise; either may be
+ * NULL.  Non-finite pixels are a CHIP_ERR_CF_CACHE error. */
+int chip_cfcache_load(const chip_cfcache *c, uint64_t index, float *imaging, float *weight, char **err);
+/* Every imaging cell of the cache, in key order, as the gridder's AW atlas
+ * (chip_gridder_attach_aw with the cache's sampling and WIncr): cell k of the
+ * atlas is cache cell k.  All cells must share one sampling. */
+int chip_gridder_attach_aw_cache(chip_gridder *g, const chip_cfcache *c, uint32_t n_pointings,
+                                 const double *phase_gradients, char **err);
+/* Per-sample cell of a rows x channels input for an atlas from
+ * chip_gridder_attach_aw_cache, with chip_cfcache_select_key's semantics on
+ * the device: sample frequency chan_freq_hz[chan] (host array, or NULL for
+ * lambda_scale * c), w = uvw_m[2] * lambda_scale, PA pa_deg_dev[row] (device,
+ * or NULL for pa_deg), one Mueller element for the call.  cf_cell_dev: uint32
+ * per sample for chip_samples.cf_cell_dev, 0xffffffff where no cell is
+ * selected (the planner rejects it). */
+int chip_cfcache_select_cells(chip_gridder *g, const chip_samples *s, const double *chan_freq_hz,
+                              double reference_frequency_hz, int32_t mueller, const double *pa_deg_dev,
+                              double pa_deg, int conjugate_beams, int prediction, uint32_t *cf_cell_dev,
+                              char **err);
 
 /* ---- phase-centre rotation of a row stream ------------------------------- */
 /* Host FP64: uvrot = gridft_uvw_rotation_matrix(source_dir, target_dir)

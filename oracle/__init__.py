@@ -1183,3 +1183,121 @@ def hogbom(residual, psf, model, gain, budget, threshold=0.0, cycle_threshold=0.
     lib().co_hogbom(planes, nx, ny, _ptr(r), k.shape[1], k.shape[2], _ptr(k), _ptr(m), _ptr(mk), float(gain),
                     float(threshold), _ptr(ct), _ptr(ns), int(budget), _ptr(upd), _ptr(stop))
     return r, m, upd, stop
+
+
+# --------------------------------------------------------------------------
+# CASA CF-cache key selection (awprojection.rs:1068-1340), restated on key
+# tuples (frequency_hz, w_value_lambda, mueller_element, parallactic_angle_deg).
+# "Stable key" order compares the IEEE-754 bit patterns (awprojection.rs:66-95).
+
+def _bits(x: float) -> int:
+    return int(np.array(float(x), np.float64).view(np.uint64))
+
+
+def cf_stable_key(key):
+    f, w, m, p = key
+    return (_bits(f), _bits(w), int(m), _bits(p))
+
+
+def cf_circular_degrees(v: float) -> float:
+    """(v + 180).rem_euclid(360) - 180 (awprojection.rs:2046-2048)."""
+    r = float(np.fmod(v + 180.0, 360.0))
+    if r < 0.0:
+        r += 360.0
+    return r - 180.0
+
+
+def cf_sorted_unique(values):
+    """sorted_f64_bits (awprojection.rs:1974-1978): unique bit patterns in value order."""
+    uniq = {_bits(v): float(v) for v in values}
+    return sorted(uniq.values(), key=lambda v: (v, not np.signbit(v)))
+
+
+def cf_nearest_key(keys, frequency_hz, w_value_lambda, mueller_element, parallactic_angle_deg):
+    """nearest_key (awprojection.rs:1071-1104): Mueller exact, minimum of the
+    relative frequency + relative W + |dPA|/360 distance, ties by stable key."""
+    if not (np.isfinite(frequency_hz) and frequency_hz > 0 and np.isfinite(w_value_lambda)
+            and np.isfinite(parallactic_angle_deg)):
+        return None
+    fs, ws = max(abs(frequency_hz), 1.0), max(abs(w_value_lambda), 1.0)
+    best = None
+    for k in keys:
+        if k[2] != mueller_element:
+            continue
+        d = (abs((k[0] - frequency_hz) / fs) + abs((k[1] - w_value_lambda) / ws)
+             + abs(cf_circular_degrees(k[3] - parallactic_angle_deg)) / 360.0)
+        cand = (d, cf_stable_key(k))
+        if best is None or cand < best[0]:
+            best = (cand, k)
+    return None if best is None else best[1]
+
+
+def _cf_requested(sample_frequency_hz, image_reference_frequency_hz, conjugate_beams):
+    if conjugate_beams:
+        rad = 2.0 * image_reference_frequency_hz * image_reference_frequency_hz - sample_frequency_hz ** 2
+        if not (np.isfinite(rad) and rad > 0.0):
+            return None
+        return float(np.sqrt(rad))
+    return sample_frequency_hz
+
+
+def _cf_w_index(w_increment, n_w, w_lambda):
+    if n_w <= 1 or w_increment == 0.0:
+        return 0
+    r = float(np.sqrt(w_increment * abs(w_lambda)))
+    f = float(np.floor(r))
+    r = f + 1.0 if r - f >= 0.5 else f  # f64::round, half away from zero (r >= 0; r - f is exact)
+    return int(min(max(r, 0.0), float(n_w - 1)))
+
+
+def cf_select_scanned(keys, w_increment, requested_frequency_hz, w_lambda, mueller, pa):
+    """select_key_for_sample_scanned (awprojection.rs:1212-1258): candidates of
+    the Mueller element, the W plane round(sqrt(WIncr |w|)) among their sorted W
+    values, then the lexicographic minimum of (|df|, |circular dPA|, stable key)."""
+    cands = [k for k in keys if k[2] == mueller]
+    if not cands:
+        return None
+    ws = cf_sorted_unique([k[1] for k in cands])
+    sel_w = ws[_cf_w_index(w_increment, len(ws), w_lambda)]
+    pool = [k for k in cands if _bits(k[1]) == _bits(sel_w)]
+    return min(pool, key=lambda k: (abs(k[0] - requested_frequency_hz),
+                                    abs(cf_circular_degrees(k[3] - pa)), cf_stable_key(k)))
+
+
+def cf_select_direct(keys, w_increment, requested_frequency_hz, w_lambda, mueller, pa):
+    """select_key_for_sample_direct (awprojection.rs:1260-1322): per-axis
+    nearest values from the inventory, then an exact key lookup."""
+    if mueller not in {k[2] for k in keys}:
+        return None
+    ws = cf_sorted_unique([k[1] for k in keys])
+    fs = cf_sorted_unique([k[0] for k in keys])
+    ps = cf_sorted_unique([k[3] for k in keys])
+    w = ws[_cf_w_index(w_increment, len(ws), w_lambda)]
+    f = min(fs, key=lambda v: (abs(v - requested_frequency_hz), _bits(v)))
+    p = min(ps, key=lambda v: (abs(cf_circular_degrees(v - pa)), _bits(v)))
+    want = cf_stable_key((f, w, mueller, p))
+    for k in keys:
+        if cf_stable_key(k) == want:
+            return k
+    return None
+
+
+def cf_select_key(keys, w_increment, sample_frequency_hz, image_reference_frequency_hz, w_lambda, mueller_element,
+                  parallactic_angle_deg, conjugate_beams=False, prediction=False, direct=False):
+    """select_key_for_sample (prediction False, awprojection.rs:1106-1172) and
+    select_key_for_prediction_sample (prediction True, :1174-1210)."""
+    if not (np.isfinite(sample_frequency_hz) and sample_frequency_hz > 0 and np.isfinite(w_lambda)
+            and np.isfinite(parallactic_angle_deg)):
+        return None
+    if prediction:
+        req = sample_frequency_hz
+        m = 15 - mueller_element if w_lambda > 0.0 else mueller_element
+    else:
+        if not (np.isfinite(image_reference_frequency_hz) and image_reference_frequency_hz > 0):
+            return None
+        req = _cf_requested(sample_frequency_hz, image_reference_frequency_hz, conjugate_beams)
+        if req is None:
+            return None
+        m = mueller_element if w_lambda > 0.0 else 15 - mueller_element
+    pick = cf_select_direct if direct else cf_select_scanned
+    return pick(keys, w_increment, req, w_lambda, m, parallactic_angle_deg)
