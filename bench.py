@@ -613,6 +613,8 @@ def main():
     ap.add_argument("--oracle-channels", type=int, default=None,
                     help="channels (planes) checked against the CPU FP64 oracle (default 2, w4096 1; 0: none)")
     ap.add_argument("--seed", type=int, default=2026)
+    ap.add_argument("--wave-groups", type=int, default=0, choices=(0, 1, 2),
+                    help="groups per wavefront of the binned kernel (0: two where the halo allows)")
     ap.add_argument("--schedule", default="", metavar="TILE,ORDER,THREADS,GROUPS",
                     help="fix the binned kernel's schedule instead of autotuning it (profiling runs)")
     ap.add_argument("--compare-hipfft", type=int, default=1,
@@ -715,6 +717,7 @@ def main():
             kernel_ms.append(vs.stats()["last_kernel_ms"])
         return r32
 
+    wave_groups = vs.set_wave_groups(args.wave_groups)
     if args.schedule:
         # a fixed binned-kernel schedule (profiling runs: no autotuner launches)
         tile, order, threads, gpi = (int(x) for x in args.schedule.split(","))
@@ -743,6 +746,7 @@ def main():
     log(f"batched FP32 major cycle: {t32 * 1e3:.2f} ms/step, {rate32:.1f} Mvis/s; steps "
         f"{min(step_s) * 1e3:.2f}..{max(step_s) * 1e3:.2f} ms; binned kernel {np.mean(kernel_ms):.3f} ms")
     sched = vs.schedule()
+    wave_groups = vs.set_wave_groups(args.wave_groups)  # the value in use after the autotuner
     log(f"binned schedule: {sched['tile']}^2 uv tiles, groups by {('anchor', 'plan')[sched['order']]}, "
         f"{sched['threads']} threads x {sched['groups_per_item']} groups per item ("
         + ("fixed by --schedule" if args.schedule else
@@ -762,6 +766,7 @@ def main():
         transform_shape=[info["columns"], info["column_threads"], info["rows_per_group"]],
         transform_fwd_inv_ms=info["transform_ms"],
         binned_schedule={"tile": sched["tile"], "order": ("anchor", "plan")[sched["order"]],
+                         "groups_per_wavefront": wave_groups,
                          "threads": sched["threads"], "groups_per_item": sched["groups_per_item"],
                          "candidates_timed": len(sched["timed"])})
     pass_ms = None

@@ -23,6 +23,7 @@ void ingest_download(chip_ingest &ing, double *uvw, float *vis, float *weight, f
 void vset_cycle_strategy(chip_vset &vs, int strategy);
 void vset_cycle_graph(chip_vset &vs, int enable);
 void vset_schedule(chip_vset &vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t gpi);
+int binned_wave_groups(const chip_vset &vs);
 uint64_t vset_cycle_replays(const chip_vset &vs);
 void vset_cycle_passes(chip_vset &vs, int enable, double *pass_ms);
 void vset_item_order(chip_vset &vs, uint32_t order);
@@ -687,6 +688,15 @@ int chip_vset_major_cycle(chip_vset *vs, const float *model_dev, const double *n
         if (!vs) fail(CHIP_ERR_INVALID, "NULL vset");
         set_device(vs->g->ctx);
         vset_major_cycle(*vs, model_dev, nsum, psf_peak, residual_out_dev);
+    });
+}
+
+int chip_vset_wave_groups(chip_vset *vs, uint32_t groups, uint32_t *in_use, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        if (groups > 2) fail(CHIP_ERR_INVALID, "groups per wavefront must be 0 (automatic), 1 or 2");
+        vs->wave_groups = groups;
+        if (in_use) *in_use = static_cast<uint32_t>(binned_wave_groups(*vs));
     });
 }
 

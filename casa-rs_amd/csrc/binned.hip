@@ -25,6 +25,7 @@
 // when w > 0; grid += value*tap; degrid += conj(cwt)*grid.  A group's value is
 // the sum over its samples (sum w_i V_i; residual: sum w_i V_i - p sum w_i).
 #include <climits>
+#include <type_traits>
 
 #include "device.hpp"
 
@@ -101,7 +102,23 @@ __device__ __forceinline__ void div_conj_rcp(float &re, float &im, float nr, flo
     im = (b * c - a * d) * inv;
 }
 
-template <int MODE, bool WP, int KS>
+// Register slots per lane for a footprint side M = 2 halo + 1 with 64 lanes:
+// ceil(M / floor(64 / M)) rounded up to an instantiated count (1, 2, 4, 7),
+// 0 (LDS-atomic chunks) beyond 7 (halo > 10).
+__host__ __device__ __forceinline__ int slots_for_halo(int halo) {
+    const int m = 2 * halo + 1, p = 64 / m;
+    if (p == 0) return 0;
+    const int need = (m + p - 1) / p;
+    return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 7 ? 7 : 0;
+}
+
+// Two groups per wavefront (GPW = 2): the lanes of each half-wavefront own
+// the register slots of one group's footprint, so the per-group work that
+// does not scale with the footprint (group record broadcast, footprint,
+// slot bookkeeping, the degrid reduction) serves two groups at once.  Each
+// half walks its own contiguous run of the wavefront's groups.  Used for
+// halos up to 4 (a 9 x 9 footprint in 27 of 32 lanes with 3 slots each).
+template <int MODE, bool WP, int KS, int GPW>
 __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const WorkItem *__restrict__ items,
                                                 const uint4 *__restrict__ grec,
                                                 const float4 *__restrict__ gsum,
@@ -137,16 +154,20 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // gridding modes read each group's precomputed sums instead of its samples
     constexpr bool kSums = MODE == kModePsf || MODE == kModeDirty || MODE == kModeResidual;
 
+    static_assert(GPW == 1 || (GPW == 2 && KS > 0), "two groups per wavefront need register slots");
+    constexpr int LW = 64 / GPW;  // lanes per group
     const int tid = threadIdx.x;
     const int nth = blockDim.x;
     const int lane = tid & 63;
+    const int hl = lane & (LW - 1);             // lane within its group's lanes
+    const bool upper = GPW == 2 && lane >= 32;  // the half that runs the second group run
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int nwaves = nth >> 6;
     const uint32_t tile_x = it.tile / gp.tiles_y;
     const uint32_t tile_y = it.tile - tile_x * gp.tiles_y;
     const int x0 = static_cast<int>(tile_x * gp.tile) - static_cast<int>(gp.halo);
     const int y0 = static_cast<int>(tile_y * gp.tile) - static_cast<int>(gp.halo);
-    const uint64_t chan_off = static_cast<uint64_t>(it.chan) * gp.cstride;
+    const uint64_t chan_off = static_cast<uint64_t>(it.chan & 0xffffffu) * gp.cstride;  // top 8 bits: item halo
 
     if (!WP)
         for (int i = tid; i < static_cast<int>(gp.tap_rows) * kTaps; i += nth) taps[i] = gp.taps[i];
@@ -189,22 +210,31 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // gridding radio-telescope data on GPUs", ICS 2012).  A lane's y residue
     // is fixed, so per group the y offset is computed once and the x offsets
     // of its slots follow by steps of P.
-    constexpr int NS = KS > 0 ? KS : 1;
+    // The slot layout follows the work item's own largest half-support (W /
+    // AW: the top 8 bits of its channel word; groups of a bin are ordered by
+    // support class, so an item mostly holds one class): the group loop is
+    // compiled for every slot count up to the launch's, and the item runs
+    // the one its footprints need.
+    const int item_halo =
+        WP ? min(static_cast<int>(it.chan >> 24), static_cast<int>(gp.halo)) : static_cast<int>(gp.halo);
+    auto run = [&](auto ks_tag) {
+    constexpr int KSV = decltype(ks_tag)::value;
+    constexpr int NS = KSV > 0 ? KSV : 1;
     // Many slots (large supports): incremental slot offsets and fused
     // multiply-add chains issue fewer instructions; with one or two slots the
     // plain forms measured faster.
     constexpr bool kChains = NS > 2;
-    const int Mside = 2 * static_cast<int>(gp.halo) + 1;
+    const int Mside = 2 * item_halo + 1;
     const float invM = 1.0f / static_cast<float>(Mside);
-    const int Pl = 64 / Mside;
-    const int lane_pg = static_cast<int>((static_cast<float>(lane) + 0.5f) * invM);  // lane / M
-    const int lane_py = lane - lane_pg * Mside;
+    const int Pl = LW / Mside;
+    const int lane_pg = static_cast<int>((static_cast<float>(hl) + 0.5f) * invM);  // hl / M
+    const int lane_py = hl - lane_pg * Mside;
     uint32_t slot_mask = 0u;  // bit k: slot k owns an x residue
     int cell_k[NS];
     float accr[NS], acci[NS], pacr[NS], paci[NS], mre[NS], mim[NS];
 #pragma unroll
     for (int k = 0; k < NS; ++k) {
-        if (KS > 0 && lane_pg < Pl && lane_pg + k * Pl < Mside) slot_mask |= 1u << k;
+        if (KSV > 0 && lane_pg < Pl && lane_pg + k * Pl < Mside) slot_mask |= 1u << k;
         cell_k[k] = -1;
         accr[k] = acci[k] = pacr[k] = paci[k] = mre[k] = mim[k] = 0.0f;
     }
@@ -225,6 +255,15 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     const uint32_t per = (it.count + nwaves - 1) / nwaves;
     const uint32_t g_begin = it.start + static_cast<uint32_t>(wave) * per;
     const uint32_t g_end = min(it.start + it.count, g_begin + per);
+    // this lane's run of groups [h_begin, h_end) and the loop trip count
+    uint32_t h_begin = g_begin, h_end = g_end;
+    uint32_t n_iter = g_end > g_begin ? g_end - g_begin : 0u;
+    if (GPW == 2) {
+        const uint32_t n0 = (n_iter + 1) / 2;
+        h_begin = g_begin + (upper ? n0 : 0u);
+        h_end = upper ? g_end : g_begin + n0;
+        n_iter = n0;
+    }
 
     // Footprint of a group from its packed plan (wavefront-uniform).
     struct Foot {
@@ -274,8 +313,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         float2 tap[NS];
         int cell[NS];
     };
-    auto load_group = [&](uint32_t s0, uint32_t s1, const Foot &f, GroupData &d) {
-        const uint32_t j = s0 + lane;
+    auto load_group = [&](uint32_t s0, uint32_t s1, const Foot &f, GroupData &d, bool valid) {
+        const uint32_t j = s0 + hl;
         d.v = make_float2(0.0f, 0.0f);
         d.wj = 0.0f;
         d.fj = 0u;
@@ -320,7 +359,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                     toff -= wrap_t;
                 }
             }
-            if (!((slot_mask >> k) & 1u) || !in_y || i >= f.side) continue;
+            if (!((slot_mask >> k) & 1u) || !in_y || i >= f.side || !valid) continue;
             if (kChains) {
                 d.cell[k] = cell;
                 d.tap[k] = WP ? *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(f.ktap) + toff)
@@ -345,7 +384,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             w = gs.x;
             vr = gs.y;
             vi = gs.z;
-            if (MODE == kModeResidual && w == 0.0f) return;
+            if (GPW == 1 && MODE == kModeResidual && w == 0.0f) return;
         } else if (MODE != kModeDegrid) {
             auto add = [&](uint32_t fj, float wj, float2 vj) {
                 if (MODE == kModeAdjoint) {
@@ -365,12 +404,15 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                     }
                 }
             };
-            if (static_cast<uint32_t>(lane) < count) add(d.fj, d.wj, d.v);
-            for (uint32_t j = s0 + 64 + lane; j < s1; j += 64) {
+            if (static_cast<uint32_t>(hl) < count) add(d.fj, d.wj, d.v);
+            for (uint32_t j = s0 + LW + hl; j < s1; j += LW) {
                 if (MODE == kModeAdjoint) add(0u, 0.0f, values[index[j]]);
                 else add(plan[j].y, wt[j], vis[j]);
             }
-            if (count > 1) {
+            if (GPW == 2) {
+                half_sum_f2(vr, vi, upper);
+                if (MODE != kModeAdjoint) w = half_sum_f(w, upper);
+            } else if (count > 1) {
                 wave_sum_f2(vr, vi);
                 if (MODE != kModeAdjoint) w = wave_sum_f(w);
             } else {
@@ -378,12 +420,12 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 vi = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(vi)));
                 w = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(w)));
             }
-            if (MODE == kModeResidual && w == 0.0f) return;
+            if (GPW == 1 && MODE == kModeResidual && w == 0.0f) return;
         }
         auto write_degrid = [&](float pr, float pi) {
             // every sample of the group has the same prediction
-            if (static_cast<uint32_t>(lane) < count) vis_out[d.idx] = make_float2(pr, pi);
-            for (uint32_t j = s0 + 64 + lane; j < s1; j += 64) vis_out[index[j]] = make_float2(pr, pi);
+            if (static_cast<uint32_t>(hl) < count) vis_out[d.idx] = make_float2(pr, pi);
+            for (uint32_t j = s0 + LW + hl; j < s1; j += LW) vis_out[index[j]] = make_float2(pr, pi);
         };
         const int side = f.side, T = side * side, bx = f.bx, by = f.by;
         const float2 *ktap = f.ktap;
@@ -392,7 +434,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         const float2 awn = f.awn;
         (void)pk;
 
-        if (KS == 0) {
+        if (KSV == 0) {
             // Supports above the register slots: taps straight to LDS, in
             // chunks of 64 * kBig taps whose loads every lane issues together
             // and keeps in registers for both the degrid and the grid pass
@@ -501,7 +543,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             }
         }
         if (kModel) {
-            wave_sum_f2(pr, pi);
+            if (GPW == 2) half_sum_f2(pr, pi, upper);
+            else wave_sum_f2(pr, pi);
             if (WP && gp.aw_norm) {
                 if (kChains) div_conj_rcp(pr, pi, awn.x, awn.y);
                 else div_conj<float>(pr, pi, awn.x, awn.y);
@@ -539,50 +582,58 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // scalar arithmetic and no scalar load waits behind the LDS traffic of the
     // group being processed.  Software pipeline: while group g is processed,
     // the taps and samples of group g + 1 are in flight.
-    auto rec_load = [&](uint32_t base) {
-        const uint32_t g = base + static_cast<uint32_t>(lane);
-        return g < g_end ? grec[g] : make_uint4(0u, 0u, 0u, 0u);
+    // GPW = 2: each half-wavefront holds a window of 32 records of its own
+    // run; both halves advance one group per iteration, so their windows
+    // move together, and a record reaches its half's lanes by two readlanes
+    // and a select.
+    auto rec_load = [&](uint32_t wbase) {
+        const uint32_t g = h_begin + wbase + static_cast<uint32_t>(hl);
+        return g < h_end ? grec[g] : make_uint4(0u, 0u, 0u, 0u);
     };
-    auto sum_load = [&](uint32_t base) {
-        const uint32_t g = base + static_cast<uint32_t>(lane);
-        return (kSums && g < g_end) ? gsum[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto sum_load = [&](uint32_t wbase) {
+        const uint32_t g = h_begin + wbase + static_cast<uint32_t>(hl);
+        return (kSums && g < h_end) ? gsum[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    };
+    auto bcast = [&](uint32_t x, uint32_t k) {
+        const int kk = static_cast<int>(k);
+        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), kk));
+        if (GPW == 1) return lo;
+        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), kk + 32));
+        return upper ? hi : lo;
     };
     auto sum_at = [&](const float4 &sb, uint32_t k) {
-        const int kk = static_cast<int>(k);
-        return make_float4(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(sb.x), kk)),
-                           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sb.y), kk)),
-                           __int_as_float(__builtin_amdgcn_readlane(__float_as_int(sb.z), kk)), 0.0f);
+        return make_float4(__uint_as_float(bcast(__float_as_uint(sb.x), k)),
+                           __uint_as_float(bcast(__float_as_uint(sb.y), k)),
+                           __uint_as_float(bcast(__float_as_uint(sb.z), k)), 0.0f);
     };
     auto rec_at = [&](const uint4 &rb, uint32_t k) {
-        const int kk = static_cast<int>(k);
-        return make_uint4(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.x), kk)),
-                          static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.y), kk)),
-                          static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.z), kk)),
-                          static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(rb.w), kk)));
+        return make_uint4(bcast(rb.x, k), bcast(rb.y, k), bcast(rb.z, k), bcast(rb.w, k));
     };
-    if (g_begin < g_end) {
-        uint32_t base = g_begin;
-        uint4 rb = rec_load(base);
-        float4 sb = sum_load(base);
+    if (n_iter > 0) {
+        uint32_t wbase = 0;
+        uint4 rb = rec_load(wbase);
+        float4 sb = sum_load(wbase);
         const uint4 c0 = rec_at(rb, 0);
         uint32_t c_s0 = c0.x, c_s1 = c0.y;
         uint2 c_pk = make_uint2(c0.z, c0.w);
         float4 c_g = kSums ? sum_at(sb, 0) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        bool c_valid = h_begin < h_end;
         Foot c_f = footprint(c_pk);
         GroupData c_d;
-        load_group(c_s0, c_s1, c_f, c_d);
-        for (uint32_t grp = g_begin; grp < g_end; ++grp) {
+        load_group(c_s0, c_s1, c_f, c_d, c_valid);
+        for (uint32_t t = 0; t < n_iter; ++t) {
             Foot n_f = c_f;
             GroupData n_d = c_d;
             uint32_t n_s0 = 0, n_s1 = 0;
             uint2 n_pk = make_uint2(0u, 0u);
             float4 n_g = c_g;
-            if (grp + 1 < g_end) {
-                uint32_t k = grp + 1 - base;
-                if (k == 64) {
-                    base += 64;
-                    rb = rec_load(base);
-                    sb = sum_load(base);
+            bool n_valid = false;
+            if (t + 1 < n_iter) {
+                uint32_t k = t + 1 - wbase;
+                if (k == static_cast<uint32_t>(LW)) {
+                    wbase += LW;
+                    rb = rec_load(wbase);
+                    sb = sum_load(wbase);
                     k = 0;
                 }
                 const uint4 nr = rec_at(rb, k);
@@ -590,8 +641,9 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 n_s1 = nr.y;
                 n_pk = make_uint2(nr.z, nr.w);
                 if (kSums) n_g = sum_at(sb, k);
+                n_valid = h_begin + t + 1 < h_end;
                 n_f = footprint(n_pk);
-                load_group(n_s0, n_s1, n_f, n_d);
+                load_group(n_s0, n_s1, n_f, n_d, n_valid);
             }
             process(c_s0, c_s1, c_pk, c_f, c_d, c_g);
             c_s0 = n_s0;
@@ -600,10 +652,43 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             c_f = n_f;
             c_d = n_d;
             c_g = n_g;
+            c_valid = n_valid;
         }
+        (void)c_valid;
     }
 #pragma unroll
     for (int k = 0; k < NS; ++k) spill(k);
+    };
+    if constexpr (GPW == 2 || !WP || KS == 1) {
+        run(std::integral_constant<int, KS>{});
+    } else {
+        const int need = slots_for_halo(item_halo);
+        if constexpr (KS == 0) {
+            switch (need) {
+                case 1: run(std::integral_constant<int, 1>{}); break;
+                case 2: run(std::integral_constant<int, 2>{}); break;
+                case 4: run(std::integral_constant<int, 4>{}); break;
+                case 7: run(std::integral_constant<int, 7>{}); break;
+                default: run(std::integral_constant<int, 0>{}); break;
+            }
+        } else if constexpr (KS == 7) {
+            switch (need) {
+                case 1: run(std::integral_constant<int, 1>{}); break;
+                case 2: run(std::integral_constant<int, 2>{}); break;
+                case 4: run(std::integral_constant<int, 4>{}); break;
+                default: run(std::integral_constant<int, 7>{}); break;
+            }
+        } else if constexpr (KS == 4) {
+            switch (need) {
+                case 1: run(std::integral_constant<int, 1>{}); break;
+                case 2: run(std::integral_constant<int, 2>{}); break;
+                default: run(std::integral_constant<int, 4>{}); break;
+            }
+        } else {
+            if (need == 1) run(std::integral_constant<int, 1>{});
+            else run(std::integral_constant<int, KS>{});
+        }
+    }
     if (MODE == kModeDegrid) return;
     __syncthreads();
 
@@ -650,17 +735,17 @@ static void check_wk_chunk(const chip_vset &vs) {
         fail(CHIP_ERR_INVALID, "the W kernel residency changed after the visibility set was created");
 }
 
-template <int MODE, bool WP, int KS>
+template <int MODE, bool WP, int KS, int GPW = 1>
 static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *values, const float2 *model,
                             float *psf, float *acc, float2 *vis_out, size_t lds, hipStream_t st, bool timed) {
     if (lds > 64 * 1024)
-        CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP, KS>),
+        CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP, KS, GPW>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const float4 *gsum = MODE == kModeResidual                         ? vs.gsum_r.as<float4>()
                          : (MODE == kModePsf || MODE == kModeDirty) ? vs.gsum_d.as<float4>()
                                                                      : nullptr;
     auto launch = [&](const GridParams &p, uint64_t first, uint64_t count) {
-        hipLaunchKernelGGL((k_binned<MODE, WP, KS>), dim3(static_cast<unsigned>(count)), dim3(vs.threads),
+        hipLaunchKernelGGL((k_binned<MODE, WP, KS, GPW>), dim3(static_cast<unsigned>(count)), dim3(vs.threads),
                            lds, st, p, vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(), gsum,
                            vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
                            vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
@@ -703,16 +788,25 @@ static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *v
 // Register slots per lane for a tile halo: the x residues per lane,
 // ceil(M / floor(64 / M)) with M = 2 halo + 1, rounded up to an instantiated
 // count; 0 (LDS-atomic chunks) beyond 7 (halo > 10).
-static int register_slots(uint32_t halo) {
-    const uint32_t m = 2 * halo + 1, p = 64 / m;
+static int register_slots(uint32_t halo) { return slots_for_halo(static_cast<int>(halo)); }
+
+// Register slots per lane with two groups per wavefront (32 lanes per
+// group): 1 - 3 for halos up to 4, 0 (not offered) above.
+static int register_slots_half(uint32_t halo) {
+    const uint32_t m = 2 * halo + 1, p = 32 / m;
     if (p == 0) return 0;
-    const uint32_t need = (m + p - 1) / p;  // x residues per lane
-    if (need <= 1) return 1;
-    if (need <= 2) return 2;
-    if (need <= 4) return 4;
-    if (need <= 7) return 7;
-    return 0;
+    const uint32_t need = (m + p - 1) / p;
+    return need <= 3 ? static_cast<int>(need) : 0;
 }
+
+// Groups per wavefront in use: 1, or 2 where the halo allows it and it was
+// asked for (wave_groups 2) or the autotuner chose it (wave_groups 0).
+int binned_wave_groups(const chip_vset &vs) {
+    if (register_slots_half(vs.halo) == 0 || vs.wave_groups == 1) return 1;
+    return (vs.wave_groups == 2 || vs.tuned_wave_groups == 2) ? 2 : 1;
+}
+
+bool binned_two_groups_allowed(const chip_vset &vs) { return register_slots_half(vs.halo) > 0; }
 
 template <int MODE, bool WP>
 static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
@@ -724,6 +818,19 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
                                    std::to_string(gp.halo) + ")");
     if (vs.n_items == 0) return;
     const int ks = register_slots(gp.halo);
+    const int ks2 = binned_wave_groups(vs) == 2 ? register_slots_half(gp.halo) : 0;
+    if (ks2 > 0) {  // two groups per wavefront
+        if constexpr (!WP) {
+            launch_binned_k<MODE, WP, 2, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed);
+        } else {
+            switch (ks2) {
+                case 1: launch_binned_k<MODE, WP, 1, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+                case 2: launch_binned_k<MODE, WP, 2, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+                default: launch_binned_k<MODE, WP, 3, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+            }
+        }
+        return;
+    }
     if constexpr (!WP) {
         if (ks != 1) fail(CHIP_ERR_INVALID, "standard gridder halo must be the 7x7 support");
         launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed);

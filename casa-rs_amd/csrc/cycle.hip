@@ -33,6 +33,7 @@ namespace chip {
 void vset_residual_with(chip_vset &vs, const GridParams &gp, const float2 *model, float *res, hipStream_t st,
                         bool timed);
 void vset_split_items(chip_vset &vs, uint32_t gpi);
+bool binned_two_groups_allowed(const chip_vset &vs);
 void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order, bool force = false);
 
 namespace {
@@ -136,9 +137,10 @@ __global__ void k_touch(const WorkItem *__restrict__ items, uint64_t n_items, To
     const int64_t xa = x0 < 0 ? 0 : x0, ya = y0 < 0 ? 0 : y0;
     const int64_t xb = std::min<int64_t>(x0 + L - 1, tg.gx - 1), yb = std::min<int64_t>(y0 + L - 1, tg.gy - 1);
     if (xa > xb || ya > yb) return;
-    const TouchOut t{touch + static_cast<uint64_t>(it.chan) * tg.nyb_h * tg.nxc,
+    const uint32_t chan = it.chan & 0xffffffu;  // (top 8 bits: the item's largest half-support)
+    const TouchOut t{touch + static_cast<uint64_t>(chan) * tg.nyb_h * tg.nxc,
                      touch + static_cast<uint64_t>(tg.n_chan) * tg.nyb_h * tg.nxc +
-                         static_cast<uint64_t>(it.chan) * tg.nyb_h};
+                         static_cast<uint64_t>(chan) * tg.nyb_h};
     const int64_t hy = tg.gy / 2;
     const uint32_t uxa = static_cast<uint32_t>(xa), uxb = static_cast<uint32_t>(xb);
     if (ya <= hy) touch_mark(t, tg, static_cast<uint32_t>(ya), static_cast<uint32_t>(std::min(yb, hy)), uxa, uxb);
@@ -217,6 +219,26 @@ void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg
     vset_rebin(vs, best_tile, best_order);
     vset_split_items(vs, best_g);
     vs.threads = best_t;
+    // two groups per wavefront, timed at the chosen schedule (recorded with
+    // order + 2); kept when it wins by more than 1 %
+    vs.tuned_wave_groups = 1;
+    if (vs.wave_groups == 0 && binned_two_groups_allowed(vs) && spent <= kBudget) {
+        const GridParams gp = make_gp();
+        vs.tuned_wave_groups = 2;
+        float ms = -1.0f;
+        for (int rep = 0; rep < 4; ++rep) {
+            vset_residual_with(vs, gp, M, Rg, st, true);
+            CHIP_HIP(hipEventSynchronize(vs.ev_stop));
+            float m = 0.0f;
+            CHIP_HIP(hipEventElapsedTime(&m, vs.ev_start, vs.ev_stop));
+            if (rep == 0 && m < kLong) continue;
+            ms = ms < 0.0f ? m : std::min(ms, m);
+            if (m >= kLong) break;
+        }
+        vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(best_tile), static_cast<double>(best_order + 2),
+                                             static_cast<double>(best_t), static_cast<double>(best_g), ms});
+        if (!(ms < 0.99 * best)) vs.tuned_wave_groups = 1;
+    }
     vs.schedule_tuned = true;
 }
 

@@ -35,7 +35,7 @@ struct Plan {
 
 // One work item: a run of groups (identical plans) of one (channel, tile).
 struct WorkItem {
-    uint32_t chan;
+    uint32_t chan;   // channel (low 24 bits); W / AW: the largest half-support of the groups (top 8 bits)
     uint32_t tile;   // tile_x * tiles_y + tile_y
     uint32_t start;  // first group
     uint32_t count;  // groups
@@ -387,6 +387,39 @@ __device__ __forceinline__ void wave_sum_f2(float &a, float &b) {
     b = CHIP_DPP_ADD_ROWS(b, 0x143, 0x8);
     a = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
     b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
+}
+
+// Sums over each half-wavefront (lanes 0-31 and 32-63), returned to the
+// lanes of that half: row sums, then row_bcast15 into rows 1 and 3, whose
+// last lanes (31, 63) hold the two half totals.
+__device__ __forceinline__ void half_sum_f2(float &a, float &b, bool upper) {
+    a = CHIP_DPP_ADD(a, 0xB1);
+    b = CHIP_DPP_ADD(b, 0xB1);
+    a = CHIP_DPP_ADD(a, 0x4E);
+    b = CHIP_DPP_ADD(b, 0x4E);
+    a = CHIP_DPP_ADD(a, 0x141);
+    b = CHIP_DPP_ADD(b, 0x141);
+    a = CHIP_DPP_ADD(a, 0x140);
+    b = CHIP_DPP_ADD(b, 0x140);
+    a = CHIP_DPP_ADD_ROWS(a, 0x142, 0xA);
+    b = CHIP_DPP_ADD_ROWS(b, 0x142, 0xA);
+    const float a0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 31));
+    const float a1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
+    const float b0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 31));
+    const float b1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
+    a = upper ? a1 : a0;
+    b = upper ? b1 : b0;
+}
+
+__device__ __forceinline__ float half_sum_f(float v, bool upper) {
+    v = CHIP_DPP_ADD(v, 0xB1);
+    v = CHIP_DPP_ADD(v, 0x4E);
+    v = CHIP_DPP_ADD(v, 0x141);
+    v = CHIP_DPP_ADD(v, 0x140);
+    v = CHIP_DPP_ADD_ROWS(v, 0x142, 0xA);
+    const float v0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    const float v1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return upper ? v1 : v0;
 }
 
 // Wavefront-wide sum of a double (64 lanes).

@@ -404,6 +404,10 @@ struct chip_vset {
     // per work item (chosen by the autotuner at the first major cycle unless
     // fixed by chip_vset_schedule)
     uint32_t threads = 512, groups_per_item = 1024;
+    // groups per wavefront of the binned kernel: 0 = the autotuner's choice
+    // (tuned_wave_groups, one until tuned), 1 = one, 2 = two where the halo
+    // allows it (up to 4)
+    uint32_t wave_groups = 0, tuned_wave_groups = 1;
     uint32_t order = 0;  // groups of a bin by anchor cell (0) or by plan (1)
     // Planes whose samples share their plans (rows x channels input with equal
     // lambda scales in runs of `sibling`, e.g. the correlations of one

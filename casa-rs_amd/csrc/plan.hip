@@ -170,7 +170,18 @@ struct BinParams {
     uint32_t cell_bits;     // log2(tile * tile)
     uint32_t plan_bits;
     uint32_t n_sel;         // selector values per axis
+    uint32_t class_bits;    // W / AW: support class of the group's tap table
+    const int32_t *supports;  // half-support per tap table (W / AW)
 };
+
+// Support class of a tap table, in the order of the binned kernel's register
+// slots per lane (1, 2, 4, 7, then LDS-atomic chunks): groups of one bin are
+// ordered by class first, so that a work item holds one class and its
+// kernel runs with the fewest slots its footprints need.
+__host__ __device__ __forceinline__ uint32_t support_class(int32_t support) {
+    return support <= 3 ? 0u : support == 4 ? 1u : support <= 7 ? 2u : support <= 10 ? 3u : 4u;
+}
+constexpr uint32_t kClassBits = 3;
 
 __device__ __forceinline__ uint64_t plan_code(const PlanParams &p, const BinParams &b, const Plan &pl) {
     if (p.wproject) {
@@ -190,7 +201,7 @@ __global__ __launch_bounds__(256) void k_keys(PlanParams p, Source s, BinParams 
     if (i < n) {
         const SampleIn smp = load_sample(s, i);
         Plan pl;
-        uint64_t key = static_cast<uint64_t>(b.n_bins) << (b.cell_bits + b.plan_bits);
+        uint64_t key = static_cast<uint64_t>(b.n_bins) << (b.class_bits + b.cell_bits + b.plan_bits);
         if (sample_accepted(p, smp, pl)) {
             ok = true;
             const uint32_t ax = static_cast<uint32_t>(p.wproject ? pl.x : pl.x + kSupport);
@@ -200,7 +211,9 @@ __global__ __launch_bounds__(256) void k_keys(PlanParams p, Source s, BinParams 
             if (b.chunk_planes)
                 bin += static_cast<uint64_t>(static_cast<uint32_t>(pl.plane) / b.chunk_planes) * b.n_chan * b.n_tiles;
             const uint64_t cell = static_cast<uint64_t>((ax % b.tile) * b.tile + (ay % b.tile));
-            key = (bin << (b.cell_bits + b.plan_bits)) | (cell << b.plan_bits);
+            key = (bin << (b.class_bits + b.cell_bits + b.plan_bits)) | (cell << b.plan_bits);
+            if (b.class_bits)
+                key |= static_cast<uint64_t>(support_class(b.supports[pl.plane])) << (b.cell_bits + b.plan_bits);
             if (b.plan_bits) key |= plan_code(p, b, pl);
         }
         keys[i] = key;
@@ -351,9 +364,12 @@ __global__ void k_items_per_bin(const uint32_t *bin_groups, uint32_t n_bins, uin
     items_per_bin[k] = (bin_groups[k] + gpi - 1) / gpi;
 }
 
+// Work items of every bin; with W tables the item's channel word carries, in
+// its top 8 bits, the largest half-support of the item's groups (the binned
+// kernel sizes its register slots by it).
 __global__ void k_write_items(const uint32_t *bin_groups, const uint32_t *bin_first_group,
                               const uint32_t *item_start, uint32_t n_bins, uint32_t n_tiles,
-                              uint32_t n_chan, uint32_t gpi, WorkItem *items) {
+                              uint32_t n_chan, uint32_t gpi, const uint4 *grec, WorkItem *items) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n_bins) return;
     const uint32_t count = bin_groups[k];
@@ -366,7 +382,10 @@ __global__ void k_write_items(const uint32_t *bin_groups, const uint32_t *bin_fi
     uint32_t pos = bin_first_group[k];
     for (uint32_t j = 0; j < nit; ++j) {
         const uint32_t c = base + (j < extra ? 1u : 0u);
-        items[item_start[k] + j] = WorkItem{chan, tile, pos, c};
+        uint32_t halo = 0;
+        if (grec)
+            for (uint32_t q = pos; q < pos + c; ++q) halo = max(halo, grec[q].w >> 26);
+        items[item_start[k] + j] = WorkItem{chan | (halo << 24), tile, pos, c};
         pos += c;
     }
 }
@@ -415,7 +434,8 @@ void vset_split_items(chip_vset &vs, uint32_t gpi) {
     hipLaunchKernelGGL(k_write_items, dim3(blocks_for(n_bins, 256)), dim3(256), 0, st,
                        vs.bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(),
                        item_start.as<uint32_t>(), n_bins, vs.n_tiles_all,
-                       static_cast<uint32_t>(vs.n_chan), gpi, vs.items.as<WorkItem>());
+                       static_cast<uint32_t>(vs.n_chan), gpi, vs.g->wproject ? vs.grec.as<uint4>() : nullptr,
+                       vs.items.as<WorkItem>());
     CHIP_HIP(hipGetLastError());
     // first work item of each plane chunk
     const uint64_t per_chunk = static_cast<uint64_t>(vs.n_tiles_all) * vs.n_chan;
@@ -504,15 +524,19 @@ BinSetup bin_setup(chip_vset &vs, uint32_t tile) {
     b.n_chan = static_cast<uint32_t>(vs.n_chan);
     b.chunk_planes = vs.wk_chunk;
     b.cell_bits = static_cast<uint32_t>(bits_for(static_cast<uint64_t>(tile) * tile - 1));
+    // W tables recomputed per plane chunk keep one class (their launches
+    // follow the chunks)
+    b.class_bits = (g.wproject && !vs.wk_chunk) ? kClassBits : 0u;
+    b.supports = g.wproject ? g.d_supports.as<int32_t>() : nullptr;
     b.n_sel = g.wproject ? static_cast<uint32_t>(2 * g.norm_radius + 1) : g.os + 1;
     const uint64_t max_code = g.wproject
         ? (static_cast<uint64_t>(g.planes) * 2 * b.n_sel) * b.n_sel - 1
         : static_cast<uint64_t>(b.n_sel) * b.n_sel - 1;
     const int bin_bits = bits_for(s.n_bins);  // includes the invalid bin
     const int plan_bits = bits_for(max_code);
-    s.merge = bin_bits + static_cast<int>(b.cell_bits) + plan_bits <= 64;
+    s.merge = bin_bits + static_cast<int>(b.class_bits + b.cell_bits) + plan_bits <= 64;
     b.plan_bits = s.merge ? static_cast<uint32_t>(plan_bits) : 0u;
-    s.end_bit = bin_bits + static_cast<int>(b.cell_bits + b.plan_bits);
+    s.end_bit = bin_bits + static_cast<int>(b.class_bits + b.cell_bits + b.plan_bits);
     if (s.end_bit > 64) fail(CHIP_ERR_INVALID, "sort key exceeds 64 bits");
     return s;
 }
@@ -548,7 +572,7 @@ void build_groups(chip_vset &vs, const BinSetup &s, const DevBuf &keys_s) {
     DevBuf gstart((static_cast<uint64_t>(n_groups) + 1) * 4);
     DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
     CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
-    const uint32_t low_bits = s.b.cell_bits + s.b.plan_bits;
+    const uint32_t low_bits = s.b.class_bits + s.b.cell_bits + s.b.plan_bits;
     hipLaunchKernelGGL(k_group_starts, dim3(blocks_for(n_valid, 256)), dim3(256), 0, st,
                        keys_s.as<uint64_t>(), flags.as<uint32_t>(), gid.as<uint32_t>(), n_valid,
                        low_bits, gstart.as<uint32_t>(), bin_groups.as<uint32_t>());
@@ -603,7 +627,8 @@ __device__ __forceinline__ uint64_t rekey_one(const uint2 pk, uint32_t chan, con
     uint64_t bin = static_cast<uint64_t>(chan) * b.n_tiles + tile;
     if (b.chunk_planes) bin += static_cast<uint64_t>(plane / b.chunk_planes) * b.n_chan * b.n_tiles;
     const uint64_t cell = static_cast<uint64_t>((ax % b.tile) * b.tile + (ay % b.tile));
-    uint64_t key = bin << (b.cell_bits + b.plan_bits);
+    uint64_t key = bin << (b.class_bits + b.cell_bits + b.plan_bits);
+    if (b.class_bits) key |= static_cast<uint64_t>(support_class(b.supports[plane])) << (b.cell_bits + b.plan_bits);
     if (!b.plan_bits) {
         key |= cell;
     } else {
@@ -675,6 +700,7 @@ void vset_build(chip_vset &vs, const chip_samples &cs) {
     vs.n_samples = n;
     vs.n_chan = src.n_chan;
     vs.sibling = 1;
+    if (src.n_chan >= (1u << 24)) fail(CHIP_ERR_INVALID, "a visibility set holds fewer than 2^24 channels");
     if (src.uvw_m && src.n_chan > 1) {
         // runs of planes with bit-identical lambda scales share their plans
         std::vector<double> ls(src.n_chan);
@@ -797,7 +823,7 @@ void vset_rebin(chip_vset &vs, uint32_t tile, uint32_t order, bool force) {
         DevBuf bin_groups((static_cast<uint64_t>(n_bins) + 1) * 4);
         CHIP_HIP(hipMemsetAsync(bin_groups.p, 0, bin_groups.bytes, st));
         hipLaunchKernelGGL(k_bin_counts, dim3(blocks_for(ng, 256)), dim3(256), 0, st, keys_s.as<uint64_t>(), ng,
-                           bs.b.cell_bits + bs.b.plan_bits, bin_groups.as<uint32_t>());
+                           bs.b.class_bits + bs.b.cell_bits + bs.b.plan_bits, bin_groups.as<uint32_t>());
         CHIP_HIP(hipGetLastError());
         vs.bin_first.alloc((static_cast<uint64_t>(n_bins) + 1) * 4);
         exclusive_sum(bin_groups.as<uint32_t>(), vs.bin_first.as<uint32_t>(), n_bins + 1, st);

@@ -1028,6 +1028,13 @@ class VisibilitySet:
         again at the next major cycle."""
         abi.call("chip_vset_schedule", self.handle, int(tile), int(order), int(threads), int(groups_per_item))
 
+    def set_wave_groups(self, groups: int = 0) -> int:
+        """Groups per wavefront of the binned kernel (0 automatic, 1, 2);
+        returns the value in use for this set's halo."""
+        used = ctypes.c_uint32()
+        abi.call("chip_vset_wave_groups", self.handle, int(groups), ctypes.byref(used))
+        return int(used.value)
+
     def schedule(self):
         """Schedule in use and the autotuner's timed candidates
         [(tile, order, threads, groups, ms)]."""

@@ -609,10 +609,20 @@ int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
  * for autotuning again at the next major cycle.  Results do not depend on the
  * schedule beyond the order of float additions.  No reference counterpart
  * (the Metal path dispatches fixed-size chunks, lib.rs:72180-72290). */
+/* Groups the binned kernel processes per wavefront: 2 runs two groups at
+ * once, one per half-wavefront (32 lanes per footprint), which shares the
+ * per-group work that does not scale with the footprint; it applies to
+ * halos up to 4 and otherwise falls back to 1.  groups = 0 (the default)
+ * leaves the choice to the schedule autotuner of the first major cycle
+ * (one until then).  in_use (may be NULL) receives the value in use.
+ * Results agree within FP32 accumulation order.  No reference counterpart
+ * (a launch-shape knob of the batched kernel). */
+int chip_vset_wave_groups(chip_vset *vs, uint32_t groups, uint32_t *in_use, char **err);
 int chip_vset_schedule(chip_vset *vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t groups_per_item,
                        char **err);
 /* Schedule in use and, after autotuning, the timed candidates: n_timed
- * quintuples (tile, order, threads, groups per item, ms) in timed (at most 96
+ * quintuples (tile, order, threads, groups per item, ms) in timed (order + 2:
+ * timed with two groups per wavefront, chip_vset_wave_groups; at most 96
  * quintuples; may be NULL).  Any pointer may be NULL. */
 int chip_vset_schedule_info(const chip_vset *vs, uint32_t *tile, uint32_t *order, uint32_t *threads,
                             uint32_t *groups_per_item, uint32_t *n_timed, double *timed, char **err);

@@ -77,8 +77,15 @@ def test_first_major_cycle_keeps_the_fastest_candidate(gpu):
     fastest_anchor = min(x[4] for x in anchor)
     assert any(x[0] in plan_tile and x[4] <= fastest_anchor / 0.99 + 1e-9 for x in anchor)
     assert all(ms > 0 for *_, ms in timed)
-    chosen = [x for x in timed if x[:4] == (sch["tile"], sch["order"], sch["threads"], sch["groups_per_item"])]
-    assert len(chosen) == 1 and chosen[0][4] <= min(x[4] for x in timed) / 0.99 + 1e-9
+    one = [x for x in timed if x[1] < 2]
+    chosen = [x for x in one if x[:4] == (sch["tile"], sch["order"], sch["threads"], sch["groups_per_item"])]
+    assert len(chosen) == 1 and chosen[0][4] <= min(x[4] for x in one) / 0.99 + 1e-9
+    # then two groups per wavefront at the chosen schedule, kept if it wins by 1 %
+    two = [x for x in timed if x[1] >= 2]
+    assert len(two) <= 1
+    if two:
+        assert two[0][:4] == (sch["tile"], sch["order"] + 2, sch["threads"], sch["groups_per_item"])
+        assert vs.set_wave_groups(0) == (2 if two[0][4] < 0.99 * chosen[0][4] else 1)
     # a second cycle keeps the schedule without timing again
     vs.major_cycle(model, nsum, peaks)
     assert vs.schedule() == sch
@@ -140,3 +147,50 @@ def test_schedule_limits(gpu):
             vs.set_schedule(threads, gpi, tile=tile, order=order)
     vs.set_schedule(0, 0)  # back to autotuning
     assert vs.schedule()["threads"] == 512 and vs.schedule()["tile"] == 32
+
+
+@pytest.mark.parametrize("wproj", [True, False])
+def test_two_groups_per_wavefront_match_one(gpu, wproj):
+    """chip_vset_wave_groups: two groups per wavefront (one per half, the
+    default where the halo is at most 4) and one group per wavefront give the
+    same grids, degridded values and residual images to FP32 accumulation
+    order; the parity tests against the oracle run the default."""
+    g, batch, n_chan, nx = setup(gpu)
+    cell = g.geometry.cell_x
+    g = gpu.StandardGridder(nx, nx, cell, cell)
+    if wproj:
+        # a small W range keeps every support within the halo of 4 that two
+        # groups per wavefront need (larger |w| use the last plane)
+        g.attach_wproject(20.0, planes=4)
+    one = g.visibility_set(batch)
+    two = g.visibility_set(batch)
+    assert one.set_wave_groups(1) == 1
+    assert two.set_wave_groups(0) == 1  # the autotuner's choice; one until it has run
+    assert two.set_wave_groups(2) == 2
+    for vs in (one, two):
+        vs.set_schedule(256, 512)
+    pr, dr = g.new_grids(n_chan), g.new_grids(n_chan)
+    ps, ds = g.new_grids(n_chan), g.new_grids(n_chan)
+    one.grid(gpu.MODE_DIRTY, psf=pr, dirty=dr)
+    two.grid(gpu.MODE_DIRTY, psf=ps, dirty=ds)
+    assert rel(ps.cpu().numpy(), pr.cpu().numpy()) < 1e-6
+    assert rel(ds.cpu().numpy(), dr.cpu().numpy()) < 1e-5
+    rng = np.random.default_rng(19)
+    img = np.zeros((n_chan, nx, nx), np.float32)
+    for c in range(n_chan):
+        img[c, rng.integers(20, nx - 20, 10), rng.integers(20, nx - 20, 10)] = rng.uniform(0.1, 1.0, 10)
+    mg = g.model_to_grid(t(img))
+    a = one.degrid(mg).cpu().numpy()
+    b = two.degrid(mg).cpu().numpy()
+    assert rel(b, a) < 1e-6
+    vals = (rng.normal(size=a.shape) + 1j * rng.normal(size=a.shape)).astype(np.complex64)
+    ga, gb = g.new_grids(n_chan), g.new_grids(n_chan)
+    one.grid_values(t(vals).cuda(), ga)
+    two.grid_values(t(vals).cuda(), gb)
+    assert rel(gb.cpu().numpy(), ga.cpu().numpy()) < 1e-5
+    nsum = np.array([s["normalization_sumwt"] for s in one.sums])
+    _, _, peaks = g.finish_products(pr, None, nsum)
+    r1 = one.major_cycle(t(img), nsum, peaks).cpu().numpy()
+    r2 = two.major_cycle(t(img), nsum, peaks).cpu().numpy()
+    print(f"wave groups: residual {rel(r2, r1):.2e} of peak")
+    assert rel(r2, r1) < 3e-5
