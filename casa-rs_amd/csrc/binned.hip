@@ -211,10 +211,9 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // is fixed, so per group the y offset is computed once and the x offsets
     // of its slots follow by steps of P.
     // The slot layout follows the work item's own largest half-support (W /
-    // AW: the top 8 bits of its channel word; groups of a bin are ordered by
-    // support class, so an item mostly holds one class): the group loop is
-    // compiled for every slot count up to the launch's, and the item runs
-    // the one its footprints need.
+    // AW: the top 8 bits of its channel word), at most the launch's halo:
+    // items are launched by support class, each class with the LDS tile and
+    // register slots of its largest half-support.
     const int item_halo =
         WP ? min(static_cast<int>(it.chan >> 24), static_cast<int>(gp.halo)) : static_cast<int>(gp.halo);
     auto run = [&](auto ks_tag) {
@@ -659,36 +658,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
 #pragma unroll
     for (int k = 0; k < NS; ++k) spill(k);
     };
-    if constexpr (GPW == 2 || !WP || KS == 1) {
-        run(std::integral_constant<int, KS>{});
-    } else {
-        const int need = slots_for_halo(item_halo);
-        if constexpr (KS == 0) {
-            switch (need) {
-                case 1: run(std::integral_constant<int, 1>{}); break;
-                case 2: run(std::integral_constant<int, 2>{}); break;
-                case 4: run(std::integral_constant<int, 4>{}); break;
-                case 7: run(std::integral_constant<int, 7>{}); break;
-                default: run(std::integral_constant<int, 0>{}); break;
-            }
-        } else if constexpr (KS == 7) {
-            switch (need) {
-                case 1: run(std::integral_constant<int, 1>{}); break;
-                case 2: run(std::integral_constant<int, 2>{}); break;
-                case 4: run(std::integral_constant<int, 4>{}); break;
-                default: run(std::integral_constant<int, 7>{}); break;
-            }
-        } else if constexpr (KS == 4) {
-            switch (need) {
-                case 1: run(std::integral_constant<int, 1>{}); break;
-                case 2: run(std::integral_constant<int, 2>{}); break;
-                default: run(std::integral_constant<int, 4>{}); break;
-            }
-        } else {
-            if (need == 1) run(std::integral_constant<int, 1>{});
-            else run(std::integral_constant<int, KS>{});
-        }
-    }
+    run(std::integral_constant<int, KS>{});
     if (MODE == kModeDegrid) return;
     __syncthreads();
 
@@ -735,54 +705,22 @@ static void check_wk_chunk(const chip_vset &vs) {
         fail(CHIP_ERR_INVALID, "the W kernel residency changed after the visibility set was created");
 }
 
+// One launch over the work items [first, first + count).
 template <int MODE, bool WP, int KS, int GPW = 1>
 static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *values, const float2 *model,
-                            float *psf, float *acc, float2 *vis_out, size_t lds, hipStream_t st, bool timed) {
+                            float *psf, float *acc, float2 *vis_out, size_t lds, hipStream_t st, uint64_t first,
+                            uint64_t count) {
     if (lds > 64 * 1024)
         CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned<MODE, WP, KS, GPW>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const float4 *gsum = MODE == kModeResidual                         ? vs.gsum_r.as<float4>()
                          : (MODE == kModePsf || MODE == kModeDirty) ? vs.gsum_d.as<float4>()
                                                                      : nullptr;
-    auto launch = [&](const GridParams &p, uint64_t first, uint64_t count) {
-        hipLaunchKernelGGL((k_binned<MODE, WP, KS, GPW>), dim3(static_cast<unsigned>(count)), dim3(vs.threads),
-                           lds, st, p, vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(), gsum,
-                           vs.plan.as<uint2>(), vs.vis.as<float2>(), vs.wt.as<float>(),
-                           vs.index.as<uint32_t>(), values, model, psf, acc, vis_out);
-        CHIP_HIP(hipGetLastError());
-    };
-    if (vs.threads == 0 || vs.threads > kBinnedThreads || vs.threads % 64)
-        fail(CHIP_ERR_INVALID, "binned kernel threads must be a multiple of 64 up to 512");
-    if (!timed) {  // captured into a graph: kernels only
-        if (vs.wk_chunk) fail(CHIP_ERR_INVALID, "W tables recomputed on the fly cannot be captured");
-        launch(gp, 0, vs.n_items);
-        vs.timed = false;
-        return;
-    }
-    // Events on the launch stream bracket the kernel (chip_vset_stats); with
-    // W tables recomputed on the fly they bracket the recomputation too.
-    CHIP_HIP(hipEventRecord(vs.ev_start, st));
-    if (!vs.wk_chunk) {
-        launch(gp, 0, vs.n_items);
-    } else {
-        chip_gridder &g = *vs.g;
-        WkScratch ws;
-        wk_prepare(g, ws);
-        GridParams p = gp;
-        for (size_t c = 0; c + 1 < vs.chunk_items.size(); ++c) {
-            const uint64_t first = vs.chunk_items[c], count = vs.chunk_items[c + 1] - first;
-            if (count == 0) continue;  // no samples in these planes: nothing to recompute
-            const uint32_t lo = static_cast<uint32_t>(c) * vs.wk_chunk;
-            wk_fill(g, ws, lo, std::min(g.planes, lo + vs.wk_chunk), p);
-            launch(p, first, count);
-        }
-        CHIP_HIP(hipEventRecord(vs.ev_stop, st));
-        CHIP_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
-        vs.timed = true;
-        return;
-    }
-    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
-    vs.timed = true;
+    hipLaunchKernelGGL((k_binned<MODE, WP, KS, GPW>), dim3(static_cast<unsigned>(count)), dim3(vs.threads), lds,
+                       st, gp, vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(), gsum, vs.plan.as<uint2>(),
+                       vs.vis.as<float2>(), vs.wt.as<float>(), vs.index.as<uint32_t>(), values, model, psf, acc,
+                       vis_out);
+    CHIP_HIP(hipGetLastError());
 }
 
 // Register slots per lane for a tile halo: the x residues per lane,
@@ -799,15 +737,83 @@ static int register_slots_half(uint32_t halo) {
     return need <= 3 ? static_cast<int>(need) : 0;
 }
 
-// Groups per wavefront in use: 1, or 2 where the halo allows it and it was
-// asked for (wave_groups 2) or the autotuner chose it (wave_groups 0).
-int binned_wave_groups(const chip_vset &vs) {
-    if (register_slots_half(vs.halo) == 0 || vs.wave_groups == 1) return 1;
+// Largest half-support of a support class (classes as support_class in
+// plan.hip: up to 3, 4, 5 - 7, 8 - 10, above), at most the set's halo.
+static uint32_t class_halo(const chip_vset &vs, uint32_t cls) {
+    static constexpr uint32_t kCap[] = {3u, 4u, 7u, 10u};
+    return (vs.n_classes > 1 && cls < 4) ? std::min(kCap[cls], vs.halo) : vs.halo;
+}
+
+static bool class_used(const chip_vset &vs, uint32_t cls) {
+    const uint64_t n_chunks = vs.range_first.empty() ? 0 : (vs.range_first.size() - 1) / vs.n_classes;
+    for (uint64_t c = 0; c < n_chunks; ++c)
+        if (vs.range_first[c * vs.n_classes + cls + 1] > vs.range_first[c * vs.n_classes + cls]) return true;
+    return false;
+}
+
+// Groups per wavefront for a launch with this halo: 1, or 2 where the halo
+// allows it and it was asked for (wave_groups 2) or the autotuner chose it
+// (wave_groups 0).
+static int wave_groups_for(const chip_vset &vs, uint32_t halo) {
+    if (register_slots_half(halo) == 0 || vs.wave_groups == 1) return 1;
     return (vs.wave_groups == 2 || vs.tuned_wave_groups == 2) ? 2 : 1;
 }
 
-bool binned_two_groups_allowed(const chip_vset &vs) { return register_slots_half(vs.halo) > 0; }
+// Two groups per wavefront in use for some class with work items.
+int binned_wave_groups(const chip_vset &vs) {
+    for (uint32_t k = 0; k < vs.n_classes; ++k)
+        if (class_used(vs, k) && wave_groups_for(vs, class_halo(vs, k)) == 2) return 2;
+    return 1;
+}
 
+bool binned_two_groups_allowed(const chip_vset &vs) {
+    for (uint32_t k = 0; k < vs.n_classes; ++k)
+        if (class_used(vs, k) && register_slots_half(class_halo(vs, k)) > 0) return true;
+    return false;
+}
+
+// One range of work items with the kernel its halo needs.
+template <int MODE, bool WP>
+static void launch_range(chip_vset &vs, const GridParams &gp, const float2 *values, const float2 *model,
+                         float *psf, float *acc, float2 *vis_out, hipStream_t st, uint64_t first, uint64_t count) {
+    const size_t lds = binned_lds_bytes<MODE, WP>(gp);
+    if (lds > lds_limit(vs.device))
+        fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " + std::to_string(gp.halo) + ")");
+    const int ks = register_slots(gp.halo);
+    const int ks2 = wave_groups_for(vs, gp.halo) == 2 ? register_slots_half(gp.halo) : 0;
+#define CHIP_RANGE(KS_, GPW_) \
+    launch_binned_k<MODE, WP, KS_, GPW_>(vs, gp, values, model, psf, acc, vis_out, lds, st, first, count)
+    if (ks2 > 0) {  // two groups per wavefront
+        if constexpr (!WP) {
+            CHIP_RANGE(2, 2);
+        } else {
+            switch (ks2) {
+                case 1: CHIP_RANGE(1, 2); break;
+                case 2: CHIP_RANGE(2, 2); break;
+                default: CHIP_RANGE(3, 2); break;
+            }
+        }
+        return;
+    }
+    if constexpr (!WP) {
+        if (ks != 1) fail(CHIP_ERR_INVALID, "standard gridder halo must be the 7x7 support");
+        CHIP_RANGE(1, 1);
+    } else {
+        switch (ks) {
+            case 1: CHIP_RANGE(1, 1); break;
+            case 2: CHIP_RANGE(2, 1); break;
+            case 4: CHIP_RANGE(4, 1); break;
+            case 7: CHIP_RANGE(7, 1); break;
+            default: CHIP_RANGE(0, 1); break;
+        }
+    }
+#undef CHIP_RANGE
+}
+
+// Every (plane chunk, support class) range of work items: each class with the
+// LDS tile and register slots of its largest half-support.  Events on the
+// launch stream bracket the launches (chip_vset_stats); with W tables
+// recomputed on the fly they bracket the recomputation too.
 template <int MODE, bool WP>
 static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *values,
                             const float2 *model, float *psf, float *acc, float2 *vis_out,
@@ -817,32 +823,47 @@ static void launch_binned_t(chip_vset &vs, const GridParams &gp, const float2 *v
         fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " +
                                    std::to_string(gp.halo) + ")");
     if (vs.n_items == 0) return;
-    const int ks = register_slots(gp.halo);
-    const int ks2 = binned_wave_groups(vs) == 2 ? register_slots_half(gp.halo) : 0;
-    if (ks2 > 0) {  // two groups per wavefront
-        if constexpr (!WP) {
-            launch_binned_k<MODE, WP, 2, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed);
-        } else {
-            switch (ks2) {
-                case 1: launch_binned_k<MODE, WP, 1, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-                case 2: launch_binned_k<MODE, WP, 2, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-                default: launch_binned_k<MODE, WP, 3, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-            }
+    if (vs.threads == 0 || vs.threads > kBinnedThreads || vs.threads % 64)
+        fail(CHIP_ERR_INVALID, "binned kernel threads must be a multiple of 64 up to 512");
+    auto launch_chunk = [&](const GridParams &p, uint64_t c) {
+        for (uint32_t k = 0; k < vs.n_classes; ++k) {
+            const uint64_t first = vs.range_first[c * vs.n_classes + k];
+            const uint64_t end = vs.range_first[c * vs.n_classes + k + 1];
+            if (end == first) continue;
+            GridParams pc = p;
+            pc.halo = class_halo(vs, k);
+            pc.L = pc.tile + 2 * pc.halo;
+            pc.Ls = pc.L | 1u;
+            launch_range<MODE, WP>(vs, pc, values, model, psf, acc, vis_out, st, first, end - first);
         }
+    };
+    if (!timed) {  // captured into a graph: kernels only
+        if (vs.wk_chunk) fail(CHIP_ERR_INVALID, "W tables recomputed on the fly cannot be captured");
+        launch_chunk(gp, 0);
+        vs.timed = false;
         return;
     }
-    if constexpr (!WP) {
-        if (ks != 1) fail(CHIP_ERR_INVALID, "standard gridder halo must be the 7x7 support");
-        launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed);
+    CHIP_HIP(hipEventRecord(vs.ev_start, st));
+    if (!vs.wk_chunk) {
+        launch_chunk(gp, 0);
     } else {
-        switch (ks) {
-            case 1: launch_binned_k<MODE, WP, 1>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-            case 2: launch_binned_k<MODE, WP, 2>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-            case 4: launch_binned_k<MODE, WP, 4>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-            case 7: launch_binned_k<MODE, WP, 7>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
-            default: launch_binned_k<MODE, WP, 0>(vs, gp, values, model, psf, acc, vis_out, lds, st, timed); break;
+        chip_gridder &g = *vs.g;
+        WkScratch ws;
+        wk_prepare(g, ws);
+        GridParams p = gp;
+        for (size_t c = 0; c + 1 < vs.chunk_items.size(); ++c) {
+            if (vs.chunk_items[c + 1] == vs.chunk_items[c]) continue;  // no samples in these planes
+            const uint32_t lo = static_cast<uint32_t>(c) * vs.wk_chunk;
+            wk_fill(g, ws, lo, std::min(g.planes, lo + vs.wk_chunk), p);
+            launch_chunk(p, c);
         }
+        CHIP_HIP(hipEventRecord(vs.ev_stop, st));
+        CHIP_HIP(hipStreamSynchronize(st));  // the scratch is freed on return
+        vs.timed = true;
+        return;
     }
+    CHIP_HIP(hipEventRecord(vs.ev_stop, st));
+    vs.timed = true;
 }
 
 template <int MODE>

@@ -396,6 +396,12 @@ struct chip_vset {
     // n_items at the end).  wk_chunk = 0: one chunk.
     uint32_t wk_chunk = 0;
     std::vector<uint32_t> chunk_items;
+    // items are ordered by (plane chunk, support class of their largest
+    // half-support, bin); range_first[chunk * n_classes + class] is the first
+    // item of each range (plus n_items at the end).  The standard gridder has
+    // one class.
+    std::vector<uint32_t> range_first;
+    uint32_t n_classes = 1;
     // bins (plane chunk, channel, tile): groups per bin and first group, kept
     // so that the schedule autotuner can split the work items again
     chip::DevBuf bin_groups, bin_first;

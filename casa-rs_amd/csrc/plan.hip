@@ -182,6 +182,7 @@ __host__ __device__ __forceinline__ uint32_t support_class(int32_t support) {
     return support <= 3 ? 0u : support == 4 ? 1u : support <= 7 ? 2u : support <= 10 ? 3u : 4u;
 }
 constexpr uint32_t kClassBits = 3;
+constexpr uint32_t kSupportClasses = 5;
 
 __device__ __forceinline__ uint64_t plan_code(const PlanParams &p, const BinParams &b, const Plan &pl) {
     if (p.wproject) {
@@ -437,11 +438,37 @@ void vset_split_items(chip_vset &vs, uint32_t gpi) {
                        static_cast<uint32_t>(vs.n_chan), gpi, vs.g->wproject ? vs.grec.as<uint4>() : nullptr,
                        vs.items.as<WorkItem>());
     CHIP_HIP(hipGetLastError());
-    // first work item of each plane chunk
-    const uint64_t per_chunk = static_cast<uint64_t>(vs.n_tiles_all) * vs.n_chan;
-    for (uint64_t c = 0; c < vs.chunk_items.size(); ++c) vs.chunk_items[c] = starts[c * per_chunk];
     CHIP_HIP(hipStreamSynchronize(st));
-    if (vs.item_order == 1 && vs.sibling > 1 && vs.chunk_items.size() == 2 && n_items) {
+    // Items ordered by (plane chunk, support class, bin): each (chunk, class)
+    // range is launched with the LDS tile and register slots of its class
+    // (range_first[chunk * n_classes + class] is its first item).  The class
+    // of an item is that of its largest half-support (top 8 bits of its
+    // channel word); the standard gridder has one class.
+    const uint32_t n_cls = vs.g->wproject ? kSupportClasses : 1u;
+    const uint64_t n_chunks = vs.chunk_items.size() - 1;
+    const uint64_t per_chunk = static_cast<uint64_t>(vs.n_tiles_all) * vs.n_chan;
+    std::vector<WorkItem> items(n_items);
+    if (n_items) {
+        CHIP_HIP(hipMemcpyAsync(items.data(), vs.items.p, n_items * sizeof(WorkItem), hipMemcpyDeviceToHost, st));
+        CHIP_HIP(hipStreamSynchronize(st));
+    }
+    auto class_of = [&](const WorkItem &w) { return n_cls == 1 ? 0u : support_class(static_cast<int32_t>(w.chan >> 24)); };
+    // per (chunk, class): the items of every bin, in bin order
+    std::vector<std::vector<uint32_t>> range_items(n_chunks * n_cls);
+    for (uint64_t b = 0; b < n_bins; ++b)
+        for (uint32_t k = starts[b]; k < starts[b + 1]; ++k)
+            range_items[(b / per_chunk) * n_cls + class_of(items[k])].push_back(k);
+    const bool xcd = vs.item_order == 1 && vs.sibling > 1 && n_chunks == 1;
+    std::vector<WorkItem> order;
+    order.reserve(n_items);
+    vs.range_first.assign(n_chunks * n_cls + 1, 0u);
+    for (uint64_t r = 0; r < range_items.size(); ++r) {
+        vs.range_first[r] = static_cast<uint32_t>(order.size());
+        const std::vector<uint32_t> &ri = range_items[r];
+        if (!xcd || ri.empty()) {
+            for (uint32_t k : ri) order.push_back(items[k]);
+            continue;
+        }
         // Sets of sibling work items: the items of one (sibling group, tile,
         // split) of the G planes of the group.  Set k goes to XCD k mod 8;
         // each XCD's queue holds its sets' items one after the other, and
@@ -450,38 +477,49 @@ void vset_split_items(chip_vset &vs, uint32_t gpi) {
         // item of queue p mod 8; shorter queues are padded with items of no
         // groups.
         constexpr uint32_t kXcd = 8;
-        std::vector<WorkItem> items(n_items);
-        CHIP_HIP(hipMemcpyAsync(items.data(), vs.items.p, n_items * sizeof(WorkItem), hipMemcpyDeviceToHost, st));
-        CHIP_HIP(hipStreamSynchronize(st));
         const uint32_t G = vs.sibling, n_tiles = vs.n_tiles_all;
+        // this range's items of bin b: ri[bstart[b] .. bstart[b + 1]) (ri is in bin order)
+        std::vector<uint32_t> bstart(static_cast<uint64_t>(n_bins) + 1, 0u);
+        {
+            uint64_t b = 0;
+            for (uint32_t k : ri) {
+                while (starts[b + 1] <= k) ++b;
+                ++bstart[b + 1];
+            }
+            for (uint64_t q = 0; q < n_bins; ++q) bstart[q + 1] += bstart[q];
+        }
+        auto bin_count = [&](uint64_t b) { return static_cast<size_t>(bstart[b + 1] - bstart[b]); };
         std::vector<std::vector<WorkItem>> queue(kXcd);
         uint64_t set = 0;
         for (uint32_t grp = 0; grp * G < vs.n_chan; ++grp) {
             for (uint32_t t = 0; t < n_tiles; ++t) {
-                uint32_t most = 0;
-                for (uint32_t m = 0; m < G; ++m) {
-                    const uint64_t b = static_cast<uint64_t>(grp * G + m) * n_tiles + t;
-                    most = std::max(most, starts[b + 1] - starts[b]);
-                }
-                for (uint32_t r = 0; r < most; ++r, ++set) {
-                    std::vector<WorkItem> &q = queue[set % kXcd];
-                    for (uint32_t m = 0; m < G; ++m) {
+                size_t most = 0;
+                for (uint32_t m = 0; m < G && grp * G + m < vs.n_chan; ++m)
+                    most = std::max(most, bin_count(static_cast<uint64_t>(grp * G + m) * n_tiles + t));
+                for (size_t q = 0; q < most; ++q, ++set) {
+                    std::vector<WorkItem> &qu = queue[set % kXcd];
+                    for (uint32_t m = 0; m < G && grp * G + m < vs.n_chan; ++m) {
                         const uint64_t b = static_cast<uint64_t>(grp * G + m) * n_tiles + t;
-                        if (r < starts[b + 1] - starts[b]) q.push_back(items[starts[b] + r]);
+                        if (q < bin_count(b)) qu.push_back(items[ri[bstart[b] + q]]);
                     }
                 }
             }
         }
         size_t longest = 0;
         for (const auto &q : queue) longest = std::max(longest, q.size());
-        std::vector<WorkItem> order(longest * kXcd, WorkItem{0u, 0u, 0u, 0u});
+        const size_t base = order.size();
+        order.resize(base + longest * kXcd, WorkItem{0u, 0u, 0u, 0u});
         for (uint32_t x = 0; x < kXcd; ++x)
-            for (size_t k = 0; k < queue[x].size(); ++k) order[k * kXcd + x] = queue[x][k];
-        vs.n_items = order.size();
-        vs.items.alloc(vs.n_items * sizeof(WorkItem));
+            for (size_t k = 0; k < queue[x].size(); ++k) order[base + k * kXcd + x] = queue[x][k];
+    }
+    vs.range_first.back() = static_cast<uint32_t>(order.size());
+    vs.n_classes = n_cls;
+    for (uint64_t c = 0; c <= n_chunks; ++c) vs.chunk_items[c] = vs.range_first[c * n_cls];
+    vs.n_items = order.size();
+    vs.items.alloc(std::max<uint64_t>(vs.n_items, 1) * sizeof(WorkItem));
+    if (vs.n_items) {
         CHIP_HIP(hipMemcpyAsync(vs.items.p, order.data(), vs.n_items * sizeof(WorkItem), hipMemcpyHostToDevice, st));
         CHIP_HIP(hipStreamSynchronize(st));
-        vs.chunk_items[1] = static_cast<uint32_t>(vs.n_items);
     }
     vs.items_version += 1;
 }
