@@ -492,11 +492,21 @@ def reduce_traffic(fetch_dir, write_dir, out_json, out_txt, commands):
         entry = {}
         for name, counter in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
             matches = [(kn, v) for kn, v in per_counter[name].items() if k + "<" in kn or k + "(" in kn]
+            if k == "k_binned":
+                # the residual pass of a step is one launch per support class:
+                # every residual-mode instantiation, summed
+                matches = [(kn, v) for kn, v in matches if "k_binned<2," in kn]
             if not matches:
                 break
-            kname, vals = max(matches, key=lambda kv: len(kv[1]))
-            raw = sum(vals) / len(vals)
-            entry[name] = dict(counter=counter, kernel=kname[:160], dispatches=len(vals), kib_per_dispatch=raw,
+            if k == "k_binned":
+                kname = " + ".join(sorted(kn[:60] for kn, _ in matches))
+                raw = sum(sum(v) / len(v) for _, v in matches)
+                n_disp = min(len(v) for _, v in matches)
+            else:
+                kname, vals = max(matches, key=lambda kv: len(kv[1]))
+                raw = sum(vals) / len(vals)
+                n_disp = len(vals)
+            entry[name] = dict(counter=counter, kernel=kname[:400], dispatches=n_disp, kib_per_dispatch=raw,
                                bytes_per_dispatch=raw * scales[name] * 1024.0)
         if len(entry) == 2:
             entry["bytes_per_dispatch"] = entry["fetch"]["bytes_per_dispatch"] + entry["write"]["bytes_per_dispatch"]
@@ -515,16 +525,21 @@ def reduce_traffic(fetch_dir, write_dir, out_json, out_txt, commands):
 def reduce_counters(directory, kernel, out_txt, command, kernel_ms=None):
     """rocprofv3 --pmc run -> per-counter mean over the dispatches of `kernel`,
     with the FP32 operation rate when the FP32 instruction counters are there."""
-    per = {}
+    per = {}  # counter -> kernel -> values
     for r in _profile_csv(directory, "counter_collection.csv"):
         if kernel in r["Kernel_Name"]:
-            per.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+            per.setdefault(r["Counter_Name"], {}).setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
     if not per:
         raise SystemExit(f"no dispatches of {kernel}")
-    lines = [f"command: {command}", "", f"kernel: {kernel}"]
-    mean = {k: sum(v) / len(v) for k, v in per.items()}
+    names = sorted({kn for d in per.values() for kn in d})
+    lines = [f"command: {command}", "", f"kernel: {kernel} (per step: the sum over {len(names)} instantiation(s), "
+                                        "one launch per support class)"]
+    lines += [f"  {kn[:110]}" for kn in names]
+    # per step: the mean per dispatch of every instantiation, summed
+    mean = {c: sum(sum(v) / len(v) for v in d.values()) for c, d in per.items()}
     for k in sorted(mean):
-        lines.append(f"  {k:28s} {mean[k]:16.4f}   ({len(per[k])} dispatches)")
+        n = min(len(v) for v in per[k].values())
+        lines.append(f"  {k:28s} {mean[k]:16.4f}   ({n} dispatches each)")
     f32 = ("SQ_INSTS_VALU_FMA_F32", "SQ_INSTS_VALU_ADD_F32", "SQ_INSTS_VALU_MUL_F32", "SQ_INSTS_VALU_TRANS_F32")
     if all(k in mean for k in f32):
         ops = (2 * mean[f32[0]] + mean[f32[1]] + mean[f32[2]] + mean[f32[3]]) * 64
