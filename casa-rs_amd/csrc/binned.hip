@@ -738,10 +738,10 @@ static int register_slots_half(uint32_t halo) {
 }
 
 // Largest half-support of a support class (classes as support_class in
-// plan.hip: up to 3, 4, 5 - 7, 8 - 10, above), at most the set's halo.
+// plan.hip: up to 3, 4, 7, 10, 15, 22, above), at most the set's halo.
 static uint32_t class_halo(const chip_vset &vs, uint32_t cls) {
-    static constexpr uint32_t kCap[] = {3u, 4u, 7u, 10u};
-    return (vs.n_classes > 1 && cls < 4) ? std::min(kCap[cls], vs.halo) : vs.halo;
+    static constexpr uint32_t kCap[] = {3u, 4u, 7u, 10u, 15u, 22u};
+    return (vs.n_classes > 1 && cls < 6) ? std::min(kCap[cls], vs.halo) : vs.halo;
 }
 
 static bool class_used(const chip_vset &vs, uint32_t cls) {

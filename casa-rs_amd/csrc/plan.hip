@@ -174,15 +174,23 @@ struct BinParams {
     const int32_t *supports;  // half-support per tap table (W / AW)
 };
 
-// Support class of a tap table, in the order of the binned kernel's register
-// slots per lane (1, 2, 4, 7, then LDS-atomic chunks): groups of one bin are
-// ordered by class first, so that a work item holds one class and its
-// kernel runs with the fewest slots its footprints need.
+// Support class of a tap table: the binned kernel's register slots per lane
+// (1, 2, 4, 7 for half-supports up to 3, 4, 7, 10), then LDS-atomic chunks
+// with LDS tiles for half-supports up to 15, 22 and above.  Groups of one
+// bin are ordered by class first, so that a work item holds one class and
+// its launch runs with the fewest slots and the smallest LDS tile its
+// footprints need.
 __host__ __device__ __forceinline__ uint32_t support_class(int32_t support) {
-    return support <= 3 ? 0u : support == 4 ? 1u : support <= 7 ? 2u : support <= 10 ? 3u : 4u;
+    return support <= 3    ? 0u
+           : support == 4  ? 1u
+           : support <= 7  ? 2u
+           : support <= 10 ? 3u
+           : support <= 15 ? 4u
+           : support <= 22 ? 5u
+                           : 6u;
 }
 constexpr uint32_t kClassBits = 3;
-constexpr uint32_t kSupportClasses = 5;
+constexpr uint32_t kSupportClasses = 7;
 
 __device__ __forceinline__ uint64_t plan_code(const PlanParams &p, const BinParams &b, const Plan &pl) {
     if (p.wproject) {
