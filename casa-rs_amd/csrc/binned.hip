@@ -103,13 +103,13 @@ __device__ __forceinline__ void div_conj_rcp(float &re, float &im, float nr, flo
 }
 
 // Register slots per lane for a footprint side M = 2 halo + 1 with 64 lanes:
-// ceil(M / floor(64 / M)) rounded up to an instantiated count (1, 2, 4, 7),
-// 0 (LDS-atomic chunks) beyond 7 (halo > 10).
+// ceil(M / floor(64 / M)) rounded up to an instantiated count (1, 2, 4, 7,
+// 16), 0 (LDS-atomic chunks) beyond 16 (halo > 15).
 __host__ __device__ __forceinline__ int slots_for_halo(int halo) {
     const int m = 2 * halo + 1, p = 64 / m;
     if (p == 0) return 0;
     const int need = (m + p - 1) / p;
-    return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 7 ? 7 : 0;
+    return need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : need <= 7 ? 7 : need <= 16 ? 16 : 0;
 }
 
 // Two groups per wavefront (GPW = 2): the lanes of each half-wavefront own
@@ -725,7 +725,7 @@ static void launch_binned_k(chip_vset &vs, const GridParams &gp, const float2 *v
 
 // Register slots per lane for a tile halo: the x residues per lane,
 // ceil(M / floor(64 / M)) with M = 2 halo + 1, rounded up to an instantiated
-// count; 0 (LDS-atomic chunks) beyond 7 (halo > 10).
+// count; 0 (LDS-atomic chunks) beyond 16 (halo > 15).
 static int register_slots(uint32_t halo) { return slots_for_halo(static_cast<int>(halo)); }
 
 // Register slots per lane with two groups per wavefront (32 lanes per
@@ -804,6 +804,7 @@ static void launch_range(chip_vset &vs, const GridParams &gp, const float2 *valu
             case 2: CHIP_RANGE(2, 1); break;
             case 4: CHIP_RANGE(4, 1); break;
             case 7: CHIP_RANGE(7, 1); break;
+            case 16: CHIP_RANGE(16, 1); break;
             default: CHIP_RANGE(0, 1); break;
         }
     }
