@@ -583,8 +583,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // the taps and samples of group g + 1 are in flight.
     // GPW = 2: each half-wavefront holds a window of 32 records of its own
     // run; both halves advance one group per iteration, so their windows
-    // move together, and a record reaches its half's lanes by two readlanes
-    // and a select.
+    // move together, and a record reaches its half's lanes by one backward
+    // permute (lane l reads lane (l & 32) + k).
     auto rec_load = [&](uint32_t wbase) {
         const uint32_t g = h_begin + wbase + static_cast<uint32_t>(hl);
         return g < h_end ? grec[g] : make_uint4(0u, 0u, 0u, 0u);
@@ -593,12 +593,11 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         const uint32_t g = h_begin + wbase + static_cast<uint32_t>(hl);
         return (kSums && g < h_end) ? gsum[g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     };
+    const int perm_half = (lane & 32) << 2;  // byte address of the half's first lane
     auto bcast = [&](uint32_t x, uint32_t k) {
-        const int kk = static_cast<int>(k);
-        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), kk));
-        if (GPW == 1) return lo;
-        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), kk + 32));
-        return upper ? hi : lo;
+        if (GPW == 1) return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), static_cast<int>(k)));
+        return static_cast<uint32_t>(
+            __builtin_amdgcn_ds_bpermute(perm_half + static_cast<int>(k << 2), static_cast<int>(x)));
     };
     auto sum_at = [&](const float4 &sb, uint32_t k) {
         return make_float4(__uint_as_float(bcast(__float_as_uint(sb.x), k)),
@@ -608,52 +607,44 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     auto rec_at = [&](const uint4 &rb, uint32_t k) {
         return make_uint4(bcast(rb.x, k), bcast(rb.y, k), bcast(rb.z, k), bcast(rb.w, k));
     };
-    if (n_iter > 0) {
-        uint32_t wbase = 0;
-        uint4 rb = rec_load(wbase);
-        float4 sb = sum_load(wbase);
-        const uint4 c0 = rec_at(rb, 0);
-        uint32_t c_s0 = c0.x, c_s1 = c0.y;
-        uint2 c_pk = make_uint2(c0.z, c0.w);
-        float4 c_g = kSums ? sum_at(sb, 0) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-        bool c_valid = h_begin < h_end;
-        Foot c_f = footprint(c_pk);
-        GroupData c_d;
-        load_group(c_s0, c_s1, c_f, c_d, c_valid);
-        for (uint32_t t = 0; t < n_iter; ++t) {
-            Foot n_f = c_f;
-            GroupData n_d = c_d;
-            uint32_t n_s0 = 0, n_s1 = 0;
-            uint2 n_pk = make_uint2(0u, 0u);
-            float4 n_g = c_g;
-            bool n_valid = false;
-            if (t + 1 < n_iter) {
-                uint32_t k = t + 1 - wbase;
-                if (k == static_cast<uint32_t>(LW)) {
-                    wbase += LW;
-                    rb = rec_load(wbase);
-                    sb = sum_load(wbase);
-                    k = 0;
-                }
-                const uint4 nr = rec_at(rb, k);
-                n_s0 = nr.x;
-                n_s1 = nr.y;
-                n_pk = make_uint2(nr.z, nr.w);
-                if (kSums) n_g = sum_at(sb, k);
-                n_valid = h_begin + t + 1 < h_end;
-                n_f = footprint(n_pk);
-                load_group(n_s0, n_s1, n_f, n_d, n_valid);
-            }
-            process(c_s0, c_s1, c_pk, c_f, c_d, c_g);
-            c_s0 = n_s0;
-            c_s1 = n_s1;
-            c_pk = n_pk;
-            c_f = n_f;
-            c_d = n_d;
-            c_g = n_g;
-            c_valid = n_valid;
+    // one group in flight: its record, sums, footprint and per-lane inputs
+    struct Stage {
+        uint32_t s0, s1;
+        uint2 pk;
+        float4 g;
+        Foot f;
+        GroupData d;
+    };
+    uint32_t wbase = 0;
+    uint4 rb = make_uint4(0u, 0u, 0u, 0u);
+    float4 sb = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    auto fetch = [&](uint32_t t, Stage &x) {
+        uint32_t k = t - wbase;
+        if (k == static_cast<uint32_t>(LW)) {
+            wbase += LW;
+            rb = rec_load(wbase);
+            sb = sum_load(wbase);
+            k = 0;
         }
-        (void)c_valid;
+        const uint4 r = rec_at(rb, k);
+        x.s0 = r.x;
+        x.s1 = r.y;
+        x.pk = make_uint2(r.z, r.w);
+        x.g = kSums ? sum_at(sb, k) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        x.f = footprint(x.pk);
+        load_group(x.s0, x.s1, x.f, x.d, h_begin + t < h_end);
+    };
+    if (n_iter > 0) {
+        rb = rec_load(0);
+        sb = sum_load(0);
+        Stage c;
+        fetch(0, c);
+        for (uint32_t t = 0; t < n_iter; ++t) {
+            Stage n = c;
+            if (t + 1 < n_iter) fetch(t + 1, n);
+            process(c.s0, c.s1, c.pk, c.f, c.d, c.g);
+            c = n;
+        }
     }
 #pragma unroll
     for (int k = 0; k < NS; ++k) spill(k);
