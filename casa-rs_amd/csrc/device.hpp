@@ -389,10 +389,17 @@ __device__ __forceinline__ void wave_sum_f2(float &a, float &b) {
     b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
 }
 
-// Sums over each half-wavefront (lanes 0-31 and 32-63), returned to the
-// lanes of that half: row sums, then row_bcast15 into rows 1 and 3, whose
-// last lanes (31, 63) hold the two half totals.
+// Sums over each half-wavefront (lanes 0-31 and 32-63), returned to every
+// lane of that half: row sums, then v_permlane16_swap of the row sums with
+// themselves, which leaves rows (0, 0, 2, 2) in one register and (1, 1, 3, 3)
+// in the other, so one add gives rows 0 + 1 and 2 + 3.  (Same additions, in
+// the same order, as row_bcast15: bit-identical sums.)
+__device__ __forceinline__ float pair_rows(float v) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
 __device__ __forceinline__ void half_sum_f2(float &a, float &b, bool upper) {
+    (void)upper;
     a = CHIP_DPP_ADD(a, 0xB1);
     b = CHIP_DPP_ADD(b, 0xB1);
     a = CHIP_DPP_ADD(a, 0x4E);
@@ -401,25 +408,17 @@ __device__ __forceinline__ void half_sum_f2(float &a, float &b, bool upper) {
     b = CHIP_DPP_ADD(b, 0x141);
     a = CHIP_DPP_ADD(a, 0x140);
     b = CHIP_DPP_ADD(b, 0x140);
-    a = CHIP_DPP_ADD_ROWS(a, 0x142, 0xA);
-    b = CHIP_DPP_ADD_ROWS(b, 0x142, 0xA);
-    const float a0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 31));
-    const float a1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), 63));
-    const float b0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 31));
-    const float b1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(b), 63));
-    a = upper ? a1 : a0;
-    b = upper ? b1 : b0;
+    a = pair_rows(a);
+    b = pair_rows(b);
 }
 
 __device__ __forceinline__ float half_sum_f(float v, bool upper) {
+    (void)upper;
     v = CHIP_DPP_ADD(v, 0xB1);
     v = CHIP_DPP_ADD(v, 0x4E);
     v = CHIP_DPP_ADD(v, 0x141);
     v = CHIP_DPP_ADD(v, 0x140);
-    v = CHIP_DPP_ADD_ROWS(v, 0x142, 0xA);
-    const float v0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
-    const float v1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-    return upper ? v1 : v0;
+    return pair_rows(v);
 }
 
 // Wavefront-wide sum of a double (64 lanes).
