@@ -141,13 +141,15 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     float *a1 = a0 + cells;
     float *a2 = a1 + cells;
     float *a3 = a2 + cells;
-    float *P_re = nullptr, *P_im = nullptr, *A_re = nullptr, *A_im = nullptr, *M_re = nullptr,
-          *M_im = nullptr;
+    // the staged model is interleaved (re, im) over the first two arrays: one
+    // 64-bit LDS read per cell
+    float *P_re = nullptr, *P_im = nullptr, *A_re = nullptr, *A_im = nullptr;
+    float2 *M2 = nullptr;
     if (MODE == kModePsf) { P_re = a0; P_im = a1; }
     if (MODE == kModeDirty) { P_re = a0; P_im = a1; A_re = a2; A_im = a3; }
-    if (MODE == kModeResidual) { M_re = a0; M_im = a1; A_re = a2; A_im = a3; }
+    if (MODE == kModeResidual) { M2 = reinterpret_cast<float2 *>(a0); A_re = a2; A_im = a3; }
     if (MODE == kModeAdjoint) { A_re = a0; A_im = a1; }
-    if (MODE == kModeDegrid) { M_re = a0; M_im = a1; }
+    if (MODE == kModeDegrid) { M2 = reinterpret_cast<float2 *>(a0); }
     constexpr bool kModel = MODE == kModeResidual || MODE == kModeDegrid;
     constexpr bool kAcc = MODE == kModeResidual || MODE == kModeDirty || MODE == kModeAdjoint;
     constexpr bool kPsf = MODE == kModePsf || MODE == kModeDirty;
@@ -190,8 +192,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                     m.y = -m.y;
                 }
             }
-            M_re[c] = m.x;
-            M_im[c] = m.y;
+            M2[c] = m;
         }
         if (kAcc) { A_re[c] = 0.0f; A_im[c] = 0.0f; }
         if (kPsf) { P_re[c] = 0.0f; P_im[c] = 0.0f; }
@@ -468,7 +469,8 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                             const int tx = static_cast<int>((static_cast<float>(f) + 0.5f) * inv);
                             const int cell = (bx + tx) * Ls + (by + f - tx * side);
                             const float cr = tv[k].x, ci = -(tv[k].y * csign);
-                            const float mr = M_re[cell], mi = M_im[cell];
+                            const float2 mc = M2[cell];
+                            const float mr = mc.x, mi = mc.y;
                             pr += cr * mr - ci * mi;
                             pi += cr * mi + ci * mr;
                         }
@@ -521,8 +523,9 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
                 spill(k);
                 cell_k[k] = cell;
                 if (kModel) {
-                    mre[k] = M_re[cell];
-                    mim[k] = M_im[cell];
+                    const float2 mc = M2[cell];
+                    mre[k] = mc.x;
+                    mim[k] = mc.y;
                 }
             }
             cr[k] = d.tap[k].x;
