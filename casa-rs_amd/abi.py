@@ -61,7 +61,7 @@ EXPORTS = (
     "chip_density_weights", "chip_phase_rotation", "chip_rotate_rows",
     "chip_vset_create", "chip_vset_destroy", "chip_vset_sums", "chip_vset_stats", "chip_vset_grid",
     "chip_vset_degrid", "chip_vset_grid_values", "chip_vset_major_cycle", "chip_vset_cycle_info",
-    "chip_vset_cycle_strategy", "chip_vset_cycle_graph", "chip_vset_wave_groups", "chip_vset_cycle_passes", "chip_vset_item_order", "chip_vset_schedule", "chip_vset_schedule_info",
+    "chip_vset_cycle_strategy", "chip_vset_cycle_graph", "chip_vset_wave_groups", "chip_vset_two_phase", "chip_vset_cycle_passes", "chip_vset_item_order", "chip_vset_schedule", "chip_vset_schedule_info",
     "chip_grid_plain", "chip_grid_plain_noalloc", "chip_degrid_plain",
     "chip_finish_products", "chip_model_to_grid", "chip_grid_to_model_adjoint", "chip_predict_model",
     "chip_gridder_attach_aw", "chip_gridder_attach_aw_airy", "chip_gridder_attach_aw_airy_freqs",
@@ -277,6 +277,7 @@ def load():
         "chip_vset_item_order": (ctypes.c_int, [_P, _U32, _P, _PP]),
         "chip_vset_schedule": (ctypes.c_int, [_P, _U32, _U32, _U32, _U32, _PP]),
         "chip_vset_wave_groups": (ctypes.c_int, [_P, _U32, _P, _PP]),
+        "chip_vset_two_phase": (ctypes.c_int, [_P, ctypes.c_int, _PP]),
         "chip_hogbom": (ctypes.c_int, [_P, ctypes.POINTER(HogbomParams), _U64, _U64, _U64, _P, _U64, _U64, _P, _P,
                                        _P, _P, _P, _P, _P, _PP]),
         "chip_vset_schedule_info": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P, _PP]),

@@ -700,6 +700,13 @@ int chip_vset_wave_groups(chip_vset *vs, uint32_t groups, uint32_t *in_use, char
     });
 }
 
+int chip_vset_two_phase(chip_vset *vs, int enable, char **err) {
+    return guarded(err, [&] {
+        if (!vs) fail(CHIP_ERR_INVALID, "NULL visibility set");
+        vs->wide = enable != 0;
+    });
+}
+
 int chip_vset_schedule(chip_vset *vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t groups_per_item,
                        char **err) {
     return guarded(err, [&] {

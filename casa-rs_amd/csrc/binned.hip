@@ -102,6 +102,69 @@ __device__ __forceinline__ void div_conj_rcp(float &re, float &im, float nr, flo
     im = (b * c - a * d) * inv;
 }
 
+// Model value of grid cell (gxx, gyy) of one channel for an LDS tile: zero
+// off the grid; read through the half-spectrum mirror (conjugated) and the
+// checkerboard sign of the major cycle's grid layout.
+__device__ __forceinline__ float2 tile_model(const GridParams &gp, const float2 *__restrict__ model,
+                                             uint64_t chan_off, int gxx, int gyy) {
+    float2 m = make_float2(0.0f, 0.0f);
+    if (!model || gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy)) return m;
+    if (gp.model_half && gyy > static_cast<int>(gp.hy)) {
+        const uint32_t mx = gxx == 0 ? 0u : gp.gx - static_cast<uint32_t>(gxx);
+        m = model[chan_off + cell_offset(gp, mx, gp.gy - static_cast<uint32_t>(gyy))];
+        m.y = -m.y;
+    } else {
+        m = model[chan_off + cell_offset(gp, gxx, gyy)];
+    }
+    if (gp.model_signed && ((gxx + gyy + gp.sign_base) & 1)) {
+        m.x = -m.x;
+        m.y = -m.y;
+    }
+    return m;
+}
+
+// Adds the non-zero cells of an LDS tile (origin (x0, y0), L x Ls cells) to
+// the global grids with float atomics; with acc_hermitian the accumulator
+// goes in as 2H (the cell and, for stored mirror columns, its conjugate at
+// (-x, -y)).
+template <bool kPsf, bool kAcc>
+__device__ __forceinline__ void flush_tile(const GridParams &gp, uint64_t chan_off, int x0, int y0, int L, int Ls,
+                                           const float *P_re, const float *P_im, const float *A_re,
+                                           const float *A_im, float *psf, float *acc_out) {
+    const int cells = L * Ls;
+    for (int c = threadIdx.x; c < cells; c += blockDim.x) {
+        const int lx = c / Ls, ly = c - lx * Ls;
+        const int gxx = x0 + lx, gyy = y0 + ly;
+        if (ly >= L || gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy))
+            continue;
+        const uint64_t gi = 2 * (chan_off + cell_offset(gp, gxx, gyy));
+        if (kPsf) {
+            const float r = P_re[c], i = P_im[c];
+            if (r != 0.0f) unsafeAtomicAdd(psf + gi, r);
+            if (i != 0.0f) unsafeAtomicAdd(psf + gi + 1, i);
+        }
+        if (kAcc) {
+            const float r = A_re[c], i = A_im[c];
+            if (!gp.acc_hermitian) {
+                if (r != 0.0f) unsafeAtomicAdd(acc_out + gi, r);
+                if (i != 0.0f) unsafeAtomicAdd(acc_out + gi + 1, i);
+            } else if (r != 0.0f || i != 0.0f) {
+                const uint32_t hy = gp.hy, ux = static_cast<uint32_t>(gxx), uy = static_cast<uint32_t>(gyy);
+                const uint32_t mx = ux == 0 ? 0u : gp.gx - ux;
+                if (uy <= hy) {
+                    unsafeAtomicAdd(acc_out + gi, r);
+                    unsafeAtomicAdd(acc_out + gi + 1, i);
+                }
+                if (uy == 0 || uy >= hy) {  // conj v to (-x, -y) when that is a stored column
+                    const uint64_t mi = 2 * (chan_off + cell_offset(gp, mx, uy == 0 ? 0u : gp.gy - uy));
+                    unsafeAtomicAdd(acc_out + mi, r);
+                    unsafeAtomicAdd(acc_out + mi + 1, -i);
+                }
+            }
+        }
+    }
+}
+
 // Register slots per lane for a footprint side M = 2 halo + 1 with 64 lanes:
 // ceil(M / floor(64 / M)) rounded up to an instantiated count (1, 2, 4, 7,
 // 16), 0 (LDS-atomic chunks) beyond 16 (halo > 15).
@@ -176,23 +239,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     for (int c = tid; c < cells; c += nth) {
         if (kModel) {
             const int lx = c / Ls, ly = c - lx * Ls;
-            const int gxx = x0 + lx, gyy = y0 + ly;
-            float2 m = make_float2(0.0f, 0.0f);
-            if (model && ly < L && gxx >= 0 && gyy >= 0 && gxx < static_cast<int>(gp.gx) &&
-                gyy < static_cast<int>(gp.gy)) {
-                if (gp.model_half && gyy > static_cast<int>(gp.hy)) {
-                    const uint32_t mx = gxx == 0 ? 0u : gp.gx - static_cast<uint32_t>(gxx);
-                    m = model[chan_off + cell_offset(gp, mx, gp.gy - static_cast<uint32_t>(gyy))];
-                    m.y = -m.y;
-                } else {
-                    m = model[chan_off + cell_offset(gp, gxx, gyy)];
-                }
-                if (gp.model_signed && ((gxx + gyy + gp.sign_base) & 1)) {
-                    m.x = -m.x;
-                    m.y = -m.y;
-                }
-            }
-            M2[c] = m;
+            M2[c] = ly < L ? tile_model(gp, model, chan_off, x0 + lx, y0 + ly) : make_float2(0.0f, 0.0f);
         }
         if (kAcc) { A_re[c] = 0.0f; A_im[c] = 0.0f; }
         if (kPsf) { P_re[c] = 0.0f; P_im[c] = 0.0f; }
@@ -657,37 +704,210 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     __syncthreads();
 
     // Flush the non-zero cells of the LDS tile to the global grid.
-    for (int c = tid; c < cells; c += nth) {
+    flush_tile<kPsf, kAcc>(gp, chan_off, x0, y0, L, Ls, P_re, P_im, A_re, A_im, psf, acc_out);
+}
+
+// Residual refresh of work items whose largest half-support is 16 - 31
+// (footprint sides M = 33 - 63).  One lane per footprint row leaves a single
+// wavefront no room for register slots (up to 63 x residues per lane), and
+// per-tap LDS atomics (the chunked path) bound the time, so the item's
+// groups go through two phases:
+//   1. predictions: the wavefronts split the groups; a group's prediction is
+//      a chunked sum over its taps and the staged model (LDS reads only),
+//      and its residual value sum w V - p sum w goes to an LDS array;
+//   2. gridding: the wavefronts form sets of W = wset that split the x
+//      residues (wavefront j of a set owns residues j kWideSlots + s, lane l
+//      the row residue l), and each set walks a contiguous run of the item's
+//      groups: every wavefront sees each group of its run, accumulates its
+//      slots in registers and spills a slot to the LDS tile (two atomics)
+//      only when the slot's cell changes (Romein's work distribution with
+//      the residues spread over wavefronts instead of lanes).
+// The tile is then flushed as in k_binned.  Values of the taps and sums are
+// those of the residual mode of k_binned.
+constexpr int kWideSlots = 16;
+constexpr int kWideSets = 2;
+
+__global__ __launch_bounds__(kBinnedThreads) void k_binned_wide(GridParams gp, const WorkItem *__restrict__ items,
+                                                               const uint4 *__restrict__ grec,
+                                                               const float4 *__restrict__ gsum,
+                                                               const float2 *__restrict__ model, float *acc_out,
+                                                               uint32_t pred_cap, uint32_t wset) {
+    extern __shared__ float lds[];
+    const WorkItem it = items[blockIdx.x];
+    if (it.count == 0 || it.count > pred_cap) return;  // padding of the XCD-aligned item order
+    const int L = static_cast<int>(gp.L);
+    const int Ls = static_cast<int>(gp.Ls);
+    const int cells = L * Ls;
+    float2 *M2 = reinterpret_cast<float2 *>(lds);
+    float *A_re = lds + 2 * cells;
+    float *A_im = A_re + cells;
+    float2 *pred = reinterpret_cast<float2 *>(A_im + cells);
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nwaves = static_cast<int>(blockDim.x) >> 6;
+    const uint32_t tile_x = it.tile / gp.tiles_y;
+    const uint32_t tile_y = it.tile - tile_x * gp.tiles_y;
+    const int x0 = static_cast<int>(tile_x * gp.tile) - static_cast<int>(gp.halo);
+    const int y0 = static_cast<int>(tile_y * gp.tile) - static_cast<int>(gp.halo);
+    const uint64_t chan_off = static_cast<uint64_t>(it.chan & 0xffffffu) * gp.cstride;
+    for (int c = tid; c < cells; c += static_cast<int>(blockDim.x)) {
         const int lx = c / Ls, ly = c - lx * Ls;
-        const int gxx = x0 + lx, gyy = y0 + ly;
-        if (ly >= L || gxx < 0 || gyy < 0 || gxx >= static_cast<int>(gp.gx) || gyy >= static_cast<int>(gp.gy))
-            continue;
-        const uint64_t gi = 2 * (chan_off + cell_offset(gp, gxx, gyy));
-        if (kPsf) {
-            const float r = P_re[c], i = P_im[c];
-            if (r != 0.0f) unsafeAtomicAdd(psf + gi, r);
-            if (i != 0.0f) unsafeAtomicAdd(psf + gi + 1, i);
-        }
-        if (kAcc) {
-            const float r = A_re[c], i = A_im[c];
-            if (!gp.acc_hermitian) {
-                if (r != 0.0f) unsafeAtomicAdd(acc_out + gi, r);
-                if (i != 0.0f) unsafeAtomicAdd(acc_out + gi + 1, i);
-            } else if (r != 0.0f || i != 0.0f) {
-                const uint32_t hy = gp.hy, ux = static_cast<uint32_t>(gxx), uy = static_cast<uint32_t>(gyy);
-                const uint32_t mx = ux == 0 ? 0u : gp.gx - ux;
-                if (uy <= hy) {
-                    unsafeAtomicAdd(acc_out + gi, r);
-                    unsafeAtomicAdd(acc_out + gi + 1, i);
+        M2[c] = ly < L ? tile_model(gp, model, chan_off, x0 + lx, y0 + ly) : make_float2(0.0f, 0.0f);
+        A_re[c] = 0.0f;
+        A_im[c] = 0.0f;
+    }
+    __syncthreads();
+
+    struct Foot {
+        int side, bx, by;
+        const float2 *ktap;
+        float csign;
+        uint32_t po;
+    };
+    auto footprint = [&](uint32_t anchor, uint32_t word) {
+        // the group record's fourth word: tap-table index (25 bits),
+        // conjugation (bit 25), half-support (bits 26-31)
+        Foot f;
+        f.po = word & 0x1ffffffu;
+        f.side = 2 * static_cast<int>(word >> 26) + 1;
+        const uint64_t local = f.po - gp.plane_lo * gp.n_off * gp.n_off;
+        f.ktap = gp.compact + local * gp.tap_stride;
+        f.csign = ((word >> 25) & 1u) ? -1.0f : 1.0f;
+        f.bx = static_cast<int>(anchor & 0xffffu) - x0 - (f.side >> 1);
+        f.by = static_cast<int>(anchor >> 16) - y0 - (f.side >> 1);
+        return f;
+    };
+    auto rl = [](uint32_t x, uint32_t k) {
+        return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(x), static_cast<int>(k)));
+    };
+
+    // 1. predictions and residual values, groups split over the wavefronts
+    {
+        const uint32_t per = (it.count + nwaves - 1) / nwaves;
+        const uint32_t b = it.start + static_cast<uint32_t>(wave) * per;
+        const uint32_t e = min(it.start + it.count, b + per);
+        for (uint32_t base = b; base < e; base += 64) {
+            const uint32_t gl = base + static_cast<uint32_t>(lane);
+            const uint4 rw = gl < e ? grec[gl] : make_uint4(0u, 0u, 0u, 0u);
+            const float4 sw = gl < e ? gsum[gl] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const uint32_t n = min(64u, e - base);
+            for (uint32_t k = 0; k < n; ++k) {
+                const float w = __uint_as_float(rl(__float_as_uint(sw.x), k));
+                float2 v = make_float2(0.0f, 0.0f);
+                if (w != 0.0f) {
+                    const Foot f = footprint(rl(rw.z, k), rl(rw.w, k));
+                    const int T = f.side * f.side;
+                    const float inv = 1.0f / static_cast<float>(f.side);
+                    float pr = 0.0f, pi = 0.0f;
+                    for (int c0 = 0; c0 < T; c0 += 64 * kBig) {
+                        float2 tv[kBig];
+#pragma unroll
+                        for (int q = 0; q < kBig; ++q) {
+                            const int t = c0 + 64 * q + lane;
+                            tv[q] = t < T ? f.ktap[t] : make_float2(0.0f, 0.0f);
+                        }
+#pragma unroll
+                        for (int q = 0; q < kBig; ++q) {
+                            const int t = c0 + 64 * q + lane;
+                            if (t < T) {
+                                const int tx = static_cast<int>((static_cast<float>(t) + 0.5f) * inv);
+                                const float2 m = M2[(f.bx + tx) * Ls + (f.by + t - tx * f.side)];
+                                const float cr = tv[q].x, ci = -(tv[q].y * f.csign);
+                                pr += cr * m.x - ci * m.y;
+                                pi += cr * m.y + ci * m.x;
+                            }
+                        }
+                    }
+                    wave_sum_f2(pr, pi);
+                    if (gp.aw_norm) {
+                        const float2 awn = gp.aw_norm[f.po];
+                        div_conj<float>(pr, pi, awn.x, awn.y);
+                    }
+                    v = make_float2(__uint_as_float(rl(__float_as_uint(sw.y), k)) - pr * w,
+                                    __uint_as_float(rl(__float_as_uint(sw.z), k)) - pi * w);
                 }
-                if (uy == 0 || uy >= hy) {  // conj v to (-x, -y) when that is a stored column
-                    const uint64_t mi = 2 * (chan_off + cell_offset(gp, mx, uy == 0 ? 0u : gp.gy - uy));
-                    unsafeAtomicAdd(acc_out + mi, r);
-                    unsafeAtomicAdd(acc_out + mi + 1, -i);
-                }
+                if (lane == 0) pred[base + k - it.start] = v;
             }
         }
     }
+    __syncthreads();
+
+    // 2. gridding: x residues split over the wavefronts of a set, groups over
+    // the sets
+    const int W = static_cast<int>(wset);
+    const int sets = nwaves / W;
+    const int set = wave / W;
+    if (set < sets) {
+        const int xw = wave - set * W;
+        const int M = 2 * min(static_cast<int>(it.chan >> 24), static_cast<int>(gp.halo)) + 1;
+        const float invM = 1.0f / static_cast<float>(M);
+        const int r0 = xw * kWideSlots;  // first x residue of this wavefront
+        const uint32_t per = (it.count + sets - 1) / sets;
+        const uint32_t b = it.start + static_cast<uint32_t>(set) * per;
+        const uint32_t e = min(it.start + it.count, b + per);
+        int cell_k[kWideSlots];
+        float accr[kWideSlots], acci[kWideSlots];
+#pragma unroll
+        for (int s = 0; s < kWideSlots; ++s) {
+            cell_k[s] = -1;
+            accr[s] = acci[s] = 0.0f;
+        }
+        auto spill = [&](int s) {
+            if (cell_k[s] >= 0) {
+                if (accr[s] != 0.0f) atomicAdd(A_re + cell_k[s], accr[s]);
+                if (acci[s] != 0.0f) atomicAdd(A_im + cell_k[s], acci[s]);
+            }
+            accr[s] = acci[s] = 0.0f;
+        };
+        for (uint32_t base = b; base < e; base += 64) {
+            const uint32_t gl = base + static_cast<uint32_t>(lane);
+            const uint4 rw = gl < e ? grec[gl] : make_uint4(0u, 0u, 0u, 0u);
+            const float2 vw = gl < e ? pred[gl - it.start] : make_float2(0.0f, 0.0f);
+            const uint32_t n = min(64u, e - base);
+            for (uint32_t k = 0; k < n; ++k) {
+                const float vr = __uint_as_float(rl(__float_as_uint(vw.x), k));
+                const float vi = __uint_as_float(rl(__float_as_uint(vw.y), k));
+                if (vr == 0.0f && vi == 0.0f) continue;
+                const Foot f = footprint(rl(rw.z, k), rl(rw.w, k));
+                // this lane's row: y offset (lane - by) mod M; bx, by >= 0
+                const int qy = static_cast<int>((static_cast<float>(f.by - lane + M - 1) + 0.5f) * invM);
+                const int jj = lane + qy * M - f.by;
+                const bool row_ok = lane < M && jj < f.side;
+                const int cy = f.by + jj;
+                // x offset of the first residue: (r0 - bx) mod M (wavefront-uniform)
+                const int qx = static_cast<int>((static_cast<float>(f.bx - r0 + M - 1) + 0.5f) * invM);
+                int i = __builtin_amdgcn_readfirstlane(r0 + qx * M - f.bx);
+                const float sr = vr, si = vi * f.csign, tr = vr * f.csign;  // value times conj(tap) sign
+                float2 tv[kWideSlots];
+                int cell[kWideSlots];
+#pragma unroll
+                for (int s = 0; s < kWideSlots; ++s) {
+                    const bool on = r0 + s < M && i < f.side;
+                    cell[s] = (on && row_ok) ? (f.bx + i) * Ls + cy : -1;
+                    tv[s] = (on && row_ok) ? f.ktap[i * f.side + jj] : make_float2(0.0f, 0.0f);
+                    i = i + 1 == M ? 0 : i + 1;
+                }
+#pragma unroll
+                for (int s = 0; s < kWideSlots; ++s) {
+                    if (cell[s] < 0) continue;
+                    if (cell[s] != cell_k[s]) {
+                        spill(s);
+                        cell_k[s] = cell[s];
+                    }
+                    // value * tap, the tap conjugated when csign = -1
+                    accr[s] = fmaf(sr, tv[s].x, accr[s]);
+                    accr[s] = fmaf(-si, tv[s].y, accr[s]);
+                    acci[s] = fmaf(tr, tv[s].y, acci[s]);
+                    acci[s] = fmaf(vi, tv[s].x, acci[s]);
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kWideSlots; ++s) spill(s);
+    }
+    __syncthreads();
+    flush_tile<false, true>(gp, chan_off, x0, y0, L, Ls, nullptr, nullptr, A_re, A_im, nullptr, acc_out);
 }
 
 // The stream is binned for the W-table residency the gridder had when the set
@@ -766,6 +986,26 @@ bool binned_two_groups_allowed(const chip_vset &vs) {
     return false;
 }
 
+// Residual range of half-supports 16 - 31: the two-phase kernel, when its
+// LDS (tile, accumulators and one value per group) fits.
+static bool launch_wide(chip_vset &vs, const GridParams &gp, const float2 *model, float *acc, hipStream_t st,
+                        uint64_t first, uint64_t count) {
+    if (gp.halo < 16 || gp.halo > 31) return false;
+    const uint32_t cap = vs.groups_per_item;
+    const size_t lds = 4 * static_cast<size_t>(gp.L) * gp.Ls * sizeof(float) + static_cast<size_t>(cap) * sizeof(float2);
+    const uint32_t wset = (2 * gp.halo + 1 + kWideSlots - 1) / kWideSlots;
+    const uint32_t threads = 64 * wset * kWideSets;
+    if (lds > lds_limit(vs.device) || threads > static_cast<uint32_t>(kBinnedThreads)) return false;
+    if (lds > 64 * 1024)
+        CHIP_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_binned_wide),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    hipLaunchKernelGGL(k_binned_wide, dim3(static_cast<unsigned>(count)), dim3(threads), lds, st, gp,
+                       vs.items.as<WorkItem>() + first, vs.grec.as<uint4>(), vs.gsum_r.as<float4>(), model, acc,
+                       cap, wset);
+    CHIP_HIP(hipGetLastError());
+    return true;
+}
+
 // One range of work items with the kernel its halo needs.
 template <int MODE, bool WP>
 static void launch_range(chip_vset &vs, const GridParams &gp, const float2 *values, const float2 *model,
@@ -774,6 +1014,9 @@ static void launch_range(chip_vset &vs, const GridParams &gp, const float2 *valu
     if (lds > lds_limit(vs.device))
         fail(CHIP_ERR_INVALID, "kernel support too large for the LDS tile (halo " + std::to_string(gp.halo) + ")");
     const int ks = register_slots(gp.halo);
+    if constexpr (MODE == kModeResidual && WP) {
+        if (ks == 0 && vs.wide && launch_wide(vs, gp, model, acc, st, first, count)) return;
+    }
     const int ks2 = wave_groups_for(vs, gp.halo) == 2 ? register_slots_half(gp.halo) : 0;
 #define CHIP_RANGE(KS_, GPW_) \
     launch_binned_k<MODE, WP, KS_, GPW_>(vs, gp, values, model, psf, acc, vis_out, lds, st, first, count)

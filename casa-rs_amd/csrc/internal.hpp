@@ -402,6 +402,9 @@ struct chip_vset {
     // one class.
     std::vector<uint32_t> range_first;
     uint32_t n_classes = 1;
+    // residual launches of half-supports 16 - 31 take the two-phase kernel
+    // (chip_vset_two_phase); false: the chunked LDS-atomic path
+    bool wide = true;
     // bins (plane chunk, channel, tile): groups per bin and first group, kept
     // so that the schedule autotuner can split the work items again
     chip::DevBuf bin_groups, bin_first;

@@ -1035,6 +1035,11 @@ class VisibilitySet:
         abi.call("chip_vset_wave_groups", self.handle, int(groups), ctypes.byref(used))
         return int(used.value)
 
+    def set_two_phase(self, enable: bool = True):
+        """Residual refreshes of half-supports 16 - 31: the two-phase kernel
+        (default) or the chunked LDS-atomic kernel."""
+        abi.call("chip_vset_two_phase", self.handle, 1 if enable else 0)
+
     def schedule(self):
         """Schedule in use and the autotuner's timed candidates
         [(tile, order, threads, groups, ms)]."""

@@ -618,6 +618,15 @@ int chip_vset_cycle_strategy(chip_vset *vs, int strategy, char **err);
  * Results agree within FP32 accumulation order.  No reference counterpart
  * (a launch-shape knob of the batched kernel). */
 int chip_vset_wave_groups(chip_vset *vs, uint32_t groups, uint32_t *in_use, char **err);
+/* Residual refreshes of work items whose largest half-support is 16 - 31
+ * (footprint sides 33 - 63) run a two-phase kernel by default: predictions of
+ * the item's groups first, then gridding with the footprint's x residues
+ * spread over wavefronts and register accumulation.  enable = 0 selects the
+ * chunked kernel with per-tap LDS atomics instead (both agree within FP32
+ * accumulation order).  Other modes and larger supports always take the
+ * chunked kernel.  No reference counterpart (a kernel choice of the batched
+ * path). */
+int chip_vset_two_phase(chip_vset *vs, int enable, char **err);
 int chip_vset_schedule(chip_vset *vs, uint32_t tile, uint32_t order, uint32_t threads, uint32_t groups_per_item,
                        char **err);
 /* Schedule in use and, after autotuning, the timed candidates: n_timed

@@ -164,6 +164,58 @@ def test_w_large_supports(gpu, oracle_mod, nx, arcsec, planes, sampling):
     assert rel(res[0], ref_res) < 2e-5
 
 
+def test_two_phase_large_supports(gpu, oracle_mod):
+    """Residual refreshes of half-supports 16 - 28 (supports 4..28: both
+    large classes within the two-phase kernel's range) against the oracle,
+    and the two-phase kernel against the chunked LDS-atomic kernel on the
+    same set (grids and major cycle within FP32 accumulation order)."""
+    nx, arcsec, planes, sampling = 384, 240.0, 64, 8
+    cell = np.deg2rad(arcsec / 3600.0)
+    g = gpu.StandardGridder(nx, nx, cell, cell)
+    g.attach_wproject(1.0, planes=planes, sampling=sampling)
+    sup, ker, nt = g.wproject_tables()
+    assert 23 <= sup.max() <= 31 and (sup >= 16).sum() >= 8
+    o = oracle_mod.StandardGridder(nx, nx, cell, cell)
+    wp = oracle_mod.WProjector(o, 1.0, explicit_planes=planes, sampling=sampling).with_kernels(ker, sup, nt)
+    max_w = (planes - 1) ** 2 / g.info.w_scale
+    umax = 0.5 * g.info.gx * g.info.du
+    u, v, w, wt, sf, vis, gr = synth.random_samples(nx + 1, 30000, umax, wmax=1.2 * max_w)
+    u[:10000] *= 0.2
+    v[:10000] *= 0.2
+    b = batch(gpu, u, v, w, wt, sf, vis, gr)
+    plan, _ = g.plan(b)
+    used = plan[:, 4].cpu().numpy()[plan[:, 0].cpu().numpy() >= 0]
+    assert (sup[used] >= 23).sum() > 1000 and ((sup[used] >= 16) & (sup[used] < 23)).sum() > 1000
+
+    rng = np.random.default_rng(nx)
+    model = np.zeros((nx, nx), np.float32)
+    model[rng.integers(40, nx - 40, 25), rng.integers(40, nx - 40, 25)] = rng.uniform(0.1, 1.0, 25)
+    gm = g.model_to_grid(t(model))
+    opsf, _, osums = wp.grid(u, v, w, wt, sf, vis, gridable=gr, mode=1)
+    _, ores, _ = wp.grid(u, v, w, wt, sf, vis, gridable=gr, mode=2, model=wp.model_grid(model))
+    vs = g.visibility_set(b)
+    s = vs.sums[0]
+    res = {}
+    for two_phase in (True, False):
+        vs.set_two_phase(two_phase)
+        r32 = g.new_grids(1)
+        vs.grid(gpu.MODE_RESIDUAL, model=gm, dirty=r32)
+        res[two_phase] = r32[0].cpu().numpy()
+        assert rel(res[two_phase], ores) < 1e-5, two_phase
+    assert rel(res[True], res[False]) < 1e-5
+
+    psf32 = g.new_grids(1)
+    vs.grid(gpu.MODE_PSF, psf=psf32)
+    _, _, peak = g.finish_products(psf32, None, s["normalization_sumwt"])
+    _, ref_res, _ = wp.dirty_products(opsf, ores, osums)
+    cyc = {}
+    for two_phase in (True, False):
+        vs.set_two_phase(two_phase)
+        cyc[two_phase] = vs.major_cycle(t(model[None]), s["normalization_sumwt"], peak).cpu().numpy()[0]
+        assert rel(cyc[two_phase], ref_res) < 2e-5, two_phase
+    assert rel(cyc[True], cyc[False]) < 1e-5
+
+
 # --------------------------------------------------------------------------
 # differentiable imaging at size
 # --------------------------------------------------------------------------
