@@ -494,14 +494,20 @@ def reduce_traffic(fetch_dir, write_dir, out_json, out_txt, commands):
             matches = [(kn, v) for kn, v in per_counter[name].items() if k + "<" in kn or k + "(" in kn]
             if k == "k_binned":
                 # the residual pass of a step is one launch per support class:
-                # every residual-mode instantiation, summed
-                matches = [(kn, v) for kn, v in matches if "k_binned<2," in kn]
+                # every residual-mode instantiation and the two-phase kernel of
+                # the largest supports, summed
+                matches = [(kn, v) for kn, v in per_counter[name].items()
+                           if "k_binned<2," in kn or "k_binned_wide(" in kn]
             if not matches:
                 break
             if k == "k_binned":
+                # per step: every dispatch of every symbol over the number of
+                # steps profiled (the dispatch count of a symbol launched once
+                # per step; a symbol that serves two support classes is
+                # launched twice per step)
                 kname = " + ".join(sorted(kn[:60] for kn, _ in matches))
-                raw = sum(sum(v) / len(v) for _, v in matches)
                 n_disp = min(len(v) for _, v in matches)
+                raw = sum(sum(v) for _, v in matches) / n_disp
             else:
                 kname, vals = max(matches, key=lambda kv: len(kv[1]))
                 raw = sum(vals) / len(vals)
@@ -523,11 +529,13 @@ def reduce_traffic(fetch_dir, write_dir, out_json, out_txt, commands):
 
 
 def reduce_counters(directory, kernel, out_txt, command, kernel_ms=None):
-    """rocprofv3 --pmc run -> per-counter mean over the dispatches of `kernel`,
-    with the FP32 operation rate when the FP32 instruction counters are there."""
+    """rocprofv3 --pmc run -> per-counter mean over the dispatches of `kernel`
+    (substrings separated by '|': any of them), with the FP32 operation rate
+    when the FP32 instruction counters are there."""
     per = {}  # counter -> kernel -> values
+    keys = kernel.split("|")
     for r in _profile_csv(directory, "counter_collection.csv"):
-        if kernel in r["Kernel_Name"]:
+        if any(k in r["Kernel_Name"] for k in keys):
             per.setdefault(r["Counter_Name"], {}).setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
     if not per:
         raise SystemExit(f"no dispatches of {kernel}")
@@ -535,8 +543,9 @@ def reduce_counters(directory, kernel, out_txt, command, kernel_ms=None):
     lines = [f"command: {command}", "", f"kernel: {kernel} (per step: the sum over {len(names)} instantiation(s), "
                                         "one launch per support class)"]
     lines += [f"  {kn[:110]}" for kn in names]
-    # per step: the mean per dispatch of every instantiation, summed
-    mean = {c: sum(sum(v) / len(v) for v in d.values()) for c, d in per.items()}
+    # per step: every dispatch of every instantiation over the number of steps
+    # profiled (the dispatch count of an instantiation launched once per step)
+    mean = {c: sum(sum(v) for v in d.values()) / min(len(v) for v in d.values()) for c, d in per.items()}
     for k in sorted(mean):
         n = min(len(v) for v in per[k].values())
         lines.append(f"  {k:28s} {mean[k]:16.4f}   ({n} dispatches each)")
