@@ -385,33 +385,29 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
         int i = lane_pg + qx * Mside - f.bx;
         const bool in_y = jj < f.side;
         const int cy = f.by + jj;
-        // the slot's LDS cell and tap byte offset, advanced with i (no
-        // per-slot multiplies); taps load from the group's table base plus a
-        // 32-bit offset
-        int cell = (f.bx + i) * Ls + cy;
-        uint32_t toff = static_cast<uint32_t>(i * f.side + jj) * 8u;
-        const int step_c = Pl * Ls, wrap_c = Mside * Ls;
-        const uint32_t step_t = static_cast<uint32_t>(Pl * f.side) * 8u, wrap_t = static_cast<uint32_t>(Mside * f.side) * 8u;
+        // Per slot: the x offset advances by P and wraps past the footprint
+        // side as min(i, i - M) in unsigned arithmetic (i < 2M), and the LDS
+        // cell and the tap's byte offset from the group's table base follow
+        // by one 24-bit multiply-add each from per-group bases (no selects,
+        // no carried offsets).
+        const uint32_t cbase = static_cast<uint32_t>(f.bx * Ls + cy);
+        const uint32_t tbase = static_cast<uint32_t>(jj) * 8u;
+        const uint32_t side8 = static_cast<uint32_t>(f.side) * 8u;
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
-            d.cell[k] = -1;
-            d.tap[k] = make_float2(0.0f, 0.0f);
+            d.cell[k] = -1;  // (the tap of a slot without a cell is never read)
             if (k > 0) {
                 i += Pl;
-                cell += step_c;
-                toff += step_t;
-                if (i >= Mside) {
-                    i -= Mside;
-                    cell -= wrap_c;
-                    toff -= wrap_t;
-                }
+                i = static_cast<int>(min(static_cast<uint32_t>(i), static_cast<uint32_t>(i - Mside)));
             }
             if (!((slot_mask >> k) & 1u) || !in_y || i >= f.side || !valid) continue;
             if (kChains) {
-                d.cell[k] = cell;
-                d.tap[k] = WP ? *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(f.ktap) + toff)
+                const uint32_t ui = static_cast<uint32_t>(i);
+                d.cell[k] = static_cast<int>(cbase + __umul24(ui, static_cast<uint32_t>(Ls)));
+                d.tap[k] = WP ? *reinterpret_cast<const float2 *>(reinterpret_cast<const char *>(f.ktap) +
+                                                                   (tbase + __umul24(ui, side8)))
                               : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
-            } else {  // (few slots: the multiplies cost less than the carried offsets)
+            } else {  // (few slots: plain index arithmetic)
                 d.cell[k] = (f.bx + i) * Ls + cy;
                 d.tap[k] = WP ? f.ktap[i * f.side + jj] : make_float2(f.wxr[i] * f.wyr[jj], 0.0f);
             }
