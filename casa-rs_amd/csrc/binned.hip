@@ -271,6 +271,10 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
     // multiply-add chains issue fewer instructions; with one or two slots the
     // plain forms measured faster.
     constexpr bool kChains = NS > 2;
+    // Slots keep their cell's model value in registers, except with 16 slots,
+    // where those 32 registers cost a wavefront per SIMD and the model is read
+    // from LDS at every group instead.
+    constexpr bool kStageModel = NS < 16;
     const int Mside = 2 * item_halo + 1;
     const float invM = 1.0f / static_cast<float>(Mside);
     const int Pl = LW / Mside;
@@ -565,7 +569,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             if (cell != cell_k[k]) {
                 spill(k);
                 cell_k[k] = cell;
-                if (kModel) {
+                if (kModel && kStageModel) {
                     const float2 mc = M2[cell];
                     mre[k] = mc.x;
                     mim[k] = mc.y;
@@ -573,6 +577,11 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
             }
             cr[k] = d.tap[k].x;
             ci[k] = d.tap[k].y * csign;
+            if (kModel && !kStageModel) {  // read at every group: registers for occupancy
+                const float2 mc = M2[cell];
+                mre[k] = mc.x;
+                mim[k] = mc.y;
+            }
             if (kModel) {
                 // conj(cwt) * model (many slots: fused multiply-add chains,
                 // fewer instructions; few slots: shorter dependency chains)
