@@ -721,12 +721,13 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned(GridParams gp, const 
 //      a chunked sum over its taps and the staged model (LDS reads only),
 //      and its residual value sum w V - p sum w goes to an LDS array;
 //   2. gridding: the wavefronts form sets of W = wset that split the x
-//      residues (wavefront j of a set owns residues j kWideSlots + s, lane l
-//      the row residue l), and each set walks a contiguous run of the item's
-//      groups: every wavefront sees each group of its run, accumulates its
-//      slots in registers and spills a slot to the LDS tile (two atomics)
-//      only when the slot's cell changes (Romein's work distribution with
-//      the residues spread over wavefronts instead of lanes).
+//      residues (wavefront j of a set owns residues j S + s, s < S =
+//      ceil(M / W) <= kWideSlots, lane l the row residue l), and each set
+//      walks a contiguous run of the item's groups: every wavefront sees
+//      each group of its run, accumulates its slots in registers and spills
+//      a slot to the LDS tile (two atomics) only when the slot's cell
+//      changes (Romein's work distribution with the residues spread over
+//      wavefronts instead of lanes).
 // The tile is then flushed as in k_binned.  Values of the taps and sums are
 // those of the residual mode of k_binned.
 constexpr int kWideSlots = 16;
@@ -847,7 +848,10 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned_wide(GridParams gp, c
         const int xw = wave - set * W;
         const int M = 2 * min(static_cast<int>(it.chan >> 24), static_cast<int>(gp.halo)) + 1;
         const float invM = 1.0f / static_cast<float>(M);
-        const int r0 = xw * kWideSlots;  // first x residue of this wavefront
+        // the item's M residues dealt evenly over the W wavefronts of a set
+        // (at most kWideSlots each: W kWideSlots >= the launch's M >= M)
+        const int Sw = (M + W - 1) / W;
+        const int r0 = xw * Sw;  // first x residue of this wavefront
         const uint32_t per = (it.count + sets - 1) / sets;
         const uint32_t b = it.start + static_cast<uint32_t>(set) * per;
         const uint32_t e = min(it.start + it.count, b + per);
@@ -888,7 +892,7 @@ __global__ __launch_bounds__(kBinnedThreads) void k_binned_wide(GridParams gp, c
                 int cell[kWideSlots];
 #pragma unroll
                 for (int s = 0; s < kWideSlots; ++s) {
-                    const bool on = r0 + s < M && i < f.side;
+                    const bool on = s < Sw && r0 + s < M && i < f.side;
                     cell[s] = (on && row_ok) ? (f.bx + i) * Ls + cy : -1;
                     tv[s] = (on && row_ok) ? f.ktap[i * f.side + jj] : make_float2(0.0f, 0.0f);
                     i = i + 1 == M ? 0 : i + 1;
