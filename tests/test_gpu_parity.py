@@ -509,13 +509,15 @@ def test_major_cycle_rejects_bad_normalization(gpu, oracle_mod):
         vs.major_cycle(torch.zeros((2, g.nx, g.ny)), 1.0, 1.0)
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 64), (96, 64), (200, 150), (520, 512), (3413, 1707), (2400, 3413)])
+@pytest.mark.parametrize("nx,ny", [(64, 64), (96, 64), (200, 150), (520, 512), (1400, 1400), (3413, 1707),
+                                   (2400, 3413)])
 def test_major_cycle_fused_shapes(gpu, oracle_mod, nx, ny):
     """Fused transforms on square and rectangular grids whose lengths use
     every radix (2, 3, 4, 5): the residual image against the oracle (FP64
-    grids and FFT) to 1e-5 of peak, and against the hipFFT strategy.  The two
-    large shapes have axes above 2048 points (4096 x 2048 and 2880 x 4096
-    grids): 128 threads per transform and half-block column workgroups."""
+    grids and FFT) to 1e-5 of peak, and against the hipFFT strategy.  The
+    three large shapes have axes above 1536 points (1728 x 1728, 4096 x 2048
+    and 2880 x 4096 grids): 128 threads per transform and half-block column
+    workgroups."""
     cell = np.deg2rad(5.0 / 3600.0)
     g = gpu.StandardGridder(nx, ny, cell, cell)
     o = oracle_mod.StandardGridder(nx, ny, cell, cell)
