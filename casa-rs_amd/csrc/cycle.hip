@@ -219,25 +219,41 @@ void autotune_schedule(chip_vset &vs, MakeGp make_gp, const float2 *M, float *Rg
     vset_rebin(vs, best_tile, best_order);
     vset_split_items(vs, best_g);
     vs.threads = best_t;
-    // two groups per wavefront, timed at the chosen schedule (recorded with
-    // order + 2); kept when it wins by more than 1 %
+    // two groups per wavefront, timed at the chosen uv tile, order and groups
+    // per item with every thread count (recorded with order + 2: its best
+    // thread count can differ from one group's); kept when it wins by more
+    // than 1 %
     vs.tuned_wave_groups = 1;
     if (vs.wave_groups == 0 && binned_two_groups_allowed(vs) && spent <= kBudget) {
         const GridParams gp = make_gp();
         vs.tuned_wave_groups = 2;
-        float ms = -1.0f;
-        for (int rep = 0; rep < 4; ++rep) {
-            vset_residual_with(vs, gp, M, Rg, st, true);
-            CHIP_HIP(hipEventSynchronize(vs.ev_stop));
-            float m = 0.0f;
-            CHIP_HIP(hipEventElapsedTime(&m, vs.ev_start, vs.ev_stop));
-            if (rep == 0 && m < kLong) continue;
-            ms = ms < 0.0f ? m : std::min(ms, m);
-            if (m >= kLong) break;
+        double best2 = -1.0;
+        uint32_t best2_t = best_t;
+        for (uint32_t t : kThreads) {
+            vs.threads = t;
+            float ms = -1.0f;
+            for (int rep = 0; rep < 4; ++rep) {
+                vset_residual_with(vs, gp, M, Rg, st, true);
+                CHIP_HIP(hipEventSynchronize(vs.ev_stop));
+                float m = 0.0f;
+                CHIP_HIP(hipEventElapsedTime(&m, vs.ev_start, vs.ev_stop));
+                if (rep == 0 && m < kLong) continue;
+                ms = ms < 0.0f ? m : std::min(ms, m);
+                if (m >= kLong) break;
+            }
+            vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(best_tile), static_cast<double>(best_order + 2),
+                                                 static_cast<double>(t), static_cast<double>(best_g), ms});
+            if (best2 < 0.0 || ms < best2) {
+                best2 = ms;
+                best2_t = t;
+            }
         }
-        vs.tune_ms.insert(vs.tune_ms.end(), {static_cast<double>(best_tile), static_cast<double>(best_order + 2),
-                                             static_cast<double>(best_t), static_cast<double>(best_g), ms});
-        if (!(ms < 0.99 * best)) vs.tuned_wave_groups = 1;
+        if (best2 > 0.0 && best2 < 0.99 * best) {
+            vs.threads = best2_t;
+        } else {
+            vs.tuned_wave_groups = 1;
+            vs.threads = best_t;
+        }
     }
     vs.schedule_tuned = true;
 }
