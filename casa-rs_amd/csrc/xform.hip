@@ -303,53 +303,50 @@ __device__ __forceinline__ void run_fft(float2 *buf, const Axis &ax, uint32_t pi
 // Coverage mask of one upper block as kRowWords 64-bit words (row chunk ch
 // at bit ch % 64 of word ch / 64), gathered by wavefront ballots.
 struct RowMask {
-    uint64_t w[kRowWords];
+    // one named word per 64 row chunks (kRowWords = 4): named members, not an
+    // array, so that a lane-dependent choice of word stays in registers
+    uint64_t w0, w1, w2, w3;
 };
+static_assert(kRowWords == 4, "RowMask holds four words");
 __device__ __forceinline__ RowMask row_mask(const XGeom &xg, uint32_t c, uint32_t yb) {
-    RowMask m;
-    if (!xg.touch) {
-#pragma unroll
-        for (uint32_t q = 0; q < kRowWords; ++q) m.w[q] = ~0ull;
-        return m;
-    }
+    if (!xg.touch) return RowMask{~0ull, ~0ull, ~0ull, ~0ull};
     const uint8_t *t = xg.touch + (static_cast<uint64_t>(c) * xg.nyb_h + yb) * xg.nxc;
     const uint32_t lane = threadIdx.x & 63u;
-    uint8_t f[kRowWords];
-#pragma unroll
-    for (uint32_t q = 0; q < kRowWords; ++q) f[q] = t[lane + 64 * q < xg.nxc ? lane + 64 * q : 0];
-#pragma unroll
-    for (uint32_t q = 0; q < kRowWords; ++q) m.w[q] = __ballot(lane + 64 * q < xg.nxc && f[q] != 0);
-    return m;
+    auto word = [&](uint32_t q) {
+        const uint32_t i = lane + 64 * q;
+        const uint8_t f = t[i < xg.nxc ? i : 0];
+        return static_cast<uint64_t>(__ballot(i < xg.nxc && f != 0));
+    };
+    return RowMask{word(0), word(1), word(2), word(3)};
 }
 __device__ __forceinline__ bool row_touched(const RowMask &m, uint32_t x) {
     const uint32_t ch = x / kTouchRows;
-    return ch < 64 * kRowWords && ((m.w[ch >> 6] >> (ch & 63u)) & 1ull) != 0;
+    const uint32_t q = ch >> 6;
+    const uint64_t w = q == 0 ? m.w0 : q == 1 ? m.w1 : q == 2 ? m.w2 : m.w3;
+    return ch < 64 * kRowWords && ((w >> (ch & 63u)) & 1ull) != 0;
 }
 
 // Block flags of one channel (touch_blk) as kBlockWords 64-bit words,
 // gathered by wavefront ballots; every block without a mask.
 struct BlockMask {
-    uint64_t w[kBlockWords];
+    uint64_t w0, w1, w2, w3, w4;  // (named, as RowMask)
 };
+static_assert(kBlockWords == 5, "BlockMask holds five words");
 __device__ __forceinline__ BlockMask block_mask(const XGeom &xg, uint32_t c) {
-    BlockMask m;
-    if (!xg.touch_blk) {
-#pragma unroll
-        for (uint32_t q = 0; q < kBlockWords; ++q) m.w[q] = ~0ull;
-        return m;
-    }
+    if (!xg.touch_blk) return BlockMask{~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
     const uint8_t *t = xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h;
     const uint32_t lane = threadIdx.x & 63u;
-    uint8_t f[kBlockWords];
-#pragma unroll
-    for (uint32_t q = 0; q < kBlockWords; ++q) f[q] = t[lane + 64 * q < xg.nyb_h ? lane + 64 * q : 0];
-#pragma unroll
-    for (uint32_t q = 0; q < kBlockWords; ++q) m.w[q] = __ballot(lane + 64 * q < xg.nyb_h && f[q] != 0);
-    return m;
+    auto word = [&](uint32_t q) {
+        const uint32_t i = lane + 64 * q;
+        const uint8_t f = t[i < xg.nyb_h ? i : 0];
+        return static_cast<uint64_t>(__ballot(i < xg.nyb_h && f != 0));
+    };
+    return BlockMask{word(0), word(1), word(2), word(3), word(4)};
 }
 __device__ __forceinline__ bool block_live(const BlockMask &m, uint32_t yb) {
     const uint32_t q = yb >> 6;
-    return q < kBlockWords && ((m.w[q] >> (yb & 63u)) & 1ull) != 0;
+    const uint64_t w = q == 0 ? m.w0 : q == 1 ? m.w1 : q == 2 ? m.w2 : q == 3 ? m.w3 : m.w4;
+    return q < kBlockWords && ((w >> (yb & 63u)) & 1ull) != 0;
 }
 
 __device__ __forceinline__ uint64_t half_at(const XGeom &xg, uint32_t yb, uint32_t x, uint32_t lane) {
