@@ -415,10 +415,12 @@ __global__ __launch_bounds__(kPairs * LANES) __attribute__((amdgpu_waves_per_eu(
     // pass skips them)
     float2 *h = half + static_cast<uint64_t>(c) * xg.hstride;
     const uint32_t hy = n / 2;
-    const BlockMask bm = block_mask(xg, c);
+    // the block flags straight from the cached flag bytes (a mask word chosen
+    // per lane would sit in scratch memory)
+    const uint8_t *tb = xg.touch_blk ? xg.touch_blk + static_cast<uint64_t>(c) * xg.nyb_h : nullptr;
     for (uint32_t e = tid; e < xg.nyb_h * (kPairs * kBlock); e += NT) {
         const uint32_t yb = e / (kPairs * kBlock), pp = (e / kBlock) % kPairs, lane = e % kBlock;
-        if (!block_live(bm, yb)) continue;
+        if (tb && !tb[yb]) continue;
         const uint32_t k = yb * kBlock + lane;
         float2 A = make_float2(0.0f, 0.0f), B = A;
         if (k <= hy) {
@@ -467,11 +469,12 @@ __global__ __launch_bounds__(LANES << LOGB) __attribute__((amdgpu_waves_per_eu(E
     const uint64_t bstride = static_cast<uint64_t>(n) * kBlock;
     float2 *up = grid + static_cast<uint64_t>(c) * xg.gstride + yb * bstride;
     const uint32_t hy = xg.gy / 2;
-    const RowMask rm = row_mask(xg, c, yb);
+    // row chunks from the cached flag bytes (as in the rows pass)
+    const uint8_t *tr = xg.touch ? xg.touch + (static_cast<uint64_t>(c) * xg.nyb_h + yb) * xg.nxc : nullptr;
     for (uint32_t e = tid; e < n * BLK; e += NT) {
         const uint32_t x = e >> LOGB, lane = col0 + (e & (BLK - 1));
         const uint32_t ky = yb * kBlock + lane;
-        if (ky <= hy && row_touched(rm, x)) up[x * kBlock + lane] = buf[e];
+        if (ky <= hy && (!tr || tr[x / kTouchRows])) up[x * kBlock + lane] = buf[e];
     }
 }
 
