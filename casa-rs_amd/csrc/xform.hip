@@ -623,11 +623,12 @@ __global__ __launch_bounds__(kPairs * LANES) __attribute__((amdgpu_waves_per_eu(
 // 256 threads; a row of a block is 8 float2 = 4 float4.
 __global__ __launch_bounds__(256) void k_zero_touched(XGeom xg, float4 *__restrict__ grid) {
     const uint32_t yb = blockIdx.x, c = blockIdx.y;
-    const RowMask rm = row_mask(xg, c, yb);
+    // row chunks from the cached flag bytes (as in the forward passes)
+    const uint8_t *tr = xg.touch ? xg.touch + (static_cast<uint64_t>(c) * xg.nyb_h + yb) * xg.nxc : nullptr;
     float4 *blk = grid + (static_cast<uint64_t>(c) * xg.gstride + static_cast<uint64_t>(yb) * xg.gx * kBlock) / 2;
     const uint32_t n4 = xg.gx * (kBlock / 2);
     for (uint32_t q = threadIdx.x; q < n4; q += 256)
-        if (row_touched(rm, q >> 2)) blk[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (!tr || tr[(q >> 2) / kTouchRows]) blk[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // ---------------------------------------------------------------- host side
